@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Throughput of the NOVA hot path on one MI355X.
+
+Workload: incompressible Taylor-Green vortex, 256^3 periodic on [0, 2pi)^3,
+FP64, Re = 1600, explicit 6th-order stencils, SSP-RK3 + one FFT pressure
+projection per step, dt = 2e-3 (the reference's TG_Poisson step,
+app/TG_Piosson/RKLoop.H, with the RK3 composition).  Inputs are resident in
+HBM before the timed region.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+METRIC = "TG 256^3 FP64 Mpoint-steps/s, % HBM roofline; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+DT = 2e-3
+NU = 1.0 / 1600.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def alg_bytes_per_point(name: str, n: int) -> float:
+    """Algorithmic HBM bytes per grid point for one launch of each kernel
+    (unique FP64 values read once + written once)."""
+    half = (n // 2 + 1) / n * 16.0  # half-spectrum complex value per real point
+    return {
+        "ns_stage": (48.0 + 72.0 + 72.0) / 3.0,  # stage 1 reads U only
+        "fft_div_r2c_z": 24.0 + half,
+        "fft_y": 2 * half,
+        "fft_x_solve": 2 * half,
+        "fft_c2r_z": half + 8.0,
+        "grad_correct": 8.0 + 24.0 + 24.0,
+        "copy": 48.0,
+    }.get(name, float("nan"))
+
+
+def tg_field(nv, G, n):
+    import torch
+    x = torch.arange(n, dtype=torch.float64, device=G.device) * (2 * math.pi / n)
+    X, Y, Z = torch.meshgrid(x, x, x, indexing="ij")
+    U = torch.stack([torch.sin(X) * torch.cos(Y) * torch.cos(Z),
+                     -torch.cos(X) * torch.sin(Y) * torch.cos(Z),
+                     torch.zeros_like(X)]).contiguous()
+    return U
+
+
+def cpu_baseline(n: int, steps: int) -> dict:
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    import pyoracle
+    L = (2 * math.pi,) * 3
+    O = pyoracle.OMesh((n, n, n), L)
+    P = pyoracle.OPoisson(O)
+    x = np.arange(n) * (2 * math.pi / n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    c = np.stack([np.sin(X) * np.cos(Y) * np.cos(Z), -np.cos(X) * np.sin(Y) * np.cos(Z), 0 * X])
+    del X, Y, Z
+    U = O.from_compact(c)
+    del c
+    p, Us = O.field(1), O.field(3)
+    P.ns_step(U, p, NU, DT, pyoracle.RK_SSP3, Us)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        P.ns_step(U, p, NU, DT, pyoracle.RK_SSP3, Us)
+    dt = time.perf_counter() - t0
+    cores = pyoracle.num_threads()
+    return {"value": n ** 3 * steps / dt / 1e6, "unit": "Mpoint-steps/s", "cores": cores, "kind": "port",
+            "sample": f"oracle TG {n}^3 SSP-RK3 + FFT projection, {steps} timed steps after 1 warm-up, "
+                      f"{cores} OpenMP threads, {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of graph replay")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is design-only",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    sys.path.insert(0, str(ROOT))
+    import novaclone_amd as nv
+
+    n = args.n
+    K, W = args.steps, args.warmup
+    G = nv.Mesh((n, n, n), (2 * math.pi,) * 3)
+    P = nv.Poisson(G)
+    S = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=NU, graph=not args.eager)
+    U = tg_field(nv, G, n)
+    p = G.field(1)
+    log(f"bench: TG {n}^3, warmup {W}, steps {K}, graph={not args.eager}")
+    S.step(DT, W, U, p)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    S.step(DT, K, U, p)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = e0.elapsed_time(e1)
+    ms_per_step = ms_total / K
+    value = n ** 3 * K / (ms_total / 1e3) / 1e6
+    log(f"bench: {ms_per_step:.3f} ms/step (events), wall {wall * 1e3 / K:.3f} ms/step, {value:.1f} Mpt-steps/s")
+    ke = nv.ke(G, U)
+    maxdiv = nv.maxabs_div(G, U)
+
+    # per-kernel times with HIP events around each launch on the library stream
+    S_eager = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=NU)
+    G.timing(True)
+    S_eager.step(DT, K, U, p)
+    stats = G.timing_collect()
+    G.timing(False)
+    kernels = {}
+    for name, (cnt, tot) in stats.items():
+        avg_ms = tot / cnt
+        b = alg_bytes_per_point(name, n) * n ** 3
+        kernels[name] = {"launches": cnt, "avg_ms": round(avg_ms, 5), "alg_bytes": b,
+                         "GBps": round(b / (avg_ms / 1e3) / 1e9, 1)}
+    dom = max(stats, key=lambda k: stats[k][1])
+    traffic = None
+    tfile = ROOT / "profiles" / "pmc_traffic.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get(dom, {}).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    dk = kernels[dom]
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(dk["GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_ms": dk["avg_ms"]}
+    step_alg = 384.0 * n ** 3  # minimal-traffic model of one RK3 + projection step
+    step_roofline = {"alg_bytes_per_step": step_alg, "achieved_GBps": round(step_alg / (ms_per_step / 1e3) / 1e9, 1),
+                     "frac": round(step_alg / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        log("bench: CPU baseline (oracle) ...")
+        cpu = cpu_baseline(n, args.cpu_steps)
+        log(f"bench: CPU {cpu['value']:.3f} Mpt-steps/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpoint-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Taylor-Green initial condition)",
+        "config": {"workload": f"TG {n}^3 periodic FP64 Re=1600, explicit 6th-order FD, SSP-RK3 + 1 FFT "
+                               f"projection per step, dt=2e-3", "N": n, "scheme": "explicit-m6", "rk": "ssp3",
+                   "graph": not args.eager, "parallelism": "single-gpu"},
+        "roofline": roofline,
+        "step_roofline": step_roofline,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "check": {"ke": ke, "maxabs_div": maxdiv},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

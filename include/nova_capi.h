@@ -1,0 +1,171 @@
+/*
+ * nova_capi.h -- C ABI of the MI355X (gfx950) NOVA hot path.
+ *
+ * The reference (NOVA) has no FFI: its boundary is the C++ header API that
+ * the app sources include (Field/Mesh/ex::/poisson/Time).  Each entry point
+ * below names the reference interface it replaces (paths relative to the
+ * reference repository root).  The C++ drop-in layer
+ * (novaclone_amd/host/nova/) and the Python harness (novaclone_amd/) both
+ * call only these functions.
+ *
+ * Conventions
+ *  - Every function returns 0 (NOVA_OK) or a negative NOVA_ERR_* code and
+ *    never throws; nova_last_error() returns the message of the calling
+ *    thread's most recent failure.
+ *  - Fields live on the device in the compact layout: structure of arrays
+ *    over components, each component a dense s0*s1*s2 block, index
+ *    (i*s1 + j)*s2 + k (k fastest, like the reference's Field, minus the
+ *    ghost layers and the duplicated periodic endpoint).  s_d = n_d cells
+ *    for a periodic direction, n_d + 1 nodes for a walled one.  Ghosts are
+ *    not stored: every stencil applies the boundary condition when it
+ *    loads (periodic wrap, or reflection about the wall node; the wall-
+ *    normal component 1 of a 3-component field is odd, everything else
+ *    even).
+ *  - Device pointers are caller-owned (nova_alloc or any HIP allocation).
+ *  - Work is asynchronous on the context's stream (nova_ctx_set_stream;
+ *    default: the null stream).  One host thread per context at a time.
+ */
+#ifndef NOVA_CAPI_H
+#define NOVA_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    NOVA_OK = 0,
+    NOVA_ERR_ARG = -1,
+    NOVA_ERR_HIP = -2,
+    NOVA_ERR_UNSUPPORTED = -3,
+    NOVA_ERR_STATE = -4,
+    NOVA_ERR_NOMEM = -5
+};
+
+enum { NOVA_BC_PERIODIC = 0, NOVA_BC_WALL = 1 };
+
+/* Time-integration compositions (nova_solver_create). */
+enum {
+    NOVA_RK_LITERAL4 = 0, /* app/TG_Piosson/RKLoop.H:15-38: frozen-stage RK4, one projection */
+    NOVA_RK_SSP3 = 1,     /* SSP-RK3 (Shu-Osher), one projection per step */
+    NOVA_RK_SHEAR4 = 2    /* app/shearLayer/UEqn.H:14-38: projection at every stage */
+};
+
+enum { NOVA_MODEL_NS = 0, NOVA_MODEL_MHD = 1 };
+
+/* Reference BC kinds for the padded-layout fill (src/BC/BCenums.h:8). */
+enum { NOVA_PBC_PERIODIC = 0, NOVA_PBC_NEUMANN = 1, NOVA_PBC_DIRICHLET = 2,
+       NOVA_PBC_EVEN = 10, NOVA_PBC_ODD = 11 };
+
+typedef struct nova_grid_desc {
+    int n[3];          /* cells per direction (src/Mesh/Mesh.cpp:49-52 "N:") */
+    double L[3];       /* lengths ("L:") */
+    double origin[3];  /* ("origin:") */
+    int m;             /* explicit stencil order 2/4/6/8 (src/settings/settings.cpp:24) */
+    int bc[3];         /* NOVA_BC_* per direction */
+    int hyp;           /* hyperbolic z stretching ("hypz:", Mesh.cpp:57-60) */
+    double s1, s2;
+} nova_grid_desc;
+
+typedef struct nova_ctx nova_ctx;
+typedef struct nova_poisson nova_poisson;
+typedef struct nova_solver nova_solver;
+
+/* ---- context / memory (replaces Mesh construction, Mesh.cpp:24-179, and
+ * Field storage, src/Field/Field.hxx:21-134) ---- */
+int nova_ctx_create(const nova_grid_desc *desc, int device, nova_ctx **out);
+int nova_ctx_destroy(nova_ctx *ctx);
+int nova_ctx_set_stream(nova_ctx *ctx, void *hip_stream);
+int nova_ctx_sync(nova_ctx *ctx);
+/* out: s0, s1, s2, points per component, padded n_i, n_j, n_k, m */
+int nova_ctx_info(const nova_ctx *ctx, int64_t out[8]);
+/* Per-k metric on the stored k range (Mesh.cpp:147-179): dzdxi, d2zdxi2. */
+int nova_ctx_metric(const nova_ctx *ctx, double *dzdxi, double *d2zdxi2);
+const char *nova_last_error(void);
+/* Per-kernel timing (no reference counterpart; the reference prints whole-
+ * step wall time, app/TG_Piosson/TG.cpp:142): when enabled, every kernel
+ * the library launches is bracketed by HIP events on the context stream.
+ * collect synchronises and writes "name launches total_ms" lines. */
+int nova_timing_enable(nova_ctx *ctx, int enable);
+int nova_timing_collect(nova_ctx *ctx, char *buf, size_t buflen);
+int nova_alloc(nova_ctx *ctx, int ncomp, double **dptr);
+int nova_free(nova_ctx *ctx, double *dptr);
+int nova_memcpy_h2d(nova_ctx *ctx, double *dst, const double *src, size_t count);
+int nova_memcpy_d2h(nova_ctx *ctx, double *dst, const double *src, size_t count);
+
+/* ---- reference-layout boundary (Field::correctBoundaryConditions,
+ * src/Field/Field.hxx:310-401, with periodicBC/dirichletBC/neumannBC) ---- */
+/* compact SoA -> padded AoS (the reference's Field layout) with the ghost
+ * layers and the duplicated endpoint filled by the grid's BCs. */
+int nova_pack_padded(nova_ctx *ctx, const double *compact, int ncomp, double *padded);
+/* padded AoS -> compact SoA (interior nodes only). */
+int nova_unpack_padded(nova_ctx *ctx, const double *padded, int ncomp, double *compact);
+/* In-place ghost fill of a padded AoS field, entries applied in order with
+ * the reference's semantics: src/BC/periodicBC.hxx:1-66,
+ * src/BC/dirichletBC.hxx, src/BC/neumannBC.hxx (dir: 0 north .. 5 bottom,
+ * src/BC/BCenums.h:9). */
+int nova_bc_fill_padded(nova_ctx *ctx, double *padded, int ncomp, int nent,
+                        const int *dir, const int *kind, const int *comp, const double *val);
+
+/* ---- explicit operators, compact layout (src/ex) ---- */
+/* ex::div(Field<vector>, Field<T>), src/ex/div/div.hxx:84-172 */
+int nova_div_adv(nova_ctx *ctx, const double *U, const double *phi, int ncomp_phi, double *out);
+/* ex::div(Field<vector>), src/ex/div/div.hxx:22-81 */
+int nova_div(nova_ctx *ctx, const double *U, double *out);
+/* ex::grad(Field<scalar>), src/ex/grad/grad.cpp:25-62 */
+int nova_grad(nova_ctx *ctx, const double *p, double *out);
+/* ex::laplacian(scalar, Field<T>), src/ex/laplacian/laplacian.hxx:130-191 */
+int nova_laplacian(nova_ctx *ctx, double nu, const double *f, int ncomp, double *out);
+/* ex::curl, src/ex/curl/curl.cpp:25-57 (half-shifted literal stencil) */
+int nova_curl_literal(nova_ctx *ctx, const double *U, double *out);
+/* consistent curl from the central stencil (no reference counterpart) */
+int nova_curl(nova_ctx *ctx, const double *U, double *out);
+/* out = a*x + b*y over count doubles (Field algebra, Field.hxx:617-1150) */
+int nova_axpby(nova_ctx *ctx, size_t count, double a, const double *x, double b,
+               const double *y, double *out);
+
+/* ---- Poisson (src/poisson/poisson.cpp) ---- */
+/* poisson::poisson + wavenumbersFFT_, poisson.cpp:6-164,337-396.  Periodic
+ * directions use FFTs; a walled y direction uses the even (DCT-I) solve. */
+int nova_poisson_create(nova_ctx *ctx, nova_poisson **out);
+int nova_poisson_destroy(nova_poisson *ps);
+/* poisson::rhs + poisson::solve, poisson.cpp:167-181,223-265: p = lap^-1 rhs
+ * with the modified wavenumbers, null modes (k'^2 < 1e-16) set to zero. */
+int nova_poisson_solve(nova_poisson *ps, const double *rhs, double *p);
+/* Projection F <- F - s*grad(p), lap p = div(F)/s
+ * (app/TG_Piosson/RKLoop.H:30-38 with s = dt; app/OT_Poisson/BEqn.H:37-41
+ * with s = 1).  p (scalar, may be NULL) receives the potential. */
+int nova_project(nova_poisson *ps, double *F, double s, double *p);
+
+/* ---- time stepping (the app-level RK loops) ---- */
+int nova_solver_create(nova_poisson *ps, int model, int mode, double nu, double eta,
+                       nova_solver **out);
+int nova_solver_destroy(nova_solver *sv);
+/* Advance nsteps of size dt.  U (and B for MHD) in/out, p (and pB) out.
+ * NS: app/TG_Piosson/RKLoop.H; MHD: app/OT_Poisson/{UEqn,BEqn}.H. */
+int nova_solver_step(nova_solver *sv, double dt, int nsteps, double *U, double *B,
+                     double *p, double *pB);
+/* Record the step sequence as a HIP graph and replay it (1) or launch
+ * kernels one by one (0). */
+int nova_solver_use_graph(nova_solver *sv, int enable);
+
+/* ---- diagnostics (app/TG_Piosson/TG.cpp:148-166, src/Tools/Tools.h:19-36,
+ * app/OT_Poisson/OT.cpp:134-176) ---- */
+int nova_reduce_ke(nova_ctx *ctx, const double *U, double *out);
+int nova_reduce_maxabs_div(nova_ctx *ctx, const double *U, double *out);
+/* out = {Ek, Em, Jav, Jmax, omega, epsilon, Umax, 0} */
+int nova_mhd_diag(nova_ctx *ctx, const double *U, const double *B, double mu, double eta,
+                  double out[8]);
+
+/* ---- batched tridiagonal (src/Tools/Tools.cpp:43-62, tools::tdma) ----
+ * nbatch systems of size n, system s at a[s*n ...]; cyclic != 0 adds the
+ * periodic corner entries a[0] (row 0, col n-1) and c[n-1] (row n-1, col 0). */
+int nova_tdma_batched(nova_ctx *ctx, int n, int nbatch, const double *a, const double *b,
+                      const double *c, const double *d, double *x, int cyclic);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,365 @@
+"""MI355X-native NOVA hot path: Python host mirror over the C ABI.
+
+The names follow the reference's host API (``Mesh``, ``ex.div``/``grad``/
+``laplacian``/``curl``, ``poisson.rhs``/``solve``, the app-level RK loop) so
+that a NOVA user finds the same operators.  Every call goes to the gfx950
+library ``libnova_hip.so`` through the C ABI declared in
+``include/nova_capi.h``; there is no CPU fallback: importing this package
+without the built library, or calling it without a GPU, raises.
+
+Device memory and streams come from PyTorch (plumbing only): a field is a
+float64 CUDA tensor of shape ``(ncomp, s0, s1, s2)`` in the compact layout
+(k fastest, no ghost layers).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import torch
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libnova_hip.so"
+
+NOVA_OK = 0
+BC_PERIODIC, BC_WALL = 0, 1
+RK_LITERAL4, RK_SSP3, RK_SHEAR4 = 0, 1, 2
+MODEL_NS, MODEL_MHD = 0, 1
+# reference BC kinds for the padded-layout fill (src/BC/BCenums.h:8)
+PBC_PERIODIC, PBC_NEUMANN, PBC_DIRICHLET, PBC_EVEN, PBC_ODD = 0, 1, 2, 10, 11
+DIRS = {"north": 0, "south": 1, "east": 2, "west": 3, "top": 4, "bottom": 5}
+
+
+class NovaError(RuntimeError):
+    pass
+
+
+class GridDesc(C.Structure):
+    _fields_ = [
+        ("n", C.c_int * 3),
+        ("L", C.c_double * 3),
+        ("origin", C.c_double * 3),
+        ("m", C.c_int),
+        ("bc", C.c_int * 3),
+        ("hyp", C.c_int),
+        ("s1", C.c_double),
+        ("s2", C.c_double),
+    ]
+
+
+_P = C.c_void_p
+_D = C.c_double
+_I = C.c_int
+_DP = C.POINTER(C.c_double)
+
+# name -> (argtypes); every function returns int status unless noted
+_SIGS = {
+    "nova_ctx_create": [C.POINTER(GridDesc), _I, C.POINTER(_P)],
+    "nova_ctx_destroy": [_P],
+    "nova_ctx_set_stream": [_P, _P],
+    "nova_ctx_sync": [_P],
+    "nova_ctx_info": [_P, C.POINTER(C.c_int64)],
+    "nova_ctx_metric": [_P, _DP, _DP],
+    "nova_timing_enable": [_P, _I],
+    "nova_timing_collect": [_P, C.c_char_p, C.c_size_t],
+    "nova_alloc": [_P, _I, C.POINTER(_P)],
+    "nova_free": [_P, _P],
+    "nova_memcpy_h2d": [_P, _P, _P, C.c_size_t],
+    "nova_memcpy_d2h": [_P, _P, _P, C.c_size_t],
+    "nova_pack_padded": [_P, _P, _I, _P],
+    "nova_unpack_padded": [_P, _P, _I, _P],
+    "nova_bc_fill_padded": [_P, _P, _I, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), _DP],
+    "nova_div_adv": [_P, _P, _P, _I, _P],
+    "nova_div": [_P, _P, _P],
+    "nova_grad": [_P, _P, _P],
+    "nova_laplacian": [_P, _D, _P, _I, _P],
+    "nova_curl_literal": [_P, _P, _P],
+    "nova_curl": [_P, _P, _P],
+    "nova_axpby": [_P, C.c_size_t, _D, _P, _D, _P, _P],
+    "nova_poisson_create": [_P, C.POINTER(_P)],
+    "nova_poisson_destroy": [_P],
+    "nova_poisson_solve": [_P, _P, _P],
+    "nova_project": [_P, _P, _D, _P],
+    "nova_solver_create": [_P, _I, _I, _D, _D, C.POINTER(_P)],
+    "nova_solver_destroy": [_P],
+    "nova_solver_step": [_P, _D, _I, _P, _P, _P, _P],
+    "nova_solver_use_graph": [_P, _I],
+    "nova_reduce_ke": [_P, _P, _DP],
+    "nova_reduce_maxabs_div": [_P, _P, _DP],
+    "nova_mhd_diag": [_P, _P, _P, _D, _D, _DP],
+    "nova_tdma_batched": [_P, _I, _I, _P, _P, _P, _P, _P, _I],
+}
+
+
+def load_library(path: os.PathLike | str = LIB_PATH) -> C.CDLL:
+    """Load the HIP library; raises if it has not been built."""
+    p = Path(path)
+    if not p.exists():
+        raise NovaError(
+            f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(there is no CPU fallback)")
+    lib = C.CDLL(str(p))
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.nova_last_error.argtypes = []
+    lib.nova_last_error.restype = C.c_char_p
+    return lib
+
+
+_lib: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        _lib = load_library()
+    return _lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != NOVA_OK:
+        msg = lib().nova_last_error().decode(errors="replace")
+        raise NovaError(f"{what} failed ({rc}): {msg}")
+
+
+def _ptr(t: torch.Tensor | None):
+    if t is None:
+        return None
+    if not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous():
+        raise NovaError("fields must be contiguous float64 CUDA tensors")
+    return C.c_void_p(t.data_ptr())
+
+
+class Mesh:
+    """Grid + device context (reference ``Mesh``, src/Mesh/Mesh.cpp:24-179).
+
+    ``N`` cells, ``L`` lengths, ``origin``, stencil order ``m`` and a BC kind
+    per direction (``"periodic"`` or ``"wall"``: reflecting walls at nodes 0
+    and N, stored as N+1 nodes)."""
+
+    def __init__(self, N, L, origin=(0.0, 0.0, 0.0), m: int = 6, bc=("periodic",) * 3,
+                 device: int = 0, hyp: tuple[float, float] | None = None):
+        if not torch.cuda.is_available():
+            raise NovaError("no GPU visible: the NOVA HIP path has no CPU fallback")
+        d = GridDesc()
+        for a in range(3):
+            d.n[a] = int(N[a])
+            d.L[a] = float(L[a])
+            d.origin[a] = float(origin[a])
+            d.bc[a] = BC_WALL if bc[a] == "wall" else BC_PERIODIC
+        d.m = int(m)
+        d.hyp = 1 if hyp else 0
+        d.s1, d.s2 = (float(hyp[0]), float(hyp[1])) if hyp else (0.0, 0.0)
+        self.device = torch.device("cuda", device)
+        torch.cuda.set_device(self.device)
+        h = C.c_void_p()
+        _check(lib().nova_ctx_create(C.byref(d), int(device), C.byref(h)), "nova_ctx_create")
+        self._h = h
+        self.desc = d
+        self.N = tuple(int(x) for x in N)
+        self.L = tuple(float(x) for x in L)
+        self.m = int(m)
+        self.bc = tuple(bc)
+        info = (C.c_int64 * 8)()
+        _check(lib().nova_ctx_info(h, info), "nova_ctx_info")
+        self.shape = (info[0], info[1], info[2])
+        self.np = info[3]
+        self.padded = (info[4], info[5], info[6])
+        self.sync_stream()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def sync_stream(self) -> None:
+        """Issue work on torch's current stream."""
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        _check(lib().nova_ctx_set_stream(self._h, C.c_void_p(s)), "nova_ctx_set_stream")
+
+    def timing(self, enable: bool) -> None:
+        """Bracket every library kernel with HIP events (per-kernel times)."""
+        _check(lib().nova_timing_enable(self._h, int(enable)), "nova_timing_enable")
+
+    def timing_collect(self) -> dict:
+        """{kernel: (launches, total_ms)} since the last collect."""
+        buf = C.create_string_buffer(1 << 16)
+        _check(lib().nova_timing_collect(self._h, buf, len(buf)), "nova_timing_collect")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms = line.split()
+            out[name] = (int(cnt), float(ms))
+        return out
+
+    def field(self, ncomp: int = 3) -> torch.Tensor:
+        return torch.zeros((ncomp, *self.shape), dtype=torch.float64, device=self.device)
+
+    def dzdxi(self):
+        import numpy as np
+        a = np.zeros(self.shape[2])
+        b = np.zeros(self.shape[2])
+        _check(lib().nova_ctx_metric(self._h, a.ctypes.data_as(_DP), b.ctypes.data_as(_DP)), "nova_ctx_metric")
+        return a, b
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().nova_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # reference-layout boundary
+    def pack_padded(self, f: torch.Tensor) -> torch.Tensor:
+        nc = f.shape[0]
+        out = torch.empty((*self.padded, nc), dtype=torch.float64, device=self.device)
+        _check(lib().nova_pack_padded(self._h, _ptr(f), nc, _ptr(out)), "nova_pack_padded")
+        return out
+
+    def unpack_padded(self, padded: torch.Tensor) -> torch.Tensor:
+        nc = padded.shape[-1]
+        out = self.field(nc)
+        _check(lib().nova_unpack_padded(self._h, _ptr(padded.contiguous()), nc, _ptr(out)), "nova_unpack_padded")
+        return out
+
+    def bc_fill_padded(self, padded: torch.Tensor, entries) -> None:
+        """Apply reference BC entries (dir, kind, comp, val) in order to a
+        padded AoS field in place."""
+        n = len(entries)
+        dirs = (C.c_int * max(n, 1))(*[DIRS[e[0]] if isinstance(e[0], str) else e[0] for e in entries])
+        kinds = (C.c_int * max(n, 1))(*[e[1] for e in entries])
+        comps = (C.c_int * max(n, 1))(*[e[2] for e in entries])
+        vals = (C.c_double * max(n, 1))(*[float(e[3]) for e in entries])
+        _check(lib().nova_bc_fill_padded(self._h, _ptr(padded), padded.shape[-1], n, dirs, kinds, comps, vals),
+               "nova_bc_fill_padded")
+
+
+class ex:
+    """Explicit operators (reference namespace ``ex``, src/ex)."""
+
+    @staticmethod
+    def div(mesh: Mesh, U: torch.Tensor, phi: torch.Tensor | None = None) -> torch.Tensor:
+        if phi is None:
+            out = mesh.field(1)
+            _check(lib().nova_div(mesh.handle, _ptr(U), _ptr(out)), "nova_div")
+            return out
+        out = mesh.field(phi.shape[0])
+        _check(lib().nova_div_adv(mesh.handle, _ptr(U), _ptr(phi), phi.shape[0], _ptr(out)), "nova_div_adv")
+        return out
+
+    @staticmethod
+    def grad(mesh: Mesh, p: torch.Tensor) -> torch.Tensor:
+        out = mesh.field(3)
+        _check(lib().nova_grad(mesh.handle, _ptr(p), _ptr(out)), "nova_grad")
+        return out
+
+    @staticmethod
+    def laplacian(mesh: Mesh, nu: float, f: torch.Tensor) -> torch.Tensor:
+        out = mesh.field(f.shape[0])
+        _check(lib().nova_laplacian(mesh.handle, float(nu), _ptr(f), f.shape[0], _ptr(out)), "nova_laplacian")
+        return out
+
+    @staticmethod
+    def curl(mesh: Mesh, U: torch.Tensor, literal: bool = True) -> torch.Tensor:
+        out = mesh.field(3)
+        fn = lib().nova_curl_literal if literal else lib().nova_curl
+        _check(fn(mesh.handle, _ptr(U), _ptr(out)), "nova_curl")
+        return out
+
+
+class Poisson:
+    """FFT (and wall-y DCT) Poisson solver (reference ``poisson``,
+    src/poisson/poisson.cpp)."""
+
+    def __init__(self, mesh: Mesh):
+        self.mesh = mesh
+        h = C.c_void_p()
+        _check(lib().nova_poisson_create(mesh.handle, C.byref(h)), "nova_poisson_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def solve(self, rhs: torch.Tensor, p: torch.Tensor | None = None) -> torch.Tensor:
+        if p is None:
+            p = self.mesh.field(1)
+        _check(lib().nova_poisson_solve(self._h, _ptr(rhs), _ptr(p)), "nova_poisson_solve")
+        return p
+
+    def project(self, F: torch.Tensor, s: float = 1.0, p: torch.Tensor | None = None) -> None:
+        _check(lib().nova_project(self._h, _ptr(F), float(s), _ptr(p)), "nova_project")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nova_poisson_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Solver:
+    """The app-level time loop (app/TG_Piosson/RKLoop.H, app/OT_Poisson/
+    {UEqn,BEqn}.H, app/shearLayer/UEqn.H) as fused stage kernels."""
+
+    def __init__(self, poisson: Poisson, model: int = MODEL_NS, mode: int = RK_SSP3, nu: float = 0.0,
+                 eta: float = 0.0, graph: bool = False):
+        self.poisson = poisson
+        self.mesh = poisson.mesh
+        h = C.c_void_p()
+        _check(lib().nova_solver_create(poisson.handle, model, mode, float(nu), float(eta), C.byref(h)),
+               "nova_solver_create")
+        self._h = h
+        self.model = model
+        if graph:
+            _check(lib().nova_solver_use_graph(h, 1), "nova_solver_use_graph")
+
+    def step(self, dt: float, nsteps: int, U, p, B=None, pB=None) -> None:
+        _check(lib().nova_solver_step(self._h, float(dt), int(nsteps), _ptr(U), _ptr(B), _ptr(p), _ptr(pB)),
+               "nova_solver_step")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nova_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ke(mesh: Mesh, U: torch.Tensor) -> float:
+    out = C.c_double()
+    _check(lib().nova_reduce_ke(mesh.handle, _ptr(U), C.byref(out)), "nova_reduce_ke")
+    return out.value
+
+
+def maxabs_div(mesh: Mesh, U: torch.Tensor) -> float:
+    out = C.c_double()
+    _check(lib().nova_reduce_maxabs_div(mesh.handle, _ptr(U), C.byref(out)), "nova_reduce_maxabs_div")
+    return out.value
+
+
+def mhd_diag(mesh: Mesh, U, B, mu: float, eta: float):
+    out = (C.c_double * 8)()
+    _check(lib().nova_mhd_diag(mesh.handle, _ptr(U), _ptr(B), float(mu), float(eta), out), "nova_mhd_diag")
+    return list(out)
+
+
+def tdma_batched(mesh: Mesh, a, b, c, d, cyclic: bool = False) -> torch.Tensor:
+    nb, n = a.shape
+    x = torch.empty_like(d)
+    _check(lib().nova_tdma_batched(mesh.handle, n, nb, _ptr(a), _ptr(b), _ptr(c), _ptr(d), _ptr(x), int(cyclic)),
+           "nova_tdma_batched")
+    return x

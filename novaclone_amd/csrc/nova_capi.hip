@@ -1,0 +1,490 @@
+// nova_capi.hip -- the C ABI: context set-up (the reference Mesh on one
+// device), memory, and thin wrappers over the kernel launchers.
+#include "nova_internal.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <new>
+
+struct nova_poisson;
+struct nova_solver;
+
+namespace nova {
+
+namespace {
+thread_local std::string g_last_error;
+
+// Central-difference weights, src/settings/settings.cpp:25-31.
+const double kCoef[4][4] = {{1.0, 0.0, 0.0, 0.0},
+                            {4.0 / 3.0, -1.0 / 3.0, 0.0, 0.0},
+                            {6.0 / 4.0, -12.0 / 20.0, 1.0 / 10.0, 0.0},
+                            {8.0 / 5.0, -4.0 / 5.0, 24.0 / 105.0, -8.0 / 280.0}};
+}  // namespace
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int fail(int code, const std::string &msg)
+{
+    set_error(msg);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return NOVA_ERR_HIP;
+}
+
+int check_launch(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, what);
+    return NOVA_OK;
+}
+
+static hipEvent_t take_event(nova_ctx *ctx)
+{
+    if (!ctx->evpool.empty()) {
+        hipEvent_t e = ctx->evpool.back();
+        ctx->evpool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+TimedScope::TimedScope(nova_ctx *ctx, const char *name) : ctx_(ctx), name_(name)
+{
+    if (!ctx_->timing) return;
+    start_ = take_event(ctx_);
+    if (start_) (void)hipEventRecord(start_, ctx_->stream);
+}
+
+TimedScope::~TimedScope()
+{
+    if (!ctx_->timing || !start_) return;
+    hipEvent_t stop = take_event(ctx_);
+    if (!stop) return;
+    (void)hipEventRecord(stop, ctx_->stream);
+    ctx_->tlog.push_back({name_, start_, stop});
+}
+
+int ensure_red(nova_ctx *ctx, size_t n)
+{
+    if (ctx->red_cap >= n) return NOVA_OK;
+    if (ctx->d_red) NOVA_HIP(hipFree(ctx->d_red));
+    ctx->d_red = nullptr;
+    ctx->red_cap = 0;
+    NOVA_HIP(hipMalloc((void **)&ctx->d_red, sizeof(double) * n));
+    ctx->red_cap = n;
+    return NOVA_OK;
+}
+
+int poisson_create(nova_ctx *ctx, nova_poisson **out);
+int poisson_destroy(nova_poisson *ps);
+int poisson_solve(nova_poisson *ps, const double *rhs, double *p);
+int project(nova_poisson *ps, double *F, double s, double *p);
+int solver_create(nova_poisson *ps, int model, int mode, double nu, double eta, nova_solver **out);
+int solver_destroy(nova_solver *sv);
+int solver_step(nova_solver *sv, double dt, int nsteps, double *U, double *B, double *p, double *pB);
+int solver_use_graph(nova_solver *sv, int enable);
+nova_ctx *poisson_ctx(nova_poisson *ps);
+
+// Mesh::z (src/Mesh/Mesh.cpp:373-407) for padded index kp on one rank.
+static double mesh_z(const nova_grid_desc &d, double dz, double hb, int m, int kp)
+{
+    if (d.hyp) {
+        const double A = std::pow(d.s1 / d.s2, 0.5);
+        const int globK = kp - m / 2;
+        const int globN = d.n[2] + 1;
+        const double R = double(globK) / double(globN - 1) - 0.5;
+        const double U = 1.0 + std::tanh(hb * R) / std::tanh(hb / 2.0);
+        return U / (2.0 * A + (1 - A) * U) * d.L[2];
+    }
+    return (kp - m / 2) * dz + d.origin[2];
+}
+
+static int ctx_init(nova_ctx *ctx, const nova_grid_desc *desc, int device)
+{
+    const nova_grid_desc &d = *desc;
+    if (d.m != 2 && d.m != 4 && d.m != 6 && d.m != 8) return fail(NOVA_ERR_ARG, "m must be 2, 4, 6 or 8");
+    const int h = d.m / 2;
+    for (int a = 0; a < 3; a++) {
+        if (d.n[a] < 2 * h || d.n[a] < 4) return fail(NOVA_ERR_ARG, "grid too small for the stencil");
+        if (!(d.L[a] > 0.0)) return fail(NOVA_ERR_ARG, "lengths must be positive");
+        if (d.bc[a] != NOVA_BC_PERIODIC && d.bc[a] != NOVA_BC_WALL) return fail(NOVA_ERR_ARG, "bad bc kind");
+    }
+    if (d.hyp && !(d.s1 > 0.0 && d.s2 > 0.0)) return fail(NOVA_ERR_ARG, "stretching needs s1, s2 > 0");
+    ctx->device = device;
+    ctx->desc = d;
+    ctx->m = d.m;
+    for (int l = 0; l < 4; l++) ctx->coef[l] = kCoef[h - 1][l];
+    Grid &g = ctx->g;
+    g.h = h;
+    for (int a = 0; a < 3; a++) {
+        g.n[a] = d.n[a];
+        g.wall[a] = d.bc[a] == NOVA_BC_WALL;
+        g.s[a] = d.n[a] + (g.wall[a] ? 1 : 0);
+        ctx->d[a] = d.L[a] / d.n[a];  // Mesh.cpp:68-70
+        ctx->pad[a] = d.n[a] + 1 + d.m;  // Mesh.cpp:89-91,136-138
+        for (int l = 0; l < kMaxH; l++) {
+            const double c = l < h ? ctx->coef[l] : 0.0;
+            g.a[a][l] = c / 2.0 / (l + 1) / ctx->d[a];
+            g.b[a][l] = c / std::pow((l + 1) * ctx->d[a], 2);
+            g.al[a][l] = c / (l + 1) / ctx->d[a];
+        }
+    }
+    g.np = (int64_t)g.s[0] * g.s[1] * g.s[2];
+    // z metric (Mesh.cpp:147-179) on the stored k range
+    double hb = 1.0;
+    if (d.hyp) {
+        const int globN = d.n[2] + 1;
+        for (int it = 1; it < 10000; it++) hb = std::asinh(hb / ((globN - 1) * std::pow(d.s1 * d.s2, 0.5)));
+    }
+    const double dz = ctx->d[2];
+    const int s2 = g.s[2];
+    ctx->dzdxi.assign(s2, 0.0);
+    ctx->d2zdxi2.assign(s2, 0.0);
+    for (int k = 0; k < s2; k++) {
+        const int kp = k + h;
+        double a1 = 0.0, a2 = 0.0;
+        for (int l = 0; l < h; l++) {
+            const double c = ctx->coef[l];
+            a1 += (c / 2.0 / (l + 1) / dz * mesh_z(d, dz, hb, d.m, kp + 1 + l) -
+                   c / 2.0 / (l + 1) / dz * mesh_z(d, dz, hb, d.m, kp - 1 - l));
+            a2 += (c / std::pow((l + 1) * dz, 2) * mesh_z(d, dz, hb, d.m, kp + 1 + l) +
+                   c / std::pow((l + 1) * dz, 2) * mesh_z(d, dz, hb, d.m, kp - 1 - l) +
+                   (-2.0 * c / std::pow((l + 1) * dz, 2)) * mesh_z(d, dz, hb, d.m, kp));
+        }
+        ctx->dzdxi[k] = a1;
+        ctx->d2zdxi2[k] = a2;
+    }
+    std::vector<double> met(3 * (size_t)s2);
+    for (int k = 0; k < s2; k++) {
+        met[k] = 1.0 / ctx->dzdxi[k];
+        met[s2 + k] = 1.0 / std::pow(ctx->dzdxi[k], 2);
+        met[2 * s2 + k] = ctx->d2zdxi2[k] / std::pow(ctx->dzdxi[k], 3);
+    }
+    NOVA_HIP(hipSetDevice(device));
+    NOVA_HIP(hipMalloc((void **)&ctx->d_metric, sizeof(double) * met.size()));
+    NOVA_HIP(hipMemcpy(ctx->d_metric, met.data(), sizeof(double) * met.size(), hipMemcpyHostToDevice));
+    g.zinv = ctx->d_metric;
+    g.zlap = ctx->d_metric + s2;
+    g.zmet = ctx->d_metric + 2 * s2;
+    NOVA_HIP(hipHostMalloc((void **)&ctx->h_red, sizeof(double) * 16, hipHostMallocDefault));
+    return NOVA_OK;
+}
+
+static void ctx_release(nova_ctx *ctx)
+{
+    for (auto &t : ctx->tlog) {
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    for (auto e : ctx->evpool) (void)hipEventDestroy(e);
+    if (ctx->d_metric) (void)hipFree(ctx->d_metric);
+    if (ctx->d_red) (void)hipFree(ctx->d_red);
+    if (ctx->h_red) (void)hipHostFree(ctx->h_red);
+}
+
+}  // namespace nova
+
+using namespace nova;
+
+#define NOVA_GUARD(expr)                                                         \
+    try {                                                                        \
+        return (expr);                                                           \
+    } catch (const std::bad_alloc &) {                                           \
+        return fail(NOVA_ERR_NOMEM, "host allocation failed");                   \
+    } catch (...) {                                                              \
+        return fail(NOVA_ERR_STATE, "unexpected exception");                     \
+    }
+
+#define NOVA_REQUIRE(cond, msg) \
+    do {                        \
+        if (!(cond)) return fail(NOVA_ERR_ARG, msg); \
+    } while (0)
+
+extern "C" {
+
+const char *nova_last_error(void) { return g_last_error.c_str(); }
+
+int nova_ctx_create(const nova_grid_desc *desc, int device, nova_ctx **out)
+{
+    NOVA_REQUIRE(desc && out, "null argument");
+    nova_ctx *ctx = new (std::nothrow) nova_ctx();
+    if (!ctx) return fail(NOVA_ERR_NOMEM, "host allocation failed");
+    int rc;
+    try {
+        rc = ctx_init(ctx, desc, device);
+    } catch (...) {
+        rc = fail(NOVA_ERR_NOMEM, "context set-up failed");
+    }
+    if (rc) {
+        ctx_release(ctx);
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return NOVA_OK;
+}
+
+int nova_ctx_destroy(nova_ctx *ctx)
+{
+    if (!ctx) return NOVA_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx_release(ctx);
+    delete ctx;
+    return NOVA_OK;
+}
+
+int nova_ctx_set_stream(nova_ctx *ctx, void *stream)
+{
+    NOVA_REQUIRE(ctx, "null context");
+    ctx->stream = (hipStream_t)stream;
+    return NOVA_OK;
+}
+
+int nova_ctx_sync(nova_ctx *ctx)
+{
+    NOVA_REQUIRE(ctx, "null context");
+    NOVA_HIP(hipStreamSynchronize(ctx->stream));
+    return NOVA_OK;
+}
+
+int nova_ctx_info(const nova_ctx *ctx, int64_t out[8])
+{
+    NOVA_REQUIRE(ctx && out, "null argument");
+    out[0] = ctx->g.s[0];
+    out[1] = ctx->g.s[1];
+    out[2] = ctx->g.s[2];
+    out[3] = ctx->g.np;
+    out[4] = ctx->pad[0];
+    out[5] = ctx->pad[1];
+    out[6] = ctx->pad[2];
+    out[7] = ctx->m;
+    return NOVA_OK;
+}
+
+int nova_ctx_metric(const nova_ctx *ctx, double *dzdxi, double *d2zdxi2)
+{
+    NOVA_REQUIRE(ctx, "null context");
+    if (dzdxi) std::memcpy(dzdxi, ctx->dzdxi.data(), sizeof(double) * ctx->dzdxi.size());
+    if (d2zdxi2) std::memcpy(d2zdxi2, ctx->d2zdxi2.data(), sizeof(double) * ctx->d2zdxi2.size());
+    return NOVA_OK;
+}
+
+int nova_timing_enable(nova_ctx *ctx, int enable)
+{
+    NOVA_REQUIRE(ctx, "null context");
+    ctx->timing = enable != 0;
+    return NOVA_OK;
+}
+
+int nova_timing_collect(nova_ctx *ctx, char *buf, size_t buflen)
+{
+    NOVA_REQUIRE(ctx && buf && buflen > 0, "bad argument");
+    NOVA_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<std::string> names;
+    std::vector<double> tot;
+    std::vector<long> cnt;
+    for (auto &t : ctx->tlog) {
+        float ms = 0.0f;
+        NOVA_HIP(hipEventElapsedTime(&ms, t.start, t.stop));
+        size_t q = 0;
+        while (q < names.size() && names[q] != t.name) q++;
+        if (q == names.size()) {
+            names.push_back(t.name);
+            tot.push_back(0.0);
+            cnt.push_back(0);
+        }
+        tot[q] += ms;
+        cnt[q] += 1;
+        ctx->evpool.push_back(t.start);
+        ctx->evpool.push_back(t.stop);
+    }
+    ctx->tlog.clear();
+    std::string out;
+    for (size_t q = 0; q < names.size(); q++)
+        out += names[q] + " " + std::to_string(cnt[q]) + " " + std::to_string(tot[q]) + "\n";
+    if (out.size() + 1 > buflen) return fail(NOVA_ERR_ARG, "timing buffer too small");
+    std::memcpy(buf, out.c_str(), out.size() + 1);
+    return NOVA_OK;
+}
+
+int nova_alloc(nova_ctx *ctx, int ncomp, double **dptr)
+{
+    NOVA_REQUIRE(ctx && dptr && ncomp > 0, "bad argument");
+    NOVA_HIP(hipMalloc((void **)dptr, sizeof(double) * (size_t)ncomp * ctx->g.np));
+    return NOVA_OK;
+}
+
+int nova_free(nova_ctx *ctx, double *dptr)
+{
+    NOVA_REQUIRE(ctx, "null context");
+    if (dptr) NOVA_HIP(hipFree(dptr));
+    return NOVA_OK;
+}
+
+int nova_memcpy_h2d(nova_ctx *ctx, double *dst, const double *src, size_t count)
+{
+    NOVA_REQUIRE(ctx && dst && src, "null argument");
+    NOVA_HIP(hipMemcpyAsync(dst, src, sizeof(double) * count, hipMemcpyHostToDevice, ctx->stream));
+    NOVA_HIP(hipStreamSynchronize(ctx->stream));
+    return NOVA_OK;
+}
+
+int nova_memcpy_d2h(nova_ctx *ctx, double *dst, const double *src, size_t count)
+{
+    NOVA_REQUIRE(ctx && dst && src, "null argument");
+    NOVA_HIP(hipMemcpyAsync(dst, src, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
+    NOVA_HIP(hipStreamSynchronize(ctx->stream));
+    return NOVA_OK;
+}
+
+int nova_pack_padded(nova_ctx *ctx, const double *compact, int ncomp, double *padded)
+{
+    NOVA_REQUIRE(ctx && compact && padded && (ncomp == 1 || ncomp == 3), "bad argument");
+    NOVA_GUARD(launch_pack(ctx, compact, ncomp, padded));
+}
+
+int nova_unpack_padded(nova_ctx *ctx, const double *padded, int ncomp, double *compact)
+{
+    NOVA_REQUIRE(ctx && compact && padded && ncomp > 0, "bad argument");
+    NOVA_GUARD(launch_unpack(ctx, padded, ncomp, compact));
+}
+
+int nova_bc_fill_padded(nova_ctx *ctx, double *padded, int ncomp, int nent, const int *dir, const int *kind,
+                        const int *comp, const double *val)
+{
+    NOVA_REQUIRE(ctx && padded && ncomp > 0 && nent >= 0 && (nent == 0 || (dir && kind && comp)), "bad argument");
+    for (int e = 0; e < nent; e++) {
+        NOVA_REQUIRE(comp[e] >= 0 && comp[e] < ncomp, "bc component out of range");
+        if (int rc = launch_bc_fill(ctx, padded, ncomp, dir[e], kind[e], comp[e], val ? val[e] : 0.0)) return rc;
+    }
+    return NOVA_OK;
+}
+
+int nova_div_adv(nova_ctx *ctx, const double *U, const double *phi, int ncomp_phi, double *out)
+{
+    NOVA_REQUIRE(ctx && U && phi && out && (ncomp_phi == 1 || ncomp_phi == 3), "bad argument");
+    NOVA_REQUIRE(out != U && out != phi, "out must not alias an input");
+    NOVA_GUARD(launch_div_adv(ctx, U, phi, ncomp_phi, out));
+}
+
+int nova_div(nova_ctx *ctx, const double *U, double *out)
+{
+    NOVA_REQUIRE(ctx && U && out && out != U, "bad argument");
+    NOVA_GUARD(launch_div(ctx, U, 1.0, out));
+}
+
+int nova_grad(nova_ctx *ctx, const double *p, double *out)
+{
+    NOVA_REQUIRE(ctx && p && out && out != p, "bad argument");
+    NOVA_GUARD(launch_grad(ctx, p, out));
+}
+
+int nova_laplacian(nova_ctx *ctx, double nu, const double *f, int ncomp, double *out)
+{
+    NOVA_REQUIRE(ctx && f && out && out != f && (ncomp == 1 || ncomp == 3), "bad argument");
+    NOVA_GUARD(launch_laplacian(ctx, nu, f, ncomp, out));
+}
+
+int nova_curl_literal(nova_ctx *ctx, const double *U, double *out)
+{
+    NOVA_REQUIRE(ctx && U && out && out != U, "bad argument");
+    NOVA_GUARD(launch_curl(ctx, U, 1, out));
+}
+
+int nova_curl(nova_ctx *ctx, const double *U, double *out)
+{
+    NOVA_REQUIRE(ctx && U && out && out != U, "bad argument");
+    NOVA_GUARD(launch_curl(ctx, U, 0, out));
+}
+
+int nova_axpby(nova_ctx *ctx, size_t count, double a, const double *x, double b, const double *y, double *out)
+{
+    NOVA_REQUIRE(ctx && x && out, "bad argument");
+    NOVA_GUARD(launch_axpby(ctx, count, a, x, b, y, out));
+}
+
+int nova_poisson_create(nova_ctx *ctx, nova_poisson **out)
+{
+    NOVA_REQUIRE(ctx && out, "null argument");
+    NOVA_GUARD(poisson_create(ctx, out));
+}
+
+int nova_poisson_destroy(nova_poisson *ps) { NOVA_GUARD(poisson_destroy(ps)); }
+
+int nova_poisson_solve(nova_poisson *ps, const double *rhs, double *p)
+{
+    NOVA_REQUIRE(ps && rhs && p && rhs != p, "bad argument");
+    NOVA_GUARD(poisson_solve(ps, rhs, p));
+}
+
+int nova_project(nova_poisson *ps, double *F, double s, double *p)
+{
+    NOVA_REQUIRE(ps && F && s != 0.0 && F != p, "bad argument");
+    NOVA_GUARD(project(ps, F, s, p));
+}
+
+int nova_solver_create(nova_poisson *ps, int model, int mode, double nu, double eta, nova_solver **out)
+{
+    NOVA_REQUIRE(ps && out, "null argument");
+    NOVA_GUARD(solver_create(ps, model, mode, nu, eta, out));
+}
+
+int nova_solver_destroy(nova_solver *sv) { NOVA_GUARD(solver_destroy(sv)); }
+
+int nova_solver_step(nova_solver *sv, double dt, int nsteps, double *U, double *B, double *p, double *pB)
+{
+    NOVA_REQUIRE(sv && U && p && nsteps >= 0, "bad argument");
+    NOVA_GUARD(solver_step(sv, dt, nsteps, U, B, p, pB));
+}
+
+int nova_solver_use_graph(nova_solver *sv, int enable)
+{
+    NOVA_REQUIRE(sv, "null solver");
+    NOVA_GUARD(solver_use_graph(sv, enable));
+}
+
+int nova_reduce_ke(nova_ctx *ctx, const double *U, double *out)
+{
+    NOVA_REQUIRE(ctx && U && out, "bad argument");
+    NOVA_GUARD(reduce_ke(ctx, U, out));
+}
+
+int nova_reduce_maxabs_div(nova_ctx *ctx, const double *U, double *out)
+{
+    NOVA_REQUIRE(ctx && U && out, "bad argument");
+    double *d = nullptr;
+    NOVA_HIP(hipMallocAsync((void **)&d, sizeof(double) * ctx->g.np, ctx->stream));
+    int rc = launch_div(ctx, U, 1.0, d);
+    if (!rc) rc = reduce_maxabs(ctx, d, out);
+    (void)hipFreeAsync(d, ctx->stream);
+    return rc;
+}
+
+int nova_mhd_diag(nova_ctx *ctx, const double *U, const double *B, double mu, double eta, double out[8])
+{
+    NOVA_REQUIRE(ctx && U && B && out, "bad argument");
+    double *J = nullptr;
+    NOVA_HIP(hipMallocAsync((void **)&J, sizeof(double) * 6 * ctx->g.np, ctx->stream));
+    double *W = J + 3 * ctx->g.np;
+    int rc = launch_curl(ctx, B, 1, J);
+    if (!rc) rc = launch_curl(ctx, U, 1, W);
+    if (!rc) rc = reduce_mhd(ctx, U, B, J, W, mu, eta, out);
+    (void)hipFreeAsync(J, ctx->stream);
+    return rc;
+}
+
+int nova_tdma_batched(nova_ctx *ctx, int n, int nbatch, const double *a, const double *b, const double *c,
+                      const double *d, double *x, int cyclic)
+{
+    NOVA_REQUIRE(ctx && a && b && c && d && x && n >= 2 && nbatch >= 0, "bad argument");
+    if (nbatch == 0) return NOVA_OK;
+    NOVA_GUARD(launch_tdma(ctx, n, nbatch, a, b, c, d, x, cyclic));
+}
+
+}  // extern "C"

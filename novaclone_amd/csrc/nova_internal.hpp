@@ -1,0 +1,142 @@
+// nova_internal.hpp -- shared definitions of the gfx950 NOVA hot path.
+//
+// Device layout (compact SoA): component c of a field at base + c*np,
+// node (i,j,k) at (i*s1 + j)*s2 + k.  Boundary conditions are applied at
+// load time through index maps (bmap) instead of stored ghost layers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/nova_capi.h"
+
+namespace nova {
+
+constexpr int kMaxH = 4;  // m/2 for m = 8
+
+// Per-launch description of the grid, passed by value to kernels.
+struct Grid {
+    int s[3];          // stored extents
+    int n[3];          // cells
+    int wall[3];       // 1 = reflecting walls at nodes 0 and n
+    int h;             // m/2
+    int64_t np;        // s0*s1*s2
+    double a[3][kMaxH];  // first derivative weights c_l / (2 (l+1) h_d)
+    double b[3][kMaxH];  // second derivative weights c_l / ((l+1) h_d)^2
+    double al[3][kMaxH]; // literal-gradient weights c_l / ((l+1) h_d)
+    const double *zinv;  // per stored k: 1 / dzdxi(k)
+    const double *zlap;  // per stored k: 1 / dzdxi(k)^2
+    const double *zmet;  // per stored k: d2zdxi2(k) / dzdxi(k)^3
+};
+
+// Map a possibly out-of-range node index along one direction to a stored
+// index.  Periodic: wrap modulo n.  Wall: reflect about node 0 and node n
+// (flip = 1 when the load came through a reflection).  Indices beyond one
+// reflection (threads of a partial tile) are clamped; they are never
+// stored.
+__host__ __device__ inline int bmap(int q, int n, int wall, int &flip)
+{
+    flip = 0;
+    if (!wall) {
+        q %= n;
+        if (q < 0) q += n;
+        return q;
+    }
+    if (q < 0) { q = -q; flip = 1; }
+    if (q > n) { q = 2 * n - q; flip = 1; }
+    if (q < 0) q = 0;
+    if (q > n) q = n;
+    return q;
+}
+
+// Sign of component c of an ncomp-component field reached through a
+// reflection across a wall normal to direction d: the wall-normal
+// component of a vector is odd, everything else even.
+__host__ __device__ inline double bsign(int ncomp, int c, int d, int flip)
+{
+    return (flip && ncomp == 3 && c == d) ? -1.0 : 1.0;
+}
+
+struct DevBuf {
+    double *p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace nova
+
+namespace nova {
+// One timed launch: events recorded on the context stream around it.
+struct TimedLaunch {
+    const char *name;
+    hipEvent_t start, stop;
+};
+}  // namespace nova
+
+struct nova_ctx {
+    int device = 0;
+    int timing = 0;                        // record events around launches
+    std::vector<nova::TimedLaunch> tlog;   // launches since the last collect
+    std::vector<hipEvent_t> evpool;        // reusable events
+    hipStream_t stream = nullptr;
+    nova_grid_desc desc{};
+    nova::Grid g{};
+    int m = 6;
+    int pad[3] = {0, 0, 0};          // reference padded extents n_d = N_d + 1 + m
+    double d[3] = {0, 0, 0};         // spacings
+    double coef[4] = {0, 0, 0, 0};
+    std::vector<double> dzdxi, d2zdxi2;  // stored-k metric
+    double *d_metric = nullptr;      // zinv | zlap | zmet on the device
+    double *d_red = nullptr;         // reduction partials
+    size_t red_cap = 0;
+    double *h_red = nullptr;         // pinned host result slot
+};
+
+namespace nova {
+
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+#define NOVA_HIP(call)                                   \
+    do {                                                 \
+        hipError_t e_ = (call);                          \
+        if (e_ != hipSuccess) return nova::hip_fail(e_, #call); \
+    } while (0)
+
+int check_launch(const char *what);
+
+// Brackets the launches issued in its scope with events on ctx->stream
+// when the context's timing mode is on.
+class TimedScope {
+public:
+    TimedScope(nova_ctx *ctx, const char *name);
+    ~TimedScope();
+
+private:
+    nova_ctx *ctx_;
+    const char *name_;
+    hipEvent_t start_ = nullptr;
+};
+int ensure_red(nova_ctx *ctx, size_t n);
+
+// Kernel launchers (defined in the .hip files).
+int launch_div_adv(nova_ctx *ctx, const double *U, const double *phi, int nc, double *out);
+int launch_div(nova_ctx *ctx, const double *U, double scale, double *out);
+int launch_grad(nova_ctx *ctx, const double *p, double *out);
+int launch_laplacian(nova_ctx *ctx, double nu, const double *f, int nc, double *out);
+int launch_curl(nova_ctx *ctx, const double *U, int literal, double *out);
+int launch_axpby(nova_ctx *ctx, size_t n, double a, const double *x, double b, const double *y,
+                 double *out);
+int launch_pack(nova_ctx *ctx, const double *compact, int nc, double *padded);
+int launch_unpack(nova_ctx *ctx, const double *padded, int nc, double *compact);
+int launch_bc_fill(nova_ctx *ctx, double *padded, int nc, int dir, int kind, int comp, double val);
+int reduce_ke(nova_ctx *ctx, const double *U, double *out);
+int reduce_maxabs(nova_ctx *ctx, const double *f, double *out);
+int reduce_mhd(nova_ctx *ctx, const double *U, const double *B, const double *J,
+               const double *W, double mu, double eta, double out[8]);
+int launch_tdma(nova_ctx *ctx, int n, int nb, const double *a, const double *b, const double *c,
+                const double *d, double *x, int cyclic);
+
+}  // namespace nova

@@ -1,0 +1,955 @@
+/*
+ * nova_oracle.c -- CPU restatement of NOVA's per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library, never part of
+ * the product.  Each function cites the reference file:line it restates
+ * (paths relative to the reference repository root).  Arithmetic follows
+ * the reference's loop order and expression grouping (l-outer sweeps,
+ * left-to-right sums, pool temporaries zero-filled), so that this code and
+ * the reference round the same way wherever the C library calls (sin, pow)
+ * agree.  OpenMP parallelises only over independent output points; the
+ * accumulation order of every point is unchanged.
+ *
+ * Layout: padded AoS, index ((i*nj + j)*nk + k)*nc + c (k fastest).
+ */
+#include "nova_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+struct orc_mesh {
+    int N[3];
+    int m, h;
+    int n[3];          /* padded extents ni, nj, nk */
+    double d[3];       /* dx, dy, dz */
+    double L[3];
+    double o[3];
+    int hyp;
+    double s1, s2, hb; /* stretching parameters and the solved b */
+    int wall_y;
+    double *dzdxi, *d2zdxi2;
+    double coef[4];
+};
+
+struct orc_poisson {
+    const orc_mesh *mesh;
+    int N[3];
+    int ext;           /* 1: y is solved on the even extension (walls) */
+    int M[3];          /* transform extents (M[1] = 2*N[1] with walls) */
+    double *k2;        /* M0*M1*M2, i fastest */
+    double complex *buf;
+};
+
+/* ------------------------------------------------------------------ */
+/* Constants: src/settings/settings.cpp:25-31 (central-difference weights
+ * for orders 2, 4, 6, 8), src/Tools/Tools.cpp:6-7 (pi, eps). */
+static const double COEF[4][4] = {
+    {1.0, 0.0, 0.0, 0.0},
+    {4.0 / 3.0, -1.0 / 3.0, 0.0, 0.0},
+    {6.0 / 4.0, -12.0 / 20.0, 1.0 / 10.0, 0.0},
+    {8.0 / 5.0, -4.0 / 5.0, 24.0 / 105.0, -8.0 / 280.0}};
+static const double ORC_PI = 3.14159265358979323846;
+static const double ORC_EPS = 1.0e-16;
+/* settings.cpp:44-49, last row: the RK4 accumulation weights. */
+static const double RK4W[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+/* settings.cpp:51 */
+static const double RK4C[4] = {0.5, 0.5, 1.0, 1.0};
+
+void orc_coef(int m, double out[4])
+{
+    int r = m / 2 - 1;
+    if (r < 0) r = 0;
+    if (r > 3) r = 3;
+    for (int l = 0; l < 4; l++) out[l] = COEF[r][l];
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+#define IX(M, i, j, k) ((((size_t)(i)) * (size_t)(M)->n[1] + (size_t)(j)) * (size_t)(M)->n[2] + (size_t)(k))
+#define NPTS(M) ((size_t)(M)->n[0] * (size_t)(M)->n[1] * (size_t)(M)->n[2])
+
+/* Mesh::z, src/Mesh/Mesh.cpp:373-407 (single rank: globK = k - m/2). */
+double orc_mesh_z(const orc_mesh *M, int k)
+{
+    if (M->hyp) {
+        double A = pow(M->s1 / M->s2, 0.5);
+        int globK = k - M->m / 2;
+        int globN = M->N[2] + 1;
+        double R = (double)globK / (double)(globN - 1) - 0.5;
+        double b = M->hb;
+        double U = 1.0 + tanh(b * R) / tanh(b / 2.0);
+        return U / (2.0 * A + (1 - A) * U) * M->L[2];
+    }
+    return (k - M->m / 2) * M->d[2] + M->o[2];
+}
+
+/* Mesh::Mesh, src/Mesh/Mesh.cpp:64-179 on one rank. */
+orc_mesh *orc_mesh_create(const int N[3], const double L[3], const double o[3],
+                          int m, int hyp, double s1, double s2, int wall_y)
+{
+    orc_mesh *M = (orc_mesh *)calloc(1, sizeof(orc_mesh));
+    M->m = m;
+    M->h = m / 2;
+    orc_coef(m, M->coef);
+    for (int d = 0; d < 3; d++) {
+        M->N[d] = N[d];
+        M->L[d] = L[d];
+        M->o[d] = o[d];
+        M->d[d] = L[d] / N[d];       /* Mesh.cpp:68-70 */
+        M->n[d] = N[d] + 1 + m;      /* Mesh.cpp:89-91,136-138 */
+    }
+    M->hyp = hyp;
+    M->s1 = s1;
+    M->s2 = s2;
+    M->wall_y = wall_y;
+    if (hyp) { /* fixed point of Mesh.cpp:386-396 */
+        double b = 1.0;
+        int globN = N[2] + 1;
+        for (int i = 1; i < 10000; i++) b = asinh(b / ((globN - 1) * pow(s1 * s2, 0.5)));
+        M->hb = b;
+    }
+    int nk = M->n[2];
+    M->dzdxi = (double *)calloc((size_t)nk, sizeof(double));
+    M->d2zdxi2 = (double *)calloc((size_t)nk, sizeof(double));
+    const double dz = M->d[2];
+    for (int k = M->h; k < nk - M->h; k++) { /* Mesh.cpp:149-160 */
+        M->dzdxi[k] = 0;
+        for (int l = 0; l < M->h; l++) {
+            M->dzdxi[k] += (M->coef[l] / 2.0 / (l + 1) / dz * orc_mesh_z(M, k + 1 + l)
+                            - M->coef[l] / 2.0 / (l + 1) / dz * orc_mesh_z(M, k - 1 - l));
+        }
+    }
+    for (int k = M->h; k < nk - M->h; k++) { /* Mesh.cpp:165-179 */
+        M->d2zdxi2[k] = 0;
+        for (int l = 0; l < M->h; l++) {
+            M->d2zdxi2[k] += (M->coef[l] / pow((l + 1) * dz, 2) * orc_mesh_z(M, k + 1 + l)
+                              + M->coef[l] / pow((l + 1) * dz, 2) * orc_mesh_z(M, k - 1 - l)
+                              + (-2.0 * M->coef[l] / pow((l + 1) * dz, 2)) * orc_mesh_z(M, k));
+        }
+    }
+    return M;
+}
+
+void orc_mesh_free(orc_mesh *M)
+{
+    if (!M) return;
+    free(M->dzdxi);
+    free(M->d2zdxi2);
+    free(M);
+}
+
+void orc_mesh_dims(const orc_mesh *M, int n_out[3])
+{
+    for (int d = 0; d < 3; d++) n_out[d] = M->n[d];
+}
+
+void orc_mesh_metric(const orc_mesh *M, double *dzdxi, double *d2zdxi2)
+{
+    memcpy(dzdxi, M->dzdxi, sizeof(double) * (size_t)M->n[2]);
+    memcpy(d2zdxi2, M->d2zdxi2, sizeof(double) * (size_t)M->n[2]);
+}
+
+/* ------------------------------------------------------------------ */
+/* Ghost fills. */
+
+/* periodicBC<T>::update, src/BC/periodicBC.hxx:1-66: copies the whole node
+ * value (all components) for the direction it belongs to. */
+static void bc_periodic(const orc_mesh *M, double *f, int nc, int dir)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], m = M->m, h = M->h;
+    if (dir == ORC_EAST) {
+        #pragma omp parallel for collapse(2)
+        for (int k = 0; k < nk; k++)
+            for (int j = 0; j < nj; j++) {
+                for (int i = 0; i < h; i++)
+                    memcpy(&f[IX(M, i, j, k) * nc], &f[IX(M, i + ni - m - 1, j, k) * nc], sizeof(double) * nc);
+                for (int i = ni - h - 1; i < ni; i++)
+                    memcpy(&f[IX(M, i, j, k) * nc], &f[IX(M, i - ni + m + 1, j, k) * nc], sizeof(double) * nc);
+            }
+    }
+    if (dir == ORC_SOUTH) {
+        #pragma omp parallel for collapse(2)
+        for (int k = 0; k < nk; k++)
+            for (int i = 0; i < ni; i++) {
+                for (int j = 0; j < h; j++)
+                    memcpy(&f[IX(M, i, j, k) * nc], &f[IX(M, i, j + nj - m - 1, k) * nc], sizeof(double) * nc);
+                for (int j = nj - h - 1; j < nj; j++)
+                    memcpy(&f[IX(M, i, j, k) * nc], &f[IX(M, i, j - nj + m + 1, k) * nc], sizeof(double) * nc);
+            }
+    }
+    if (dir == ORC_BOTTOM) {
+        #pragma omp parallel for collapse(2)
+        for (int i = 0; i < ni; i++)
+            for (int j = 0; j < nj; j++) {
+                for (int k = 0; k < h; k++)
+                    memcpy(&f[IX(M, i, j, k) * nc], &f[IX(M, i, j, k + nk - m - 1) * nc], sizeof(double) * nc);
+                for (int k = nk - h - 1; k < nk; k++)
+                    memcpy(&f[IX(M, i, j, k) * nc], &f[IX(M, i, j, k - nk + m + 1) * nc], sizeof(double) * nc);
+            }
+    }
+}
+
+/* One face of a wall-type fill for component c.  `w` is the wall node,
+ * `s` = -1 for a low face (ghosts at w-(l+1)) and +1 for a high face.
+ * Kinds: DIRICHLET (src/BC/dirichletBC.hxx), NEUMANN
+ * (src/BC/neumannBC.hxx), EVEN/ODD (pure reflections, no reference row). */
+static void bc_wall(const orc_mesh *M, double *f, int nc, int dir, int kind, int c, double val)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    int axis, low;
+    switch (dir) {
+    case ORC_EAST: axis = 0; low = 1; break;
+    case ORC_WEST: axis = 0; low = 0; break;
+    case ORC_SOUTH: axis = 1; low = 1; break;
+    case ORC_NORTH: axis = 1; low = 0; break;
+    case ORC_BOTTOM: axis = 2; low = 1; break;
+    default: axis = 2; low = 0; break;
+    }
+    const int n = M->n[axis];
+    const int w = low ? h : n - h - 1;
+    const int s = low ? -1 : 1;
+    const double hd = M->d[axis];
+    const int na = axis == 0 ? nj : ni, nb = axis == 2 ? nj : nk;
+    #pragma omp parallel for collapse(2)
+    for (int a = 0; a < na; a++)
+        for (int b = 0; b < nb; b++) {
+            size_t base, stride;
+            if (axis == 0) { base = IX(M, 0, a, b); stride = (size_t)nj * nk; }
+            else if (axis == 1) { base = IX(M, a, 0, b); stride = (size_t)nk; }
+            else { base = IX(M, a, b, 0); stride = 1; }
+#define V(q) f[(base + (size_t)(q) * stride) * nc + c]
+            if (kind == ORC_BC_DIRICHLET) {
+                V(w) = val;
+                for (int l = 0; l < h; l++)
+                    V(w + s * (l + 1)) = 2.0 * (l + 1) * val - V(w - s * (l + 1));
+            } else if (kind == ORC_BC_NEUMANN) {
+                if (low) V(w) = V(w + 1) - val * hd;
+                else V(w) = V(w - 1) + val * hd;
+                for (int l = 0; l < h; l++)
+                    V(w + s * (l + 1)) = V(w - s * (l + 1)) - 2.0 * (l + 1) * val * hd;
+            } else if (kind == ORC_BC_EVEN) {
+                for (int l = 0; l < h; l++) V(w + s * (l + 1)) = V(w - s * (l + 1));
+            } else if (kind == ORC_BC_ODD) {
+                V(w) = 0.0;
+                for (int l = 0; l < h; l++) V(w + s * (l + 1)) = -V(w - s * (l + 1));
+            }
+#undef V
+        }
+    (void)ni;
+}
+
+void orc_bc_apply(const orc_mesh *M, double *f, int nc, int nent, const int *dir,
+                  const int *kind, const int *comp, const double *val)
+{
+    for (int e = 0; e < nent; e++) {
+        if (kind[e] == ORC_BC_PERIODIC) bc_periodic(M, f, nc, dir[e]);
+        else bc_wall(M, f, nc, dir[e], kind[e], comp[e], val ? val[e] : 0.0);
+    }
+}
+
+/* The fill used by the time steppers: the reference's periodic dispatch
+ * order south, east, bottom (entries north..bottom per component,
+ * Field.hxx:316-322 with periodicBC acting on south/east/bottom only); with
+ * y walls the north/south entries are reflections. */
+void orc_fill(const orc_mesh *M, double *f, int nc, int odd_comp)
+{
+    for (int c = 0; c < nc; c++) {
+        if (M->wall_y) {
+            int kind = (c == odd_comp) ? ORC_BC_ODD : ORC_BC_EVEN;
+            bc_wall(M, f, nc, ORC_NORTH, kind, c, 0.0);
+            bc_wall(M, f, nc, ORC_SOUTH, kind, c, 0.0);
+        } else {
+            bc_periodic(M, f, nc, ORC_SOUTH);
+        }
+        bc_periodic(M, f, nc, ORC_EAST);
+        bc_periodic(M, f, nc, ORC_BOTTOM);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Explicit operators. */
+
+/* ex::div(Field<vector>, Field<T>), src/ex/div/div.hxx:84-172. */
+void orc_div_adv(const orc_mesh *M, const double *U, const double *phi, int nc, double *out)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    const double dx = M->d[0], dy = M->d[1], dz = M->d[2];
+    const double *dzdxi = M->dzdxi;
+    memset(out, 0, sizeof(double) * NPTS(M) * nc);
+    for (int l = 0; l < h; l++) {
+        const double c = M->coef[l];
+        #pragma omp parallel for
+        for (int i = h; i < ni - h; i++) {
+            for (int j = h; j < nj - h; j++) {
+                for (int k = h; k < nk - h; k++) {
+                    const double Vp = U[IX(M, i, j + 1 + l, k) * 3 + 1];
+                    const double Vm = U[IX(M, i, j - 1 - l, k) * 3 + 1];
+                    const double Up = U[IX(M, i + 1 + l, j, k) * 3 + 0];
+                    const double Um = U[IX(M, i - 1 - l, j, k) * 3 + 0];
+                    const double Wp = U[IX(M, i, j, k + 1 + l) * 3 + 2];
+                    const double Wm = U[IX(M, i, j, k - 1 - l) * 3 + 2];
+                    const double *Uc = &U[IX(M, i, j, k) * 3];
+                    const double *pN = &phi[IX(M, i, j + 1 + l, k) * nc];
+                    const double *pS = &phi[IX(M, i, j - 1 - l, k) * nc];
+                    const double *pE = &phi[IX(M, i + 1 + l, j, k) * nc];
+                    const double *pW = &phi[IX(M, i - 1 - l, j, k) * nc];
+                    const double *pT = &phi[IX(M, i, j, k + 1 + l) * nc];
+                    const double *pB = &phi[IX(M, i, j, k - 1 - l) * nc];
+                    const double *pP = &phi[IX(M, i, j, k) * nc];
+                    double *o = &out[IX(M, i, j, k) * nc];
+                    for (int q = 0; q < nc; q++) { /* div.hxx:120-128 */
+                        o[q] += (0.5 * c / 2.0 / (l + 1) / dy * pN[q] * Vp
+                                 - 0.5 * c / 2.0 / (l + 1) / dy * pS[q] * Vm
+                                 + 0.5 * c / 2.0 / (l + 1) / dx * pE[q] * Up);
+                    }
+                    for (int q = 0; q < nc; q++) { /* div.hxx:129-138 */
+                        o[q] += (-0.5 * c / 2.0 / (l + 1) / dx * pW[q] * Um
+                                 + 0.5 * c / 2.0 / (l + 1) / dz * pT[q] * Wp / dzdxi[k]
+                                 - 0.5 * c / 2.0 / (l + 1) / dz * pB[q] * Wm / dzdxi[k]);
+                    }
+                    for (int q = 0; q < nc; q++) { /* div.hxx:141-152 */
+                        o[q] += (0.5 * Uc[1] * c / 2.0 / (l + 1) / dy * pN[q]
+                                 - 0.5 * Uc[1] * c / 2.0 / (l + 1) / dy * pS[q]
+                                 + 0.5 * Uc[0] * c / 2.0 / (l + 1) / dx * pE[q]
+                                 - 0.5 * Uc[0] * c / 2.0 / (l + 1) / dx * pW[q]
+                                 + 0.5 * Uc[2] * c / 2.0 / (l + 1) / dz * pT[q] / dzdxi[k]
+                                 - 0.5 * Uc[2] * c / 2.0 / (l + 1) / dz * pB[q] / dzdxi[k]);
+                    }
+                    for (int q = 0; q < nc; q++) { /* div.hxx:154-165 */
+                        o[q] += (0.5 * pP[q] * (c / 2.0 / (l + 1) / dy * Vp)
+                                 - 0.5 * pP[q] * (c / 2.0 / (l + 1) / dy * Vm)
+                                 + 0.5 * pP[q] * (c / 2.0 / (l + 1) / dx * Up)
+                                 - 0.5 * pP[q] * (c / 2.0 / (l + 1) / dx * Um)
+                                 + 0.5 * pP[q] * (c / 2.0 / (l + 1) / dz * Wp) / dzdxi[k]
+                                 - 0.5 * pP[q] * (c / 2.0 / (l + 1) / dz * Wm) / dzdxi[k]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* ex::div(Field<vector>), src/ex/div/div.hxx:22-81. */
+void orc_div(const orc_mesh *M, const double *U, double *out)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    const double dx = M->d[0], dy = M->d[1], dz = M->d[2];
+    const double *dzdxi = M->dzdxi;
+    memset(out, 0, sizeof(double) * NPTS(M));
+    for (int l = 0; l < h; l++) {
+        const double c = M->coef[l];
+        #pragma omp parallel for
+        for (int i = h; i < ni - h; i++)
+            for (int j = h; j < nj - h; j++)
+                for (int k = h; k < nk - h; k++) {
+                    const double Vp = U[IX(M, i, j + 1 + l, k) * 3 + 1];
+                    const double Vm = U[IX(M, i, j - 1 - l, k) * 3 + 1];
+                    const double Up = U[IX(M, i + 1 + l, j, k) * 3 + 0];
+                    const double Um = U[IX(M, i - 1 - l, j, k) * 3 + 0];
+                    const double Wp = U[IX(M, i, j, k + 1 + l) * 3 + 2];
+                    const double Wm = U[IX(M, i, j, k - 1 - l) * 3 + 2];
+                    double *o = &out[IX(M, i, j, k)];
+                    *o += (c / 2.0 / (l + 1) / dy * Vp - c / 2.0 / (l + 1) / dy * Vm
+                           + c / 2.0 / (l + 1) / dx * Up);
+                    *o += (-c / 2.0 / (l + 1) / dx * Um + c / 2.0 / (l + 1) / dz * Wp / dzdxi[k]
+                           - c / 2.0 / (l + 1) / dz * Wm / dzdxi[k]);
+                }
+    }
+}
+
+/* ex::grad(Field<scalar>), src/ex/grad/grad.cpp:25-62. */
+void orc_grad(const orc_mesh *M, const double *p, double *out)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    const double dx = M->d[0], dy = M->d[1], dz = M->d[2];
+    const double *dzdxi = M->dzdxi;
+    memset(out, 0, sizeof(double) * NPTS(M) * 3);
+    for (int l = 0; l < h; l++) {
+        const double c = M->coef[l];
+        #pragma omp parallel for
+        for (int i = h; i < ni - h; i++)
+            for (int j = h; j < nj - h; j++)
+                for (int k = h; k < nk - h; k++) {
+                    double *o = &out[IX(M, i, j, k) * 3];
+                    o[0] += c / 2.0 / (l + 1) / dx * p[IX(M, i + 1 + l, j, k)]
+                            - c / 2.0 / (l + 1) / dx * p[IX(M, i - 1 - l, j, k)];
+                    o[1] += c / 2.0 / (l + 1) / dy * p[IX(M, i, j + 1 + l, k)]
+                            - c / 2.0 / (l + 1) / dy * p[IX(M, i, j - 1 - l, k)];
+                    o[2] += c / 2.0 / (l + 1) / dz * p[IX(M, i, j, k + 1 + l)] / dzdxi[k]
+                            - c / 2.0 / (l + 1) / dz * p[IX(M, i, j, k - 1 - l)] / dzdxi[k];
+                }
+    }
+}
+
+/* ex::laplacian(scalar nu, Field<T>), src/ex/laplacian/laplacian.hxx:130-191. */
+void orc_laplacian(const orc_mesh *M, double nu, const double *f, int nc, double *out)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    const double dx = M->d[0], dy = M->d[1], dz = M->d[2];
+    const double *dzdxi = M->dzdxi, *d2 = M->d2zdxi2;
+    memset(out, 0, sizeof(double) * NPTS(M) * nc);
+    for (int l = 0; l < h; l++) {
+        const double c = M->coef[l];
+        #pragma omp parallel for
+        for (int i = h; i < ni - h; i++)
+            for (int j = h; j < nj - h; j++)
+                for (int k = h; k < nk - h; k++) {
+                    const double *fN = &f[IX(M, i, j + 1 + l, k) * nc];
+                    const double *fS = &f[IX(M, i, j - 1 - l, k) * nc];
+                    const double *fE = &f[IX(M, i + 1 + l, j, k) * nc];
+                    const double *fW = &f[IX(M, i - 1 - l, j, k) * nc];
+                    const double *fT = &f[IX(M, i, j, k + 1 + l) * nc];
+                    const double *fB = &f[IX(M, i, j, k - 1 - l) * nc];
+                    const double *fP = &f[IX(M, i, j, k) * nc];
+                    double *o = &out[IX(M, i, j, k) * nc];
+                    const double cP = (-(nu + nu) * c / pow((l + 1) * dx, 2)
+                                       - (nu + nu) * c / pow((l + 1) * dy, 2)
+                                       - (nu + nu) * c / pow((l + 1) * dz, 2) / pow(dzdxi[k], 2));
+                    for (int q = 0; q < nc; q++) {
+                        o[q] += (nu * c / pow((l + 1) * dy, 2) * fN[q]
+                                 + nu * c / pow((l + 1) * dy, 2) * fS[q]
+                                 + nu * c / pow((l + 1) * dx, 2) * fE[q]
+                                 + nu * c / pow((l + 1) * dx, 2) * fW[q]
+                                 + nu * c / pow((l + 1) * dz, 2) * fT[q] / pow(dzdxi[k], 2)
+                                 + nu * c / pow((l + 1) * dz, 2) * fB[q] / pow(dzdxi[k], 2)
+                                 + cP * fP[q]);
+                    }
+                }
+    }
+    for (int l = 0; l < h; l++) { /* laplacian.hxx:170-188 (z-metric term) */
+        const double c = M->coef[l];
+        #pragma omp parallel for
+        for (int i = h; i < ni - h; i++)
+            for (int j = h; j < nj - h; j++)
+                for (int k = h; k < nk - h; k++) {
+                    const double *fT = &f[IX(M, i, j, k + 1 + l) * nc];
+                    const double *fB = &f[IX(M, i, j, k - 1 - l) * nc];
+                    double *o = &out[IX(M, i, j, k) * nc];
+                    for (int q = 0; q < nc; q++)
+                        o[q] -= (c / 2.0 / (l + 1) / dz * fT[q] - c / 2.0 / (l + 1) / dz * fB[q])
+                                * nu * d2[k] / pow(dzdxi[k], 3);
+                }
+    }
+}
+
+/* ex::grad(Field<vector>) -> tensor, src/ex/grad/grad.cpp:65-102 (the
+ * half-shifted stencil f(i+l) - f(i-1-l)); out9 is row-major
+ * (xx, xy, xz, yx, ...), row = component, column = direction. */
+void orc_grad_vec_literal(const orc_mesh *M, const double *U, double *out)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    memset(out, 0, sizeof(double) * NPTS(M) * 9);
+    #pragma omp parallel for
+    for (int i = h; i < ni - h; i++)
+        for (int j = h; j < nj - h; j++)
+            for (int k = h; k < nk - h; k++) {
+                double *o = &out[IX(M, i, j, k) * 9];
+                for (int q = 0; q < 9; q++) o[q] = 0.0;
+                for (int l = 0; l < h; l++) {
+                    const double c = M->coef[l];
+                    const double dx = c / (l + 1) / M->d[0];
+                    const double dy = c / (l + 1) / M->d[1];
+                    const double dz = c / (l + 1) / M->d[2] / M->dzdxi[k];
+                    for (int q = 0; q < 3; q++) {
+                        o[3 * q + 0] += dx * (U[IX(M, i + l, j, k) * 3 + q] - U[IX(M, i - 1 - l, j, k) * 3 + q]);
+                        o[3 * q + 1] += dy * (U[IX(M, i, j + l, k) * 3 + q] - U[IX(M, i, j - 1 - l, k) * 3 + q]);
+                        o[3 * q + 2] += dz * (U[IX(M, i, j, k + l) * 3 + q] - U[IX(M, i, j, k - 1 - l) * 3 + q]);
+                    }
+                }
+            }
+}
+
+/* ex::curl, src/ex/curl/curl.cpp:25-57: antisymmetric part of the literal
+ * vector gradient. */
+void orc_curl_literal(const orc_mesh *M, const double *U, double *out)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    double *g = (double *)malloc(sizeof(double) * NPTS(M) * 9);
+    orc_grad_vec_literal(M, U, g);
+    memset(out, 0, sizeof(double) * NPTS(M) * 3);
+    #pragma omp parallel for
+    for (int i = h; i < ni - h; i++)
+        for (int j = h; j < nj - h; j++)
+            for (int k = h; k < nk - h; k++) {
+                const double *t = &g[IX(M, i, j, k) * 9];
+                double *o = &out[IX(M, i, j, k) * 3];
+                o[0] = t[7] - t[5]; /* zy - yz */
+                o[1] = t[2] - t[6]; /* xz - zx */
+                o[2] = t[3] - t[1]; /* yx - xy */
+            }
+    free(g);
+}
+
+/* Consistent curl: central gradient of each component (the stencil of
+ * grad.cpp:25-62), used for the enstrophy objective.  No reference row. */
+void orc_curl_central(const orc_mesh *M, const double *U, double *out)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    memset(out, 0, sizeof(double) * NPTS(M) * 3);
+    #pragma omp parallel for
+    for (int i = h; i < ni - h; i++)
+        for (int j = h; j < nj - h; j++)
+            for (int k = h; k < nk - h; k++) {
+                double g[3][3] = {{0}};
+                for (int l = 0; l < h; l++) {
+                    const double c = M->coef[l];
+                    const double ax = c / 2.0 / (l + 1) / M->d[0];
+                    const double ay = c / 2.0 / (l + 1) / M->d[1];
+                    const double az = c / 2.0 / (l + 1) / M->d[2] / M->dzdxi[k];
+                    for (int q = 0; q < 3; q++) {
+                        g[q][0] += ax * (U[IX(M, i + 1 + l, j, k) * 3 + q] - U[IX(M, i - 1 - l, j, k) * 3 + q]);
+                        g[q][1] += ay * (U[IX(M, i, j + 1 + l, k) * 3 + q] - U[IX(M, i, j - 1 - l, k) * 3 + q]);
+                        g[q][2] += az * (U[IX(M, i, j, k + 1 + l) * 3 + q] - U[IX(M, i, j, k - 1 - l) * 3 + q]);
+                    }
+                }
+                double *o = &out[IX(M, i, j, k) * 3];
+                o[0] = g[2][1] - g[1][2];
+                o[1] = g[0][2] - g[2][0];
+                o[2] = g[1][0] - g[0][1];
+            }
+}
+
+/* ------------------------------------------------------------------ */
+/* FFT: recursive mixed-radix decimation in time, any length (radix 2, 3,
+ * 4, 5 and a direct DFT for other prime factors).  Stands in for
+ * heFFTe/FFTW (src/poisson/poisson.cpp:109,225,250): forward uses e^{-i},
+ * backward e^{+i}, neither normalises. */
+static int smallest_factor(int n)
+{
+    if (n % 4 == 0) return 4;
+    if (n % 2 == 0) return 2;
+    for (int p = 3; p * p <= n; p += 2)
+        if (n % p == 0) return p;
+    return n;
+}
+
+static void fft_rec(int n, int stride, const double complex *in, double complex *out,
+                    int sign, double complex *tmp)
+{
+    if (n == 1) { out[0] = in[0]; return; }
+    int p = smallest_factor(n);
+    int m = n / p;
+    for (int r = 0; r < p; r++) fft_rec(m, stride * p, in + (size_t)r * stride, out + (size_t)r * m, sign, tmp);
+    /* combine: X[k + q m] = sum_r W_n^{r (k + q m)} Y_r[k] */
+    const double ang = sign * 2.0 * ORC_PI / n;
+    for (int k = 0; k < m; k++) {
+        for (int r = 0; r < p; r++) {
+            long e = (long)r * k;
+            double complex w = (e == 0) ? 1.0 : (cos(ang * e) + I * sin(ang * e));
+            tmp[r] = out[(size_t)r * m + k] * w;
+        }
+        for (int q = 0; q < p; q++) {
+            double complex s = 0;
+            for (int r = 0; r < p; r++) {
+                long e = ((long)r * q) % p;
+                double complex w = (e == 0) ? 1.0 : (cos(2.0 * ORC_PI * sign * e / p) + I * sin(2.0 * ORC_PI * sign * e / p));
+                s += tmp[r] * w;
+            }
+            out[(size_t)k + (size_t)q * m] = s;
+        }
+    }
+}
+
+void orc_fft1d(int n, double *data, int sign)
+{
+    double complex *x = (double complex *)data;
+    double complex *y = (double complex *)malloc(sizeof(double complex) * (size_t)n);
+    double complex tmp[64];
+    double complex *t = n > 64 ? (double complex *)malloc(sizeof(double complex) * (size_t)n) : tmp;
+    fft_rec(n, 1, x, y, sign, t);
+    memcpy(x, y, sizeof(double complex) * (size_t)n);
+    if (t != tmp) free(t);
+    free(y);
+}
+
+void orc_fft3d(const int n[3], double *data, int sign)
+{
+    double complex *x = (double complex *)data;
+    const size_t n0 = n[0], n1 = n[1], n2 = n[2];
+    for (int ax = 0; ax < 3; ax++) {
+        const size_t len = (size_t)n[ax];
+        const size_t stride = ax == 0 ? 1 : (ax == 1 ? n0 : n0 * n1);
+        const size_t nlines = n0 * n1 * n2 / len;
+        #pragma omp parallel
+        {
+            double complex *a = (double complex *)malloc(sizeof(double complex) * len);
+            double complex *b = (double complex *)malloc(sizeof(double complex) * len);
+            double complex *t = (double complex *)malloc(sizeof(double complex) * len);
+            #pragma omp for
+            for (size_t ln = 0; ln < nlines; ln++) {
+                size_t base;
+                if (ax == 0) base = ln * n0;
+                else if (ax == 1) base = (ln % n0) + (ln / n0) * n0 * n1;
+                else base = ln;
+                for (size_t q = 0; q < len; q++) a[q] = x[base + q * stride];
+                fft_rec((int)len, 1, a, b, sign, t);
+                for (size_t q = 0; q < len; q++) x[base + q * stride] = b[q];
+            }
+            free(a); free(b); free(t);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Poisson.  poisson::wavenumbersFFT_ (poisson.cpp:337-396) with
+ * localStart = 0; with y walls the solve runs on the even extension of
+ * the y direction (period 2 L_y, 2 N_y points), which is the DCT-I solve
+ * for even-reflected data (no reference row: solveDCT, poisson.cpp:268-335,
+ * is not a Poisson inverse). */
+static double kmod(double kk, double h, const double *coef, int hm)
+{
+    double kp = 0.0;
+    if (kk > ORC_EPS)
+        for (int ll = 0; ll < hm; ll++)
+            kp += kk * coef[ll] / (ll + 1) * sin((ll + 1) * kk * h) / (kk * h);
+    return kp;
+}
+
+orc_poisson *orc_poisson_create(const orc_mesh *M)
+{
+    orc_poisson *P = (orc_poisson *)calloc(1, sizeof(orc_poisson));
+    P->mesh = M;
+    for (int d = 0; d < 3; d++) { P->N[d] = M->N[d]; P->M[d] = M->N[d]; }
+    P->ext = M->wall_y;
+    double Ly = M->L[1];
+    if (P->ext) { P->M[1] = 2 * M->N[1]; Ly = 2.0 * M->L[1]; }
+    const int M0 = P->M[0], M1 = P->M[1], M2 = P->M[2];
+    P->k2 = (double *)malloc(sizeof(double) * (size_t)M0 * M1 * M2);
+    P->buf = (double complex *)malloc(sizeof(double complex) * (size_t)M0 * M1 * M2);
+    const double pi = ORC_PI;
+    size_t l = 0;
+    for (int k = 0; k < M2; k++)
+        for (int j = 0; j < M1; j++)
+            for (int i = 0; i < M0; i++) {
+                double kx, ky, kz;
+                if (2 * i < M0) kx = 2.0 * pi * i / M->L[0];
+                else kx = 2.0 * pi * (M0 - i) / M->L[0];
+                if (2 * j < M1) ky = 2.0 * pi * j / Ly;
+                else ky = 2.0 * pi * (M1 - j) / Ly;
+                kz = 2.0 * pi * k / M->L[2];
+                double kpx = kmod(kx, M->d[0], M->coef, M->h);
+                double kpy = kmod(ky, M->d[1], M->coef, M->h);
+                double kpz = kmod(kz, M->d[2], M->coef, M->h);
+                P->k2[l++] = kpx * kpx + kpy * kpy + kpz * kpz;
+            }
+    return P;
+}
+
+void orc_poisson_free(orc_poisson *P)
+{
+    if (!P) return;
+    free(P->k2);
+    free(P->buf);
+    free(P);
+}
+
+void orc_poisson_k2(const orc_poisson *P, double *out)
+{
+    memcpy(out, P->k2, sizeof(double) * (size_t)P->M[0] * P->M[1] * P->M[2]);
+}
+
+/* poisson::rhs + poisson::solve, poisson.cpp:167-181,223-265. */
+void orc_poisson_solve(orc_poisson *P, const double *rhs, double *p)
+{
+    const orc_mesh *M = P->mesh;
+    const int h = M->h;
+    const int M0 = P->M[0], M1 = P->M[1], M2 = P->M[2];
+    const int Ny = M->N[1];
+    double complex *phi = P->buf;
+    /* gather (rhs over [h, n-1-h), i fastest); with walls, the y line is
+     * extended evenly: index j in [0, 2Ny) maps to node min(j, 2Ny - j). */
+    #pragma omp parallel for
+    for (int k = 0; k < M2; k++)
+        for (int j = 0; j < M1; j++) {
+            int jj = j;
+            if (P->ext && j > Ny) jj = 2 * Ny - j;
+            for (int i = 0; i < M0; i++)
+                phi[(size_t)i + (size_t)M0 * ((size_t)j + (size_t)M1 * k)] = rhs[IX(M, i + h, jj + h, k + h)];
+        }
+    orc_fft3d(P->M, (double *)phi, -1);
+    size_t total = (size_t)M0 * M1 * M2;
+    #pragma omp parallel for
+    for (size_t l = 0; l < total; l++) { /* poisson.cpp:231-249 */
+        if (P->k2[l] < ORC_EPS) phi[l] = 0.0;
+        else phi[l] = (-creal(phi[l]) / P->k2[l]) + I * (-cimag(phi[l]) / P->k2[l]);
+    }
+    orc_fft3d(P->M, (double *)phi, +1);
+    const int jmax = P->ext ? Ny + 1 : M1;
+    const double norm = (double)((long)M0 * (long)M1 * (long)M2);
+    #pragma omp parallel for
+    for (int k = 0; k < M2; k++) /* poisson.cpp:252-264 */
+        for (int j = 0; j < jmax; j++)
+            for (int i = 0; i < M0; i++)
+                p[IX(M, i + h, j + h, k + h)] = creal(phi[(size_t)i + (size_t)M0 * ((size_t)j + (size_t)M1 * k)]) / norm;
+}
+
+/* ------------------------------------------------------------------ */
+/* Field algebra helpers (src/Field/Field.hxx:617-1150: elementwise over
+ * the whole padded array). */
+static double *falloc(const orc_mesh *M, int nc)
+{
+    return (double *)calloc(NPTS(M) * (size_t)nc, sizeof(double));
+}
+
+/* dU = dt * ((-div(U,U)) + laplacian(nu,U)) [+ div(B,B) - grad(0.5*(B&B))]
+ * (app/TG_Piosson/RKLoop.H:17-22, app/OT_Poisson/UEqn.H:16-23). */
+static void rhs_u(const orc_mesh *M, const double *U, const double *B, double nu, double dt,
+                  double *dU, double *t1, double *t2, double *s1, double *s2)
+{
+    const size_t n3 = NPTS(M) * 3;
+    orc_div_adv(M, U, U, 3, t1);
+    orc_laplacian(M, nu, U, 3, t2);
+    #pragma omp parallel for
+    for (size_t q = 0; q < n3; q++) dU[q] = (-t1[q]) + t2[q];
+    if (B) {
+        orc_div_adv(M, B, B, 3, t1);
+        const size_t n1 = NPTS(M);
+        #pragma omp parallel for
+        for (size_t q = 0; q < n1; q++) /* vector.h inner product, then 0.5* */
+            s1[q] = 0.5 * (B[3 * q] * B[3 * q] + B[3 * q + 1] * B[3 * q + 1] + B[3 * q + 2] * B[3 * q + 2]);
+        orc_grad(M, s1, t2);
+        #pragma omp parallel for
+        for (size_t q = 0; q < n3; q++) dU[q] = (dU[q] + t1[q]) - t2[q];
+    }
+    (void)s2;
+    #pragma omp parallel for
+    for (size_t q = 0; q < n3; q++) dU[q] = dt * dU[q];
+}
+
+/* dB = dt * ((-div(U,B) + div(B,U)) + laplacian(eta,B)),
+ * app/OT_Poisson/BEqn.H:15-23. */
+static void rhs_b(const orc_mesh *M, const double *U, const double *B, double eta, double dt,
+                  double *dB, double *t1, double *t2)
+{
+    const size_t n3 = NPTS(M) * 3;
+    orc_div_adv(M, U, B, 3, t1);
+    orc_div_adv(M, B, U, 3, t2);
+    #pragma omp parallel for
+    for (size_t q = 0; q < n3; q++) dB[q] = (-t1[q]) + t2[q];
+    orc_laplacian(M, eta, B, 3, t1);
+    #pragma omp parallel for
+    for (size_t q = 0; q < n3; q++) dB[q] = dt * (dB[q] + t1[q]);
+}
+
+/* Projection F <- F - s*grad(p), lap p = div(F)/s (RKLoop.H:30-38 with
+ * s = dt; BEqn.H:37-41 with s = 1). */
+void orc_project(const orc_mesh *M, orc_poisson *P, double *F, double *p, double s)
+{
+    const size_t n1 = NPTS(M), n3 = n1 * 3;
+    double *d = falloc(M, 1);
+    double *g = falloc(M, 3);
+    orc_fill(M, F, 3, 1);
+    orc_div(M, F, d);
+    #pragma omp parallel for
+    for (size_t q = 0; q < n1; q++) d[q] = d[q] / s;
+    orc_poisson_solve(P, d, p);
+    orc_fill(M, p, 1, -1);
+    orc_grad(M, p, g);
+    #pragma omp parallel for
+    for (size_t q = 0; q < n3; q++) F[q] = F[q] - s * g[q];
+    orc_fill(M, F, 3, 1);
+    free(d);
+    free(g);
+}
+
+void orc_ns_step(const orc_mesh *M, orc_poisson *P, double *U, double *Ustar, double *p,
+                 double nu, double dt, int mode)
+{
+    const size_t n3 = NPTS(M) * 3;
+    double *U0 = falloc(M, 3), *dU = falloc(M, 3), *t1 = falloc(M, 3), *t2 = falloc(M, 3);
+    memcpy(U0, U, sizeof(double) * n3);
+    if (mode == ORC_RK_LITERAL4) {
+        /* RKLoop.H:15-29: every stage evaluates the RHS at the same U, so
+         * the four dU are equal; the last stage's accumulation is kept. */
+        rhs_u(M, U, NULL, nu, dt, dU, t1, t2, NULL, NULL);
+        #pragma omp parallel for
+        for (size_t q = 0; q < n3; q++) {
+            double s = U0[q];
+            for (int j = 0; j < 4; j++) s = s + RK4W[j] * dU[q];
+            Ustar[q] = s;
+        }
+        orc_project(M, P, Ustar, p, dt);
+        memcpy(U, Ustar, sizeof(double) * n3);
+    } else if (mode == ORC_RK_SSP3) {
+        /* U1 = U0 + L(U0); U2 = 3/4 U0 + 1/4 (U1 + L(U1));
+         * U3 = 1/3 U0 + 2/3 (U2 + L(U2)); then one projection. */
+        double *Us = Ustar;
+        memcpy(Us, U0, sizeof(double) * n3);
+        for (int s = 0; s < 3; s++) {
+            rhs_u(M, Us, NULL, nu, dt, dU, t1, t2, NULL, NULL);
+            #pragma omp parallel for
+            for (size_t q = 0; q < n3; q++) {
+                double v = Us[q] + dU[q];
+                if (s == 0) Us[q] = v;
+                else if (s == 1) Us[q] = 0.75 * U0[q] + 0.25 * v;
+                else Us[q] = (1.0 / 3.0) * U0[q] + (2.0 / 3.0) * v;
+            }
+            orc_fill(M, Us, 3, 1);
+        }
+        orc_project(M, P, Us, p, dt);
+        memcpy(U, Us, sizeof(double) * n3);
+    } else { /* ORC_RK_SHEAR4: app/shearLayer/UEqn.H:14-38 */
+        for (int rk = 0; rk < 4; rk++) {
+            rhs_u(M, U, NULL, nu, dt, dU, t1, t2, NULL, NULL);
+            #pragma omp parallel for
+            for (size_t q = 0; q < n3; q++) Ustar[q] = U0[q] + RK4W[rk] * dU[q];
+            orc_project(M, P, Ustar, p, dt * RK4C[rk]);
+            memcpy(U, Ustar, sizeof(double) * n3);
+        }
+    }
+    free(U0); free(dU); free(t1); free(t2);
+}
+
+void orc_mhd_step(const orc_mesh *M, orc_poisson *P, double *U, double *B, double *p,
+                  double *pB, double nu, double eta, double dt, int mode)
+{
+    const size_t n1 = NPTS(M), n3 = n1 * 3;
+    double *U0 = falloc(M, 3), *B0 = falloc(M, 3), *dU = falloc(M, 3), *dB[4];
+    double *t1 = falloc(M, 3), *t2 = falloc(M, 3), *s1 = falloc(M, 1);
+    for (int r = 0; r < 4; r++) dB[r] = falloc(M, 3);
+    memcpy(U0, U, sizeof(double) * n3);
+    memcpy(B0, B, sizeof(double) * n3);
+    if (mode == 0) {
+        /* UEqn.H:14-39 (frozen-stage RK4 + projection) */
+        rhs_u(M, U, B, nu, dt, dU, t1, t2, s1, NULL);
+        #pragma omp parallel for
+        for (size_t q = 0; q < n3; q++) {
+            double s = U0[q];
+            for (int j = 0; j < 4; j++) s = s + RK4W[j] * dU[q];
+            U[q] = s;
+        }
+        orc_project(M, P, U, p, dt);
+        /* BEqn.H:15-41 (staged RK4 of B with U frozen, then cleaning) */
+        static const double A[4][4] = {{0.5, 0, 0, 0}, {0, 0.5, 0, 0}, {0, 0, 1.0, 0},
+                                       {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0}};
+        for (int rk = 0; rk < 4; rk++) {
+            rhs_b(M, U, B, eta, dt, dB[rk], t1, t2);
+            #pragma omp parallel for
+            for (size_t q = 0; q < n3; q++) {
+                double s = B0[q];
+                for (int j = 0; j <= rk; j++) s = s + A[rk][j] * dB[j][q];
+                B[q] = s;
+            }
+            orc_fill(M, B, 3, 1);
+        }
+        orc_project(M, P, B, pB, 1.0);
+    } else {
+        double *Bs = dB[1], *Ut = dB[2], *Bt = dB[3];
+        for (int s = 0; s < 3; s++) {
+            rhs_u(M, U, B, nu, dt, dU, t1, t2, s1, NULL);
+            rhs_b(M, U, B, eta, dt, dB[0], t1, t2);
+            #pragma omp parallel for
+            for (size_t q = 0; q < n3; q++) {
+                double vu = U[q] + dU[q], vb = B[q] + dB[0][q];
+                if (s == 0) { Ut[q] = vu; Bt[q] = vb; }
+                else if (s == 1) { Ut[q] = 0.75 * U0[q] + 0.25 * vu; Bt[q] = 0.75 * B0[q] + 0.25 * vb; }
+                else { Ut[q] = (1.0 / 3.0) * U0[q] + (2.0 / 3.0) * vu; Bt[q] = (1.0 / 3.0) * B0[q] + (2.0 / 3.0) * vb; }
+            }
+            memcpy(U, Ut, sizeof(double) * n3);
+            memcpy(B, Bt, sizeof(double) * n3);
+            orc_project(M, P, U, p, dt);
+            orc_project(M, P, B, pB, 1.0);
+        }
+        (void)Bs;
+    }
+    free(U0); free(B0); free(dU); free(t1); free(t2); free(s1);
+    for (int r = 0; r < 4; r++) free(dB[r]);
+}
+
+/* ------------------------------------------------------------------ */
+/* Diagnostics. */
+
+/* app/TG_Piosson/TG.cpp:148-166: mean of 0.5|U|^2 over [h, n-1-h). */
+double orc_ke(const orc_mesh *M, const double *U)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    double Ek = 0.0;
+    long n = 0;
+    for (int i = h; i < ni - h - 1; i++)
+        for (int j = h; j < nj - h - 1; j++)
+            for (int k = h; k < nk - h - 1; k++) {
+                const double *u = &U[IX(M, i, j, k) * 3];
+                Ek += 0.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+                n++;
+            }
+    return Ek / n;
+}
+
+/* tools::maximum(ex::div(U)), src/Tools/Tools.h:19-36 over [h, n-h). */
+double orc_maxabs_div(const orc_mesh *M, const double *U)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    double *d = falloc(M, 1);
+    orc_div(M, U, d);
+    double mx = 0.0;
+    for (int i = h; i < ni - h; i++)
+        for (int j = h; j < nj - h; j++)
+            for (int k = h; k < nk - h; k++) {
+                double a = fabs(d[IX(M, i, j, k)]);
+                mx = a > mx ? a : mx;
+            }
+    free(d);
+    return mx;
+}
+
+/* app/OT_Poisson/OT.cpp:134-176: out = {Ek, Em, Jav, Jmax, omega,
+ * epsilon, Umax, 0} with J = curl(B), omega = curl(U) (literal curl). */
+void orc_mhd_diag(const orc_mesh *M, const double *U, const double *B, double mu, double eta,
+                  double out[8])
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    double *J = falloc(M, 3), *W = falloc(M, 3);
+    orc_curl_literal(M, B, J);
+    orc_curl_literal(M, U, W);
+    double Ek = 0, Em = 0, Jmax = 0, Umax = 0, Jav = 0, om = 0;
+    long n = 0;
+    for (int i = h; i < ni - h - 1; i++)
+        for (int j = h; j < nj - h - 1; j++)
+            for (int k = h; k < nk - h - 1; k++) {
+                const size_t q = IX(M, i, j, k) * 3;
+                const double *u = &U[q], *b = &B[q], *jj = &J[q], *w = &W[q];
+                Ek += 0.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+                Em += 0.5 * (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+                double jm = sqrt(jj[0] * jj[0] + jj[1] * jj[1] + jj[2] * jj[2]);
+                double um = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+                Jmax = Jmax > jm ? Jmax : jm;
+                Umax = Umax > um ? Umax : um;
+                om += sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+                Jav += jm;
+                n++;
+            }
+    Ek /= n; Em /= n; Jav /= n; om /= n;
+    out[0] = Ek; out[1] = Em; out[2] = Jav; out[3] = Jmax; out[4] = om;
+    out[5] = -mu * (om * om) - eta * (Jav * Jav);
+    out[6] = Umax; out[7] = 0.0;
+    free(J); free(W);
+}
+
+/* tools::tdma, src/Tools/Tools.cpp:43-62 (Thomas, no pivoting). */
+void orc_tdma(int n, const double *a, const double *b, const double *c, const double *d, double *x)
+{
+    double *cp = (double *)malloc(sizeof(double) * (size_t)n);
+    double *dp = (double *)malloc(sizeof(double) * (size_t)n);
+    cp[0] = c[0] / b[0];
+    dp[0] = d[0] / b[0];
+    for (int i = 1; i < n; i++) {
+        cp[i] = c[i] / (b[i] - a[i] * cp[i - 1]);
+        dp[i] = (d[i] - a[i] * dp[i - 1]) / (b[i] - a[i] * cp[i - 1]);
+    }
+    x[n - 1] = dp[n - 1];
+    for (int i = n - 2; i >= 0; i--) x[i] = dp[i] - cp[i] * x[i + 1];
+    free(cp);
+    free(dp);
+}

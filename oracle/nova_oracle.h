@@ -1,0 +1,114 @@
+/*
+ * nova_oracle.h -- CPU restatement of NOVA's per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: the parity tests,
+ * the smoke check and the bench's CPU-baseline leg call it.  The product
+ * (libnova_hip.so) never links, loads or calls it.
+ *
+ * Every array is in the reference's padded layout: a node-based grid with
+ * n_d = N_d + 1 + m nodes per direction (duplicated periodic endpoint plus
+ * m/2 ghost layers per side), array-of-structures over the components,
+ * index ((i*nj + j)*nk + k)*nc + c, i.e. k fastest
+ * (reference src/Field/Field.h:82-85, src/Mesh/Mesh.cpp:89-91,136-138).
+ */
+#ifndef NOVA_ORACLE_H
+#define NOVA_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* BC directions, same numbering as the reference (src/BC/BCenums.h:9):
+ * north = j high, south = j low, east = i low, west = i high,
+ * top = k high, bottom = k low. */
+enum { ORC_NORTH = 0, ORC_SOUTH, ORC_EAST, ORC_WEST, ORC_TOP, ORC_BOTTOM };
+
+/* BC kinds.  PERIODIC/NEUMANN/DIRICHLET follow the reference's literal
+ * ghost fills; EVEN/ODD are pure reflections about the wall node (EVEN keeps
+ * the wall node, ODD sets it to zero), used by the free-slip wall case. */
+enum { ORC_BC_PERIODIC = 0, ORC_BC_NEUMANN = 1, ORC_BC_DIRICHLET = 2,
+       ORC_BC_EVEN = 10, ORC_BC_ODD = 11 };
+
+/* Time-integration compositions. */
+enum {
+    ORC_RK_LITERAL4 = 0,   /* frozen-stage RK4 + one projection per step */
+    ORC_RK_SSP3     = 1,   /* SSP-RK3 (Shu-Osher) + one projection per step */
+    ORC_RK_SHEAR4   = 2    /* RK4 weights, projection at every stage */
+};
+
+typedef struct orc_mesh orc_mesh;
+typedef struct orc_poisson orc_poisson;
+
+/* Mesh (src/Mesh/Mesh.cpp:24-179, z(k) :373-407).  N = cells per direction,
+ * L = lengths, o = origin, m = stencil order, hyp/s1/s2 = optional
+ * hyperbolic z stretching.  wall_y != 0 declares reflecting walls at the two
+ * y ends (the y direction then has no periodic images). */
+orc_mesh *orc_mesh_create(const int N[3], const double L[3], const double o[3],
+                          int m, int hyp, double s1, double s2, int wall_y);
+void orc_mesh_free(orc_mesh *);
+void orc_mesh_dims(const orc_mesh *, int n_out[3]);
+void orc_mesh_metric(const orc_mesh *, double *dzdxi, double *d2zdxi2);
+double orc_mesh_z(const orc_mesh *, int k);
+void orc_coef(int m, double out[4]);
+
+/* Ghost fill.  Entries are applied in the given order, as
+ * Field::correctBoundaryConditions walks its BC list (Field.hxx:316-322). */
+void orc_bc_apply(const orc_mesh *, double *f, int nc, int nent,
+                  const int *dir, const int *kind, const int *comp,
+                  const double *val);
+/* The standard fill of the mesh: periodic in every direction without walls,
+ * EVEN/ODD reflections in y when wall_y is set (component `odd_comp` is odd;
+ * -1 means none). */
+void orc_fill(const orc_mesh *, double *f, int nc, int odd_comp);
+
+/* Explicit operators (src/ex).  `out` is overwritten with zero first, then
+ * accumulated over the interior [m/2, n-m/2), as the reference's pool
+ * temporaries are (src/reuseTmp/reuseTmp.hxx:47). */
+void orc_div_adv(const orc_mesh *, const double *U, const double *phi, int ncphi, double *out);
+void orc_div(const orc_mesh *, const double *U, double *out);
+void orc_grad(const orc_mesh *, const double *p, double *out);
+void orc_laplacian(const orc_mesh *, double nu, const double *f, int nc, double *out);
+void orc_grad_vec_literal(const orc_mesh *, const double *U, double *out9);
+void orc_curl_literal(const orc_mesh *, const double *U, double *out);
+void orc_curl_central(const orc_mesh *, const double *U, double *out);
+
+/* FFT (unnormalised; sign -1 forward, +1 backward), complex interleaved. */
+void orc_fft1d(int n, double *data, int sign);
+void orc_fft3d(const int n[3], double *data, int sign); /* data index i + n0*(j + n1*k) */
+
+/* Poisson (src/poisson/poisson.cpp:167-181,223-265,337-396). */
+orc_poisson *orc_poisson_create(const orc_mesh *);
+void orc_poisson_free(orc_poisson *);
+void orc_poisson_k2(const orc_poisson *, double *out); /* N0*N1*N2, i fastest */
+/* rhs and p are padded scalars; p's interior [m/2, n-1-m/2) is overwritten. */
+void orc_poisson_solve(orc_poisson *, const double *rhs, double *p);
+
+/* One time step of the incompressible TG/NS case (app/TG_Piosson/RKLoop.H
+ * and the declared compositions).  Ustar and p are caller-owned work/output
+ * fields (padded vector and scalar). */
+void orc_ns_step(const orc_mesh *, orc_poisson *, double *U, double *Ustar,
+                 double *p, double nu, double dt, int mode);
+/* One MHD step.  mode 0 = app/OT_Poisson/{UEqn,BEqn}.H literally;
+ * mode 1 = SSP-RK3 with u and B projected at every stage. */
+void orc_mhd_step(const orc_mesh *, orc_poisson *, double *U, double *B,
+                  double *p, double *pB, double nu, double eta, double dt, int mode);
+/* Projection: F <- F - s*grad(lap^-1(div(F)/s)), p receives the potential. */
+void orc_project(const orc_mesh *, orc_poisson *, double *F, double *p, double s);
+
+/* Diagnostics (app/TG_Piosson/TG.cpp:148-166, src/Tools/Tools.h:19-36,
+ * app/OT_Poisson/OT.cpp:134-181). */
+double orc_ke(const orc_mesh *, const double *U);
+double orc_maxabs_div(const orc_mesh *, const double *U);
+void orc_mhd_diag(const orc_mesh *, const double *U, const double *B,
+                  double mu, double eta, double out[8]);
+
+/* Batched tridiagonal Thomas solve, src/Tools/Tools.cpp:43-62 semantics. */
+void orc_tdma(int n, const double *a, const double *b, const double *c,
+              const double *d, double *x);
+
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

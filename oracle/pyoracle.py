@@ -1,0 +1,262 @@
+"""ctypes binding of the oracle library (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg
+import this module; it is the checker, never the product.  Fields are numpy
+arrays in the reference's padded AoS layout, shape (n_i, n_j, n_k, ncomp).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "libnova_oracle.so"
+
+NORTH, SOUTH, EAST, WEST, TOP, BOTTOM = range(6)
+BC_PERIODIC, BC_NEUMANN, BC_DIRICHLET, BC_EVEN, BC_ODD = 0, 1, 2, 10, 11
+RK_LITERAL4, RK_SSP3, RK_SHEAR4 = 0, 1, 2
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+_P = C.c_void_p
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def _load() -> C.CDLL:
+    if not LIB.exists():
+        build()
+    lib = C.CDLL(str(LIB))
+    sig = {
+        "orc_mesh_create": ([_I, _D, _D, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int], _P),
+        "orc_mesh_free": ([_P], None),
+        "orc_mesh_dims": ([_P, _I], None),
+        "orc_mesh_metric": ([_P, _D, _D], None),
+        "orc_mesh_z": ([_P, C.c_int], C.c_double),
+        "orc_coef": ([C.c_int, _D], None),
+        "orc_bc_apply": ([_P, _D, C.c_int, C.c_int, _I, _I, _I, _D], None),
+        "orc_fill": ([_P, _D, C.c_int, C.c_int], None),
+        "orc_div_adv": ([_P, _D, _D, C.c_int, _D], None),
+        "orc_div": ([_P, _D, _D], None),
+        "orc_grad": ([_P, _D, _D], None),
+        "orc_laplacian": ([_P, C.c_double, _D, C.c_int, _D], None),
+        "orc_grad_vec_literal": ([_P, _D, _D], None),
+        "orc_curl_literal": ([_P, _D, _D], None),
+        "orc_curl_central": ([_P, _D, _D], None),
+        "orc_fft1d": ([C.c_int, _D, C.c_int], None),
+        "orc_fft3d": ([_I, _D, C.c_int], None),
+        "orc_poisson_create": ([_P], _P),
+        "orc_poisson_free": ([_P], None),
+        "orc_poisson_k2": ([_P, _D], None),
+        "orc_poisson_solve": ([_P, _D, _D], None),
+        "orc_ns_step": ([_P, _P, _D, _D, _D, C.c_double, C.c_double, C.c_int], None),
+        "orc_mhd_step": ([_P, _P, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_int], None),
+        "orc_project": ([_P, _P, _D, _D, C.c_double], None),
+        "orc_ke": ([_P, _D], C.c_double),
+        "orc_maxabs_div": ([_P, _D], C.c_double),
+        "orc_mhd_diag": ([_P, _D, _D, C.c_double, C.c_double, _D], None),
+        "orc_tdma": ([C.c_int, _D, _D, _D, _D, _D], None),
+        "orc_num_threads": ([], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def _d(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+class OMesh:
+    def __init__(self, N, L, origin=(0.0, 0.0, 0.0), m=6, hyp=None, wall_y=False):
+        n = (C.c_int * 3)(*N)
+        l = (C.c_double * 3)(*L)
+        o = (C.c_double * 3)(*origin)
+        s1, s2 = hyp if hyp else (0.0, 0.0)
+        self._h = lib().orc_mesh_create(n, l, o, m, 1 if hyp else 0, s1, s2, 1 if wall_y else 0)
+        dims = (C.c_int * 3)()
+        lib().orc_mesh_dims(self._h, dims)
+        self.n = tuple(dims)
+        self.N = tuple(N)
+        self.L = tuple(L)
+        self.m = m
+        self.h = m // 2
+        self.wall_y = wall_y
+        self.stored = (N[0], N[1] + (1 if wall_y else 0), N[2])
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_mesh_free(self._h)
+            self._h = None
+
+    def field(self, nc=3):
+        return np.zeros((*self.n, nc))
+
+    def metric(self):
+        a = np.zeros(self.n[2])
+        b = np.zeros(self.n[2])
+        lib().orc_mesh_metric(self._h, _d(a), _d(b))
+        return a, b
+
+    # layout conversions
+    def interior(self, f: np.ndarray) -> np.ndarray:
+        """padded AoS -> compact SoA (ncomp, s0, s1, s2)."""
+        h = self.h
+        s0, s1, s2 = self.stored
+        return np.ascontiguousarray(np.moveaxis(f[h:h + s0, h:h + s1, h:h + s2, :], -1, 0))
+
+    def from_compact(self, c: np.ndarray, odd_comp=1) -> np.ndarray:
+        """compact SoA -> padded AoS with the standard ghost fill."""
+        nc = c.shape[0]
+        f = self.field(nc)
+        h = self.h
+        s0, s1, s2 = self.stored
+        f[h:h + s0, h:h + s1, h:h + s2, :] = np.moveaxis(c, 0, -1)
+        self.fill(f, odd_comp if nc == 3 else -1)
+        return f
+
+    def fill(self, f, odd_comp=1):
+        lib().orc_fill(self._h, _d(f), f.shape[-1], odd_comp)
+
+    def bc_apply(self, f, entries):
+        n = len(entries)
+        dirs = (C.c_int * max(n, 1))(*[e[0] for e in entries])
+        kinds = (C.c_int * max(n, 1))(*[e[1] for e in entries])
+        comps = (C.c_int * max(n, 1))(*[e[2] for e in entries])
+        vals = (C.c_double * max(n, 1))(*[float(e[3]) for e in entries])
+        lib().orc_bc_apply(self._h, _d(f), f.shape[-1], n, dirs, kinds, comps, vals)
+
+    def div_adv(self, U, phi):
+        out = self.field(phi.shape[-1])
+        lib().orc_div_adv(self._h, _d(U), _d(phi), phi.shape[-1], _d(out))
+        return out
+
+    def div(self, U):
+        out = self.field(1)
+        lib().orc_div(self._h, _d(U), _d(out))
+        return out
+
+    def grad(self, p):
+        out = self.field(3)
+        lib().orc_grad(self._h, _d(p), _d(out))
+        return out
+
+    def laplacian(self, nu, f):
+        out = self.field(f.shape[-1])
+        lib().orc_laplacian(self._h, nu, _d(f), f.shape[-1], _d(out))
+        return out
+
+    def grad_vec_literal(self, U):
+        out = self.field(9)
+        lib().orc_grad_vec_literal(self._h, _d(U), _d(out))
+        return out
+
+    def curl_literal(self, U):
+        out = self.field(3)
+        lib().orc_curl_literal(self._h, _d(U), _d(out))
+        return out
+
+    def curl_central(self, U):
+        out = self.field(3)
+        lib().orc_curl_central(self._h, _d(U), _d(out))
+        return out
+
+    def ke(self, U):
+        return lib().orc_ke(self._h, _d(U))
+
+    def maxabs_div(self, U):
+        return lib().orc_maxabs_div(self._h, _d(U))
+
+    def mhd_diag(self, U, B, mu, eta):
+        out = np.zeros(8)
+        lib().orc_mhd_diag(self._h, _d(U), _d(B), mu, eta, _d(out))
+        return out
+
+
+class OPoisson:
+    def __init__(self, mesh: OMesh):
+        self.mesh = mesh
+        self._h = lib().orc_poisson_create(mesh._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_poisson_free(self._h)
+            self._h = None
+
+    def k2(self):
+        N = self.mesh.N
+        M1 = 2 * N[1] if self.mesh.wall_y else N[1]
+        out = np.zeros(N[0] * M1 * N[2])
+        lib().orc_poisson_k2(self._h, _d(out))
+        return out.reshape(N[2], M1, N[0])  # [k][j][i]
+
+    def solve(self, rhs, p=None):
+        if p is None:
+            p = self.mesh.field(1)
+        lib().orc_poisson_solve(self._h, _d(rhs), _d(p))
+        return p
+
+    def project(self, F, s, p=None):
+        if p is None:
+            p = self.mesh.field(1)
+        lib().orc_project(self.mesh._h, self._h, _d(F), _d(p), s)
+        return p
+
+    def ns_step(self, U, p, nu, dt, mode, Ustar=None):
+        if Ustar is None:
+            Ustar = self.mesh.field(3)
+        lib().orc_ns_step(self.mesh._h, self._h, _d(U), _d(Ustar), _d(p), nu, dt, mode)
+
+    def mhd_step(self, U, B, p, pB, nu, eta, dt, mode):
+        lib().orc_mhd_step(self.mesh._h, self._h, _d(U), _d(B), _d(p), _d(pB), nu, eta, dt, mode)
+
+
+def coef(m):
+    out = np.zeros(4)
+    lib().orc_coef(m, _d(out))
+    return out
+
+
+def fft1d(x: np.ndarray, sign=-1) -> np.ndarray:
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    lib().orc_fft1d(a.size, a.view(np.float64).ctypes.data_as(_D), sign)
+    return a
+
+
+def fft3d(x: np.ndarray, sign=-1) -> np.ndarray:
+    """x has numpy shape (n2, n1, n0) (i fastest)."""
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    n = (C.c_int * 3)(a.shape[2], a.shape[1], a.shape[0])
+    lib().orc_fft3d(n, a.view(np.float64).ctypes.data_as(_D), sign)
+    return a
+
+
+def tdma(a, b, c, d):
+    n = len(b)
+    x = np.zeros(n)
+    arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, c, d)]
+    lib().orc_tdma(n, *[_d(v) for v in arrs], _d(x))
+    return x
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()

@@ -1,0 +1,301 @@
+"""Parity of the gfx950 library (through the C ABI) with the oracle.
+
+Inputs are seeded random or analytic fields; the oracle runs the reference
+semantics on the padded AoS layout, the library on the compact SoA layout,
+and the interiors are compared.  Bar: relative L2 <= 1e-12 per field
+(floating point, FP64), bitwise for pure data movement (ghost fills)."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TWO_PI = 2.0 * math.pi
+RTOL = 1e-12  # per-field relative L2 after one operator / step
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b.ravel())
+    return np.linalg.norm((a - b).ravel()) / (nb if nb > 0 else 1.0)
+
+
+@pytest.fixture(scope="module")
+def nv():
+    import torch
+    import novaclone_amd as nv
+    assert torch.cuda.is_available(), "GPU tests need a device"
+    nv.lib()
+    return nv
+
+
+def pair(nv, orc, N, L, m=6, wall_y=False):
+    bc = ("periodic", "wall" if wall_y else "periodic", "periodic")
+    G = nv.Mesh(N, L, m=m, bc=bc)
+    O = orc.OMesh(N, L, m=m, wall_y=wall_y)
+    assert G.shape == O.stored
+    return G, O
+
+
+def dev(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def wall_clean(O, c):
+    """Zero the wall-normal component on the walls (odd fields vanish there)."""
+    if O.wall_y and c.shape[0] == 3:
+        c = c.copy()
+        c[1, :, 0, :] = 0.0
+        c[1, :, -1, :] = 0.0
+    return c
+
+
+def rand_fields(O, seed, nc):
+    rng = np.random.default_rng(seed)
+    c = wall_clean(O, rng.standard_normal((nc, *O.stored)))
+    return c, O.from_compact(c)
+
+
+CASES = [
+    dict(N=(16, 24, 12), L=(1.0, 1.7, 0.8), m=6, wall_y=False),
+    dict(N=(12, 12, 16), L=(TWO_PI, TWO_PI, TWO_PI), m=4, wall_y=False),
+    dict(N=(16, 16, 16), L=(1.0, 1.0, 1.0), m=8, wall_y=False),
+    dict(N=(16, 12, 16), L=(2.0, 1.0, 2.0), m=6, wall_y=True),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['N']}-m{c['m']}-wall{int(c['wall_y'])}")
+def test_operators_match_oracle(nv, orc, case):
+    G, O = pair(nv, orc, **case)
+    print("case", case, flush=True)
+    cu, pu = rand_fields(O, 1, 3)
+    cs, ps = rand_fields(O, 2, 1)
+    cv, pv = rand_fields(O, 3, 3)
+    U, s, V = dev(cu), dev(cs), dev(cv)
+    chk = {
+        "div_adv(U,s)": (nv.ex.div(G, U, s), O.div_adv(pu, ps)),
+        "div_adv(U,V)": (nv.ex.div(G, U, V), O.div_adv(pu, pv)),
+        "div(U)": (nv.ex.div(G, U), O.div(pu)),
+        "grad(s)": (nv.ex.grad(G, s), O.grad(ps)),
+        "lap(s)": (nv.ex.laplacian(G, 0.3, s), O.laplacian(0.3, ps)),
+        "lap(V)": (nv.ex.laplacian(G, 0.3, V), O.laplacian(0.3, pv)),
+        "curl_literal(U)": (nv.ex.curl(G, U, literal=True), O.curl_literal(pu)),
+        "curl(U)": (nv.ex.curl(G, U, literal=False), O.curl_central(pu)),
+    }
+    for name, (g, o) in chk.items():
+        err = rel_l2(host(g), O.interior(o))
+        assert err <= RTOL, (name, err)
+
+
+@pytest.mark.parametrize("wall_y", [False, True])
+def test_pack_padded_equals_reference_ghost_fill(nv, orc, wall_y):
+    G, O = pair(nv, orc, (8, 10, 12), (1.0, 1.0, 1.0), wall_y=wall_y)
+    for nc in (1, 3):
+        c, p = rand_fields(O, 5 + nc, nc)
+        got = host(G.pack_padded(dev(c)))
+        assert np.array_equal(got, p)
+        back = host(G.unpack_padded(dev(p)))
+        assert np.array_equal(back, c)
+
+
+def test_reference_bc_entries_bitwise(nv, orc):
+    G, O = pair(nv, orc, (8, 8, 8), (1.0, 1.2, 0.9))
+    rng = np.random.default_rng(8)
+    f = rng.standard_normal((*O.n, 3))
+    entries = [(orc.NORTH, orc.BC_NEUMANN, 0, 0.25), (orc.SOUTH, orc.BC_NEUMANN, 0, -0.5),
+               (orc.EAST, orc.BC_PERIODIC, 0, 0.0), (orc.WEST, orc.BC_PERIODIC, 0, 0.0),
+               (orc.TOP, orc.BC_PERIODIC, 0, 0.0), (orc.BOTTOM, orc.BC_PERIODIC, 0, 0.0),
+               (orc.NORTH, orc.BC_DIRICHLET, 2, 1.5), (orc.SOUTH, orc.BC_DIRICHLET, 2, 0.0),
+               (orc.TOP, orc.BC_NEUMANN, 1, 0.7), (orc.BOTTOM, orc.BC_DIRICHLET, 1, -0.2),
+               (orc.EAST, orc.BC_ODD, 1, 0.0), (orc.WEST, orc.BC_EVEN, 2, 0.0)]
+    o = f.copy()
+    O.bc_apply(o, entries)
+    g = dev(f)
+    G.bc_fill_padded(g, entries)
+    assert np.array_equal(host(g), o)
+
+
+@pytest.mark.parametrize("N,L,wall_y", [((16, 24, 12), (1.0, 1.7, 0.8), False),
+                                         ((32, 32, 32), (TWO_PI,) * 3, False),
+                                         ((12, 16, 24), (1.0, 2.0, 1.0), True)])
+def test_poisson_and_projection_match_oracle(nv, orc, N, L, wall_y):
+    G, O = pair(nv, orc, N, L, wall_y=wall_y)
+    P = nv.Poisson(G)
+    OP = orc.OPoisson(O)
+    cs, ps = rand_fields(O, 11, 1)
+    p = P.solve(dev(cs))
+    po = OP.solve(ps)
+    assert rel_l2(host(p), O.interior(po)) <= RTOL
+    cu, pu = rand_fields(O, 12, 3)
+    U = dev(cu)
+    pg = G.field(1)
+    P.project(U, 0.01, pg)
+    po = OP.project(pu, 0.01)
+    assert rel_l2(host(U), O.interior(pu)) <= RTOL
+    assert rel_l2(host(pg), O.interior(po)) <= RTOL
+    assert nv.maxabs_div(G, U) < 1e-10
+
+
+def tg_compact(O):
+    x = [np.arange(O.stored[a]) * O.L[a] / O.N[a] for a in range(3)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    return np.stack([np.sin(X) * np.cos(Y) * np.cos(Z), -np.cos(X) * np.sin(Y) * np.cos(Z), 0 * X])
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("N", [(32, 32, 32), (24, 16, 32)])
+def test_ns_one_step_matches_oracle(nv, orc, mode, N):
+    G, O = pair(nv, orc, N, (TWO_PI,) * 3)
+    rng = np.random.default_rng(21)
+    c = tg_compact(O) + 1e-2 * rng.standard_normal((3, *O.stored))
+    Uo = O.from_compact(c)
+    po = O.field(1)
+    OP = orc.OPoisson(O)
+    OP.ns_step(Uo, po, 1.0 / 1600.0, 2e-3, mode)
+    U, p = dev(c), G.field(1)
+    S = nv.Solver(nv.Poisson(G), nv.MODEL_NS, mode, nu=1.0 / 1600.0)
+    S.step(2e-3, 1, U, p)
+    assert rel_l2(host(U), O.interior(Uo)) <= RTOL
+    assert rel_l2(host(p), O.interior(po)) <= 1e-10  # p is O(dt) small; its noise is relative to U
+
+
+def test_wall_case_step_matches_oracle(nv, orc):
+    N, L = (16, 24, 16), (4.0, 3.0, 4.0)
+    G, O = pair(nv, orc, N, L, wall_y=True)
+    rng = np.random.default_rng(31)
+    y = np.arange(O.stored[1]) * L[1] / N[1] - L[1] / 2
+    c = np.zeros((3, *O.stored))
+    c[0] = np.tanh(2 * y)[None, :, None]
+    c += wall_clean(O, 1e-2 * rng.standard_normal((3, *O.stored)))
+    OP = orc.OPoisson(O)
+    Uo = O.from_compact(c)
+    OP.project(Uo, 1.0)
+    c = O.interior(Uo)
+    po = O.field(1)
+    OP.ns_step(Uo, po, 1e-3, 0.02, 1)
+    U, p = dev(c), G.field(1)
+    S = nv.Solver(nv.Poisson(G), nv.MODEL_NS, nv.RK_SSP3, nu=1e-3)
+    S.step(0.02, 1, U, p)
+    assert rel_l2(host(U), O.interior(Uo)) <= RTOL
+
+
+def test_mhd_one_step_matches_oracle(nv, orc):
+    N = (24, 24, 24)
+    G, O = pair(nv, orc, N, (TWO_PI,) * 3)
+    x = [np.arange(O.stored[a]) * TWO_PI / N[a] for a in range(3)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    rng = np.random.default_rng(3)
+    cu = np.stack([-np.sin(Y), np.sin(X), 0 * X]) + 1e-3 * rng.standard_normal((3, *O.stored))
+    cb = np.stack([-np.sin(Y), np.sin(2 * X), 0 * X]) + 1e-3 * rng.standard_normal((3, *O.stored))
+    OP = orc.OPoisson(O)
+    Uo, Bo = O.from_compact(cu), O.from_compact(cb)
+    OP.project(Uo, 1.0)
+    OP.project(Bo, 1.0)
+    cu, cb = O.interior(Uo), O.interior(Bo)
+    po, pBo = O.field(1), O.field(1)
+    OP.mhd_step(Uo, Bo, po, pBo, 1e-3, 1e-3, 2e-3, 1)
+    U, B, p, pB = dev(cu), dev(cb), G.field(1), G.field(1)
+    S = nv.Solver(nv.Poisson(G), nv.MODEL_MHD, nv.RK_SSP3, nu=1e-3, eta=1e-3)
+    S.step(2e-3, 1, U, p, B, pB)
+    assert rel_l2(host(U), O.interior(Uo)) <= RTOL
+    assert rel_l2(host(B), O.interior(Bo)) <= RTOL
+    d = nv.mhd_diag(G, U, B, 1e-3, 1e-3)
+    do = O.mhd_diag(Uo, Bo, 1e-3, 1e-3)
+    assert np.allclose(d[:7], do[:7], rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_tg_histories_100_steps_match_oracle(nv, orc, mode):
+    """KE and dissipation histories within 1e-10 relative over 100 steps
+    (64^3, Re = 1600, dt = 2e-3); divergence stays at roundoff on both."""
+    N = (64, 64, 64)
+    G, O = pair(nv, orc, N, (TWO_PI,) * 3)
+    c = tg_compact(O)
+    Uo = O.from_compact(c)
+    po = O.field(1)
+    OP = orc.OPoisson(O)
+    U, p = dev(c), G.field(1)
+    S = nv.Solver(nv.Poisson(G), nv.MODEL_NS, mode, nu=1.0 / 1600.0)
+    dt = 2e-3
+    ke_g, ke_o = [], []
+    Ustar = O.field(3)
+    for step in range(100):
+        OP.ns_step(Uo, po, 1.0 / 1600.0, dt, mode, Ustar)
+        S.step(dt, 1, U, p)
+        ke_o.append(O.ke(Uo))
+        ke_g.append(nv.ke(G, U))
+        if step % 25 == 24:
+            print("step", step + 1, ke_g[-1], ke_o[-1], flush=True)
+    ke_g, ke_o = np.array(ke_g), np.array(ke_o)
+    assert np.max(np.abs(ke_g - ke_o) / ke_o) <= 1e-10
+    dis_g, dis_o = -np.diff(ke_g) / dt, -np.diff(ke_o) / dt
+    assert np.max(np.abs(dis_g - dis_o)) <= 1e-10 * np.max(np.abs(dis_o))
+    assert nv.maxabs_div(G, U) < 1e-10 and O.maxabs_div(Uo) < 1e-10
+    assert rel_l2(host(U), O.interior(Uo)) <= 1e-10
+
+
+def test_graph_replay_equals_eager(nv, orc):
+    N = (32, 32, 32)
+    G, O = pair(nv, orc, N, (TWO_PI,) * 3)
+    c = tg_compact(O)
+    P = nv.Poisson(G)
+    U1, p1 = dev(c), G.field(1)
+    U2, p2 = dev(c), G.field(1)
+    S1 = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-3)
+    S2 = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-3, graph=True)
+    for _ in range(3):  # replays interleaved with eager work and syncs
+        S1.step(2e-3, 2, U1, p1)
+        S2.step(2e-3, 2, U2, p2)
+        assert np.array_equal(host(U1), host(U2))
+        assert np.array_equal(host(p1), host(p2))
+
+
+def test_large_grid_properties(nv):
+    """256^3 TG: Ek(0) = 1/8, projection leaves roundoff divergence, a
+    step is deterministic bit for bit."""
+    import torch
+    N = (256, 256, 256)
+    G = nv.Mesh(N, (TWO_PI,) * 3)
+    x = torch.arange(256, dtype=torch.float64, device="cuda") * (TWO_PI / 256)
+    X, Y, Z = torch.meshgrid(x, x, x, indexing="ij")
+    U0 = torch.stack([torch.sin(X) * torch.cos(Y) * torch.cos(Z), -torch.cos(X) * torch.sin(Y) * torch.cos(Z),
+                      torch.zeros_like(X)]).contiguous()
+    del X, Y, Z
+    assert abs(nv.ke(G, U0) - 0.125) < 1e-14
+    P = nv.Poisson(G)
+    S = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1.0 / 1600.0)
+    Ua, Ub = U0.clone(), U0.clone()
+    pa, pb = G.field(1), G.field(1)
+    S.step(2e-3, 2, Ua, pa)
+    S.step(2e-3, 2, Ub, pb)
+    torch.cuda.synchronize()
+    assert torch.equal(Ua, Ub) and torch.equal(pa, pb)
+    assert nv.maxabs_div(G, Ua) < 1e-11
+    assert 0.1245 < nv.ke(G, Ua) < 0.125
+
+
+@pytest.mark.parametrize("cyclic", [False, True])
+def test_batched_tdma_matches_dense(nv, cyclic):
+    import torch
+    G = nv.Mesh((8, 8, 8), (1.0, 1.0, 1.0))
+    rng = np.random.default_rng(4)
+    nb, n = 64, 33
+    a, c = rng.standard_normal((nb, n)), rng.standard_normal((nb, n))
+    b = 4.0 + np.abs(rng.standard_normal((nb, n)))
+    d = rng.standard_normal((nb, n))
+    x = host(nv.tdma_batched(G, dev(a), dev(b), dev(c), dev(d), cyclic=cyclic))
+    for s in range(nb):
+        A = np.diag(b[s]) + np.diag(a[s, 1:], -1) + np.diag(c[s, :-1], 1)
+        if cyclic:
+            A[0, n - 1] = a[s, 0]
+            A[n - 1, 0] = c[s, n - 1]
+        assert np.allclose(x[s], np.linalg.solve(A, d[s]), atol=1e-12)
