@@ -87,6 +87,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of graph replay")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a plain device copy of 2^30 bytes 5 times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is design-only",
@@ -94,6 +96,14 @@ def main() -> int:
         return 2
 
     import torch
+    if args.copy_calib:
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("copy-calib: 5 copies of 1073741824 bytes")
+        return 0
     sys.path.insert(0, str(ROOT))
     import novaclone_amd as nv
 

@@ -90,6 +90,11 @@ const char *nova_last_error(void);
  * collect synchronises and writes "name launches total_ms" lines. */
 int nova_timing_enable(nova_ctx *ctx, int enable);
 int nova_timing_collect(nova_ctx *ctx, char *buf, size_t buflen);
+/* Launch-shape choice of the fused stage kernel (no reference
+ * counterpart): variant 0 stages each (j,k) plane in LDS, variant 1 streams
+ * rows without LDS; stage_ichunk = planes per streamed chunk (0 = auto).
+ * Results are identical up to rounding; only speed differs. */
+int nova_ctx_tune(nova_ctx *ctx, int stage_variant, int stage_ichunk);
 int nova_alloc(nova_ctx *ctx, int ncomp, double **dptr);
 int nova_free(nova_ctx *ctx, double *dptr);
 int nova_memcpy_h2d(nova_ctx *ctx, double *dst, const double *src, size_t count);
@@ -155,6 +160,10 @@ int nova_solver_use_graph(nova_solver *sv, int enable);
  * app/OT_Poisson/OT.cpp:134-176) ---- */
 int nova_reduce_ke(nova_ctx *ctx, const double *U, double *out);
 int nova_reduce_maxabs_div(nova_ctx *ctx, const double *U, double *out);
+/* nu * mean |grad U|^2 (central stencil per component): the viscous
+ * dissipation rate, the well-conditioned counterpart of the -dEk/dt column
+ * of app/TG_Piosson/TG.cpp:170. */
+int nova_reduce_dissipation(nova_ctx *ctx, const double *U, double nu, double *out);
 /* out = {Ek, Em, Jav, Jmax, omega, epsilon, Umax, 0} */
 int nova_mhd_diag(nova_ctx *ctx, const double *U, const double *B, double mu, double eta,
                   double out[8]);

@@ -62,6 +62,7 @@ _SIGS = {
     "nova_ctx_info": [_P, C.POINTER(C.c_int64)],
     "nova_ctx_metric": [_P, _DP, _DP],
     "nova_timing_enable": [_P, _I],
+    "nova_ctx_tune": [_P, _I, _I],
     "nova_timing_collect": [_P, C.c_char_p, C.c_size_t],
     "nova_alloc": [_P, _I, C.POINTER(_P)],
     "nova_free": [_P, _P],
@@ -87,6 +88,7 @@ _SIGS = {
     "nova_solver_use_graph": [_P, _I],
     "nova_reduce_ke": [_P, _P, _DP],
     "nova_reduce_maxabs_div": [_P, _P, _DP],
+    "nova_reduce_dissipation": [_P, _P, _D, _DP],
     "nova_mhd_diag": [_P, _P, _P, _D, _D, _DP],
     "nova_tdma_batched": [_P, _I, _I, _P, _P, _P, _P, _P, _I],
 }
@@ -178,6 +180,10 @@ class Mesh:
         """Issue work on torch's current stream."""
         s = torch.cuda.current_stream(self.device).cuda_stream
         _check(lib().nova_ctx_set_stream(self._h, C.c_void_p(s)), "nova_ctx_set_stream")
+
+    def tune(self, stage_variant: int = 0, stage_ichunk: int = 0) -> None:
+        """Pick the stage-kernel variant and its i-chunk (0 = automatic)."""
+        _check(lib().nova_ctx_tune(self._h, int(stage_variant), int(stage_ichunk)), "nova_ctx_tune")
 
     def timing(self, enable: bool) -> None:
         """Bracket every library kernel with HIP events (per-kernel times)."""
@@ -348,6 +354,12 @@ def ke(mesh: Mesh, U: torch.Tensor) -> float:
 def maxabs_div(mesh: Mesh, U: torch.Tensor) -> float:
     out = C.c_double()
     _check(lib().nova_reduce_maxabs_div(mesh.handle, _ptr(U), C.byref(out)), "nova_reduce_maxabs_div")
+    return out.value
+
+
+def dissipation(mesh: Mesh, U: torch.Tensor, nu: float) -> float:
+    out = C.c_double()
+    _check(lib().nova_reduce_dissipation(mesh.handle, _ptr(U), float(nu), C.byref(out)), "nova_reduce_dissipation")
     return out.value
 
 

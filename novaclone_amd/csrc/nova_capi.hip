@@ -313,6 +313,14 @@ int nova_timing_collect(nova_ctx *ctx, char *buf, size_t buflen)
     return NOVA_OK;
 }
 
+int nova_ctx_tune(nova_ctx *ctx, int stage_variant, int stage_ichunk)
+{
+    NOVA_REQUIRE(ctx && (stage_variant == 0 || stage_variant == 1) && stage_ichunk >= 0, "bad argument");
+    ctx->stage_variant = stage_variant;
+    ctx->stage_ichunk = stage_ichunk;
+    return NOVA_OK;
+}
+
 int nova_alloc(nova_ctx *ctx, int ncomp, double **dptr)
 {
     NOVA_REQUIRE(ctx && dptr && ncomp > 0, "bad argument");
@@ -464,6 +472,12 @@ int nova_reduce_maxabs_div(nova_ctx *ctx, const double *U, double *out)
     if (!rc) rc = reduce_maxabs(ctx, d, out);
     (void)hipFreeAsync(d, ctx->stream);
     return rc;
+}
+
+int nova_reduce_dissipation(nova_ctx *ctx, const double *U, double nu, double *out)
+{
+    NOVA_REQUIRE(ctx && U && out, "bad argument");
+    NOVA_GUARD(reduce_dissipation(ctx, U, nu, out));
 }
 
 int nova_mhd_diag(nova_ctx *ctx, const double *U, const double *B, double mu, double eta, double out[8])

@@ -35,11 +35,6 @@ struct FftLen {
     int *perm = nullptr;    // DIT load position of element e
 };
 
-struct FftDev {
-    int n, nr;
-    int radix[16];
-};
-
 }  // namespace nova
 
 struct nova_poisson {
@@ -72,89 +67,124 @@ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x -
 // multiply by -i (forward) or +i (inverse)
 __device__ inline double2 rot(double2 a, bool inv) { return inv ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x); }
 
-__device__ inline void dft_small(double2 *a, int R, bool inv)
+// ---- compile-time mixed-radix plans: radix 4 first, then 2, then 3 (the
+// host-side digit-reversal table uses the same order) ----
+constexpr int plan_nr(int n)
 {
-    if (R == 2) {
-        double2 t = a[0];
+    int c = 0;
+    while (n % 4 == 0) { n /= 4; c++; }
+    while (n % 2 == 0) { n /= 2; c++; }
+    while (n % 3 == 0) { n /= 3; c++; }
+    return n == 1 ? c : -1;
+}
+
+constexpr int plan_radix(int n, int s)
+{
+    int idx = 0;
+    while (n % 4 == 0) { if (idx == s) return 4; n /= 4; idx++; }
+    while (n % 2 == 0) { if (idx == s) return 2; n /= 2; idx++; }
+    while (n % 3 == 0) { if (idx == s) return 3; n /= 3; idx++; }
+    return 1;
+}
+
+constexpr int plan_lp(int n, int s)
+{
+    int acc = 1;
+    for (int t = 0; t < s; t++) acc *= plan_radix(n, t);
+    return acc;
+}
+
+template <int R, bool INV>
+__device__ inline void dft_small(double2 *a)
+{
+    if constexpr (R == 2) {
+        const double2 t = a[0];
         a[0] = cadd(t, a[1]);
         a[1] = csub(t, a[1]);
-    } else if (R == 4) {
-        double2 s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
-        double2 s13 = cadd(a[1], a[3]), d13 = rot(csub(a[1], a[3]), inv);
+    } else if constexpr (R == 4) {
+        const double2 s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+        const double2 s13 = cadd(a[1], a[3]), d13 = rot(csub(a[1], a[3]), INV);
         a[0] = cadd(s02, s13);
         a[2] = csub(s02, s13);
         a[1] = cadd(d02, d13);
         a[3] = csub(d02, d13);
-    } else if (R == 3) {
+    } else if constexpr (R == 3) {
         const double c = -0.5, s = 0.86602540378443864676;
-        double2 sum = cadd(a[1], a[2]);
-        double2 dif = csub(a[1], a[2]);
-        double2 t = make_double2(a[0].x + c * sum.x, a[0].y + c * sum.y);
-        double2 r = rot(make_double2(s * dif.x, s * dif.y), inv);  // -i s (a1-a2) forward
+        const double2 sum = cadd(a[1], a[2]);
+        const double2 dif = csub(a[1], a[2]);
+        const double2 t = make_double2(a[0].x + c * sum.x, a[0].y + c * sum.y);
+        const double2 r = rot(make_double2(s * dif.x, s * dif.y), INV);  // -i s (a1 - a2) forward
         a[0] = cadd(a[0], sum);
         a[1] = cadd(t, r);
         a[2] = csub(t, r);
     }
 }
 
-// In-place line FFTs in LDS.  Element e of line l sits at
-// buf[l*lstride + e*estride].  DIT expects digit-reversed input and leaves
-// natural order; DIF takes natural order and leaves digit-reversed output.
-// batch_fast picks which index consecutive threads walk (the one that is
-// contiguous in LDS).
-__device__ void lds_fft(double2 *buf, int nlines, int lstride, int estride, const FftDev &P,
-                        const double2 *tw, bool inv, bool dit, bool batch_fast)
+// One radix pass over NLINES line FFTs of length N held in LDS; element e
+// of line l at buf[l*LS + e*ES].  DIT twiddles before the butterfly, DIF
+// after it.  Consecutive threads walk the index that is contiguous in LDS.
+template <int N, int S, bool DIT, bool INV, int NLINES, int LS, int ES>
+__device__ inline void fft_pass(double2 *buf, const double2 *tw)
 {
-    const int n = P.n;
-    int Lps[16];
-    {
-        int acc = 1;
-        for (int s = 0; s < P.nr; s++) { Lps[s] = acc; acc *= P.radix[s]; }
-    }
-    for (int it = 0; it < P.nr; it++) {
-        const int s = dit ? it : P.nr - 1 - it;
-        const int R = P.radix[s];
-        const int Lp = Lps[s];
-        const int L = Lp * R;
-        const int nb = n / R;
-        const int tstep = n / L;
-        const int total = nb * nlines;
-        for (int w = threadIdx.x; w < total; w += blockDim.x) {
-            int line, bf;
-            if (batch_fast) { line = w % nlines; bf = w / nlines; }
-            else { line = w / nb; bf = w - line * nb; }
-            const int g = bf / Lp;
-            const int q = bf - g * Lp;
-            double2 *b0 = buf + line * lstride + (g * L + q) * estride;
-            const int st = Lp * estride;
-            double2 a[4];
-            for (int p = 0; p < R; p++) a[p] = b0[p * st];
-            if (dit) {
-                for (int p = 1; p < R; p++) {
-                    double2 wv = tw[p * q * tstep];
-                    if (inv) wv.y = -wv.y;
-                    a[p] = cmul(a[p], wv);
-                }
-                dft_small(a, R, inv);
-            } else {
-                dft_small(a, R, inv);
-                for (int p = 1; p < R; p++) {
-                    double2 wv = tw[p * q * tstep];
-                    if (inv) wv.y = -wv.y;
-                    a[p] = cmul(a[p], wv);
-                }
+    constexpr int R = plan_radix(N, S);
+    constexpr int Lp = plan_lp(N, S);
+    constexpr int L = Lp * R;
+    constexpr int NB = N / R;
+    constexpr int TSTEP = N / L;
+    constexpr int TOTAL = NB * NLINES;
+    constexpr bool BATCH_FAST = ES != 1;
+    for (int w = threadIdx.x; w < TOTAL; w += kFftThreads) {
+        int line, bf;
+        if constexpr (BATCH_FAST) { line = w % NLINES; bf = w / NLINES; }
+        else { line = w / NB; bf = w % NB; }
+        const int g = bf / Lp;
+        const int q = bf % Lp;
+        double2 *b0 = buf + line * LS + (g * L + q) * ES;
+        double2 a[R];
+#pragma unroll
+        for (int p = 0; p < R; p++) a[p] = b0[p * Lp * ES];
+        if constexpr (DIT) {
+#pragma unroll
+            for (int p = 1; p < R; p++) {
+                double2 wv = tw[p * q * TSTEP];
+                if (INV) wv.y = -wv.y;
+                a[p] = cmul(a[p], wv);
             }
-            for (int p = 0; p < R; p++) b0[p * st] = a[p];
+            dft_small<R, INV>(a);
+        } else {
+            dft_small<R, INV>(a);
+#pragma unroll
+            for (int p = 1; p < R; p++) {
+                double2 wv = tw[p * q * TSTEP];
+                if (INV) wv.y = -wv.y;
+                a[p] = cmul(a[p], wv);
+            }
         }
-        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < R; p++) b0[p * Lp * ES] = a[p];
+    }
+    __syncthreads();
+}
+
+// All passes: DIT expects digit-reversed input and leaves natural order;
+// DIF takes natural order and leaves digit-reversed output.
+template <int N, bool DIT, bool INV, int NLINES, int LS, int ES, int IT = 0>
+__device__ inline void lds_fft(double2 *buf, const double2 *tw)
+{
+    constexpr int NR = plan_nr(N);
+    static_assert(NR > 0, "FFT length must be 2^a 3^b");
+    if constexpr (IT < NR) {
+        fft_pass<N, DIT ? IT : NR - 1 - IT, DIT, INV, NLINES, LS, ES>(buf, tw);
+        lds_fft<N, DIT, INV, NLINES, LS, ES, IT + 1>(buf, tw);
     }
 }
 
-__device__ inline void load_tables(double2 *stw, int *sperm, const double2 *tw, const int *perm, int n)
+template <int N>
+__device__ inline void load_tables(double2 *stw, int *sperm, const double2 *tw, const int *perm)
 {
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    for (int e = threadIdx.x; e < N; e += kFftThreads) {
         stw[e] = tw[e];
-        if (sperm) sperm[e] = perm[e];
+        sperm[e] = perm[e];
     }
     __syncthreads();
 }
@@ -164,63 +194,85 @@ __device__ inline int64_t lin(const Grid &g, int i, int j, int k)
     return ((int64_t)i * g.s[1] + j) * g.s[2] + k;
 }
 
-__device__ inline double ldg(const double *__restrict__ f, const Grid &g, int nc, int c, int i, int j, int k)
+// Stored index of node q along a direction: one wrap suffices on a periodic
+// axis (|offset| <= 4 < n); walls reflect.
+template <bool WALLS>
+__device__ inline int wrapq(int q, int n, int wall, int &flip)
+{
+    if (!WALLS) {
+        flip = 0;
+        q += q < 0 ? n : 0;
+        q -= q >= n ? n : 0;
+        return q;
+    }
+    return bmap(q, n, wall, flip);
+}
+
+template <bool WALLS>
+__device__ inline double ldw(const double *__restrict__ f, const Grid &g, int nc, int c, int i, int j, int k)
 {
     int fi, fj, fk;
-    int ii = bmap(i, g.n[0], g.wall[0], fi);
-    int jj = bmap(j, g.n[1], g.wall[1], fj);
-    int kk = bmap(k, g.n[2], g.wall[2], fk);
-    double s = bsign(nc, c, 0, fi) * bsign(nc, c, 1, fj) * bsign(nc, c, 2, fk);
-    return s * f[c * g.np + lin(g, ii, jj, kk)];
+    const int ii = wrapq<WALLS>(i, g.n[0], g.wall[0], fi);
+    const int jj = wrapq<WALLS>(j, g.n[1], g.wall[1], fj);
+    const int kk = wrapq<WALLS>(k, g.n[2], g.wall[2], fk);
+    const double v = f[c * g.np + lin(g, ii, jj, kk)];
+    if (!WALLS) return v;
+    return bsign(nc, c, 0, fi) * bsign(nc, c, 1, fj) * bsign(nc, c, 2, fk) * v;
 }
 
 // div(F) * scale at a stored node (src/ex/div/div.hxx:37-77).
+template <bool WALLS>
 __device__ inline double div_at(const Grid &g, const double *__restrict__ F, int i, int j, int k, double scale)
 {
-    double acc = 0.0;
     double ax = 0.0, ay = 0.0, az = 0.0;
     for (int l = 0; l < g.h; l++) {
-        ax += g.a[0][l] * (ldg(F, g, 3, 0, i + 1 + l, j, k) - ldg(F, g, 3, 0, i - 1 - l, j, k));
-        ay += g.a[1][l] * (ldg(F, g, 3, 1, i, j + 1 + l, k) - ldg(F, g, 3, 1, i, j - 1 - l, k));
-        az += g.a[2][l] * (ldg(F, g, 3, 2, i, j, k + 1 + l) - ldg(F, g, 3, 2, i, j, k - 1 - l));
+        ax += g.a[0][l] * (ldw<WALLS>(F, g, 3, 0, i + 1 + l, j, k) - ldw<WALLS>(F, g, 3, 0, i - 1 - l, j, k));
+        ay += g.a[1][l] * (ldw<WALLS>(F, g, 3, 1, i, j + 1 + l, k) - ldw<WALLS>(F, g, 3, 1, i, j - 1 - l, k));
+        az += g.a[2][l] * (ldw<WALLS>(F, g, 3, 2, i, j, k + 1 + l) - ldw<WALLS>(F, g, 3, 2, i, j, k - 1 - l));
     }
-    acc = ax + ay + az * g.zinv[k];
-    return acc * scale;
+    return (ax + ay + az * g.zinv[k]) * scale;
 }
 
-constexpr int kZLines = 16;  // real lines per block in the k passes
+// Real lines per block in the k passes.
+template <int N>
+constexpr int zlines() { return N <= 384 ? 16 : (N <= 512 ? 8 : 4); }
+// Lines per block along j, and batch width along i.
+template <int N>
+constexpr int ylines() { return N >= 2048 ? 1 : (2048 / N > 16 ? 16 : 2048 / N); }
+template <int N>
+constexpr int xbatch_t() { return N >= 2048 ? 1 : (2048 / N > 8 ? 8 : 2048 / N); }
 
 // pass 1: source -> half spectrum H[kc][i][j].
-__global__ void __launch_bounds__(kFftThreads) k_fwd_z(Grid g, FftDev P, const double2 *__restrict__ tw,
+template <int N, bool WALLS>
+__global__ void __launch_bounds__(kFftThreads) k_fwd_z(Grid g, const double2 *__restrict__ tw,
                                                        const int *__restrict__ perm, const double *__restrict__ src,
                                                        int src_is_vector, double scale, double2 *__restrict__ H,
                                                        int Nx, int sy)
 {
-    extern __shared__ double2 smem[];
-    const int n = P.n;  // Nz
-    double2 *stw = smem;
-    int *sperm = (int *)(smem + n);
-    double2 *buf = smem + n + (n + 1) / 2;  // 8 complex lines
-    load_tables(stw, sperm, tw, perm, n);
+    constexpr int ZL = zlines<N>();
+    __shared__ double2 stw[N];
+    __shared__ int sperm[N];
+    __shared__ double2 buf[(ZL / 2) * N];
+    load_tables<N>(stw, sperm, tw, perm);
     const int i = blockIdx.y;
-    const int j0 = blockIdx.x * kZLines;
-    for (int e = threadIdx.x; e < kZLines * n; e += blockDim.x) {
-        const int jj = e / n, k = e - jj * n;
+    const int j0 = blockIdx.x * ZL;
+    for (int e = threadIdx.x; e < ZL * N; e += kFftThreads) {
+        const int jj = e / N, k = e % N;
         const int j = j0 + jj;
         double v = 0.0;
-        if (j < sy) v = src_is_vector ? div_at(g, src, i, j, k, scale) : src[lin(g, i, j, k)] * scale;
-        double *slot = (double *)&buf[(jj >> 1) * n + sperm[k]];
+        if (j < sy) v = src_is_vector ? div_at<WALLS>(g, src, i, j, k, scale) : src[lin(g, i, j, k)] * scale;
+        double *slot = (double *)&buf[(jj >> 1) * N + sperm[k]];
         slot[jj & 1] = v;
     }
     __syncthreads();
-    lds_fft(buf, kZLines / 2, n, 1, P, stw, false, true, false);
-    const int Kc = n / 2 + 1;
-    for (int e = threadIdx.x; e < kZLines * Kc; e += blockDim.x) {
-        const int jj = e % kZLines, kc = e / kZLines;
+    lds_fft<N, true, false, ZL / 2, N, 1>(buf, stw);
+    constexpr int Kc = N / 2 + 1;
+    for (int e = threadIdx.x; e < ZL * Kc; e += kFftThreads) {
+        const int jj = e % ZL, kc = e / ZL;
         const int j = j0 + jj;
         if (j >= sy) continue;
-        const double2 *Z = buf + (jj >> 1) * n;
-        const double2 z = Z[kc], zm = Z[(n - kc) % n];
+        const double2 *Z = buf + (jj >> 1) * N;
+        const double2 z = Z[kc], zm = Z[(N - kc) % N];
         double2 out;
         if ((jj & 1) == 0) out = make_double2(0.5 * (z.x + zm.x), 0.5 * (z.y - zm.y));
         else out = make_double2(0.5 * (z.y + zm.y), -0.5 * (z.x - zm.x));
@@ -228,68 +280,66 @@ __global__ void __launch_bounds__(kFftThreads) k_fwd_z(Grid g, FftDev P, const d
     }
 }
 
-// passes 2 and 4: lines along j (contiguous).  `ext` = even extension of
-// the sy = N_y + 1 stored nodes to length 2 N_y (DCT-I through the FFT).
-__global__ void __launch_bounds__(kFftThreads) k_fft_y(FftDev P, const double2 *__restrict__ tw,
-                                                       const int *__restrict__ perm, double2 *__restrict__ H,
-                                                       int64_t nlines_total, int sy, int lb, int ext, int inv)
+// passes 2 and 4: lines along j (contiguous).  EXT = even extension of the
+// sy = N_y + 1 stored nodes to length N = 2 N_y (DCT-I through the FFT).
+template <int N, bool EXT, bool INV>
+__global__ void __launch_bounds__(kFftThreads) k_fft_y(const double2 *__restrict__ tw, const int *__restrict__ perm,
+                                                       double2 *__restrict__ H, int64_t nlines_total, int sy)
 {
-    extern __shared__ double2 smem[];
-    const int n = P.n;
-    double2 *stw = smem;
-    int *sperm = (int *)(smem + n);
-    double2 *buf = smem + n + (n + 1) / 2;
-    load_tables(stw, sperm, tw, perm, n);
-    const int64_t l0 = (int64_t)blockIdx.x * lb;
-    const int Ny = ext ? sy - 1 : sy;
-    for (int e = threadIdx.x; e < lb * n; e += blockDim.x) {
-        const int ln = e / n, q = e - ln * n;
+    constexpr int LB = ylines<N>();
+    __shared__ double2 stw[N];
+    __shared__ int sperm[N];
+    __shared__ double2 buf[LB * N];
+    load_tables<N>(stw, sperm, tw, perm);
+    const int64_t l0 = (int64_t)blockIdx.x * LB;
+    constexpr int Ny = EXT ? N / 2 : N;
+    for (int e = threadIdx.x; e < LB * N; e += kFftThreads) {
+        const int ln = e / N, q = e % N;
         const int64_t L = l0 + ln;
         double2 v = make_double2(0.0, 0.0);
         if (L < nlines_total) {
-            const int src = (ext && q > Ny) ? 2 * Ny - q : q;
+            const int src = (EXT && q > Ny) ? 2 * Ny - q : q;
             v = H[L * sy + src];
         }
-        buf[ln * n + sperm[q]] = v;
+        buf[ln * N + sperm[q]] = v;
     }
     __syncthreads();
-    lds_fft(buf, lb, n, 1, P, stw, inv != 0, true, false);
-    for (int e = threadIdx.x; e < lb * sy; e += blockDim.x) {
-        const int ln = e / sy, q = e - ln * sy;
+    lds_fft<N, true, INV, LB, N, 1>(buf, stw);
+    for (int e = threadIdx.x; e < LB * sy; e += kFftThreads) {
+        const int ln = e / sy, q = e % sy;
         const int64_t L = l0 + ln;
-        if (L < nlines_total) H[L * sy + q] = buf[ln * n + q];
+        if (L < nlines_total) H[L * sy + q] = buf[ln * N + q];
     }
 }
 
-// pass 3: lines along i, batch of jb consecutive j; forward, divide by
+// pass 3: lines along i, a batch of consecutive j; forward, divide by
 // -k'^2, inverse.
-__global__ void __launch_bounds__(kFftThreads) k_fft_x(FftDev P, const double2 *__restrict__ tw,
-                                                       const int *__restrict__ perm, double2 *__restrict__ H,
-                                                       int Nx, int sy, int jb, const double *__restrict__ kpx,
-                                                       const double *__restrict__ kpy,
+template <int N>
+__global__ void __launch_bounds__(kFftThreads) k_fft_x(const double2 *__restrict__ tw, const int *__restrict__ perm,
+                                                       double2 *__restrict__ H, int sy,
+                                                       const double *__restrict__ kpx, const double *__restrict__ kpy,
                                                        const double *__restrict__ kpz)
 {
-    extern __shared__ double2 smem[];
-    const int n = P.n;  // Nx
-    double2 *stw = smem;
-    int *sperm = (int *)(smem + n);
-    double2 *buf = smem + n + (n + 1) / 2;
-    load_tables(stw, sperm, tw, perm, n);
+    constexpr int JB = xbatch_t<N>();
+    __shared__ double2 stw[N];
+    __shared__ int sperm[N];
+    __shared__ double2 buf[N * JB];
+    load_tables<N>(stw, sperm, tw, perm);
     const int kc = blockIdx.y;
-    const int j0 = blockIdx.x * jb;
-    const double2 *Hs = H + (int64_t)kc * Nx * sy;
-    for (int e = threadIdx.x; e < n * jb; e += blockDim.x) {
-        const int jj = e % jb, i = e / jb;
+    const int j0 = blockIdx.x * JB;
+    double2 *Hs = H + (int64_t)kc * N * sy;
+    for (int e = threadIdx.x; e < N * JB; e += kFftThreads) {
+        const int jj = e % JB, i = e / JB;
         const int j = j0 + jj;
         double2 v = make_double2(0.0, 0.0);
         if (j < sy) v = Hs[(int64_t)i * sy + j];
-        buf[sperm[i] * jb + jj] = v;
+        buf[sperm[i] * JB + jj] = v;
     }
     __syncthreads();
-    lds_fft(buf, jb, 1, jb, P, stw, false, true, true);
+    lds_fft<N, true, false, JB, 1, JB>(buf, stw);
     const double kz = kpz[kc];
-    for (int e = threadIdx.x; e < n * jb; e += blockDim.x) {
-        const int jj = e % jb, i = e / jb;
+    for (int e = threadIdx.x; e < N * JB; e += kFftThreads) {
+        const int jj = e % JB, i = e / JB;
         const int j = j0 + jj;
         if (j >= sy) continue;
         double k2;
@@ -298,70 +348,70 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x(FftDev P, const double2 *
             const double kx = kpx[i], ky = kpy[j];
             k2 = kx * kx + ky * ky + kz * kz;
         }
-        double2 v = buf[i * jb + jj];
+        double2 v = buf[i * JB + jj];
         if (k2 < kEps) v = make_double2(0.0, 0.0);
         else v = make_double2(-v.x / k2, -v.y / k2);
-        buf[i * jb + jj] = v;
+        buf[i * JB + jj] = v;
     }
     __syncthreads();
-    lds_fft(buf, jb, 1, jb, P, stw, true, false, true);
-    double2 *Hd = H + (int64_t)kc * Nx * sy;
-    for (int e = threadIdx.x; e < n * jb; e += blockDim.x) {
-        const int jj = e % jb, i = e / jb;
+    lds_fft<N, false, true, JB, 1, JB>(buf, stw);
+    for (int e = threadIdx.x; e < N * JB; e += kFftThreads) {
+        const int jj = e % JB, i = e / JB;
         const int j = j0 + jj;
-        if (j < sy) Hd[(int64_t)i * sy + j] = buf[sperm[i] * jb + jj];
+        if (j < sy) Hs[(int64_t)i * sy + j] = buf[sperm[i] * JB + jj];
     }
 }
 
 // pass 5: half spectrum -> real lines along k, p = v / norm.
-__global__ void __launch_bounds__(kFftThreads) k_inv_z(Grid g, FftDev P, const double2 *__restrict__ tw,
-                                                       const int *__restrict__ perm,
-                                                       const double2 *__restrict__ H, double *__restrict__ p,
-                                                       int Nx, int sy, double inv_norm)
+template <int N>
+__global__ void __launch_bounds__(kFftThreads) k_inv_z(Grid g, const double2 *__restrict__ tw,
+                                                       const int *__restrict__ perm, const double2 *__restrict__ H,
+                                                       double *__restrict__ p, int Nx, int sy, double inv_norm)
 {
-    extern __shared__ double2 smem[];
-    const int n = P.n;
-    double2 *stw = smem;
-    int *sperm = (int *)(smem + n);
-    double2 *buf = smem + n + (n + 1) / 2;
-    load_tables(stw, sperm, tw, perm, n);
+    constexpr int ZL = zlines<N>();
+    __shared__ double2 stw[N];
+    __shared__ int sperm[N];
+    __shared__ double2 buf[(ZL / 2) * N];
+    load_tables<N>(stw, sperm, tw, perm);
     const int i = blockIdx.y;
-    const int j0 = blockIdx.x * kZLines;
-    const int Kc = n / 2 + 1;
+    const int j0 = blockIdx.x * ZL;
+    constexpr int Kc = N / 2 + 1;
     // even j: A terms; odd j: i*B terms (Z = A + iB, Hermitian-extended)
+#pragma unroll
     for (int phase = 0; phase < 2; phase++) {
-        for (int e = threadIdx.x; e < (kZLines / 2) * Kc; e += blockDim.x) {
-            const int pr = e % (kZLines / 2), kc = e / (kZLines / 2);
+        for (int e = threadIdx.x; e < (ZL / 2) * Kc; e += kFftThreads) {
+            const int pr = e % (ZL / 2), kc = e / (ZL / 2);
             const int j = j0 + 2 * pr + phase;
             double2 v = make_double2(0.0, 0.0);
             if (j < sy) v = H[((int64_t)kc * Nx + i) * sy + j];
-            double2 *Z = buf + pr * n;
-            const bool mirror = kc > 0 && 2 * kc < n;
+            double2 *Z = buf + pr * N;
+            const bool mirror = kc > 0 && 2 * kc < N;
             if (phase == 0) {
                 Z[sperm[kc]] = v;
-                if (mirror) Z[sperm[n - kc]] = make_double2(v.x, -v.y);
+                if (mirror) Z[sperm[N - kc]] = make_double2(v.x, -v.y);
             } else {
-                double2 t = Z[sperm[kc]];
+                const double2 t = Z[sperm[kc]];
                 Z[sperm[kc]] = make_double2(t.x - v.y, t.y + v.x);
                 if (mirror) {
-                    double2 u = Z[sperm[n - kc]];  // + i * conj(v)
-                    Z[sperm[n - kc]] = make_double2(u.x + v.y, u.y + v.x);
+                    const double2 u = Z[sperm[N - kc]];  // + i * conj(v)
+                    Z[sperm[N - kc]] = make_double2(u.x + v.y, u.y + v.x);
                 }
             }
         }
         __syncthreads();
     }
-    lds_fft(buf, kZLines / 2, n, 1, P, stw, true, true, false);
-    for (int e = threadIdx.x; e < kZLines * n; e += blockDim.x) {
-        const int jj = e / n, k = e - jj * n;
+    lds_fft<N, true, true, ZL / 2, N, 1>(buf, stw);
+    for (int e = threadIdx.x; e < ZL * N; e += kFftThreads) {
+        const int jj = e / N, k = e % N;
         const int j = j0 + jj;
         if (j >= sy) continue;
-        const double2 z = buf[(jj >> 1) * n + k];
+        const double2 z = buf[(jj >> 1) * N + k];
         p[lin(g, i, j, k)] = ((jj & 1) ? z.y : z.x) * inv_norm;
     }
 }
 
 // F <- F - s * grad(p) (app/TG_Piosson/RKLoop.H:36, app/OT_Poisson/BEqn.H:40).
+template <bool WALLS>
 __global__ void k_correct(Grid g, const double *__restrict__ p, double s, double *__restrict__ F)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -370,15 +420,105 @@ __global__ void k_correct(Grid g, const double *__restrict__ p, double s, double
     if (k >= g.s[2] || j >= g.s[1]) return;
     double gx = 0.0, gy = 0.0, gz = 0.0;
     for (int l = 0; l < g.h; l++) {
-        gx += g.a[0][l] * (ldg(p, g, 1, 0, i + 1 + l, j, k) - ldg(p, g, 1, 0, i - 1 - l, j, k));
-        gy += g.a[1][l] * (ldg(p, g, 1, 0, i, j + 1 + l, k) - ldg(p, g, 1, 0, i, j - 1 - l, k));
-        gz += g.a[2][l] * (ldg(p, g, 1, 0, i, j, k + 1 + l) - ldg(p, g, 1, 0, i, j, k - 1 - l));
+        gx += g.a[0][l] * (ldw<WALLS>(p, g, 1, 0, i + 1 + l, j, k) - ldw<WALLS>(p, g, 1, 0, i - 1 - l, j, k));
+        gy += g.a[1][l] * (ldw<WALLS>(p, g, 1, 0, i, j + 1 + l, k) - ldw<WALLS>(p, g, 1, 0, i, j - 1 - l, k));
+        gz += g.a[2][l] * (ldw<WALLS>(p, g, 1, 0, i, j, k + 1 + l) - ldw<WALLS>(p, g, 1, 0, i, j, k - 1 - l));
     }
     gz *= g.zinv[k];
     const int64_t q = lin(g, i, j, k);
     F[q] = F[q] - s * gx;
     F[g.np + q] = F[g.np + q] - s * gy;
     F[2 * g.np + q] = F[2 * g.np + q] - s * gz;
+}
+
+// Launchers over the supported transform lengths.
+#define NOVA_FFT_LENGTHS(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768)
+
+bool fft_length_supported(int n)
+{
+    switch (n) {
+#define CASE(L) case L:
+        NOVA_FFT_LENGTHS(CASE)
+#undef CASE
+        return true;
+    }
+    return false;
+}
+
+int launch_fwd_z(hipStream_t st, int n, dim3 grid, const Grid &g, const double2 *tw, const int *perm,
+                 const double *src, int vec, double scale, double2 *H, int Nx, int sy, bool walls)
+{
+    switch (n) {
+#define CASE(L)                                                                                                    \
+    case L:                                                                                                        \
+        if (walls) hipLaunchKernelGGL((k_fwd_z<L, true>), grid, dim3(kFftThreads), 0, st, g, tw, perm, src, vec,   \
+                                      scale, H, Nx, sy);                                                           \
+        else hipLaunchKernelGGL((k_fwd_z<L, false>), grid, dim3(kFftThreads), 0, st, g, tw, perm, src, vec, scale, \
+                                H, Nx, sy);                                                                        \
+        return NOVA_OK;
+        NOVA_FFT_LENGTHS(CASE)
+#undef CASE
+    }
+    return fail(NOVA_ERR_UNSUPPORTED, "FFT length");
+}
+
+int launch_fft_y(hipStream_t st, int n, bool ext, bool inv, const double2 *tw, const int *perm, double2 *H,
+                 int64_t nlines, int sy)
+{
+    switch (n) {
+#define CASE(L)                                                                                   \
+    case L: {                                                                                     \
+        const unsigned blocks = (unsigned)((nlines + ylines<L>() - 1) / ylines<L>());             \
+        if (ext && inv) hipLaunchKernelGGL((k_fft_y<L, true, true>), dim3(blocks), dim3(kFftThreads), 0, st, tw, perm, H, nlines, sy);   \
+        else if (ext) hipLaunchKernelGGL((k_fft_y<L, true, false>), dim3(blocks), dim3(kFftThreads), 0, st, tw, perm, H, nlines, sy);   \
+        else if (inv) hipLaunchKernelGGL((k_fft_y<L, false, true>), dim3(blocks), dim3(kFftThreads), 0, st, tw, perm, H, nlines, sy);   \
+        else hipLaunchKernelGGL((k_fft_y<L, false, false>), dim3(blocks), dim3(kFftThreads), 0, st, tw, perm, H, nlines, sy);          \
+        return NOVA_OK;                                                                           \
+    }
+        NOVA_FFT_LENGTHS(CASE)
+#undef CASE
+    }
+    return fail(NOVA_ERR_UNSUPPORTED, "FFT length");
+}
+
+int launch_fft_x(hipStream_t st, int n, int Kc, const double2 *tw, const int *perm, double2 *H, int sy,
+                 const double *kpx, const double *kpy, const double *kpz)
+{
+    switch (n) {
+#define CASE(L)                                                                                           \
+    case L: {                                                                                             \
+        dim3 grid((sy + xbatch_t<L>() - 1) / xbatch_t<L>(), Kc);                                           \
+        hipLaunchKernelGGL((k_fft_x<L>), grid, dim3(kFftThreads), 0, st, tw, perm, H, sy, kpx, kpy, kpz); \
+        return NOVA_OK;                                                                                   \
+    }
+        NOVA_FFT_LENGTHS(CASE)
+#undef CASE
+    }
+    return fail(NOVA_ERR_UNSUPPORTED, "FFT length");
+}
+
+int launch_inv_z(hipStream_t st, int n, dim3 grid, const Grid &g, const double2 *tw, const int *perm,
+                 const double2 *H, double *p, int Nx, int sy, double inv_norm)
+{
+    switch (n) {
+#define CASE(L)                                                                                                 \
+    case L:                                                                                                     \
+        hipLaunchKernelGGL((k_inv_z<L>), grid, dim3(kFftThreads), 0, st, g, tw, perm, H, p, Nx, sy, inv_norm); \
+        return NOVA_OK;
+        NOVA_FFT_LENGTHS(CASE)
+#undef CASE
+    }
+    return fail(NOVA_ERR_UNSUPPORTED, "FFT length");
+}
+
+int zlines_rt(int n)
+{
+    switch (n) {
+#define CASE(L) case L: return zlines<L>();
+        NOVA_FFT_LENGTHS(CASE)
+#undef CASE
+    }
+    return 16;
 }
 
 // ---- host side ----
@@ -430,30 +570,7 @@ void free_len(FftLen &f)
     f.perm = nullptr;
 }
 
-FftDev dev(const FftLen &f)
-{
-    FftDev d;
-    d.n = f.n;
-    d.nr = f.nr;
-    for (int s = 0; s < 16; s++) d.radix[s] = f.radix[s];
-    return d;
-}
 
-size_t smem_bytes(int n, int lines) { return sizeof(double2) * ((size_t)n + (n + 1) / 2 + (size_t)n * lines); }
-
-constexpr size_t kLdsLimit = 64 * 1024;  // dynamic LDS per block without opt-in
-
-int ylines_per_block(int n)
-{
-    int lb = 2048 / n;
-    return lb < 1 ? 1 : (lb > 16 ? 16 : lb);
-}
-
-int xbatch(int n)
-{
-    int jb = 2048 / n;
-    return jb < 1 ? 1 : (jb > 8 ? 8 : jb);
-}
 
 // poisson::wavenumbersFFT_ modified wavenumber (poisson.cpp:365-389).
 double kmod(double kk, double h, const double *coef, int hm)
@@ -487,11 +604,10 @@ int poisson_create(nova_ctx *ctx, nova_poisson **out)
         delete ps;
         return rc;
     }
-    if (smem_bytes(ps->Nz, kZLines / 2) > kLdsLimit || smem_bytes(ps->Nx, xbatch(ps->Nx)) > kLdsLimit ||
-        smem_bytes(ps->Ty, ylines_per_block(ps->Ty)) > kLdsLimit) {
+    if (!fft_length_supported(ps->Nz) || !fft_length_supported(ps->Nx) || !fft_length_supported(ps->Ty)) {
         free_len(ps->fx); free_len(ps->fy); free_len(ps->fz);
         delete ps;
-        return fail(NOVA_ERR_UNSUPPORTED, "grid too large for the LDS line FFT");
+        return fail(NOVA_ERR_UNSUPPORTED, "FFT lengths must be one of 8..768 (2^a 3^b, listed in nova_fft.hip)");
     }
     const size_t nh = (size_t)ps->Kc * ps->Nx * ps->sy;
     NOVA_HIP(hipMalloc((void **)&ps->H, sizeof(double2) * nh));
@@ -538,44 +654,37 @@ int poisson_run(nova_poisson *ps, const double *src, int src_is_vector, double s
     nova_ctx *ctx = ps->ctx;
     const Grid &g = ctx->g;
     hipStream_t st = ctx->stream;
-    const FftDev dz = dev(ps->fz), dy = dev(ps->fy), dx = dev(ps->fx);
+    const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
+    const int zl = zlines_rt(ps->Nz);
+    const dim3 zgrid((ps->sy + zl - 1) / zl, ps->Nx);
+    const int64_t ylines_total = (int64_t)ps->Kc * ps->Nx;
+    int rc;
     {
-        dim3 grid((ps->sy + kZLines - 1) / kZLines, ps->Nx);
         TimedScope ts(ctx, src_is_vector ? "fft_div_r2c_z" : "fft_r2c_z");
-        hipLaunchKernelGGL(k_fwd_z, grid, dim3(kFftThreads), smem_bytes(ps->Nz, kZLines / 2), st, g, dz, ps->fz.tw,
-                           ps->fz.perm, src, src_is_vector, scale, ps->H, ps->Nx, ps->sy);
-        if (int rc = check_launch("poisson fwd_z")) return rc;
+        rc = launch_fwd_z(st, ps->Nz, zgrid, g, ps->fz.tw, ps->fz.perm, src, src_is_vector, scale, ps->H, ps->Nx,
+                          ps->sy, walls);
     }
-    const int64_t ylines = (int64_t)ps->Kc * ps->Nx;
-    const int lb = ylines_per_block(ps->Ty);
-    const unsigned yblocks = (unsigned)((ylines + lb - 1) / lb);
+    if (rc || (rc = check_launch("poisson fwd_z"))) return rc;
     {
-    TimedScope ts(ctx, "fft_y");
-    hipLaunchKernelGGL(k_fft_y, dim3(yblocks), dim3(kFftThreads), smem_bytes(ps->Ty, lb), st, dy, ps->fy.tw,
-                       ps->fy.perm, ps->H, ylines, ps->sy, lb, ps->wall_y, 0);
+        TimedScope ts(ctx, "fft_y");
+        rc = launch_fft_y(st, ps->Ty, ps->wall_y, false, ps->fy.tw, ps->fy.perm, ps->H, ylines_total, ps->sy);
     }
-    if (int rc = check_launch("poisson fft_y")) return rc;
-    const int jb = xbatch(ps->Nx);
+    if (rc || (rc = check_launch("poisson fft_y"))) return rc;
     {
-        dim3 grid((ps->sy + jb - 1) / jb, ps->Kc);
         TimedScope ts(ctx, "fft_x_solve");
-        hipLaunchKernelGGL(k_fft_x, grid, dim3(kFftThreads), smem_bytes(ps->Nx, jb), st, dx, ps->fx.tw,
-                           ps->fx.perm, ps->H, ps->Nx, ps->sy, jb, ps->kpx, ps->kpy, ps->kpz);
-        if (int rc = check_launch("poisson fft_x")) return rc;
+        rc = launch_fft_x(st, ps->Nx, ps->Kc, ps->fx.tw, ps->fx.perm, ps->H, ps->sy, ps->kpx, ps->kpy, ps->kpz);
     }
+    if (rc || (rc = check_launch("poisson fft_x"))) return rc;
     {
-    TimedScope ts(ctx, "fft_y");
-    hipLaunchKernelGGL(k_fft_y, dim3(yblocks), dim3(kFftThreads), smem_bytes(ps->Ty, lb), st, dy, ps->fy.tw,
-                       ps->fy.perm, ps->H, ylines, ps->sy, lb, ps->wall_y, 1);
+        TimedScope ts(ctx, "fft_y");
+        rc = launch_fft_y(st, ps->Ty, ps->wall_y, true, ps->fy.tw, ps->fy.perm, ps->H, ylines_total, ps->sy);
     }
-    if (int rc = check_launch("poisson ifft_y")) return rc;
+    if (rc || (rc = check_launch("poisson ifft_y"))) return rc;
     {
-        dim3 grid((ps->sy + kZLines - 1) / kZLines, ps->Nx);
         TimedScope ts(ctx, "fft_c2r_z");
-        hipLaunchKernelGGL(k_inv_z, grid, dim3(kFftThreads), smem_bytes(ps->Nz, kZLines / 2), st, g, dz, ps->fz.tw,
-                           ps->fz.perm, ps->H, p, ps->Nx, ps->sy, 1.0 / ps->norm);
-        if (int rc = check_launch("poisson inv_z")) return rc;
+        rc = launch_inv_z(st, ps->Nz, zgrid, g, ps->fz.tw, ps->fz.perm, ps->H, p, ps->Nx, ps->sy, 1.0 / ps->norm);
     }
+    if (rc || (rc = check_launch("poisson inv_z"))) return rc;
     return NOVA_OK;
 }
 
@@ -595,7 +704,10 @@ int project(nova_poisson *ps, double *F, double s, double *p)
     const Grid &g = ctx->g;
     dim3 grid((g.s[2] + 63) / 64, (g.s[1] + 3) / 4, g.s[0]);
     TimedScope ts(ctx, "grad_correct");
-    hipLaunchKernelGGL(k_correct, grid, dim3(64, 4, 1), 0, ctx->stream, g, p, s, F);
+    if (g.wall[0] || g.wall[1] || g.wall[2])
+        hipLaunchKernelGGL(k_correct<true>, grid, dim3(64, 4, 1), 0, ctx->stream, g, p, s, F);
+    else
+        hipLaunchKernelGGL(k_correct<false>, grid, dim3(64, 4, 1), 0, ctx->stream, g, p, s, F);
     return check_launch("project correct");
 }
 

@@ -78,6 +78,8 @@ struct TimedLaunch {
 struct nova_ctx {
     int device = 0;
     int timing = 0;                        // record events around launches
+    int stage_variant = 0;                 // 0: LDS-staged planes, 1: row streams without LDS
+    int stage_ichunk = 0;                  // planes per streamed chunk (0: automatic)
     std::vector<nova::TimedLaunch> tlog;   // launches since the last collect
     std::vector<hipEvent_t> evpool;        // reusable events
     hipStream_t stream = nullptr;
@@ -134,6 +136,7 @@ int launch_unpack(nova_ctx *ctx, const double *padded, int nc, double *compact);
 int launch_bc_fill(nova_ctx *ctx, double *padded, int nc, int dir, int kind, int comp, double val);
 int reduce_ke(nova_ctx *ctx, const double *U, double *out);
 int reduce_maxabs(nova_ctx *ctx, const double *f, double *out);
+int reduce_dissipation(nova_ctx *ctx, const double *U, double nu, double *out);
 int reduce_mhd(nova_ctx *ctx, const double *U, const double *B, const double *J,
                const double *W, double mu, double eta, double out[8]);
 int launch_tdma(nova_ctx *ctx, int n, int nb, const double *a, const double *b, const double *c,

@@ -205,6 +205,7 @@ __global__ void k_unpack(Grid g, int h, int p1, int p2, const double *__restrict
 __global__ void k_bc_entry(int p0, int p1, int p2, int h, double hd, double *__restrict__ f, int nc,
                            int axis, int low, int kind, int comp, double val)
 {
+#pragma clang fp contract(off)
     const int a = blockIdx.y * blockDim.y + threadIdx.y;  // first transverse index
     const int b = blockIdx.x * blockDim.x + threadIdx.x;  // second transverse index
     const int na = axis == 0 ? p1 : p0, nb = axis == 2 ? p1 : p2;
@@ -303,6 +304,35 @@ __global__ void k_ke_part(Grid g, const double *__restrict__ U, double *__restri
         const int64_t o = lin(g, i, j, k);
         const double u = U[o], v = U[g.np + o], w = U[2 * g.np + o];
         a += 0.5 * (u * u + v * v + w * w);
+    }
+    double v1[1] = {a};
+    const int mx[1] = {0};
+    block_reduce<1>(v1, mx, part);
+}
+
+// nu * mean over nodes [0, n_d) of sum_{c,d} (D_d U_c)^2 with the central
+// stencil of src/ex/grad/grad.cpp:33-58 applied to each component: the
+// viscous dissipation rate of the velocity field.
+__global__ void k_diss_part(Grid g, const double *__restrict__ U, double *__restrict__ part)
+{
+    const int64_t tot = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    double a = 0.0;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(q % g.n[2]);
+        const int64_t r = q / g.n[2];
+        const int j = (int)(r % g.n[1]);
+        const int i = (int)(r / g.n[1]);
+        for (int c = 0; c < 3; c++)
+            for (int d = 0; d < 3; d++) {
+                double ad = 0.0;
+                for (int l = 0; l < g.h; l++) {
+                    int di, dj, dk;
+                    offs(d, l + 1, di, dj, dk);
+                    ad += g.a[d][l] * (ld(U, g, 3, c, i + di, j + dj, k + dk) - ld(U, g, 3, c, i - di, j - dj, k - dk));
+                }
+                if (d == 2) ad *= g.zinv[k];
+                a += ad * ad;
+            }
     }
     double v1[1] = {a};
     const int mx[1] = {0};
@@ -506,6 +536,22 @@ int reduce_ke(nova_ctx *ctx, const double *U, double *out)
     NOVA_HIP(hipMemcpyAsync(ctx->h_red, ctx->d_red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     NOVA_HIP(hipStreamSynchronize(ctx->stream));
     *out = ctx->h_red[0] / (double)tot;
+    return NOVA_OK;
+}
+
+int reduce_dissipation(nova_ctx *ctx, const double *U, double nu, double *out)
+{
+    const Grid &g = ctx->g;
+    const int64_t tot = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    const int nb = red_blocks(tot);
+    if (int rc = ensure_red(ctx, (size_t)nb * 8 + 8)) return rc;
+    hipLaunchKernelGGL(k_diss_part, dim3(nb), dim3(kRedThreads), 0, ctx->stream, g, U, ctx->d_red + 8);
+    hipLaunchKernelGGL(k_red_final<1>, dim3(1), dim3(kRedThreads), 0, ctx->stream, ctx->d_red + 8, nb, ctx->d_red,
+                       0, 0, 0, 0, 0, 0, 0, 0);
+    if (int rc = check_launch("reduce_dissipation")) return rc;
+    NOVA_HIP(hipMemcpyAsync(ctx->h_red, ctx->d_red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    NOVA_HIP(hipStreamSynchronize(ctx->stream));
+    *out = nu * ctx->h_red[0] / (double)tot;
     return NOVA_OK;
 }
 

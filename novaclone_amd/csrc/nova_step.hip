@@ -60,58 +60,25 @@ __device__ inline double combine(const StageArgs &a, double u, double u0, double
     return u0 + a.beta * du;
 }
 
-// Neighbour table of one node: mapped indices and reflection flags for
-// offsets -H..H along each direction.
-template <int H>
-struct Nbr {
-    int64_t off[3][2 * H + 1];  // linear offset contribution of each axis
-    unsigned char flip[3][2 * H + 1];
-};
-
-template <int H>
-__device__ inline void make_nbr(const Grid &g, int i, int j, int k, Nbr<H> &nb)
-{
-    for (int o = -H; o <= H; o++) {
-        int f;
-        int ii = bmap(i + o, g.n[0], g.wall[0], f);
-        nb.off[0][o + H] = (int64_t)ii * g.s[1] * g.s[2];
-        nb.flip[0][o + H] = (unsigned char)f;
-        int jj = bmap(j + o, g.n[1], g.wall[1], f);
-        nb.off[1][o + H] = (int64_t)jj * g.s[2];
-        nb.flip[1][o + H] = (unsigned char)f;
-        int kk = bmap(k + o, g.n[2], g.wall[2], f);
-        nb.off[2][o + H] = kk;
-        nb.flip[2][o + H] = (unsigned char)f;
-    }
-}
-
-// Value of component c of a 3-component field at the node displaced by o
-// along direction d.
-template <int H>
-__device__ inline double at(const double *__restrict__ f, const Grid &g, const Nbr<H> &nb, int c, int d, int o)
-{
-    const int64_t q = (d == 0 ? nb.off[0][o + H] : nb.off[0][H]) + (d == 1 ? nb.off[1][o + H] : nb.off[1][H]) +
-                      (d == 2 ? nb.off[2][o + H] : nb.off[2][H]);
-    const double v = f[c * g.np + q];
-    return (nb.flip[d][o + H] && c == d) ? -v : v;
-}
+// ---- right-hand sides, written against an accessor A(f, c, d, o) that
+// returns component c of field f at the node displaced by o along
+// direction d (d = 0: i, 1: j, 2: k; o = 0 is the node itself) ----
 
 // div(X, Y_c): 0.5 sum_d sum_l a [ (X_d Y_c)^+ - (X_d Y_c)^- + X_d (Y_c^+ - Y_c^-)
 // + Y_c (X_d^+ - X_d^-) ]  (src/ex/div/div.hxx:101-169).
-template <int H>
-__device__ inline double div_xy(const Grid &g, const Nbr<H> &nb, const double *__restrict__ X,
-                                const double *__restrict__ Y, int c, double zi)
+template <int H, class Acc>
+__device__ inline double div_xy(const Grid &g, const Acc &A, int fx, int fy, int c, double zi)
 {
-    const double yc = at<H>(Y, g, nb, c, 0, 0);
+    const double yc = A(fy, c, 0, 0);
     double acc = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        const double xd = at<H>(X, g, nb, d, 0, 0);
+        const double xd = A(fx, d, 0, 0);
         double ad = 0.0;
 #pragma unroll
         for (int l = 0; l < H; l++) {
-            const double xp = at<H>(X, g, nb, d, d, l + 1), xm = at<H>(X, g, nb, d, d, -l - 1);
-            const double yp = at<H>(Y, g, nb, c, d, l + 1), ym = at<H>(Y, g, nb, c, d, -l - 1);
+            const double xp = A(fx, d, d, l + 1), xm = A(fx, d, d, -l - 1);
+            const double yp = A(fy, c, d, l + 1), ym = A(fy, c, d, -l - 1);
             ad += g.a[d][l] * ((yp * xp - ym * xm) + xd * (yp - ym) + yc * (xp - xm));
         }
         acc += d == 2 ? ad * zi : ad;
@@ -121,29 +88,28 @@ __device__ inline double div_xy(const Grid &g, const Nbr<H> &nb, const double *_
 
 // nu * laplacian(f_c) with the z-metric term
 // (src/ex/laplacian/laplacian.hxx:140-188).
-template <int H>
-__device__ inline double lap_c(const Grid &g, const Nbr<H> &nb, const double *__restrict__ f, int c, double nu,
-                               int k)
+template <int H, class Acc>
+__device__ inline double lap_c(const Grid &g, const Acc &A, int f, int c, double nu, double zl, double zm)
 {
-    const double f0 = at<H>(f, g, nb, c, 0, 0);
+    const double f0 = A(f, c, 0, 0);
     double acc = 0.0, dz1 = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         double ad = 0.0;
 #pragma unroll
         for (int l = 0; l < H; l++) {
-            const double fp = at<H>(f, g, nb, c, d, l + 1), fm = at<H>(f, g, nb, c, d, -l - 1);
+            const double fp = A(f, c, d, l + 1), fm = A(f, c, d, -l - 1);
             ad += g.b[d][l] * ((fp + fm) - 2.0 * f0);
             if (d == 2) dz1 += g.a[2][l] * (fp - fm);
         }
-        acc += d == 2 ? ad * g.zlap[k] : ad;
+        acc += d == 2 ? ad * zl : ad;
     }
-    return nu * acc - nu * g.zmet[k] * dz1;
+    return nu * acc - nu * zm * dz1;
 }
 
 // d/dx_d of 0.5|B|^2 (app/OT_Poisson/UEqn.H:22).
-template <int H>
-__device__ inline double grad_pm(const Grid &g, const Nbr<H> &nb, const double *__restrict__ B, int d, double zi)
+template <int H, class Acc>
+__device__ inline double grad_pm(const Grid &g, const Acc &A, int fb, int d, double zi)
 {
     double ad = 0.0;
 #pragma unroll
@@ -151,7 +117,7 @@ __device__ inline double grad_pm(const Grid &g, const Nbr<H> &nb, const double *
         double bp = 0.0, bm = 0.0;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const double vp = at<H>(B, g, nb, c, d, l + 1), vm = at<H>(B, g, nb, c, d, -l - 1);
+            const double vp = A(fb, c, d, l + 1), vm = A(fb, c, d, -l - 1);
             bp += vp * vp;
             bm += vm * vm;
         }
@@ -160,110 +126,436 @@ __device__ inline double grad_pm(const Grid &g, const Nbr<H> &nb, const double *
     return d == 2 ? ad * zi : ad;
 }
 
-// Incompressible NS stage: dU = dt*((-div(U,U)) + nu*lap(U)),
-// out = combine(U, U0, dU).
-template <int H>
-__global__ void __launch_bounds__(256) k_ns_stage(Grid g, const double *__restrict__ U,
-                                                  const double *U0, double *out, StageArgs sa,
-                                                  double nu)
+// NS right-hand side of all three components, direction by direction:
+// out[c] = -div(U, U_c) + nu lap(U_c) with the same terms as div_xy and
+// lap_c, organised so that only one direction's neighbours are live.
+template <int H, class Acc>
+__device__ inline void ns_rhs(const Grid &g, const Acc &A, double nu, double zi, double zl, double zm,
+                              double (&out)[3])
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k >= g.s[2] || j >= g.s[1]) return;
-    Nbr<H> nb;
-    make_nbr<H>(g, i, j, k, nb);
-    const double zi = g.zinv[k];
-    const int64_t q = ((int64_t)i * g.s[1] + j) * g.s[2] + k;
-    double res[3];
+    double u0[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const double du = sa.dt * ((-div_xy<H>(g, nb, U, U, c, zi)) + lap_c<H>(g, nb, U, c, nu, k));
-        const double u = U[c * g.np + q];
-        const double u0 = U0 ? U0[c * g.np + q] : u;
-        res[c] = combine(sa, u, u0, du);
+    for (int c = 0; c < 3; c++) u0[c] = A(0, c, 0, 0);
+    double adv[3] = {0.0, 0.0, 0.0}, lap[3] = {0.0, 0.0, 0.0}, dz1[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double ad[3] = {0.0, 0.0, 0.0}, ld[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int l = 0; l < H; l++) {
+            const double up = A(0, d, d, l + 1), um = A(0, d, d, -l - 1);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double cp = c == d ? up : A(0, c, d, l + 1);
+                const double cm = c == d ? um : A(0, c, d, -l - 1);
+                ad[c] += g.a[d][l] * ((cp * up - cm * um) + u0[d] * (cp - cm) + u0[c] * (up - um));
+                ld[c] += g.b[d][l] * ((cp + cm) - 2.0 * u0[c]);
+                if (d == 2) dz1[c] += g.a[2][l] * (cp - cm);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            adv[c] += d == 2 ? ad[c] * zi : ad[c];
+            lap[c] += d == 2 ? ld[c] * zl : ld[c];
+        }
+        // keep one direction's neighbours live at a time
+        __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
-    for (int c = 0; c < 3; c++) out[c * g.np + q] = res[c];
+    for (int c = 0; c < 3; c++) out[c] = (-(0.5 * adv[c])) + (nu * lap[c] - nu * zm * dz1[c]);
 }
 
-// Incompressible MHD stage (app/OT_Poisson/UEqn.H:16-23, BEqn.H:15-23):
-// dU = dt*((((-div(U,U)) + nu lap U) + div(B,B)) - grad(0.5 B.B)),
-// dB = dt*(((-div(U,B)) + div(B,U)) + eta lap B).
-template <int H>
-__global__ void __launch_bounds__(256) k_mhd_stage(Grid g, const double *__restrict__ U,
-                                                   const double *__restrict__ B, const double *U0,
-                                                   const double *B0, double *outU, double *outB,
-                                                   StageArgs sa, double nu, double eta)
+// ---- the streamed stage kernel ----
+//
+// A 64 x 4 block owns a (k, j) tile and walks i over a chunk of planes.
+// The i-neighbours of its own column live in a register queue q (planes
+// i-H .. i+H); the current plane, with its j and k halos, is staged in
+// LDS.  Every input value is loaded from global memory once per block
+// (plus the halo ring and the 2H warm-up planes of the chunk), and the
+// loads for plane i+1 are issued before plane i is computed.
+constexpr int kTK = 64, kTJ = 4;
+
+template <int NF>
+struct StreamArgs {
+    const double *X[NF];   // stage inputs (stencil)
+    const double *X0[NF];  // step-start fields for the stage combination (may be null)
+    double *out[NF];
+    StageArgs sa;
+    double nu, eta;
+    int ichunk;
+    int nkt, njt, nic;
+};
+
+// Sign of a value reached through a reflection (walls only; a periodic
+// grid has no reflections and carries no signs).
+template <bool WALLS>
+__device__ inline double rsign(int c, int d, int flip)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k >= g.s[2] || j >= g.s[1]) return;
-    Nbr<H> nb;
-    make_nbr<H>(g, i, j, k, nb);
-    const double zi = g.zinv[k];
-    const int64_t q = ((int64_t)i * g.s[1] + j) * g.s[2] + k;
-    double ru[3], rb[3];
+    return WALLS ? bsign(3, c, d, flip) : 1.0;
+}
+
+template <int H, int NF, bool WALLS>
+__global__ void __launch_bounds__(kTK *kTJ) k_stage_stream(Grid g, StreamArgs<NF> a)
+{
+    constexpr int NC = 3 * NF;
+    constexpr int LK = kTK + 2 * H, LJ = kTJ + 2 * H;
+    constexpr int NHALO = 2 * H * (kTK + kTJ);
+    constexpr int NSLOT = (NHALO + kTK * kTJ - 1) / (kTK * kTJ);
+    __shared__ double sh[NC][LJ][LK];
+
+    // tile decode; consecutive tiles share halos, so each XCD (blocks are
+    // dealt round-robin over 8) takes a contiguous run of tiles
+    const int T = a.nkt * a.njt * a.nic;
+    int t = blockIdx.x;
+    if (T % 8 == 0) t = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
+    const int kt = t % a.nkt;
+    const int jt = (t / a.nkt) % a.njt;
+    const int ic = t / (a.nkt * a.njt);
+    const int tk = threadIdx.x, tj = threadIdx.y;
+    const int tid = tj * kTK + tk;
+    const int k = kt * kTK + tk, j = jt * kTJ + tj;
+    const bool valid = k < g.s[2] && j < g.s[1];
+    const int i0 = ic * a.ichunk;
+    int i1 = i0 + a.ichunk;
+    if (i1 > g.s[0]) i1 = g.s[0];
+    const int64_t plane = (int64_t)g.s[1] * g.s[2];
+
+    // own column
+    int fj, fk;
+    const int jm = bmap(j, g.n[1], g.wall[1], fj);
+    const int km = bmap(k, g.n[2], g.wall[2], fk);
+    const int64_t col = (int64_t)jm * g.s[2] + km;
+    double sgn_col[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const double du = sa.dt * ((((-div_xy<H>(g, nb, U, U, c, zi)) + lap_c<H>(g, nb, U, c, nu, k)) +
-                                    div_xy<H>(g, nb, B, B, c, zi)) - grad_pm<H>(g, nb, B, c, zi));
-        const double db = sa.dt * (((-div_xy<H>(g, nb, U, B, c, zi)) + div_xy<H>(g, nb, B, U, c, zi)) +
-                                   lap_c<H>(g, nb, B, c, eta, k));
-        const double u = U[c * g.np + q], b = B[c * g.np + q];
-        ru[c] = combine(sa, u, U0 ? U0[c * g.np + q] : u, du);
-        rb[c] = combine(sa, b, B0 ? B0[c * g.np + q] : b, db);
+    for (int c = 0; c < 3; c++) sgn_col[c] = rsign<WALLS>(c, 1, fj) * rsign<WALLS>(c, 2, fk);
+
+    // halo slots of this thread: LDS position, source column, signs
+    int hrow[NSLOT], hcolk[NSLOT];
+    int64_t hsrc[NSLOT];
+    double hsg[NSLOT][3];
+    bool hon[NSLOT];
+#pragma unroll
+    for (int s = 0; s < NSLOT; s++) {
+        const int e = tid + s * kTK * kTJ;
+        hon[s] = e < NHALO;
+        int r = 0, cc = 0, jj = 0, kk = 0;
+        if (e < 2 * H * kTK) {  // j halo rows
+            const int rr = e / kTK;
+            cc = e % kTK + H;
+            r = rr < H ? rr : kTJ + rr;
+            jj = jt * kTJ + (rr < H ? rr - H : kTJ + rr - H);
+            kk = kt * kTK + (cc - H);
+        } else if (e < NHALO) {  // k halo columns
+            const int e2 = e - 2 * H * kTK;
+            const int rr = e2 / (2 * H), c2 = e2 % (2 * H);
+            r = rr + H;
+            cc = c2 < H ? c2 : kTK + c2;
+            jj = jt * kTJ + rr;
+            kk = kt * kTK + (c2 < H ? c2 - H : kTK + c2 - H);
+        }
+        int f1, f2;
+        const int jm2 = bmap(jj, g.n[1], g.wall[1], f1);
+        const int km2 = bmap(kk, g.n[2], g.wall[2], f2);
+        hrow[s] = r;
+        hcolk[s] = cc;
+        hsrc[s] = (int64_t)jm2 * g.s[2] + km2;
+#pragma unroll
+        for (int c = 0; c < 3; c++) hsg[s][c] = rsign<WALLS>(c, 1, f1) * rsign<WALLS>(c, 2, f2);
     }
+
+    // loaders
+    auto plane_off = [&](int ii, int &fi) -> int64_t {
+        const int im = bmap(ii, g.n[0], g.wall[0], fi);
+        return (int64_t)im * plane;
+    };
+
+    double q[NF][3][2 * H + 1];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        outU[c * g.np + q] = ru[c];
-        outB[c * g.np + q] = rb[c];
+    for (int o = 0; o < 2 * H + 1; o++) {
+        int fi;
+        const int64_t po = plane_off(i0 - H + o, fi);
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                q[f][c][o] = rsign<WALLS>(c, 0, fi) * sgn_col[c] * a.X[f][c * g.np + po + col];
+    }
+    double hv[NSLOT][NC];
+    {
+        int fi;
+        const int64_t po = plane_off(i0, fi);
+#pragma unroll
+        for (int s = 0; s < NSLOT; s++)
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    hv[s][f * 3 + c] =
+                        hon[s] ? rsign<WALLS>(c, 0, fi) * hsg[s][c] * a.X[f][c * g.np + po + hsrc[s]] : 0.0;
+    }
+
+    const double zi = g.zinv[km], zl = g.zlap[km], zm = g.zmet[km];
+
+    for (int i = i0; i < i1; i++) {
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) sh[f * 3 + c][tj + H][tk + H] = q[f][c][H];
+#pragma unroll
+        for (int s = 0; s < NSLOT; s++)
+            if (hon[s])
+#pragma unroll
+                for (int n = 0; n < NC; n++) sh[n][hrow[s]][hcolk[s]] = hv[s][n];
+        __syncthreads();
+
+        // prefetch plane i+H+1 for the queue and the halo of plane i+1
+        double nq[NF][3];
+        if (i + 1 < i1) {
+            int fi;
+            const int64_t po = plane_off(i + H + 1, fi);
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    nq[f][c] = rsign<WALLS>(c, 0, fi) * sgn_col[c] * a.X[f][c * g.np + po + col];
+            int fh;
+            const int64_t ph = plane_off(i + 1, fh);
+#pragma unroll
+            for (int s = 0; s < NSLOT; s++)
+#pragma unroll
+                for (int f = 0; f < NF; f++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++)
+                        hv[s][f * 3 + c] =
+                            hon[s] ? rsign<WALLS>(c, 0, fh) * hsg[s][c] * a.X[f][c * g.np + ph + hsrc[s]] : 0.0;
+        }
+
+        if (valid) {
+            auto A = [&](int f, int c, int d, int o) -> double {
+                if (d == 0) return q[f][c][H + o];
+                if (d == 1) return sh[f * 3 + c][tj + H + o][tk + H];
+                return sh[f * 3 + c][tj + H][tk + H + o];
+            };
+            const int64_t qi = (int64_t)i * plane + col;
+            const StageArgs &sa = a.sa;
+            if (NF == 1) {
+                double r[3];
+                ns_rhs<H>(g, A, a.nu, zi, zl, zm, r);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const double du = sa.dt * r[c];
+                    const double u = q[0][c][H];
+                    const double u0 = a.X0[0] ? a.X0[0][c * g.np + qi] : u;
+                    a.out[0][c * g.np + qi] = combine(sa, u, u0, du);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const double du =
+                        sa.dt * ((((-div_xy<H>(g, A, 0, 0, c, zi)) + lap_c<H>(g, A, 0, c, a.nu, zl, zm)) +
+                                  div_xy<H>(g, A, 1, 1, c, zi)) - grad_pm<H>(g, A, 1, c, zi));
+                    const double db = sa.dt * (((-div_xy<H>(g, A, 0, 1, c, zi)) + div_xy<H>(g, A, 1, 0, c, zi)) +
+                                               lap_c<H>(g, A, 1, c, a.eta, zl, zm));
+                    const double u = q[0][c][H], b = q[NF - 1][c][H];
+                    const double u0 = a.X0[0] ? a.X0[0][c * g.np + qi] : u;
+                    const double b0 = a.X0[NF - 1] ? a.X0[NF - 1][c * g.np + qi] : b;
+                    a.out[0][c * g.np + qi] = combine(sa, u, u0, du);
+                    a.out[NF - 1][c * g.np + qi] = combine(sa, b, b0, db);
+                }
+            }
+        }
+        __syncthreads();
+        if (i + 1 < i1) {
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+#pragma unroll
+                    for (int o = 0; o < 2 * H; o++) q[f][c][o] = q[f][c][o + 1];
+                    q[f][c][2 * H] = nq[f][c];
+                }
+        }
     }
 }
 
-template <int H>
-int launch_ns_stage_h(nova_ctx *ctx, const double *U, const double *U0, double *out, const StageArgs &sa, double nu)
+// Variant without LDS and without barriers: each wave owns one 64-wide k
+// row and streams i with the same register queue; its j and k neighbours
+// are loaded straight from global memory (the four rows of a block share
+// L1 lines, adjacent blocks share L2 lines).
+template <bool WALLS>
+__device__ inline int wrap(int q, int n, int wall, int &flip)
+{
+    if (!WALLS) {
+        flip = 0;
+        q += q < 0 ? n : 0;
+        q -= q >= n ? n : 0;
+        return q;
+    }
+    return bmap(q, n, wall, flip);
+}
+
+template <int H, int NF, bool WALLS>
+__global__ void __launch_bounds__(kTK *kTJ) k_stage_rows(Grid g, StreamArgs<NF> a)
+{
+    const int T = a.nkt * a.njt * a.nic;
+    int t = blockIdx.x;
+    if (T % 8 == 0) t = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
+    const int kt = t % a.nkt;
+    const int jt = (t / a.nkt) % a.njt;
+    const int ic = t / (a.nkt * a.njt);
+    const int k = kt * kTK + threadIdx.x, j = jt * kTJ + threadIdx.y;
+    if (k >= g.s[2] || j >= g.s[1]) return;  // no barriers below
+    const int i0 = ic * a.ichunk;
+    int i1 = i0 + a.ichunk;
+    if (i1 > g.s[0]) i1 = g.s[0];
+    const int64_t plane = (int64_t)g.s[1] * g.s[2];
+    const int s2 = g.s[2];
+    const int col = j * s2 + k;
+
+    auto plane_off = [&](int ii, int &fi) -> int64_t {
+        const int im = wrap<WALLS>(ii, g.n[0], g.wall[0], fi);
+        return (int64_t)im * plane;
+    };
+
+    double q[NF][3][2 * H + 1];
+#pragma unroll
+    for (int o = 0; o < 2 * H + 1; o++) {
+        int fi;
+        const int64_t po = plane_off(i0 - H + o, fi);
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) q[f][c][o] = rsign<WALLS>(c, 0, fi) * a.X[f][c * g.np + po + col];
+    }
+    const double zi = g.zinv[k], zl = g.zlap[k], zm = g.zmet[k];
+
+    for (int i = i0; i < i1; i++) {
+        double nq[NF][3];
+        if (i + 1 < i1) {
+            int fi;
+            const int64_t po = plane_off(i + H + 1, fi);
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) nq[f][c] = rsign<WALLS>(c, 0, fi) * a.X[f][c * g.np + po + col];
+        }
+        const int64_t pi = (int64_t)i * plane;
+        auto A = [&](int f, int c, int d, int o) -> double {
+            if (d == 0 || o == 0) return q[f][c][H + (d == 0 ? o : 0)];
+            int fl;
+            int64_t off;
+            if (d == 1) off = pi + (int64_t)wrap<WALLS>(j + o, g.n[1], g.wall[1], fl) * s2 + k;
+            else off = pi + (int64_t)j * s2 + wrap<WALLS>(k + o, g.n[2], g.wall[2], fl);
+            return rsign<WALLS>(c, d, fl) * a.X[f][c * g.np + off];
+        };
+        const int64_t qi = pi + col;
+        const StageArgs &sa = a.sa;
+        if (NF == 1) {
+            double r[3];
+            ns_rhs<H>(g, A, a.nu, zi, zl, zm, r);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double u = q[0][c][H];
+                const double u0 = a.X0[0] ? a.X0[0][c * g.np + qi] : u;
+                a.out[0][c * g.np + qi] = combine(sa, u, u0, sa.dt * r[c]);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double du = sa.dt * ((((-div_xy<H>(g, A, 0, 0, c, zi)) + lap_c<H>(g, A, 0, c, a.nu, zl, zm)) +
+                                            div_xy<H>(g, A, 1, 1, c, zi)) - grad_pm<H>(g, A, 1, c, zi));
+                const double db = sa.dt * (((-div_xy<H>(g, A, 0, 1, c, zi)) + div_xy<H>(g, A, 1, 0, c, zi)) +
+                                           lap_c<H>(g, A, 1, c, a.eta, zl, zm));
+                const double u = q[0][c][H], b = q[NF - 1][c][H];
+                const double u0 = a.X0[0] ? a.X0[0][c * g.np + qi] : u;
+                const double b0 = a.X0[NF - 1] ? a.X0[NF - 1][c * g.np + qi] : b;
+                a.out[0][c * g.np + qi] = combine(sa, u, u0, du);
+                a.out[NF - 1][c * g.np + qi] = combine(sa, b, b0, db);
+            }
+        }
+        if (i + 1 < i1) {
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+#pragma unroll
+                    for (int o = 0; o < 2 * H; o++) q[f][c][o] = q[f][c][o + 1];
+                    q[f][c][2 * H] = nq[f][c];
+                }
+        }
+    }
+}
+
+int pick_ichunk(const Grid &g, int nkt, int njt)
+{
+    // about 2048 tiles (8 blocks per CU on 256 CUs), chunks of 8..64 planes
+    int64_t per = (int64_t)g.s[0] * nkt * njt / 2048;
+    int c = (int)(per < 8 ? 8 : (per > 64 ? 64 : per));
+    if (c > g.s[0]) c = g.s[0];
+    return c;
+}
+
+template <int H, int NF>
+int launch_stream_h(nova_ctx *ctx, StreamArgs<NF> a, const char *name)
 {
     const Grid &g = ctx->g;
-    dim3 grid((g.s[2] + 63) / 64, (g.s[1] + 3) / 4, g.s[0]);
-    TimedScope ts(ctx, "ns_stage");
-    hipLaunchKernelGGL(k_ns_stage<H>, grid, dim3(64, 4, 1), 0, ctx->stream, g, U, U0, out, sa, nu);
-    return check_launch("ns_stage");
+    a.nkt = (g.s[2] + kTK - 1) / kTK;
+    a.njt = (g.s[1] + kTJ - 1) / kTJ;
+    a.ichunk = ctx->stage_ichunk > 0 ? (ctx->stage_ichunk < g.s[0] ? ctx->stage_ichunk : g.s[0])
+                                     : pick_ichunk(g, a.nkt, a.njt);
+    a.nic = (g.s[0] + a.ichunk - 1) / a.ichunk;
+    const int T = a.nkt * a.njt * a.nic;
+    TimedScope ts(ctx, name);
+    const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
+    if (ctx->stage_variant == 1) {
+        if (walls)
+            hipLaunchKernelGGL((k_stage_rows<H, NF, true>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
+        else
+            hipLaunchKernelGGL((k_stage_rows<H, NF, false>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
+    } else if (walls) {
+        hipLaunchKernelGGL((k_stage_stream<H, NF, true>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
+    } else {
+        hipLaunchKernelGGL((k_stage_stream<H, NF, false>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
+    }
+    return check_launch(name);
 }
 
-template <int H>
-int launch_mhd_stage_h(nova_ctx *ctx, const double *U, const double *B, const double *U0, const double *B0,
-                       double *oU, double *oB, const StageArgs &sa, double nu, double eta)
+template <int NF>
+int launch_stream(nova_ctx *ctx, const StreamArgs<NF> &a, const char *name)
 {
-    const Grid &g = ctx->g;
-    dim3 grid((g.s[2] + 63) / 64, (g.s[1] + 3) / 4, g.s[0]);
-    TimedScope ts(ctx, "mhd_stage");
-    hipLaunchKernelGGL(k_mhd_stage<H>, grid, dim3(64, 4, 1), 0, ctx->stream, g, U, B, U0, B0, oU, oB, sa, nu, eta);
-    return check_launch("mhd_stage");
+    switch (ctx->g.h) {
+    case 1: return launch_stream_h<1, NF>(ctx, a, name);
+    case 2: return launch_stream_h<2, NF>(ctx, a, name);
+    case 3: return launch_stream_h<3, NF>(ctx, a, name);
+    case 4: return launch_stream_h<4, NF>(ctx, a, name);
+    }
+    return fail(NOVA_ERR_ARG, "stencil order");
 }
 
 int ns_stage(nova_ctx *ctx, const double *U, const double *U0, double *out, const StageArgs &sa, double nu)
 {
-    switch (ctx->g.h) {
-    case 1: return launch_ns_stage_h<1>(ctx, U, U0, out, sa, nu);
-    case 2: return launch_ns_stage_h<2>(ctx, U, U0, out, sa, nu);
-    case 3: return launch_ns_stage_h<3>(ctx, U, U0, out, sa, nu);
-    case 4: return launch_ns_stage_h<4>(ctx, U, U0, out, sa, nu);
-    }
-    return fail(NOVA_ERR_ARG, "stencil order");
+    StreamArgs<1> a{};
+    a.X[0] = U;
+    a.X0[0] = U0;
+    a.out[0] = out;
+    a.sa = sa;
+    a.nu = nu;
+    return launch_stream<1>(ctx, a, "ns_stage");
 }
 
 int mhd_stage(nova_ctx *ctx, const double *U, const double *B, const double *U0, const double *B0, double *oU,
               double *oB, const StageArgs &sa, double nu, double eta)
 {
-    switch (ctx->g.h) {
-    case 1: return launch_mhd_stage_h<1>(ctx, U, B, U0, B0, oU, oB, sa, nu, eta);
-    case 2: return launch_mhd_stage_h<2>(ctx, U, B, U0, B0, oU, oB, sa, nu, eta);
-    case 3: return launch_mhd_stage_h<3>(ctx, U, B, U0, B0, oU, oB, sa, nu, eta);
-    case 4: return launch_mhd_stage_h<4>(ctx, U, B, U0, B0, oU, oB, sa, nu, eta);
-    }
-    return fail(NOVA_ERR_ARG, "stencil order");
+    StreamArgs<2> a{};
+    a.X[0] = U;
+    a.X[1] = B;
+    a.X0[0] = U0;
+    a.X0[1] = B0;
+    a.out[0] = oU;
+    a.out[1] = oB;
+    a.sa = sa;
+    a.nu = nu;
+    a.eta = eta;
+    return launch_stream<2>(ctx, a, "mhd_stage");
 }
 
 const double kRK4W[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};  // settings.cpp:49

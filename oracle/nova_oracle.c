@@ -887,6 +887,32 @@ double orc_ke(const orc_mesh *M, const double *U)
     return Ek / n;
 }
 
+/* nu * mean over [h, n-1-h) of sum_{c,d} (D_d U_c)^2, D = the central
+ * stencil of src/ex/grad/grad.cpp:33-58 per component (no reference row:
+ * the viscous dissipation rate). */
+double orc_dissipation(const orc_mesh *M, const double *U, double nu)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    double *g = falloc(M, 3), *s = falloc(M, 1);
+    double acc = 0.0;
+    long n = 0;
+    for (int c = 0; c < 3; c++) {
+        const size_t np = NPTS(M);
+        for (size_t q = 0; q < np; q++) s[q] = U[3 * q + c];
+        orc_grad(M, s, g);
+        for (int i = h; i < ni - h - 1; i++)
+            for (int j = h; j < nj - h - 1; j++)
+                for (int k = h; k < nk - h - 1; k++) {
+                    const double *v = &g[IX(M, i, j, k) * 3];
+                    acc += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                    if (c == 0) n++;
+                }
+    }
+    free(g);
+    free(s);
+    return nu * acc / n;
+}
+
 /* tools::maximum(ex::div(U)), src/Tools/Tools.h:19-36 over [h, n-h). */
 double orc_maxabs_div(const orc_mesh *M, const double *U)
 {

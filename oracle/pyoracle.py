@@ -60,6 +60,7 @@ def _load() -> C.CDLL:
         "orc_project": ([_P, _P, _D, _D, C.c_double], None),
         "orc_ke": ([_P, _D], C.c_double),
         "orc_maxabs_div": ([_P, _D], C.c_double),
+        "orc_dissipation": ([_P, _D, C.c_double], C.c_double),
         "orc_mhd_diag": ([_P, _D, _D, C.c_double, C.c_double, _D], None),
         "orc_tdma": ([C.c_int, _D, _D, _D, _D, _D], None),
         "orc_num_threads": ([], C.c_int),
@@ -185,6 +186,9 @@ class OMesh:
 
     def maxabs_div(self, U):
         return lib().orc_maxabs_div(self._h, _d(U))
+
+    def dissipation(self, U, nu):
+        return lib().orc_dissipation(self._h, _d(U), nu)
 
     def mhd_diag(self, U, B, mu, eta):
         out = np.zeros(8)

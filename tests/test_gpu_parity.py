@@ -216,30 +216,42 @@ def test_mhd_one_step_matches_oracle(nv, orc):
 @pytest.mark.parametrize("mode", [0, 1])
 def test_tg_histories_100_steps_match_oracle(nv, orc, mode):
     """KE and dissipation histories within 1e-10 relative over 100 steps
-    (64^3, Re = 1600, dt = 2e-3); divergence stays at roundoff on both."""
+    (64^3, Re = 1600, dt = 2e-3); divergence stays at roundoff on both.
+
+    The dissipation compared to 1e-10 is the viscous rate nu <|grad u|^2>;
+    the reference's printed column -(Ek - Ek_old)/dt (TG.cpp:170) is a
+    difference of two nearly equal energies, so it can only agree to the
+    energy agreement divided by dt, and is checked against that bound."""
     N = (64, 64, 64)
+    nu = 1.0 / 1600.0
     G, O = pair(nv, orc, N, (TWO_PI,) * 3)
     c = tg_compact(O)
     Uo = O.from_compact(c)
     po = O.field(1)
     OP = orc.OPoisson(O)
     U, p = dev(c), G.field(1)
-    S = nv.Solver(nv.Poisson(G), nv.MODEL_NS, mode, nu=1.0 / 1600.0)
+    S = nv.Solver(nv.Poisson(G), nv.MODEL_NS, mode, nu=nu)
     dt = 2e-3
-    ke_g, ke_o = [], []
+    ke_g, ke_o, eps_g, eps_o = [], [], [], []
     Ustar = O.field(3)
     for step in range(100):
-        OP.ns_step(Uo, po, 1.0 / 1600.0, dt, mode, Ustar)
+        OP.ns_step(Uo, po, nu, dt, mode, Ustar)
         S.step(dt, 1, U, p)
         ke_o.append(O.ke(Uo))
         ke_g.append(nv.ke(G, U))
+        eps_o.append(O.dissipation(Uo, nu))
+        eps_g.append(nv.dissipation(G, U, nu))
+        if step % 10 == 9:
+            dg, do = nv.maxabs_div(G, U), O.maxabs_div(Uo)
+            assert dg < 1e-12 and do < 1e-12, (step, dg, do)
         if step % 25 == 24:
-            print("step", step + 1, ke_g[-1], ke_o[-1], flush=True)
+            print("step", step + 1, ke_g[-1], ke_o[-1], eps_g[-1], eps_o[-1], flush=True)
     ke_g, ke_o = np.array(ke_g), np.array(ke_o)
+    eps_g, eps_o = np.array(eps_g), np.array(eps_o)
     assert np.max(np.abs(ke_g - ke_o) / ke_o) <= 1e-10
+    assert np.max(np.abs(eps_g - eps_o) / eps_o) <= 1e-10
     dis_g, dis_o = -np.diff(ke_g) / dt, -np.diff(ke_o) / dt
-    assert np.max(np.abs(dis_g - dis_o)) <= 1e-10 * np.max(np.abs(dis_o))
-    assert nv.maxabs_div(G, U) < 1e-10 and O.maxabs_div(Uo) < 1e-10
+    assert np.max(np.abs(dis_g - dis_o)) <= 2.0 * np.max(np.abs(ke_g - ke_o)) / dt + 1e-15
     assert rel_l2(host(U), O.interior(Uo)) <= 1e-10
 
 
