@@ -92,7 +92,8 @@ int nova_timing_enable(nova_ctx *ctx, int enable);
 int nova_timing_collect(nova_ctx *ctx, char *buf, size_t buflen);
 /* Launch-shape choice of the fused stage kernel (no reference
  * counterpart): variant 0 stages each (j,k) plane in LDS, variant 1 streams
- * rows without LDS; stage_ichunk = planes per streamed chunk (0 = auto).
+ * rows without LDS, variant 2 is variant 0 register-limited to three waves
+ * per SIMD; stage_ichunk = planes per streamed chunk (0 = auto).
  * Results are identical up to rounding; only speed differs. */
 int nova_ctx_tune(nova_ctx *ctx, int stage_variant, int stage_ichunk);
 int nova_alloc(nova_ctx *ctx, int ncomp, double **dptr);

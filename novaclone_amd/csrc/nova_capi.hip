@@ -135,6 +135,17 @@ static int ctx_init(nova_ctx *ctx, const nova_grid_desc *desc, int device)
             g.al[a][l] = c / (l + 1) / ctx->d[a];
         }
     }
+    g.wb_sum2 = 0.0;
+    for (int l = 0; l < kMaxH; l++) {
+        const double c = l < h ? ctx->coef[l] : 0.0;
+        g.wa[l] = c / 2.0 / (l + 1);
+        g.wb[l] = c / ((l + 1) * (l + 1));
+        g.wb_sum2 += 2.0 * g.wb[l];
+    }
+    for (int a = 0; a < 3; a++) {
+        g.ih[a] = 1.0 / ctx->d[a];
+        g.ih2[a] = 1.0 / (ctx->d[a] * ctx->d[a]);
+    }
     g.np = (int64_t)g.s[0] * g.s[1] * g.s[2];
     // z metric (Mesh.cpp:147-179) on the stored k range
     double hb = 1.0;
@@ -315,7 +326,7 @@ int nova_timing_collect(nova_ctx *ctx, char *buf, size_t buflen)
 
 int nova_ctx_tune(nova_ctx *ctx, int stage_variant, int stage_ichunk)
 {
-    NOVA_REQUIRE(ctx && (stage_variant == 0 || stage_variant == 1) && stage_ichunk >= 0, "bad argument");
+    NOVA_REQUIRE(ctx && stage_variant >= 0 && stage_variant <= 2 && stage_ichunk >= 0, "bad argument");
     ctx->stage_variant = stage_variant;
     ctx->stage_ichunk = stage_ichunk;
     return NOVA_OK;

@@ -27,6 +27,10 @@ struct Grid {
     double a[3][kMaxH];  // first derivative weights c_l / (2 (l+1) h_d)
     double b[3][kMaxH];  // second derivative weights c_l / ((l+1) h_d)^2
     double al[3][kMaxH]; // literal-gradient weights c_l / ((l+1) h_d)
+    double wa[kMaxH];    // c_l / (2 (l+1))
+    double wb[kMaxH];    // c_l / (l+1)^2
+    double wb_sum2;      // 2 sum_l wb[l]
+    double ih[3], ih2[3];  // 1/h_d, 1/h_d^2
     const double *zinv;  // per stored k: 1 / dzdxi(k)
     const double *zlap;  // per stored k: 1 / dzdxi(k)^2
     const double *zmet;  // per stored k: d2zdxi2(k) / dzdxi(k)^3

@@ -133,13 +133,20 @@ template <int H, class Acc>
 __device__ inline void ns_rhs(const Grid &g, const Acc &A, double nu, double zi, double zl, double zm,
                               double (&out)[3])
 {
+    // per direction d and component c, with the weights factored as in the
+    // reference (c_l / (2(l+1)) times 1/h_d):
+    //   S1 = sum_l wa_l (U_c^+ U_d^+ - U_c^- U_d^-)   conservative flux
+    //   S2 = sum_l wa_l (U_c^+ - U_c^-)               first derivative
+    //   SL = sum_l wb_l (U_c^+ + U_c^-)               second derivative
+    // adv_c = sum_d s_d (S1 + U_d S2) + U_c div(U);  -div(U,U_c) = -adv_c / 2
     double u0[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) u0[c] = A(0, c, 0, 0);
     double adv[3] = {0.0, 0.0, 0.0}, lap[3] = {0.0, 0.0, 0.0}, dz1[3] = {0.0, 0.0, 0.0};
+    double divu = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        double ad[3] = {0.0, 0.0, 0.0}, ld[3] = {0.0, 0.0, 0.0};
+        double s1[3] = {0.0, 0.0, 0.0}, s2[3] = {0.0, 0.0, 0.0}, sl[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int l = 0; l < H; l++) {
             const double up = A(0, d, d, l + 1), um = A(0, d, d, -l - 1);
@@ -147,21 +154,24 @@ __device__ inline void ns_rhs(const Grid &g, const Acc &A, double nu, double zi,
             for (int c = 0; c < 3; c++) {
                 const double cp = c == d ? up : A(0, c, d, l + 1);
                 const double cm = c == d ? um : A(0, c, d, -l - 1);
-                ad[c] += g.a[d][l] * ((cp * up - cm * um) + u0[d] * (cp - cm) + u0[c] * (up - um));
-                ld[c] += g.b[d][l] * ((cp + cm) - 2.0 * u0[c]);
-                if (d == 2) dz1[c] += g.a[2][l] * (cp - cm);
+                s1[c] += g.wa[l] * (cp * up - cm * um);
+                s2[c] += g.wa[l] * (cp - cm);
+                sl[c] += g.wb[l] * (cp + cm);
             }
         }
+        const double sa = d == 2 ? g.ih[2] * zi : g.ih[d];
+        const double sb = d == 2 ? g.ih2[2] * zl : g.ih2[d];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            adv[c] += d == 2 ? ad[c] * zi : ad[c];
-            lap[c] += d == 2 ? ld[c] * zl : ld[c];
+            adv[c] += sa * (s1[c] + u0[d] * s2[c]);
+            lap[c] += sb * (sl[c] - g.wb_sum2 * u0[c]);
+            if (d == 2) dz1[c] = g.ih[2] * s2[c];
         }
-        // keep one direction's neighbours live at a time
-        __builtin_amdgcn_sched_barrier(0);
+        divu += sa * s2[d];
     }
 #pragma unroll
-    for (int c = 0; c < 3; c++) out[c] = (-(0.5 * adv[c])) + (nu * lap[c] - nu * zm * dz1[c]);
+    for (int c = 0; c < 3; c++)
+        out[c] = (-(0.5 * (adv[c] + u0[c] * divu))) + nu * (lap[c] - zm * dz1[c]);
 }
 
 // ---- the streamed stage kernel ----
@@ -193,8 +203,8 @@ __device__ inline double rsign(int c, int d, int flip)
     return WALLS ? bsign(3, c, d, flip) : 1.0;
 }
 
-template <int H, int NF, bool WALLS>
-__global__ void __launch_bounds__(kTK *kTJ) k_stage_stream(Grid g, StreamArgs<NF> a)
+template <int H, int NF, bool WALLS, int MINW = 1>
+__global__ void __launch_bounds__(kTK *kTJ) __attribute__((amdgpu_waves_per_eu(MINW))) k_stage_stream(Grid g, StreamArgs<NF> a)
 {
     constexpr int NC = 3 * NF;
     constexpr int LK = kTK + 2 * H, LJ = kTJ + 2 * H;
@@ -511,6 +521,11 @@ int launch_stream_h(nova_ctx *ctx, StreamArgs<NF> a, const char *name)
             hipLaunchKernelGGL((k_stage_rows<H, NF, true>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
         else
             hipLaunchKernelGGL((k_stage_rows<H, NF, false>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
+    } else if (ctx->stage_variant == 2) {
+        if (walls)
+            hipLaunchKernelGGL((k_stage_stream<H, NF, true, 3>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
+        else
+            hipLaunchKernelGGL((k_stage_stream<H, NF, false, 3>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
     } else if (walls) {
         hipLaunchKernelGGL((k_stage_stream<H, NF, true>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
     } else {
