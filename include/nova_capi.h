@@ -93,9 +93,12 @@ int nova_timing_collect(nova_ctx *ctx, char *buf, size_t buflen);
 /* Launch-shape choice of the fused stage kernel (no reference
  * counterpart): variant 0 stages each (j,k) plane in LDS, variant 1 streams
  * rows without LDS, variant 2 is variant 0 register-limited to three waves
- * per SIMD; stage_ichunk = planes per streamed chunk (0 = auto).
- * Results are identical up to rounding; only speed differs. */
-int nova_ctx_tune(nova_ctx *ctx, int stage_variant, int stage_ichunk);
+ * per SIMD, variants 3 and 4 are variant 2 with 32 x 8 and 16 x 16 (k x j)
+ * tiles instead of 64 x 4; stage_ichunk = planes per streamed chunk (0 = auto);
+ * proj_variant 0 fuses the divergence into the first FFT pass, 1 computes it
+ * (and the gradient correction) with streamed kernels.  Results are
+ * identical up to rounding; only speed differs. */
+int nova_ctx_tune(nova_ctx *ctx, int stage_variant, int stage_ichunk, int proj_variant);
 int nova_alloc(nova_ctx *ctx, int ncomp, double **dptr);
 int nova_free(nova_ctx *ctx, double *dptr);
 int nova_memcpy_h2d(nova_ctx *ctx, double *dst, const double *src, size_t count);

@@ -62,7 +62,7 @@ _SIGS = {
     "nova_ctx_info": [_P, C.POINTER(C.c_int64)],
     "nova_ctx_metric": [_P, _DP, _DP],
     "nova_timing_enable": [_P, _I],
-    "nova_ctx_tune": [_P, _I, _I],
+    "nova_ctx_tune": [_P, _I, _I, _I],
     "nova_timing_collect": [_P, C.c_char_p, C.c_size_t],
     "nova_alloc": [_P, _I, C.POINTER(_P)],
     "nova_free": [_P, _P],
@@ -181,9 +181,11 @@ class Mesh:
         s = torch.cuda.current_stream(self.device).cuda_stream
         _check(lib().nova_ctx_set_stream(self._h, C.c_void_p(s)), "nova_ctx_set_stream")
 
-    def tune(self, stage_variant: int = 0, stage_ichunk: int = 0) -> None:
-        """Pick the stage-kernel variant and its i-chunk (0 = automatic)."""
-        _check(lib().nova_ctx_tune(self._h, int(stage_variant), int(stage_ichunk)), "nova_ctx_tune")
+    def tune(self, stage_variant: int = 0, stage_ichunk: int = 0, proj_variant: int = 1) -> None:
+        """Pick the stage-kernel variant, its i-chunk (0 = automatic) and the
+        projection variant."""
+        _check(lib().nova_ctx_tune(self._h, int(stage_variant), int(stage_ichunk), int(proj_variant)),
+               "nova_ctx_tune")
 
     def timing(self, enable: bool) -> None:
         """Bracket every library kernel with HIP events (per-kernel times)."""

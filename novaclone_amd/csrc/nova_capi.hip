@@ -324,11 +324,14 @@ int nova_timing_collect(nova_ctx *ctx, char *buf, size_t buflen)
     return NOVA_OK;
 }
 
-int nova_ctx_tune(nova_ctx *ctx, int stage_variant, int stage_ichunk)
+int nova_ctx_tune(nova_ctx *ctx, int stage_variant, int stage_ichunk, int proj_variant)
 {
-    NOVA_REQUIRE(ctx && stage_variant >= 0 && stage_variant <= 2 && stage_ichunk >= 0, "bad argument");
+    NOVA_REQUIRE(ctx && stage_variant >= 0 && stage_variant <= 4 && stage_ichunk >= 0 && proj_variant >= 0 &&
+                     proj_variant <= 1,
+                 "bad argument");
     ctx->stage_variant = stage_variant;
     ctx->stage_ichunk = stage_ichunk;
+    ctx->proj_variant = proj_variant;
     return NOVA_OK;
 }
 

@@ -49,6 +49,7 @@ struct nova_poisson {
     double *kpx = nullptr, *kpy = nullptr, *kpz = nullptr;
     double norm = 1.0;
     double *ptmp = nullptr;  // potential when the caller passes none
+    double *dtmp = nullptr;  // divergence scratch of the split projection
 };
 
 namespace nova {
@@ -431,6 +432,246 @@ __global__ void k_correct(Grid g, const double *__restrict__ p, double s, double
     F[2 * g.np + q] = F[2 * g.np + q] - s * gz;
 }
 
+// ---- streamed divergence and gradient correction ----
+//
+// One thread per (j,k) column walks i over a chunk of planes; the i
+// neighbours of its column stay in a register queue, the j and k neighbours
+// come from global memory (shared L1/L2 lines between adjacent rows).
+constexpr int kRK = 64, kRJ = 4;
+
+struct RowTiles {
+    int ichunk, nkt, njt, nic;
+};
+
+__device__ inline bool row_tile(const Grid &g, const RowTiles &t, int &k, int &j, int &i0, int &i1)
+{
+    const int T = t.nkt * t.njt * t.nic;
+    int b = blockIdx.x;
+    if (T % 8 == 0) b = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
+    const int kt = b % t.nkt;
+    const int jt = (b / t.nkt) % t.njt;
+    const int ic = b / (t.nkt * t.njt);
+    k = kt * kRK + threadIdx.x;
+    j = jt * kRJ + threadIdx.y;
+    i0 = ic * t.ichunk;
+    i1 = i0 + t.ichunk;
+    if (i1 > g.s[0]) i1 = g.s[0];
+    return k < g.s[2] && j < g.s[1];
+}
+
+// Loop-invariant neighbour offsets of a (j,k) column: column offsets of
+// the j+-(l+1) and k+-(l+1) neighbours and their reflection flags.
+template <int H, bool WALLS>
+struct ColNbrs {
+    int jp[H], jm[H], kp[H], km[H];
+    unsigned fl;  // bit 4l+0..3: flips of jp, jm, kp, km
+    __device__ inline void init(const Grid &g, int j, int k)
+    {
+        fl = 0;
+        const int s2 = g.s[2];
+#pragma unroll
+        for (int l = 0; l < H; l++) {
+            int f0, f1, f2, f3;
+            jp[l] = wrapq<WALLS>(j + 1 + l, g.n[1], g.wall[1], f0) * s2 + k;
+            jm[l] = wrapq<WALLS>(j - 1 - l, g.n[1], g.wall[1], f1) * s2 + k;
+            kp[l] = j * s2 + wrapq<WALLS>(k + 1 + l, g.n[2], g.wall[2], f2);
+            km[l] = j * s2 + wrapq<WALLS>(k - 1 - l, g.n[2], g.wall[2], f3);
+            fl |= (unsigned)(f0 | (f1 << 1) | (f2 << 2) | (f3 << 3)) << (4 * l);
+        }
+    }
+    __device__ inline double sg(int l, int which) const
+    {
+        return (WALLS && ((fl >> (4 * l + which)) & 1u)) ? -1.0 : 1.0;
+    }
+};
+
+// out = div(F) * scale (src/ex/div/div.hxx:37-77).  All loads of plane i+1
+// are issued before plane i is combined.
+template <int H, bool WALLS>
+__global__ void __launch_bounds__(kRK *kRJ) k_div_rows(Grid g, RowTiles t, const double *__restrict__ F,
+                                                       double scale, double *__restrict__ out)
+{
+    int k, j, i0, i1;
+    if (!row_tile(g, t, k, j, i0, i1)) return;
+    const int64_t plane = (int64_t)g.s[1] * g.s[2];
+    const int col = j * g.s[2] + k;
+    ColNbrs<H, WALLS> nb;
+    nb.init(g, j, k);
+    const double *Fx = F, *Fy = F + g.np, *Fz = F + 2 * g.np;
+    double q[2 * H + 1];
+#pragma unroll
+    for (int o = 0; o < 2 * H + 1; o++) {
+        int fi;
+        const int im = wrapq<WALLS>(i0 - H + o, g.n[0], g.wall[0], fi);
+        q[o] = (WALLS && fi ? -1.0 : 1.0) * Fx[(int64_t)im * plane + col];
+    }
+    double vy[2 * H], wz[2 * H];
+    auto load_plane = [&](int i, double (&y)[2 * H], double (&z)[2 * H]) {
+        const int64_t pi = (int64_t)i * plane;
+#pragma unroll
+        for (int l = 0; l < H; l++) {
+            y[l] = nb.sg(l, 0) * Fy[pi + nb.jp[l]];
+            y[H + l] = nb.sg(l, 1) * Fy[pi + nb.jm[l]];
+            z[l] = (g.wall[2] ? nb.sg(l, 2) : 1.0) * Fz[pi + nb.kp[l]];
+            z[H + l] = (g.wall[2] ? nb.sg(l, 3) : 1.0) * Fz[pi + nb.km[l]];
+        }
+    };
+    load_plane(i0, vy, wz);
+    const double zi = g.zinv[k];
+    for (int i = i0; i < i1; i++) {
+        double nq = 0.0, ny[2 * H], nz[2 * H];
+        if (i + 1 < i1) {
+            int fi;
+            const int im = wrapq<WALLS>(i + H + 1, g.n[0], g.wall[0], fi);
+            nq = (WALLS && fi ? -1.0 : 1.0) * Fx[(int64_t)im * plane + col];
+            load_plane(i + 1, ny, nz);
+        }
+        double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+        for (int l = 0; l < H; l++) {
+            ax += g.a[0][l] * (q[H + 1 + l] - q[H - 1 - l]);
+            ay += g.a[1][l] * (vy[l] - vy[H + l]);
+            az += g.a[2][l] * (wz[l] - wz[H + l]);
+        }
+        out[(int64_t)i * plane + col] = (ax + ay + az * zi) * scale;
+        if (i + 1 < i1) {
+#pragma unroll
+            for (int o = 0; o < 2 * H; o++) q[o] = q[o + 1];
+            q[2 * H] = nq;
+#pragma unroll
+            for (int o = 0; o < 2 * H; o++) {
+                vy[o] = ny[o];
+                wz[o] = nz[o];
+            }
+        }
+    }
+}
+
+// F <- F - s * grad(p) (app/TG_Piosson/RKLoop.H:36, app/OT_Poisson/BEqn.H:40).
+template <int H, bool WALLS>
+__global__ void __launch_bounds__(kRK *kRJ) k_correct_rows(Grid g, RowTiles t, const double *__restrict__ p,
+                                                           double s, double *__restrict__ F)
+{
+    int k, j, i0, i1;
+    if (!row_tile(g, t, k, j, i0, i1)) return;
+    const int64_t plane = (int64_t)g.s[1] * g.s[2];
+    const int col = j * g.s[2] + k;
+    ColNbrs<H, WALLS> nb;
+    nb.init(g, j, k);
+    double q[2 * H + 1];
+#pragma unroll
+    for (int o = 0; o < 2 * H + 1; o++) {
+        int fi;
+        const int im = wrapq<WALLS>(i0 - H + o, g.n[0], g.wall[0], fi);
+        q[o] = p[(int64_t)im * plane + col];
+    }
+    double py[2 * H], pz[2 * H], f[3];
+    auto load_plane = [&](int i, double (&y)[2 * H], double (&z)[2 * H], double (&ff)[3]) {
+        const int64_t pi = (int64_t)i * plane;
+#pragma unroll
+        for (int l = 0; l < H; l++) {
+            y[l] = p[pi + nb.jp[l]];
+            y[H + l] = p[pi + nb.jm[l]];
+            z[l] = p[pi + nb.kp[l]];
+            z[H + l] = p[pi + nb.km[l]];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) ff[c] = F[c * g.np + pi + col];
+    };
+    load_plane(i0, py, pz, f);
+    const double zi = g.zinv[k];
+    for (int i = i0; i < i1; i++) {
+        double nq = 0.0, ny[2 * H], nz[2 * H], nf[3];
+        if (i + 1 < i1) {
+            int fi;
+            const int im = wrapq<WALLS>(i + H + 1, g.n[0], g.wall[0], fi);
+            nq = p[(int64_t)im * plane + col];
+            load_plane(i + 1, ny, nz, nf);
+        }
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+        for (int l = 0; l < H; l++) {
+            gx += g.a[0][l] * (q[H + 1 + l] - q[H - 1 - l]);
+            gy += g.a[1][l] * (py[l] - py[H + l]);
+            gz += g.a[2][l] * (pz[l] - pz[H + l]);
+        }
+        gz *= zi;
+        const int64_t q0 = (int64_t)i * plane + col;
+        F[q0] = f[0] - s * gx;
+        F[g.np + q0] = f[1] - s * gy;
+        F[2 * g.np + q0] = f[2] - s * gz;
+        if (i + 1 < i1) {
+#pragma unroll
+            for (int o = 0; o < 2 * H; o++) q[o] = q[o + 1];
+            q[2 * H] = nq;
+#pragma unroll
+            for (int o = 0; o < 2 * H; o++) {
+                py[o] = ny[o];
+                pz[o] = nz[o];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) f[c] = nf[c];
+        }
+    }
+}
+
+RowTiles row_tiles(const Grid &g, int ichunk)
+{
+    RowTiles t;
+    t.nkt = (g.s[2] + kRK - 1) / kRK;
+    t.njt = (g.s[1] + kRJ - 1) / kRJ;
+    if (ichunk <= 0) {
+        // light kernels: about 8192 blocks, chunks of 8..32 planes
+        int64_t per = (int64_t)g.s[0] * t.nkt * t.njt / 8192;
+        ichunk = (int)(per < 8 ? 8 : (per > 32 ? 32 : per));
+    }
+    t.ichunk = ichunk > g.s[0] ? g.s[0] : ichunk;
+    t.nic = (g.s[0] + t.ichunk - 1) / t.ichunk;
+    return t;
+}
+
+template <int H>
+void launch_div_rows_h(hipStream_t st, const Grid &g, const RowTiles &t, const double *F, double scale, double *out)
+{
+    const unsigned T = (unsigned)(t.nkt * t.njt * t.nic);
+    const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
+    if (walls) hipLaunchKernelGGL((k_div_rows<H, true>), dim3(T), dim3(kRK, kRJ), 0, st, g, t, F, scale, out);
+    else hipLaunchKernelGGL((k_div_rows<H, false>), dim3(T), dim3(kRK, kRJ), 0, st, g, t, F, scale, out);
+}
+
+template <int H>
+void launch_correct_rows_h(hipStream_t st, const Grid &g, const RowTiles &t, const double *p, double s, double *F)
+{
+    const unsigned T = (unsigned)(t.nkt * t.njt * t.nic);
+    const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
+    if (walls) hipLaunchKernelGGL((k_correct_rows<H, true>), dim3(T), dim3(kRK, kRJ), 0, st, g, t, p, s, F);
+    else hipLaunchKernelGGL((k_correct_rows<H, false>), dim3(T), dim3(kRK, kRJ), 0, st, g, t, p, s, F);
+}
+
+int launch_div_rows(nova_ctx *ctx, const double *F, double scale, double *out)
+{
+    const RowTiles t = row_tiles(ctx->g, 0);
+    switch (ctx->g.h) {
+    case 1: launch_div_rows_h<1>(ctx->stream, ctx->g, t, F, scale, out); break;
+    case 2: launch_div_rows_h<2>(ctx->stream, ctx->g, t, F, scale, out); break;
+    case 3: launch_div_rows_h<3>(ctx->stream, ctx->g, t, F, scale, out); break;
+    default: launch_div_rows_h<4>(ctx->stream, ctx->g, t, F, scale, out); break;
+    }
+    return check_launch("div_rows");
+}
+
+int launch_correct_rows(nova_ctx *ctx, const double *p, double s, double *F)
+{
+    const RowTiles t = row_tiles(ctx->g, 0);
+    switch (ctx->g.h) {
+    case 1: launch_correct_rows_h<1>(ctx->stream, ctx->g, t, p, s, F); break;
+    case 2: launch_correct_rows_h<2>(ctx->stream, ctx->g, t, p, s, F); break;
+    case 3: launch_correct_rows_h<3>(ctx->stream, ctx->g, t, p, s, F); break;
+    default: launch_correct_rows_h<4>(ctx->stream, ctx->g, t, p, s, F); break;
+    }
+    return check_launch("correct_rows");
+}
+
 // Launchers over the supported transform lengths.
 #define NOVA_FFT_LENGTHS(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768)
 
@@ -611,6 +852,7 @@ int poisson_create(nova_ctx *ctx, nova_poisson **out)
     }
     const size_t nh = (size_t)ps->Kc * ps->Nx * ps->sy;
     NOVA_HIP(hipMalloc((void **)&ps->H, sizeof(double2) * nh));
+    NOVA_HIP(hipMalloc((void **)&ps->dtmp, sizeof(double) * g.np));
     // modified wavenumbers per index, with the reference's wrap
     const double pi = 3.14159265358979323846;
     std::vector<double> kx(ps->Nx), ky(ps->sy), kz(ps->Kc);
@@ -643,6 +885,7 @@ int poisson_destroy(nova_poisson *ps)
     if (ps->H) (void)hipFree(ps->H);
     if (ps->kpx) (void)hipFree(ps->kpx);
     if (ps->ptmp) (void)hipFree(ps->ptmp);
+    if (ps->dtmp) (void)hipFree(ps->dtmp);
     delete ps;
     return NOVA_OK;
 }
@@ -659,6 +902,17 @@ int poisson_run(nova_poisson *ps, const double *src, int src_is_vector, double s
     const dim3 zgrid((ps->sy + zl - 1) / zl, ps->Nx);
     const int64_t ylines_total = (int64_t)ps->Kc * ps->Nx;
     int rc;
+    if (src_is_vector && ctx->proj_variant == 1) {
+        // divergence by a streamed kernel into a scalar, then a plain r2c
+        {
+            TimedScope ts(ctx, "div_rows");
+            rc = launch_div_rows(ctx, src, scale, ps->dtmp);
+        }
+        if (rc) return rc;
+        src = ps->dtmp;
+        src_is_vector = 0;
+        scale = 1.0;
+    }
     {
         TimedScope ts(ctx, src_is_vector ? "fft_div_r2c_z" : "fft_r2c_z");
         rc = launch_fwd_z(st, ps->Nz, zgrid, g, ps->fz.tw, ps->fz.perm, src, src_is_vector, scale, ps->H, ps->Nx,
@@ -704,6 +958,7 @@ int project(nova_poisson *ps, double *F, double s, double *p)
     const Grid &g = ctx->g;
     dim3 grid((g.s[2] + 63) / 64, (g.s[1] + 3) / 4, g.s[0]);
     TimedScope ts(ctx, "grad_correct");
+    if (ctx->proj_variant == 1) return launch_correct_rows(ctx, p, s, F);
     if (g.wall[0] || g.wall[1] || g.wall[2])
         hipLaunchKernelGGL(k_correct<true>, grid, dim3(64, 4, 1), 0, ctx->stream, g, p, s, F);
     else

@@ -82,8 +82,9 @@ struct TimedLaunch {
 struct nova_ctx {
     int device = 0;
     int timing = 0;                        // record events around launches
-    int stage_variant = 0;                 // 0: LDS-staged planes, 1: row streams without LDS
+    int stage_variant = 2;                 // 0: LDS-staged planes, 1: row streams without LDS, 2: 0 at 3 waves/SIMD
     int stage_ichunk = 0;                  // planes per streamed chunk (0: automatic)
+    int proj_variant = 1;                  // 0: divergence fused into the r2c pass, 1: streamed kernels
     std::vector<nova::TimedLaunch> tlog;   // launches since the last collect
     std::vector<hipEvent_t> evpool;        // reusable events
     hipStream_t stream = nullptr;

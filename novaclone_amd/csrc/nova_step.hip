@@ -203,9 +203,10 @@ __device__ inline double rsign(int c, int d, int flip)
     return WALLS ? bsign(3, c, d, flip) : 1.0;
 }
 
-template <int H, int NF, bool WALLS, int MINW = 1>
-__global__ void __launch_bounds__(kTK *kTJ) __attribute__((amdgpu_waves_per_eu(MINW))) k_stage_stream(Grid g, StreamArgs<NF> a)
+template <int H, int NF, bool WALLS, int MINW, int kTK>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MINW))) k_stage_stream(Grid g, StreamArgs<NF> a)
 {
+    constexpr int kTJ = 256 / kTK;
     constexpr int NC = 3 * NF;
     constexpr int LK = kTK + 2 * H, LJ = kTJ + 2 * H;
     constexpr int NHALO = 2 * H * (kTK + kTJ);
@@ -304,6 +305,13 @@ __global__ void __launch_bounds__(kTK *kTJ) __attribute__((amdgpu_waves_per_eu(M
     }
 
     const double zi = g.zinv[km], zl = g.zlap[km], zm = g.zmet[km];
+    // step-start values of the own column, one plane ahead like the queue
+    double x0c[NF][3];
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            x0c[f][c] = (a.X0[f] && valid) ? a.X0[f][c * g.np + (int64_t)i0 * plane + col] : 0.0;
 
     for (int i = i0; i < i1; i++) {
 #pragma unroll
@@ -317,9 +325,15 @@ __global__ void __launch_bounds__(kTK *kTJ) __attribute__((amdgpu_waves_per_eu(M
                 for (int n = 0; n < NC; n++) sh[n][hrow[s]][hcolk[s]] = hv[s][n];
         __syncthreads();
 
-        // prefetch plane i+H+1 for the queue and the halo of plane i+1
-        double nq[NF][3];
+        // prefetch plane i+H+1 for the queue, the halo of plane i+1 and the
+        // step-start values of plane i+1
+        double nq[NF][3], nx0[NF][3];
         if (i + 1 < i1) {
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    nx0[f][c] = (a.X0[f] && valid) ? a.X0[f][c * g.np + (int64_t)(i + 1) * plane + col] : 0.0;
             int fi;
             const int64_t po = plane_off(i + H + 1, fi);
 #pragma unroll
@@ -354,7 +368,7 @@ __global__ void __launch_bounds__(kTK *kTJ) __attribute__((amdgpu_waves_per_eu(M
                 for (int c = 0; c < 3; c++) {
                     const double du = sa.dt * r[c];
                     const double u = q[0][c][H];
-                    const double u0 = a.X0[0] ? a.X0[0][c * g.np + qi] : u;
+                    const double u0 = a.X0[0] ? x0c[0][c] : u;
                     a.out[0][c * g.np + qi] = combine(sa, u, u0, du);
                 }
             } else {
@@ -366,8 +380,8 @@ __global__ void __launch_bounds__(kTK *kTJ) __attribute__((amdgpu_waves_per_eu(M
                     const double db = sa.dt * (((-div_xy<H>(g, A, 0, 1, c, zi)) + div_xy<H>(g, A, 1, 0, c, zi)) +
                                                lap_c<H>(g, A, 1, c, a.eta, zl, zm));
                     const double u = q[0][c][H], b = q[NF - 1][c][H];
-                    const double u0 = a.X0[0] ? a.X0[0][c * g.np + qi] : u;
-                    const double b0 = a.X0[NF - 1] ? a.X0[NF - 1][c * g.np + qi] : b;
+                    const double u0 = a.X0[0] ? x0c[0][c] : u;
+                    const double b0 = a.X0[NF - 1] ? x0c[NF - 1][c] : b;
                     a.out[0][c * g.np + qi] = combine(sa, u, u0, du);
                     a.out[NF - 1][c * g.np + qi] = combine(sa, b, b0, db);
                 }
@@ -382,6 +396,7 @@ __global__ void __launch_bounds__(kTK *kTJ) __attribute__((amdgpu_waves_per_eu(M
 #pragma unroll
                     for (int o = 0; o < 2 * H; o++) q[f][c][o] = q[f][c][o + 1];
                     q[f][c][2 * H] = nq[f][c];
+                    x0c[f][c] = nx0[f][c];
                 }
         }
     }
@@ -504,32 +519,52 @@ int pick_ichunk(const Grid &g, int nkt, int njt)
     return c;
 }
 
-template <int H, int NF>
-int launch_stream_h(nova_ctx *ctx, StreamArgs<NF> a, const char *name)
+template <int H, int NF, bool WALLS, int MINW, int TK>
+void launch_stream_tile(nova_ctx *ctx, StreamArgs<NF> a)
 {
     const Grid &g = ctx->g;
-    a.nkt = (g.s[2] + kTK - 1) / kTK;
-    a.njt = (g.s[1] + kTJ - 1) / kTJ;
+    constexpr int TJ = 256 / TK;
+    a.nkt = (g.s[2] + TK - 1) / TK;
+    a.njt = (g.s[1] + TJ - 1) / TJ;
     a.ichunk = ctx->stage_ichunk > 0 ? (ctx->stage_ichunk < g.s[0] ? ctx->stage_ichunk : g.s[0])
                                      : pick_ichunk(g, a.nkt, a.njt);
     a.nic = (g.s[0] + a.ichunk - 1) / a.ichunk;
     const int T = a.nkt * a.njt * a.nic;
-    TimedScope ts(ctx, name);
+    hipLaunchKernelGGL((k_stage_stream<H, NF, WALLS, MINW, TK>), dim3(T), dim3(TK, TJ, 1), 0, ctx->stream, g, a);
+}
+
+template <int H, int NF, bool WALLS>
+void launch_stream_w(nova_ctx *ctx, const StreamArgs<NF> &a)
+{
+    switch (ctx->stage_variant) {
+    case 0: launch_stream_tile<H, NF, WALLS, 1, 64>(ctx, a); break;
+    case 3: launch_stream_tile<H, NF, WALLS, 3, 32>(ctx, a); break;
+    case 4: launch_stream_tile<H, NF, WALLS, 3, 16>(ctx, a); break;
+    default: launch_stream_tile<H, NF, WALLS, 3, 64>(ctx, a); break;
+    }
+}
+
+template <int H, int NF>
+int launch_stream_h(nova_ctx *ctx, StreamArgs<NF> a, const char *name)
+{
+    const Grid &g = ctx->g;
     const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
+    TimedScope ts(ctx, name);
     if (ctx->stage_variant == 1) {
+        a.nkt = (g.s[2] + kTK - 1) / kTK;
+        a.njt = (g.s[1] + kTJ - 1) / kTJ;
+        a.ichunk = ctx->stage_ichunk > 0 ? (ctx->stage_ichunk < g.s[0] ? ctx->stage_ichunk : g.s[0])
+                                         : pick_ichunk(g, a.nkt, a.njt);
+        a.nic = (g.s[0] + a.ichunk - 1) / a.ichunk;
+        const int T = a.nkt * a.njt * a.nic;
         if (walls)
             hipLaunchKernelGGL((k_stage_rows<H, NF, true>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
         else
             hipLaunchKernelGGL((k_stage_rows<H, NF, false>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
-    } else if (ctx->stage_variant == 2) {
-        if (walls)
-            hipLaunchKernelGGL((k_stage_stream<H, NF, true, 3>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
-        else
-            hipLaunchKernelGGL((k_stage_stream<H, NF, false, 3>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
     } else if (walls) {
-        hipLaunchKernelGGL((k_stage_stream<H, NF, true>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
+        launch_stream_w<H, NF, true>(ctx, a);
     } else {
-        hipLaunchKernelGGL((k_stage_stream<H, NF, false>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
+        launch_stream_w<H, NF, false>(ctx, a);
     }
     return check_launch(name);
 }
