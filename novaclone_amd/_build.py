@@ -50,9 +50,27 @@ def build(verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_apps()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+APPS = ["tg_poisson"]
+
+
+def build_apps() -> None:
+    """Drivers written against the C++ drop-in API (novaclone_amd/host/nova),
+    linked to the library with an rpath relative to the binary."""
+    out = PKG / "host" / "bin"
+    out.mkdir(exist_ok=True)
+    for app in APPS:
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", f"-I{PKG / 'host'}",
+               str(PKG / "host" / "apps" / f"{app}.cpp"), f"-L{PKG}", "-lnova_hip",
+               "-Wl,-rpath,$ORIGIN/../..", "-o", str(out / app)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {app}:\n{r.stdout}\n{r.stderr}")
 
 
 if __name__ == "__main__":

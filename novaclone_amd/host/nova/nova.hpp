@@ -1,0 +1,507 @@
+// nova.hpp -- C++ drop-in mirror of the reference's host API over the C ABI.
+//
+// The reference apps are written against Field<T>, Mesh, Time, settings,
+// the ex:: operators, poisson and tools (src/Field/Field.h:56-367,
+// src/Mesh/Mesh.h:36-178, src/Time/Time.h:29-93,
+// src/settings/settings.h:28-87, src/ex/*, src/poisson/poisson.h:36-115,
+// src/Tools/Tools.h:8-37).  This header keeps those names, argument
+// meanings and index conventions (padded node indices, k fastest, ghosts of
+// width m/2 plus the duplicated periodic endpoint), while the data lives on
+// the GPU in the compact layout and every operator is a gfx950 kernel
+// reached through include/nova_capi.h.  A host mirror in the reference's
+// padded array-of-structures layout backs operator()(i,j,k); it is filled
+// from the device (ghosts included) on first host access after a device
+// write, and uploaded before the next device operation after host writes.
+//
+// Errors from the library throw nova::Error (the reference asserts).
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "nova_capi.h"
+
+typedef double scalar;
+
+class vector {
+    scalar v_[3];
+
+public:
+    vector() : v_{0.0, 0.0, 0.0} {}
+    vector(scalar a) : v_{a, a, a} {}
+    vector(scalar a, scalar b, scalar c) : v_{a, b, c} {}
+    scalar x() const { return v_[0]; }
+    scalar y() const { return v_[1]; }
+    scalar z() const { return v_[2]; }
+    scalar &x() { return v_[0]; }
+    scalar &y() { return v_[1]; }
+    scalar &z() { return v_[2]; }
+    scalar operator[](int i) const { return v_[i]; }
+    scalar &operator[](int i) { return v_[i]; }
+};
+
+template <class T> struct pTraits;
+template <> struct pTraits<scalar> { static constexpr int nComp = 1; };
+template <> struct pTraits<vector> { static constexpr int nComp = 3; };
+
+namespace nova {
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void check(int rc, const char *what)
+{
+    if (rc != NOVA_OK) throw Error(std::string(what) + ": " + nova_last_error());
+}
+
+}  // namespace nova
+
+// ---- settings (src/settings/settings.cpp:21-77) ----
+class settings {
+public:
+    static int &m()
+    {
+        static int m_ = 6;
+        return m_;
+    }
+    static std::string &zoneName()
+    {
+        static std::string z = ".";
+        return z;
+    }
+    static bool &restart()
+    {
+        static bool r = false;
+        return r;
+    }
+    // central-difference weights of orders 2, 4, 6, 8
+    static const std::array<scalar, 4> &coef()
+    {
+        static const std::array<std::array<scalar, 4>, 4> c = {
+            std::array<scalar, 4>{{1.0, 0.0, 0.0, 0.0}}, std::array<scalar, 4>{{4.0 / 3.0, -1.0 / 3.0, 0.0, 0.0}},
+            std::array<scalar, 4>{{6.0 / 4.0, -12.0 / 20.0, 1.0 / 10.0, 0.0}},
+            std::array<scalar, 4>{{8.0 / 5.0, -4.0 / 5.0, 24.0 / 105.0, -8.0 / 280.0}}};
+        return c[m() / 2 - 1];
+    }
+    // RK4 stage table and stage times
+    static const std::array<std::array<scalar, 4>, 4> &RK4a()
+    {
+        static const std::array<std::array<scalar, 4>, 4> a = {
+            std::array<scalar, 4>{{0.5, 0, 0, 0}}, std::array<scalar, 4>{{0, 0.5, 0, 0}},
+            std::array<scalar, 4>{{0, 0, 1.0, 0}},
+            std::array<scalar, 4>{{1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0}}};
+        return a;
+    }
+    static const std::array<scalar, 4> &RK4c()
+    {
+        static const std::array<scalar, 4> c = {0.5, 0.5, 1.0, 1.0};
+        return c;
+    }
+    // command line: -restart, -zone <dir>, -m <even order>
+    static void process(int argc, char *argv[])
+    {
+        for (int i = 1; i < argc; i++) {
+            const std::string a = argv[i];
+            if (a == "-restart") restart() = true;
+            if (a == "-zone" && i + 1 < argc) zoneName() = argv[i + 1];
+            if (a == "-m" && i + 1 < argc) {
+                m() = std::atoi(argv[i + 1]);
+                if (m() % 2 || m() < 2 || m() > 8) throw nova::Error("-m must be 2, 4, 6 or 8");
+            }
+        }
+    }
+};
+
+// ---- Time (src/Time/Time.cpp:22-140) ----
+class Time {
+    scalar dt_, curTime_ = 0.0, endTime_;
+    int writeEvery_, timeStep_ = 0, fileNum_ = 0;
+    bool write_ = false, log_ = true;
+
+public:
+    Time(scalar dt, scalar endTime, int writeEvery) : dt_(dt), endTime_(endTime), writeEvery_(writeEvery) {}
+    scalar &dt() { return dt_; }
+    bool run() const { return curTime_ < endTime_; }
+    bool write() const { return write_; }
+    bool writelog() const { return log_; }
+    scalar curTime() const { return curTime_; }
+    int timeStep() const { return timeStep_; }
+    int fileNum() const { return fileNum_; }
+    void operator++(int)
+    {
+        curTime_ += dt_;
+        timeStep_++;
+        write_ = writeEvery_ > 0 && timeStep_ % writeEvery_ == 0;
+        if (write_) fileNum_++;
+        log_ = (timeStep_ % 100 == 0);
+    }
+};
+
+// single-rank reductions (src/parallelCom/parallelCom.h:244-282)
+template <class T> struct plusOp { T operator()(T a, T b) const { return a + b; } };
+template <class T> struct maxOp { T operator()(T a, T b) const { return a > b ? a : b; } };
+template <class T> struct minOp { T operator()(T a, T b) const { return a < b ? a : b; } };
+template <class T, class Op> inline void reduce(T &, Op) {}
+template <class T, class Op> inline void all_reduce(T &, Op) {}
+
+// ---- Mesh (src/Mesh/Mesh.cpp:24-221): grid + device context ----
+class Mesh {
+    struct Ctx {
+        nova_ctx *ctx = nullptr;
+        nova_poisson *ps = nullptr;
+        std::map<int, std::vector<double *>> pool;  // free device buffers by component count
+        ~Ctx()
+        {
+            for (auto &kv : pool)
+                for (double *p : kv.second) nova_free(ctx, p);
+            if (ps) nova_poisson_destroy(ps);
+            if (ctx) nova_ctx_destroy(ctx);
+        }
+    };
+    std::shared_ptr<Ctx> c_;
+    nova_grid_desc d_{};
+    int n_[3] = {0, 0, 0};  // padded extents
+    int64_t np_ = 0;
+    Time *time_ = nullptr;
+
+    void open(int device)
+    {
+        c_ = std::make_shared<Ctx>();
+        nova::check(nova_ctx_create(&d_, device, &c_->ctx), "nova_ctx_create");
+        int64_t info[8];
+        nova::check(nova_ctx_info(c_->ctx, info), "nova_ctx_info");
+        np_ = info[3];
+        n_[0] = (int)info[4];
+        n_[1] = (int)info[5];
+        n_[2] = (int)info[6];
+    }
+
+public:
+    // mesh file keys "origin:", "N:", "L:", "hypz:" (Mesh.cpp:38-62); a
+    // non-periodic north/south entry in <zone>/UBC makes y walled.
+    Mesh(const std::string &fileName, Time &runTime, int device = 0) : time_(&runTime)
+    {
+        std::ifstream fin(fileName);
+        if (!fin) throw nova::Error("cannot open mesh file " + fileName);
+        std::string line;
+        while (std::getline(fin, line)) {
+            std::stringstream sp(line);
+            std::string key;
+            sp >> key;
+            if (key == "origin:") sp >> d_.origin[0] >> d_.origin[1] >> d_.origin[2];
+            else if (key == "N:") sp >> d_.n[0] >> d_.n[1] >> d_.n[2];
+            else if (key == "L:") sp >> d_.L[0] >> d_.L[1] >> d_.L[2];
+            else if (key == "hypz:") { d_.hyp = 1; sp >> d_.s1 >> d_.s2; }
+        }
+        d_.m = settings::m();
+        const std::string zone = fileName.substr(0, fileName.find_last_of('/') == std::string::npos
+                                                        ? 0 : fileName.find_last_of('/'));
+        std::ifstream ubc((zone.empty() ? std::string(".") : zone) + "/UBC");
+        std::string dir, type;
+        while (ubc >> dir >> type) {
+            if ((dir == "north" || dir == "south") && type != "periodic") d_.bc[1] = NOVA_BC_WALL;
+            std::getline(ubc, line);
+        }
+        open(device);
+    }
+    Mesh(const std::array<int, 3> &N, const std::array<scalar, 3> &L, Time &runTime,
+         const std::array<scalar, 3> &origin = {0.0, 0.0, 0.0}, bool wallY = false, int device = 0)
+        : time_(&runTime)
+    {
+        for (int a = 0; a < 3; a++) {
+            d_.n[a] = N[a];
+            d_.L[a] = L[a];
+            d_.origin[a] = origin[a];
+        }
+        d_.m = settings::m();
+        d_.bc[1] = wallY ? NOVA_BC_WALL : NOVA_BC_PERIODIC;
+        open(device);
+    }
+    int ni() const { return n_[0]; }
+    int nj() const { return n_[1]; }
+    int nk() const { return n_[2]; }
+    scalar dx() const { return d_.L[0] / d_.n[0]; }
+    scalar dy() const { return d_.L[1] / d_.n[1]; }
+    scalar dz() const { return d_.L[2] / d_.n[2]; }
+    scalar lx() const { return d_.L[0]; }
+    scalar ly() const { return d_.L[1]; }
+    scalar lz() const { return d_.L[2]; }
+    vector origin() const { return vector(d_.origin[0], d_.origin[1], d_.origin[2]); }
+    Time &time() const { return *time_; }
+    nova_ctx *ctx() const { return c_->ctx; }
+    int64_t points() const { return np_; }
+    nova_poisson *poissonHandle() const
+    {
+        if (!c_->ps) nova::check(nova_poisson_create(c_->ctx, &c_->ps), "nova_poisson_create");
+        return c_->ps;
+    }
+    // pool-owned device buffers (the reference's reuseTmp, reuseTmp.hxx:19-50)
+    double *take(int nc) const
+    {
+        auto &v = c_->pool[nc];
+        if (!v.empty()) {
+            double *p = v.back();
+            v.pop_back();
+            return p;
+        }
+        double *p = nullptr;
+        nova::check(nova_alloc(c_->ctx, nc, &p), "nova_alloc");
+        return p;
+    }
+    void give(int nc, double *p) const { c_->pool[nc].push_back(p); }
+    std::shared_ptr<Ctx> keep() const { return c_; }
+};
+
+template <class T>
+class Field {
+    static constexpr int NC = pTraits<T>::nComp;
+    Mesh m_;
+    std::string name_;
+    double *d_ = nullptr;                 // device, compact SoA
+    mutable std::vector<T> h_;            // host mirror, padded AoS
+    mutable bool hostValid_ = false;      // mirror equals device
+    mutable bool hostDirty_ = false;      // mirror newer than device
+
+    void syncHost() const
+    {
+        if (hostValid_) return;
+        h_.resize((size_t)m_.ni() * m_.nj() * m_.nk());
+        const size_t np = h_.size() * NC;
+        std::vector<double> tmp(np);
+        double *dpad = padBuffer();
+        nova::check(nova_pack_padded(m_.ctx(), d_, NC, dpad), "nova_pack_padded");
+        nova::check(nova_memcpy_d2h(m_.ctx(), tmp.data(), dpad, np), "nova_memcpy_d2h");
+        releasePad(dpad);
+        std::memcpy((void *)h_.data(), tmp.data(), np * sizeof(double));
+        hostValid_ = true;
+    }
+    double *padBuffer() const
+    {
+        // padded extent in units of the compact field: round up
+        const int64_t padded = (int64_t)m_.ni() * m_.nj() * m_.nk() * NC;
+        const int nc = (int)((padded + m_.points() - 1) / m_.points());
+        return m_.take(nc);
+    }
+    void releasePad(double *p) const
+    {
+        const int64_t padded = (int64_t)m_.ni() * m_.nj() * m_.nk() * NC;
+        m_.give((int)((padded + m_.points() - 1) / m_.points()), p);
+    }
+
+public:
+    // device-side synchronisation (host writes -> device)
+    void syncDevice() const
+    {
+        if (!hostDirty_) return;
+        const size_t np = h_.size() * NC;
+        double *dpad = padBuffer();
+        nova::check(nova_memcpy_h2d(m_.ctx(), dpad, (const double *)h_.data(), np), "nova_memcpy_h2d");
+        nova::check(nova_unpack_padded(m_.ctx(), dpad, NC, d_), "nova_unpack_padded");
+        releasePad(dpad);
+        hostDirty_ = false;
+    }
+    void deviceWritten() { hostValid_ = false; hostDirty_ = false; }
+
+    Field(Mesh &m, const T &init, std::string name = "") : m_(m), name_(std::move(name))
+    {
+        d_ = m_.take(NC);
+        *this = init;
+    }
+    Field(Mesh &m) : Field(m, T(0.0)) {}
+    Field(const Field &f) : m_(f.m_), name_(f.name_)
+    {
+        d_ = m_.take(NC);
+        *this = f;
+    }
+    ~Field()
+    {
+        if (d_) m_.give(NC, d_);
+    }
+
+    int index(int i, int j, int k) const { return k + j * nk() + i * (nk() * nj()); }
+    T &operator()(int i, int j, int k) const
+    {
+        syncHost();
+        hostDirty_ = true;
+        return h_[(size_t)index(i, j, k)];
+    }
+    T &operator()(int i) const
+    {
+        syncHost();
+        hostDirty_ = true;
+        return h_[(size_t)i];
+    }
+    const Mesh &mesh() const { return m_; }
+    Mesh &mesh() { return m_; }
+    int ni() const { return m_.ni(); }
+    int nj() const { return m_.nj(); }
+    int nk() const { return m_.nk(); }
+    std::string &name() { return name_; }
+    const double *dev() const
+    {
+        syncDevice();
+        return d_;
+    }
+    double *devw()
+    {
+        syncDevice();
+        hostValid_ = false;
+        return d_;
+    }
+    // ghosts are never stored on the device: every kernel applies the BC
+    // on load, and the host mirror is filled with them on transfer
+    void correctBoundaryConditions()
+    {
+        if (hostDirty_) syncDevice();
+        hostValid_ = false;
+    }
+
+    void operator=(const Field &f)
+    {
+        if (&f == this) return;
+        nova::check(nova_axpby(m_.ctx(), (size_t)NC * m_.points(), 1.0, f.dev(), 0.0, nullptr, devw()), "copy");
+    }
+    void operator=(const std::shared_ptr<Field> &f) { *this = *f; }
+    void operator=(const T &v)
+    {
+        // constant fill through the host mirror of a single node pattern
+        std::vector<double> tmp((size_t)NC * m_.points());
+        for (int c = 0; c < NC; c++)
+            for (int64_t q = 0; q < m_.points(); q++) tmp[(size_t)c * m_.points() + q] = comp(v, c);
+        nova::check(nova_memcpy_h2d(m_.ctx(), d_, tmp.data(), tmp.size()), "fill");
+        hostValid_ = false;
+        hostDirty_ = false;
+    }
+    void operator+=(const Field &f) { axpby(1.0, *this, 1.0, &f); }
+    void operator+=(const std::shared_ptr<Field> &f) { axpby(1.0, *this, 1.0, f.get()); }
+    void operator-=(const Field &f) { axpby(1.0, *this, -1.0, &f); }
+    void operator-=(const std::shared_ptr<Field> &f) { axpby(1.0, *this, -1.0, f.get()); }
+    void operator*=(const scalar s) { axpby(s, *this, 0.0, nullptr); }
+    void operator/=(const scalar s) { axpby(1.0 / s, *this, 0.0, nullptr); }
+
+    static scalar comp(const T &v, int c);
+    void axpby(scalar a, const Field &x, scalar b, const Field *y)
+    {
+        const double *xp = x.dev();
+        const double *yp = y ? y->dev() : nullptr;
+        nova::check(nova_axpby(m_.ctx(), (size_t)NC * m_.points(), a, xp, b, yp, devw()), "axpby");
+    }
+};
+
+template <> inline scalar Field<scalar>::comp(const scalar &v, int) { return v; }
+template <> inline scalar Field<vector>::comp(const vector &v, int c) { return v[c]; }
+
+template <class T> using FieldPtr = std::shared_ptr<Field<T>>;
+
+namespace nova {
+template <class T>
+FieldPtr<T> tmp(const Mesh &m)
+{
+    return std::make_shared<Field<T>>(const_cast<Mesh &>(m), T(0.0));
+}
+template <class T>
+FieldPtr<T> lincomb(scalar a, const Field<T> &x, scalar b, const Field<T> *y)
+{
+    auto r = tmp<T>(x.mesh());
+    r->axpby(a, x, b, y);
+    return r;
+}
+}  // namespace nova
+
+// ---- Field algebra returning pool temporaries (Field.h:160-367) ----
+template <class T> FieldPtr<T> operator+(const Field<T> &a, const Field<T> &b) { return nova::lincomb(1.0, a, 1.0, &b); }
+template <class T> FieldPtr<T> operator+(const FieldPtr<T> &a, const FieldPtr<T> &b) { return *a + *b; }
+template <class T> FieldPtr<T> operator+(const Field<T> &a, const FieldPtr<T> &b) { return a + *b; }
+template <class T> FieldPtr<T> operator+(const FieldPtr<T> &a, const Field<T> &b) { return *a + b; }
+template <class T> FieldPtr<T> operator-(const Field<T> &a, const Field<T> &b) { return nova::lincomb(1.0, a, -1.0, &b); }
+template <class T> FieldPtr<T> operator-(const FieldPtr<T> &a, const FieldPtr<T> &b) { return *a - *b; }
+template <class T> FieldPtr<T> operator-(const Field<T> &a, const FieldPtr<T> &b) { return a - *b; }
+template <class T> FieldPtr<T> operator-(const FieldPtr<T> &a, const Field<T> &b) { return *a - b; }
+template <class T> FieldPtr<T> operator-(const FieldPtr<T> &a) { return nova::lincomb<T>(-1.0, *a, 0.0, nullptr); }
+template <class T> FieldPtr<T> operator*(scalar s, const Field<T> &a) { return nova::lincomb<T>(s, a, 0.0, nullptr); }
+template <class T> FieldPtr<T> operator*(scalar s, const FieldPtr<T> &a) { return s * *a; }
+template <class T> FieldPtr<T> operator/(const FieldPtr<T> &a, scalar s) { return nova::lincomb<T>(1.0 / s, *a, 0.0, nullptr); }
+
+// ---- explicit operators (src/ex) ----
+namespace ex {
+// ex::div(U, phi), src/ex/div/div.hxx:84-172
+template <class T>
+FieldPtr<T> div(const Field<vector> &U, const Field<T> &phi)
+{
+    auto r = nova::tmp<T>(U.mesh());
+    nova::check(nova_div_adv(U.mesh().ctx(), U.dev(), phi.dev(), pTraits<T>::nComp, r->devw()), "ex::div");
+    return r;
+}
+template <class T> FieldPtr<T> div(const FieldPtr<vector> &U, const FieldPtr<T> &phi) { return div(*U, *phi); }
+// ex::div(U), src/ex/div/div.hxx:22-81
+inline FieldPtr<scalar> div(const Field<vector> &U)
+{
+    auto r = nova::tmp<scalar>(U.mesh());
+    nova::check(nova_div(U.mesh().ctx(), U.dev(), r->devw()), "ex::div");
+    return r;
+}
+inline FieldPtr<scalar> div(const FieldPtr<vector> &U) { return div(*U); }
+// ex::grad(p), src/ex/grad/grad.cpp:25-62
+inline FieldPtr<vector> grad(const Field<scalar> &p)
+{
+    auto r = nova::tmp<vector>(p.mesh());
+    nova::check(nova_grad(p.mesh().ctx(), p.dev(), r->devw()), "ex::grad");
+    return r;
+}
+inline FieldPtr<vector> grad(const FieldPtr<scalar> &p) { return grad(*p); }
+// ex::laplacian(nu, f), src/ex/laplacian/laplacian.hxx:130-191
+template <class T>
+FieldPtr<T> laplacian(const scalar &nu, const Field<T> &f)
+{
+    auto r = nova::tmp<T>(f.mesh());
+    nova::check(nova_laplacian(f.mesh().ctx(), nu, f.dev(), pTraits<T>::nComp, r->devw()), "ex::laplacian");
+    return r;
+}
+// ex::curl, src/ex/curl/curl.cpp:25-57
+inline FieldPtr<vector> curl(const Field<vector> &U)
+{
+    auto r = nova::tmp<vector>(U.mesh());
+    nova::check(nova_curl_literal(U.mesh().ctx(), U.dev(), r->devw()), "ex::curl");
+    return r;
+}
+}  // namespace ex
+
+// ---- poisson (src/poisson/poisson.cpp:6-265) ----
+class poisson {
+    FieldPtr<scalar> p_;
+    FieldPtr<scalar> rhs_;
+
+public:
+    explicit poisson(FieldPtr<scalar> p) : p_(std::move(p)) { p_->mesh().poissonHandle(); }
+    void rhs(const FieldPtr<scalar> &f) { rhs_ = f; }
+    void solve()
+    {
+        if (!rhs_) throw nova::Error("poisson::solve without rhs");
+        nova::check(nova_poisson_solve(p_->mesh().poissonHandle(), rhs_->dev(), p_->devw()), "poisson::solve");
+    }
+};
+
+// ---- tools (src/Tools/Tools.h:19-36, Tools.cpp:43-62) ----
+namespace tools {
+const scalar pi = 3.14159265358979323846;
+const scalar eps = 1.0e-16;
+inline scalar maximum(const FieldPtr<scalar> &f)
+{
+    // max |f| over [m/2, n - m/2) = every stored node
+    std::vector<double> h((size_t)f->mesh().points());
+    nova::check(nova_memcpy_d2h(f->mesh().ctx(), h.data(), f->dev(), h.size()), "maximum");
+    scalar mx = 0.0;
+    for (double v : h) mx = std::fabs(v) > mx ? std::fabs(v) : mx;
+    return mx;
+}
+}  // namespace tools
