@@ -35,6 +35,8 @@ def alg_bytes_per_point(name: str, n: int) -> float:
     return {
         "ns_stage": (48.0 + 72.0 + 72.0) / 3.0,  # stage 1 reads U only
         "fft_div_r2c_z": 24.0 + half,
+        "div_rows": 24.0 + 8.0,
+        "fft_r2c_z": 8.0 + half,
         "fft_y": 2 * half,
         "fft_x_solve": 2 * half,
         "fft_c2r_z": half + 8.0,
@@ -78,6 +80,67 @@ def cpu_baseline(n: int, steps: int) -> dict:
                       f"{cores} OpenMP threads, {dt:.1f} s"}
 
 
+def shear_layer(nv, G, seed=5):
+    """u = tanh(2y) on y in [-6, 6] plus a seeded random-phase perturbation of
+    amplitude 1e-3 on the modes kx, kz <= 8 (localised in y), projected."""
+    import torch
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    sx, sy, sz = G.shape
+    Lx, Ly, Lz = G.L
+    x = torch.arange(sx, dtype=torch.float64, device=G.device) * (Lx / G.N[0])
+    y = torch.arange(sy, dtype=torch.float64, device=G.device) * (Ly / G.N[1]) - Ly / 2
+    z = torch.arange(sz, dtype=torch.float64, device=G.device) * (Lz / G.N[2])
+    U = torch.zeros((3, sx, sy, sz), dtype=torch.float64, device=G.device)
+    U[0] += torch.tanh(2 * y)[None, :, None]
+    env = torch.exp(-y * y)[None, :, None]
+    for c in range(3):
+        for kx in range(9):
+            for kz in range(9):
+                if kx == 0 and kz == 0:
+                    continue
+                ph = float(torch.rand(1, generator=gen)) * 2 * math.pi
+                U[c] += 1e-3 * torch.cos(2 * math.pi * kx * x[:, None, None] / Lx
+                                         + 2 * math.pi * kz * z[None, None, :] / Lz + ph) * env
+    U[1, :, 0, :] = 0.0
+    U[1, :, -1, :] = 0.0
+    return U.contiguous()
+
+
+def adjoint_ratio(nv, steps: int, every: int) -> dict:
+    """Time of the checkpointed adjoint sweep over the time of the forward
+    solve, shear layer 256x384x256 (x, z periodic, free-slip y walls)."""
+    import torch
+    N, L = (256, 384, 256), (8 * math.pi, 12.0, 8 * math.pi)
+    G = nv.Mesh(N, L, bc=("periodic", "wall", "periodic"))
+    P = nv.Poisson(G)
+    S = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-3, graph=True)
+    U0 = shear_layer(nv, G)
+    P.project(U0, 1.0)
+    A = nv.Adjoint(nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-3), every, steps)
+    dt = 0.02
+    U, p = U0.clone(), G.field(1)
+    S.step(dt, 2, U, p)
+    torch.cuda.synchronize()
+    U = U0.clone()
+    t0 = time.perf_counter()
+    S.step(dt, steps, U, p)
+    torch.cuda.synchronize()
+    tf = time.perf_counter() - t0
+    UT = A.forward(dt, steps, U0)
+    J, g = nv.enstrophy(G, UT, grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g0 = A.backward(g)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    gn = nv.weighted_dot(G, g0, g0) ** 0.5
+    A.close()
+    return {"ratio": round(tb / tf, 3), "forward_s": round(tf, 3), "adjoint_s": round(tb, 3),
+            "workload": f"shear layer 256x384x256 FP64, free-slip y walls (DCT-I), Re=1000, dt=0.02, "
+                        f"{steps} SSP-RK3 steps, checkpoints every {every}; gradient of final enstrophy",
+            "J": J, "grad_norm": gn}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -86,6 +149,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=500)
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of graph replay")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain device copy of 2^30 bytes 5 times (counter calibration)")
@@ -147,10 +212,10 @@ def main() -> int:
     dom = max(stats, key=lambda k: stats[k][1])
     traffic = None
     tfile = ROOT / "profiles" / "pmc_traffic.json"
-    if tfile.exists():
+    if tfile.exists() and n == 256:
         try:
             traffic = json.loads(tfile.read_text()).get(dom, {}).get("bytes_per_launch")
-        except Exception:
+        except (ValueError, AttributeError):
             traffic = None
     dk = kernels[dom]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -159,6 +224,14 @@ def main() -> int:
     step_alg = 384.0 * n ** 3  # minimal-traffic model of one RK3 + projection step
     step_roofline = {"alg_bytes_per_step": step_alg, "achieved_GBps": round(step_alg / (ms_per_step / 1e3) / 1e9, 1),
                      "frac": round(step_alg / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    adj = None
+    if not args.no_adjoint:
+        del S, S_eager, U, p
+        torch.cuda.empty_cache()
+        log("bench: adjoint (shear layer) ...")
+        adj = adjoint_ratio(nv, args.adjoint_steps, 25)
+        log(f"bench: adjoint/forward ratio {adj['ratio']}")
 
     cpu = None
     if not args.no_cpu:
@@ -187,6 +260,7 @@ def main() -> int:
         "kernels": kernels,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "adjoint": adj,
         "check": {"ke": ke, "maxabs_div": maxdiv},
     }
     print(json.dumps(out), flush=True)

@@ -72,6 +72,7 @@ typedef struct nova_grid_desc {
 typedef struct nova_ctx nova_ctx;
 typedef struct nova_poisson nova_poisson;
 typedef struct nova_solver nova_solver;
+typedef struct nova_adjoint nova_adjoint;
 
 /* ---- context / memory (replaces Mesh construction, Mesh.cpp:24-179, and
  * Field storage, src/Field/Field.hxx:21-134) ---- */
@@ -171,6 +172,26 @@ int nova_reduce_dissipation(nova_ctx *ctx, const double *U, double nu, double *o
 /* out = {Ek, Em, Jav, Jmax, omega, epsilon, Umax, 0} */
 int nova_mhd_diag(nova_ctx *ctx, const double *U, const double *B, double mu, double eta,
                   double out[8]);
+
+/* ---- discrete adjoint of the SSP-RK3 + projection step (no reference
+ * counterpart: NOVA has no adjoint).  Inner product: sum over stored nodes,
+ * weight 1/2 on wall nodes (nova_weighted_dot); gradients are with respect
+ * to that inner product. ---- */
+/* Reserve checkpoint memory for up to max_steps steps, one checkpoint of U
+ * every checkpoint_every steps (plus 3*checkpoint_every stage fields). */
+int nova_adjoint_create(nova_solver *sv, int checkpoint_every, int max_steps, nova_adjoint **out);
+int nova_adjoint_destroy(nova_adjoint *ad);
+/* U = forward(U0) over nsteps of dt, storing the checkpoints. */
+int nova_adjoint_forward(nova_adjoint *ad, double dt, int nsteps, const double *U0, double *U);
+/* lam0 = (dU_final/dU0)^T lam_final along the last forward trajectory. */
+int nova_adjoint_backward(nova_adjoint *ad, const double *lam_final, double *lam0);
+/* dU_final = (dU_final/dU0) dU0 (tangent-linear model). */
+int nova_adjoint_tlm(nova_adjoint *ad, const double *dU0, double *dU_final);
+/* out = sum_nodes w a.b over ncomp components. */
+int nova_weighted_dot(nova_ctx *ctx, const double *a, const double *b, int ncomp, double *out);
+/* J = dV sum_nodes w |curl U|^2 (central stencil curl); dJdU (may be NULL)
+ * receives its gradient 2 dV curl(curl U). */
+int nova_enstrophy(nova_ctx *ctx, const double *U, double *J, double *dJdU);
 
 /* ---- batched tridiagonal (src/Tools/Tools.cpp:43-62, tools::tdma) ----
  * nbatch systems of size n, system s at a[s*n ...]; cyclic != 0 adds the

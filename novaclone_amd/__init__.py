@@ -91,6 +91,13 @@ _SIGS = {
     "nova_reduce_dissipation": [_P, _P, _D, _DP],
     "nova_mhd_diag": [_P, _P, _P, _D, _D, _DP],
     "nova_tdma_batched": [_P, _I, _I, _P, _P, _P, _P, _P, _I],
+    "nova_adjoint_create": [_P, _I, _I, C.POINTER(_P)],
+    "nova_adjoint_destroy": [_P],
+    "nova_adjoint_forward": [_P, _D, _I, _P, _P],
+    "nova_adjoint_backward": [_P, _P, _P],
+    "nova_adjoint_tlm": [_P, _P, _P],
+    "nova_weighted_dot": [_P, _P, _P, _I, _DP],
+    "nova_enstrophy": [_P, _P, _DP, _P],
 }
 
 
@@ -377,3 +384,77 @@ def tdma_batched(mesh: Mesh, a, b, c, d, cyclic: bool = False) -> torch.Tensor:
     _check(lib().nova_tdma_batched(mesh.handle, n, nb, _ptr(a), _ptr(b), _ptr(c), _ptr(d), _ptr(x), int(cyclic)),
            "nova_tdma_batched")
     return x
+
+
+def weighted_dot(mesh: Mesh, a: torch.Tensor, b: torch.Tensor) -> float:
+    """sum over nodes of w a.b, w = 1/2 on wall nodes."""
+    out = C.c_double()
+    _check(lib().nova_weighted_dot(mesh.handle, _ptr(a), _ptr(b), a.shape[0], C.byref(out)), "nova_weighted_dot")
+    return out.value
+
+
+def enstrophy(mesh: Mesh, U: torch.Tensor, grad: bool = False):
+    """J = dV sum w |curl U|^2; with grad=True also dJ/dU = 2 dV curl curl U."""
+    J = C.c_double()
+    g = mesh.field(3) if grad else None
+    _check(lib().nova_enstrophy(mesh.handle, _ptr(U), C.byref(J), _ptr(g)), "nova_enstrophy")
+    return (J.value, g) if grad else J.value
+
+
+class Adjoint:
+    """Checkpointed discrete adjoint and tangent-linear model of the SSP-RK3
+    + projection step (nova_adjoint_*)."""
+
+    def __init__(self, solver: Solver, checkpoint_every: int, max_steps: int):
+        self.solver = solver
+        self.mesh = solver.mesh
+        h = C.c_void_p()
+        _check(lib().nova_adjoint_create(solver._h, int(checkpoint_every), int(max_steps), C.byref(h)),
+               "nova_adjoint_create")
+        self._h = h
+
+    def forward(self, dt: float, nsteps: int, U0: torch.Tensor) -> torch.Tensor:
+        U = torch.empty_like(U0)
+        _check(lib().nova_adjoint_forward(self._h, float(dt), int(nsteps), _ptr(U0), _ptr(U)), "nova_adjoint_forward")
+        return U
+
+    def backward(self, lam: torch.Tensor) -> torch.Tensor:
+        out = torch.empty_like(lam)
+        _check(lib().nova_adjoint_backward(self._h, _ptr(lam.contiguous()), _ptr(out)), "nova_adjoint_backward")
+        return out
+
+    def tlm(self, d0: torch.Tensor) -> torch.Tensor:
+        out = torch.empty_like(d0)
+        _check(lib().nova_adjoint_tlm(self._h, _ptr(d0.contiguous()), _ptr(out)), "nova_adjoint_tlm")
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nova_adjoint_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class EnstrophyAfterSteps(torch.autograd.Function):
+    """J(U0) = enstrophy of the flow advanced nsteps from U0; backward runs the
+    discrete adjoint.  Gradients are with respect to the node-sum inner
+    product (weight 1/2 on wall nodes)."""
+
+    @staticmethod
+    def forward(ctx, U0: torch.Tensor, adjoint: "Adjoint", dt: float, nsteps: int):
+        U = adjoint.forward(dt, nsteps, U0.detach().contiguous())
+        J, g = enstrophy(adjoint.mesh, U, grad=True)
+        ctx.adjoint = adjoint
+        ctx.save_for_backward(g)
+        return torch.tensor(J, dtype=torch.float64, device=U0.device)
+
+    @staticmethod
+    def backward(ctx, gJ: torch.Tensor):
+        (g,) = ctx.saved_tensors
+        lam0 = ctx.adjoint.backward(g)
+        return gJ * lam0, None, None, None

@@ -8,6 +8,7 @@
 
 struct nova_poisson;
 struct nova_solver;
+struct nova_adjoint;
 
 namespace nova {
 
@@ -90,6 +91,11 @@ int solver_destroy(nova_solver *sv);
 int solver_step(nova_solver *sv, double dt, int nsteps, double *U, double *B, double *p, double *pB);
 int solver_use_graph(nova_solver *sv, int enable);
 nova_ctx *poisson_ctx(nova_poisson *ps);
+int adjoint_create(nova_solver *sv, int every, int max_steps, nova_adjoint **out);
+int adjoint_destroy(nova_adjoint *ad);
+int adjoint_forward(nova_adjoint *ad, double dt, int nsteps, const double *U0, double *U);
+int adjoint_backward(nova_adjoint *ad, const double *lamT, double *lam0);
+int adjoint_tlm(nova_adjoint *ad, const double *d0, double *dT);
 
 // Mesh::z (src/Mesh/Mesh.cpp:373-407) for padded index kp on one rank.
 static double mesh_z(const nova_grid_desc &d, double dz, double hb, int m, int kp)
@@ -504,6 +510,56 @@ int nova_mhd_diag(nova_ctx *ctx, const double *U, const double *B, double mu, do
     if (!rc) rc = launch_curl(ctx, U, 1, W);
     if (!rc) rc = reduce_mhd(ctx, U, B, J, W, mu, eta, out);
     (void)hipFreeAsync(J, ctx->stream);
+    return rc;
+}
+
+int nova_adjoint_create(nova_solver *sv, int checkpoint_every, int max_steps, nova_adjoint **out)
+{
+    NOVA_REQUIRE(sv && out, "null argument");
+    NOVA_GUARD(adjoint_create(sv, checkpoint_every, max_steps, out));
+}
+
+int nova_adjoint_destroy(nova_adjoint *ad) { NOVA_GUARD(adjoint_destroy(ad)); }
+
+int nova_adjoint_forward(nova_adjoint *ad, double dt, int nsteps, const double *U0, double *U)
+{
+    NOVA_REQUIRE(ad && U0 && U, "bad argument");
+    NOVA_GUARD(adjoint_forward(ad, dt, nsteps, U0, U));
+}
+
+int nova_adjoint_backward(nova_adjoint *ad, const double *lam_final, double *lam0)
+{
+    NOVA_REQUIRE(ad && lam_final && lam0, "bad argument");
+    NOVA_GUARD(adjoint_backward(ad, lam_final, lam0));
+}
+
+int nova_adjoint_tlm(nova_adjoint *ad, const double *dU0, double *dU_final)
+{
+    NOVA_REQUIRE(ad && dU0 && dU_final, "bad argument");
+    NOVA_GUARD(adjoint_tlm(ad, dU0, dU_final));
+}
+
+int nova_weighted_dot(nova_ctx *ctx, const double *a, const double *b, int ncomp, double *out)
+{
+    NOVA_REQUIRE(ctx && a && b && out && ncomp > 0, "bad argument");
+    NOVA_GUARD(reduce_wdot(ctx, a, b, ncomp, out));
+}
+
+int nova_enstrophy(nova_ctx *ctx, const double *U, double *J, double *dJdU)
+{
+    NOVA_REQUIRE(ctx && U && J && dJdU != U, "bad argument");
+    const double dV = ctx->d[0] * ctx->d[1] * ctx->d[2];
+    double *W = nullptr;
+    NOVA_HIP(hipMallocAsync((void **)&W, sizeof(double) * 3 * ctx->g.np, ctx->stream));
+    int rc = launch_curl(ctx, U, 0, W);
+    double s = 0.0;
+    if (!rc) rc = reduce_wdot(ctx, W, W, 3, &s);
+    if (!rc) *J = dV * s;
+    if (!rc && dJdU) {
+        rc = launch_curl(ctx, W, 0, dJdU, 1);  // W is a curl: pseudovector parity at walls
+        if (!rc) rc = launch_axpby(ctx, 3 * (size_t)ctx->g.np, 2.0 * dV, dJdU, 0.0, nullptr, dJdU);
+    }
+    (void)hipFreeAsync(W, ctx->stream);
     return rc;
 }
 

@@ -133,7 +133,7 @@ int launch_div_adv(nova_ctx *ctx, const double *U, const double *phi, int nc, do
 int launch_div(nova_ctx *ctx, const double *U, double scale, double *out);
 int launch_grad(nova_ctx *ctx, const double *p, double *out);
 int launch_laplacian(nova_ctx *ctx, double nu, const double *f, int nc, double *out);
-int launch_curl(nova_ctx *ctx, const double *U, int literal, double *out);
+int launch_curl(nova_ctx *ctx, const double *U, int literal, double *out, int pseudo = 0);
 int launch_axpby(nova_ctx *ctx, size_t n, double a, const double *x, double b, const double *y,
                  double *out);
 int launch_pack(nova_ctx *ctx, const double *compact, int nc, double *padded);
@@ -142,6 +142,7 @@ int launch_bc_fill(nova_ctx *ctx, double *padded, int nc, int dir, int kind, int
 int reduce_ke(nova_ctx *ctx, const double *U, double *out);
 int reduce_maxabs(nova_ctx *ctx, const double *f, double *out);
 int reduce_dissipation(nova_ctx *ctx, const double *U, double nu, double *out);
+int reduce_wdot(nova_ctx *ctx, const double *a, const double *b, int nc, double *out);
 int reduce_mhd(nova_ctx *ctx, const double *U, const double *B, const double *J,
                const double *W, double mu, double eta, double out[8]);
 int launch_tdma(nova_ctx *ctx, int n, int nb, const double *a, const double *b, const double *c,

@@ -135,10 +135,22 @@ __global__ void k_laplacian(Grid g, double nu, const double *__restrict__ f, int
     }
 }
 
+// Load of a 3-component field with the reflection parity of a pseudovector
+// (the curl of a velocity: the wall-normal component even, the others odd).
+__device__ inline double ld_pseudo(const double *__restrict__ f, const Grid &g, int c, int i, int j, int k)
+{
+    int fi, fj, fk;
+    int ii = bmap(i, g.n[0], g.wall[0], fi);
+    int jj = bmap(j, g.n[1], g.wall[1], fj);
+    int kk = bmap(k, g.n[2], g.wall[2], fk);
+    double s = (fi && c != 0 ? -1.0 : 1.0) * (fj && c != 1 ? -1.0 : 1.0) * (fk && c != 2 ? -1.0 : 1.0);
+    return s * f[c * g.np + lin(g, ii, jj, kk)];
+}
+
 // curl: literal = antisymmetric part of the half-shifted gradient
 // (src/ex/grad/grad.cpp:73-98, src/ex/curl/curl.cpp:38-53); otherwise the
-// central stencil per component.
-__global__ void k_curl(Grid g, const double *__restrict__ U, int literal, double *__restrict__ out)
+// central stencil per component.  pseudo: the input is itself a curl.
+__global__ void k_curl(Grid g, const double *__restrict__ U, int literal, int pseudo, double *__restrict__ out)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -155,6 +167,9 @@ __global__ void k_curl(Grid g, const double *__restrict__ U, int literal, double
                     int ei, ej, ek;
                     offs(d, l + 1, ei, ej, ek);
                     ad += g.al[d][l] * (ld(U, g, 3, c, i + di, j + dj, k + dk) - ld(U, g, 3, c, i - ei, j - ej, k - ek));
+                } else if (pseudo) {
+                    offs(d, l + 1, di, dj, dk);
+                    ad += g.a[d][l] * (ld_pseudo(U, g, c, i + di, j + dj, k + dk) - ld_pseudo(U, g, c, i - di, j - dj, k - dk));
                 } else {
                     offs(d, l + 1, di, dj, dk);
                     ad += g.a[d][l] * (ld(U, g, 3, c, i + di, j + dj, k + dk) - ld(U, g, 3, c, i - di, j - dj, k - dk));
@@ -339,6 +354,31 @@ __global__ void k_diss_part(Grid g, const double *__restrict__ U, double *__rest
     block_reduce<1>(v1, mx, part);
 }
 
+// Weighted inner product sum_nodes w sum_c a_c b_c, w = 1/2 on wall nodes
+// (the trapezoidal weight under which the central differences with
+// reflecting walls satisfy D^T = -D).
+__global__ void k_wdot_part(Grid g, const double *__restrict__ a, const double *__restrict__ b, int nc,
+                            double *__restrict__ part)
+{
+    double acc = 0.0;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < g.np; q += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(q % g.s[2]);
+        const int64_t r = q / g.s[2];
+        const int j = (int)(r % g.s[1]);
+        const int i = (int)(r / g.s[1]);
+        double w = 1.0;
+        if (g.wall[0] && (i == 0 || i == g.n[0])) w *= 0.5;
+        if (g.wall[1] && (j == 0 || j == g.n[1])) w *= 0.5;
+        if (g.wall[2] && (k == 0 || k == g.n[2])) w *= 0.5;
+        double v = 0.0;
+        for (int c = 0; c < nc; c++) v += a[c * g.np + q] * b[c * g.np + q];
+        acc += w * v;
+    }
+    double v1[1] = {acc};
+    const int mx[1] = {0};
+    block_reduce<1>(v1, mx, part);
+}
+
 __global__ void k_maxabs_part(int64_t n, const double *__restrict__ f, double *__restrict__ part)
 {
     double a = 0.0;
@@ -464,9 +504,9 @@ int launch_laplacian(nova_ctx *ctx, double nu, const double *f, int nc, double *
     return check_launch("laplacian");
 }
 
-int launch_curl(nova_ctx *ctx, const double *U, int literal, double *out)
+int launch_curl(nova_ctx *ctx, const double *U, int literal, double *out, int pseudo)
 {
-    hipLaunchKernelGGL(k_curl, pt_grid(ctx->g), pt_block(), 0, ctx->stream, ctx->g, U, literal, out);
+    hipLaunchKernelGGL(k_curl, pt_grid(ctx->g), pt_block(), 0, ctx->stream, ctx->g, U, literal, pseudo, out);
     return check_launch("curl");
 }
 
@@ -552,6 +592,20 @@ int reduce_dissipation(nova_ctx *ctx, const double *U, double nu, double *out)
     NOVA_HIP(hipMemcpyAsync(ctx->h_red, ctx->d_red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     NOVA_HIP(hipStreamSynchronize(ctx->stream));
     *out = nu * ctx->h_red[0] / (double)tot;
+    return NOVA_OK;
+}
+
+int reduce_wdot(nova_ctx *ctx, const double *a, const double *b, int nc, double *out)
+{
+    const int nb = red_blocks(ctx->g.np);
+    if (int rc = ensure_red(ctx, (size_t)nb * 8 + 8)) return rc;
+    hipLaunchKernelGGL(k_wdot_part, dim3(nb), dim3(kRedThreads), 0, ctx->stream, ctx->g, a, b, nc, ctx->d_red + 8);
+    hipLaunchKernelGGL(k_red_final<1>, dim3(1), dim3(kRedThreads), 0, ctx->stream, ctx->d_red + 8, nb, ctx->d_red,
+                       0, 0, 0, 0, 0, 0, 0, 0);
+    if (int rc = check_launch("reduce_wdot")) return rc;
+    NOVA_HIP(hipMemcpyAsync(ctx->h_red, ctx->d_red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    NOVA_HIP(hipStreamSynchronize(ctx->stream));
+    *out = ctx->h_red[0];
     return NOVA_OK;
 }
 

@@ -8,6 +8,7 @@
 // Lorentz/induction) right-hand side and the stage combination are
 // evaluated per node and written once.
 #include "nova_internal.hpp"
+#include "nova_stencil.hpp"
 
 struct nova_poisson;
 
@@ -36,144 +37,6 @@ namespace nova {
 
 namespace {
 
-// Stage combination forms.
-enum { FORM_SSP = 0, FORM_LIT4 = 1, FORM_SHEAR = 2 };
-
-struct StageArgs {
-    int form;
-    double alpha, beta;  // FORM_SSP: alpha*U0 + beta*(U + dU); FORM_SHEAR: U0 + beta*dU
-    double dt;
-    double w[4];         // FORM_LIT4 accumulation weights
-};
-
-__device__ inline double combine(const StageArgs &a, double u, double u0, double du)
-{
-    if (a.form == FORM_SSP) {
-        const double v = u + du;
-        return a.alpha != 0.0 ? a.alpha * u0 + a.beta * v : v;
-    }
-    if (a.form == FORM_LIT4) {
-        double s = u;
-        for (int j = 0; j < 4; j++) s = s + a.w[j] * du;
-        return s;
-    }
-    return u0 + a.beta * du;
-}
-
-// ---- right-hand sides, written against an accessor A(f, c, d, o) that
-// returns component c of field f at the node displaced by o along
-// direction d (d = 0: i, 1: j, 2: k; o = 0 is the node itself) ----
-
-// div(X, Y_c): 0.5 sum_d sum_l a [ (X_d Y_c)^+ - (X_d Y_c)^- + X_d (Y_c^+ - Y_c^-)
-// + Y_c (X_d^+ - X_d^-) ]  (src/ex/div/div.hxx:101-169).
-template <int H, class Acc>
-__device__ inline double div_xy(const Grid &g, const Acc &A, int fx, int fy, int c, double zi)
-{
-    const double yc = A(fy, c, 0, 0);
-    double acc = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const double xd = A(fx, d, 0, 0);
-        double ad = 0.0;
-#pragma unroll
-        for (int l = 0; l < H; l++) {
-            const double xp = A(fx, d, d, l + 1), xm = A(fx, d, d, -l - 1);
-            const double yp = A(fy, c, d, l + 1), ym = A(fy, c, d, -l - 1);
-            ad += g.a[d][l] * ((yp * xp - ym * xm) + xd * (yp - ym) + yc * (xp - xm));
-        }
-        acc += d == 2 ? ad * zi : ad;
-    }
-    return 0.5 * acc;
-}
-
-// nu * laplacian(f_c) with the z-metric term
-// (src/ex/laplacian/laplacian.hxx:140-188).
-template <int H, class Acc>
-__device__ inline double lap_c(const Grid &g, const Acc &A, int f, int c, double nu, double zl, double zm)
-{
-    const double f0 = A(f, c, 0, 0);
-    double acc = 0.0, dz1 = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        double ad = 0.0;
-#pragma unroll
-        for (int l = 0; l < H; l++) {
-            const double fp = A(f, c, d, l + 1), fm = A(f, c, d, -l - 1);
-            ad += g.b[d][l] * ((fp + fm) - 2.0 * f0);
-            if (d == 2) dz1 += g.a[2][l] * (fp - fm);
-        }
-        acc += d == 2 ? ad * zl : ad;
-    }
-    return nu * acc - nu * zm * dz1;
-}
-
-// d/dx_d of 0.5|B|^2 (app/OT_Poisson/UEqn.H:22).
-template <int H, class Acc>
-__device__ inline double grad_pm(const Grid &g, const Acc &A, int fb, int d, double zi)
-{
-    double ad = 0.0;
-#pragma unroll
-    for (int l = 0; l < H; l++) {
-        double bp = 0.0, bm = 0.0;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const double vp = A(fb, c, d, l + 1), vm = A(fb, c, d, -l - 1);
-            bp += vp * vp;
-            bm += vm * vm;
-        }
-        ad += g.a[d][l] * (0.5 * bp - 0.5 * bm);
-    }
-    return d == 2 ? ad * zi : ad;
-}
-
-// NS right-hand side of all three components, direction by direction:
-// out[c] = -div(U, U_c) + nu lap(U_c) with the same terms as div_xy and
-// lap_c, organised so that only one direction's neighbours are live.
-template <int H, class Acc>
-__device__ inline void ns_rhs(const Grid &g, const Acc &A, double nu, double zi, double zl, double zm,
-                              double (&out)[3])
-{
-    // per direction d and component c, with the weights factored as in the
-    // reference (c_l / (2(l+1)) times 1/h_d):
-    //   S1 = sum_l wa_l (U_c^+ U_d^+ - U_c^- U_d^-)   conservative flux
-    //   S2 = sum_l wa_l (U_c^+ - U_c^-)               first derivative
-    //   SL = sum_l wb_l (U_c^+ + U_c^-)               second derivative
-    // adv_c = sum_d s_d (S1 + U_d S2) + U_c div(U);  -div(U,U_c) = -adv_c / 2
-    double u0[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) u0[c] = A(0, c, 0, 0);
-    double adv[3] = {0.0, 0.0, 0.0}, lap[3] = {0.0, 0.0, 0.0}, dz1[3] = {0.0, 0.0, 0.0};
-    double divu = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        double s1[3] = {0.0, 0.0, 0.0}, s2[3] = {0.0, 0.0, 0.0}, sl[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-        for (int l = 0; l < H; l++) {
-            const double up = A(0, d, d, l + 1), um = A(0, d, d, -l - 1);
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const double cp = c == d ? up : A(0, c, d, l + 1);
-                const double cm = c == d ? um : A(0, c, d, -l - 1);
-                s1[c] += g.wa[l] * (cp * up - cm * um);
-                s2[c] += g.wa[l] * (cp - cm);
-                sl[c] += g.wb[l] * (cp + cm);
-            }
-        }
-        const double sa = d == 2 ? g.ih[2] * zi : g.ih[d];
-        const double sb = d == 2 ? g.ih2[2] * zl : g.ih2[d];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            adv[c] += sa * (s1[c] + u0[d] * s2[c]);
-            lap[c] += sb * (sl[c] - g.wb_sum2 * u0[c]);
-            if (d == 2) dz1[c] = g.ih[2] * s2[c];
-        }
-        divu += sa * s2[d];
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-        out[c] = (-(0.5 * (adv[c] + u0[c] * divu))) + nu * (lap[c] - zm * dz1[c]);
-}
-
 // ---- the streamed stage kernel ----
 //
 // A 64 x 4 block owns a (k, j) tile and walks i over a chunk of planes.
@@ -194,14 +57,6 @@ struct StreamArgs {
     int ichunk;
     int nkt, njt, nic;
 };
-
-// Sign of a value reached through a reflection (walls only; a periodic
-// grid has no reflections and carries no signs).
-template <bool WALLS>
-__device__ inline double rsign(int c, int d, int flip)
-{
-    return WALLS ? bsign(3, c, d, flip) : 1.0;
-}
 
 template <int H, int NF, bool WALLS, int MINW, int kTK>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MINW))) k_stage_stream(Grid g, StreamArgs<NF> a)
@@ -400,22 +255,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MINW))
                 }
         }
     }
-}
-
-// Variant without LDS and without barriers: each wave owns one 64-wide k
-// row and streams i with the same register queue; its j and k neighbours
-// are loaded straight from global memory (the four rows of a block share
-// L1 lines, adjacent blocks share L2 lines).
-template <bool WALLS>
-__device__ inline int wrap(int q, int n, int wall, int &flip)
-{
-    if (!WALLS) {
-        flip = 0;
-        q += q < 0 ? n : 0;
-        q -= q >= n ? n : 0;
-        return q;
-    }
-    return bmap(q, n, wall, flip);
 }
 
 template <int H, int NF, bool WALLS>
@@ -678,6 +517,21 @@ int mhd_step_once(nova_solver *sv, double dt, double *U, double *B, double *p, d
 }
 
 }  // namespace
+
+int stage_ns(nova_ctx *ctx, const double *U, const double *U0, double *out, const StageArgs &sa, double nu)
+{
+    return ns_stage(ctx, U, U0, out, sa, nu);
+}
+
+nova_poisson *solver_poisson(nova_solver *sv) { return sv->ps; }
+
+int solver_model(nova_solver *sv, int *mode, double *nu)
+{
+    if (sv->model != NOVA_MODEL_NS) return fail(NOVA_ERR_UNSUPPORTED, "the adjoint covers the NS model");
+    *mode = sv->mode;
+    *nu = sv->nu;
+    return NOVA_OK;
+}
 
 int solver_create(nova_poisson *ps, int model, int mode, double nu, double eta, nova_solver **out)
 {

@@ -1,0 +1,150 @@
+"""Discrete adjoint of the SSP-RK3 + projection step (no reference
+counterpart; pinned by mathematics): the dot-product identity between the
+tangent-linear model and the adjoint, central finite differences of the
+enstrophy objective, and the torch.autograd harness.  Tolerances are the
+north star's 1e-8 for FD agreement and roundoff-level for the dot product."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+TWO_PI = 2.0 * math.pi
+
+
+@pytest.fixture(scope="module")
+def nv():
+    import torch
+    import novaclone_amd as nv
+    assert torch.cuda.is_available()
+    nv.lib()
+    return nv
+
+
+def flow(nv, N, L, wall=False, seed=0, amp=1e-2):
+    import torch
+    bc = ("periodic", "wall" if wall else "periodic", "periodic")
+    G = nv.Mesh(N, L, bc=bc)
+    P = nv.Poisson(G)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    s = G.shape
+    x = [torch.arange(s[a], dtype=torch.float64, device="cuda") * (L[a] / N[a]) for a in range(3)]
+    X, Y, Z = torch.meshgrid(*x, indexing="ij")
+    if wall:
+        U = torch.stack([torch.tanh(2 * (Y - L[1] / 2)), 0 * X, 0 * X])
+    else:
+        U = torch.stack([torch.sin(X) * torch.cos(Y) * torch.cos(Z), -torch.cos(X) * torch.sin(Y) * torch.cos(Z),
+                         0 * X])
+    U = U + amp * torch.randn(U.shape, generator=gen, dtype=torch.float64, device="cuda")
+    if wall:
+        U[1, :, 0, :] = 0.0
+        U[1, :, -1, :] = 0.0
+    U = U.contiguous()
+    P.project(U, 1.0)
+    return G, P, U
+
+
+def rand_like(U, seed, wall=False):
+    import torch
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    r = torch.randn(U.shape, generator=gen, dtype=torch.float64, device="cuda")
+    if wall:
+        r[1, :, 0, :] = 0.0
+        r[1, :, -1, :] = 0.0
+    return r.contiguous()
+
+
+@pytest.mark.parametrize("wall,nsteps,every", [(False, 1, 1), (False, 7, 3), (True, 5, 2)])
+def test_dot_product_tlm_adjoint(nv, wall, nsteps, every):
+    N = (24, 16, 32) if not wall else (16, 16, 24)
+    L = (TWO_PI,) * 3 if not wall else (4.0, 3.0, 4.0)
+    G, P, U0 = flow(nv, N, L, wall=wall, seed=1)
+    S = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=2e-2)
+    A = nv.Adjoint(S, every, 16)
+    A.forward(0.01, nsteps, U0)
+    d = rand_like(U0, 2, wall)
+    m = rand_like(U0, 3, wall)
+    Ld = A.tlm(d)
+    LTm = A.backward(m)
+    lhs = nv.weighted_dot(G, Ld, m)
+    rhs = nv.weighted_dot(G, d, LTm)
+    print("dot", wall, nsteps, lhs, rhs, flush=True)
+    assert abs(lhs - rhs) <= 1e-11 * max(abs(lhs), abs(rhs))
+
+
+@pytest.mark.parametrize("wall", [False, True])
+def test_enstrophy_gradient_matches_central_differences(nv, wall):
+    """16 seeded directions.  The 4th-order central difference (eps 1e-4,
+    truncation ~eps^4, roundoff ~u J / eps) must agree to 1e-8; the
+    2nd-order one at eps = 1e-6 must agree to 1e-8 plus its roundoff bound
+    4 u J / eps (u = 2^-53), which is of the same size for J ~ 1e3."""
+    import torch
+    if wall:
+        N, L = (16, 16, 16), (4.0, 3.0, 4.0)
+    else:
+        N, L = (32, 32, 32), (TWO_PI,) * 3
+    G, P, U0 = flow(nv, N, L, wall=wall, seed=5, amp=5e-2)
+    S = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-2)
+    A = nv.Adjoint(S, 4, 16)
+    dt, n = 0.01, 10
+    U = A.forward(dt, n, U0)
+    J, gT = nv.enstrophy(G, U, grad=True)
+    g0 = A.backward(gT)
+    Jf = lambda X: nv.enstrophy(G, A.forward(dt, n, X.contiguous()))
+    u = 2.0 ** -53
+    for seed in range(16):
+        d = rand_like(U0, 100 + seed, wall)
+        d = d / d.abs().max()
+        ad = nv.weighted_dot(G, g0, d)
+        e4 = 1e-4
+        fd4 = (-Jf(U0 + 2 * e4 * d) + 8 * Jf(U0 + e4 * d) - 8 * Jf(U0 - e4 * d) + Jf(U0 - 2 * e4 * d)) / (12 * e4)
+        e2 = 1e-6
+        fd2 = (Jf(U0 + e2 * d) - Jf(U0 - e2 * d)) / (2 * e2)
+        print(f"fd wall={wall} seed={seed} ad={ad:.12e} fd4 rel={abs(fd4 - ad) / abs(ad):.2e} "
+              f"fd2 rel={abs(fd2 - ad) / abs(ad):.2e} J={J:.3e}", flush=True)
+        assert abs(fd4 - ad) <= 1e-8 * abs(ad)
+        assert abs(fd2 - ad) <= 1e-8 * abs(ad) + 4 * u * J / e2
+
+
+def test_autograd_harness(nv):
+    import torch
+    N, L = (16, 16, 16), (TWO_PI,) * 3
+    G, P, U0 = flow(nv, N, L, seed=7)
+    S = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-2)
+    A = nv.Adjoint(S, 2, 8)
+    X = U0.clone().requires_grad_(True)
+    J = nv.EnstrophyAfterSteps.apply(X, A, 0.01, 5)
+    J.backward()
+    U = A.forward(0.01, 5, U0)
+    J2, gT = nv.enstrophy(G, U, grad=True)
+    g0 = A.backward(gT)
+    assert abs(J.item() - J2) <= 1e-14 * abs(J2)
+    assert torch.equal(X.grad, g0)
+
+
+def test_adjoint_cost_ratio(nv):
+    """Adjoint sweep (segment recompute + reverse stages) against the plain
+    forward solve of the same steps, at 64^3 (timing only, reported)."""
+    import time
+    import torch
+    N, L = (64, 64, 64), (TWO_PI,) * 3
+    G, P, U0 = flow(nv, N, L, seed=9)
+    S = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-2)
+    A = nv.Adjoint(S, 5, 20)
+    U = U0.clone()
+    p = G.field(1)
+    S.step(0.01, 2, U, p)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S.step(0.01, 20, U, p)
+    torch.cuda.synchronize()
+    tf = time.perf_counter() - t0
+    UT = A.forward(0.01, 20, U0)
+    _, g = nv.enstrophy(G, UT, grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    A.backward(g)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    print(f"adjoint/forward time ratio at 64^3: {tb / tf:.2f}", flush=True)
+    assert tb / tf < 10.0
