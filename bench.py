@@ -118,6 +118,7 @@ def adjoint_ratio(nv, steps: int, every: int) -> dict:
     P.project(U0, 1.0)
     A = nv.Adjoint(nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-3), every, steps)
     dt = 0.02
+    tuned = chosen(S.autotune(U0, reps=3))
     U, p = U0.clone(), G.field(1)
     S.step(dt, 2, U, p)
     torch.cuda.synchronize()
@@ -127,6 +128,7 @@ def adjoint_ratio(nv, steps: int, every: int) -> dict:
     torch.cuda.synchronize()
     tf = time.perf_counter() - t0
     UT = A.forward(dt, steps, U0)
+    tuned += chosen(A.autotune(reps=3))
     J, g = nv.enstrophy(G, UT, grad=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -138,7 +140,12 @@ def adjoint_ratio(nv, steps: int, every: int) -> dict:
     return {"ratio": round(tb / tf, 3), "forward_s": round(tf, 3), "adjoint_s": round(tb, 3),
             "workload": f"shear layer 256x384x256 FP64, free-slip y walls (DCT-I), Re=1000, dt=0.02, "
                         f"{steps} SSP-RK3 steps, checkpoints every {every}; gradient of final enstrophy",
-            "J": J, "grad_norm": gn}
+            "J": J, "grad_norm": gn, "tuning": tuned}
+
+
+def chosen(report: str) -> list:
+    """The 'chosen' lines of an autotuner report."""
+    return [ln for ln in report.splitlines() if "chosen" in ln]
 
 
 def main() -> int:
@@ -179,6 +186,8 @@ def main() -> int:
     S = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=NU, graph=not args.eager)
     U = tg_field(nv, G, n)
     p = G.field(1)
+    tuned = chosen(S.autotune(U, reps=3))
+    log(f"bench: autotuner: {tuned}")
     log(f"bench: TG {n}^3, warmup {W}, steps {K}, graph={not args.eager}")
     S.step(DT, W, U, p)
     torch.cuda.synchronize()
@@ -254,7 +263,7 @@ def main() -> int:
         "data": "synthetic (Taylor-Green initial condition)",
         "config": {"workload": f"TG {n}^3 periodic FP64 Re=1600, explicit 6th-order FD, SSP-RK3 + 1 FFT "
                                f"projection per step, dt=2e-3", "N": n, "scheme": "explicit-m6", "rk": "ssp3",
-                   "graph": not args.eager, "parallelism": "single-gpu"},
+                   "graph": not args.eager, "parallelism": "single-gpu", "tuning": tuned},
         "roofline": roofline,
         "step_roofline": step_roofline,
         "kernels": kernels,

@@ -91,15 +91,26 @@ const char *nova_last_error(void);
  * collect synchronises and writes "name launches total_ms" lines. */
 int nova_timing_enable(nova_ctx *ctx, int enable);
 int nova_timing_collect(nova_ctx *ctx, char *buf, size_t buflen);
-/* Launch-shape choice of the fused stage kernel (no reference
- * counterpart): variant 0 stages each (j,k) plane in LDS, variant 1 streams
- * rows without LDS, variant 2 is variant 0 register-limited to three waves
- * per SIMD, variants 3 and 4 are variant 2 with 32 x 8 and 16 x 16 (k x j)
- * tiles instead of 64 x 4; stage_ichunk = planes per streamed chunk (0 = auto);
- * proj_variant 0 fuses the divergence into the first FFT pass, 1 computes it
- * (and the gradient correction) with streamed kernels.  Results are
- * identical up to rounding; only speed differs. */
+/* Launch-shape choice of the fused stage kernels (no reference
+ * counterpart), for every stage model at once: variant 0 stages each (j,k)
+ * plane of a 64 x 4 (k x j) tile in LDS with no register limit, variant 1
+ * streams rows without LDS (forward models only; the others use variant 3),
+ * variants 2, 3 and 4 stage 64 x 4, 32 x 8 and 16 x 16 tiles with one-field
+ * models held to three waves per SIMD (default: 3); stage_ichunk = planes per
+ * streamed chunk (0 = auto); proj_variant 0 fuses the divergence into the
+ * first FFT pass, 1 computes it (and the gradient correction) with streamed
+ * kernels.  Results are identical up to rounding; only speed differs. */
 int nova_ctx_tune(nova_ctx *ctx, int stage_variant, int stage_ichunk, int proj_variant);
+/* Stage kernels that carry their own launch shape. */
+enum {
+    NOVA_KERNEL_NS_STAGE = 0,
+    NOVA_KERNEL_MHD_STAGE = 1,
+    NOVA_KERNEL_TLM_STAGE = 2,
+    NOVA_KERNEL_ADJ_STAGE = 3
+};
+/* Launch shape of one stage kernel (variants as for nova_ctx_tune). */
+int nova_ctx_tune_kernel(nova_ctx *ctx, int kernel, int stage_variant, int stage_ichunk);
+
 int nova_alloc(nova_ctx *ctx, int ncomp, double **dptr);
 int nova_free(nova_ctx *ctx, double *dptr);
 int nova_memcpy_h2d(nova_ctx *ctx, double *dst, const double *src, size_t count);
@@ -160,6 +171,13 @@ int nova_solver_step(nova_solver *sv, double dt, int nsteps, double *U, double *
 /* Record the step sequence as a HIP graph and replay it (1) or launch
  * kernels one by one (0). */
 int nova_solver_use_graph(nova_solver *sv, int enable);
+/* Autotuner (no reference counterpart): times every launch shape of the
+ * solver's stage kernel (variants x chunks of 0, 8, 16, 32, 64 planes) and
+ * both projection variants on U (and B), reps launches each after one
+ * untimed launch, and keeps the fastest.  U and B are not modified.  When
+ * report is non-null it receives one line per trial and the choices. */
+int nova_solver_autotune(nova_solver *sv, const double *U, const double *B, int reps, char *report,
+                         size_t report_len);
 
 /* ---- diagnostics (app/TG_Piosson/TG.cpp:148-166, src/Tools/Tools.h:19-36,
  * app/OT_Poisson/OT.cpp:134-176) ---- */
@@ -187,6 +205,9 @@ int nova_adjoint_forward(nova_adjoint *ad, double dt, int nsteps, const double *
 int nova_adjoint_backward(nova_adjoint *ad, const double *lam_final, double *lam0);
 /* dU_final = (dU_final/dU0) dU0 (tangent-linear model). */
 int nova_adjoint_tlm(nova_adjoint *ad, const double *dU0, double *dU_final);
+/* Autotuner for the adjoint and tangent-linear stage kernels, run on the
+ * first checkpoint of the last forward run (see nova_solver_autotune). */
+int nova_adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t report_len);
 /* out = sum_nodes w a.b over ncomp components. */
 int nova_weighted_dot(nova_ctx *ctx, const double *a, const double *b, int ncomp, double *out);
 /* J = dV sum_nodes w |curl U|^2 (central stencil curl); dJdU (may be NULL)

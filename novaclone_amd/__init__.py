@@ -26,6 +26,7 @@ NOVA_OK = 0
 BC_PERIODIC, BC_WALL = 0, 1
 RK_LITERAL4, RK_SSP3, RK_SHEAR4 = 0, 1, 2
 MODEL_NS, MODEL_MHD = 0, 1
+KERNEL_NS_STAGE, KERNEL_MHD_STAGE, KERNEL_TLM_STAGE, KERNEL_ADJ_STAGE = 0, 1, 2, 3
 # reference BC kinds for the padded-layout fill (src/BC/BCenums.h:8)
 PBC_PERIODIC, PBC_NEUMANN, PBC_DIRICHLET, PBC_EVEN, PBC_ODD = 0, 1, 2, 10, 11
 DIRS = {"north": 0, "south": 1, "east": 2, "west": 3, "top": 4, "bottom": 5}
@@ -63,6 +64,7 @@ _SIGS = {
     "nova_ctx_metric": [_P, _DP, _DP],
     "nova_timing_enable": [_P, _I],
     "nova_ctx_tune": [_P, _I, _I, _I],
+    "nova_ctx_tune_kernel": [_P, _I, _I, _I],
     "nova_timing_collect": [_P, C.c_char_p, C.c_size_t],
     "nova_alloc": [_P, _I, C.POINTER(_P)],
     "nova_free": [_P, _P],
@@ -86,6 +88,7 @@ _SIGS = {
     "nova_solver_destroy": [_P],
     "nova_solver_step": [_P, _D, _I, _P, _P, _P, _P],
     "nova_solver_use_graph": [_P, _I],
+    "nova_solver_autotune": [_P, _P, _P, _I, C.c_char_p, C.c_size_t],
     "nova_reduce_ke": [_P, _P, _DP],
     "nova_reduce_maxabs_div": [_P, _P, _DP],
     "nova_reduce_dissipation": [_P, _P, _D, _DP],
@@ -96,6 +99,7 @@ _SIGS = {
     "nova_adjoint_forward": [_P, _D, _I, _P, _P],
     "nova_adjoint_backward": [_P, _P, _P],
     "nova_adjoint_tlm": [_P, _P, _P],
+    "nova_adjoint_autotune": [_P, _I, C.c_char_p, C.c_size_t],
     "nova_weighted_dot": [_P, _P, _P, _I, _DP],
     "nova_enstrophy": [_P, _P, _DP, _P],
 }
@@ -188,11 +192,16 @@ class Mesh:
         s = torch.cuda.current_stream(self.device).cuda_stream
         _check(lib().nova_ctx_set_stream(self._h, C.c_void_p(s)), "nova_ctx_set_stream")
 
-    def tune(self, stage_variant: int = 0, stage_ichunk: int = 0, proj_variant: int = 1) -> None:
-        """Pick the stage-kernel variant, its i-chunk (0 = automatic) and the
-        projection variant."""
+    def tune(self, stage_variant: int = 3, stage_ichunk: int = 0, proj_variant: int = 1) -> None:
+        """Pick the stage-kernel variant of every stage model, its i-chunk
+        (0 = automatic) and the projection variant."""
         _check(lib().nova_ctx_tune(self._h, int(stage_variant), int(stage_ichunk), int(proj_variant)),
                "nova_ctx_tune")
+
+    def tune_kernel(self, kernel: int, stage_variant: int, stage_ichunk: int = 0) -> None:
+        """Launch shape of one stage kernel (KERNEL_NS_STAGE .. KERNEL_ADJ_STAGE)."""
+        _check(lib().nova_ctx_tune_kernel(self._h, int(kernel), int(stage_variant), int(stage_ichunk)),
+               "nova_ctx_tune_kernel")
 
     def timing(self, enable: bool) -> None:
         """Bracket every library kernel with HIP events (per-kernel times)."""
@@ -342,6 +351,14 @@ class Solver:
         _check(lib().nova_solver_step(self._h, float(dt), int(nsteps), _ptr(U), _ptr(B), _ptr(p), _ptr(pB)),
                "nova_solver_step")
 
+    def autotune(self, U, B=None, reps: int = 3) -> str:
+        """Time every stage launch shape and both projection variants on U
+        (and B), keep the fastest, and return the trial report."""
+        buf = C.create_string_buffer(1 << 16)
+        _check(lib().nova_solver_autotune(self._h, _ptr(U), _ptr(B), int(reps), buf, len(buf)),
+               "nova_solver_autotune")
+        return buf.value.decode()
+
     def close(self):
         if getattr(self, "_h", None):
             lib().nova_solver_destroy(self._h)
@@ -427,6 +444,13 @@ class Adjoint:
         out = torch.empty_like(d0)
         _check(lib().nova_adjoint_tlm(self._h, _ptr(d0.contiguous()), _ptr(out)), "nova_adjoint_tlm")
         return out
+
+    def autotune(self, reps: int = 3) -> str:
+        """Tune the adjoint and tangent-linear stage kernels on the first
+        checkpoint of the last forward run; returns the trial report."""
+        buf = C.create_string_buffer(1 << 16)
+        _check(lib().nova_adjoint_autotune(self._h, int(reps), buf, len(buf)), "nova_adjoint_autotune")
+        return buf.value.decode()
 
     def close(self):
         if getattr(self, "_h", None):

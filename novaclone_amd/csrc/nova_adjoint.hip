@@ -18,6 +18,7 @@
 // in the reverse sweep, storing the stage inputs (U, U1, U2) of its steps.
 #include "nova_internal.hpp"
 #include "nova_stencil.hpp"
+#include "nova_stream.hpp"
 
 #include <vector>
 
@@ -49,178 +50,6 @@ struct nova_adjoint {
 namespace nova {
 
 namespace {
-
-enum { LIN_TLM = 0, LIN_ADJ = 1 };
-
-struct LinArgs {
-    const double *X;    // forward stage input (stencil)
-    const double *V;    // tangent or adjoint field (stencil)
-    const double *V0;   // TLM: step-start tangent for the stage combination
-    double *out;
-    const double *Y;    // ADJ: pointwise term added to the output
-    double *acc;        // ADJ: accumulator, acc = acc_a*acc + acc_b*V
-    double beta, gamma, acc_a, acc_b;
-    StageArgs sa;
-    double nu;
-    int ichunk, nkt, njt, nic;
-};
-
-constexpr int kLK = 64, kLJ = 4;
-
-// (B(., X)^T m)_d for output component d: 1/2 sum_c sum_l a_l
-// [ -X_c (m_c^+ - m_c^-) + m_c (X_c^+ - X_c^-) - ((m_c X_c)^+ - (m_c X_c)^-) ]
-template <int H, class Acc>
-__device__ inline double b_second_adj(const Grid &g, const Acc &A, int d, double zi)
-{
-    double acc = 0.0;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const double x0 = A(0, c, 0, 0), m0 = A(1, c, 0, 0);
-#pragma unroll
-        for (int l = 0; l < H; l++) {
-            const double xp = A(0, c, d, l + 1), xm = A(0, c, d, -l - 1);
-            const double mp = A(1, c, d, l + 1), mm = A(1, c, d, -l - 1);
-            acc += g.a[d][l] * (-x0 * (mp - mm) + m0 * (xp - xm) - (mp * xp - mm * xm));
-        }
-    }
-    return 0.5 * (d == 2 ? acc * zi : acc);
-}
-
-template <int H, class Acc>
-__device__ inline double div_of(const Grid &g, const Acc &A, int f, double zi)
-{
-    double acc = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        double ad = 0.0;
-#pragma unroll
-        for (int l = 0; l < H; l++) ad += g.a[d][l] * (A(f, d, d, l + 1) - A(f, d, d, -l - 1));
-        acc += d == 2 ? ad * zi : ad;
-    }
-    return acc;
-}
-
-// One thread per (j,k) column streaming i, like the row variant of the
-// forward stage kernel, with the forward input X (field 0) and the
-// tangent/adjoint V (field 1) both queued.
-template <int H, bool WALLS, int MODE>
-__global__ void __launch_bounds__(kLK *kLJ) k_lin_rows(Grid g, LinArgs a)
-{
-    const int T = a.nkt * a.njt * a.nic;
-    int t = blockIdx.x;
-    if (T % 8 == 0) t = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
-    const int kt = t % a.nkt;
-    const int jt = (t / a.nkt) % a.njt;
-    const int ic = t / (a.nkt * a.njt);
-    const int k = kt * kLK + threadIdx.x, j = jt * kLJ + threadIdx.y;
-    if (k >= g.s[2] || j >= g.s[1]) return;
-    const int i0 = ic * a.ichunk;
-    int i1 = i0 + a.ichunk;
-    if (i1 > g.s[0]) i1 = g.s[0];
-    const int64_t plane = (int64_t)g.s[1] * g.s[2];
-    const int s2 = g.s[2];
-    const int col = j * s2 + k;
-    const double *F[2] = {a.X, a.V};
-
-    double q[2][3][2 * H + 1];
-#pragma unroll
-    for (int o = 0; o < 2 * H + 1; o++) {
-        int fi;
-        const int64_t po = (int64_t)wrap<WALLS>(i0 - H + o, g.n[0], g.wall[0], fi) * plane;
-#pragma unroll
-        for (int f = 0; f < 2; f++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) q[f][c][o] = rsign<WALLS>(c, 0, fi) * F[f][c * g.np + po + col];
-    }
-    const double zi = g.zinv[k], zl = g.zlap[k], zm = g.zmet[k];
-    for (int i = i0; i < i1; i++) {
-        double nq[2][3];
-        if (i + 1 < i1) {
-            int fi;
-            const int64_t po = (int64_t)wrap<WALLS>(i + H + 1, g.n[0], g.wall[0], fi) * plane;
-#pragma unroll
-            for (int f = 0; f < 2; f++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) nq[f][c] = rsign<WALLS>(c, 0, fi) * F[f][c * g.np + po + col];
-        }
-        const int64_t pi = (int64_t)i * plane;
-        auto A = [&](int f, int c, int d, int o) -> double {
-            if (d == 0 || o == 0) return q[f][c][H + (d == 0 ? o : 0)];
-            int fl;
-            int64_t off;
-            if (d == 1) off = pi + (int64_t)wrap<WALLS>(j + o, g.n[1], g.wall[1], fl) * s2 + k;
-            else off = pi + (int64_t)j * s2 + wrap<WALLS>(k + o, g.n[2], g.wall[2], fl);
-            return rsign<WALLS>(c, d, fl) * F[f][c * g.np + off];
-        };
-        const int64_t qi = pi + col;
-        if (MODE == LIN_TLM) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const double r = (-div_xy<H>(g, A, 1, 0, c, zi)) - div_xy<H>(g, A, 0, 1, c, zi) +
-                                 lap_c<H>(g, A, 1, c, a.nu, zl, zm);
-                const double v = q[1][c][H];
-                const double v0 = a.V0 ? a.V0[c * g.np + qi] : v;
-                a.out[c * g.np + qi] = combine(a.sa, v, v0, a.sa.dt * r);
-            }
-        } else {
-            const double divx = div_of<H>(g, A, 0, zi);
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const double m = q[1][c][H];
-                const double jr = div_xy<H>(g, A, 0, 1, c, zi) - m * divx - b_second_adj<H>(g, A, c, zi) +
-                                  lap_c<H>(g, A, 1, c, a.nu, zl, 0.0);
-                const double tv = m + a.sa.dt * jr;
-                double o = a.beta * tv;
-                if (a.Y) o += a.gamma * a.Y[c * g.np + qi];
-                if (a.acc) a.acc[c * g.np + qi] = (a.acc_a != 0.0 ? a.acc_a * a.acc[c * g.np + qi] : 0.0) + a.acc_b * m;
-                a.out[c * g.np + qi] = o;
-            }
-        }
-        if (i + 1 < i1) {
-#pragma unroll
-            for (int f = 0; f < 2; f++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-#pragma unroll
-                    for (int o = 0; o < 2 * H; o++) q[f][c][o] = q[f][c][o + 1];
-                    q[f][c][2 * H] = nq[f][c];
-                }
-        }
-    }
-}
-
-template <int H, bool WALLS, int MODE>
-void launch_lin_hw(nova_ctx *ctx, LinArgs a)
-{
-    const Grid &g = ctx->g;
-    a.nkt = (g.s[2] + kLK - 1) / kLK;
-    a.njt = (g.s[1] + kLJ - 1) / kLJ;
-    int64_t per = (int64_t)g.s[0] * a.nkt * a.njt / 4096;
-    a.ichunk = (int)(per < 8 ? 8 : (per > 32 ? 32 : per));
-    if (a.ichunk > g.s[0]) a.ichunk = g.s[0];
-    a.nic = (g.s[0] + a.ichunk - 1) / a.ichunk;
-    const int T = a.nkt * a.njt * a.nic;
-    hipLaunchKernelGGL((k_lin_rows<H, WALLS, MODE>), dim3(T), dim3(kLK, kLJ, 1), 0, ctx->stream, g, a);
-}
-
-template <int MODE>
-int launch_lin(nova_ctx *ctx, const LinArgs &a, const char *name)
-{
-    const Grid &g = ctx->g;
-    const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
-    TimedScope ts(ctx, name);
-    switch (g.h * 2 + (walls ? 1 : 0)) {
-    case 2: launch_lin_hw<1, false, MODE>(ctx, a); break;
-    case 3: launch_lin_hw<1, true, MODE>(ctx, a); break;
-    case 4: launch_lin_hw<2, false, MODE>(ctx, a); break;
-    case 5: launch_lin_hw<2, true, MODE>(ctx, a); break;
-    case 6: launch_lin_hw<3, false, MODE>(ctx, a); break;
-    case 7: launch_lin_hw<3, true, MODE>(ctx, a); break;
-    case 8: launch_lin_hw<4, false, MODE>(ctx, a); break;
-    default: launch_lin_hw<4, true, MODE>(ctx, a); break;
-    }
-    return check_launch(name);
-}
 
 StageArgs ssp_args(int s, double dt)
 {
@@ -279,9 +108,13 @@ int recompute_segment(nova_adjoint *ad, int s, int *count)
     return NOVA_OK;
 }
 
-LinArgs lin_args(nova_adjoint *ad)
+using LinArgs = StreamArgs<2>;
+
+LinArgs lin_args(nova_adjoint *ad, const double *X, const double *V)
 {
     LinArgs a{};
+    a.X[0] = X;
+    a.X[1] = V;
     a.nu = ad->nu;
     a.sa = ssp_args(0, ad->dt);
     return a;
@@ -375,18 +208,53 @@ int adjoint_backward(nova_adjoint *ad, const double *lamT, double *lam0)
         for (int q = cnt - 1; q >= 0; q--) {
             double *U = ad->seg[3 * q], *U1 = ad->seg[3 * q + 1], *U2 = ad->seg[3 * q + 2];
             if ((rc = project(ad->ps, L, 1.0, ad->ptmp))) return rc;  // m3 = P m
-            LinArgs a = lin_args(ad);
-            a.X = U2; a.V = L; a.out = A; a.beta = 2.0 / 3.0; a.acc = ACC; a.acc_a = 0.0; a.acc_b = 1.0 / 3.0;
-            if ((rc = launch_lin<LIN_ADJ>(ctx, a, "adj_stage"))) return rc;
-            a = lin_args(ad);
-            a.X = U1; a.V = A; a.out = B; a.beta = 0.25; a.acc = ACC; a.acc_a = 1.0; a.acc_b = 0.75;
-            if ((rc = launch_lin<LIN_ADJ>(ctx, a, "adj_stage"))) return rc;
-            a = lin_args(ad);
-            a.X = U; a.V = B; a.out = L; a.beta = 1.0; a.Y = ACC; a.gamma = 1.0;
-            if ((rc = launch_lin<LIN_ADJ>(ctx, a, "adj_stage"))) return rc;
+            LinArgs a = lin_args(ad, U2, L);
+            a.out[0] = A; a.ob = 2.0 / 3.0; a.acc = ACC; a.acc_a = 0.0; a.acc_b = 1.0 / 3.0;
+            if ((rc = launch_stream_model<MODEL_ADJ>(ctx, a, "adj_stage"))) return rc;
+            a = lin_args(ad, U1, A);
+            a.out[0] = B; a.ob = 0.25; a.acc = ACC; a.acc_a = 1.0; a.acc_b = 0.75;
+            if ((rc = launch_stream_model<MODEL_ADJ>(ctx, a, "adj_stage"))) return rc;
+            a = lin_args(ad, U, B);
+            a.out[0] = L; a.ob = 1.0; a.Y = ACC; a.yg = 1.0;
+            if ((rc = launch_stream_model<MODEL_ADJ>(ctx, a, "adj_stage"))) return rc;
         }
     }
     return NOVA_OK;
+}
+
+// Measured choice of the launch shapes of the adjoint and tangent-linear
+// stage kernels, on the first checkpoint of the last forward run.
+int adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t len)
+{
+    if (ad->nsteps < 1) return fail(NOVA_ERR_STATE, "run nova_adjoint_forward first");
+    if (reps < 1) return fail(NOVA_ERR_ARG, "reps must be >= 1");
+    nova_ctx *ctx = ad->ctx;
+    std::string rep;
+    const int variants[] = {0, 2, 3, 4};
+    // the forward state and a distinct second field (a copy), so that the
+    // trials move the same bytes as the reverse sweep
+    const double *X = ad->ckpt[0];
+    double *V = ad->w[2];
+    if (int rc = copy3(ctx, V, X)) return rc;
+    if (int rc = copy3(ctx, ad->w[3], X)) return rc;
+    auto adj = [&]() -> int {
+        LinArgs a = lin_args(ad, X, V);
+        a.out[0] = ad->w[0];
+        a.ob = 1.0;
+        a.acc = ad->w[1];
+        a.acc_b = 1.0;
+        return launch_stream_model<MODEL_ADJ>(ctx, a, "adj_stage");
+    };
+    if (int rc = tune_stage_model(ctx, MODEL_ADJ, variants, 4, reps, adj, "adj_stage", rep)) return rc;
+    auto tlm = [&]() -> int {
+        LinArgs a = lin_args(ad, X, V);
+        a.out[0] = ad->w[0];
+        a.X0[1] = ad->w[3];
+        a.sa = ssp_args(1, ad->dt);
+        return launch_stream_model<MODEL_TLM>(ctx, a, "tlm_stage");
+    };
+    if (int rc = tune_stage_model(ctx, MODEL_TLM, variants, 4, reps, tlm, "tlm_stage", rep)) return rc;
+    return write_report(rep, report, len);
 }
 
 int adjoint_tlm(nova_adjoint *ad, const double *d0, double *dT)
@@ -402,13 +270,15 @@ int adjoint_tlm(nova_adjoint *ad, const double *d0, double *dT)
         if ((rc = recompute_segment(ad, s, &cnt))) return rc;
         for (int q = 0; q < cnt; q++) {
             double *U = ad->seg[3 * q], *U1 = ad->seg[3 * q + 1], *U2 = ad->seg[3 * q + 2];
-            LinArgs a = lin_args(ad);
-            a.X = U; a.V = D; a.V0 = nullptr; a.out = D1; a.sa = ssp_args(0, ad->dt);
-            if ((rc = launch_lin<LIN_TLM>(ctx, a, "tlm_stage"))) return rc;
-            a.X = U1; a.V = D1; a.V0 = D; a.out = D2; a.sa = ssp_args(1, ad->dt);
-            if ((rc = launch_lin<LIN_TLM>(ctx, a, "tlm_stage"))) return rc;
-            a.X = U2; a.V = D2; a.V0 = D; a.out = D; a.sa = ssp_args(2, ad->dt);
-            if ((rc = launch_lin<LIN_TLM>(ctx, a, "tlm_stage"))) return rc;
+            LinArgs a = lin_args(ad, U, D);
+            a.out[0] = D1; a.sa = ssp_args(0, ad->dt);
+            if ((rc = launch_stream_model<MODEL_TLM>(ctx, a, "tlm_stage"))) return rc;
+            a = lin_args(ad, U1, D1);
+            a.X0[1] = D; a.out[0] = D2; a.sa = ssp_args(1, ad->dt);
+            if ((rc = launch_stream_model<MODEL_TLM>(ctx, a, "tlm_stage"))) return rc;
+            a = lin_args(ad, U2, D2);
+            a.X0[1] = D; a.out[0] = D; a.sa = ssp_args(2, ad->dt);
+            if ((rc = launch_stream_model<MODEL_TLM>(ctx, a, "tlm_stage"))) return rc;
             if ((rc = project(ad->ps, D, ad->dt, ad->ptmp))) return rc;
         }
     }

@@ -36,6 +36,14 @@ int hip_fail(hipError_t e, const char *what)
     return NOVA_ERR_HIP;
 }
 
+int write_report(const std::string &rep, char *buf, size_t len)
+{
+    if (!buf) return NOVA_OK;
+    if (rep.size() + 1 > len) return fail(NOVA_ERR_ARG, "report buffer too small");
+    std::memcpy(buf, rep.c_str(), rep.size() + 1);
+    return NOVA_OK;
+}
+
 int check_launch(const char *what)
 {
     hipError_t e = hipGetLastError();
@@ -90,12 +98,14 @@ int solver_create(nova_poisson *ps, int model, int mode, double nu, double eta, 
 int solver_destroy(nova_solver *sv);
 int solver_step(nova_solver *sv, double dt, int nsteps, double *U, double *B, double *p, double *pB);
 int solver_use_graph(nova_solver *sv, int enable);
+int solver_autotune(nova_solver *sv, const double *U, const double *B, int reps, char *report, size_t len);
 nova_ctx *poisson_ctx(nova_poisson *ps);
 int adjoint_create(nova_solver *sv, int every, int max_steps, nova_adjoint **out);
 int adjoint_destroy(nova_adjoint *ad);
 int adjoint_forward(nova_adjoint *ad, double dt, int nsteps, const double *U0, double *U);
 int adjoint_backward(nova_adjoint *ad, const double *lamT, double *lam0);
 int adjoint_tlm(nova_adjoint *ad, const double *d0, double *dT);
+int adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t len);
 
 // Mesh::z (src/Mesh/Mesh.cpp:373-407) for padded index kp on one rank.
 static double mesh_z(const nova_grid_desc &d, double dz, double hb, int m, int kp)
@@ -122,6 +132,9 @@ static int ctx_init(nova_ctx *ctx, const nova_grid_desc *desc, int device)
         if (d.bc[a] != NOVA_BC_PERIODIC && d.bc[a] != NOVA_BC_WALL) return fail(NOVA_ERR_ARG, "bad bc kind");
     }
     if (d.hyp && !(d.s1 > 0.0 && d.s2 > 0.0)) return fail(NOVA_ERR_ARG, "stretching needs s1, s2 > 0");
+    // stage kernels address within an (j, k) plane by 32-bit byte offsets
+    if ((int64_t)(d.n[1] + 1) * (d.n[2] + 1) >= ((int64_t)1 << 29))
+        return fail(NOVA_ERR_UNSUPPORTED, "a (j, k) plane must hold fewer than 2^29 nodes");
     ctx->device = device;
     ctx->desc = d;
     ctx->m = d.m;
@@ -335,9 +348,23 @@ int nova_ctx_tune(nova_ctx *ctx, int stage_variant, int stage_ichunk, int proj_v
     NOVA_REQUIRE(ctx && stage_variant >= 0 && stage_variant <= 4 && stage_ichunk >= 0 && proj_variant >= 0 &&
                      proj_variant <= 1,
                  "bad argument");
-    ctx->stage_variant = stage_variant;
-    ctx->stage_ichunk = stage_ichunk;
+    for (int k = 0; k < 4; k++) {
+        ctx->stage_variant[k] = stage_variant;
+        ctx->stage_ichunk[k] = stage_ichunk;
+    }
     ctx->proj_variant = proj_variant;
+    ctx->tune_epoch++;
+    return NOVA_OK;
+}
+
+int nova_ctx_tune_kernel(nova_ctx *ctx, int kernel, int stage_variant, int stage_ichunk)
+{
+    NOVA_REQUIRE(ctx && kernel >= NOVA_KERNEL_NS_STAGE && kernel <= NOVA_KERNEL_ADJ_STAGE && stage_variant >= 0 &&
+                     stage_variant <= 4 && stage_ichunk >= 0,
+                 "bad argument");
+    ctx->stage_variant[kernel] = stage_variant;
+    ctx->stage_ichunk[kernel] = stage_ichunk;
+    ctx->tune_epoch++;
     return NOVA_OK;
 }
 
@@ -477,6 +504,13 @@ int nova_solver_use_graph(nova_solver *sv, int enable)
     NOVA_GUARD(solver_use_graph(sv, enable));
 }
 
+int nova_solver_autotune(nova_solver *sv, const double *U, const double *B, int reps, char *report,
+                         size_t report_len)
+{
+    NOVA_REQUIRE(sv && U && reps >= 1 && (!report || report_len > 0), "bad argument");
+    NOVA_GUARD(solver_autotune(sv, U, B, reps, report, report_len));
+}
+
 int nova_reduce_ke(nova_ctx *ctx, const double *U, double *out)
 {
     NOVA_REQUIRE(ctx && U && out, "bad argument");
@@ -537,6 +571,12 @@ int nova_adjoint_tlm(nova_adjoint *ad, const double *dU0, double *dU_final)
 {
     NOVA_REQUIRE(ad && dU0 && dU_final, "bad argument");
     NOVA_GUARD(adjoint_tlm(ad, dU0, dU_final));
+}
+
+int nova_adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t report_len)
+{
+    NOVA_REQUIRE(ad && reps >= 1 && (!report || report_len > 0), "bad argument");
+    NOVA_GUARD(adjoint_autotune(ad, reps, report, report_len));
 }
 
 int nova_weighted_dot(nova_ctx *ctx, const double *a, const double *b, int ncomp, double *out)

@@ -82,8 +82,11 @@ struct TimedLaunch {
 struct nova_ctx {
     int device = 0;
     int timing = 0;                        // record events around launches
-    int stage_variant = 2;                 // 0: LDS-staged planes, 1: row streams without LDS, 2: 0 at 3 waves/SIMD
-    int stage_ichunk = 0;                  // planes per streamed chunk (0: automatic)
+    // launch shape of the stage kernels per model (NS, MHD, tangent-linear,
+    // adjoint): variant 0..4 and planes per streamed chunk (0: automatic)
+    int stage_variant[4] = {3, 3, 3, 3};
+    int stage_ichunk[4] = {0, 0, 0, 0};
+    int tune_epoch = 0;                    // bumped on every change of the launch shapes
     int proj_variant = 1;                  // 0: divergence fused into the r2c pass, 1: streamed kernels
     std::vector<nova::TimedLaunch> tlog;   // launches since the last collect
     std::vector<hipEvent_t> evpool;        // reusable events
@@ -127,6 +130,67 @@ private:
     hipEvent_t start_ = nullptr;
 };
 int ensure_red(nova_ctx *ctx, size_t n);
+
+// Average duration in microseconds of reps calls of f() on the context
+// stream, measured with events after one untimed call.
+template <class F>
+int time_avg_us(nova_ctx *ctx, int reps, F &&f, double *us)
+{
+    if (int rc = f()) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    int rc = NOVA_OK;
+    if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
+    for (int r = 0; r < reps && rc == NOVA_OK && e == hipSuccess; r++) rc = f();
+    if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "tuning events");
+    *us = 1e3 * ms / reps;
+    return NOVA_OK;
+}
+
+// Measured search over the stage-kernel launch shapes of one model:
+// run(v, c) launches the stage with variant v and chunk c already set in
+// the context; the fastest pair is kept and every trial is appended to rep.
+template <class F>
+int tune_stage_model(nova_ctx *ctx, int model, const int *variants, int nvar, int reps, F &&run, const char *name,
+                     std::string &rep)
+{
+    static const int kChunks[] = {0, 8, 16, 32, 64};
+    double best = 1e300;
+    int bv = ctx->stage_variant[model], bc = ctx->stage_ichunk[model];
+    for (int iv = 0; iv < nvar; iv++)
+        for (int c : kChunks) {
+            ctx->stage_variant[model] = variants[iv];
+            ctx->stage_ichunk[model] = c;
+            double us = 0.0;
+            if (int rc = time_avg_us(ctx, reps, run, &us)) {
+                ctx->stage_variant[model] = bv;
+                ctx->stage_ichunk[model] = bc;
+                return rc;
+            }
+            rep += std::string(name) + " variant " + std::to_string(variants[iv]) + " chunk " + std::to_string(c) +
+                   " " + std::to_string(us) + " us\n";
+            if (us < best) {
+                best = us;
+                bv = variants[iv];
+                bc = c;
+            }
+        }
+    ctx->stage_variant[model] = bv;
+    ctx->stage_ichunk[model] = bc;
+    ctx->tune_epoch++;
+    rep += std::string(name) + " chosen variant " + std::to_string(bv) + " chunk " + std::to_string(bc) + "\n";
+    return NOVA_OK;
+}
+
+int write_report(const std::string &rep, char *buf, size_t len);
 
 // Kernel launchers (defined in the .hip files).
 int launch_div_adv(nova_ctx *ctx, const double *U, const double *phi, int nc, double *out);

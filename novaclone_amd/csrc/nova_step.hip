@@ -9,6 +9,7 @@
 // evaluated per node and written once.
 #include "nova_internal.hpp"
 #include "nova_stencil.hpp"
+#include "nova_stream.hpp"
 
 struct nova_poisson;
 
@@ -30,6 +31,7 @@ struct nova_solver {
     hipStream_t gstream = nullptr;
     hipGraphExec_t gexec = nullptr;
     double gdt = 0.0;
+    int gepoch = 0;  // launch-shape epoch of the context when recorded
     double *gU = nullptr, *gB = nullptr, *gp = nullptr, *gpB = nullptr;
 };
 
@@ -37,225 +39,6 @@ namespace nova {
 
 namespace {
 
-// ---- the streamed stage kernel ----
-//
-// A 64 x 4 block owns a (k, j) tile and walks i over a chunk of planes.
-// The i-neighbours of its own column live in a register queue q (planes
-// i-H .. i+H); the current plane, with its j and k halos, is staged in
-// LDS.  Every input value is loaded from global memory once per block
-// (plus the halo ring and the 2H warm-up planes of the chunk), and the
-// loads for plane i+1 are issued before plane i is computed.
-constexpr int kTK = 64, kTJ = 4;
-
-template <int NF>
-struct StreamArgs {
-    const double *X[NF];   // stage inputs (stencil)
-    const double *X0[NF];  // step-start fields for the stage combination (may be null)
-    double *out[NF];
-    StageArgs sa;
-    double nu, eta;
-    int ichunk;
-    int nkt, njt, nic;
-};
-
-template <int H, int NF, bool WALLS, int MINW, int kTK>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MINW))) k_stage_stream(Grid g, StreamArgs<NF> a)
-{
-    constexpr int kTJ = 256 / kTK;
-    constexpr int NC = 3 * NF;
-    constexpr int LK = kTK + 2 * H, LJ = kTJ + 2 * H;
-    constexpr int NHALO = 2 * H * (kTK + kTJ);
-    constexpr int NSLOT = (NHALO + kTK * kTJ - 1) / (kTK * kTJ);
-    __shared__ double sh[NC][LJ][LK];
-
-    // tile decode; consecutive tiles share halos, so each XCD (blocks are
-    // dealt round-robin over 8) takes a contiguous run of tiles
-    const int T = a.nkt * a.njt * a.nic;
-    int t = blockIdx.x;
-    if (T % 8 == 0) t = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
-    const int kt = t % a.nkt;
-    const int jt = (t / a.nkt) % a.njt;
-    const int ic = t / (a.nkt * a.njt);
-    const int tk = threadIdx.x, tj = threadIdx.y;
-    const int tid = tj * kTK + tk;
-    const int k = kt * kTK + tk, j = jt * kTJ + tj;
-    const bool valid = k < g.s[2] && j < g.s[1];
-    const int i0 = ic * a.ichunk;
-    int i1 = i0 + a.ichunk;
-    if (i1 > g.s[0]) i1 = g.s[0];
-    const int64_t plane = (int64_t)g.s[1] * g.s[2];
-
-    // own column
-    int fj, fk;
-    const int jm = bmap(j, g.n[1], g.wall[1], fj);
-    const int km = bmap(k, g.n[2], g.wall[2], fk);
-    const int64_t col = (int64_t)jm * g.s[2] + km;
-    double sgn_col[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) sgn_col[c] = rsign<WALLS>(c, 1, fj) * rsign<WALLS>(c, 2, fk);
-
-    // halo slots of this thread: LDS position, source column, signs
-    int hrow[NSLOT], hcolk[NSLOT];
-    int64_t hsrc[NSLOT];
-    double hsg[NSLOT][3];
-    bool hon[NSLOT];
-#pragma unroll
-    for (int s = 0; s < NSLOT; s++) {
-        const int e = tid + s * kTK * kTJ;
-        hon[s] = e < NHALO;
-        int r = 0, cc = 0, jj = 0, kk = 0;
-        if (e < 2 * H * kTK) {  // j halo rows
-            const int rr = e / kTK;
-            cc = e % kTK + H;
-            r = rr < H ? rr : kTJ + rr;
-            jj = jt * kTJ + (rr < H ? rr - H : kTJ + rr - H);
-            kk = kt * kTK + (cc - H);
-        } else if (e < NHALO) {  // k halo columns
-            const int e2 = e - 2 * H * kTK;
-            const int rr = e2 / (2 * H), c2 = e2 % (2 * H);
-            r = rr + H;
-            cc = c2 < H ? c2 : kTK + c2;
-            jj = jt * kTJ + rr;
-            kk = kt * kTK + (c2 < H ? c2 - H : kTK + c2 - H);
-        }
-        int f1, f2;
-        const int jm2 = bmap(jj, g.n[1], g.wall[1], f1);
-        const int km2 = bmap(kk, g.n[2], g.wall[2], f2);
-        hrow[s] = r;
-        hcolk[s] = cc;
-        hsrc[s] = (int64_t)jm2 * g.s[2] + km2;
-#pragma unroll
-        for (int c = 0; c < 3; c++) hsg[s][c] = rsign<WALLS>(c, 1, f1) * rsign<WALLS>(c, 2, f2);
-    }
-
-    // loaders
-    auto plane_off = [&](int ii, int &fi) -> int64_t {
-        const int im = bmap(ii, g.n[0], g.wall[0], fi);
-        return (int64_t)im * plane;
-    };
-
-    double q[NF][3][2 * H + 1];
-#pragma unroll
-    for (int o = 0; o < 2 * H + 1; o++) {
-        int fi;
-        const int64_t po = plane_off(i0 - H + o, fi);
-#pragma unroll
-        for (int f = 0; f < NF; f++)
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-                q[f][c][o] = rsign<WALLS>(c, 0, fi) * sgn_col[c] * a.X[f][c * g.np + po + col];
-    }
-    double hv[NSLOT][NC];
-    {
-        int fi;
-        const int64_t po = plane_off(i0, fi);
-#pragma unroll
-        for (int s = 0; s < NSLOT; s++)
-#pragma unroll
-            for (int f = 0; f < NF; f++)
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-                    hv[s][f * 3 + c] =
-                        hon[s] ? rsign<WALLS>(c, 0, fi) * hsg[s][c] * a.X[f][c * g.np + po + hsrc[s]] : 0.0;
-    }
-
-    const double zi = g.zinv[km], zl = g.zlap[km], zm = g.zmet[km];
-    // step-start values of the own column, one plane ahead like the queue
-    double x0c[NF][3];
-#pragma unroll
-    for (int f = 0; f < NF; f++)
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-            x0c[f][c] = (a.X0[f] && valid) ? a.X0[f][c * g.np + (int64_t)i0 * plane + col] : 0.0;
-
-    for (int i = i0; i < i1; i++) {
-#pragma unroll
-        for (int f = 0; f < NF; f++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) sh[f * 3 + c][tj + H][tk + H] = q[f][c][H];
-#pragma unroll
-        for (int s = 0; s < NSLOT; s++)
-            if (hon[s])
-#pragma unroll
-                for (int n = 0; n < NC; n++) sh[n][hrow[s]][hcolk[s]] = hv[s][n];
-        __syncthreads();
-
-        // prefetch plane i+H+1 for the queue, the halo of plane i+1 and the
-        // step-start values of plane i+1
-        double nq[NF][3], nx0[NF][3];
-        if (i + 1 < i1) {
-#pragma unroll
-            for (int f = 0; f < NF; f++)
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-                    nx0[f][c] = (a.X0[f] && valid) ? a.X0[f][c * g.np + (int64_t)(i + 1) * plane + col] : 0.0;
-            int fi;
-            const int64_t po = plane_off(i + H + 1, fi);
-#pragma unroll
-            for (int f = 0; f < NF; f++)
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-                    nq[f][c] = rsign<WALLS>(c, 0, fi) * sgn_col[c] * a.X[f][c * g.np + po + col];
-            int fh;
-            const int64_t ph = plane_off(i + 1, fh);
-#pragma unroll
-            for (int s = 0; s < NSLOT; s++)
-#pragma unroll
-                for (int f = 0; f < NF; f++)
-#pragma unroll
-                    for (int c = 0; c < 3; c++)
-                        hv[s][f * 3 + c] =
-                            hon[s] ? rsign<WALLS>(c, 0, fh) * hsg[s][c] * a.X[f][c * g.np + ph + hsrc[s]] : 0.0;
-        }
-
-        if (valid) {
-            auto A = [&](int f, int c, int d, int o) -> double {
-                if (d == 0) return q[f][c][H + o];
-                if (d == 1) return sh[f * 3 + c][tj + H + o][tk + H];
-                return sh[f * 3 + c][tj + H][tk + H + o];
-            };
-            const int64_t qi = (int64_t)i * plane + col;
-            const StageArgs &sa = a.sa;
-            if (NF == 1) {
-                double r[3];
-                ns_rhs<H>(g, A, a.nu, zi, zl, zm, r);
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    const double du = sa.dt * r[c];
-                    const double u = q[0][c][H];
-                    const double u0 = a.X0[0] ? x0c[0][c] : u;
-                    a.out[0][c * g.np + qi] = combine(sa, u, u0, du);
-                }
-            } else {
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    const double du =
-                        sa.dt * ((((-div_xy<H>(g, A, 0, 0, c, zi)) + lap_c<H>(g, A, 0, c, a.nu, zl, zm)) +
-                                  div_xy<H>(g, A, 1, 1, c, zi)) - grad_pm<H>(g, A, 1, c, zi));
-                    const double db = sa.dt * (((-div_xy<H>(g, A, 0, 1, c, zi)) + div_xy<H>(g, A, 1, 0, c, zi)) +
-                                               lap_c<H>(g, A, 1, c, a.eta, zl, zm));
-                    const double u = q[0][c][H], b = q[NF - 1][c][H];
-                    const double u0 = a.X0[0] ? x0c[0][c] : u;
-                    const double b0 = a.X0[NF - 1] ? x0c[NF - 1][c] : b;
-                    a.out[0][c * g.np + qi] = combine(sa, u, u0, du);
-                    a.out[NF - 1][c * g.np + qi] = combine(sa, b, b0, db);
-                }
-            }
-        }
-        __syncthreads();
-        if (i + 1 < i1) {
-#pragma unroll
-            for (int f = 0; f < NF; f++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-#pragma unroll
-                    for (int o = 0; o < 2 * H; o++) q[f][c][o] = q[f][c][o + 1];
-                    q[f][c][2 * H] = nq[f][c];
-                    x0c[f][c] = nx0[f][c];
-                }
-        }
-    }
-}
 
 template <int H, int NF, bool WALLS>
 __global__ void __launch_bounds__(kTK *kTJ) k_stage_rows(Grid g, StreamArgs<NF> a)
@@ -349,39 +132,6 @@ __global__ void __launch_bounds__(kTK *kTJ) k_stage_rows(Grid g, StreamArgs<NF> 
     }
 }
 
-int pick_ichunk(const Grid &g, int nkt, int njt)
-{
-    // about 2048 tiles (8 blocks per CU on 256 CUs), chunks of 8..64 planes
-    int64_t per = (int64_t)g.s[0] * nkt * njt / 2048;
-    int c = (int)(per < 8 ? 8 : (per > 64 ? 64 : per));
-    if (c > g.s[0]) c = g.s[0];
-    return c;
-}
-
-template <int H, int NF, bool WALLS, int MINW, int TK>
-void launch_stream_tile(nova_ctx *ctx, StreamArgs<NF> a)
-{
-    const Grid &g = ctx->g;
-    constexpr int TJ = 256 / TK;
-    a.nkt = (g.s[2] + TK - 1) / TK;
-    a.njt = (g.s[1] + TJ - 1) / TJ;
-    a.ichunk = ctx->stage_ichunk > 0 ? (ctx->stage_ichunk < g.s[0] ? ctx->stage_ichunk : g.s[0])
-                                     : pick_ichunk(g, a.nkt, a.njt);
-    a.nic = (g.s[0] + a.ichunk - 1) / a.ichunk;
-    const int T = a.nkt * a.njt * a.nic;
-    hipLaunchKernelGGL((k_stage_stream<H, NF, WALLS, MINW, TK>), dim3(T), dim3(TK, TJ, 1), 0, ctx->stream, g, a);
-}
-
-template <int H, int NF, bool WALLS>
-void launch_stream_w(nova_ctx *ctx, const StreamArgs<NF> &a)
-{
-    switch (ctx->stage_variant) {
-    case 0: launch_stream_tile<H, NF, WALLS, 1, 64>(ctx, a); break;
-    case 3: launch_stream_tile<H, NF, WALLS, 3, 32>(ctx, a); break;
-    case 4: launch_stream_tile<H, NF, WALLS, 3, 16>(ctx, a); break;
-    default: launch_stream_tile<H, NF, WALLS, 3, 64>(ctx, a); break;
-    }
-}
 
 template <int H, int NF>
 int launch_stream_h(nova_ctx *ctx, StreamArgs<NF> a, const char *name)
@@ -389,11 +139,12 @@ int launch_stream_h(nova_ctx *ctx, StreamArgs<NF> a, const char *name)
     const Grid &g = ctx->g;
     const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
     TimedScope ts(ctx, name);
-    if (ctx->stage_variant == 1) {
+    constexpr int MODEL = NF == 1 ? MODEL_NS : MODEL_MHD;
+    if (ctx->stage_variant[MODEL] == 1) {
         a.nkt = (g.s[2] + kTK - 1) / kTK;
         a.njt = (g.s[1] + kTJ - 1) / kTJ;
-        a.ichunk = ctx->stage_ichunk > 0 ? (ctx->stage_ichunk < g.s[0] ? ctx->stage_ichunk : g.s[0])
-                                         : pick_ichunk(g, a.nkt, a.njt);
+        const int ic = ctx->stage_ichunk[MODEL];
+        a.ichunk = ic > 0 ? (ic < g.s[0] ? ic : g.s[0]) : pick_ichunk(g, a.nkt, a.njt);
         a.nic = (g.s[0] + a.ichunk - 1) / a.ichunk;
         const int T = a.nkt * a.njt * a.nic;
         if (walls)
@@ -401,9 +152,9 @@ int launch_stream_h(nova_ctx *ctx, StreamArgs<NF> a, const char *name)
         else
             hipLaunchKernelGGL((k_stage_rows<H, NF, false>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
     } else if (walls) {
-        launch_stream_w<H, NF, true>(ctx, a);
+        launch_stream_w<H, MODEL, true>(ctx, a);
     } else {
-        launch_stream_w<H, NF, false>(ctx, a);
+        launch_stream_w<H, MODEL, false>(ctx, a);
     }
     return check_launch(name);
 }
@@ -580,6 +331,47 @@ static int step_once(nova_solver *sv, double dt, double *U, double *B, double *p
     return sv->model == NOVA_MODEL_NS ? ns_step_once(sv, dt, U, p) : mhd_step_once(sv, dt, U, B, p, pB);
 }
 
+// Measured choice of the solver's stage-kernel launch shape and of the
+// projection variant, on the caller's fields (which are not modified).
+int solver_autotune(nova_solver *sv, const double *U, const double *B, int reps, char *report, size_t len)
+{
+    nova_ctx *ctx = sv->ctx;
+    const bool mhd = sv->model == NOVA_MODEL_MHD;
+    if (mhd && !B) return fail(NOVA_ERR_ARG, "MHD needs B");
+    if (reps < 1) return fail(NOVA_ERR_ARG, "reps must be >= 1");
+    std::string rep;
+    const StageArgs sa = ssp(0, 1e-3);
+    const int variants[] = {0, 1, 2, 3, 4};
+    auto run = [&]() -> int {
+        if (mhd) return mhd_stage(ctx, U, B, nullptr, nullptr, sv->wa, sv->wb, sa, sv->nu, sv->eta);
+        return ns_stage(ctx, U, nullptr, sv->wa, sa, sv->nu);
+    };
+    if (int rc = tune_stage_model(ctx, mhd ? MODEL_MHD : MODEL_NS, variants, 5, reps, run,
+                                  mhd ? "mhd_stage" : "ns_stage", rep))
+        return rc;
+    // projection variants, on a copy of U
+    NOVA_HIP(hipMemcpyAsync(sv->wa, U, sizeof(double) * 3 * ctx->g.np, hipMemcpyDeviceToDevice, ctx->stream));
+    double best = 1e300;
+    int bp = ctx->proj_variant;
+    for (int pv = 0; pv <= 1; pv++) {
+        ctx->proj_variant = pv;
+        double us = 0.0;
+        if (int rc = time_avg_us(ctx, reps, [&]() { return project(sv->ps, sv->wa, 1.0, nullptr); }, &us)) {
+            ctx->proj_variant = bp;
+            return rc;
+        }
+        rep += "projection variant " + std::to_string(pv) + " " + std::to_string(us) + " us\n";
+        if (us < best) {
+            best = us;
+            bp = pv;
+        }
+    }
+    ctx->proj_variant = bp;
+    ctx->tune_epoch++;
+    rep += "projection chosen variant " + std::to_string(bp) + "\n";
+    return write_report(rep, report, len);
+}
+
 int solver_use_graph(nova_solver *sv, int enable)
 {
     sv->use_graph = enable != 0;
@@ -612,6 +404,7 @@ static int record_step(nova_solver *sv, double dt, double *U, double *B, double 
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
     sv->gdt = dt;
+    sv->gepoch = sv->ctx->tune_epoch;
     sv->gU = U;
     sv->gB = B;
     sv->gp = p;
@@ -629,7 +422,8 @@ int solver_step(nova_solver *sv, double dt, int nsteps, double *U, double *B, do
     }
     if (nsteps == 0) return NOVA_OK;
     nova_ctx *ctx = sv->ctx;
-    if (!sv->gexec || sv->gdt != dt || sv->gU != U || sv->gB != B || sv->gp != p || sv->gpB != pB)
+    if (!sv->gexec || sv->gdt != dt || sv->gU != U || sv->gB != B || sv->gp != p || sv->gpB != pB ||
+        sv->gepoch != ctx->tune_epoch)
         if (int rc = record_step(sv, dt, U, B, p, pB)) return rc;
     // order the replays after the caller's stream and the caller after them
     hipEvent_t ev;

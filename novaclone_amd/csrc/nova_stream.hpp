@@ -1,0 +1,341 @@
+// nova_stream.hpp -- the streamed (LDS plane + register queue) stage
+// kernel shared by the forward stages (Navier-Stokes, MHD) and the
+// tangent-linear and adjoint stages.
+#pragma once
+#include "nova_internal.hpp"
+#include "nova_stencil.hpp"
+
+namespace nova {
+namespace {
+
+enum { MODEL_NS = 0, MODEL_MHD = 1, MODEL_TLM = 2, MODEL_ADJ = 3 };
+template <int MODEL>
+constexpr int model_nf() { return MODEL == MODEL_NS ? 1 : 2; }
+// whether the stage combination of the model reads a step-start field f
+template <int MODEL>
+constexpr bool model_x0(int f) { return MODEL == MODEL_MHD || (MODEL == MODEL_NS && f == 0) || (MODEL == MODEL_TLM && f == 1); }
+
+// ---- the streamed stage kernel ----
+//
+// A 64 x 4 block owns a (k, j) tile and walks i over a chunk of planes.
+// The i-neighbours of its own column live in a register queue q (planes
+// i-H .. i+H); the current plane, with its j and k halos, is staged in
+// LDS.  Every input value is loaded from global memory once per block
+// (plus the halo ring and the 2H warm-up planes of the chunk), and the
+// loads for plane i+1 are issued before plane i is computed.
+constexpr int kTK = 64, kTJ = 4;
+
+template <int NF>
+struct StreamArgs {
+    const double *X[NF];   // stage inputs (stencil)
+    const double *X0[NF];  // step-start fields for the stage combination (may be null)
+    double *out[NF];
+    StageArgs sa;
+    double nu, eta;
+    // adjoint stage only: out = ob * T + yg * Y, acc = acc_a * acc + acc_b * m
+    const double *Y;
+    double *acc;
+    double ob, yg, acc_a, acc_b;
+    int ichunk;
+    int nkt, njt, nic;
+};
+
+// MODEL_NS: X = {U}; MODEL_MHD: X = {U, B};  out = stage combination.
+// MODEL_TLM: X = {U, d}, X0[1] = step-start tangent; out[0] = tangent stage.
+// MODEL_ADJ: X = {U, m}; out[0] = ob (m + dt J_R(U)^T m) + yg Y.
+template <int H, int MODEL, bool WALLS, int MINW, int kTK>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MINW)))
+k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
+{
+    constexpr int NF = model_nf<MODEL>();
+    constexpr int kTJ = 256 / kTK;
+    constexpr int QW = 2 * H + 1;
+    constexpr int NC = 3 * NF;
+    constexpr int LK = kTK + 2 * H, LJ = kTJ + 2 * H;
+    constexpr int NHALO = 2 * H * (kTK + kTJ);
+    constexpr int NSLOT = (NHALO + kTK * kTJ - 1) / (kTK * kTJ);
+    __shared__ double sh[NC][LJ][LK];
+
+    // tile decode; consecutive tiles share halos, so each XCD (blocks are
+    // dealt round-robin over 8) takes a contiguous run of tiles
+    const int T = a.nkt * a.njt * a.nic;
+    int t = blockIdx.x;
+    if (T % 8 == 0) t = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
+    const int kt = t % a.nkt;
+    const int jt = (t / a.nkt) % a.njt;
+    const int ic = t / (a.nkt * a.njt);
+    const int tk = threadIdx.x, tj = threadIdx.y;
+    const int tid = tj * kTK + tk;
+    const int k = kt * kTK + tk, j = jt * kTJ + tj;
+    const bool valid = k < g.s[2] && j < g.s[1];
+    const int i0 = ic * a.ichunk;
+    int i1 = i0 + a.ichunk;
+    if (i1 > g.s[0]) i1 = g.s[0];
+    const int64_t plane = (int64_t)g.s[1] * g.s[2];
+    // element (component c, plane offset po, in-plane offset o): the plane
+    // base is block-uniform and 64-bit, the lane offset a 32-bit byte offset
+    // (a plane holds far fewer than 2^29 nodes), so loads address from a
+    // scalar base plus one 32-bit register
+    auto ldp = [&](const double *b, int c, int64_t po, uint32_t o) -> double {
+        return *(const double *)((const char *)(b + c * g.np + po) + (uint32_t)(o * 8u));
+    };
+    auto stp = [&](double *b, int c, int64_t po, uint32_t o, double v) {
+        *(double *)((char *)(b + c * g.np + po) + (uint32_t)(o * 8u)) = v;
+    };
+
+    // own column
+    int fj, fk;
+    const int jm = bmap(j, g.n[1], g.wall[1], fj);
+    const int km = bmap(k, g.n[2], g.wall[2], fk);
+    const uint32_t col = (uint32_t)jm * g.s[2] + km;
+    double sgn_col[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) sgn_col[c] = rsign<WALLS>(c, 1, fj) * rsign<WALLS>(c, 2, fk);
+
+    // halo slots of this thread: LDS position, source column, signs
+    int hrow[NSLOT], hcolk[NSLOT];
+    uint32_t hsrc[NSLOT];
+    double hsg[NSLOT][3];
+    bool hon[NSLOT];
+#pragma unroll
+    for (int s = 0; s < NSLOT; s++) {
+        const int e = tid + s * kTK * kTJ;
+        hon[s] = e < NHALO;
+        int r = 0, cc = 0, jj = 0, kk = 0;
+        if (e < 2 * H * kTK) {  // j halo rows
+            const int rr = e / kTK;
+            cc = e % kTK + H;
+            r = rr < H ? rr : kTJ + rr;
+            jj = jt * kTJ + (rr < H ? rr - H : kTJ + rr - H);
+            kk = kt * kTK + (cc - H);
+        } else if (e < NHALO) {  // k halo columns
+            const int e2 = e - 2 * H * kTK;
+            const int rr = e2 / (2 * H), c2 = e2 % (2 * H);
+            r = rr + H;
+            cc = c2 < H ? c2 : kTK + c2;
+            jj = jt * kTJ + rr;
+            kk = kt * kTK + (c2 < H ? c2 - H : kTK + c2 - H);
+        }
+        int f1, f2;
+        const int jm2 = bmap(jj, g.n[1], g.wall[1], f1);
+        const int km2 = bmap(kk, g.n[2], g.wall[2], f2);
+        hrow[s] = r;
+        hcolk[s] = cc;
+        hsrc[s] = (uint32_t)jm2 * g.s[2] + km2;
+#pragma unroll
+        for (int c = 0; c < 3; c++) hsg[s][c] = rsign<WALLS>(c, 1, f1) * rsign<WALLS>(c, 2, f2);
+    }
+
+    // loaders
+    auto plane_off = [&](int ii, int &fi) -> int64_t {
+        const int im = bmap(ii, g.n[0], g.wall[0], fi);
+        return (int64_t)im * plane;
+    };
+
+    double q[NF][3][QW];
+#pragma unroll
+    for (int o = 0; o < QW; o++) {
+        int fi;
+        const int64_t po = plane_off(i0 - H + o, fi);
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                q[f][c][o] = rsign<WALLS>(c, 0, fi) * sgn_col[c] * ldp(a.X[f], c, po, col);
+            }
+    }
+    double hv[NSLOT][NC];
+    {
+        int fi;
+        const int64_t po = plane_off(i0, fi);
+#pragma unroll
+        for (int s = 0; s < NSLOT; s++)
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    hv[s][f * 3 + c] =
+                        hon[s] ? rsign<WALLS>(c, 0, fi) * hsg[s][c] * ldp(a.X[f], c, po, hsrc[s]) : 0.0;
+    }
+
+    const double zi = g.zinv[km], zl = g.zlap[km], zm = g.zmet[km];
+    // step-start values of the own column, one plane ahead like the queue
+    double x0c[NF][3];
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            x0c[f][c] = (model_x0<MODEL>(f) && a.X0[f] && valid) ? ldp(a.X0[f], c, (int64_t)i0 * plane, col) : 0.0;
+
+    for (int i = i0; i < i1; i++) {
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) sh[f * 3 + c][tj + H][tk + H] = q[f][c][H];
+#pragma unroll
+        for (int s = 0; s < NSLOT; s++)
+            if (hon[s])
+#pragma unroll
+                for (int n = 0; n < NC; n++) sh[n][hrow[s]][hcolk[s]] = hv[s][n];
+        __syncthreads();
+
+        // prefetch plane i+H+1 for the queue, the halo of plane i+1 and the
+        // step-start values of plane i+1
+        double nq[NF][3], nx0[NF][3];
+        if (i + 1 < i1) {
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    nx0[f][c] = (model_x0<MODEL>(f) && a.X0[f] && valid) ? ldp(a.X0[f], c, (int64_t)(i + 1) * plane, col) : 0.0;
+            int fi;
+            const int64_t po = plane_off(i + H + 1, fi);
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    nq[f][c] = rsign<WALLS>(c, 0, fi) * sgn_col[c] * ldp(a.X[f], c, po, col);
+            int fh;
+            const int64_t ph = plane_off(i + 1, fh);
+#pragma unroll
+            for (int s = 0; s < NSLOT; s++)
+#pragma unroll
+                for (int f = 0; f < NF; f++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++)
+                        hv[s][f * 3 + c] =
+                            hon[s] ? rsign<WALLS>(c, 0, fh) * hsg[s][c] * ldp(a.X[f], c, ph, hsrc[s]) : 0.0;
+        }
+
+        if (valid) {
+            auto A = [&](int f, int c, int d, int o) -> double {
+                if (d == 0) return q[f][c][H + o];
+                if (d == 1) return sh[f * 3 + c][tj + H + o][tk + H];
+                return sh[f * 3 + c][tj + H][tk + H + o];
+            };
+            const int64_t pi = (int64_t)i * plane;
+            const StageArgs &sa = a.sa;
+            if constexpr (MODEL == MODEL_NS) {
+                double r[3];
+                ns_rhs<H>(g, A, a.nu, zi, zl, zm, r);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const double du = sa.dt * r[c];
+                    const double u = A(0, c, 0, 0);
+                    const double u0 = a.X0[0] ? x0c[0][c] : u;
+                    stp(a.out[0], c, pi, col, combine(sa, u, u0, du));
+                }
+            } else if constexpr (MODEL == MODEL_TLM) {
+                double r[3];
+                tlm_rhs<H>(g, A, a.nu, zi, zl, zm, r);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const double v = q[1][c][H];
+                    const double v0 = a.X0[1] ? x0c[1][c] : v;
+                    stp(a.out[0], c, pi, col, combine(sa, v, v0, sa.dt * r[c]));
+                }
+            } else if constexpr (MODEL == MODEL_ADJ) {
+                double r[3];
+                adj_rhs<H>(g, A, a.nu, zi, zl, r);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const double m = q[1][c][H];
+                    double o = a.ob * (m + sa.dt * r[c]);
+                    if (a.Y) o += a.yg * ldp(a.Y, c, pi, col);
+                    if (a.acc)
+                        stp(a.acc, c, pi, col, (a.acc_a != 0.0 ? a.acc_a * ldp(a.acc, c, pi, col) : 0.0) + a.acc_b * m);
+                    stp(a.out[0], c, pi, col, o);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const double du =
+                        sa.dt * ((((-div_xy<H>(g, A, 0, 0, c, zi)) + lap_c<H>(g, A, 0, c, a.nu, zl, zm)) +
+                                  div_xy<H>(g, A, 1, 1, c, zi)) - grad_pm<H>(g, A, 1, c, zi));
+                    const double db = sa.dt * (((-div_xy<H>(g, A, 0, 1, c, zi)) + div_xy<H>(g, A, 1, 0, c, zi)) +
+                                               lap_c<H>(g, A, 1, c, a.eta, zl, zm));
+                    const double u = A(0, c, 0, 0), b = q[NF - 1][c][H];
+                    const double u0 = a.X0[0] ? x0c[0][c] : u;
+                    const double b0 = a.X0[NF - 1] ? x0c[NF - 1][c] : b;
+                    stp(a.out[0], c, pi, col, combine(sa, u, u0, du));
+                    stp(a.out[NF - 1], c, pi, col, combine(sa, b, b0, db));
+                }
+            }
+        }
+        __syncthreads();
+        if (i + 1 < i1) {
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+#pragma unroll
+                    for (int o = 0; o < 2 * H; o++) q[f][c][o] = q[f][c][o + 1];
+                    q[f][c][2 * H] = nq[f][c];
+                    x0c[f][c] = nx0[f][c];
+                }
+        }
+    }
+}
+
+int pick_ichunk(const Grid &g, int nkt, int njt)
+{
+    // about 2048 tiles (8 blocks per CU on 256 CUs), chunks of 8..64 planes
+    int64_t per = (int64_t)g.s[0] * nkt * njt / 2048;
+    int c = (int)(per < 8 ? 8 : (per > 64 ? 64 : per));
+    if (c > g.s[0]) c = g.s[0];
+    return c;
+}
+
+template <int H, int MODEL, bool WALLS, int MINW, int TK>
+void launch_stream_tile(nova_ctx *ctx, StreamArgs<model_nf<MODEL>()> a)
+{
+    const Grid &g = ctx->g;
+    constexpr int TJ = 256 / TK;
+    a.nkt = (g.s[2] + TK - 1) / TK;
+    a.njt = (g.s[1] + TJ - 1) / TJ;
+    const int ic = ctx->stage_ichunk[MODEL];
+    a.ichunk = ic > 0 ? (ic < g.s[0] ? ic : g.s[0]) : pick_ichunk(g, a.nkt, a.njt);
+    a.nic = (g.s[0] + a.ichunk - 1) / a.ichunk;
+    const int T = a.nkt * a.njt * a.nic;
+    hipLaunchKernelGGL((k_stage_stream<H, MODEL, WALLS, MINW, TK>), dim3(T), dim3(TK, TJ, 1), 0, ctx->stream, g, a);
+}
+
+template <int H, int MODEL, bool WALLS>
+void launch_stream_w(nova_ctx *ctx, const StreamArgs<model_nf<MODEL>()> &a)
+{
+    // variants 2-4 hold one-field models to 3 waves per SIMD; two queued
+    // fields need more than 256 registers and run unspilled at one wave per
+    // SIMD (spilled shapes measure 3-4x slower)
+    constexpr int W = model_nf<MODEL>() == 1 ? 3 : 1;
+    switch (ctx->stage_variant[MODEL]) {
+    case 0: launch_stream_tile<H, MODEL, WALLS, 1, 64>(ctx, a); break;
+    case 2: launch_stream_tile<H, MODEL, WALLS, W, 64>(ctx, a); break;
+    case 4: launch_stream_tile<H, MODEL, WALLS, W, 16>(ctx, a); break;
+    default: launch_stream_tile<H, MODEL, WALLS, W, 32>(ctx, a); break;
+    }
+}
+
+// Streamed launch for any model and stencil half-width (the row-stream
+// variant of the forward stages is dispatched by their own launcher).
+template <int MODEL>
+int launch_stream_model(nova_ctx *ctx, const StreamArgs<model_nf<MODEL>()> &a, const char *name)
+{
+    const Grid &g = ctx->g;
+    const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
+    TimedScope ts(ctx, name);
+    switch (g.h * 2 + (walls ? 1 : 0)) {
+    case 2: launch_stream_w<1, MODEL, false>(ctx, a); break;
+    case 3: launch_stream_w<1, MODEL, true>(ctx, a); break;
+    case 4: launch_stream_w<2, MODEL, false>(ctx, a); break;
+    case 5: launch_stream_w<2, MODEL, true>(ctx, a); break;
+    case 6: launch_stream_w<3, MODEL, false>(ctx, a); break;
+    case 7: launch_stream_w<3, MODEL, true>(ctx, a); break;
+    case 8: launch_stream_w<4, MODEL, false>(ctx, a); break;
+    case 9: launch_stream_w<4, MODEL, true>(ctx, a); break;
+    default: return fail(NOVA_ERR_ARG, "stencil order");
+    }
+    return check_launch(name);
+}
+
+}  // namespace
+}  // namespace nova

@@ -1,0 +1,112 @@
+"""Launch-shape autotuner and variant equivalence (no reference counterpart:
+every launch shape must give the same step up to rounding, and tuning must
+not change results or break a recorded graph)."""
+import math
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+TWO_PI = 2.0 * math.pi
+
+
+@pytest.fixture(scope="module")
+def nv():
+    import torch
+    import novaclone_amd as nv
+    assert torch.cuda.is_available()
+    nv.lib()
+    return nv
+
+
+def tg(nv, G):
+    import torch
+    s = G.shape
+    x = [torch.arange(s[a], dtype=torch.float64, device="cuda") * (TWO_PI / s[a]) for a in range(3)]
+    X, Y, Z = torch.meshgrid(*x, indexing="ij")
+    return torch.stack([torch.sin(X) * torch.cos(Y) * torch.cos(Z), -torch.cos(X) * torch.sin(Y) * torch.cos(Z),
+                        0 * X]).contiguous()
+
+
+def rel(a, b):
+    return ((a - b).norm() / b.norm()).item()
+
+
+@pytest.mark.parametrize("wall", [False, True])
+def test_every_stage_variant_gives_the_same_step(nv, wall):
+    import torch
+    N = (24, 12, 32)
+    G = nv.Mesh(N, (TWO_PI, 3.0 if wall else TWO_PI, TWO_PI), bc=("periodic", "wall" if wall else "periodic",
+                                                                    "periodic"))
+    P = nv.Poisson(G)
+    S = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-2)
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    U0 = torch.randn((3, *G.shape), generator=gen, dtype=torch.float64, device="cuda")
+    if wall:
+        U0[1, :, 0, :] = 0.0
+        U0[1, :, -1, :] = 0.0
+    P.project(U0, 1.0)
+    ref = None
+    for v in (3, 0, 1, 2, 4):
+        for c in (0, 5):
+            G.tune(v, c, 1)
+            U = U0.clone()
+            S.step(1e-3, 2, U, G.field(1))
+            if ref is None:
+                ref = U
+            else:
+                print("variant", v, "chunk", c, "rel", rel(U, ref), flush=True)
+                assert rel(U, ref) <= 1e-14
+
+
+def test_solver_autotune_keeps_results_and_rerecords_graph(nv):
+    import torch
+    G = nv.Mesh((32, 32, 32), (TWO_PI,) * 3)
+    P = nv.Poisson(G)
+    U0 = tg(nv, G)
+    Se = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-2)
+    Sg = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-2, graph=True)
+    Ue, Ug = U0.clone(), U0.clone()
+    p = G.field(1)
+    Se.step(2e-3, 3, Ue, p)
+    Sg.step(2e-3, 3, Ug, p)
+    rep = Sg.autotune(U0, reps=2)
+    print(rep, flush=True)
+    assert "ns_stage chosen variant" in rep and "projection chosen variant" in rep
+    assert rep.count("ns_stage variant") == 25
+    Se.step(2e-3, 3, Ue, p)
+    Sg.step(2e-3, 3, Ug, p)  # the graph is recorded again with the tuned shapes
+    torch.cuda.synchronize()
+    assert rel(Ug, Ue) <= 1e-14
+
+
+def test_mhd_autotune(nv):
+    import torch
+    G = nv.Mesh((24, 24, 24), (TWO_PI,) * 3)
+    P = nv.Poisson(G)
+    U = tg(nv, G)
+    B = U.flip(0).contiguous()
+    S = nv.Solver(P, nv.MODEL_MHD, nv.RK_SSP3, nu=1e-2, eta=1e-2)
+    U1, B1 = U.clone(), B.clone()
+    S.step(1e-3, 2, U1, G.field(1), B1, G.field(1))
+    rep = S.autotune(U, B, reps=1)
+    assert "mhd_stage chosen variant" in rep
+    U2, B2 = U.clone(), B.clone()
+    S.step(1e-3, 2, U2, G.field(1), B2, G.field(1))
+    assert rel(U2, U1) <= 1e-14 and rel(B2, B1) <= 1e-14
+
+
+def test_adjoint_autotune_keeps_gradient(nv):
+    import torch
+    G = nv.Mesh((24, 24, 24), (TWO_PI,) * 3)
+    P = nv.Poisson(G)
+    U0 = tg(nv, G)
+    A = nv.Adjoint(nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-2), 2, 6)
+    UT = A.forward(2e-3, 5, U0)
+    _, g = nv.enstrophy(G, UT, grad=True)
+    l1 = A.backward(g)
+    d1 = A.tlm(g)
+    rep = A.autotune(reps=1)
+    assert "adj_stage chosen variant" in rep and "tlm_stage chosen variant" in rep
+    l2 = A.backward(g)
+    d2 = A.tlm(g)
+    assert rel(l2, l1) <= 1e-14 and rel(d2, d1) <= 1e-14
