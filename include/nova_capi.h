@@ -73,6 +73,7 @@ typedef struct nova_ctx nova_ctx;
 typedef struct nova_poisson nova_poisson;
 typedef struct nova_solver nova_solver;
 typedef struct nova_adjoint nova_adjoint;
+typedef struct nova_ensemble nova_ensemble;
 
 /* ---- context / memory (replaces Mesh construction, Mesh.cpp:24-179, and
  * Field storage, src/Field/Field.hxx:21-134) ---- */
@@ -213,6 +214,35 @@ int nova_weighted_dot(nova_ctx *ctx, const double *a, const double *b, int ncomp
 /* J = dV sum_nodes w |curl U|^2 (central stencil curl); dJdU (may be NULL)
  * receives its gradient 2 dV curl(curl U). */
 int nova_enstrophy(nova_ctx *ctx, const double *U, double *J, double *dJdU);
+
+/* ---- ensembles of uneven instances (no reference counterpart: NOVA runs
+ * one case per process; the step of each instance is that of
+ * nova_solver_step with NOVA_RK_SSP3, app/TG_Piosson/RKLoop.H for
+ * NOVA_MODEL_NS and app/OT_Poisson/{UEqn,BEqn}.H for NOVA_MODEL_MHD) ----
+ * Instances share the stencil order m and are periodic; instances with the
+ * same grid form a size class solved by one batched Poisson pipeline.  Each
+ * RK stage of all instances is one persistent kernel launch, and with the
+ * graph enabled a whole step is one replayed HIP graph.  Every instance's
+ * fields equal those of its standalone nova_solver_step run bit for bit. */
+typedef struct nova_instance_desc {
+    nova_grid_desc grid;
+    int model;      /* NOVA_MODEL_NS or NOVA_MODEL_MHD */
+    double nu, eta; /* viscosity and (MHD) magnetic diffusivity */
+} nova_instance_desc;
+int nova_ensemble_create(const nova_instance_desc *inst, int n, int device, nova_ensemble **out);
+int nova_ensemble_destroy(nova_ensemble *e);
+int nova_ensemble_set_stream(nova_ensemble *e, void *stream);
+/* Copy instance i's fields in (store = 0: U, B from the caller) or out
+ * (store = 1: U, B, p, pB to the caller); device pointers in the stored
+ * layout of the instance's grid, NULL skips a field. */
+int nova_ensemble_copy(nova_ensemble *e, int i, int store, double *U, double *B, double *p, double *pB);
+/* Advance every instance nsteps of dt (SSP-RK3). */
+int nova_ensemble_step(nova_ensemble *e, double dt, int nsteps);
+/* Record a step as a HIP graph and replay it (1) or launch directly (0). */
+int nova_ensemble_use_graph(nova_ensemble *e, int enable);
+/* out: instances, size classes, work-list tiles per stage, persistent
+ * blocks, total nodes, field slots. */
+int nova_ensemble_info(nova_ensemble *e, int64_t out[6]);
 
 /* ---- batched tridiagonal (src/Tools/Tools.cpp:43-62, tools::tdma) ----
  * nbatch systems of size n, system s at a[s*n ...]; cyclic != 0 adds the

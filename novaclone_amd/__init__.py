@@ -49,6 +49,10 @@ class GridDesc(C.Structure):
     ]
 
 
+class InstanceDesc(C.Structure):
+    _fields_ = [("grid", GridDesc), ("model", C.c_int), ("nu", C.c_double), ("eta", C.c_double)]
+
+
 _P = C.c_void_p
 _D = C.c_double
 _I = C.c_int
@@ -94,6 +98,13 @@ _SIGS = {
     "nova_reduce_dissipation": [_P, _P, _D, _DP],
     "nova_mhd_diag": [_P, _P, _P, _D, _D, _DP],
     "nova_tdma_batched": [_P, _I, _I, _P, _P, _P, _P, _P, _I],
+    "nova_ensemble_create": [C.POINTER(InstanceDesc), _I, _I, C.POINTER(_P)],
+    "nova_ensemble_destroy": [_P],
+    "nova_ensemble_set_stream": [_P, _P],
+    "nova_ensemble_copy": [_P, _I, _I, _P, _P, _P, _P],
+    "nova_ensemble_step": [_P, _D, _I],
+    "nova_ensemble_use_graph": [_P, _I],
+    "nova_ensemble_info": [_P, C.POINTER(C.c_int64)],
     "nova_adjoint_create": [_P, _I, _I, C.POINTER(_P)],
     "nova_adjoint_destroy": [_P],
     "nova_adjoint_forward": [_P, _D, _I, _P, _P],
@@ -455,6 +466,82 @@ class Adjoint:
     def close(self):
         if getattr(self, "_h", None):
             lib().nova_adjoint_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """Batched ensemble of uneven periodic instances (nova_ensemble_*).
+
+    ``instances`` is a list of ``(N, L, model, nu, eta)``; every stage of all
+    instances is one persistent launch and, with ``graph``, a step is one
+    replayed HIP graph.  Fields are copied in and out with ``load``/``store``."""
+
+    def __init__(self, instances, m: int = 6, device: int = 0, graph: bool = True):
+        if not torch.cuda.is_available():
+            raise NovaError("no GPU visible: the NOVA HIP path has no CPU fallback")
+        n = len(instances)
+        arr = (InstanceDesc * n)()
+        self.shapes, self.models = [], []
+        for q, (N, L, model, nu, eta) in enumerate(instances):
+            d = arr[q]
+            for a in range(3):
+                d.grid.n[a] = int(N[a])
+                d.grid.L[a] = float(L[a])
+                d.grid.origin[a] = 0.0
+                d.grid.bc[a] = BC_PERIODIC
+            d.grid.m = int(m)
+            d.model = int(model)
+            d.nu = float(nu)
+            d.eta = float(eta)
+            self.shapes.append(tuple(int(x) for x in N))
+            self.models.append(int(model))
+        self.device = torch.device("cuda", device)
+        torch.cuda.set_device(self.device)
+        h = C.c_void_p()
+        _check(lib().nova_ensemble_create(arr, n, int(device), C.byref(h)), "nova_ensemble_create")
+        self._h = h
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        _check(lib().nova_ensemble_set_stream(h, C.c_void_p(s)), "nova_ensemble_set_stream")
+        if graph:
+            _check(lib().nova_ensemble_use_graph(h, 1), "nova_ensemble_use_graph")
+
+    def __len__(self) -> int:
+        return len(self.shapes)
+
+    def info(self) -> dict:
+        out = (C.c_int64 * 6)()
+        _check(lib().nova_ensemble_info(self._h, out), "nova_ensemble_info")
+        return {"instances": out[0], "classes": out[1], "tiles": out[2], "blocks": out[3], "nodes": out[4],
+                "slots": out[5]}
+
+    def load(self, i: int, U: torch.Tensor, B: torch.Tensor | None = None) -> None:
+        _check(lib().nova_ensemble_copy(self._h, int(i), 0, _ptr(U.contiguous()), _ptr(B), None, None),
+               "nova_ensemble_copy")
+
+    def store(self, i: int):
+        """(U, B, p, pB) of instance i (B and pB are None for NS)."""
+        shp = self.shapes[i]
+        U = torch.empty((3, *shp), dtype=torch.float64, device=self.device)
+        p = torch.empty((1, *shp), dtype=torch.float64, device=self.device)
+        mhd = self.models[i] == MODEL_MHD
+        B = torch.empty_like(U) if mhd else None
+        pB = torch.empty_like(p) if mhd else None
+        _check(lib().nova_ensemble_copy(self._h, int(i), 1, _ptr(U), _ptr(B), _ptr(p), _ptr(pB)),
+               "nova_ensemble_copy")
+        return U, B, p, pB
+
+    def step(self, dt: float, nsteps: int) -> None:
+        _check(lib().nova_ensemble_step(self._h, float(dt), int(nsteps)), "nova_ensemble_step")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nova_ensemble_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -17,14 +17,18 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_objs"
 LIB = PKG / "libnova_hip.so"
-SOURCES = ["nova_capi.hip", "nova_ops.hip", "nova_fft.hip", "nova_step.hip", "nova_adjoint.hip"]
+SOURCES = ["nova_capi.hip", "nova_ops.hip", "nova_fft.hip", "nova_step.hip", "nova_adjoint.hip", "nova_ensemble.hip"]
 ARCH = os.environ.get("NOVA_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed-FP32 instructions are disabled for every HIP object (the code is
 # FP64, and the packed forms are kept out of the device code entirely).
+# Multiply-adds are fused only within one source expression
+# (-ffp-contract=on), so every kernel that evaluates the same expression
+# rounds the same way: launch shapes, the grouped ensemble kernel and the
+# single-grid kernels agree bit for bit.
 FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=on",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-result",
     f"-I{ROOT / 'include'}",

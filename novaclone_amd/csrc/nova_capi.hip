@@ -90,7 +90,7 @@ int ensure_red(nova_ctx *ctx, size_t n)
     return NOVA_OK;
 }
 
-int poisson_create(nova_ctx *ctx, nova_poisson **out);
+int poisson_create(nova_ctx *ctx, nova_poisson **out, int batch_cap = 1);
 int poisson_destroy(nova_poisson *ps);
 int poisson_solve(nova_poisson *ps, const double *rhs, double *p);
 int project(nova_poisson *ps, double *F, double s, double *p);
@@ -106,6 +106,13 @@ int adjoint_forward(nova_adjoint *ad, double dt, int nsteps, const double *U0, d
 int adjoint_backward(nova_adjoint *ad, const double *lamT, double *lam0);
 int adjoint_tlm(nova_adjoint *ad, const double *d0, double *dT);
 int adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t len);
+int ensemble_create(const nova_instance_desc *inst, int n, int device, nova_ensemble **out);
+int ensemble_destroy(nova_ensemble *e);
+int ensemble_set_stream(nova_ensemble *e, hipStream_t st);
+int ensemble_copy(nova_ensemble *e, int i, int store, double *U, double *B, double *p, double *pB);
+int ensemble_step(nova_ensemble *e, double dt, int nsteps);
+int ensemble_use_graph(nova_ensemble *e, int enable);
+int ensemble_info(nova_ensemble *e, int64_t out[6]);
 
 // Mesh::z (src/Mesh/Mesh.cpp:373-407) for padded index kp on one rank.
 static double mesh_z(const nova_grid_desc &d, double dz, double hb, int m, int kp)
@@ -577,6 +584,48 @@ int nova_adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t repor
 {
     NOVA_REQUIRE(ad && reps >= 1 && (!report || report_len > 0), "bad argument");
     NOVA_GUARD(adjoint_autotune(ad, reps, report, report_len));
+}
+
+int nova_ensemble_create(const nova_instance_desc *inst, int n, int device, nova_ensemble **out)
+{
+    NOVA_REQUIRE(inst && out && n > 0, "bad argument");
+    NOVA_HIP(hipSetDevice(device));
+    NOVA_GUARD(ensemble_create(inst, n, device, out));
+}
+
+int nova_ensemble_destroy(nova_ensemble *e)
+{
+    NOVA_GUARD(ensemble_destroy(e));
+}
+
+int nova_ensemble_set_stream(nova_ensemble *e, void *stream)
+{
+    NOVA_REQUIRE(e, "null ensemble");
+    NOVA_GUARD(ensemble_set_stream(e, (hipStream_t)stream));
+}
+
+int nova_ensemble_copy(nova_ensemble *e, int i, int store, double *U, double *B, double *p, double *pB)
+{
+    NOVA_REQUIRE(e, "null ensemble");
+    NOVA_GUARD(ensemble_copy(e, i, store, U, B, p, pB));
+}
+
+int nova_ensemble_step(nova_ensemble *e, double dt, int nsteps)
+{
+    NOVA_REQUIRE(e && nsteps >= 0, "bad argument");
+    NOVA_GUARD(ensemble_step(e, dt, nsteps));
+}
+
+int nova_ensemble_use_graph(nova_ensemble *e, int enable)
+{
+    NOVA_REQUIRE(e, "null ensemble");
+    NOVA_GUARD(ensemble_use_graph(e, enable));
+}
+
+int nova_ensemble_info(nova_ensemble *e, int64_t out[6])
+{
+    NOVA_REQUIRE(e && out, "bad argument");
+    NOVA_GUARD(ensemble_info(e, out));
 }
 
 int nova_weighted_dot(nova_ctx *ctx, const double *a, const double *b, int ncomp, double *out)

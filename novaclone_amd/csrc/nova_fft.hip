@@ -50,6 +50,7 @@ struct nova_poisson {
     double norm = 1.0;
     double *ptmp = nullptr;  // potential when the caller passes none
     double *dtmp = nullptr;  // divergence scratch of the split projection
+    int batch_cap = 1;       // fields one batched solve can take
 };
 
 namespace nova {
@@ -248,9 +249,12 @@ template <int N, bool WALLS>
 __global__ void __launch_bounds__(kFftThreads) k_fwd_z(Grid g, const double2 *__restrict__ tw,
                                                        const int *__restrict__ perm, const double *__restrict__ src,
                                                        int src_is_vector, double scale, double2 *__restrict__ H,
-                                                       int Nx, int sy)
+                                                       int Nx, int sy, int64_t src_bs)
 {
     constexpr int ZL = zlines<N>();
+    // batch member blockIdx.z: its source and half spectrum
+    src += blockIdx.z * src_bs;
+    H += (int64_t)blockIdx.z * (N / 2 + 1) * Nx * sy;
     __shared__ double2 stw[N];
     __shared__ int sperm[N];
     __shared__ double2 buf[(ZL / 2) * N];
@@ -319,16 +323,17 @@ template <int N>
 __global__ void __launch_bounds__(kFftThreads) k_fft_x(const double2 *__restrict__ tw, const int *__restrict__ perm,
                                                        double2 *__restrict__ H, int sy,
                                                        const double *__restrict__ kpx, const double *__restrict__ kpy,
-                                                       const double *__restrict__ kpz)
+                                                       const double *__restrict__ kpz, int Kc)
 {
     constexpr int JB = xbatch_t<N>();
     __shared__ double2 stw[N];
     __shared__ int sperm[N];
     __shared__ double2 buf[N * JB];
     load_tables<N>(stw, sperm, tw, perm);
-    const int kc = blockIdx.y;
+    // blockIdx.y runs over (batch member, kc) of the [b][kc][i][j] layout
+    const int kc = blockIdx.y % Kc;
     const int j0 = blockIdx.x * JB;
-    double2 *Hs = H + (int64_t)kc * N * sy;
+    double2 *Hs = H + (int64_t)blockIdx.y * N * sy;
     for (int e = threadIdx.x; e < N * JB; e += kFftThreads) {
         const int jj = e % JB, i = e / JB;
         const int j = j0 + jj;
@@ -367,9 +372,12 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x(const double2 *__restrict
 template <int N>
 __global__ void __launch_bounds__(kFftThreads) k_inv_z(Grid g, const double2 *__restrict__ tw,
                                                        const int *__restrict__ perm, const double2 *__restrict__ H,
-                                                       double *__restrict__ p, int Nx, int sy, double inv_norm)
+                                                       double *__restrict__ p, int Nx, int sy, double inv_norm,
+                                                       int64_t p_bs)
 {
     constexpr int ZL = zlines<N>();
+    H += (int64_t)blockIdx.z * (N / 2 + 1) * Nx * sy;
+    p += blockIdx.z * p_bs;
     __shared__ double2 stw[N];
     __shared__ int sperm[N];
     __shared__ double2 buf[(ZL / 2) * N];
@@ -443,6 +451,14 @@ struct RowTiles {
     int ichunk, nkt, njt, nic;
 };
 
+// A batch of same-grid fields: member b at F + b f_bs (potential p + b p_bs);
+// members b >= split use the second scale.
+struct BatchArgs {
+    int64_t f_bs = 0, p_bs = 0;
+    int split = 1;
+    double scale_hi = 1.0, s_hi = 1.0;
+};
+
 __device__ inline bool row_tile(const Grid &g, const RowTiles &t, int &k, int &j, int &i0, int &i1)
 {
     const int T = t.nkt * t.njt * t.nic;
@@ -489,10 +505,14 @@ struct ColNbrs {
 // are issued before plane i is combined.
 template <int H, bool WALLS>
 __global__ void __launch_bounds__(kRK *kRJ) k_div_rows(Grid g, RowTiles t, const double *__restrict__ F,
-                                                       double scale, double *__restrict__ out)
+                                                       double scale, double *__restrict__ out, BatchArgs ba)
 {
     int k, j, i0, i1;
     if (!row_tile(g, t, k, j, i0, i1)) return;
+    // batch member blockIdx.y: field, output and scale group
+    F += blockIdx.y * ba.f_bs;
+    out += blockIdx.y * g.np;
+    if ((int)blockIdx.y >= ba.split) scale = ba.scale_hi;
     const int64_t plane = (int64_t)g.s[1] * g.s[2];
     const int col = j * g.s[2] + k;
     ColNbrs<H, WALLS> nb;
@@ -550,10 +570,13 @@ __global__ void __launch_bounds__(kRK *kRJ) k_div_rows(Grid g, RowTiles t, const
 // F <- F - s * grad(p) (app/TG_Piosson/RKLoop.H:36, app/OT_Poisson/BEqn.H:40).
 template <int H, bool WALLS>
 __global__ void __launch_bounds__(kRK *kRJ) k_correct_rows(Grid g, RowTiles t, const double *__restrict__ p,
-                                                           double s, double *__restrict__ F)
+                                                           double s, double *__restrict__ F, BatchArgs ba)
 {
     int k, j, i0, i1;
     if (!row_tile(g, t, k, j, i0, i1)) return;
+    F += blockIdx.y * ba.f_bs;
+    p += blockIdx.y * ba.p_bs;
+    if ((int)blockIdx.y >= ba.split) s = ba.s_hi;
     const int64_t plane = (int64_t)g.s[1] * g.s[2];
     const int col = j * g.s[2] + k;
     ColNbrs<H, WALLS> nb;
@@ -631,43 +654,47 @@ RowTiles row_tiles(const Grid &g, int ichunk)
 }
 
 template <int H>
-void launch_div_rows_h(hipStream_t st, const Grid &g, const RowTiles &t, const double *F, double scale, double *out)
+void launch_div_rows_h(hipStream_t st, const Grid &g, const RowTiles &t, const double *F, double scale, double *out,
+                       int B, const BatchArgs &ba)
 {
-    const unsigned T = (unsigned)(t.nkt * t.njt * t.nic);
+    const dim3 grid((unsigned)(t.nkt * t.njt * t.nic), (unsigned)B);
     const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
-    if (walls) hipLaunchKernelGGL((k_div_rows<H, true>), dim3(T), dim3(kRK, kRJ), 0, st, g, t, F, scale, out);
-    else hipLaunchKernelGGL((k_div_rows<H, false>), dim3(T), dim3(kRK, kRJ), 0, st, g, t, F, scale, out);
+    if (walls) hipLaunchKernelGGL((k_div_rows<H, true>), grid, dim3(kRK, kRJ), 0, st, g, t, F, scale, out, ba);
+    else hipLaunchKernelGGL((k_div_rows<H, false>), grid, dim3(kRK, kRJ), 0, st, g, t, F, scale, out, ba);
 }
 
 template <int H>
-void launch_correct_rows_h(hipStream_t st, const Grid &g, const RowTiles &t, const double *p, double s, double *F)
+void launch_correct_rows_h(hipStream_t st, const Grid &g, const RowTiles &t, const double *p, double s, double *F,
+                           int B, const BatchArgs &ba)
 {
-    const unsigned T = (unsigned)(t.nkt * t.njt * t.nic);
+    const dim3 grid((unsigned)(t.nkt * t.njt * t.nic), (unsigned)B);
     const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
-    if (walls) hipLaunchKernelGGL((k_correct_rows<H, true>), dim3(T), dim3(kRK, kRJ), 0, st, g, t, p, s, F);
-    else hipLaunchKernelGGL((k_correct_rows<H, false>), dim3(T), dim3(kRK, kRJ), 0, st, g, t, p, s, F);
+    if (walls) hipLaunchKernelGGL((k_correct_rows<H, true>), grid, dim3(kRK, kRJ), 0, st, g, t, p, s, F, ba);
+    else hipLaunchKernelGGL((k_correct_rows<H, false>), grid, dim3(kRK, kRJ), 0, st, g, t, p, s, F, ba);
 }
 
-int launch_div_rows(nova_ctx *ctx, const double *F, double scale, double *out)
+int launch_div_rows(nova_ctx *ctx, const double *F, double scale, double *out, int B = 1,
+                    const BatchArgs &ba = BatchArgs())
 {
     const RowTiles t = row_tiles(ctx->g, 0);
     switch (ctx->g.h) {
-    case 1: launch_div_rows_h<1>(ctx->stream, ctx->g, t, F, scale, out); break;
-    case 2: launch_div_rows_h<2>(ctx->stream, ctx->g, t, F, scale, out); break;
-    case 3: launch_div_rows_h<3>(ctx->stream, ctx->g, t, F, scale, out); break;
-    default: launch_div_rows_h<4>(ctx->stream, ctx->g, t, F, scale, out); break;
+    case 1: launch_div_rows_h<1>(ctx->stream, ctx->g, t, F, scale, out, B, ba); break;
+    case 2: launch_div_rows_h<2>(ctx->stream, ctx->g, t, F, scale, out, B, ba); break;
+    case 3: launch_div_rows_h<3>(ctx->stream, ctx->g, t, F, scale, out, B, ba); break;
+    default: launch_div_rows_h<4>(ctx->stream, ctx->g, t, F, scale, out, B, ba); break;
     }
     return check_launch("div_rows");
 }
 
-int launch_correct_rows(nova_ctx *ctx, const double *p, double s, double *F)
+int launch_correct_rows(nova_ctx *ctx, const double *p, double s, double *F, int B = 1,
+                        const BatchArgs &ba = BatchArgs())
 {
     const RowTiles t = row_tiles(ctx->g, 0);
     switch (ctx->g.h) {
-    case 1: launch_correct_rows_h<1>(ctx->stream, ctx->g, t, p, s, F); break;
-    case 2: launch_correct_rows_h<2>(ctx->stream, ctx->g, t, p, s, F); break;
-    case 3: launch_correct_rows_h<3>(ctx->stream, ctx->g, t, p, s, F); break;
-    default: launch_correct_rows_h<4>(ctx->stream, ctx->g, t, p, s, F); break;
+    case 1: launch_correct_rows_h<1>(ctx->stream, ctx->g, t, p, s, F, B, ba); break;
+    case 2: launch_correct_rows_h<2>(ctx->stream, ctx->g, t, p, s, F, B, ba); break;
+    case 3: launch_correct_rows_h<3>(ctx->stream, ctx->g, t, p, s, F, B, ba); break;
+    default: launch_correct_rows_h<4>(ctx->stream, ctx->g, t, p, s, F, B, ba); break;
     }
     return check_launch("correct_rows");
 }
@@ -687,15 +714,15 @@ bool fft_length_supported(int n)
 }
 
 int launch_fwd_z(hipStream_t st, int n, dim3 grid, const Grid &g, const double2 *tw, const int *perm,
-                 const double *src, int vec, double scale, double2 *H, int Nx, int sy, bool walls)
+                 const double *src, int vec, double scale, double2 *H, int Nx, int sy, bool walls, int64_t src_bs)
 {
     switch (n) {
 #define CASE(L)                                                                                                    \
     case L:                                                                                                        \
         if (walls) hipLaunchKernelGGL((k_fwd_z<L, true>), grid, dim3(kFftThreads), 0, st, g, tw, perm, src, vec,   \
-                                      scale, H, Nx, sy);                                                           \
+                                      scale, H, Nx, sy, src_bs);                                                   \
         else hipLaunchKernelGGL((k_fwd_z<L, false>), grid, dim3(kFftThreads), 0, st, g, tw, perm, src, vec, scale, \
-                                H, Nx, sy);                                                                        \
+                                H, Nx, sy, src_bs);                                                                \
         return NOVA_OK;
         NOVA_FFT_LENGTHS(CASE)
 #undef CASE
@@ -722,14 +749,14 @@ int launch_fft_y(hipStream_t st, int n, bool ext, bool inv, const double2 *tw, c
     return fail(NOVA_ERR_UNSUPPORTED, "FFT length");
 }
 
-int launch_fft_x(hipStream_t st, int n, int Kc, const double2 *tw, const int *perm, double2 *H, int sy,
+int launch_fft_x(hipStream_t st, int n, int Kc, int B, const double2 *tw, const int *perm, double2 *H, int sy,
                  const double *kpx, const double *kpy, const double *kpz)
 {
     switch (n) {
 #define CASE(L)                                                                                           \
     case L: {                                                                                             \
-        dim3 grid((sy + xbatch_t<L>() - 1) / xbatch_t<L>(), Kc);                                           \
-        hipLaunchKernelGGL((k_fft_x<L>), grid, dim3(kFftThreads), 0, st, tw, perm, H, sy, kpx, kpy, kpz); \
+        dim3 grid((sy + xbatch_t<L>() - 1) / xbatch_t<L>(), Kc * B);                                       \
+        hipLaunchKernelGGL((k_fft_x<L>), grid, dim3(kFftThreads), 0, st, tw, perm, H, sy, kpx, kpy, kpz, Kc); \
         return NOVA_OK;                                                                                   \
     }
         NOVA_FFT_LENGTHS(CASE)
@@ -739,12 +766,12 @@ int launch_fft_x(hipStream_t st, int n, int Kc, const double2 *tw, const int *pe
 }
 
 int launch_inv_z(hipStream_t st, int n, dim3 grid, const Grid &g, const double2 *tw, const int *perm,
-                 const double2 *H, double *p, int Nx, int sy, double inv_norm)
+                 const double2 *H, double *p, int Nx, int sy, double inv_norm, int64_t p_bs)
 {
     switch (n) {
 #define CASE(L)                                                                                                 \
     case L:                                                                                                     \
-        hipLaunchKernelGGL((k_inv_z<L>), grid, dim3(kFftThreads), 0, st, g, tw, perm, H, p, Nx, sy, inv_norm); \
+        hipLaunchKernelGGL((k_inv_z<L>), grid, dim3(kFftThreads), 0, st, g, tw, perm, H, p, Nx, sy, inv_norm, p_bs); \
         return NOVA_OK;
         NOVA_FFT_LENGTHS(CASE)
 #undef CASE
@@ -824,9 +851,10 @@ double kmod(double kk, double h, const double *coef, int hm)
 
 }  // namespace
 
-int poisson_create(nova_ctx *ctx, nova_poisson **out)
+int poisson_create(nova_ctx *ctx, nova_poisson **out, int batch_cap)
 {
     const Grid &g = ctx->g;
+    if (batch_cap < 1) return fail(NOVA_ERR_ARG, "batch capacity must be >= 1");
     if (g.wall[0] || g.wall[2]) return fail(NOVA_ERR_UNSUPPORTED, "Poisson walls are supported in y only");
     if (g.n[2] % 2) return fail(NOVA_ERR_UNSUPPORTED, "N_z must be even");
     nova_poisson *ps = new nova_poisson();
@@ -839,6 +867,7 @@ int poisson_create(nova_ctx *ctx, nova_poisson **out)
     ps->Ty = ps->wall_y ? 2 * ps->Ny : ps->Ny;
     ps->Kc = ps->Nz / 2 + 1;
     ps->norm = (double)((long)ps->Nx * (long)ps->Ty * (long)ps->Nz);
+    ps->batch_cap = batch_cap;
     int rc;
     if ((rc = make_len(ps->fx, ps->Nx)) || (rc = make_len(ps->fy, ps->Ty)) || (rc = make_len(ps->fz, ps->Nz))) {
         free_len(ps->fx); free_len(ps->fy); free_len(ps->fz);
@@ -851,8 +880,8 @@ int poisson_create(nova_ctx *ctx, nova_poisson **out)
         return fail(NOVA_ERR_UNSUPPORTED, "FFT lengths must be one of 8..768 (2^a 3^b, listed in nova_fft.hip)");
     }
     const size_t nh = (size_t)ps->Kc * ps->Nx * ps->sy;
-    NOVA_HIP(hipMalloc((void **)&ps->H, sizeof(double2) * nh));
-    NOVA_HIP(hipMalloc((void **)&ps->dtmp, sizeof(double) * g.np));
+    NOVA_HIP(hipMalloc((void **)&ps->H, sizeof(double2) * nh * batch_cap));
+    NOVA_HIP(hipMalloc((void **)&ps->dtmp, sizeof(double) * g.np * batch_cap));
     // modified wavenumbers per index, with the reference's wrap
     const double pi = 3.14159265358979323846;
     std::vector<double> kx(ps->Nx), ky(ps->sy), kz(ps->Kc);
@@ -892,31 +921,37 @@ int poisson_destroy(nova_poisson *ps)
 
 // rhs: scalar (src_is_vector = 0) or the vector whose divergence*scale is
 // the right-hand side.
-int poisson_run(nova_poisson *ps, const double *src, int src_is_vector, double scale, double *p)
+// The transform pipeline over B same-grid members: member b's source is
+// src + b src_bs (a vector field whose divergence times its scale group's
+// factor is taken, or a scalar), its potential p + b ba.p_bs.
+static int run_pipeline(nova_poisson *ps, const double *src, int src_is_vector, double scale, double *p, int B,
+                        const BatchArgs &ba)
 {
     nova_ctx *ctx = ps->ctx;
     const Grid &g = ctx->g;
     hipStream_t st = ctx->stream;
     const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
     const int zl = zlines_rt(ps->Nz);
-    const dim3 zgrid((ps->sy + zl - 1) / zl, ps->Nx);
-    const int64_t ylines_total = (int64_t)ps->Kc * ps->Nx;
+    const dim3 zgrid((ps->sy + zl - 1) / zl, ps->Nx, B);
+    const int64_t ylines_total = (int64_t)ps->Kc * ps->Nx * B;
+    int64_t src_bs = ba.f_bs;
     int rc;
-    if (src_is_vector && ctx->proj_variant == 1) {
+    if (src_is_vector && (ctx->proj_variant == 1 || B > 1)) {
         // divergence by a streamed kernel into a scalar, then a plain r2c
         {
             TimedScope ts(ctx, "div_rows");
-            rc = launch_div_rows(ctx, src, scale, ps->dtmp);
+            rc = launch_div_rows(ctx, src, scale, ps->dtmp, B, ba);
         }
         if (rc) return rc;
         src = ps->dtmp;
         src_is_vector = 0;
         scale = 1.0;
+        src_bs = g.np;
     }
     {
         TimedScope ts(ctx, src_is_vector ? "fft_div_r2c_z" : "fft_r2c_z");
         rc = launch_fwd_z(st, ps->Nz, zgrid, g, ps->fz.tw, ps->fz.perm, src, src_is_vector, scale, ps->H, ps->Nx,
-                          ps->sy, walls);
+                          ps->sy, walls, src_bs);
     }
     if (rc || (rc = check_launch("poisson fwd_z"))) return rc;
     {
@@ -926,7 +961,7 @@ int poisson_run(nova_poisson *ps, const double *src, int src_is_vector, double s
     if (rc || (rc = check_launch("poisson fft_y"))) return rc;
     {
         TimedScope ts(ctx, "fft_x_solve");
-        rc = launch_fft_x(st, ps->Nx, ps->Kc, ps->fx.tw, ps->fx.perm, ps->H, ps->sy, ps->kpx, ps->kpy, ps->kpz);
+        rc = launch_fft_x(st, ps->Nx, ps->Kc, B, ps->fx.tw, ps->fx.perm, ps->H, ps->sy, ps->kpx, ps->kpy, ps->kpz);
     }
     if (rc || (rc = check_launch("poisson fft_x"))) return rc;
     {
@@ -936,10 +971,16 @@ int poisson_run(nova_poisson *ps, const double *src, int src_is_vector, double s
     if (rc || (rc = check_launch("poisson ifft_y"))) return rc;
     {
         TimedScope ts(ctx, "fft_c2r_z");
-        rc = launch_inv_z(st, ps->Nz, zgrid, g, ps->fz.tw, ps->fz.perm, ps->H, p, ps->Nx, ps->sy, 1.0 / ps->norm);
+        rc = launch_inv_z(st, ps->Nz, zgrid, g, ps->fz.tw, ps->fz.perm, ps->H, p, ps->Nx, ps->sy, 1.0 / ps->norm,
+                          ba.p_bs);
     }
     if (rc || (rc = check_launch("poisson inv_z"))) return rc;
     return NOVA_OK;
+}
+
+int poisson_run(nova_poisson *ps, const double *src, int src_is_vector, double scale, double *p)
+{
+    return run_pipeline(ps, src, src_is_vector, scale, p, 1, BatchArgs());
 }
 
 int poisson_solve(nova_poisson *ps, const double *rhs, double *p)
@@ -964,6 +1005,25 @@ int project(nova_poisson *ps, double *F, double s, double *p)
     else
         hipLaunchKernelGGL(k_correct<false>, grid, dim3(64, 4, 1), 0, ctx->stream, g, p, s, F);
     return check_launch("project correct");
+}
+
+// Projection of B same-grid fields F + b f_bs (potentials p + b p_bs):
+// members b < split use s_lo, the others s_hi.  Always the streamed
+// divergence / correction pipeline.
+int project_batch(nova_poisson *ps, double *F, int64_t f_bs, int B, int split, double s_lo, double s_hi, double *p,
+                  int64_t p_bs)
+{
+    nova_ctx *ctx = ps->ctx;
+    if (B < 1 || B > ps->batch_cap) return fail(NOVA_ERR_ARG, "batch exceeds the Poisson batch capacity");
+    BatchArgs ba;
+    ba.f_bs = f_bs;
+    ba.p_bs = p_bs;
+    ba.split = split;
+    ba.scale_hi = 1.0 / s_hi;
+    ba.s_hi = s_hi;
+    if (int rc = run_pipeline(ps, F, 1, 1.0 / s_lo, p, B, ba)) return rc;
+    TimedScope ts(ctx, "grad_correct");
+    return launch_correct_rows(ctx, p, s_lo, F, B, ba);
 }
 
 }  // namespace nova

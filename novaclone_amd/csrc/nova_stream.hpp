@@ -43,9 +43,18 @@ struct StreamArgs {
 // MODEL_NS: X = {U}; MODEL_MHD: X = {U, B};  out = stage combination.
 // MODEL_TLM: X = {U, d}, X0[1] = step-start tangent; out[0] = tangent stage.
 // MODEL_ADJ: X = {U, m}; out[0] = ob (m + dt J_R(U)^T m) + yg Y.
-template <int H, int MODEL, bool WALLS, int MINW, int kTK>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MINW)))
-k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
+// LDS doubles of one tile's staged plane
+template <int H, int MODEL, int kTK>
+constexpr int stage_lds()
+{
+    return 3 * model_nf<MODEL>() * (256 / kTK + 2 * H) * (kTK + 2 * H);
+}
+
+// One tile t = (kt, jt, ic) of a stage: the block streams planes i of its
+// chunk for the (k, j) tile.  shbuf holds stage_lds() doubles of LDS.
+template <int H, int MODEL, bool WALLS, int kTK>
+__device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, const StreamArgs<model_nf<MODEL>()> &a,
+                                                                 int t, double *shbuf)
 {
     constexpr int NF = model_nf<MODEL>();
     constexpr int kTJ = 256 / kTK;
@@ -54,13 +63,7 @@ k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
     constexpr int LK = kTK + 2 * H, LJ = kTJ + 2 * H;
     constexpr int NHALO = 2 * H * (kTK + kTJ);
     constexpr int NSLOT = (NHALO + kTK * kTJ - 1) / (kTK * kTJ);
-    __shared__ double sh[NC][LJ][LK];
-
-    // tile decode; consecutive tiles share halos, so each XCD (blocks are
-    // dealt round-robin over 8) takes a contiguous run of tiles
-    const int T = a.nkt * a.njt * a.nic;
-    int t = blockIdx.x;
-    if (T % 8 == 0) t = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
+    auto sh = [&](int n, int r, int c) -> double & { return shbuf[(n * LJ + r) * LK + c]; };
     const int kt = t % a.nkt;
     const int jt = (t / a.nkt) % a.njt;
     const int ic = t / (a.nkt * a.njt);
@@ -171,12 +174,12 @@ k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
 #pragma unroll
         for (int f = 0; f < NF; f++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) sh[f * 3 + c][tj + H][tk + H] = q[f][c][H];
+            for (int c = 0; c < 3; c++) sh(f * 3 + c, tj + H, tk + H) = q[f][c][H];
 #pragma unroll
         for (int s = 0; s < NSLOT; s++)
             if (hon[s])
 #pragma unroll
-                for (int n = 0; n < NC; n++) sh[n][hrow[s]][hcolk[s]] = hv[s][n];
+                for (int n = 0; n < NC; n++) sh(n, hrow[s], hcolk[s]) = hv[s][n];
         __syncthreads();
 
         // prefetch plane i+H+1 for the queue, the halo of plane i+1 and the
@@ -210,8 +213,8 @@ k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
         if (valid) {
             auto A = [&](int f, int c, int d, int o) -> double {
                 if (d == 0) return q[f][c][H + o];
-                if (d == 1) return sh[f * 3 + c][tj + H + o][tk + H];
-                return sh[f * 3 + c][tj + H][tk + H + o];
+                if (d == 1) return sh(f * 3 + c, tj + H + o, tk + H);
+                return sh(f * 3 + c, tj + H, tk + H + o);
             };
             const int64_t pi = (int64_t)i * plane;
             const StageArgs &sa = a.sa;
@@ -275,6 +278,19 @@ k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
                 }
         }
     }
+}
+
+template <int H, int MODEL, bool WALLS, int MINW, int kTK>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MINW)))
+k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
+{
+    __shared__ double sh[stage_lds<H, MODEL, kTK>()];
+    // consecutive tiles share halos, so each XCD (blocks are dealt
+    // round-robin over 8) takes a contiguous run of tiles
+    const int T = a.nkt * a.njt * a.nic;
+    int t = blockIdx.x;
+    if (T % 8 == 0) t = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
+    stage_tile<H, MODEL, WALLS, kTK>(g, a, t, sh);
 }
 
 int pick_ichunk(const Grid &g, int nkt, int njt)
