@@ -979,3 +979,218 @@ void orc_tdma(int n, const double *a, const double *b, const double *c, const do
     free(cp);
     free(dp);
 }
+
+/* ------------------------------------------------------------------ */
+/* Compact 6th-order scheme (no reference counterpart).  Lele (1992),
+ * J. Comput. Phys. 103:16-42: first derivative (2.1.7) with alpha = 1/3,
+ * a = 14/9, b = 1/9; second derivative (2.2.7) with alpha = 2/11,
+ * a = 12/11, b = 3/11; periodic lines through the N unique nodes. */
+static const double CMP_AL[2] = {1.0 / 3.0, 2.0 / 11.0};
+static const double CMP_A[2] = {14.0 / 9.0, 12.0 / 11.0};
+static const double CMP_B[2] = {1.0 / 9.0, 3.0 / 11.0};
+
+/* Sherman-Morrison reduction of the cyclic system to two Thomas solves
+ * (Press et al., Numerical Recipes, 2nd ed., sec. 2.7, "cyclic"). */
+void orc_cyclic_tridiag(int n, double a, double b, double c, const double *d, double *x)
+{
+    const double alpha = c, beta = a;  /* corners (n-1, 0) and (0, n-1) */
+    const double gamma = -b;
+    double *bb = (double *)malloc(sizeof(double) * n);
+    double *u = (double *)calloc(n, sizeof(double));
+    double *z = (double *)malloc(sizeof(double) * n);
+    double *cp = (double *)malloc(sizeof(double) * n);
+    for (int i = 0; i < n; i++) bb[i] = b;
+    bb[0] = b - gamma;
+    bb[n - 1] = b - alpha * beta / gamma;
+    /* Thomas on (a, bb, c) for two right-hand sides */
+    const double *rhs[2] = {d, u};
+    double *sol[2] = {x, z};
+    u[0] = gamma;
+    u[n - 1] = alpha;
+    for (int r = 0; r < 2; r++) {
+        const double *dd = rhs[r];
+        double *xx = sol[r];
+        cp[0] = c / bb[0];
+        xx[0] = dd[0] / bb[0];
+        for (int i = 1; i < n; i++) {
+            const double den = bb[i] - a * cp[i - 1];
+            cp[i] = c / den;
+            xx[i] = (dd[i] - a * xx[i - 1]) / den;
+        }
+        for (int i = n - 2; i >= 0; i--) xx[i] = xx[i] - cp[i] * xx[i + 1];
+    }
+    const double fact = (x[0] + beta * x[n - 1] / gamma) / (1.0 + z[0] + beta * z[n - 1] / gamma);
+    for (int i = 0; i < n; i++) x[i] = x[i] - fact * z[i];
+    free(bb); free(u); free(z); free(cp);
+}
+
+double orc_compact_kmod(double kh, int order)
+{
+    const int o = order - 1;
+    if (order == 1)
+        return (CMP_A[o] * sin(kh) + 0.5 * CMP_B[o] * sin(2.0 * kh)) / (1.0 + 2.0 * CMP_AL[o] * cos(kh));
+    return (2.0 * CMP_A[o] * (1.0 - cos(kh)) + 0.5 * CMP_B[o] * (1.0 - cos(2.0 * kh))) /
+           (1.0 + 2.0 * CMP_AL[o] * cos(kh));
+}
+
+/* one periodic line of n values -> its compact derivative */
+static void compact_line(int n, double h, int order, const double *f, double *out, double *rhs)
+{
+    const int o = order - 1;
+    for (int i = 0; i < n; i++) {
+        const double fm2 = f[(i - 2 + 2 * n) % n], fm1 = f[(i - 1 + n) % n];
+        const double fp1 = f[(i + 1) % n], fp2 = f[(i + 2) % n];
+        if (order == 1)
+            rhs[i] = CMP_A[o] / (2.0 * h) * (fp1 - fm1) + CMP_B[o] / (4.0 * h) * (fp2 - fm2);
+        else
+            rhs[i] = CMP_A[o] / (h * h) * (fp1 - 2.0 * f[i] + fm1) +
+                     CMP_B[o] / (4.0 * h * h) * (fp2 - 2.0 * f[i] + fm2);
+    }
+    orc_cyclic_tridiag(n, CMP_AL[o], 1.0, CMP_AL[o], rhs, out);
+}
+
+void orc_compact_deriv(const orc_mesh *M, const double *f, int nc, int dir, int order, double *out)
+{
+    const int h = M->h;
+    const int N[3] = {M->N[0], M->N[1], M->N[2]};
+    const int a1 = (dir + 1) % 3, a2 = (dir + 2) % 3;
+    const double hd = M->d[dir];
+    #pragma omp parallel
+    {
+        const int n = N[dir];
+        double *line = (double *)malloc(sizeof(double) * n);
+        double *res = (double *)malloc(sizeof(double) * n);
+        double *rhs = (double *)malloc(sizeof(double) * n);
+        #pragma omp for collapse(2)
+        for (int p = 0; p < N[a1]; p++)
+            for (int q = 0; q < N[a2]; q++)
+                for (int c = 0; c < nc; c++) {
+                    int ix[3];
+                    ix[a1] = p + h;
+                    ix[a2] = q + h;
+                    for (int t = 0; t < n; t++) {
+                        ix[dir] = t + h;
+                        line[t] = f[IX(M, ix[0], ix[1], ix[2]) * nc + c];
+                    }
+                    compact_line(n, hd, order, line, res, rhs);
+                    for (int t = 0; t < n; t++) {
+                        ix[dir] = t + h;
+                        out[IX(M, ix[0], ix[1], ix[2]) * nc + c] = res[t];
+                    }
+                }
+        free(line); free(res); free(rhs);
+    }
+    orc_fill(M, out, nc, nc == 3 ? 1 : -1);
+}
+
+orc_poisson *orc_poisson_create_compact(const orc_mesh *M)
+{
+    orc_poisson *P = orc_poisson_create(M);
+    const int M0 = P->M[0], M1 = P->M[1], M2 = P->M[2];
+    const double pi = ORC_PI;
+    size_t l = 0;
+    for (int k = 0; k < M2; k++)
+        for (int j = 0; j < M1; j++)
+            for (int i = 0; i < M0; i++) {
+                const double kx = (2 * i < M0) ? 2.0 * pi * i / M->L[0] : 2.0 * pi * (M0 - i) / M->L[0];
+                const double ky = (2 * j < M1) ? 2.0 * pi * j / M->L[1] : 2.0 * pi * (M1 - j) / M->L[1];
+                const double kz = 2.0 * pi * k / M->L[2];
+                const double kpx = orc_compact_kmod(kx * M->d[0], 1) / M->d[0];
+                const double kpy = orc_compact_kmod(ky * M->d[1], 1) / M->d[1];
+                const double kpz = orc_compact_kmod(kz * M->d[2], 1) / M->d[2];
+                P->k2[l++] = kpx * kpx + kpy * kpy + kpz * kpz;
+            }
+    return P;
+}
+
+static void compact_div(const orc_mesh *M, const double *F, double *out)
+{
+    const size_t n1 = NPTS(M);
+    double *comp = falloc(M, 1), *dd = falloc(M, 1);
+    memset(out, 0, sizeof(double) * n1);
+    for (int d = 0; d < 3; d++) {
+        for (size_t q = 0; q < n1; q++) comp[q] = F[3 * q + d];
+        orc_fill(M, comp, 1, -1);
+        orc_compact_deriv(M, comp, 1, d, 1, dd);
+        for (size_t q = 0; q < n1; q++) out[q] += dd[q];
+    }
+    free(comp); free(dd);
+}
+
+void orc_compact_ns_rhs(const orc_mesh *M, const double *U, double nu, double *out)
+{
+    const size_t n1 = NPTS(M), n3 = n1 * 3;
+    double *flux = falloc(M, 3), *dflux = falloc(M, 3), *dU = falloc(M, 3), *d2U = falloc(M, 3);
+    double *adv = falloc(M, 3), *lap = falloc(M, 3);
+    for (int d = 0; d < 3; d++) {
+        for (size_t q = 0; q < n1; q++)
+            for (int c = 0; c < 3; c++) flux[3 * q + c] = U[3 * q + d] * U[3 * q + c];
+        orc_fill(M, flux, 3, -1);
+        orc_compact_deriv(M, flux, 3, d, 1, dflux);
+        orc_compact_deriv(M, U, 3, d, 1, dU);
+        orc_compact_deriv(M, U, 3, d, 2, d2U);
+        for (size_t q = 0; q < n1; q++)
+            for (int c = 0; c < 3; c++) {
+                adv[3 * q + c] += dflux[3 * q + c] + U[3 * q + d] * dU[3 * q + c] + U[3 * q + c] * dU[3 * q + d];
+                lap[3 * q + c] += d2U[3 * q + c];
+            }
+    }
+    for (size_t q = 0; q < n3; q++) out[q] = -0.5 * adv[q] + nu * lap[q];
+    orc_fill(M, out, 3, 1);
+    free(flux); free(dflux); free(dU); free(d2U); free(adv); free(lap);
+}
+
+void orc_compact_project(const orc_mesh *M, orc_poisson *P, double *F, double *p, double s)
+{
+    const size_t n1 = NPTS(M);
+    double *d = falloc(M, 1), *g = falloc(M, 1);
+    orc_fill(M, F, 3, 1);
+    compact_div(M, F, d);
+    for (size_t q = 0; q < n1; q++) d[q] = d[q] / s;
+    orc_poisson_solve(P, d, p);
+    orc_fill(M, p, 1, -1);
+    for (int c = 0; c < 3; c++) {
+        orc_compact_deriv(M, p, 1, c, 1, g);
+        for (size_t q = 0; q < n1; q++) F[3 * q + c] = F[3 * q + c] - s * g[q];
+    }
+    orc_fill(M, F, 3, 1);
+    free(d); free(g);
+}
+
+void orc_compact_ns_step(const orc_mesh *M, orc_poisson *P, double *U, double *Ustar, double *p,
+                         double nu, double dt)
+{
+    const size_t n3 = NPTS(M) * 3;
+    double *U0 = falloc(M, 3), *R = falloc(M, 3);
+    memcpy(U0, U, sizeof(double) * n3);
+    memcpy(Ustar, U0, sizeof(double) * n3);
+    for (int s = 0; s < 3; s++) {
+        orc_compact_ns_rhs(M, Ustar, nu, R);
+        for (size_t q = 0; q < n3; q++) {
+            const double v = Ustar[q] + dt * R[q];
+            if (s == 0) Ustar[q] = v;
+            else if (s == 1) Ustar[q] = 0.75 * U0[q] + 0.25 * v;
+            else Ustar[q] = (1.0 / 3.0) * U0[q] + (2.0 / 3.0) * v;
+        }
+        orc_fill(M, Ustar, 3, 1);
+    }
+    orc_compact_project(M, P, Ustar, p, dt);
+    memcpy(U, Ustar, sizeof(double) * n3);
+    free(U0); free(R);
+}
+
+double orc_compact_maxabs_div(const orc_mesh *M, const double *U)
+{
+    double *d = falloc(M, 1);
+    compact_div(M, U, d);
+    double mx = 0.0;
+    const int h = M->h;
+    for (int i = h; i < h + M->N[0]; i++)
+        for (int j = h; j < h + M->N[1]; j++)
+            for (int k = h; k < h + M->N[2]; k++) {
+                const double v = fabs(d[IX(M, i, j, k)]);
+                if (v > mx) mx = v;
+            }
+    free(d);
+    return mx;
+}

@@ -107,6 +107,33 @@ void orc_mhd_diag(const orc_mesh *, const double *U, const double *B,
 void orc_tdma(int n, const double *a, const double *b, const double *c,
               const double *d, double *x);
 
+/* ---- compact 6th-order scheme (no reference counterpart: the reference
+ * computes only the explicit stencils above).  Lele (1992), J. Comput.
+ * Phys. 103:16-42, tridiagonal schemes with alpha = 1/3, a = 14/9, b = 1/9
+ * (first derivative) and alpha = 2/11, a = 12/11, b = 3/11 (second
+ * derivative), on periodic lines. ---- */
+/* Cyclic tridiagonal solve with constant sub-diagonal a, diagonal b,
+ * super-diagonal c (corners: row 0 col n-1 = a, row n-1 col 0 = c). */
+void orc_cyclic_tridiag(int n, double a, double b, double c, const double *d, double *x);
+/* order 1: k'h = (a sin kh + b/2 sin 2kh) / (1 + 2 alpha cos kh);
+ * order 2: k''^2 h^2 = (2a (1 - cos kh) + b/2 (1 - cos 2kh)) / (1 + 2 alpha cos kh). */
+double orc_compact_kmod(double kh, int order);
+/* out = compact derivative (order 1 or 2) of every component of the padded
+ * periodic field f along direction dir (0, 1, 2); ghosts of out are filled. */
+void orc_compact_deriv(const orc_mesh *, const double *f, int nc, int dir, int order, double *out);
+/* Poisson solver whose k'^2 is the compact first-derivative wavenumber
+ * squared, the symbol of (compact div)(compact grad). */
+orc_poisson *orc_poisson_create_compact(const orc_mesh *);
+/* R_c = -1/2 sum_d [D_d(U_d U_c) + U_d D_d U_c + U_c D_d U_d] + nu sum_d D2_d U_c */
+void orc_compact_ns_rhs(const orc_mesh *, const double *U, double nu, double *out);
+/* F <- F - s D p with (compact div)(D p) = (compact div F) / s. */
+void orc_compact_project(const orc_mesh *, orc_poisson *, double *F, double *p, double s);
+/* SSP-RK3 step with the compact right-hand side and one compact projection. */
+void orc_compact_ns_step(const orc_mesh *, orc_poisson *, double *U, double *Ustar, double *p,
+                         double nu, double dt);
+/* max |compact div U| over the unique nodes. */
+double orc_compact_maxabs_div(const orc_mesh *, const double *U);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus

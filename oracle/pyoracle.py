@@ -64,6 +64,14 @@ def _load() -> C.CDLL:
         "orc_mhd_diag": ([_P, _D, _D, C.c_double, C.c_double, _D], None),
         "orc_tdma": ([C.c_int, _D, _D, _D, _D, _D], None),
         "orc_num_threads": ([], C.c_int),
+        "orc_cyclic_tridiag": ([C.c_int, C.c_double, C.c_double, C.c_double, _D, _D], None),
+        "orc_compact_kmod": ([C.c_double, C.c_int], C.c_double),
+        "orc_compact_deriv": ([_P, _D, C.c_int, C.c_int, C.c_int, _D], None),
+        "orc_poisson_create_compact": ([_P], _P),
+        "orc_compact_ns_rhs": ([_P, _D, C.c_double, _D], None),
+        "orc_compact_project": ([_P, _P, _D, _D, C.c_double], None),
+        "orc_compact_ns_step": ([_P, _P, _D, _D, _D, C.c_double, C.c_double], None),
+        "orc_compact_maxabs_div": ([_P, _D], C.c_double),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -195,11 +203,26 @@ class OMesh:
         lib().orc_mhd_diag(self._h, _d(U), _d(B), mu, eta, _d(out))
         return out
 
+    # compact 6th-order scheme
+    def compact_deriv(self, f, direction, order=1):
+        out = self.field(f.shape[-1])
+        lib().orc_compact_deriv(self._h, _d(f), f.shape[-1], direction, order, _d(out))
+        return out
+
+    def compact_ns_rhs(self, U, nu):
+        out = self.field(3)
+        lib().orc_compact_ns_rhs(self._h, _d(U), nu, _d(out))
+        return out
+
+    def compact_maxabs_div(self, U):
+        return lib().orc_compact_maxabs_div(self._h, _d(U))
+
 
 class OPoisson:
-    def __init__(self, mesh: OMesh):
+    def __init__(self, mesh: OMesh, compact: bool = False):
         self.mesh = mesh
-        self._h = lib().orc_poisson_create(mesh._h)
+        self.compact = compact
+        self._h = (lib().orc_poisson_create_compact if compact else lib().orc_poisson_create)(mesh._h)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -222,8 +245,14 @@ class OPoisson:
     def project(self, F, s, p=None):
         if p is None:
             p = self.mesh.field(1)
-        lib().orc_project(self.mesh._h, self._h, _d(F), _d(p), s)
+        fn = lib().orc_compact_project if self.compact else lib().orc_project
+        fn(self.mesh._h, self._h, _d(F), _d(p), s)
         return p
+
+    def compact_ns_step(self, U, p, nu, dt, Ustar=None):
+        if Ustar is None:
+            Ustar = self.mesh.field(3)
+        lib().orc_compact_ns_step(self.mesh._h, self._h, _d(U), _d(Ustar), _d(p), nu, dt)
 
     def ns_step(self, U, p, nu, dt, mode, Ustar=None):
         if Ustar is None:
@@ -260,6 +289,17 @@ def tdma(a, b, c, d):
     arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, c, d)]
     lib().orc_tdma(n, *[_d(v) for v in arrs], _d(x))
     return x
+
+
+def cyclic_tridiag(a, b, c, d):
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    x = np.zeros_like(d)
+    lib().orc_cyclic_tridiag(len(d), a, b, c, _d(d), _d(x))
+    return x
+
+
+def compact_kmod(kh, order=1):
+    return lib().orc_compact_kmod(float(kh), int(order))
 
 
 def num_threads() -> int:
