@@ -54,6 +54,10 @@ enum {
 };
 
 enum { NOVA_MODEL_NS = 0, NOVA_MODEL_MHD = 1 };
+/* Derivative schemes: the reference's explicit central stencils of order m
+ * (src/ex), or the compact sixth-order tridiagonal scheme (no reference
+ * counterpart; uniform periodic grids). */
+enum { NOVA_SCHEME_EXPLICIT = 0, NOVA_SCHEME_COMPACT6 = 1 };
 
 /* Reference BC kinds for the padded-layout fill (src/BC/BCenums.h:8). */
 enum { NOVA_PBC_PERIODIC = 0, NOVA_PBC_NEUMANN = 1, NOVA_PBC_DIRICHLET = 2,
@@ -160,6 +164,13 @@ int nova_poisson_solve(nova_poisson *ps, const double *rhs, double *p);
  * (app/TG_Piosson/RKLoop.H:30-38 with s = dt; app/OT_Poisson/BEqn.H:37-41
  * with s = 1).  p (scalar, may be NULL) receives the potential. */
 int nova_project(nova_poisson *ps, double *F, double s, double *p);
+/* Derivative scheme of the projection (NOVA_SCHEME_*): compact divergence
+ * and gradient passes and the compact modified wavenumbers in the solve. */
+int nova_poisson_set_scheme(nova_poisson *ps, int scheme);
+/* Compact sixth-order derivative (order 1 or 2) along direction dir of each
+ * of ncomp components (no reference counterpart; Lele's tridiagonal scheme
+ * with alpha = 1/3 resp. 2/11, solved per line by a wavefront). */
+int nova_compact_deriv(nova_ctx *ctx, const double *f, int ncomp, int dir, int order, double *out);
 
 /* ---- time stepping (the app-level RK loops) ---- */
 int nova_solver_create(nova_poisson *ps, int model, int mode, double nu, double eta,
@@ -172,6 +183,10 @@ int nova_solver_step(nova_solver *sv, double dt, int nsteps, double *U, double *
 /* Record the step sequence as a HIP graph and replay it (1) or launch
  * kernels one by one (0). */
 int nova_solver_use_graph(nova_solver *sv, int enable);
+/* Derivative scheme of the solver's right-hand side (NOVA_SCHEME_*); the
+ * compact scheme needs the NS model, SSP-RK3 and a Poisson solver set to the
+ * same scheme. */
+int nova_solver_set_scheme(nova_solver *sv, int scheme);
 /* Autotuner (no reference counterpart): times every launch shape of the
  * solver's stage kernel (variants x chunks of 0, 8, 16, 32, 64 planes) and
  * both projection variants on U (and B), reps launches each after one

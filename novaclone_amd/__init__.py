@@ -27,6 +27,7 @@ BC_PERIODIC, BC_WALL = 0, 1
 RK_LITERAL4, RK_SSP3, RK_SHEAR4 = 0, 1, 2
 MODEL_NS, MODEL_MHD = 0, 1
 KERNEL_NS_STAGE, KERNEL_MHD_STAGE, KERNEL_TLM_STAGE, KERNEL_ADJ_STAGE = 0, 1, 2, 3
+SCHEME_EXPLICIT, SCHEME_COMPACT6 = 0, 1
 # reference BC kinds for the padded-layout fill (src/BC/BCenums.h:8)
 PBC_PERIODIC, PBC_NEUMANN, PBC_DIRICHLET, PBC_EVEN, PBC_ODD = 0, 1, 2, 10, 11
 DIRS = {"north": 0, "south": 1, "east": 2, "west": 3, "top": 4, "bottom": 5}
@@ -92,6 +93,9 @@ _SIGS = {
     "nova_solver_destroy": [_P],
     "nova_solver_step": [_P, _D, _I, _P, _P, _P, _P],
     "nova_solver_use_graph": [_P, _I],
+    "nova_solver_set_scheme": [_P, _I],
+    "nova_poisson_set_scheme": [_P, _I],
+    "nova_compact_deriv": [_P, _P, _I, _I, _I, _P],
     "nova_solver_autotune": [_P, _P, _P, _I, C.c_char_p, C.c_size_t],
     "nova_reduce_ke": [_P, _P, _DP],
     "nova_reduce_maxabs_div": [_P, _P, _DP],
@@ -307,15 +311,27 @@ class ex:
         return out
 
 
+def compact_deriv(mesh: "Mesh", f: torch.Tensor, direction: int, order: int = 1) -> torch.Tensor:
+    """Compact sixth-order derivative (order 1 or 2) of every component of f
+    along a direction (nova_compact_deriv)."""
+    out = torch.empty_like(f)
+    _check(lib().nova_compact_deriv(mesh.handle, _ptr(f), f.shape[0], int(direction), int(order), _ptr(out)),
+           "nova_compact_deriv")
+    return out
+
+
 class Poisson:
     """FFT (and wall-y DCT) Poisson solver (reference ``poisson``,
     src/poisson/poisson.cpp)."""
 
-    def __init__(self, mesh: Mesh):
+    def __init__(self, mesh: Mesh, scheme: int = SCHEME_EXPLICIT):
         self.mesh = mesh
         h = C.c_void_p()
         _check(lib().nova_poisson_create(mesh.handle, C.byref(h)), "nova_poisson_create")
         self._h = h
+        self.scheme = scheme
+        if scheme != SCHEME_EXPLICIT:
+            _check(lib().nova_poisson_set_scheme(h, int(scheme)), "nova_poisson_set_scheme")
 
     @property
     def handle(self):
@@ -355,6 +371,8 @@ class Solver:
                "nova_solver_create")
         self._h = h
         self.model = model
+        if poisson.scheme != SCHEME_EXPLICIT:
+            _check(lib().nova_solver_set_scheme(h, int(poisson.scheme)), "nova_solver_set_scheme")
         if graph:
             _check(lib().nova_solver_use_graph(h, 1), "nova_solver_use_graph")
 

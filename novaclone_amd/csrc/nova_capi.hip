@@ -99,6 +99,8 @@ int solver_destroy(nova_solver *sv);
 int solver_step(nova_solver *sv, double dt, int nsteps, double *U, double *B, double *p, double *pB);
 int solver_use_graph(nova_solver *sv, int enable);
 int solver_autotune(nova_solver *sv, const double *U, const double *B, int reps, char *report, size_t len);
+int solver_set_scheme(nova_solver *sv, int scheme);
+int poisson_set_scheme(nova_poisson *ps, int scheme);
 nova_ctx *poisson_ctx(nova_poisson *ps);
 int adjoint_create(nova_solver *sv, int every, int max_steps, nova_adjoint **out);
 int adjoint_destroy(nova_adjoint *ad);
@@ -509,6 +511,24 @@ int nova_solver_use_graph(nova_solver *sv, int enable)
 {
     NOVA_REQUIRE(sv, "null solver");
     NOVA_GUARD(solver_use_graph(sv, enable));
+}
+
+int nova_solver_set_scheme(nova_solver *sv, int scheme)
+{
+    NOVA_REQUIRE(sv, "null solver");
+    NOVA_GUARD(solver_set_scheme(sv, scheme));
+}
+
+int nova_poisson_set_scheme(nova_poisson *ps, int scheme)
+{
+    NOVA_REQUIRE(ps, "null Poisson solver");
+    NOVA_GUARD(poisson_set_scheme(ps, scheme));
+}
+
+int nova_compact_deriv(nova_ctx *ctx, const double *f, int ncomp, int dir, int order, double *out)
+{
+    NOVA_REQUIRE(ctx && f && out && ncomp > 0, "bad argument");
+    NOVA_GUARD(compact_deriv(ctx, f, ncomp, dir, order, out));
 }
 
 int nova_solver_autotune(nova_solver *sv, const double *U, const double *B, int reps, char *report,

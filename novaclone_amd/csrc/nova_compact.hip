@@ -1,0 +1,476 @@
+// nova_compact.hip -- compact sixth-order derivatives on periodic lines (no
+// reference counterpart: the reference computes only the explicit central
+// stencils of src/ex; this is the tridiagonal scheme of Lele (1992),
+// J. Comput. Phys. 103:16-42, first derivative alpha = 1/3, a = 14/9,
+// b = 1/9; second derivative alpha = 2/11, a = 12/11, b = 3/11).
+//
+// Line solver.  A = tridiag(alpha, 1, alpha) (cyclic) factors exactly as
+// A = kappa (1 + r E^-1)(1 + r E) with alpha r^2 - r + alpha = 0, |r| < 1,
+// kappa = alpha / r (E = shift).  A x = rhs is then two first-order
+// recursions, w_i = rhs_i / kappa - r w_{i-1} and x_i = w_i - r x_{i+1}, whose
+// periodic solutions are geometric sums.  One wavefront owns a line: lane p
+// holds the MP consecutive points [p MP, (p+1) MP) in registers, runs each
+// recursion over its chunk (Thomas), and obtains the carry into its chunk
+// from a cyclic Kogge-Stone scan across the P = N / MP lanes with
+// multiplier rho = (-r)^MP (partition method, PCR-like across lanes); the
+// scan stops once rho^(2^K) is below 1e-20, so the result is the exact
+// periodic solution to rounding.
+//
+// Pencil passes.  A pass computes, for every line along one direction, the
+// compact derivatives a model needs and combines them pointwise.  Lines
+// along k are contiguous and read straight into registers; lines along i and
+// j are staged through an LDS tile of TL consecutive k values, stored so
+// that each lane's chunk element j sits at j P + lane (conflict-free reads).
+#include "nova_internal.hpp"
+#include "nova_stencil.hpp"
+
+#include <cmath>
+
+struct nova_poisson;
+
+namespace nova {
+
+namespace {
+
+constexpr int kScanMax = 6;
+
+// One compact scheme along one direction.
+struct CmpScheme {
+    double r;               // factor root
+    double ca, cb;          // right-hand-side weights, divided by kappa
+    double rp[9];           // (-r)^q, q = 0..8
+    double rho2[kScanMax];  // rho^(2^s), rho = (-r)^MP
+    int K;                  // scan steps
+};
+
+struct CmpDir {
+    CmpScheme s[2];  // first and second derivative
+    int N, P;        // line length and lanes per line
+};
+
+constexpr double kAl[2] = {1.0 / 3.0, 2.0 / 11.0};
+constexpr double kA[2] = {14.0 / 9.0, 12.0 / 11.0};
+constexpr double kB[2] = {1.0 / 9.0, 3.0 / 11.0};
+
+int pick_mp(int n)
+{
+    if (n <= 64) return 1;
+    if (n <= 128) return 2;
+    if (n <= 256) return 4;
+    if (n <= 512) return 8;
+    return 0;
+}
+
+CmpDir make_dir(int n, double h, int mp)
+{
+    CmpDir D{};
+    D.N = n;
+    D.P = n / mp;
+    for (int o = 0; o < 2; o++) {
+        CmpScheme &S = D.s[o];
+        const double al = kAl[o];
+        S.r = (1.0 - std::sqrt(1.0 - 4.0 * al * al)) / (2.0 * al);
+        const double kappa = al / S.r;
+        if (o == 0) {
+            S.ca = kA[o] / (2.0 * h) / kappa;
+            S.cb = kB[o] / (4.0 * h) / kappa;
+        } else {
+            S.ca = kA[o] / (h * h) / kappa;
+            S.cb = kB[o] / (4.0 * h * h) / kappa;
+        }
+        double q = 1.0;
+        for (int e = 0; e <= 8; e++) {
+            S.rp[e] = q;
+            q *= -S.r;
+        }
+        const double rho = S.rp[mp];
+        double rr = rho;
+        S.K = 0;
+        for (int s = 0; s < kScanMax; s++) {
+            S.rho2[s] = rr;
+            if (std::fabs(rr) > 1e-20) S.K = s + 1;
+            rr *= rr;
+        }
+    }
+    return D;
+}
+
+// ---- wavefront line algebra ----
+
+template <int MP>
+__device__ inline void cmp_solve(const CmpScheme &S, int P, int lane, double (&x)[MP])
+{
+    const double r = S.r;
+    double w[MP];
+    w[0] = x[0];
+#pragma unroll
+    for (int j = 1; j < MP; j++) w[j] = x[j] - r * w[j - 1];
+    double acc = w[MP - 1];
+    for (int s = 0; s < S.K; s++) {
+        const int d = (1 << s) % P;
+        const int src = lane - d < 0 ? lane - d + P : lane - d;
+        const double o = __shfl(acc, src, 64);
+        acc = acc + S.rho2[s] * o;
+    }
+    const double cin = __shfl(acc, lane == 0 ? P - 1 : lane - 1, 64);
+#pragma unroll
+    for (int j = 0; j < MP; j++) w[j] = w[j] + S.rp[j + 1] * cin;
+    x[MP - 1] = w[MP - 1];
+#pragma unroll
+    for (int j = MP - 2; j >= 0; j--) x[j] = w[j] - r * x[j + 1];
+    acc = x[0];
+    for (int s = 0; s < S.K; s++) {
+        const int d = (1 << s) % P;
+        const int src = lane + d >= P ? lane + d - P : lane + d;
+        const double o = __shfl(acc, src, 64);
+        acc = acc + S.rho2[s] * o;
+    }
+    const double cin2 = __shfl(acc, lane + 1 >= P ? lane + 1 - P : lane + 1, 64);
+#pragma unroll
+    for (int j = 0; j < MP; j++) x[j] = x[j] + S.rp[MP - j] * cin2;
+}
+
+// right-hand side of the compact scheme (ORDER 1 or 2) from the chunk f
+template <int MP, int ORDER>
+__device__ inline void cmp_rhs(const CmpScheme &S, int P, int lane, const double (&f)[MP], double (&e)[MP])
+{
+    double ext[MP + 4];
+#pragma unroll
+    for (int j = 0; j < MP; j++) ext[j + 2] = f[j];
+    const int lm1 = lane == 0 ? P - 1 : lane - 1;
+    const int lp1 = lane + 1 >= P ? lane + 1 - P : lane + 1;
+    if constexpr (MP >= 2) {
+        ext[1] = __shfl(f[MP - 1], lm1, 64);
+        ext[0] = __shfl(f[MP - 2], lm1, 64);
+        ext[MP + 2] = __shfl(f[0], lp1, 64);
+        ext[MP + 3] = __shfl(f[1], lp1, 64);
+    } else {
+        const int lm2 = lane - 2 < 0 ? lane - 2 + P : lane - 2;
+        const int lp2 = lane + 2 >= P ? lane + 2 - P : lane + 2;
+        ext[1] = __shfl(f[0], lm1, 64);
+        ext[0] = __shfl(f[0], lm2, 64);
+        ext[3] = __shfl(f[0], lp1, 64);
+        ext[4] = __shfl(f[0], lp2, 64);
+    }
+#pragma unroll
+    for (int j = 0; j < MP; j++) {
+        if constexpr (ORDER == 1)
+            e[j] = S.ca * (ext[j + 3] - ext[j + 1]) + S.cb * (ext[j + 4] - ext[j]);
+        else
+            e[j] = S.ca * ((ext[j + 3] - 2.0 * ext[j + 2]) + ext[j + 1]) +
+                   S.cb * ((ext[j + 4] - 2.0 * ext[j + 2]) + ext[j]);
+    }
+}
+
+template <int MP, int ORDER>
+__device__ inline void cmp_deriv(const CmpDir &D, int lane, const double (&f)[MP], double (&out)[MP])
+{
+    cmp_rhs<MP, ORDER>(D.s[ORDER - 1], D.P, lane, f, out);
+    cmp_solve<MP>(D.s[ORDER - 1], D.P, lane, out);
+}
+
+// ---- pass policies: inputs differentiated along the pass direction and
+// the pointwise use of the results ----
+
+// out = D^order f (one component)
+struct DerivW {
+    static constexpr int NI = 1, NO = 1;
+    struct Args {
+        const double *f;
+        double *out;
+        int order;
+    };
+    __device__ static const double *input(const Args &a, int) { return a.f; }
+    template <int MP, int DIR>
+    __device__ static void compute(const Args &a, const CmpDir &D, int lane, double (&in)[NI][MP],
+                                   double (&out)[NO][MP])
+    {
+        if (a.order == 1) cmp_deriv<MP, 1>(D, lane, in[0], out[0]);
+        else cmp_deriv<MP, 2>(D, lane, in[0], out[0]);
+    }
+    __device__ static void store(const Args &a, const Grid &, int, int64_t q, double v, double) { a.out[q] = v; }
+};
+
+// Navier-Stokes right-hand side along one direction d:
+//   c_d = -1/2 [D_d(U_d U_c) + U_d D_d U_c + U_c D_d U_d] + nu D2_d U_c,
+// mode 0: R = c_d; 1: R += c_d; 2: out = stage combination with dt (R + c_d).
+struct NsW {
+    static constexpr int NI = 3, NO = 3;
+    struct Args {
+        const double *U;
+        const double *U0;
+        double *R;
+        double *out;
+        StageArgs sa;
+        double nu;
+        int mode;
+    };
+    __device__ static const double *input(const Args &a, int c) { return a.U; }
+    template <int MP, int DIR>
+    __device__ static void compute(const Args &a, const CmpDir &D, int lane, double (&u)[NI][MP],
+                                   double (&out)[NO][MP])
+    {
+        double dud[MP];
+        cmp_deriv<MP, 1>(D, lane, u[DIR], dud);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double duc[MP], fl[MP], dfl[MP], l2[MP];
+            if (c == DIR) {
+#pragma unroll
+                for (int j = 0; j < MP; j++) duc[j] = dud[j];
+            } else {
+                cmp_deriv<MP, 1>(D, lane, u[c], duc);
+            }
+#pragma unroll
+            for (int j = 0; j < MP; j++) fl[j] = u[DIR][j] * u[c][j];
+            cmp_deriv<MP, 1>(D, lane, fl, dfl);
+            cmp_deriv<MP, 2>(D, lane, u[c], l2);
+#pragma unroll
+            for (int j = 0; j < MP; j++)
+                out[c][j] = -0.5 * ((dfl[j] + u[DIR][j] * duc[j]) + u[c][j] * dud[j]) + a.nu * l2[j];
+        }
+    }
+    __device__ static void store(const Args &a, const Grid &g, int o, int64_t q, double v, double x)
+    {
+        const int64_t e = o * g.np + q;
+        if (a.mode == 0) {
+            a.R[e] = v;
+        } else if (a.mode == 1) {
+            a.R[e] = a.R[e] + v;
+        } else {
+            const double du = a.sa.dt * (a.R[e] + v);
+            const double x0 = a.U0 ? a.U0[e] : x;
+            a.out[e] = combine(a.sa, x, x0, du);
+        }
+    }
+};
+
+// divergence along one direction: mode 0: d = D F_d; 1: d += D F_d;
+// 2: d = (d + D F_d) * scale
+struct DivW {
+    static constexpr int NI = 1, NO = 1;
+    struct Args {
+        const double *F;  // the pass direction's component
+        double *d;
+        double scale;
+        int mode;
+    };
+    __device__ static const double *input(const Args &a, int) { return a.F; }
+    template <int MP, int DIR>
+    __device__ static void compute(const Args &, const CmpDir &D, int lane, double (&in)[NI][MP],
+                                   double (&out)[NO][MP])
+    {
+        cmp_deriv<MP, 1>(D, lane, in[0], out[0]);
+    }
+    __device__ static void store(const Args &a, const Grid &, int, int64_t q, double v, double)
+    {
+        if (a.mode == 0) a.d[q] = v;
+        else if (a.mode == 1) a.d[q] = a.d[q] + v;
+        else a.d[q] = (a.d[q] + v) * a.scale;
+    }
+};
+
+// gradient correction along one direction: F_d -= s D_d p
+struct GradW {
+    static constexpr int NI = 1, NO = 1;
+    struct Args {
+        const double *p;
+        double *F;  // the pass direction's component
+        double s;
+    };
+    __device__ static const double *input(const Args &a, int) { return a.p; }
+    template <int MP, int DIR>
+    __device__ static void compute(const Args &, const CmpDir &D, int lane, double (&in)[NI][MP],
+                                   double (&out)[NO][MP])
+    {
+        cmp_deriv<MP, 1>(D, lane, in[0], out[0]);
+    }
+    __device__ static void store(const Args &a, const Grid &, int, int64_t q, double v, double)
+    {
+        a.F[q] = a.F[q] - a.s * v;
+    }
+};
+
+template <int MP>
+constexpr int tile_lines() { return MP <= 4 ? 8 : 4; }
+
+// Lines along k: one wavefront per (i, j) line, no LDS.
+template <int MP, class W>
+__global__ void __launch_bounds__(256) k_cmp_pass_k(Grid g, CmpDir D, typename W::Args a)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t line = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (line >= (int64_t)g.s[0] * g.s[1]) return;  // whole wavefronts leave; no block barriers below
+    const int64_t base = line * g.s[2];
+    const bool on = lane < D.P;
+    double in[W::NI][MP], out[W::NO][MP];
+#pragma unroll
+    for (int c = 0; c < W::NI; c++) {
+        const double *f = W::input(a, c) + c * g.np * (W::NI == 3 ? 1 : 0) + base;
+#pragma unroll
+        for (int j = 0; j < MP; j++) in[c][j] = on ? f[lane * MP + j] : 0.0;
+    }
+    W::template compute<MP, 2>(a, D, lane, in, out);
+    if (!on) return;
+#pragma unroll
+    for (int o = 0; o < W::NO; o++)
+#pragma unroll
+        for (int j = 0; j < MP; j++) W::store(a, g, o, base + lane * MP + j, out[o][j], in[W::NI == 3 ? o : 0][j]);
+}
+
+// Lines along i (DIR 0) or j (DIR 1): a block stages TL consecutive k
+// lines through LDS; each wavefront solves TL / 4 of them.
+template <int DIR, int MP, class W>
+__global__ void __launch_bounds__(256) k_cmp_pass_t(Grid g, CmpDir D, typename W::Args a)
+{
+    constexpr int TL = tile_lines<MP>();
+    constexpr int NB = W::NI > W::NO ? W::NI : W::NO;
+    constexpr int NMAX = 64 * MP;
+    __shared__ double sh[NB * TL * NMAX];
+    const int N = D.N, P = D.P;
+    const int nkt = (g.s[2] + TL - 1) / TL;
+    const int fixed = blockIdx.x / nkt;  // i for DIR 1, j for DIR 0
+    const int k0 = (blockIdx.x % nkt) * TL;
+    const int64_t lstride = DIR == 0 ? (int64_t)g.s[1] * g.s[2] : g.s[2];  // step along the line
+    const int64_t base = (DIR == 0 ? (int64_t)fixed * g.s[2] : (int64_t)fixed * g.s[1] * g.s[2]) + k0;
+    auto slot = [&](int c, int kk, int q) { return (c * TL + kk) * NMAX + (q % MP) * P + q / MP; };
+    for (int e = threadIdx.x; e < W::NI * N * TL; e += 256) {
+        const int c = e / (N * TL), rem = e % (N * TL);
+        const int q = rem / TL, kk = rem % TL;
+        double v = 0.0;
+        if (k0 + kk < g.s[2]) v = (W::input(a, c) + c * g.np * (W::NI == 3 ? 1 : 0))[base + q * lstride + kk];
+        sh[slot(c, kk, q)] = v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool on = lane < P;
+    for (int kk = wave; kk < TL; kk += 4) {
+        double in[W::NI][MP], out[W::NO][MP];
+#pragma unroll
+        for (int c = 0; c < W::NI; c++)
+#pragma unroll
+            for (int j = 0; j < MP; j++) in[c][j] = on ? sh[(c * TL + kk) * NMAX + j * P + lane] : 0.0;
+        W::template compute<MP, DIR>(a, D, lane, in, out);
+        if (on)
+#pragma unroll
+            for (int o = 0; o < W::NO; o++)
+#pragma unroll
+                for (int j = 0; j < MP; j++) sh[(o * TL + kk) * NMAX + j * P + lane] = out[o][j];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < W::NO * N * TL; e += 256) {
+        const int o = e / (N * TL), rem = e % (N * TL);
+        const int q = rem / TL, kk = rem % TL;
+        if (k0 + kk < g.s[2]) W::store(a, g, o, base + q * lstride + kk, sh[slot(o, kk, q)], 0.0);
+    }
+}
+
+template <int MP, class W>
+int launch_pass_mp(nova_ctx *ctx, int dir, const CmpDir &D, const typename W::Args &a)
+{
+    const Grid &g = ctx->g;
+    if (dir == 2) {
+        const int64_t lines = (int64_t)g.s[0] * g.s[1];
+        hipLaunchKernelGGL((k_cmp_pass_k<MP, W>), dim3((unsigned)((lines + 3) / 4)), dim3(256), 0, ctx->stream, g, D,
+                           a);
+    } else {
+        constexpr int TL = tile_lines<MP>();
+        const int nkt = (g.s[2] + TL - 1) / TL;
+        const int fixed = dir == 0 ? g.s[1] : g.s[0];
+        if (dir == 0)
+            hipLaunchKernelGGL((k_cmp_pass_t<0, MP, W>), dim3((unsigned)(fixed * nkt)), dim3(256), 0, ctx->stream, g,
+                               D, a);
+        else
+            hipLaunchKernelGGL((k_cmp_pass_t<1, MP, W>), dim3((unsigned)(fixed * nkt)), dim3(256), 0, ctx->stream, g,
+                               D, a);
+    }
+    return check_launch("compact pass");
+}
+
+template <class W>
+int launch_pass(nova_ctx *ctx, int dir, const typename W::Args &a, const char *name)
+{
+    const Grid &g = ctx->g;
+    const int n = g.s[dir];
+    const int mp = pick_mp(n);
+    if (!mp || n % mp || n / mp > 64 || n < 4) return fail(NOVA_ERR_UNSUPPORTED, "compact line length");
+    const CmpDir D = make_dir(n, ctx->d[dir], mp);
+    TimedScope ts(ctx, name);
+    switch (mp) {
+    case 1: return launch_pass_mp<1, W>(ctx, dir, D, a);
+    case 2: return launch_pass_mp<2, W>(ctx, dir, D, a);
+    case 4: return launch_pass_mp<4, W>(ctx, dir, D, a);
+    default: return launch_pass_mp<8, W>(ctx, dir, D, a);
+    }
+}
+
+}  // namespace
+
+// Host side: the compact scheme is defined on uniform periodic grids.
+int compact_supported(nova_ctx *ctx)
+{
+    const Grid &g = ctx->g;
+    if (g.wall[0] || g.wall[1] || g.wall[2] || ctx->desc.hyp)
+        return fail(NOVA_ERR_UNSUPPORTED, "the compact scheme needs a uniform periodic grid");
+    for (int d = 0; d < 3; d++) {
+        const int n = g.s[d], mp = pick_mp(n);
+        if (!mp || n % mp || n / mp > 64 || n < 4)
+            return fail(NOVA_ERR_UNSUPPORTED, "compact lines need N = MP * P with MP in {1,2,4,8}, P <= 64");
+    }
+    return NOVA_OK;
+}
+
+double compact_kmod(double kh, int order)
+{
+    const int o = order - 1;
+    if (order == 1) return (kA[o] * std::sin(kh) + 0.5 * kB[o] * std::sin(2.0 * kh)) / (1.0 + 2.0 * kAl[o] * std::cos(kh));
+    return (2.0 * kA[o] * (1.0 - std::cos(kh)) + 0.5 * kB[o] * (1.0 - std::cos(2.0 * kh))) /
+           (1.0 + 2.0 * kAl[o] * std::cos(kh));
+}
+
+int compact_deriv(nova_ctx *ctx, const double *f, int nc, int dir, int order, double *out)
+{
+    if (int rc = compact_supported(ctx)) return rc;
+    if (dir < 0 || dir > 2 || (order != 1 && order != 2)) return fail(NOVA_ERR_ARG, "direction 0..2, order 1 or 2");
+    for (int c = 0; c < nc; c++) {
+        DerivW::Args a{f + c * ctx->g.np, out + c * ctx->g.np, order};
+        if (int rc = launch_pass<DerivW>(ctx, dir, a, "compact_deriv")) return rc;
+    }
+    return NOVA_OK;
+}
+
+// One SSP-type stage out = combine(U, U0, dt R(U)) with the compact NS
+// right-hand side, accumulated in R over passes i, j and finished in the k
+// pass (which has U in registers).
+int compact_ns_stage(nova_ctx *ctx, const double *U, const double *U0, double *out, const StageArgs &sa, double nu,
+                     double *R)
+{
+    NsW::Args a{U, U0, R, out, sa, nu, 0};
+    if (int rc = launch_pass<NsW>(ctx, 0, a, "compact_ns_i")) return rc;
+    a.mode = 1;
+    if (int rc = launch_pass<NsW>(ctx, 1, a, "compact_ns_j")) return rc;
+    a.mode = 2;
+    return launch_pass<NsW>(ctx, 2, a, "compact_ns_k");
+}
+
+// d = (compact div F) * scale
+int compact_div(nova_ctx *ctx, const double *F, double scale, double *d)
+{
+    for (int dir = 0; dir < 3; dir++) {
+        DivW::Args a{F + dir * ctx->g.np, d, scale, dir};
+        if (int rc = launch_pass<DivW>(ctx, dir, a, "compact_div")) return rc;
+    }
+    return NOVA_OK;
+}
+
+// F -= s (compact grad p)
+int compact_grad_correct(nova_ctx *ctx, const double *p, double s, double *F)
+{
+    for (int dir = 0; dir < 3; dir++) {
+        GradW::Args a{p, F + dir * ctx->g.np, s};
+        if (int rc = launch_pass<GradW>(ctx, dir, a, "compact_grad_correct")) return rc;
+    }
+    return NOVA_OK;
+}
+
+}  // namespace nova

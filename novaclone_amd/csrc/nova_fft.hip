@@ -51,6 +51,7 @@ struct nova_poisson {
     double *ptmp = nullptr;  // potential when the caller passes none
     double *dtmp = nullptr;  // divergence scratch of the split projection
     int batch_cap = 1;       // fields one batched solve can take
+    int scheme = NOVA_SCHEME_EXPLICIT;  // derivative scheme of div, grad and k'
 };
 
 namespace nova {
@@ -905,6 +906,38 @@ int poisson_create(nova_ctx *ctx, nova_poisson **out, int batch_cap)
     return NOVA_OK;
 }
 
+// Select the derivative scheme: the modified wavenumbers k' of the solve
+// become those of the compact first derivative, k' h = (a sin kh +
+// b/2 sin 2kh) / (1 + 2 alpha cos kh), so that the solve inverts (compact
+// div)(compact grad) exactly.
+int poisson_set_scheme(nova_poisson *ps, int scheme)
+{
+    nova_ctx *ctx = ps->ctx;
+    if (scheme != NOVA_SCHEME_EXPLICIT && scheme != NOVA_SCHEME_COMPACT6) return fail(NOVA_ERR_ARG, "unknown scheme");
+    if (scheme == NOVA_SCHEME_COMPACT6)
+        if (int rc = compact_supported(ctx)) return rc;
+    const Grid &g = ctx->g;
+    const double pi = 3.14159265358979323846;
+    std::vector<double> kx(ps->Nx), ky(ps->sy), kz(ps->Kc);
+    const double Ly = ps->wall_y ? 2.0 * ctx->desc.L[1] : ctx->desc.L[1];
+    auto km = [&](double k, int d) {
+        return scheme == NOVA_SCHEME_COMPACT6 ? compact_kmod(k * ctx->d[d], 1) / ctx->d[d]
+                                              : kmod(k, ctx->d[d], ctx->coef, g.h);
+    };
+    for (int i = 0; i < ps->Nx; i++)
+        kx[i] = km((2 * i < ps->Nx) ? 2.0 * pi * i / ctx->desc.L[0] : 2.0 * pi * (ps->Nx - i) / ctx->desc.L[0], 0);
+    for (int j = 0; j < ps->sy; j++)
+        ky[j] = km((2 * j < ps->Ty) ? 2.0 * pi * j / Ly : 2.0 * pi * (ps->Ty - j) / Ly, 1);
+    for (int k = 0; k < ps->Kc; k++) kz[k] = km(2.0 * pi * k / ctx->desc.L[2], 2);
+    NOVA_HIP(hipMemcpy(ps->kpx, kx.data(), sizeof(double) * ps->Nx, hipMemcpyHostToDevice));
+    NOVA_HIP(hipMemcpy(ps->kpy, ky.data(), sizeof(double) * ps->sy, hipMemcpyHostToDevice));
+    NOVA_HIP(hipMemcpy(ps->kpz, kz.data(), sizeof(double) * ps->Kc, hipMemcpyHostToDevice));
+    ps->scheme = scheme;
+    return NOVA_OK;
+}
+
+int poisson_scheme(nova_poisson *ps) { return ps->scheme; }
+
 int poisson_destroy(nova_poisson *ps)
 {
     if (!ps) return NOVA_OK;
@@ -995,6 +1028,13 @@ int project(nova_poisson *ps, double *F, double s, double *p)
         if (!ps->ptmp) NOVA_HIP(hipMalloc((void **)&ps->ptmp, sizeof(double) * ctx->g.np));
         p = ps->ptmp;
     }
+    if (ps->scheme == NOVA_SCHEME_COMPACT6) {
+        // compact divergence passes, the transform pipeline on the scalar,
+        // compact gradient-correction passes
+        if (int rc = compact_div(ctx, F, 1.0 / s, ps->dtmp)) return rc;
+        if (int rc = run_pipeline(ps, ps->dtmp, 0, 1.0, p, 1, BatchArgs())) return rc;
+        return compact_grad_correct(ctx, p, s, F);
+    }
     if (int rc = poisson_run(ps, F, 1, 1.0 / s, p)) return rc;
     const Grid &g = ctx->g;
     dim3 grid((g.s[2] + 63) / 64, (g.s[1] + 3) / 4, g.s[0]);
@@ -1015,6 +1055,7 @@ int project_batch(nova_poisson *ps, double *F, int64_t f_bs, int B, int split, d
 {
     nova_ctx *ctx = ps->ctx;
     if (B < 1 || B > ps->batch_cap) return fail(NOVA_ERR_ARG, "batch exceeds the Poisson batch capacity");
+    if (ps->scheme != NOVA_SCHEME_EXPLICIT) return fail(NOVA_ERR_UNSUPPORTED, "batched projections are explicit");
     BatchArgs ba;
     ba.f_bs = f_bs;
     ba.p_bs = p_bs;

@@ -211,5 +211,11 @@ int reduce_mhd(nova_ctx *ctx, const double *U, const double *B, const double *J,
                const double *W, double mu, double eta, double out[8]);
 int launch_tdma(nova_ctx *ctx, int n, int nb, const double *a, const double *b, const double *c,
                 const double *d, double *x, int cyclic);
+// compact sixth-order scheme (nova_compact.hip)
+int compact_supported(nova_ctx *ctx);
+double compact_kmod(double kh, int order);
+int compact_deriv(nova_ctx *ctx, const double *f, int nc, int dir, int order, double *out);
+int compact_div(nova_ctx *ctx, const double *F, double scale, double *d);
+int compact_grad_correct(nova_ctx *ctx, const double *p, double s, double *F);
 
 }  // namespace nova

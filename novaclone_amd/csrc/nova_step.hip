@@ -15,6 +15,9 @@ struct nova_poisson;
 
 namespace nova {
 int poisson_run(nova_poisson *ps, const double *src, int src_is_vector, double scale, double *p);
+int poisson_scheme(nova_poisson *ps);
+int compact_ns_stage(nova_ctx *ctx, const double *U, const double *U0, double *out, const StageArgs &sa, double nu,
+                     double *R);
 int project(nova_poisson *ps, double *F, double s, double *p);
 nova_ctx *poisson_ctx(nova_poisson *ps);
 }  // namespace nova
@@ -27,6 +30,8 @@ struct nova_solver {
     double *wa = nullptr, *wb = nullptr, *wc = nullptr, *wd = nullptr;  // stage buffers (3 comps each)
     double *div_tmp = nullptr;
     int use_graph = 0;
+    int scheme = NOVA_SCHEME_EXPLICIT;
+    double *rwork = nullptr;  // compact right-hand-side accumulator (3 comps)
     // replayed step graph and the arguments it was recorded with
     hipStream_t gstream = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -223,6 +228,12 @@ int ns_step_once(nova_solver *sv, double dt, double *U, double *p)
 {
     nova_ctx *ctx = sv->ctx;
     int rc;
+    if (sv->scheme == NOVA_SCHEME_COMPACT6) {
+        if ((rc = compact_ns_stage(ctx, U, nullptr, sv->wa, ssp(0, dt), sv->nu, sv->rwork))) return rc;
+        if ((rc = compact_ns_stage(ctx, sv->wa, U, sv->wb, ssp(1, dt), sv->nu, sv->rwork))) return rc;
+        if ((rc = compact_ns_stage(ctx, sv->wb, U, U, ssp(2, dt), sv->nu, sv->rwork))) return rc;
+        return project(sv->ps, U, dt, p);
+    }
     if (sv->mode == NOVA_RK_SSP3) {
         if ((rc = ns_stage(ctx, U, nullptr, sv->wa, ssp(0, dt), sv->nu))) return rc;
         if ((rc = ns_stage(ctx, sv->wa, U, sv->wb, ssp(1, dt), sv->nu))) return rc;
@@ -322,6 +333,7 @@ int solver_destroy(nova_solver *sv)
     if (sv->wc) (void)hipFree(sv->wc);
     if (sv->wd) (void)hipFree(sv->wd);
     if (sv->div_tmp) (void)hipFree(sv->div_tmp);
+    if (sv->rwork) (void)hipFree(sv->rwork);
     delete sv;
     return NOVA_OK;
 }
@@ -370,6 +382,25 @@ int solver_autotune(nova_solver *sv, const double *U, const double *B, int reps,
     ctx->tune_epoch++;
     rep += "projection chosen variant " + std::to_string(bp) + "\n";
     return write_report(rep, report, len);
+}
+
+int solver_set_scheme(nova_solver *sv, int scheme)
+{
+    if (scheme != NOVA_SCHEME_EXPLICIT && scheme != NOVA_SCHEME_COMPACT6) return fail(NOVA_ERR_ARG, "unknown scheme");
+    if (scheme == NOVA_SCHEME_COMPACT6) {
+        if (sv->model != NOVA_MODEL_NS || sv->mode != NOVA_RK_SSP3)
+            return fail(NOVA_ERR_UNSUPPORTED, "the compact scheme covers the NS model with SSP-RK3");
+        if (int rc = compact_supported(sv->ctx)) return rc;
+        if (!sv->rwork) NOVA_HIP(hipMalloc((void **)&sv->rwork, sizeof(double) * 3 * sv->ctx->g.np));
+    }
+    if (poisson_scheme(sv->ps) != scheme)
+        return fail(NOVA_ERR_ARG, "set the Poisson solver to the same scheme first (nova_poisson_set_scheme)");
+    sv->scheme = scheme;
+    if (sv->gexec) {
+        NOVA_HIP(hipGraphExecDestroy(sv->gexec));
+        sv->gexec = nullptr;
+    }
+    return NOVA_OK;
 }
 
 int solver_use_graph(nova_solver *sv, int enable)
