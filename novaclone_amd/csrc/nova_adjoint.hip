@@ -31,6 +31,13 @@ nova_ctx *poisson_ctx(nova_poisson *ps);
 int stage_ns(nova_ctx *ctx, const double *U, const double *U0, double *out, const StageArgs &sa, double nu);
 nova_poisson *solver_poisson(nova_solver *sv);
 int solver_model(nova_solver *sv, int *mode, double *nu);
+int solver_scheme(nova_solver *sv);
+int compact_ns_stage(nova_ctx *ctx, const double *U, const double *U0, double *out, const StageArgs &sa, double nu,
+                     double *R);
+int compact_tlm_stage(nova_ctx *ctx, const double *X, const double *V, const double *V0, double *out,
+                      const StageArgs &sa, double nu, double *R);
+int compact_adj_stage(nova_ctx *ctx, const double *X, const double *M, double *out, double ob, const double *Y,
+                      double yg, double *acc, double acc_a, double acc_b, double dt, double nu, double *R);
 }  // namespace nova
 
 struct nova_adjoint {
@@ -45,6 +52,8 @@ struct nova_adjoint {
     std::vector<double *> seg;    // per step of a segment: U, U1, U2
     double *w[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // work vector fields
     double *ptmp = nullptr;
+    int scheme = NOVA_SCHEME_EXPLICIT;
+    double *rwork = nullptr;  // compact right-hand-side accumulator (4 comps)
 };
 
 namespace nova {
@@ -68,15 +77,62 @@ int copy3(nova_ctx *ctx, double *dst, const double *src)
     return NOVA_OK;
 }
 
+using LinArgs = StreamArgs<2>;
+
+// forward stage out = combine(X, X0, dt R(X)) in the solver's scheme
+int fwd_stage(nova_adjoint *ad, const double *X, const double *X0, double *out, const StageArgs &sa)
+{
+    if (ad->scheme == NOVA_SCHEME_COMPACT6) return compact_ns_stage(ad->ctx, X, X0, out, sa, ad->nu, ad->rwork);
+    return stage_ns(ad->ctx, X, X0, out, sa, ad->nu);
+}
+
+// tangent-linear stage out = combine(V, V0, dt J_R(X) V)
+int tlm_stage(nova_adjoint *ad, const double *X, const double *V, const double *V0, double *out, const StageArgs &sa)
+{
+    if (ad->scheme == NOVA_SCHEME_COMPACT6) return compact_tlm_stage(ad->ctx, X, V, V0, out, sa, ad->nu, ad->rwork);
+    LinArgs a{};
+    a.X[0] = X;
+    a.X[1] = V;
+    a.X0[1] = V0;
+    a.out[0] = out;
+    a.sa = sa;
+    a.nu = ad->nu;
+    return launch_stream_model<MODEL_TLM>(ad->ctx, a, "tlm_stage");
+}
+
+// adjoint stage out = ob (M + dt J_R(X)^T M) + yg Y, acc = acc_a acc + acc_b M
+int adj_stage(nova_adjoint *ad, const double *X, const double *M, double *out, double ob, const double *Y, double yg,
+              double *acc, double acc_a, double acc_b)
+{
+    if (ad->scheme == NOVA_SCHEME_COMPACT6)
+        return compact_adj_stage(ad->ctx, X, M, out, ob, Y, yg, acc, acc_a, acc_b, ad->dt, ad->nu, ad->rwork);
+    LinArgs a{};
+    a.X[0] = X;
+    a.X[1] = M;
+    a.out[0] = out;
+    a.ob = ob;
+    a.Y = Y;
+    a.yg = yg;
+    a.acc = acc;
+    a.acc_a = acc_a;
+    a.acc_b = acc_b;
+    a.nu = ad->nu;
+    StageArgs sa{};
+    sa.form = FORM_SSP;
+    sa.dt = ad->dt;
+    a.sa = sa;
+    return launch_stream_model<MODEL_ADJ>(ad->ctx, a, "adj_stage");
+}
+
 // Forward step from U storing the stage inputs U1, U2; the next state is
 // written to Unext (which may be U itself).
 int fwd_step_store(nova_adjoint *ad, double *U, double *U1, double *U2, double *Unext)
 {
     nova_ctx *ctx = ad->ctx;
     int rc;
-    if ((rc = stage_ns(ctx, U, nullptr, U1, ssp_args(0, ad->dt), ad->nu))) return rc;
-    if ((rc = stage_ns(ctx, U1, U, U2, ssp_args(1, ad->dt), ad->nu))) return rc;
-    if ((rc = stage_ns(ctx, U2, U, Unext, ssp_args(2, ad->dt), ad->nu))) return rc;
+    if ((rc = fwd_stage(ad, U, nullptr, U1, ssp_args(0, ad->dt)))) return rc;
+    if ((rc = fwd_stage(ad, U1, U, U2, ssp_args(1, ad->dt)))) return rc;
+    if ((rc = fwd_stage(ad, U2, U, Unext, ssp_args(2, ad->dt)))) return rc;
     return project(ad->ps, Unext, ad->dt, ad->ptmp);
 }
 
@@ -98,17 +154,13 @@ int recompute_segment(nova_adjoint *ad, int s, int *count)
     {
         // stage inputs of the last step only
         const int q = n1 - 1 - n0;
-        if ((rc = stage_ns(ad->ctx, ad->seg[3 * q], nullptr, ad->seg[3 * q + 1], ssp_args(0, ad->dt), ad->nu)))
-            return rc;
-        if ((rc = stage_ns(ad->ctx, ad->seg[3 * q + 1], ad->seg[3 * q], ad->seg[3 * q + 2], ssp_args(1, ad->dt),
-                           ad->nu)))
+        if ((rc = fwd_stage(ad, ad->seg[3 * q], nullptr, ad->seg[3 * q + 1], ssp_args(0, ad->dt)))) return rc;
+        if ((rc = fwd_stage(ad, ad->seg[3 * q + 1], ad->seg[3 * q], ad->seg[3 * q + 2], ssp_args(1, ad->dt))))
             return rc;
     }
     *count = n1 - n0;
     return NOVA_OK;
 }
-
-using LinArgs = StreamArgs<2>;
 
 LinArgs lin_args(nova_adjoint *ad, const double *X, const double *V)
 {
@@ -141,6 +193,7 @@ int adjoint_create(nova_solver *sv, int every, int max_steps, nova_adjoint **out
     ad->nu = nu;
     ad->every = every;
     ad->max_steps = max_steps;
+    ad->scheme = solver_scheme(sv);
     const size_t bytes = sizeof(double) * 3 * (size_t)ctx->g.np;
     const int nck = (max_steps + every - 1) / every;
     hipError_t e = hipSuccess;
@@ -156,11 +209,14 @@ int adjoint_create(nova_solver *sv, int every, int max_steps, nova_adjoint **out
     }
     for (int q = 0; q < 5 && e == hipSuccess; q++) e = hipMalloc((void **)&ad->w[q], bytes);
     if (e == hipSuccess) e = hipMalloc((void **)&ad->ptmp, sizeof(double) * ctx->g.np);
+    if (e == hipSuccess && ad->scheme == NOVA_SCHEME_COMPACT6)
+        e = hipMalloc((void **)&ad->rwork, sizeof(double) * 4 * ctx->g.np);
     if (e != hipSuccess) {
         for (double *p : ad->ckpt) if (p) (void)hipFree(p);
         for (double *p : ad->seg) if (p) (void)hipFree(p);
         for (double *p : ad->w) if (p) (void)hipFree(p);
         if (ad->ptmp) (void)hipFree(ad->ptmp);
+        if (ad->rwork) (void)hipFree(ad->rwork);
         delete ad;
         return hip_fail(e, "adjoint checkpoint memory");
     }
@@ -176,6 +232,7 @@ int adjoint_destroy(nova_adjoint *ad)
     for (double *p : ad->seg) if (p) (void)hipFree(p);
     for (double *p : ad->w) if (p) (void)hipFree(p);
     if (ad->ptmp) (void)hipFree(ad->ptmp);
+    if (ad->rwork) (void)hipFree(ad->rwork);
     delete ad;
     return NOVA_OK;
 }
@@ -208,15 +265,9 @@ int adjoint_backward(nova_adjoint *ad, const double *lamT, double *lam0)
         for (int q = cnt - 1; q >= 0; q--) {
             double *U = ad->seg[3 * q], *U1 = ad->seg[3 * q + 1], *U2 = ad->seg[3 * q + 2];
             if ((rc = project(ad->ps, L, 1.0, ad->ptmp))) return rc;  // m3 = P m
-            LinArgs a = lin_args(ad, U2, L);
-            a.out[0] = A; a.ob = 2.0 / 3.0; a.acc = ACC; a.acc_a = 0.0; a.acc_b = 1.0 / 3.0;
-            if ((rc = launch_stream_model<MODEL_ADJ>(ctx, a, "adj_stage"))) return rc;
-            a = lin_args(ad, U1, A);
-            a.out[0] = B; a.ob = 0.25; a.acc = ACC; a.acc_a = 1.0; a.acc_b = 0.75;
-            if ((rc = launch_stream_model<MODEL_ADJ>(ctx, a, "adj_stage"))) return rc;
-            a = lin_args(ad, U, B);
-            a.out[0] = L; a.ob = 1.0; a.Y = ACC; a.yg = 1.0;
-            if ((rc = launch_stream_model<MODEL_ADJ>(ctx, a, "adj_stage"))) return rc;
+            if ((rc = adj_stage(ad, U2, L, A, 2.0 / 3.0, nullptr, 0.0, ACC, 0.0, 1.0 / 3.0))) return rc;
+            if ((rc = adj_stage(ad, U1, A, B, 0.25, nullptr, 0.0, ACC, 1.0, 0.75))) return rc;
+            if ((rc = adj_stage(ad, U, B, L, 1.0, ACC, 1.0, nullptr, 0.0, 0.0))) return rc;
         }
     }
     return NOVA_OK;
@@ -230,6 +281,8 @@ int adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t len)
     if (reps < 1) return fail(NOVA_ERR_ARG, "reps must be >= 1");
     nova_ctx *ctx = ad->ctx;
     std::string rep;
+    if (ad->scheme == NOVA_SCHEME_COMPACT6)  // pencil passes have no launch-shape choice
+        return write_report("compact scheme: no stage launch shapes to tune\n", report, len);
     const int variants[] = {0, 2, 3, 4};
     // the forward state and a distinct second field (a copy), so that the
     // trials move the same bytes as the reverse sweep
@@ -270,15 +323,9 @@ int adjoint_tlm(nova_adjoint *ad, const double *d0, double *dT)
         if ((rc = recompute_segment(ad, s, &cnt))) return rc;
         for (int q = 0; q < cnt; q++) {
             double *U = ad->seg[3 * q], *U1 = ad->seg[3 * q + 1], *U2 = ad->seg[3 * q + 2];
-            LinArgs a = lin_args(ad, U, D);
-            a.out[0] = D1; a.sa = ssp_args(0, ad->dt);
-            if ((rc = launch_stream_model<MODEL_TLM>(ctx, a, "tlm_stage"))) return rc;
-            a = lin_args(ad, U1, D1);
-            a.X0[1] = D; a.out[0] = D2; a.sa = ssp_args(1, ad->dt);
-            if ((rc = launch_stream_model<MODEL_TLM>(ctx, a, "tlm_stage"))) return rc;
-            a = lin_args(ad, U2, D2);
-            a.X0[1] = D; a.out[0] = D; a.sa = ssp_args(2, ad->dt);
-            if ((rc = launch_stream_model<MODEL_TLM>(ctx, a, "tlm_stage"))) return rc;
+            if ((rc = tlm_stage(ad, U, D, nullptr, D1, ssp_args(0, ad->dt)))) return rc;
+            if ((rc = tlm_stage(ad, U1, D1, D, D2, ssp_args(1, ad->dt)))) return rc;
+            if ((rc = tlm_stage(ad, U2, D2, D, D, ssp_args(2, ad->dt)))) return rc;
             if ((rc = project(ad->ps, D, ad->dt, ad->ptmp))) return rc;
         }
     }

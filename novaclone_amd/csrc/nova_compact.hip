@@ -180,7 +180,7 @@ struct DerivW {
         double *out;
         int order;
     };
-    __device__ static const double *input(const Args &a, int) { return a.f; }
+    __device__ static const double *input(const Args &a, const Grid &, int) { return a.f; }
     template <int MP, int DIR>
     __device__ static void compute(const Args &a, const CmpDir &D, int lane, double (&in)[NI][MP],
                                    double (&out)[NO][MP])
@@ -189,6 +189,10 @@ struct DerivW {
         else cmp_deriv<MP, 2>(D, lane, in[0], out[0]);
     }
     __device__ static void store(const Args &a, const Grid &, int, int64_t q, double v, double) { a.out[q] = v; }
+    __device__ static void store_point(const Args &a, const Grid &g, int64_t q, const double *v, const double *x)
+    {
+        store(a, g, 0, q, v[0], x[0]);
+    }
 };
 
 // Navier-Stokes right-hand side along one direction d:
@@ -205,7 +209,7 @@ struct NsW {
         double nu;
         int mode;
     };
-    __device__ static const double *input(const Args &a, int c) { return a.U; }
+    __device__ static const double *input(const Args &a, const Grid &g, int c) { return a.U + c * g.np; }
     template <int MP, int DIR>
     __device__ static void compute(const Args &a, const CmpDir &D, int lane, double (&u)[NI][MP],
                                    double (&out)[NO][MP])
@@ -242,6 +246,10 @@ struct NsW {
             const double x0 = a.U0 ? a.U0[e] : x;
             a.out[e] = combine(a.sa, x, x0, du);
         }
+    }    __device__ static void store_point(const Args &a, const Grid &g, int64_t q, const double *v, const double *x)
+    {
+#pragma unroll
+        for (int o = 0; o < 3; o++) store(a, g, o, q, v[o], x[o]);
     }
 };
 
@@ -255,7 +263,7 @@ struct DivW {
         double scale;
         int mode;
     };
-    __device__ static const double *input(const Args &a, int) { return a.F; }
+    __device__ static const double *input(const Args &a, const Grid &, int) { return a.F; }
     template <int MP, int DIR>
     __device__ static void compute(const Args &, const CmpDir &D, int lane, double (&in)[NI][MP],
                                    double (&out)[NO][MP])
@@ -268,6 +276,10 @@ struct DivW {
         else if (a.mode == 1) a.d[q] = a.d[q] + v;
         else a.d[q] = (a.d[q] + v) * a.scale;
     }
+    __device__ static void store_point(const Args &a, const Grid &g, int64_t q, const double *v, const double *x)
+    {
+        store(a, g, 0, q, v[0], x[0]);
+    }
 };
 
 // gradient correction along one direction: F_d -= s D_d p
@@ -278,7 +290,7 @@ struct GradW {
         double *F;  // the pass direction's component
         double s;
     };
-    __device__ static const double *input(const Args &a, int) { return a.p; }
+    __device__ static const double *input(const Args &a, const Grid &, int) { return a.p; }
     template <int MP, int DIR>
     __device__ static void compute(const Args &, const CmpDir &D, int lane, double (&in)[NI][MP],
                                    double (&out)[NO][MP])
@@ -289,10 +301,173 @@ struct GradW {
     {
         a.F[q] = a.F[q] - a.s * v;
     }
+    __device__ static void store_point(const Args &a, const Grid &g, int64_t q, const double *v, const double *x)
+    {
+        store(a, g, 0, q, v[0], x[0]);
+    }
 };
 
-template <int MP>
-constexpr int tile_lines() { return MP <= 4 ? 8 : 4; }
+// Tangent-linear right-hand side along direction d (X = forward state, v =
+// tangent): -1/2 [D_d(v_d X_c + X_d v_c) + v_d D_d X_c + X_c D_d v_d
+// + X_d D_d v_c + v_c D_d X_d] + nu D2_d v_c; modes as NsW with v the
+// stage input and v0 the step-start tangent.
+struct TlmW {
+    static constexpr int NI = 6, NO = 3;
+    struct Args {
+        const double *X, *V, *V0;
+        double *R, *out;
+        StageArgs sa;
+        double nu;
+        int mode;
+    };
+    __device__ static const double *input(const Args &a, const Grid &g, int c)
+    {
+        return c < 3 ? a.X + c * g.np : a.V + (c - 3) * g.np;
+    }
+    template <int MP, int DIR>
+    __device__ static void compute(const Args &a, const CmpDir &D, int lane, double (&in)[NI][MP],
+                                   double (&out)[NO][MP])
+    {
+        double (&x)[3][MP] = *reinterpret_cast<double(*)[3][MP]>(&in[0][0]);
+        double (&v)[3][MP] = *reinterpret_cast<double(*)[3][MP]>(&in[3][0]);
+        double dxd[MP], dvd[MP];
+        cmp_deriv<MP, 1>(D, lane, x[DIR], dxd);
+        cmp_deriv<MP, 1>(D, lane, v[DIR], dvd);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double dxc[MP], dvc[MP], fl[MP], dfl[MP], l2[MP];
+            if (c == DIR) {
+#pragma unroll
+                for (int j = 0; j < MP; j++) {
+                    dxc[j] = dxd[j];
+                    dvc[j] = dvd[j];
+                }
+            } else {
+                cmp_deriv<MP, 1>(D, lane, x[c], dxc);
+                cmp_deriv<MP, 1>(D, lane, v[c], dvc);
+            }
+#pragma unroll
+            for (int j = 0; j < MP; j++) fl[j] = v[DIR][j] * x[c][j] + x[DIR][j] * v[c][j];
+            cmp_deriv<MP, 1>(D, lane, fl, dfl);
+            cmp_deriv<MP, 2>(D, lane, v[c], l2);
+#pragma unroll
+            for (int j = 0; j < MP; j++)
+                out[c][j] = -0.5 * ((((dfl[j] + v[DIR][j] * dxc[j]) + x[c][j] * dvd[j]) + x[DIR][j] * dvc[j]) +
+                                    v[c][j] * dxd[j]) +
+                            a.nu * l2[j];
+        }
+    }
+    __device__ static void store(const Args &a, const Grid &g, int o, int64_t q, double v, double x)
+    {
+        const int64_t e = o * g.np + q;
+        if (a.mode == 0) {
+            a.R[e] = v;
+        } else if (a.mode == 1) {
+            a.R[e] = a.R[e] + v;
+        } else {
+            const double du = a.sa.dt * (a.R[e] + v);
+            const double x0 = a.V0 ? a.V0[e] : x;
+            a.out[e] = combine(a.sa, x, x0, du);
+        }
+    }    __device__ static void store_point(const Args &a, const Grid &g, int64_t q, const double *v, const double *x)
+    {
+#pragma unroll
+        for (int o = 0; o < 3; o++) store(a, g, o, q, v[o], x[3 + o]);
+    }
+};
+
+// Adjoint right-hand side along direction d (X = forward state, m =
+// adjoint): B(X,m)_c = 1/2 [D_d(X_d m_c) + X_d D_d m_c + m_c D_d X_d],
+// nu D2_d m_c, and for output component d the transposed advection term
+// (B(., X)^T m)_d = 1/2 [sum_c (-X_c D_d m_c + m_c D_d X_c) - D_d(sum_c m_c X_c)];
+// the fourth accumulator collects div X.  Final (k) pass:
+// out = ob (m + dt (R_c - m_c div X)) + yg Y, acc = acc_a acc + acc_b m.
+struct AdjW {
+    static constexpr int NI = 6, NO = 4;
+    struct Args {
+        const double *X, *M;
+        const double *Y;
+        double *acc;
+        double *R;  // 4 components
+        double *out;
+        double dt, nu, ob, yg, acc_a, acc_b;
+        int mode;
+    };
+    __device__ static const double *input(const Args &a, const Grid &g, int c)
+    {
+        return c < 3 ? a.X + c * g.np : a.M + (c - 3) * g.np;
+    }
+    template <int MP, int DIR>
+    __device__ static void compute(const Args &a, const CmpDir &D, int lane, double (&in)[NI][MP],
+                                   double (&out)[NO][MP])
+    {
+        double (&x)[3][MP] = *reinterpret_cast<double(*)[3][MP]>(&in[0][0]);
+        double (&m)[3][MP] = *reinterpret_cast<double(*)[3][MP]>(&in[3][0]);
+        double dx[3][MP];
+#pragma unroll
+        for (int c = 0; c < 3; c++) cmp_deriv<MP, 1>(D, lane, x[c], dx[c]);
+        double bt[MP], sp[MP], dsp[MP];
+#pragma unroll
+        for (int j = 0; j < MP; j++) {
+            bt[j] = 0.0;
+            sp[j] = (m[0][j] * x[0][j] + m[1][j] * x[1][j]) + m[2][j] * x[2][j];
+        }
+        cmp_deriv<MP, 1>(D, lane, sp, dsp);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double dm[MP], fl[MP], dfl[MP], l2[MP];
+            cmp_deriv<MP, 1>(D, lane, m[c], dm);
+#pragma unroll
+            for (int j = 0; j < MP; j++) {
+                fl[j] = x[DIR][j] * m[c][j];
+                bt[j] = bt[j] + ((-x[c][j] * dm[j]) + m[c][j] * dx[c][j]);
+            }
+            cmp_deriv<MP, 1>(D, lane, fl, dfl);
+            cmp_deriv<MP, 2>(D, lane, m[c], l2);
+#pragma unroll
+            for (int j = 0; j < MP; j++)
+                out[c][j] = 0.5 * ((dfl[j] + x[DIR][j] * dm[j]) + m[c][j] * dx[DIR][j]) + a.nu * l2[j];
+        }
+#pragma unroll
+        for (int j = 0; j < MP; j++) {
+            out[DIR][j] = out[DIR][j] - 0.5 * (bt[j] - dsp[j]);
+            out[3][j] = dx[DIR][j];
+        }
+    }
+    __device__ static void store(const Args &a, const Grid &g, int o, int64_t q, double v, double)
+    {
+        const int64_t e = o * g.np + q;
+        if (a.mode == 0) a.R[e] = v;
+        else a.R[e] = a.R[e] + v;
+    }
+    __device__ static void store_point(const Args &a, const Grid &g, int64_t q, const double *v, const double *x)
+    {
+        if (a.mode < 2) {
+#pragma unroll
+            for (int o = 0; o < 4; o++) store(a, g, o, q, v[o], 0.0);
+            return;
+        }
+        const double divx = a.R[3 * g.np + q] + v[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int64_t e = c * g.np + q;
+            const double m = x[3 + c];
+            const double jr = (a.R[e] + v[c]) - m * divx;
+            double o = a.ob * (m + a.dt * jr);
+            if (a.Y) o += a.yg * a.Y[e];
+            if (a.acc) a.acc[e] = (a.acc_a != 0.0 ? a.acc_a * a.acc[e] : 0.0) + a.acc_b * m;
+            a.out[e] = o;
+        }
+    }
+};
+
+// k lines per LDS tile: 48 KB of LDS at most
+template <int MP, class W>
+constexpr int tile_lines()
+{
+    constexpr int nb = W::NI > W::NO ? W::NI : W::NO;
+    return (nb <= 3 ? 8 : 4) / (MP <= 4 ? 1 : 2);
+}
 
 // Lines along k: one wavefront per (i, j) line, no LDS.
 template <int MP, class W>
@@ -306,16 +481,21 @@ __global__ void __launch_bounds__(256) k_cmp_pass_k(Grid g, CmpDir D, typename W
     double in[W::NI][MP], out[W::NO][MP];
 #pragma unroll
     for (int c = 0; c < W::NI; c++) {
-        const double *f = W::input(a, c) + c * g.np * (W::NI == 3 ? 1 : 0) + base;
+        const double *f = W::input(a, g, c) + base;
 #pragma unroll
         for (int j = 0; j < MP; j++) in[c][j] = on ? f[lane * MP + j] : 0.0;
     }
     W::template compute<MP, 2>(a, D, lane, in, out);
     if (!on) return;
 #pragma unroll
-    for (int o = 0; o < W::NO; o++)
+    for (int j = 0; j < MP; j++) {
+        double v[W::NO], x[W::NI];
 #pragma unroll
-        for (int j = 0; j < MP; j++) W::store(a, g, o, base + lane * MP + j, out[o][j], in[W::NI == 3 ? o : 0][j]);
+        for (int o = 0; o < W::NO; o++) v[o] = out[o][j];
+#pragma unroll
+        for (int c = 0; c < W::NI; c++) x[c] = in[c][j];
+        W::store_point(a, g, base + lane * MP + j, v, x);
+    }
 }
 
 // Lines along i (DIR 0) or j (DIR 1): a block stages TL consecutive k
@@ -323,7 +503,7 @@ __global__ void __launch_bounds__(256) k_cmp_pass_k(Grid g, CmpDir D, typename W
 template <int DIR, int MP, class W>
 __global__ void __launch_bounds__(256) k_cmp_pass_t(Grid g, CmpDir D, typename W::Args a)
 {
-    constexpr int TL = tile_lines<MP>();
+    constexpr int TL = tile_lines<MP, W>();
     constexpr int NB = W::NI > W::NO ? W::NI : W::NO;
     constexpr int NMAX = 64 * MP;
     __shared__ double sh[NB * TL * NMAX];
@@ -338,7 +518,7 @@ __global__ void __launch_bounds__(256) k_cmp_pass_t(Grid g, CmpDir D, typename W
         const int c = e / (N * TL), rem = e % (N * TL);
         const int q = rem / TL, kk = rem % TL;
         double v = 0.0;
-        if (k0 + kk < g.s[2]) v = (W::input(a, c) + c * g.np * (W::NI == 3 ? 1 : 0))[base + q * lstride + kk];
+        if (k0 + kk < g.s[2]) v = W::input(a, g, c)[base + q * lstride + kk];
         sh[slot(c, kk, q)] = v;
     }
     __syncthreads();
@@ -374,7 +554,7 @@ int launch_pass_mp(nova_ctx *ctx, int dir, const CmpDir &D, const typename W::Ar
         hipLaunchKernelGGL((k_cmp_pass_k<MP, W>), dim3((unsigned)((lines + 3) / 4)), dim3(256), 0, ctx->stream, g, D,
                            a);
     } else {
-        constexpr int TL = tile_lines<MP>();
+        constexpr int TL = tile_lines<MP, W>();
         const int nkt = (g.s[2] + TL - 1) / TL;
         const int fixed = dir == 0 ? g.s[1] : g.s[0];
         if (dir == 0)
@@ -451,6 +631,31 @@ int compact_ns_stage(nova_ctx *ctx, const double *U, const double *U0, double *o
     if (int rc = launch_pass<NsW>(ctx, 1, a, "compact_ns_j")) return rc;
     a.mode = 2;
     return launch_pass<NsW>(ctx, 2, a, "compact_ns_k");
+}
+
+// Tangent-linear stage: out = combine(V, V0, dt J_R(X) V) (passes i, j, k).
+int compact_tlm_stage(nova_ctx *ctx, const double *X, const double *V, const double *V0, double *out,
+                      const StageArgs &sa, double nu, double *R)
+{
+    TlmW::Args a{X, V, V0, R, out, sa, nu, 0};
+    if (int rc = launch_pass<TlmW>(ctx, 0, a, "compact_tlm_i")) return rc;
+    a.mode = 1;
+    if (int rc = launch_pass<TlmW>(ctx, 1, a, "compact_tlm_j")) return rc;
+    a.mode = 2;
+    return launch_pass<TlmW>(ctx, 2, a, "compact_tlm_k");
+}
+
+// Adjoint stage: out = ob (M + dt J_R(X)^T M) + yg Y, acc = acc_a acc + acc_b M
+// (R holds four components).
+int compact_adj_stage(nova_ctx *ctx, const double *X, const double *M, double *out, double ob, const double *Y,
+                      double yg, double *acc, double acc_a, double acc_b, double dt, double nu, double *R)
+{
+    AdjW::Args a{X, M, Y, acc, R, out, dt, nu, ob, yg, acc_a, acc_b, 0};
+    if (int rc = launch_pass<AdjW>(ctx, 0, a, "compact_adj_i")) return rc;
+    a.mode = 1;
+    if (int rc = launch_pass<AdjW>(ctx, 1, a, "compact_adj_j")) return rc;
+    a.mode = 2;
+    return launch_pass<AdjW>(ctx, 2, a, "compact_adj_k");
 }
 
 // d = (compact div F) * scale

@@ -286,6 +286,7 @@ int stage_ns(nova_ctx *ctx, const double *U, const double *U0, double *out, cons
 }
 
 nova_poisson *solver_poisson(nova_solver *sv) { return sv->ps; }
+int solver_scheme(nova_solver *sv) { return sv->scheme; }
 
 int solver_model(nova_solver *sv, int *mode, double *nu)
 {
