@@ -184,7 +184,7 @@ int nova_solver_step(nova_solver *sv, double dt, int nsteps, double *U, double *
  * kernels one by one (0). */
 int nova_solver_use_graph(nova_solver *sv, int enable);
 /* Derivative scheme of the solver's right-hand side (NOVA_SCHEME_*); the
- * compact scheme needs the NS model, SSP-RK3 and a Poisson solver set to the
+ * compact scheme needs SSP-RK3 (NS or MHD) and a Poisson solver set to the
  * same scheme. */
 int nova_solver_set_scheme(nova_solver *sv, int scheme);
 /* Autotuner (no reference counterpart): times every launch shape of the

@@ -461,6 +461,110 @@ struct AdjW {
     }
 };
 
+// MHD right-hand sides along direction d (app/OT_Poisson/UEqn.H:16-23,
+// BEqn.H:15-23 with compact operators), with Bf(X,Y)_c =
+// 1/2 [D_d(X_d Y_c) + X_d D_d Y_c + Y_c D_d X_d]:
+//   dU_c = -Bf(U,U)_c + Bf(B,B)_c - [c = d] D_d(|B|^2/2) + nu D2_d U_c,
+//   dB_c = -Bf(U,B)_c + Bf(B,U)_c + eta D2_d B_c;  modes as NsW (6 comps).
+struct MhdW {
+    static constexpr int NI = 6, NO = 6;
+    struct Args {
+        const double *U, *B, *U0, *B0;
+        double *R;  // 6 components
+        double *oU, *oB;
+        StageArgs sa;
+        double nu, eta;
+        int mode;
+    };
+    __device__ static const double *input(const Args &a, const Grid &g, int c)
+    {
+        return c < 3 ? a.U + c * g.np : a.B + (c - 3) * g.np;
+    }
+    template <int MP, int DIR>
+    __device__ static void compute(const Args &a, const CmpDir &D, int lane, double (&in)[NI][MP],
+                                   double (&out)[NO][MP])
+    {
+        double (&u)[3][MP] = *reinterpret_cast<double(*)[3][MP]>(&in[0][0]);
+        double (&b)[3][MP] = *reinterpret_cast<double(*)[3][MP]>(&in[3][0]);
+        double dud[MP], dbd[MP];
+        cmp_deriv<MP, 1>(D, lane, u[DIR], dud);
+        cmp_deriv<MP, 1>(D, lane, b[DIR], dbd);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double duc[MP], dbc[MP], f[MP], d1[MP], d2[MP], d3[MP], d4[MP], l2[MP];
+            if (c == DIR) {
+#pragma unroll
+                for (int j = 0; j < MP; j++) {
+                    duc[j] = dud[j];
+                    dbc[j] = dbd[j];
+                }
+            } else {
+                cmp_deriv<MP, 1>(D, lane, u[c], duc);
+                cmp_deriv<MP, 1>(D, lane, b[c], dbc);
+            }
+#pragma unroll
+            for (int j = 0; j < MP; j++) f[j] = u[DIR][j] * u[c][j];
+            cmp_deriv<MP, 1>(D, lane, f, d1);
+#pragma unroll
+            for (int j = 0; j < MP; j++) f[j] = b[DIR][j] * b[c][j];
+            cmp_deriv<MP, 1>(D, lane, f, d2);
+#pragma unroll
+            for (int j = 0; j < MP; j++) f[j] = u[DIR][j] * b[c][j];
+            cmp_deriv<MP, 1>(D, lane, f, d3);
+            if (c == DIR) {
+#pragma unroll
+                for (int j = 0; j < MP; j++) d4[j] = d3[j];  // b_d u_d = u_d b_d
+            } else {
+#pragma unroll
+                for (int j = 0; j < MP; j++) f[j] = b[DIR][j] * u[c][j];
+                cmp_deriv<MP, 1>(D, lane, f, d4);
+            }
+            cmp_deriv<MP, 2>(D, lane, u[c], l2);
+#pragma unroll
+            for (int j = 0; j < MP; j++)
+                out[c][j] = ((-0.5 * ((d1[j] + u[DIR][j] * duc[j]) + u[c][j] * dud[j])) +
+                             0.5 * ((d2[j] + b[DIR][j] * dbc[j]) + b[c][j] * dbd[j])) +
+                            a.nu * l2[j];
+            cmp_deriv<MP, 2>(D, lane, b[c], l2);
+#pragma unroll
+            for (int j = 0; j < MP; j++)
+                out[3 + c][j] = ((-0.5 * ((d3[j] + u[DIR][j] * dbc[j]) + b[c][j] * dud[j])) +
+                                 0.5 * ((d4[j] + b[DIR][j] * duc[j]) + u[c][j] * dbd[j])) +
+                                a.eta * l2[j];
+        }
+        double pm[MP], dpm[MP];
+#pragma unroll
+        for (int j = 0; j < MP; j++) pm[j] = 0.5 * ((b[0][j] * b[0][j] + b[1][j] * b[1][j]) + b[2][j] * b[2][j]);
+        cmp_deriv<MP, 1>(D, lane, pm, dpm);
+#pragma unroll
+        for (int j = 0; j < MP; j++) out[DIR][j] = out[DIR][j] - dpm[j];
+    }
+    __device__ static void store(const Args &a, const Grid &g, int o, int64_t q, double v, double)
+    {
+        const int64_t e = o * g.np + q;
+        if (a.mode == 0) a.R[e] = v;
+        else a.R[e] = a.R[e] + v;
+    }
+    __device__ static void store_point(const Args &a, const Grid &g, int64_t q, const double *v, const double *x)
+    {
+        if (a.mode < 2) {
+#pragma unroll
+            for (int o = 0; o < 6; o++) store(a, g, o, q, v[o], 0.0);
+            return;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int64_t e = c * g.np + q;
+            const double du = a.sa.dt * (a.R[e] + v[c]);
+            const double db = a.sa.dt * (a.R[(3 + c) * g.np + q] + v[3 + c]);
+            const double u0 = a.U0 ? a.U0[e] : x[c];
+            const double b0 = a.B0 ? a.B0[e] : x[3 + c];
+            a.oU[e] = combine(a.sa, x[c], u0, du);
+            a.oB[e] = combine(a.sa, x[3 + c], b0, db);
+        }
+    }
+};
+
 // k lines per LDS tile: 48 KB of LDS at most
 template <int MP, class W>
 constexpr int tile_lines()
@@ -656,6 +760,19 @@ int compact_adj_stage(nova_ctx *ctx, const double *X, const double *M, double *o
     if (int rc = launch_pass<AdjW>(ctx, 1, a, "compact_adj_j")) return rc;
     a.mode = 2;
     return launch_pass<AdjW>(ctx, 2, a, "compact_adj_k");
+}
+
+// MHD stage: (oU, oB) = combine((U, B), (U0, B0), dt (dU, dB)); R holds six
+// components.
+int compact_mhd_stage(nova_ctx *ctx, const double *U, const double *B, const double *U0, const double *B0,
+                      double *oU, double *oB, const StageArgs &sa, double nu, double eta, double *R)
+{
+    MhdW::Args a{U, B, U0, B0, R, oU, oB, sa, nu, eta, 0};
+    if (int rc = launch_pass<MhdW>(ctx, 0, a, "compact_mhd_i")) return rc;
+    a.mode = 1;
+    if (int rc = launch_pass<MhdW>(ctx, 1, a, "compact_mhd_j")) return rc;
+    a.mode = 2;
+    return launch_pass<MhdW>(ctx, 2, a, "compact_mhd_k");
 }
 
 // d = (compact div F) * scale

@@ -18,6 +18,8 @@ int poisson_run(nova_poisson *ps, const double *src, int src_is_vector, double s
 int poisson_scheme(nova_poisson *ps);
 int compact_ns_stage(nova_ctx *ctx, const double *U, const double *U0, double *out, const StageArgs &sa, double nu,
                      double *R);
+int compact_mhd_stage(nova_ctx *ctx, const double *U, const double *B, const double *U0, const double *B0,
+                      double *oU, double *oB, const StageArgs &sa, double nu, double eta, double *R);
 int project(nova_poisson *ps, double *F, double s, double *p);
 nova_ctx *poisson_ctx(nova_poisson *ps);
 }  // namespace nova
@@ -31,7 +33,7 @@ struct nova_solver {
     double *div_tmp = nullptr;
     int use_graph = 0;
     int scheme = NOVA_SCHEME_EXPLICIT;
-    double *rwork = nullptr;  // compact right-hand-side accumulator (3 comps)
+    double *rwork = nullptr;  // compact right-hand-side accumulator (3 comps, 6 for MHD)
     // replayed step graph and the arguments it was recorded with
     hipStream_t gstream = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -268,6 +270,18 @@ int mhd_step_once(nova_solver *sv, double dt, double *U, double *B, double *p, d
     nova_ctx *ctx = sv->ctx;
     int rc;
     if (sv->mode != NOVA_RK_SSP3) return fail(NOVA_ERR_UNSUPPORTED, "MHD supports the SSP-RK3 composition");
+    if (sv->scheme == NOVA_SCHEME_COMPACT6) {
+        double *R = sv->rwork;
+        if ((rc = compact_mhd_stage(ctx, U, B, nullptr, nullptr, sv->wa, sv->wb, ssp(0, dt), sv->nu, sv->eta, R)))
+            return rc;
+        if ((rc = project(sv->ps, sv->wa, dt, p)) || (rc = project(sv->ps, sv->wb, 1.0, pB))) return rc;
+        if ((rc = compact_mhd_stage(ctx, sv->wa, sv->wb, U, B, sv->wc, sv->wd, ssp(1, dt), sv->nu, sv->eta, R)))
+            return rc;
+        if ((rc = project(sv->ps, sv->wc, dt, p)) || (rc = project(sv->ps, sv->wd, 1.0, pB))) return rc;
+        if ((rc = compact_mhd_stage(ctx, sv->wc, sv->wd, U, B, U, B, ssp(2, dt), sv->nu, sv->eta, R))) return rc;
+        if ((rc = project(sv->ps, U, dt, p)) || (rc = project(sv->ps, B, 1.0, pB))) return rc;
+        return NOVA_OK;
+    }
     // stage outputs: (wa, wb) after stage 1, (wc, wd) after stage 2, (U, B) after stage 3
     if ((rc = mhd_stage(ctx, U, B, nullptr, nullptr, sv->wa, sv->wb, ssp(0, dt), sv->nu, sv->eta))) return rc;
     if ((rc = project(sv->ps, sv->wa, dt, p)) || (rc = project(sv->ps, sv->wb, 1.0, pB))) return rc;
@@ -389,10 +403,10 @@ int solver_set_scheme(nova_solver *sv, int scheme)
 {
     if (scheme != NOVA_SCHEME_EXPLICIT && scheme != NOVA_SCHEME_COMPACT6) return fail(NOVA_ERR_ARG, "unknown scheme");
     if (scheme == NOVA_SCHEME_COMPACT6) {
-        if (sv->model != NOVA_MODEL_NS || sv->mode != NOVA_RK_SSP3)
-            return fail(NOVA_ERR_UNSUPPORTED, "the compact scheme covers the NS model with SSP-RK3");
+        if (sv->mode != NOVA_RK_SSP3) return fail(NOVA_ERR_UNSUPPORTED, "the compact scheme covers SSP-RK3");
         if (int rc = compact_supported(sv->ctx)) return rc;
-        if (!sv->rwork) NOVA_HIP(hipMalloc((void **)&sv->rwork, sizeof(double) * 3 * sv->ctx->g.np));
+        const int nc = sv->model == NOVA_MODEL_MHD ? 6 : 3;
+        if (!sv->rwork) NOVA_HIP(hipMalloc((void **)&sv->rwork, sizeof(double) * nc * sv->ctx->g.np));
     }
     if (poisson_scheme(sv->ps) != scheme)
         return fail(NOVA_ERR_ARG, "set the Poisson solver to the same scheme first (nova_poisson_set_scheme)");

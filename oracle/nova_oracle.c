@@ -1194,3 +1194,74 @@ double orc_compact_maxabs_div(const orc_mesh *M, const double *U)
     free(d);
     return mx;
 }
+
+/* out_c += s * 1/2 sum_d [D_d(X_d Y_c) + X_d D_d Y_c + Y_c D_d X_d] */
+static void compact_bform(const orc_mesh *M, const double *X, const double *Y, double s, double *out)
+{
+    const size_t n1 = NPTS(M);
+    double *flux = falloc(M, 3), *dflux = falloc(M, 3), *dY = falloc(M, 3), *dX = falloc(M, 3);
+    double *acc = falloc(M, 3);
+    for (int d = 0; d < 3; d++) {
+        for (size_t q = 0; q < n1; q++)
+            for (int c = 0; c < 3; c++) flux[3 * q + c] = X[3 * q + d] * Y[3 * q + c];
+        orc_fill(M, flux, 3, -1);
+        orc_compact_deriv(M, flux, 3, d, 1, dflux);
+        orc_compact_deriv(M, Y, 3, d, 1, dY);
+        orc_compact_deriv(M, X, 3, d, 1, dX);
+        for (size_t q = 0; q < n1; q++)
+            for (int c = 0; c < 3; c++)
+                acc[3 * q + c] += dflux[3 * q + c] + X[3 * q + d] * dY[3 * q + c] + Y[3 * q + c] * dX[3 * q + d];
+    }
+    for (size_t q = 0; q < 3 * n1; q++) out[q] += s * 0.5 * acc[q];
+    free(flux); free(dflux); free(dY); free(dX); free(acc);
+}
+
+void orc_compact_mhd_rhs(const orc_mesh *M, const double *U, const double *B, double nu, double eta,
+                         double *dU, double *dB)
+{
+    const size_t n1 = NPTS(M), n3 = 3 * n1;
+    memset(dU, 0, sizeof(double) * n3);
+    memset(dB, 0, sizeof(double) * n3);
+    compact_bform(M, U, U, -1.0, dU);
+    compact_bform(M, B, B, 1.0, dU);
+    double *pm = falloc(M, 1), *g = falloc(M, 1), *l = falloc(M, 3);
+    for (size_t q = 0; q < n1; q++)
+        pm[q] = 0.5 * (B[3 * q] * B[3 * q] + B[3 * q + 1] * B[3 * q + 1] + B[3 * q + 2] * B[3 * q + 2]);
+    orc_fill(M, pm, 1, -1);
+    for (int c = 0; c < 3; c++) {
+        orc_compact_deriv(M, pm, 1, c, 1, g);
+        for (size_t q = 0; q < n1; q++) dU[3 * q + c] -= g[q];
+    }
+    for (int d = 0; d < 3; d++) {
+        orc_compact_deriv(M, U, 3, d, 2, l);
+        for (size_t q = 0; q < n3; q++) dU[q] += nu * l[q];
+        orc_compact_deriv(M, B, 3, d, 2, l);
+        for (size_t q = 0; q < n3; q++) dB[q] += eta * l[q];
+    }
+    compact_bform(M, U, B, -1.0, dB);
+    compact_bform(M, B, U, 1.0, dB);
+    orc_fill(M, dU, 3, 1);
+    orc_fill(M, dB, 3, 1);
+    free(pm); free(g); free(l);
+}
+
+void orc_compact_mhd_step(const orc_mesh *M, orc_poisson *P, double *U, double *B, double *p, double *pB,
+                          double nu, double eta, double dt)
+{
+    const size_t n3 = NPTS(M) * 3;
+    double *U0 = falloc(M, 3), *B0 = falloc(M, 3), *dU = falloc(M, 3), *dB = falloc(M, 3);
+    memcpy(U0, U, sizeof(double) * n3);
+    memcpy(B0, B, sizeof(double) * n3);
+    for (int s = 0; s < 3; s++) {
+        orc_compact_mhd_rhs(M, U, B, nu, eta, dU, dB);
+        for (size_t q = 0; q < n3; q++) {
+            const double vu = U[q] + dt * dU[q], vb = B[q] + dt * dB[q];
+            if (s == 0) { U[q] = vu; B[q] = vb; }
+            else if (s == 1) { U[q] = 0.75 * U0[q] + 0.25 * vu; B[q] = 0.75 * B0[q] + 0.25 * vb; }
+            else { U[q] = (1.0 / 3.0) * U0[q] + (2.0 / 3.0) * vu; B[q] = (1.0 / 3.0) * B0[q] + (2.0 / 3.0) * vb; }
+        }
+        orc_compact_project(M, P, U, p, dt);
+        orc_compact_project(M, P, B, pB, 1.0);
+    }
+    free(U0); free(B0); free(dU); free(dB);
+}

@@ -131,6 +131,16 @@ void orc_compact_project(const orc_mesh *, orc_poisson *, double *F, double *p, 
 /* SSP-RK3 step with the compact right-hand side and one compact projection. */
 void orc_compact_ns_step(const orc_mesh *, orc_poisson *, double *U, double *Ustar, double *p,
                          double nu, double dt);
+/* MHD right-hand sides with compact operators (the terms of
+ * app/OT_Poisson/UEqn.H:16-23 and BEqn.H:15-23):
+ *   dU = -B(U,U) + B(B,B) - D(|B|^2/2) + nu D2 U,
+ *   dB = -B(U,B) + B(B,U) + eta D2 B,
+ * B(X,Y)_c = 1/2 sum_d [D_d(X_d Y_c) + X_d D_d Y_c + Y_c D_d X_d]. */
+void orc_compact_mhd_rhs(const orc_mesh *, const double *U, const double *B, double nu, double eta,
+                         double *dU, double *dB);
+/* SSP-RK3 MHD step with compact operators, u and B projected at every stage. */
+void orc_compact_mhd_step(const orc_mesh *, orc_poisson *, double *U, double *B, double *p, double *pB,
+                          double nu, double eta, double dt);
 /* max |compact div U| over the unique nodes. */
 double orc_compact_maxabs_div(const orc_mesh *, const double *U);
 

@@ -72,6 +72,8 @@ def _load() -> C.CDLL:
         "orc_compact_project": ([_P, _P, _D, _D, C.c_double], None),
         "orc_compact_ns_step": ([_P, _P, _D, _D, _D, C.c_double, C.c_double], None),
         "orc_compact_maxabs_div": ([_P, _D], C.c_double),
+        "orc_compact_mhd_rhs": ([_P, _D, _D, C.c_double, C.c_double, _D, _D], None),
+        "orc_compact_mhd_step": ([_P, _P, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -214,6 +216,11 @@ class OMesh:
         lib().orc_compact_ns_rhs(self._h, _d(U), nu, _d(out))
         return out
 
+    def compact_mhd_rhs(self, U, B, nu, eta):
+        dU, dB = self.field(3), self.field(3)
+        lib().orc_compact_mhd_rhs(self._h, _d(U), _d(B), nu, eta, _d(dU), _d(dB))
+        return dU, dB
+
     def compact_maxabs_div(self, U):
         return lib().orc_compact_maxabs_div(self._h, _d(U))
 
@@ -248,6 +255,9 @@ class OPoisson:
         fn = lib().orc_compact_project if self.compact else lib().orc_project
         fn(self.mesh._h, self._h, _d(F), _d(p), s)
         return p
+
+    def compact_mhd_step(self, U, B, p, pB, nu, eta, dt):
+        lib().orc_compact_mhd_step(self.mesh._h, self._h, _d(U), _d(B), _d(p), _d(pB), nu, eta, dt)
 
     def compact_ns_step(self, U, p, nu, dt, Ustar=None):
         if Ustar is None:

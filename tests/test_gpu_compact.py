@@ -123,5 +123,35 @@ def test_compact_scheme_rejections(nv):
     S = nv.Solver(nv.Poisson(G), nv.MODEL_NS, nv.RK_SSP3, nu=1e-2)
     # the solver's Poisson solver is explicit: the scheme change is refused
     assert nv.lib().nova_solver_set_scheme(S._h, nv.SCHEME_COMPACT6) != 0
-    with pytest.raises(nv.NovaError):
-        nv.Solver(nv.Poisson(G, scheme=nv.SCHEME_COMPACT6), nv.MODEL_MHD, nv.RK_SSP3, nu=1e-2, eta=1e-2)
+    with pytest.raises(nv.NovaError):  # compact stages are SSP-RK3 only
+        nv.Solver(nv.Poisson(G, scheme=nv.SCHEME_COMPACT6), nv.MODEL_NS, nv.RK_LITERAL4, nu=1e-2)
+
+
+def test_compact_mhd_step_matches_oracle(nv, orc):
+    """Orszag-Tang type fields with noise; SSP-RK3 with both fields projected
+    at every stage, compact operators throughout."""
+    N, L = (24, 16, 32), (TWO_PI,) * 3
+    nu, eta, dt = 1e-2, 2e-2, 2e-3
+    G = nv.Mesh(N, L)
+    O = orc.OMesh(N, L)
+    P = nv.Poisson(G, scheme=nv.SCHEME_COMPACT6)
+    OP = orc.OPoisson(O, compact=True)
+    rng = np.random.default_rng(3)
+    x = [np.arange(O.stored[a]) * (L[a] / N[a]) for a in range(3)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    Uc = np.stack([-np.sin(Y), np.sin(X), 0 * X]) + 1e-2 * rng.standard_normal((3, *O.stored))
+    Bc = np.stack([-np.sin(Y), np.sin(2 * X), 0 * X]) + 1e-2 * rng.standard_normal((3, *O.stored))
+    Uo, Bo = O.from_compact(Uc), O.from_compact(Bc)
+    OP.project(Uo, 1.0)
+    OP.project(Bo, 1.0)
+    U, B = dev(O.interior(Uo)), dev(O.interior(Bo))
+    S = nv.Solver(P, nv.MODEL_MHD, nv.RK_SSP3, nu=nu, eta=eta)
+    p, pB = G.field(1), G.field(1)
+    po, pBo = O.field(1), O.field(1)
+    for step in range(2):
+        OP.compact_mhd_step(Uo, Bo, po, pBo, nu, eta, dt)
+        S.step(dt, 1, U, p, B, pB)
+        ru, rb = rel_l2(host(U), O.interior(Uo)), rel_l2(host(B), O.interior(Bo))
+        print("compact MHD step", step + 1, "rel U", ru, "rel B", rb, flush=True)
+        assert ru <= RTOL and rb <= RTOL
+    assert O.compact_maxabs_div(O.from_compact(host(B))) < 1e-11

@@ -158,3 +158,58 @@ def test_compact_projection_and_linear_step(orc):
     S = inner(M)
     ratio = U[S][..., :2] / np.where(np.abs(U0[S][..., :2]) > 5e-11, U0[S][..., :2], np.nan)
     assert np.nanmax(np.abs(ratio - Rz)) < 1e-9
+
+
+def test_compact_mhd_rhs_closed_forms(orc):
+    """Lorentz force of B = (sin y, 0, 0) on U = 0: -D(|B|^2 / 2) = (0, -k'(2) sin 2y / 4, 0);
+    induction of U = (0, sin x, 0) in B = (1, 0, 0): B.grad U = (0, k'(1) cos x, 0);
+    with B = 0 the velocity right-hand side is the NS one."""
+    n = 16
+    h = TWO_PI / n
+    M = orc.OMesh((n, n, n), (TWO_PI,) * 3)
+    X, Y, Z = grid(M)
+    S = inner(M)
+    zero = np.zeros((*M.n, 3))
+    nu, eta = 0.3, 0.2
+    k1 = orc.compact_kmod(h, 1) / h
+    k2 = orc.compact_kmod(2 * h, 1) / h
+    kk = orc.compact_kmod(h, 2) / h ** 2
+    B = np.ascontiguousarray(np.stack([np.sin(Y), 0 * X, 0 * X], -1))
+    dU, dB = M.compact_mhd_rhs(zero, B, nu, eta)
+    assert np.max(np.abs(dU[S][..., 1] + 0.25 * k2 * np.sin(2 * Y)[S])) < 1e-13
+    assert np.max(np.abs(dU[S][..., [0, 2]])) < 1e-14
+    assert np.max(np.abs(dB[S][..., 0] + eta * kk * np.sin(Y)[S])) < 1e-13
+    U = np.ascontiguousarray(np.stack([0 * X, np.sin(X), 0 * X], -1))
+    B = np.ascontiguousarray(np.stack([1.0 + 0 * X, 0 * X, 0 * X], -1))
+    dU, dB = M.compact_mhd_rhs(U, B, nu, 0.0)
+    assert np.max(np.abs(dB[S][..., 1] - k1 * np.cos(X)[S])) < 1e-13
+    assert np.max(np.abs(dB[S][..., [0, 2]])) < 1e-14
+    assert np.max(np.abs(dU[S][..., 1] + nu * kk * np.sin(X)[S])) < 1e-13
+    rng = np.random.default_rng(2)
+    U = M.from_compact(rng.standard_normal((3, *M.stored)))
+    dU, dB = M.compact_mhd_rhs(U, np.ascontiguousarray(zero), nu, eta)
+    assert np.max(np.abs(dU - M.compact_ns_rhs(U, nu))[S]) < 1e-12
+    assert np.max(np.abs(dB[S])) == 0.0
+
+
+def test_compact_mhd_linear_step(orc):
+    """Small-amplitude solenoidal modes: one step multiplies U and B by the
+    SSP-RK3 polynomial of their own z = -(nu|eta) dt k''^2."""
+    n = 16
+    h = TWO_PI / n
+    M = orc.OMesh((n, n, n), (TWO_PI,) * 3)
+    P = orc.OPoisson(M, compact=True)
+    X, Y, Z = grid(M)
+    amp = 1e-10
+    U = np.ascontiguousarray(amp * np.stack([np.sin(Y), 0 * X, 0 * X], -1))
+    B = np.ascontiguousarray(amp * np.stack([0 * X, 0 * X, np.sin(X)], -1))
+    U0, B0 = U.copy(), B.copy()
+    nu, eta, dt = 0.2, 0.1, 0.1
+    P.compact_mhd_step(U, B, M.field(1), M.field(1), nu, eta, dt)
+    kk = orc.compact_kmod(h, 2) / h ** 2
+    R = lambda z: 1 + z + z * z / 2 + z ** 3 / 6
+    S = inner(M)
+    m = np.abs(U0[S][..., 0]) > 5e-11
+    assert np.max(np.abs(U[S][..., 0][m] / U0[S][..., 0][m] - R(-nu * dt * kk))) < 1e-9
+    m = np.abs(B0[S][..., 2]) > 5e-11
+    assert np.max(np.abs(B[S][..., 2][m] / B0[S][..., 2][m] - R(-eta * dt * kk))) < 1e-9
