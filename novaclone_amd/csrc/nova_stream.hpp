@@ -52,7 +52,7 @@ constexpr int stage_lds()
 
 // One tile t = (kt, jt, ic) of a stage: the block streams planes i of its
 // chunk for the (k, j) tile.  shbuf holds stage_lds() doubles of LDS.
-template <int H, int MODEL, bool WALLS, int kTK>
+template <int H, int MODEL, bool WALLS, int kTK, int PF = 1>
 __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, const StreamArgs<model_nf<MODEL>()> &a,
                                                                  int t, double *shbuf)
 {
@@ -147,19 +147,47 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                 q[f][c][o] = rsign<WALLS>(c, 0, fi) * sgn_col[c] * ldp(a.X[f], c, po, col);
             }
     }
-    double hv[NSLOT][NC];
-    {
-        int fi;
-        const int64_t po = plane_off(i0, fi);
+    // prefetched data of plane p: the queue entry of plane p + H, the halo of
+    // plane p and the step-start values of plane p
+    // Values are kept as loaded; the reflection signs (walls) are applied
+    // when they are consumed, so that no arithmetic waits on a load in flight.
+    struct Pf {
+        double nq[NF][3];
+        double hv[NSLOT][NC];
+        double nx0[NF][3];
+        double ny[3], nacc[3];  // adjoint stage: pointwise Y and accumulator
+        int fq, fh;             // reflection flags of the queue plane and the halo plane
+    };
+    constexpr bool ADJ = MODEL == MODEL_ADJ;
+    auto load_pf = [&](Pf &P, int p, bool queue) {
+        if (queue) {
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    P.nx0[f][c] = (model_x0<MODEL>(f) && a.X0[f] && valid) ? ldp(a.X0[f], c, (int64_t)p * plane, col)
+                                                                            : 0.0;
+            if constexpr (ADJ) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    P.ny[c] = (a.Y && valid) ? ldp(a.Y, c, (int64_t)p * plane, col) : 0.0;
+                    P.nacc[c] = (a.acc && a.acc_a != 0.0 && valid) ? ldp(a.acc, c, (int64_t)p * plane, col) : 0.0;
+                }
+            }
+            const int64_t po = plane_off(p + H, P.fq);
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) P.nq[f][c] = ldp(a.X[f], c, po, col);
+        }
+        const int64_t ph = plane_off(p, P.fh);
 #pragma unroll
         for (int s = 0; s < NSLOT; s++)
 #pragma unroll
             for (int f = 0; f < NF; f++)
 #pragma unroll
-                for (int c = 0; c < 3; c++)
-                    hv[s][f * 3 + c] =
-                        hon[s] ? rsign<WALLS>(c, 0, fi) * hsg[s][c] * ldp(a.X[f], c, po, hsrc[s]) : 0.0;
-    }
+                for (int c = 0; c < 3; c++) P.hv[s][f * 3 + c] = hon[s] ? ldp(a.X[f], c, ph, hsrc[s]) : 0.0;
+    };
 
     const double zi = g.zinv[km], zl = g.zlap[km], zm = g.zmet[km];
     // step-start values of the own column, one plane ahead like the queue
@@ -169,8 +197,21 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
 #pragma unroll
         for (int c = 0; c < 3; c++)
             x0c[f][c] = (model_x0<MODEL>(f) && a.X0[f] && valid) ? ldp(a.X0[f], c, (int64_t)i0 * plane, col) : 0.0;
+    double yc[3] = {0.0, 0.0, 0.0}, accc[3] = {0.0, 0.0, 0.0};
+    if constexpr (ADJ) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            yc[c] = (a.Y && valid) ? ldp(a.Y, c, (int64_t)i0 * plane, col) : 0.0;
+            accc[c] = (a.acc && a.acc_a != 0.0 && valid) ? ldp(a.acc, c, (int64_t)i0 * plane, col) : 0.0;
+        }
+    }
 
-    for (int i = i0; i < i1; i++) {
+    // One plane: stage plane i (halo from Cur) in LDS, issue the loads of
+    // plane i + PF into the set they go to, compute plane i, then advance
+    // the queue with Nxt (the data of plane i + 1).  PF = 1: Cur and Nxt
+    // are one set; PF = 2: two sets alternate, so two planes of loads are
+    // in flight while a plane is computed.
+    auto body = [&](int i, Pf &Cur, Pf &Nxt) {
 #pragma unroll
         for (int f = 0; f < NF; f++)
 #pragma unroll
@@ -179,36 +220,10 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
         for (int s = 0; s < NSLOT; s++)
             if (hon[s])
 #pragma unroll
-                for (int n = 0; n < NC; n++) sh(n, hrow[s], hcolk[s]) = hv[s][n];
+                for (int n = 0; n < NC; n++)
+                    sh(n, hrow[s], hcolk[s]) = rsign<WALLS>(n % 3, 0, Cur.fh) * hsg[s][n % 3] * Cur.hv[s][n];
         __syncthreads();
-
-        // prefetch plane i+H+1 for the queue, the halo of plane i+1 and the
-        // step-start values of plane i+1
-        double nq[NF][3], nx0[NF][3];
-        if (i + 1 < i1) {
-#pragma unroll
-            for (int f = 0; f < NF; f++)
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-                    nx0[f][c] = (model_x0<MODEL>(f) && a.X0[f] && valid) ? ldp(a.X0[f], c, (int64_t)(i + 1) * plane, col) : 0.0;
-            int fi;
-            const int64_t po = plane_off(i + H + 1, fi);
-#pragma unroll
-            for (int f = 0; f < NF; f++)
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-                    nq[f][c] = rsign<WALLS>(c, 0, fi) * sgn_col[c] * ldp(a.X[f], c, po, col);
-            int fh;
-            const int64_t ph = plane_off(i + 1, fh);
-#pragma unroll
-            for (int s = 0; s < NSLOT; s++)
-#pragma unroll
-                for (int f = 0; f < NF; f++)
-#pragma unroll
-                    for (int c = 0; c < 3; c++)
-                        hv[s][f * 3 + c] =
-                            hon[s] ? rsign<WALLS>(c, 0, fh) * hsg[s][c] * ldp(a.X[f], c, ph, hsrc[s]) : 0.0;
-        }
+        if (i + PF < i1) load_pf(Cur, i + PF, true);
 
         if (valid) {
             auto A = [&](int f, int c, int d, int o) -> double {
@@ -244,9 +259,8 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                 for (int c = 0; c < 3; c++) {
                     const double m = q[1][c][H];
                     double o = a.ob * (m + sa.dt * r[c]);
-                    if (a.Y) o += a.yg * ldp(a.Y, c, pi, col);
-                    if (a.acc)
-                        stp(a.acc, c, pi, col, (a.acc_a != 0.0 ? a.acc_a * ldp(a.acc, c, pi, col) : 0.0) + a.acc_b * m);
+                    if (a.Y) o += a.yg * yc[c];
+                    if (a.acc) stp(a.acc, c, pi, col, (a.acc_a != 0.0 ? a.acc_a * accc[c] : 0.0) + a.acc_b * m);
                     stp(a.out[0], c, pi, col, o);
                 }
             } else {
@@ -273,9 +287,30 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                 for (int c = 0; c < 3; c++) {
 #pragma unroll
                     for (int o = 0; o < 2 * H; o++) q[f][c][o] = q[f][c][o + 1];
-                    q[f][c][2 * H] = nq[f][c];
-                    x0c[f][c] = nx0[f][c];
+                    q[f][c][2 * H] = rsign<WALLS>(c, 0, Nxt.fq) * sgn_col[c] * Nxt.nq[f][c];
+                    x0c[f][c] = Nxt.nx0[f][c];
                 }
+            if constexpr (ADJ) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    yc[c] = Nxt.ny[c];
+                    accc[c] = Nxt.nacc[c];
+                }
+            }
+        }
+    };
+
+    if constexpr (PF == 1) {
+        Pf S;
+        load_pf(S, i0, false);
+        for (int i = i0; i < i1; i++) body(i, S, S);
+    } else {
+        Pf A, B;
+        load_pf(A, i0, false);
+        if (i0 + 1 < i1) load_pf(B, i0 + 1, true);
+        for (int i = i0; i < i1; i += 2) {
+            body(i, A, B);
+            if (i + 1 < i1) body(i + 1, B, A);
         }
     }
 }
@@ -290,7 +325,9 @@ k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
     const int T = a.nkt * a.njt * a.nic;
     int t = blockIdx.x;
     if (T % 8 == 0) t = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
-    stage_tile<H, MODEL, WALLS, kTK>(g, a, t, sh);
+    // shapes held to one wave per SIMD have the registers for a second
+    // prefetch set
+    stage_tile<H, MODEL, WALLS, kTK, MINW == 1 ? 2 : 1>(g, a, t, sh);
 }
 
 int pick_ichunk(const Grid &g, int nkt, int njt)
