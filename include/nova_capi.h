@@ -98,10 +98,12 @@ int nova_timing_enable(nova_ctx *ctx, int enable);
 int nova_timing_collect(nova_ctx *ctx, char *buf, size_t buflen);
 /* Launch-shape choice of the fused stage kernels (no reference
  * counterpart), for every stage model at once: variant 0 stages each (j,k)
- * plane of a 64 x 4 (k x j) tile in LDS with no register limit, variant 1
- * streams rows without LDS (forward models only; the others use variant 3),
- * variants 2, 3 and 4 stage 64 x 4, 32 x 8 and 16 x 16 tiles with one-field
- * models held to three waves per SIMD (default: 3); stage_ichunk = planes per
+ * plane of a 64 x 4 (k x j) tile in LDS at one wave per SIMD with two planes
+ * of loads in flight, variant 1 streams rows without LDS (forward models
+ * only; the others use variant 3), variants 2, 3 and 4 stage 64 x 4, 32 x 8
+ * and 16 x 16 tiles with one-field models held to three waves per SIMD
+ * (default: 3), variants 5 and 6 stage 32 x 8 and 16 x 16 tiles at two waves
+ * per SIMD with two planes in flight (one-field models); stage_ichunk = planes per
  * streamed chunk (0 = auto); proj_variant 0 fuses the divergence into the
  * first FFT pass, 1 computes it (and the gradient correction) with streamed
  * kernels.  Results are identical up to rounding; only speed differs. */

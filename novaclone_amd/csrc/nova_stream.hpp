@@ -325,9 +325,9 @@ k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
     const int T = a.nkt * a.njt * a.nic;
     int t = blockIdx.x;
     if (T % 8 == 0) t = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
-    // shapes held to one wave per SIMD have the registers for a second
-    // prefetch set
-    stage_tile<H, MODEL, WALLS, kTK, MINW == 1 ? 2 : 1>(g, a, t, sh);
+    // shapes held to one or two waves per SIMD have the registers for a
+    // second prefetch set
+    stage_tile<H, MODEL, WALLS, kTK, MINW <= 2 ? 2 : 1>(g, a, t, sh);
 }
 
 int pick_ichunk(const Grid &g, int nkt, int njt)
@@ -364,6 +364,9 @@ void launch_stream_w(nova_ctx *ctx, const StreamArgs<model_nf<MODEL>()> &a)
     case 0: launch_stream_tile<H, MODEL, WALLS, 1, 64>(ctx, a); break;
     case 2: launch_stream_tile<H, MODEL, WALLS, W, 64>(ctx, a); break;
     case 4: launch_stream_tile<H, MODEL, WALLS, W, 16>(ctx, a); break;
+    // two waves per SIMD with two planes of loads in flight (one-field models)
+    case 5: launch_stream_tile<H, MODEL, WALLS, W == 3 ? 2 : 1, 32>(ctx, a); break;
+    case 6: launch_stream_tile<H, MODEL, WALLS, W == 3 ? 2 : 1, 16>(ctx, a); break;
     default: launch_stream_tile<H, MODEL, WALLS, W, 32>(ctx, a); break;
     }
 }
