@@ -106,7 +106,8 @@ int nova_timing_collect(nova_ctx *ctx, char *buf, size_t buflen);
  * per SIMD with two planes in flight (one-field models); stage_ichunk = planes per
  * streamed chunk (0 = auto); proj_variant 0 fuses the divergence into the
  * first FFT pass, 1 computes it (and the gradient correction) with streamed
- * kernels.  Results are identical up to rounding; only speed differs. */
+ * row kernels, 2 (default) with LDS-staged streamed tiles.  Results are
+ * identical up to rounding; only speed differs. */
 int nova_ctx_tune(nova_ctx *ctx, int stage_variant, int stage_ichunk, int proj_variant);
 /* Stage kernels that carry their own launch shape. */
 enum {

@@ -207,7 +207,7 @@ class Mesh:
         s = torch.cuda.current_stream(self.device).cuda_stream
         _check(lib().nova_ctx_set_stream(self._h, C.c_void_p(s)), "nova_ctx_set_stream")
 
-    def tune(self, stage_variant: int = 3, stage_ichunk: int = 0, proj_variant: int = 1) -> None:
+    def tune(self, stage_variant: int = 3, stage_ichunk: int = 0, proj_variant: int = 2) -> None:
         """Pick the stage-kernel variant of every stage model, its i-chunk
         (0 = automatic) and the projection variant."""
         _check(lib().nova_ctx_tune(self._h, int(stage_variant), int(stage_ichunk), int(proj_variant)),

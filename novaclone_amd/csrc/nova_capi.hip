@@ -355,7 +355,7 @@ int nova_timing_collect(nova_ctx *ctx, char *buf, size_t buflen)
 int nova_ctx_tune(nova_ctx *ctx, int stage_variant, int stage_ichunk, int proj_variant)
 {
     NOVA_REQUIRE(ctx && stage_variant >= 0 && stage_variant <= 6 && stage_ichunk >= 0 && proj_variant >= 0 &&
-                     proj_variant <= 1,
+                     proj_variant <= 2,
                  "bad argument");
     for (int k = 0; k < 4; k++) {
         ctx->stage_variant[k] = stage_variant;

@@ -639,6 +639,217 @@ __global__ void __launch_bounds__(kRK *kRJ) k_correct_rows(Grid g, RowTiles t, c
     }
 }
 
+// ---- LDS-staged streamed divergence and gradient correction ----
+//
+// A 32 x 8 (k x j) tile walks i over a chunk like the stage kernels: the
+// i-neighbours of its column sit in a register queue, the j and k
+// neighbours come from LDS rows and columns staged per plane, and the loads
+// of the next two planes are in flight while a plane is combined.  Values
+// are kept as loaded and the wall reflection signs applied when they are
+// staged.  DIV: out = div(F) * scale (the arithmetic of k_div_rows);
+// otherwise F -= s grad(p) (that of k_correct_rows).
+constexpr int kPK = 32, kPJ = 8;
+
+template <int H, bool DIV, bool WALLS>
+__global__ void __launch_bounds__(256) k_proj_stream(Grid g, RowTiles t, const double *__restrict__ src, double sc,
+                                                     double *__restrict__ dst, BatchArgs ba)
+{
+    constexpr int NH = 2 * H * kPK + kPJ * 2 * H;  // halo elements per plane (<= 256)
+    static_assert(NH <= 256, "one halo slot per thread");
+    const int T = t.nkt * t.njt * t.nic;
+    int bt = blockIdx.x;
+    if (T % 8 == 0) bt = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
+    const int kt = bt % t.nkt, jt = (bt / t.nkt) % t.njt, ic = bt / (t.nkt * t.njt);
+    const int tk = threadIdx.x % kPK, tj = threadIdx.x / kPK;
+    const int k = kt * kPK + tk, j = jt * kPJ + tj;
+    const bool valid = k < g.s[2] && j < g.s[1];
+    const int i0 = ic * t.ichunk;
+    const int i1 = i0 + t.ichunk < g.s[0] ? i0 + t.ichunk : g.s[0];
+    const int64_t plane = (int64_t)g.s[1] * g.s[2];
+    // batch member blockIdx.y
+    if (DIV) {
+        src += blockIdx.y * ba.f_bs;
+        dst += blockIdx.y * g.np;
+        if ((int)blockIdx.y >= ba.split) sc = ba.scale_hi;
+    } else {
+        src += blockIdx.y * ba.p_bs;
+        dst += blockIdx.y * ba.f_bs;
+        if ((int)blockIdx.y >= ba.split) sc = ba.s_hi;
+    }
+    const double *Fx = src, *Fy = src + g.np, *Fz = src + 2 * g.np;  // DIV only
+
+    __shared__ double sa[kPJ + 2 * H][kPK + 2 * H];  // DIV: F_y (j halo); else p (both halos)
+    __shared__ double sb[DIV ? kPJ : 1][DIV ? kPK + 2 * H : 1];  // DIV: F_z (k halo)
+
+    int fj, fk;
+    const int jm = bmap(j, g.n[1], g.wall[1], fj);
+    const int km = bmap(k, g.n[2], g.wall[2], fk);
+    const int64_t col = (int64_t)jm * g.s[2] + km;
+    // this thread's halo slot: kind 1 = j row (F_y or p), 2 = k column (F_z or p)
+    int hk = 0, hr = 0, hc = 0, hf = 0;
+    int64_t hsrc = 0;
+    {
+        const int e = threadIdx.x;
+        int jj = 0, kk = 0;
+        if (e < 2 * H * kPK) {
+            const int rr = e / kPK, cc = e % kPK;
+            hk = 1;
+            hr = rr < H ? rr : kPJ + rr;
+            hc = cc + H;
+            jj = jt * kPJ + (rr < H ? rr - H : kPJ + rr - H);
+            kk = kt * kPK + cc;
+        } else if (e < NH) {
+            const int e2 = e - 2 * H * kPK;
+            const int rr = e2 / (2 * H), c2 = e2 % (2 * H);
+            hk = 2;
+            hr = rr + H;
+            hc = c2 < H ? c2 : kPK + c2;
+            jj = jt * kPJ + rr;
+            kk = kt * kPK + (c2 < H ? c2 - H : kPK + c2 - H);
+        }
+        int f1 = 0, f2 = 0;
+        const int jm2 = bmap(jj, g.n[1], g.wall[1], f1);
+        const int km2 = bmap(kk, g.n[2], g.wall[2], f2);
+        hsrc = (int64_t)jm2 * g.s[2] + km2;
+        hf = hk == 1 ? f1 : f2;  // the reflection that matters for the staged component
+    }
+    // sign of a staged value: DIV stages F_y (odd across y walls) and F_z
+    // (odd across z walls); p is even
+    auto sgn = [&](int flip) -> double { return (DIV && WALLS && flip) ? -1.0 : 1.0; };
+
+    struct Pf {
+        double q;      // queue entry (F_x or p) of plane p + H
+        double cy, cz; // DIV: F_y, F_z at the own column of plane p
+        double h;      // halo value of plane p
+        double f[3];   // correction: F at the own column of plane p
+        int fq, fp;    // reflection flags of the queue plane and of plane p
+    };
+    auto plane_of = [&](int ii, int &flip) -> int64_t {
+        const int im = bmap(ii, g.n[0], g.wall[0], flip);
+        return (int64_t)im * plane;
+    };
+    const double *qsrc = DIV ? Fx : src;
+    auto load_pf = [&](Pf &P, int p, bool queue) {
+        if (queue) {
+            const int64_t po = plane_of(p + H, P.fq);
+            P.q = qsrc[po + col];
+        }
+        const int64_t pp = plane_of(p, P.fp);
+        if (DIV) {
+            P.cy = Fy[pp + col];
+            P.cz = Fz[pp + col];
+            P.h = hk == 1 ? Fy[pp + hsrc] : (hk == 2 ? Fz[pp + hsrc] : 0.0);
+        } else {
+            P.h = hk ? src[pp + hsrc] : 0.0;
+            const int64_t qq = (int64_t)p * plane + col;
+            if (valid)
+#pragma unroll
+                for (int c = 0; c < 3; c++) P.f[c] = dst[c * g.np + qq];
+        }
+    };
+
+    double q[2 * H + 1];
+#pragma unroll
+    for (int o = 0; o < 2 * H + 1; o++) {
+        int fi;
+        const int64_t po = plane_of(i0 - H + o, fi);
+        q[o] = (DIV && WALLS && fi ? -1.0 : 1.0) * qsrc[po + col];
+    }
+    const double zi = g.zinv[km];
+    const double cyz_sign_y = sgn(fj), cyz_sign_z = sgn(fk);
+
+    auto body = [&](int i, Pf &Cur, Pf &Nxt) {
+        // stage plane i
+        if (DIV) {
+            sa[tj + H][tk + H] = cyz_sign_y * Cur.cy;
+            sb[tj][tk + H] = cyz_sign_z * Cur.cz;
+            if (hk == 1) sa[hr][hc] = sgn(hf) * Cur.h;
+            else if (hk == 2) sb[hr - H][hc] = sgn(hf) * Cur.h;
+        } else {
+            sa[tj + H][tk + H] = q[H];
+            if (hk) sa[hr][hc] = Cur.h;
+        }
+        __syncthreads();
+        const double f0 = Cur.f[0], f1 = Cur.f[1], f2 = Cur.f[2];
+        if (i + 2 < i1) load_pf(Cur, i + 2, true);
+        if (valid) {
+            const int64_t q0 = (int64_t)i * plane + col;
+            if (DIV) {
+                double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+                for (int l = 0; l < H; l++) {
+                    ax += g.a[0][l] * (q[H + 1 + l] - q[H - 1 - l]);
+                    ay += g.a[1][l] * (sa[tj + H + 1 + l][tk + H] - sa[tj + H - 1 - l][tk + H]);
+                    az += g.a[2][l] * (sb[tj][tk + H + 1 + l] - sb[tj][tk + H - 1 - l]);
+                }
+                dst[q0] = (ax + ay + az * zi) * sc;
+            } else {
+                double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+                for (int l = 0; l < H; l++) {
+                    gx += g.a[0][l] * (q[H + 1 + l] - q[H - 1 - l]);
+                    gy += g.a[1][l] * (sa[tj + H + 1 + l][tk + H] - sa[tj + H - 1 - l][tk + H]);
+                    gz += g.a[2][l] * (sa[tj + H][tk + H + 1 + l] - sa[tj + H][tk + H - 1 - l]);
+                }
+                gz *= zi;
+                dst[q0] = f0 - sc * gx;
+                dst[g.np + q0] = f1 - sc * gy;
+                dst[2 * g.np + q0] = f2 - sc * gz;
+            }
+        }
+        __syncthreads();
+        if (i + 1 < i1) {
+#pragma unroll
+            for (int o = 0; o < 2 * H; o++) q[o] = q[o + 1];
+            q[2 * H] = (DIV && WALLS && Nxt.fq ? -1.0 : 1.0) * Nxt.q;
+        }
+    };
+    Pf A, B;
+    A.f[0] = A.f[1] = A.f[2] = 0.0;
+    B.f[0] = B.f[1] = B.f[2] = 0.0;
+    load_pf(A, i0, false);
+    if (i0 + 1 < i1) load_pf(B, i0 + 1, true);
+    for (int i = i0; i < i1; i += 2) {
+        body(i, A, B);
+        if (i + 1 < i1) body(i + 1, B, A);
+    }
+}
+
+template <int H, bool DIV>
+int launch_proj_stream_h(nova_ctx *ctx, const double *src, double sc, double *dst, int B, const BatchArgs &ba)
+{
+    const Grid &g = ctx->g;
+    RowTiles t;
+    t.nkt = (g.s[2] + kPK - 1) / kPK;
+    t.njt = (g.s[1] + kPJ - 1) / kPJ;
+    int64_t per = (int64_t)g.s[0] * t.nkt * t.njt / 4096;
+    int ichunk = (int)(per < 8 ? 8 : (per > 64 ? 64 : per));
+    t.ichunk = ichunk > g.s[0] ? g.s[0] : ichunk;
+    t.nic = (g.s[0] + t.ichunk - 1) / t.ichunk;
+    const dim3 grid((unsigned)(t.nkt * t.njt * t.nic), (unsigned)B);
+    const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
+    if (walls)
+        hipLaunchKernelGGL((k_proj_stream<H, DIV, true>), grid, dim3(256), 0, ctx->stream, g, t, src, sc, dst, ba);
+    else
+        hipLaunchKernelGGL((k_proj_stream<H, DIV, false>), grid, dim3(256), 0, ctx->stream, g, t, src, sc, dst, ba);
+    return check_launch(DIV ? "div_stream" : "correct_stream");
+}
+
+int launch_div_rows(nova_ctx *ctx, const double *F, double scale, double *out, int B, const BatchArgs &ba);
+int launch_correct_rows(nova_ctx *ctx, const double *p, double s, double *F, int B, const BatchArgs &ba);
+
+template <bool DIV>
+int launch_proj_stream(nova_ctx *ctx, const double *src, double sc, double *dst, int B, const BatchArgs &ba)
+{
+    switch (ctx->g.h) {
+    case 1: return launch_proj_stream_h<1, DIV>(ctx, src, sc, dst, B, ba);
+    case 2: return launch_proj_stream_h<2, DIV>(ctx, src, sc, dst, B, ba);
+    case 3: return launch_proj_stream_h<3, DIV>(ctx, src, sc, dst, B, ba);
+    }
+    // m = 8: the halo ring exceeds one slot per thread; the row kernels serve
+    return DIV ? launch_div_rows(ctx, src, sc, dst, B, ba) : launch_correct_rows(ctx, src, sc, dst, B, ba);
+}
+
 RowTiles row_tiles(const Grid &g, int ichunk)
 {
     RowTiles t;
@@ -969,11 +1180,12 @@ static int run_pipeline(nova_poisson *ps, const double *src, int src_is_vector, 
     const int64_t ylines_total = (int64_t)ps->Kc * ps->Nx * B;
     int64_t src_bs = ba.f_bs;
     int rc;
-    if (src_is_vector && (ctx->proj_variant == 1 || B > 1)) {
+    if (src_is_vector && (ctx->proj_variant >= 1 || B > 1)) {
         // divergence by a streamed kernel into a scalar, then a plain r2c
         {
             TimedScope ts(ctx, "div_rows");
-            rc = launch_div_rows(ctx, src, scale, ps->dtmp, B, ba);
+            rc = ctx->proj_variant == 2 ? launch_proj_stream<true>(ctx, src, scale, ps->dtmp, B, ba)
+                                        : launch_div_rows(ctx, src, scale, ps->dtmp, B, ba);
         }
         if (rc) return rc;
         src = ps->dtmp;
@@ -1039,6 +1251,7 @@ int project(nova_poisson *ps, double *F, double s, double *p)
     const Grid &g = ctx->g;
     dim3 grid((g.s[2] + 63) / 64, (g.s[1] + 3) / 4, g.s[0]);
     TimedScope ts(ctx, "grad_correct");
+    if (ctx->proj_variant == 2) return launch_proj_stream<false>(ctx, p, s, F, 1, BatchArgs());
     if (ctx->proj_variant == 1) return launch_correct_rows(ctx, p, s, F);
     if (g.wall[0] || g.wall[1] || g.wall[2])
         hipLaunchKernelGGL(k_correct<true>, grid, dim3(64, 4, 1), 0, ctx->stream, g, p, s, F);
@@ -1064,6 +1277,7 @@ int project_batch(nova_poisson *ps, double *F, int64_t f_bs, int B, int split, d
     ba.s_hi = s_hi;
     if (int rc = run_pipeline(ps, F, 1, 1.0 / s_lo, p, B, ba)) return rc;
     TimedScope ts(ctx, "grad_correct");
+    if (ctx->proj_variant == 2) return launch_proj_stream<false>(ctx, p, s_lo, F, B, ba);
     return launch_correct_rows(ctx, p, s_lo, F, B, ba);
 }
 

@@ -87,7 +87,7 @@ struct nova_ctx {
     int stage_variant[4] = {3, 3, 3, 3};
     int stage_ichunk[4] = {0, 0, 0, 0};
     int tune_epoch = 0;                    // bumped on every change of the launch shapes
-    int proj_variant = 1;                  // 0: divergence fused into the r2c pass, 1: streamed kernels
+    int proj_variant = 2;                  // 0: divergence fused into the r2c pass, 1: row kernels, 2: LDS tiles
     std::vector<nova::TimedLaunch> tlog;   // launches since the last collect
     std::vector<hipEvent_t> evpool;        // reusable events
     hipStream_t stream = nullptr;

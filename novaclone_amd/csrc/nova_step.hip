@@ -382,7 +382,7 @@ int solver_autotune(nova_solver *sv, const double *U, const double *B, int reps,
     NOVA_HIP(hipMemcpyAsync(sv->wa, U, sizeof(double) * 3 * ctx->g.np, hipMemcpyDeviceToDevice, ctx->stream));
     double best = 1e300;
     int bp = ctx->proj_variant;
-    for (int pv = 0; pv <= 1; pv++) {
+    for (int pv = 0; pv <= 2; pv++) {
         ctx->proj_variant = pv;
         double us = 0.0;
         if (int rc = time_avg_us(ctx, reps, [&]() { return project(sv->ps, sv->wa, 1.0, nullptr); }, &us)) {

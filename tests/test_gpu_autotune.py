@@ -112,3 +112,29 @@ def test_adjoint_autotune_keeps_gradient(nv):
     l2 = A.backward(g)
     d2 = A.tlm(g)
     assert rel(l2, l1) <= 1e-14 and rel(d2, d1) <= 1e-14
+
+
+@pytest.mark.parametrize("wall", [False, True])
+def test_projection_variants_agree(nv, wall):
+    """The LDS-staged divergence/correction (variant 2) evaluates the same
+    expressions as the row kernels (variant 1): identical bits; the fused
+    divergence (variant 0) agrees to rounding."""
+    import torch
+    N = (24, 12, 32)
+    G = nv.Mesh(N, (TWO_PI, 3.0 if wall else TWO_PI, TWO_PI), bc=("periodic", "wall" if wall else "periodic",
+                                                                    "periodic"))
+    P = nv.Poisson(G)
+    gen = torch.Generator(device="cuda").manual_seed(21)
+    U0 = torch.randn((3, *G.shape), generator=gen, dtype=torch.float64, device="cuda")
+    if wall:
+        U0[1, :, 0, :] = 0.0
+        U0[1, :, -1, :] = 0.0
+    out = {}
+    for pv in (1, 2, 0):
+        G.tune(3, 0, pv)
+        U = U0.clone()
+        p = G.field(1)
+        P.project(U, 0.7, p)
+        out[pv] = (U, p)
+    assert torch.equal(out[2][0], out[1][0]) and torch.equal(out[2][1], out[1][1])
+    assert rel(out[0][0], out[1][0]) <= 1e-14
