@@ -238,7 +238,7 @@ __device__ inline double div_at(const Grid &g, const double *__restrict__ F, int
 
 // Real lines per block in the k passes.
 template <int N>
-constexpr int zlines() { return N <= 384 ? 16 : (N <= 512 ? 8 : 4); }
+constexpr int zlines() { return (N >= 128 && N <= 256) ? 8 : (N <= 384 ? 16 : (N <= 512 ? 8 : 4)); }
 // Lines per block along j, and batch width along i.
 template <int N>
 constexpr int ylines() { return N >= 2048 ? 1 : (2048 / N > 16 ? 16 : 2048 / N); }
