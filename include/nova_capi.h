@@ -103,7 +103,9 @@ int nova_timing_collect(nova_ctx *ctx, char *buf, size_t buflen);
  * only; the others use variant 3), variants 2, 3 and 4 stage 64 x 4, 32 x 8
  * and 16 x 16 tiles with one-field models held to three waves per SIMD
  * (default: 3), variants 5 and 6 stage 32 x 8 and 16 x 16 tiles at two waves
- * per SIMD with two planes in flight (one-field models); stage_ichunk = planes per
+ * per SIMD with two planes in flight (one-field models), variants 7 and 8 the
+ * same with 512-thread 32 x 16 and 64 x 8 tiles (two-field models keep
+ * 32 x 8 and 64 x 4 at one wave); stage_ichunk = planes per
  * streamed chunk (0 = auto); proj_variant 0 fuses the divergence into the
  * first FFT pass, 1 computes it (and the gradient correction) with streamed
  * row kernels, 2 (default) with LDS-staged streamed tiles.  Results are
@@ -192,7 +194,7 @@ int nova_solver_use_graph(nova_solver *sv, int enable);
 int nova_solver_set_scheme(nova_solver *sv, int scheme);
 /* Autotuner (no reference counterpart): times every launch shape of the
  * solver's stage kernel (variants x chunks of 0, 8, 16, 32, 64 planes) and
- * both projection variants on U (and B), reps launches each after one
+ * the three projection variants on U (and B), reps launches each after one
  * untimed launch, and keeps the fastest.  U and B are not modified.  When
  * report is non-null it receives one line per trial and the choices. */
 int nova_solver_autotune(nova_solver *sv, const double *U, const double *B, int reps, char *report,

@@ -44,20 +44,19 @@ struct StreamArgs {
 // MODEL_TLM: X = {U, d}, X0[1] = step-start tangent; out[0] = tangent stage.
 // MODEL_ADJ: X = {U, m}; out[0] = ob (m + dt J_R(U)^T m) + yg Y.
 // LDS doubles of one tile's staged plane
-template <int H, int MODEL, int kTK>
+template <int H, int MODEL, int kTK, int kTJ = 256 / kTK>
 constexpr int stage_lds()
 {
-    return 3 * model_nf<MODEL>() * (256 / kTK + 2 * H) * (kTK + 2 * H);
+    return 3 * model_nf<MODEL>() * (kTJ + 2 * H) * (kTK + 2 * H);
 }
 
 // One tile t = (kt, jt, ic) of a stage: the block streams planes i of its
 // chunk for the (k, j) tile.  shbuf holds stage_lds() doubles of LDS.
-template <int H, int MODEL, bool WALLS, int kTK, int PF = 1>
+template <int H, int MODEL, bool WALLS, int kTK, int PF = 1, int kTJ = 256 / kTK>
 __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, const StreamArgs<model_nf<MODEL>()> &a,
                                                                  int t, double *shbuf)
 {
     constexpr int NF = model_nf<MODEL>();
-    constexpr int kTJ = 256 / kTK;
     constexpr int QW = 2 * H + 1;
     constexpr int NC = 3 * NF;
     constexpr int LK = kTK + 2 * H, LJ = kTJ + 2 * H;
@@ -315,11 +314,11 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
     }
 }
 
-template <int H, int MODEL, bool WALLS, int MINW, int kTK>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MINW)))
+template <int H, int MODEL, bool WALLS, int MINW, int kTK, int kTJ>
+__global__ void __launch_bounds__(kTK *kTJ) __attribute__((amdgpu_waves_per_eu(MINW)))
 k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
 {
-    __shared__ double sh[stage_lds<H, MODEL, kTK>()];
+    __shared__ double sh[stage_lds<H, MODEL, kTK, kTJ>()];
     // consecutive tiles share halos, so each XCD (blocks are dealt
     // round-robin over 8) takes a contiguous run of tiles
     const int T = a.nkt * a.njt * a.nic;
@@ -327,7 +326,7 @@ k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
     if (T % 8 == 0) t = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
     // shapes held to one or two waves per SIMD have the registers for a
     // second prefetch set
-    stage_tile<H, MODEL, WALLS, kTK, MINW <= 2 ? 2 : 1>(g, a, t, sh);
+    stage_tile<H, MODEL, WALLS, kTK, MINW <= 2 ? 2 : 1, kTJ>(g, a, t, sh);
 }
 
 int pick_ichunk(const Grid &g, int nkt, int njt)
@@ -339,18 +338,18 @@ int pick_ichunk(const Grid &g, int nkt, int njt)
     return c;
 }
 
-template <int H, int MODEL, bool WALLS, int MINW, int TK>
+template <int H, int MODEL, bool WALLS, int MINW, int TK, int TJ = 256 / TK>
 void launch_stream_tile(nova_ctx *ctx, StreamArgs<model_nf<MODEL>()> a)
 {
     const Grid &g = ctx->g;
-    constexpr int TJ = 256 / TK;
     a.nkt = (g.s[2] + TK - 1) / TK;
     a.njt = (g.s[1] + TJ - 1) / TJ;
     const int ic = ctx->stage_ichunk[MODEL];
     a.ichunk = ic > 0 ? (ic < g.s[0] ? ic : g.s[0]) : pick_ichunk(g, a.nkt, a.njt);
     a.nic = (g.s[0] + a.ichunk - 1) / a.ichunk;
     const int T = a.nkt * a.njt * a.nic;
-    hipLaunchKernelGGL((k_stage_stream<H, MODEL, WALLS, MINW, TK>), dim3(T), dim3(TK, TJ, 1), 0, ctx->stream, g, a);
+    hipLaunchKernelGGL((k_stage_stream<H, MODEL, WALLS, MINW, TK, TJ>), dim3(T), dim3(TK, TJ, 1), 0, ctx->stream, g,
+                       a);
 }
 
 template <int H, int MODEL, bool WALLS>
@@ -367,6 +366,17 @@ void launch_stream_w(nova_ctx *ctx, const StreamArgs<model_nf<MODEL>()> &a)
     // two waves per SIMD with two planes of loads in flight (one-field models)
     case 5: launch_stream_tile<H, MODEL, WALLS, W == 3 ? 2 : 1, 32>(ctx, a); break;
     case 6: launch_stream_tile<H, MODEL, WALLS, W == 3 ? 2 : 1, 16>(ctx, a); break;
+    // 512-thread tiles (32 x 16, 64 x 8): a smaller halo ring per node.  A
+    // 512-thread block is held to 256 registers per lane, which two queued
+    // fields exceed, so two-field models keep their 256-thread shapes here
+    case 7:
+        if constexpr (W == 3) launch_stream_tile<H, MODEL, WALLS, 2, 32, 16>(ctx, a);
+        else launch_stream_tile<H, MODEL, WALLS, 1, 32>(ctx, a);
+        break;
+    case 8:
+        if constexpr (W == 3) launch_stream_tile<H, MODEL, WALLS, 2, 64, 8>(ctx, a);
+        else launch_stream_tile<H, MODEL, WALLS, 1, 64>(ctx, a);
+        break;
     default: launch_stream_tile<H, MODEL, WALLS, W, 32>(ctx, a); break;
     }
 }

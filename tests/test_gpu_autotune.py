@@ -46,7 +46,7 @@ def test_every_stage_variant_gives_the_same_step(nv, wall):
         U0[1, :, -1, :] = 0.0
     P.project(U0, 1.0)
     ref = None
-    for v in (3, 0, 1, 2, 4, 5, 6):
+    for v in (3, 0, 1, 2, 4, 5, 6, 7, 8):
         for c in (0, 5):
             G.tune(v, c, 1)
             U = U0.clone()
@@ -74,7 +74,7 @@ def test_solver_autotune_keeps_results_and_rerecords_graph(nv):
     rep = Sg.autotune(U0, reps=2)
     print(rep, flush=True)
     assert "ns_stage chosen variant" in rep and "projection chosen variant" in rep
-    assert rep.count("ns_stage variant") == 35
+    assert rep.count("ns_stage variant") == 45
     Se.step(2e-3, 3, Ue, p)
     Sg.step(2e-3, 3, Ug, p)  # the graph is recorded again with the tuned shapes
     torch.cuda.synchronize()
