@@ -46,7 +46,7 @@ def test_every_stage_variant_gives_the_same_step(nv, wall):
         U0[1, :, -1, :] = 0.0
     P.project(U0, 1.0)
     ref = None
-    for v in (3, 0, 1, 2, 4, 5, 6, 7, 8, 9):
+    for v in (3, 0, 1, 2, 4, 5, 6, 7, 8):
         for c in (0, 5):
             G.tune(v, c, 1)
             U = U0.clone()
@@ -58,29 +58,6 @@ def test_every_stage_variant_gives_the_same_step(nv, wall):
                 # multiply-adds are fused per source expression, so every
                 # launch shape rounds identically
                 assert torch.equal(U, ref)
-
-
-@pytest.mark.parametrize("shape", [(16, 24, 48), (32, 32, 64)])
-def test_ring_variant_matches_default_on_partial_tiles(nv, shape):
-    # variant 9 streams 32 x 16 tiles through a six-plane LDS ring: partial
-    # tiles in j and k, chunks that end inside a turn of the ring (16 planes
-    # in chunks of 6) and the stage with and without step-start values
-    import torch
-    G = nv.Mesh(shape, (TWO_PI,) * 3)
-    P = nv.Poisson(G)
-    S = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=2e-2)
-    gen = torch.Generator(device="cuda").manual_seed(5)
-    U0 = torch.randn((3, *G.shape), generator=gen, dtype=torch.float64, device="cuda")
-    P.project(U0, 1.0)
-    G.tune(3, 0, 2)
-    ref = U0.clone()
-    S.step(1e-3, 3, ref, G.field(1))
-    for c in (0, 5, 16):
-        G.tune(9, c, 2)
-        U = U0.clone()
-        S.step(1e-3, 3, U, G.field(1))
-        print("shape", shape, "chunk", c, "rel", rel(U, ref), flush=True)
-        assert torch.equal(U, ref)
 
 
 def test_solver_autotune_keeps_results_and_rerecords_graph(nv):
@@ -97,7 +74,7 @@ def test_solver_autotune_keeps_results_and_rerecords_graph(nv):
     rep = Sg.autotune(U0, reps=2)
     print(rep, flush=True)
     assert "ns_stage chosen variant" in rep and "projection chosen variant" in rep
-    assert rep.count("ns_stage variant") == 50
+    assert rep.count("ns_stage variant") == 45
     Se.step(2e-3, 3, Ue, p)
     Sg.step(2e-3, 3, Ug, p)  # the graph is recorded again with the tuned shapes
     torch.cuda.synchronize()
