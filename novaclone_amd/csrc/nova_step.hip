@@ -158,6 +158,18 @@ int launch_stream_h(nova_ctx *ctx, StreamArgs<NF> a, const char *name)
             hipLaunchKernelGGL((k_stage_rows<H, NF, true>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
         else
             hipLaunchKernelGGL((k_stage_rows<H, NF, false>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
+    } else if constexpr (H == 3 && MODEL == MODEL_NS) {
+        // sixth-order NS stages compile the SSP combination in
+        const int cmb = a.sa.form != FORM_SSP ? 0 : (a.sa.alpha == 0.0 ? 2 : (a.X0[0] ? 1 : 0));
+        if (walls) {
+            if (cmb == 1) launch_stream_w<H, MODEL, true, 1>(ctx, a);
+            else if (cmb == 2) launch_stream_w<H, MODEL, true, 2>(ctx, a);
+            else launch_stream_w<H, MODEL, true>(ctx, a);
+        } else {
+            if (cmb == 1) launch_stream_w<H, MODEL, false, 1>(ctx, a);
+            else if (cmb == 2) launch_stream_w<H, MODEL, false, 2>(ctx, a);
+            else launch_stream_w<H, MODEL, false>(ctx, a);
+        }
     } else if (walls) {
         launch_stream_w<H, MODEL, true>(ctx, a);
     } else {

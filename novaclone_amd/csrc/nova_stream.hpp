@@ -50,9 +50,29 @@ constexpr int stage_lds()
     return 3 * model_nf<MODEL>() * (kTJ + 2 * H) * (kTK + 2 * H);
 }
 
+// A copy of x in a register of its own.  With two prefetch sets the value
+// taken into the queue must not stay in the register its load wrote: the
+// set is loaded again one plane later, and a queue entry sharing that
+// register makes the compiler move the new load's result at the loop
+// latch, which waits for the load to land.
+template <int PF>
+__device__ __attribute__((always_inline)) inline double fresh(double x)
+{
+    if constexpr (PF == 1) {
+        return x;
+    } else {
+        double r;
+        asm("v_mov_b64 %0, %1" : "=v"(r) : "v"(x));
+        return r;
+    }
+}
+
 // One tile t = (kt, jt, ic) of a stage: the block streams planes i of its
 // chunk for the (k, j) tile.  shbuf holds stage_lds() doubles of LDS.
-template <int H, int MODEL, bool WALLS, int kTK, int PF = 1, int kTJ = 256 / kTK>
+// Stage combination of a launch: CMB 0 reads the form at run time; the
+// SSP forms are compiled in where the launch knows them (1: alpha X0 +
+// beta (X + dX), step-start fields present; 2: X + dX, none read).
+template <int H, int MODEL, bool WALLS, int kTK, int PF = 1, int kTJ = 256 / kTK, int CMB = 0>
 __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, const StreamArgs<model_nf<MODEL>()> &a,
                                                                  int t, double *shbuf)
 {
@@ -169,7 +189,9 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
             for (int f = 0; f < NF; f++)
 #pragma unroll
                 for (int c = 0; c < 3; c++)
-                    P.nx0[f][c] = (model_x0<MODEL>(f) && a.X0[f]) ? ldp(a.X0[f], c, (int64_t)p * plane, col) : 0.0;
+                    P.nx0[f][c] = (model_x0<MODEL>(f) && (CMB == 1 || (CMB == 0 && a.X0[f])))
+                                      ? ldp(a.X0[f], c, (int64_t)p * plane, col)
+                                      : 0.0;
             if constexpr (ADJ) {
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
@@ -199,7 +221,9 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
     for (int f = 0; f < NF; f++)
 #pragma unroll
         for (int c = 0; c < 3; c++)
-            x0c[f][c] = (model_x0<MODEL>(f) && a.X0[f]) ? ldp(a.X0[f], c, (int64_t)i0 * plane, col) : 0.0;
+            x0c[f][c] = (model_x0<MODEL>(f) && (CMB == 1 || (CMB == 0 && a.X0[f])))
+                            ? ldp(a.X0[f], c, (int64_t)i0 * plane, col)
+                            : 0.0;
     double yc[3] = {0.0, 0.0, 0.0}, accc[3] = {0.0, 0.0, 0.0};
     if constexpr (ADJ) {
 #pragma unroll
@@ -243,8 +267,15 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                 for (int c = 0; c < 3; c++) {
                     const double du = sa.dt * r[c];
                     const double u = A(0, c, 0, 0);
-                    const double u0 = a.X0[0] ? x0c[0][c] : u;
-                    stp(a.out[0], c, pi, col, combine(sa, u, u0, du));
+                    if constexpr (CMB == 1) {
+                        const double v = u + du;
+                        stp(a.out[0], c, pi, col, sa.alpha * x0c[0][c] + sa.beta * v);
+                    } else if constexpr (CMB == 2) {
+                        stp(a.out[0], c, pi, col, u + du);
+                    } else {
+                        const double u0 = a.X0[0] ? x0c[0][c] : u;
+                        stp(a.out[0], c, pi, col, combine(sa, u, u0, du));
+                    }
                 }
             } else if constexpr (MODEL == MODEL_TLM) {
                 double r[3];
@@ -290,14 +321,14 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                 for (int c = 0; c < 3; c++) {
 #pragma unroll
                     for (int o = 0; o < 2 * H; o++) q[f][c][o] = q[f][c][o + 1];
-                    q[f][c][2 * H] = rsign<WALLS>(c, 0, Nxt.fq) * sgn_col[c] * Nxt.nq[f][c];
-                    x0c[f][c] = Nxt.nx0[f][c];
+                    q[f][c][2 * H] = fresh<PF>(rsign<WALLS>(c, 0, Nxt.fq) * sgn_col[c] * Nxt.nq[f][c]);
+                    x0c[f][c] = fresh<PF>(Nxt.nx0[f][c]);
                 }
             if constexpr (ADJ) {
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    yc[c] = Nxt.ny[c];
-                    accc[c] = Nxt.nacc[c];
+                    yc[c] = fresh<PF>(Nxt.ny[c]);
+                    accc[c] = fresh<PF>(Nxt.nacc[c]);
                 }
             }
         }
@@ -320,7 +351,7 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
     }
 }
 
-template <int H, int MODEL, bool WALLS, int MINW, int kTK, int kTJ>
+template <int H, int MODEL, bool WALLS, int MINW, int kTK, int kTJ, int CMB>
 __global__ void __launch_bounds__(kTK *kTJ) __attribute__((amdgpu_waves_per_eu(MINW)))
 k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
 {
@@ -332,7 +363,7 @@ k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
     if (T % 8 == 0) t = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
     // shapes held to one or two waves per SIMD have the registers for a
     // second prefetch set
-    stage_tile<H, MODEL, WALLS, kTK, MINW <= 2 ? 2 : 1, kTJ>(g, a, t, sh);
+    stage_tile<H, MODEL, WALLS, kTK, MINW <= 2 ? 2 : 1, kTJ, CMB>(g, a, t, sh);
 }
 
 int pick_ichunk(const Grid &g, int nkt, int njt)
@@ -344,7 +375,7 @@ int pick_ichunk(const Grid &g, int nkt, int njt)
     return c;
 }
 
-template <int H, int MODEL, bool WALLS, int MINW, int TK, int TJ = 256 / TK>
+template <int H, int MODEL, bool WALLS, int CMB, int MINW, int TK, int TJ = 256 / TK>
 void launch_stream_tile(nova_ctx *ctx, StreamArgs<model_nf<MODEL>()> a)
 {
     const Grid &g = ctx->g;
@@ -354,11 +385,11 @@ void launch_stream_tile(nova_ctx *ctx, StreamArgs<model_nf<MODEL>()> a)
     a.ichunk = ic > 0 ? (ic < g.s[0] ? ic : g.s[0]) : pick_ichunk(g, a.nkt, a.njt);
     a.nic = (g.s[0] + a.ichunk - 1) / a.ichunk;
     const int T = a.nkt * a.njt * a.nic;
-    hipLaunchKernelGGL((k_stage_stream<H, MODEL, WALLS, MINW, TK, TJ>), dim3(T), dim3(TK, TJ, 1), 0, ctx->stream, g,
+    hipLaunchKernelGGL((k_stage_stream<H, MODEL, WALLS, MINW, TK, TJ, CMB>), dim3(T), dim3(TK, TJ, 1), 0, ctx->stream, g,
                        a);
 }
 
-template <int H, int MODEL, bool WALLS>
+template <int H, int MODEL, bool WALLS, int CMB = 0>
 void launch_stream_w(nova_ctx *ctx, const StreamArgs<model_nf<MODEL>()> &a)
 {
     // variants 2-4 hold one-field models to 3 waves per SIMD; two queued
@@ -366,24 +397,24 @@ void launch_stream_w(nova_ctx *ctx, const StreamArgs<model_nf<MODEL>()> &a)
     // SIMD (spilled shapes measure 3-4x slower)
     constexpr int W = model_nf<MODEL>() == 1 ? 3 : 1;
     switch (ctx->stage_variant[MODEL]) {
-    case 0: launch_stream_tile<H, MODEL, WALLS, 1, 64>(ctx, a); break;
-    case 2: launch_stream_tile<H, MODEL, WALLS, W, 64>(ctx, a); break;
-    case 4: launch_stream_tile<H, MODEL, WALLS, W, 16>(ctx, a); break;
+    case 0: launch_stream_tile<H, MODEL, WALLS, CMB, 1, 64>(ctx, a); break;
+    case 2: launch_stream_tile<H, MODEL, WALLS, CMB, W, 64>(ctx, a); break;
+    case 4: launch_stream_tile<H, MODEL, WALLS, CMB, W, 16>(ctx, a); break;
     // two waves per SIMD with two planes of loads in flight (one-field models)
-    case 5: launch_stream_tile<H, MODEL, WALLS, W == 3 ? 2 : 1, 32>(ctx, a); break;
-    case 6: launch_stream_tile<H, MODEL, WALLS, W == 3 ? 2 : 1, 16>(ctx, a); break;
+    case 5: launch_stream_tile<H, MODEL, WALLS, CMB, W == 3 ? 2 : 1, 32>(ctx, a); break;
+    case 6: launch_stream_tile<H, MODEL, WALLS, CMB, W == 3 ? 2 : 1, 16>(ctx, a); break;
     // 512-thread tiles (32 x 16, 64 x 8): a smaller halo ring per node.  A
     // 512-thread block is held to 256 registers per lane, which two queued
     // fields exceed, so two-field models keep their 256-thread shapes here
     case 7:
-        if constexpr (W == 3) launch_stream_tile<H, MODEL, WALLS, 2, 32, 16>(ctx, a);
-        else launch_stream_tile<H, MODEL, WALLS, 1, 32>(ctx, a);
+        if constexpr (W == 3) launch_stream_tile<H, MODEL, WALLS, CMB, 2, 32, 16>(ctx, a);
+        else launch_stream_tile<H, MODEL, WALLS, CMB, 1, 32>(ctx, a);
         break;
     case 8:
-        if constexpr (W == 3) launch_stream_tile<H, MODEL, WALLS, 2, 64, 8>(ctx, a);
-        else launch_stream_tile<H, MODEL, WALLS, 1, 64>(ctx, a);
+        if constexpr (W == 3) launch_stream_tile<H, MODEL, WALLS, CMB, 2, 64, 8>(ctx, a);
+        else launch_stream_tile<H, MODEL, WALLS, CMB, 1, 64>(ctx, a);
         break;
-    default: launch_stream_tile<H, MODEL, WALLS, W, 32>(ctx, a); break;
+    default: launch_stream_tile<H, MODEL, WALLS, CMB, W, 32>(ctx, a); break;
     }
 }
 
