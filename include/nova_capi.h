@@ -78,6 +78,7 @@ typedef struct nova_poisson nova_poisson;
 typedef struct nova_solver nova_solver;
 typedef struct nova_adjoint nova_adjoint;
 typedef struct nova_ensemble nova_ensemble;
+typedef struct nova_wc nova_wc;
 
 /* ---- context / memory (replaces Mesh construction, Mesh.cpp:24-179, and
  * Field storage, src/Field/Field.hxx:21-134) ---- */
@@ -234,6 +235,19 @@ int nova_weighted_dot(nova_ctx *ctx, const double *a, const double *b, int ncomp
 /* J = dV sum_nodes w |curl U|^2 (central stencil curl); dJdU (may be NULL)
  * receives its gradient 2 dV curl(curl U). */
 int nova_enstrophy(nova_ctx *ctx, const double *U, double *J, double *dJdU);
+
+/* ---- weakly-compressible step (app/TG/TG.cpp:71-76: rhoEqn.H, pEqn.H,
+ * UEqn.H) ---- */
+/* A solver on ctx's grid with viscosity mu and p = pc rho^2 (pEqn.H:2 has
+ * pc = 50, i.e. c = 10).  Allocates six stage buffers (12 fields). */
+int nova_wc_create(nova_ctx *ctx, double mu, double pc, nova_wc **out);
+int nova_wc_destroy(nova_wc *w);
+/* Advance nsteps of size dt.  Per step: a classic RK4 of rho with U frozen
+ * (drho = dt (-div(rho U)), rhoEqn.H:19-33), p = pc rho^2, and a classic
+ * RK4 of rho U with rho and p frozen (drhoU = dt (-div(rho U, U) - grad p +
+ * laplacian(mu, U)), U = rhoUOld/rho + c drhoU/rho, UEqn.H:16-33).  U
+ * (3 components) and rho in/out, p out; device pointers, stored layout. */
+int nova_wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p);
 
 /* ---- ensembles of uneven instances (no reference counterpart: NOVA runs
  * one case per process; the step of each instance is that of

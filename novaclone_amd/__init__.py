@@ -102,6 +102,9 @@ _SIGS = {
     "nova_reduce_dissipation": [_P, _P, _D, _DP],
     "nova_mhd_diag": [_P, _P, _P, _D, _D, _DP],
     "nova_tdma_batched": [_P, _I, _I, _P, _P, _P, _P, _P, _I],
+    "nova_wc_create": [_P, _D, _D, C.POINTER(_P)],
+    "nova_wc_destroy": [_P],
+    "nova_wc_step": [_P, _D, _I, _P, _P, _P],
     "nova_ensemble_create": [C.POINTER(InstanceDesc), _I, _I, C.POINTER(_P)],
     "nova_ensemble_destroy": [_P],
     "nova_ensemble_set_stream": [_P, _P],
@@ -391,6 +394,33 @@ class Solver:
     def close(self):
         if getattr(self, "_h", None):
             lib().nova_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WC:
+    """The weakly-compressible step (app/TG/TG.cpp:71-76 with rhoEqn.H,
+    pEqn.H, UEqn.H): a classic RK4 of rho with U frozen, p = pc rho^2
+    (pc = 50 in pEqn.H), and a classic RK4 of rho U with rho and p frozen."""
+
+    def __init__(self, mesh: Mesh, mu: float, pc: float = 50.0):
+        self.mesh = mesh
+        h = C.c_void_p()
+        _check(lib().nova_wc_create(mesh.handle, float(mu), float(pc), C.byref(h)), "nova_wc_create")
+        self._h = h
+
+    def step(self, dt: float, nsteps: int, U, rho, p) -> None:
+        """Advance nsteps: U (3 components) and rho in/out, p out."""
+        _check(lib().nova_wc_step(self._h, float(dt), int(nsteps), _ptr(U), _ptr(rho), _ptr(p)), "nova_wc_step")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nova_wc_destroy(self._h)
             self._h = None
 
     def __del__(self):

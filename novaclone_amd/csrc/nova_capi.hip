@@ -115,6 +115,9 @@ int ensemble_copy(nova_ensemble *e, int i, int store, double *U, double *B, doub
 int ensemble_step(nova_ensemble *e, double dt, int nsteps);
 int ensemble_use_graph(nova_ensemble *e, int enable);
 int ensemble_info(nova_ensemble *e, int64_t out[6]);
+int wc_create(nova_ctx *ctx, double mu, double pc, nova_wc **out);
+int wc_destroy(nova_wc *w);
+int wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p);
 
 // Mesh::z (src/Mesh/Mesh.cpp:373-407) for padded index kp on one rank.
 static double mesh_z(const nova_grid_desc &d, double dz, double hb, int m, int kp)
@@ -604,6 +607,20 @@ int nova_adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t repor
 {
     NOVA_REQUIRE(ad && reps >= 1 && (!report || report_len > 0), "bad argument");
     NOVA_GUARD(adjoint_autotune(ad, reps, report, report_len));
+}
+
+int nova_wc_create(nova_ctx *ctx, double mu, double pc, nova_wc **out)
+{
+    NOVA_REQUIRE(ctx && out, "null argument");
+    NOVA_GUARD(wc_create(ctx, mu, pc, out));
+}
+
+int nova_wc_destroy(nova_wc *w) { NOVA_GUARD(wc_destroy(w)); }
+
+int nova_wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p)
+{
+    NOVA_REQUIRE(w && U && rho && p && nsteps >= 0, "bad argument");
+    NOVA_GUARD(wc_step(w, dt, nsteps, U, rho, p));
 }
 
 int nova_ensemble_create(const nova_instance_desc *inst, int n, int device, nova_ensemble **out)

@@ -1,0 +1,337 @@
+// nova_wc.hip -- the weakly-compressible step (app/TG/TG.cpp:71-76 with
+// rhoEqn.H, pEqn.H and UEqn.H) on gfx950.
+//
+// The reference makes, per stage, one pass per operator and per algebraic
+// term over pool temporaries: rho*U, ex::div, the negation, the dt scaling,
+// the accumulator update and the stage value for rho (rhoEqn.H:19-33), and
+// rho*U, ex::div(rho*U, U), ex::grad(p), ex::laplacian(mu, U), their sum,
+// rhoU accumulator and U = rhoUOld/rho + c drhoU/rho for U (UEqn.H:16-33).
+// Here each stage is one kernel: a thread owns a (j, k) column of a chunk
+// of i planes, keeps the i-neighbours of its column in register queues and
+// reads the j and k neighbours through the caches; the products rho*U are
+// formed at the neighbour, as the reference's product field holds them.
+//
+// Stage buffers: rho ping-pongs user -> ra -> rb -> ra -> user with the
+// accumulator in rc; U ping-pongs user -> ua -> ub -> ua -> user with the
+// rho U accumulator in uc.  The user's U stays the step-start value until
+// the last U stage, so rhoUOld = rho*U is formed from it at the node.
+#include "nova_internal.hpp"
+#include "nova_stencil.hpp"
+
+struct nova_wc {
+    nova_ctx *ctx = nullptr;
+    double mu = 0.0, pc = 50.0;
+    double *ra = nullptr, *rb = nullptr, *rc = nullptr;  // one component each
+    double *ua = nullptr, *ub = nullptr, *uc = nullptr;  // three components each
+};
+
+namespace nova {
+namespace {
+
+constexpr int kWcTK = 64, kWcTJ = 4;  // threads of a block along k and j
+
+struct WcRhoArgs {
+    const double *rho_in;  // stage value (stencil)
+    const double *U;       // frozen velocity (stencil)
+    const double *rho0;    // step-start rho
+    const double *rc_in;   // accumulator
+    double *rho_out, *rc_out, *p;
+    double dt, w, cs, pc;
+    int last;
+    int ichunk, nkt, njt, nic;
+};
+
+struct WcUArgs {
+    const double *U_in;    // stage value (stencil)
+    const double *U0;      // step-start U (null: U_in)
+    const double *rho;     // frozen rho (stencil)
+    const double *p;       // frozen p (stencil)
+    const double *ruc_in;  // rho U accumulator (null: rho U0)
+    double *U_out, *ruc_out;
+    double dt, w, cs, mu;
+    int last;
+    int ichunk, nkt, njt, nic;
+};
+
+// Tile and chunk of block b (consecutive tiles on one XCD).
+struct WcTile {
+    int k, j, i0, i1;
+};
+
+__device__ inline WcTile wc_tile(const Grid &g, int ichunk, int nkt, int njt, int nic)
+{
+    const int T = nkt * njt * nic;
+    int t = blockIdx.x;
+    if (T % 8 == 0) t = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
+    const int kt = t % nkt, jt = (t / nkt) % njt, ic = t / (nkt * njt);
+    WcTile w;
+    w.k = kt * kWcTK + threadIdx.x;
+    w.j = jt * kWcTJ + threadIdx.y;
+    w.i0 = ic * ichunk;
+    w.i1 = w.i0 + ichunk < g.s[0] ? w.i0 + ichunk : g.s[0];
+    return w;
+}
+
+// Node offset and reflection flags of (i, j, k) (indices at most H outside).
+template <bool WALLS>
+__device__ inline int64_t wc_node(const Grid &g, int i, int j, int k, int (&fl)[3])
+{
+    const int64_t plane = (int64_t)g.s[1] * g.s[2];
+    return (int64_t)wrap<WALLS>(i, g.n[0], g.wall[0], fl[0]) * plane +
+           (int64_t)wrap<WALLS>(j, g.n[1], g.wall[1], fl[1]) * g.s[2] + wrap<WALLS>(k, g.n[2], g.wall[2], fl[2]);
+}
+
+// Component c of a vector field at a node reached through reflections.
+template <bool WALLS>
+__device__ inline double wc_vec(const Grid &g, const double *F, int c, int64_t off, const int (&fl)[3])
+{
+    return rsign<WALLS>(c, 0, fl[0]) * rsign<WALLS>(c, 1, fl[1]) * rsign<WALLS>(c, 2, fl[2]) * F[c * g.np + off];
+}
+
+// rhoEqn.H:19-33: drho = dt * (-div(rho*U)) (src/ex/div/div.hxx:22-81),
+// rc += w drho, rho = rho0 + cs drho (the last stage: rho = rc), and on
+// the last stage p = pc rho^2 (pEqn.H:2).
+template <int H, bool WALLS>
+__global__ void __launch_bounds__(kWcTK *kWcTJ) k_wc_rho(Grid g, WcRhoArgs a)
+{
+    const WcTile tl = wc_tile(g, a.ichunk, a.nkt, a.njt, a.nic);
+    const int k = tl.k, j = tl.j;
+    if (k >= g.s[2] || j >= g.s[1]) return;  // no barriers below
+    // the d-component of rho*U at a node
+    auto flux = [&](int ii, int jj, int kk, int d) -> double {
+        int fl[3];
+        const int64_t off = wc_node<WALLS>(g, ii, jj, kk, fl);
+        return a.rho_in[off] * wc_vec<WALLS>(g, a.U, d, off, fl);
+    };
+    double q[2 * H + 1];  // x-flux of the own column, planes i - H .. i + H
+#pragma unroll
+    for (int o = 0; o < 2 * H; o++) q[o] = flux(tl.i0 - H + o, j, k, 0);
+    const double zi = g.zinv[k];
+    const int64_t plane = (int64_t)g.s[1] * g.s[2];
+    for (int i = tl.i0; i < tl.i1; i++) {
+        q[2 * H] = flux(i + H, j, k, 0);
+        double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+        for (int l = 0; l < H; l++) {
+            ax += g.a[0][l] * (q[H + 1 + l] - q[H - 1 - l]);
+            ay += g.a[1][l] * (flux(i, j + 1 + l, k, 1) - flux(i, j - 1 - l, k, 1));
+            az += g.a[2][l] * (flux(i, j, k + 1 + l, 2) - flux(i, j, k - 1 - l, 2));
+        }
+        const double dr = a.dt * (-((ax + ay) + az * zi));
+        const int64_t qi = (int64_t)i * plane + (int64_t)j * g.s[2] + k;
+        const double rc = a.rc_in[qi] + a.w * dr;
+        if (a.last) {
+            a.rho_out[qi] = rc;
+            if (a.p) a.p[qi] = a.pc * (rc * rc);
+        } else {
+            a.rho_out[qi] = a.rho0[qi] + a.cs * dr;
+            a.rc_out[qi] = rc;
+        }
+#pragma unroll
+        for (int o = 0; o < 2 * H; o++) q[o] = q[o + 1];
+    }
+}
+
+// UEqn.H:16-33: drhoU = dt * ((-div(rho*U, U) - grad(p)) + laplacian(mu, U))
+// (div.hxx:84-172, grad.cpp:25-62, laplacian.hxx:130-191), ruc += w drhoU,
+// U = rhoUOld/rho + cs (drhoU/rho) (the last stage: U = ruc/rho), with
+// rhoUOld = rho*U0.
+template <int H, bool WALLS>
+__global__ void __launch_bounds__(kWcTK *kWcTJ) k_wc_u(Grid g, WcUArgs a)
+{
+    const WcTile tl = wc_tile(g, a.ichunk, a.nkt, a.njt, a.nic);
+    const int k = tl.k, j = tl.j;
+    if (k >= g.s[2] || j >= g.s[1]) return;  // no barriers below
+    constexpr int QW = 2 * H + 1;
+    // queues of the own column, planes i - H .. i + H: rho, U, p
+    double qr[QW], qu[3][QW], qp[QW];
+    auto load_plane = [&](int o, int ii) {
+        int fl[3];
+        const int64_t off = wc_node<WALLS>(g, ii, j, k, fl);
+        qr[o] = a.rho[off];
+        qp[o] = a.p[off];
+#pragma unroll
+        for (int c = 0; c < 3; c++) qu[c][o] = wc_vec<WALLS>(g, a.U_in, c, off, fl);
+    };
+#pragma unroll
+    for (int o = 0; o < 2 * H; o++) load_plane(o, tl.i0 - H + o);
+    const double zi = g.zinv[k], zl = g.zlap[k], zm = g.zmet[k];
+    const int64_t plane = (int64_t)g.s[1] * g.s[2];
+    for (int i = tl.i0; i < tl.i1; i++) {
+        load_plane(2 * H, i + H);
+        // field f at the node displaced by o along d: 0 = rho*U, 1 = U, 2 = p
+        auto A = [&](int f, int c, int d, int o) -> double {
+            if (d == 0 || o == 0) {
+                const int e = H + (d == 0 ? o : 0);
+                return f == 0 ? qr[e] * qu[c][e] : (f == 1 ? qu[c][e] : qp[e]);
+            }
+            int fl[3];
+            const int64_t off = d == 1 ? wc_node<WALLS>(g, i, j + o, k, fl) : wc_node<WALLS>(g, i, j, k + o, fl);
+            if (f == 2) return a.p[off];
+            const double u = wc_vec<WALLS>(g, a.U_in, c, off, fl);
+            return f == 0 ? a.rho[off] * u : u;
+        };
+        const int64_t qi = (int64_t)i * plane + (int64_t)j * g.s[2] + k;
+        const double r = qr[H];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double gp = 0.0;
+#pragma unroll
+            for (int l = 0; l < H; l++) gp += g.a[c][l] * (A(2, 0, c, l + 1) - A(2, 0, c, -l - 1));
+            if (c == 2) gp *= zi;
+            const double du = a.dt * (((-div_xy<H>(g, A, 0, 1, c, zi)) - gp) + lap_c<H>(g, A, 1, c, a.mu, zl, zm));
+            const double u0 = a.U0 ? a.U0[c * g.np + qi] : qu[c][H];
+            const double ruold = r * u0;
+            const double ruc = (a.ruc_in ? a.ruc_in[c * g.np + qi] : ruold) + a.w * du;
+            if (a.last) {
+                a.U_out[c * g.np + qi] = ruc / r;
+            } else {
+                a.U_out[c * g.np + qi] = ruold / r + a.cs * (du / r);
+                a.ruc_out[c * g.np + qi] = ruc;
+            }
+        }
+#pragma unroll
+        for (int o = 0; o < 2 * H; o++) {
+            qr[o] = qr[o + 1];
+            qp[o] = qp[o + 1];
+#pragma unroll
+            for (int c = 0; c < 3; c++) qu[c][o] = qu[c][o + 1];
+        }
+    }
+}
+
+template <class Args>
+void wc_shape(const Grid &g, Args &a)
+{
+    a.nkt = (g.s[2] + kWcTK - 1) / kWcTK;
+    a.njt = (g.s[1] + kWcTJ - 1) / kWcTJ;
+    // about 2048 blocks, chunks of 8..64 planes
+    int64_t per = (int64_t)g.s[0] * a.nkt * a.njt / 2048;
+    int c = (int)(per < 8 ? 8 : (per > 64 ? 64 : per));
+    a.ichunk = c < g.s[0] ? c : g.s[0];
+    a.nic = (g.s[0] + a.ichunk - 1) / a.ichunk;
+}
+
+template <int H>
+int launch_wc_h(nova_ctx *ctx, const WcRhoArgs *ra, const WcUArgs *ua)
+{
+    const Grid &g = ctx->g;
+    const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
+    if (ra) {
+        TimedScope ts(ctx, "wc_rho_stage");
+        WcRhoArgs a = *ra;
+        wc_shape(g, a);
+        const dim3 grid(a.nkt * a.njt * a.nic), block(kWcTK, kWcTJ, 1);
+        if (walls) hipLaunchKernelGGL((k_wc_rho<H, true>), grid, block, 0, ctx->stream, g, a);
+        else hipLaunchKernelGGL((k_wc_rho<H, false>), grid, block, 0, ctx->stream, g, a);
+        return check_launch("wc_rho_stage");
+    }
+    TimedScope ts(ctx, "wc_u_stage");
+    WcUArgs a = *ua;
+    wc_shape(g, a);
+    const dim3 grid(a.nkt * a.njt * a.nic), block(kWcTK, kWcTJ, 1);
+    if (walls) hipLaunchKernelGGL((k_wc_u<H, true>), grid, block, 0, ctx->stream, g, a);
+    else hipLaunchKernelGGL((k_wc_u<H, false>), grid, block, 0, ctx->stream, g, a);
+    return check_launch("wc_u_stage");
+}
+
+int launch_wc(nova_ctx *ctx, const WcRhoArgs *ra, const WcUArgs *ua)
+{
+    switch (ctx->g.h) {
+    case 1: return launch_wc_h<1>(ctx, ra, ua);
+    case 2: return launch_wc_h<2>(ctx, ra, ua);
+    case 3: return launch_wc_h<3>(ctx, ra, ua);
+    case 4: return launch_wc_h<4>(ctx, ra, ua);
+    default: return fail(NOVA_ERR_ARG, "stencil order");
+    }
+}
+
+// Classic RK4 of the reference apps (app/TG/TG.cpp:57): weights and the
+// stage offsets (int(s/2)+1)/2.
+constexpr double kRk4W[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+constexpr double kRk4C[3] = {0.5, 0.5, 1.0};
+
+}  // namespace
+
+int wc_create(nova_ctx *ctx, double mu, double pc, nova_wc **out)
+{
+    if (!ctx || !out) return fail(NOVA_ERR_ARG, "null argument");
+    if (!(mu >= 0.0)) return fail(NOVA_ERR_ARG, "mu must be >= 0");
+    nova_wc *w = new nova_wc;
+    w->ctx = ctx;
+    w->mu = mu;
+    w->pc = pc;
+    const size_t b1 = sizeof(double) * ctx->g.np, b3 = 3 * b1;
+    hipError_t e = hipSuccess;
+    double **one[3] = {&w->ra, &w->rb, &w->rc};
+    double **three[3] = {&w->ua, &w->ub, &w->uc};
+    for (int q = 0; q < 3 && e == hipSuccess; q++) e = hipMalloc(one[q], b1);
+    for (int q = 0; q < 3 && e == hipSuccess; q++) e = hipMalloc(three[q], b3);
+    if (e != hipSuccess) {
+        for (int q = 0; q < 3; q++) {
+            if (*one[q]) (void)hipFree(*one[q]);
+            if (*three[q]) (void)hipFree(*three[q]);
+        }
+        delete w;
+        return hip_fail(e, "hipMalloc (weakly-compressible stage buffers)");
+    }
+    *out = w;
+    return NOVA_OK;
+}
+
+int wc_destroy(nova_wc *w)
+{
+    if (!w) return NOVA_OK;
+    for (double *b : {w->ra, w->rb, w->rc, w->ua, w->ub, w->uc})
+        if (b) (void)hipFree(b);
+    delete w;
+    return NOVA_OK;
+}
+
+// One step per iteration: four rho stages (U frozen), then four U stages
+// (rho and p frozen).
+int wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p)
+{
+    nova_ctx *ctx = w->ctx;
+    for (int n = 0; n < nsteps; n++) {
+        const double *rin[4] = {rho, w->ra, w->rb, w->ra};
+        double *rout[4] = {w->ra, w->rb, w->ra, rho};
+        for (int s = 0; s < 4; s++) {
+            WcRhoArgs a{};
+            a.rho_in = rin[s];
+            a.U = U;
+            a.rho0 = rho;
+            a.rc_in = s == 0 ? rho : w->rc;
+            a.rho_out = rout[s];
+            a.rc_out = w->rc;
+            a.p = p;
+            a.dt = dt;
+            a.w = kRk4W[s];
+            a.cs = s < 3 ? kRk4C[s] : 0.0;
+            a.pc = w->pc;
+            a.last = s == 3;
+            if (int rc = launch_wc(ctx, &a, nullptr)) return rc;
+        }
+        const double *uin[4] = {U, w->ua, w->ub, w->ua};
+        double *uout[4] = {w->ua, w->ub, w->ua, U};
+        for (int s = 0; s < 4; s++) {
+            WcUArgs a{};
+            a.U_in = uin[s];
+            a.U0 = (s == 0 || s == 3) ? nullptr : U;
+            a.rho = rho;
+            a.p = p;
+            a.ruc_in = s == 0 ? nullptr : w->uc;
+            a.U_out = uout[s];
+            a.ruc_out = w->uc;
+            a.dt = dt;
+            a.w = kRk4W[s];
+            a.cs = s < 3 ? kRk4C[s] : 0.0;
+            a.mu = w->mu;
+            a.last = s == 3;
+            if (int rc = launch_wc(ctx, nullptr, &a)) return rc;
+        }
+    }
+    return NOVA_OK;
+}
+
+}  // namespace nova

@@ -868,6 +868,71 @@ void orc_mhd_step(const orc_mesh *M, orc_poisson *P, double *U, double *B, doubl
     for (int r = 0; r < 4; r++) free(dB[r]);
 }
 
+/* Weakly-compressible step, app/TG/TG.cpp:71-76: rhoEqn.H:10-35 (classic
+ * RK4 of rho with U frozen), pEqn.H:2 (p = pc rho^2; pc = 50 for c = 10),
+ * UEqn.H:12-35 (classic RK4 of rho U with rho and p frozen).  Field algebra
+ * acts on every node of the padded array (Field.hxx:617-1150), the
+ * operators on the interior with zero ghosts, and rho and U are refilled
+ * after every stage (rhoEqn.H:33, UEqn.H:33).  Stage s starts from the
+ * step-start value plus (int(s/2)+1)/2 of the previous increment; the
+ * accumulator adds RK4W[s] of each increment. */
+void orc_wc_step(const orc_mesh *M, double *U, double *rho, double *p, double mu, double pc, double dt)
+{
+    const size_t n1 = NPTS(M), n3 = n1 * 3;
+    double *rU = falloc(M, 3), *dv = falloc(M, 1), *dr = falloc(M, 1), *rc = falloc(M, 1), *r0 = falloc(M, 1);
+    double *t1 = falloc(M, 3), *t2 = falloc(M, 3), *gp = falloc(M, 3), *dU = falloc(M, 3);
+    double *rUc = falloc(M, 3), *rU0 = falloc(M, 3);
+    memcpy(r0, rho, sizeof(double) * n1);
+    memcpy(rc, rho, sizeof(double) * n1);
+    for (int rk = 0; rk < 4; rk++) {
+        #pragma omp parallel for
+        for (size_t q = 0; q < n1; q++)
+            for (int c = 0; c < 3; c++) rU[3 * q + c] = rho[q] * U[3 * q + c];
+        orc_div(M, rU, dv);
+        #pragma omp parallel for
+        for (size_t q = 0; q < n1; q++) {
+            dr[q] = dt * (-dv[q]);
+            rc[q] = rc[q] + RK4W[rk] * dr[q];
+        }
+        if (rk < 3) {
+            const double cs = (rk / 2 + 1) / 2.0;
+            #pragma omp parallel for
+            for (size_t q = 0; q < n1; q++) rho[q] = r0[q] + cs * dr[q];
+        } else {
+            memcpy(rho, rc, sizeof(double) * n1);
+        }
+        orc_fill(M, rho, 1, -1);
+    }
+    #pragma omp parallel for
+    for (size_t q = 0; q < n1; q++) p[q] = pc * (rho[q] * rho[q]);
+    #pragma omp parallel for
+    for (size_t q = 0; q < n1; q++)
+        for (int c = 0; c < 3; c++) rUc[3 * q + c] = rU0[3 * q + c] = rho[q] * U[3 * q + c];
+    orc_grad(M, p, gp);
+    for (int rk = 0; rk < 4; rk++) {
+        #pragma omp parallel for
+        for (size_t q = 0; q < n1; q++)
+            for (int c = 0; c < 3; c++) rU[3 * q + c] = rho[q] * U[3 * q + c];
+        orc_div_adv(M, rU, U, 3, t1);
+        orc_laplacian(M, mu, U, 3, t2);
+        #pragma omp parallel for
+        for (size_t q = 0; q < n3; q++) {
+            dU[q] = dt * (((-t1[q]) - gp[q]) + t2[q]);
+            rUc[q] = rUc[q] + RK4W[rk] * dU[q];
+        }
+        const double cs = (rk / 2 + 1) / 2.0;
+        #pragma omp parallel for
+        for (size_t q = 0; q < n1; q++)
+            for (int c = 0; c < 3; c++)
+                U[3 * q + c] = rk < 3 ? rU0[3 * q + c] / rho[q] + cs * (dU[3 * q + c] / rho[q])
+                                      : rUc[3 * q + c] / rho[q];
+        orc_fill(M, U, 3, 1);
+    }
+    (void)n3;
+    free(rU); free(dv); free(dr); free(rc); free(r0); free(t1); free(t2); free(gp); free(dU);
+    free(rUc); free(rU0);
+}
+
 /* ------------------------------------------------------------------ */
 /* Diagnostics. */
 
