@@ -143,6 +143,47 @@ def adjoint_ratio(nv, steps: int, every: int) -> dict:
             "J": J, "grad_norm": gn, "tuning": tuned}
 
 
+def wc_bench(nv, n: int, steps: int) -> dict:
+    """Weakly-compressible Taylor-Green (app/TG/TG.cpp): n^3 periodic FP64,
+    mu = 1/1600, p = 50 rho^2, per step a classic RK4 of rho and one of rho U,
+    one kernel per stage."""
+    import torch
+    G = nv.Mesh((n, n, n), (2 * math.pi,) * 3)
+    x = torch.arange(n, dtype=torch.float64, device="cuda") * (2 * math.pi / n)
+    X, Y, Z = torch.meshgrid(x, x, x, indexing="ij")
+    U = torch.stack([torch.sin(X) * torch.cos(Y) * torch.cos(Z), -torch.cos(X) * torch.sin(Y) * torch.cos(Z),
+                     0 * X]).contiguous()
+    p = (50 + (torch.cos(2 * X) + torch.cos(2 * Y)) * (torch.cos(2 * Z) + 2) / 16).unsqueeze(0).contiguous()
+    rho = torch.sqrt(p / 50.0).contiguous()
+    del X, Y, Z
+    W = nv.WC(G, 1.0 / 1600.0, 50.0)
+    dt = 1e-3
+    W.step(dt, 2, U, rho, p)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    W.step(dt, steps, U, rho, p)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    G.timing(True)
+    W.step(dt, 2, U, rho, p)
+    st = G.timing_collect()
+    G.timing(False)
+    # minimal-traffic model per node-step: rho stages 48 + 64 + 64 + 56 B,
+    # U stages 88 + 136 + 136 + 88 B (stencil inputs once, pointwise
+    # operands, outputs)
+    alg = 680.0 * n ** 3
+    finite = bool(torch.isfinite(U).all() and torch.isfinite(rho).all())
+    return {"workload": f"weakly-compressible TG {n}^3 periodic FP64, mu=1/1600, p=50 rho^2, classic RK4 of rho "
+                        f"and of rho U (app/TG), dt={dt}, {steps} steps",
+            "value": round(n ** 3 / (ms / 1e3) / 1e6, 1), "unit": "Mpoint-steps/s", "ms_per_step": round(ms, 4),
+            "alg_bytes_per_step": alg, "achieved_GBps": round(alg / (ms / 1e3) / 1e9, 1),
+            "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "kernels_us": {k: round(v[1] / v[0] * 1e3, 1) for k, v in st.items()},
+            "ke": nv.ke(G, U), "finite": finite}
+
+
 def ensemble_bench(nv, steps: int) -> dict:
     """48 uneven instances (16 x 32^3, 16 x 48^3, 12 x 64^3, 4 x 96^3), TG and
     Orszag-Tang alternating within each size, Re_i ~ U[400, 1600], 1e-3
@@ -297,6 +338,8 @@ def main() -> int:
     ap.add_argument("--no-compact", action="store_true")
     ap.add_argument("--compact-steps", type=int, default=20)
     ap.add_argument("--no-ot", action="store_true")
+    ap.add_argument("--no-wc", action="store_true")
+    ap.add_argument("--wc-steps", type=int, default=10)
     ap.add_argument("--ot-steps", type=int, default=10)
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--copy-calib", action="store_true",
@@ -397,6 +440,14 @@ def main() -> int:
         ot = {"explicit": ot_bench(nv, n, args.ot_steps, False), "compact": ot_bench(nv, n, args.ot_steps, True)}
         log(f"bench: OT explicit {ot['explicit']['ms_per_step']} ms/step, compact {ot['compact']['ms_per_step']} ms/step")
 
+    wc = None
+    if not args.no_wc:
+        torch.cuda.empty_cache()
+        log("bench: weakly-compressible TG ...")
+        wc = wc_bench(nv, n, args.wc_steps)
+        log(f"bench: WC {wc['ms_per_step']} ms/step, {wc['value']} Mpt-steps/s, "
+            f"{wc['achieved_GBps']} GB/s algorithmic")
+
     ens = None
     if not args.no_ensemble:
         torch.cuda.empty_cache()
@@ -436,6 +487,7 @@ def main() -> int:
         "ensemble": ens,
         "compact": cmp,
         "ot_mhd": ot,
+        "weakly_compressible": wc,
         "check": {"ke": ke, "maxabs_div": maxdiv},
     }
     print(json.dumps(out), flush=True)

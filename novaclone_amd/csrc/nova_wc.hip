@@ -6,10 +6,11 @@
 // the accumulator update and the stage value for rho (rhoEqn.H:19-33), and
 // rho*U, ex::div(rho*U, U), ex::grad(p), ex::laplacian(mu, U), their sum,
 // rhoU accumulator and U = rhoUOld/rho + c drhoU/rho for U (UEqn.H:16-33).
-// Here each stage is one kernel: a thread owns a (j, k) column of a chunk
-// of i planes, keeps the i-neighbours of its column in register queues and
-// reads the j and k neighbours through the caches; the products rho*U are
-// formed at the neighbour, as the reference's product field holds them.
+// Here each stage is one kernel: a block owns a 32 x 8 (k, j) tile and a
+// chunk of i planes, stages each haloed (j, k) plane of its inputs in LDS
+// and keeps the i-neighbours of each thread's column in register queues;
+// the products rho*U are formed per node, as the reference's product field
+// holds them.
 //
 // Stage buffers: rho ping-pongs user -> ra -> rb -> ra -> user with the
 // accumulator in rc; U ping-pongs user -> ua -> ub -> ua -> user with the
@@ -28,7 +29,7 @@ struct nova_wc {
 namespace nova {
 namespace {
 
-constexpr int kWcTK = 64, kWcTJ = 4;  // threads of a block along k and j
+constexpr int kWcTK = 32, kWcTJ = 8;  // threads of a block along k and j (one node each)
 
 struct WcRhoArgs {
     const double *rho_in;  // stage value (stencil)
@@ -53,9 +54,11 @@ struct WcUArgs {
     int ichunk, nkt, njt, nic;
 };
 
-// Tile and chunk of block b (consecutive tiles on one XCD).
+// The (k, j) tile and i chunk of this block (consecutive tiles on one
+// XCD), the thread's node and whether it lies in the grid.
 struct WcTile {
-    int k, j, i0, i1;
+    int k0, j0, k, j, i0, i1;
+    bool valid;
 };
 
 __device__ inline WcTile wc_tile(const Grid &g, int ichunk, int nkt, int njt, int nic)
@@ -65,20 +68,24 @@ __device__ inline WcTile wc_tile(const Grid &g, int ichunk, int nkt, int njt, in
     if (T % 8 == 0) t = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
     const int kt = t % nkt, jt = (t / nkt) % njt, ic = t / (nkt * njt);
     WcTile w;
-    w.k = kt * kWcTK + threadIdx.x;
-    w.j = jt * kWcTJ + threadIdx.y;
+    w.k0 = kt * kWcTK;
+    w.j0 = jt * kWcTJ;
+    w.k = w.k0 + threadIdx.x;
+    w.j = w.j0 + threadIdx.y;
+    w.valid = w.k < g.s[2] && w.j < g.s[1];
     w.i0 = ic * ichunk;
     w.i1 = w.i0 + ichunk < g.s[0] ? w.i0 + ichunk : g.s[0];
     return w;
 }
 
-// Node offset and reflection flags of (i, j, k) (indices at most H outside).
+// Node offset and reflection flags of (i, j, k) (indices at most H outside
+// the grid, or inside a partial tile).
 template <bool WALLS>
 __device__ inline int64_t wc_node(const Grid &g, int i, int j, int k, int (&fl)[3])
 {
     const int64_t plane = (int64_t)g.s[1] * g.s[2];
-    return (int64_t)wrap<WALLS>(i, g.n[0], g.wall[0], fl[0]) * plane +
-           (int64_t)wrap<WALLS>(j, g.n[1], g.wall[1], fl[1]) * g.s[2] + wrap<WALLS>(k, g.n[2], g.wall[2], fl[2]);
+    return (int64_t)bmap(i, g.n[0], g.wall[0], fl[0]) * plane + (int64_t)bmap(j, g.n[1], g.wall[1], fl[1]) * g.s[2] +
+           bmap(k, g.n[2], g.wall[2], fl[2]);
 }
 
 // Component c of a vector field at a node reached through reflections.
@@ -88,45 +95,108 @@ __device__ inline double wc_vec(const Grid &g, const double *F, int c, int64_t o
     return rsign<WALLS>(c, 0, fl[0]) * rsign<WALLS>(c, 1, fl[1]) * rsign<WALLS>(c, 2, fl[2]) * F[c * g.np + off];
 }
 
+// The thread's share of the haloed (j, k) plane of the tile: slot s holds
+// position e = tid + s * threads (row r = e / LK, column cc = e % LK; rows
+// j0 - H + r, columns k0 - H + cc) with its in-plane offset and reflection
+// flags, fixed for every plane.
+template <int H>
+struct WcSlots {
+    static constexpr int LK = kWcTK + 2 * H, LJ = kWcTJ + 2 * H;
+    static constexpr int NS = (LJ * LK + kWcTK * kWcTJ - 1) / (kWcTK * kWcTJ);
+    int64_t off[NS];
+    int pos[NS];  // e, or -1 past the plane
+    int fj[NS], fk[NS];
+};
+
+template <int H>
+__device__ inline void wc_slots(const Grid &g, const WcTile &tl, WcSlots<H> &S)
+{
+    constexpr int LK = WcSlots<H>::LK, LJ = WcSlots<H>::LJ;
+    const int tid = threadIdx.y * kWcTK + threadIdx.x;
+#pragma unroll
+    for (int s = 0; s < WcSlots<H>::NS; s++) {
+        const int e = tid + s * kWcTK * kWcTJ;
+        const int r = e / LK, cc = e - r * LK;
+        S.pos[s] = e < LJ * LK ? e : -1;
+        S.off[s] = (int64_t)bmap(tl.j0 - H + r, g.n[1], g.wall[1], S.fj[s]) * g.s[2] +
+                   bmap(tl.k0 - H + cc, g.n[2], g.wall[2], S.fk[s]);
+    }
+}
+
+// Stages NC values per node of the haloed plane i of the tile in LDS
+// (sh[c][e]); val(off, fl, c) gives component c at a node.
+template <int H, int NC, class Val>
+__device__ inline void wc_stage_plane(const Grid &g, const WcSlots<H> &S, int i, double *sh, Val val)
+{
+    constexpr int NP = WcSlots<H>::LK * WcSlots<H>::LJ;
+    int fl[3];
+    const int64_t po = (int64_t)bmap(i, g.n[0], g.wall[0], fl[0]) * g.s[1] * g.s[2];
+#pragma unroll
+    for (int s = 0; s < WcSlots<H>::NS; s++) {
+        if (S.pos[s] < 0) continue;
+        fl[1] = S.fj[s];
+        fl[2] = S.fk[s];
+#pragma unroll
+        for (int c = 0; c < NC; c++) sh[c * NP + S.pos[s]] = val(po + S.off[s], fl, c);
+    }
+}
+
 // rhoEqn.H:19-33: drho = dt * (-div(rho*U)) (src/ex/div/div.hxx:22-81),
 // rc += w drho, rho = rho0 + cs drho (the last stage: rho = rc), and on
-// the last stage p = pc rho^2 (pEqn.H:2).
+// the last stage p = pc rho^2 (pEqn.H:2).  The j- and k-fluxes rho U_y and
+// rho U_z of the plane are staged in LDS, the x-flux of the own column
+// walks a register queue.
 template <int H, bool WALLS>
 __global__ void __launch_bounds__(kWcTK *kWcTJ) k_wc_rho(Grid g, WcRhoArgs a)
 {
+    constexpr int LK = kWcTK + 2 * H, LJ = kWcTJ + 2 * H;
+    __shared__ double sh[2 * LJ * LK];
     const WcTile tl = wc_tile(g, a.ichunk, a.nkt, a.njt, a.nic);
-    const int k = tl.k, j = tl.j;
-    if (k >= g.s[2] || j >= g.s[1]) return;  // no barriers below
-    // the d-component of rho*U at a node
-    auto flux = [&](int ii, int jj, int kk, int d) -> double {
-        int fl[3];
-        const int64_t off = wc_node<WALLS>(g, ii, jj, kk, fl);
+    const int tk = threadIdx.x, tj = threadIdx.y;
+    WcSlots<H> slots;
+    wc_slots<H>(g, tl, slots);
+    int f0;
+    const int km = bmap(tl.k, g.n[2], g.wall[2], f0);
+    auto flux = [&](int64_t off, const int (&fl)[3], int d) -> double {
         return a.rho_in[off] * wc_vec<WALLS>(g, a.U, d, off, fl);
     };
     double q[2 * H + 1];  // x-flux of the own column, planes i - H .. i + H
 #pragma unroll
-    for (int o = 0; o < 2 * H; o++) q[o] = flux(tl.i0 - H + o, j, k, 0);
-    const double zi = g.zinv[k];
+    for (int o = 0; o < 2 * H; o++) {
+        int fl[3];
+        q[o] = flux(wc_node<WALLS>(g, tl.i0 - H + o, tl.j, tl.k, fl), fl, 0);
+    }
+    const double zi = g.zinv[km];
     const int64_t plane = (int64_t)g.s[1] * g.s[2];
+    auto S = [&](int c, int r, int cc) -> double { return sh[(c * LJ + r) * LK + cc]; };
     for (int i = tl.i0; i < tl.i1; i++) {
-        q[2 * H] = flux(i + H, j, k, 0);
-        double ax = 0.0, ay = 0.0, az = 0.0;
+        {
+            int fl[3];
+            q[2 * H] = flux(wc_node<WALLS>(g, i + H, tl.j, tl.k, fl), fl, 0);
+        }
+        wc_stage_plane<H, 2>(g, slots, i, sh,
+                             [&](int64_t off, const int (&fl)[3], int c) { return flux(off, fl, c + 1); });
+        __syncthreads();
+        if (tl.valid) {
+            double ax = 0.0, ay = 0.0, az = 0.0;
 #pragma unroll
-        for (int l = 0; l < H; l++) {
-            ax += g.a[0][l] * (q[H + 1 + l] - q[H - 1 - l]);
-            ay += g.a[1][l] * (flux(i, j + 1 + l, k, 1) - flux(i, j - 1 - l, k, 1));
-            az += g.a[2][l] * (flux(i, j, k + 1 + l, 2) - flux(i, j, k - 1 - l, 2));
+            for (int l = 0; l < H; l++) {
+                ax += g.a[0][l] * (q[H + 1 + l] - q[H - 1 - l]);
+                ay += g.a[1][l] * (S(0, tj + H + 1 + l, tk + H) - S(0, tj + H - 1 - l, tk + H));
+                az += g.a[2][l] * (S(1, tj + H, tk + H + 1 + l) - S(1, tj + H, tk + H - 1 - l));
+            }
+            const double dr = a.dt * (-((ax + ay) + az * zi));
+            const int64_t qi = (int64_t)i * plane + (int64_t)tl.j * g.s[2] + tl.k;
+            const double rc = a.rc_in[qi] + a.w * dr;
+            if (a.last) {
+                a.rho_out[qi] = rc;
+                if (a.p) a.p[qi] = a.pc * (rc * rc);
+            } else {
+                a.rho_out[qi] = a.rho0[qi] + a.cs * dr;
+                a.rc_out[qi] = rc;
+            }
         }
-        const double dr = a.dt * (-((ax + ay) + az * zi));
-        const int64_t qi = (int64_t)i * plane + (int64_t)j * g.s[2] + k;
-        const double rc = a.rc_in[qi] + a.w * dr;
-        if (a.last) {
-            a.rho_out[qi] = rc;
-            if (a.p) a.p[qi] = a.pc * (rc * rc);
-        } else {
-            a.rho_out[qi] = a.rho0[qi] + a.cs * dr;
-            a.rc_out[qi] = rc;
-        }
+        __syncthreads();
 #pragma unroll
         for (int o = 0; o < 2 * H; o++) q[o] = q[o + 1];
     }
@@ -135,19 +205,25 @@ __global__ void __launch_bounds__(kWcTK *kWcTJ) k_wc_rho(Grid g, WcRhoArgs a)
 // UEqn.H:16-33: drhoU = dt * ((-div(rho*U, U) - grad(p)) + laplacian(mu, U))
 // (div.hxx:84-172, grad.cpp:25-62, laplacian.hxx:130-191), ruc += w drhoU,
 // U = rhoUOld/rho + cs (drhoU/rho) (the last stage: U = ruc/rho), with
-// rhoUOld = rho*U0.
+// rhoUOld = rho*U0.  rho, U and p of the plane are staged in LDS; the own
+// column's i-neighbours walk register queues.
 template <int H, bool WALLS>
 __global__ void __launch_bounds__(kWcTK *kWcTJ) k_wc_u(Grid g, WcUArgs a)
 {
-    const WcTile tl = wc_tile(g, a.ichunk, a.nkt, a.njt, a.nic);
-    const int k = tl.k, j = tl.j;
-    if (k >= g.s[2] || j >= g.s[1]) return;  // no barriers below
+    constexpr int LK = kWcTK + 2 * H, LJ = kWcTJ + 2 * H;
     constexpr int QW = 2 * H + 1;
+    __shared__ double sh[5 * LJ * LK];  // rho, U_x, U_y, U_z, p
+    const WcTile tl = wc_tile(g, a.ichunk, a.nkt, a.njt, a.nic);
+    const int tk = threadIdx.x, tj = threadIdx.y;
+    WcSlots<H> slots;
+    wc_slots<H>(g, tl, slots);
+    int f0;
+    const int km = bmap(tl.k, g.n[2], g.wall[2], f0);
     // queues of the own column, planes i - H .. i + H: rho, U, p
     double qr[QW], qu[3][QW], qp[QW];
     auto load_plane = [&](int o, int ii) {
         int fl[3];
-        const int64_t off = wc_node<WALLS>(g, ii, j, k, fl);
+        const int64_t off = wc_node<WALLS>(g, ii, tl.j, tl.k, fl);
         qr[o] = a.rho[off];
         qp[o] = a.p[off];
 #pragma unroll
@@ -155,41 +231,51 @@ __global__ void __launch_bounds__(kWcTK *kWcTJ) k_wc_u(Grid g, WcUArgs a)
     };
 #pragma unroll
     for (int o = 0; o < 2 * H; o++) load_plane(o, tl.i0 - H + o);
-    const double zi = g.zinv[k], zl = g.zlap[k], zm = g.zmet[k];
+    const double zi = g.zinv[km], zl = g.zlap[km], zm = g.zmet[km];
     const int64_t plane = (int64_t)g.s[1] * g.s[2];
+    auto S = [&](int c, int r, int cc) -> double { return sh[(c * LJ + r) * LK + cc]; };
     for (int i = tl.i0; i < tl.i1; i++) {
         load_plane(2 * H, i + H);
-        // field f at the node displaced by o along d: 0 = rho*U, 1 = U, 2 = p
-        auto A = [&](int f, int c, int d, int o) -> double {
-            if (d == 0 || o == 0) {
-                const int e = H + (d == 0 ? o : 0);
-                return f == 0 ? qr[e] * qu[c][e] : (f == 1 ? qu[c][e] : qp[e]);
-            }
-            int fl[3];
-            const int64_t off = d == 1 ? wc_node<WALLS>(g, i, j + o, k, fl) : wc_node<WALLS>(g, i, j, k + o, fl);
-            if (f == 2) return a.p[off];
-            const double u = wc_vec<WALLS>(g, a.U_in, c, off, fl);
-            return f == 0 ? a.rho[off] * u : u;
-        };
-        const int64_t qi = (int64_t)i * plane + (int64_t)j * g.s[2] + k;
-        const double r = qr[H];
+        wc_stage_plane<H, 5>(g, slots, i, sh, [&](int64_t off, const int (&fl)[3], int c) -> double {
+            if (c == 0) return a.rho[off];
+            if (c == 4) return a.p[off];
+            return wc_vec<WALLS>(g, a.U_in, c - 1, off, fl);
+        });
+        __syncthreads();
+        if (tl.valid) {
+            // field f at the node displaced by o along d: 0 = rho*U, 1 = U, 2 = p
+            auto A = [&](int f, int c, int d, int o) -> double {
+                if (d == 0 || o == 0) {
+                    const int e = H + (d == 0 ? o : 0);
+                    return f == 0 ? qr[e] * qu[c][e] : (f == 1 ? qu[c][e] : qp[e]);
+                }
+                const int r = tj + H + (d == 1 ? o : 0), cc = tk + H + (d == 2 ? o : 0);
+                if (f == 2) return S(4, r, cc);
+                const double u = S(1 + c, r, cc);
+                return f == 0 ? S(0, r, cc) * u : u;
+            };
+            const int64_t qi = (int64_t)i * plane + (int64_t)tl.j * g.s[2] + tl.k;
+            const double r = qr[H];
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            double gp = 0.0;
+            for (int c = 0; c < 3; c++) {
+                double gp = 0.0;
 #pragma unroll
-            for (int l = 0; l < H; l++) gp += g.a[c][l] * (A(2, 0, c, l + 1) - A(2, 0, c, -l - 1));
-            if (c == 2) gp *= zi;
-            const double du = a.dt * (((-div_xy<H>(g, A, 0, 1, c, zi)) - gp) + lap_c<H>(g, A, 1, c, a.mu, zl, zm));
-            const double u0 = a.U0 ? a.U0[c * g.np + qi] : qu[c][H];
-            const double ruold = r * u0;
-            const double ruc = (a.ruc_in ? a.ruc_in[c * g.np + qi] : ruold) + a.w * du;
-            if (a.last) {
-                a.U_out[c * g.np + qi] = ruc / r;
-            } else {
-                a.U_out[c * g.np + qi] = ruold / r + a.cs * (du / r);
-                a.ruc_out[c * g.np + qi] = ruc;
+                for (int l = 0; l < H; l++) gp += g.a[c][l] * (A(2, 0, c, l + 1) - A(2, 0, c, -l - 1));
+                if (c == 2) gp *= zi;
+                const double du =
+                    a.dt * (((-div_xy<H>(g, A, 0, 1, c, zi)) - gp) + lap_c<H>(g, A, 1, c, a.mu, zl, zm));
+                const double u0 = a.U0 ? a.U0[c * g.np + qi] : qu[c][H];
+                const double ruold = r * u0;
+                const double ruc = (a.ruc_in ? a.ruc_in[c * g.np + qi] : ruold) + a.w * du;
+                if (a.last) {
+                    a.U_out[c * g.np + qi] = ruc / r;
+                } else {
+                    a.U_out[c * g.np + qi] = ruold / r + a.cs * (du / r);
+                    a.ruc_out[c * g.np + qi] = ruc;
+                }
             }
         }
+        __syncthreads();
 #pragma unroll
         for (int o = 0; o < 2 * H; o++) {
             qr[o] = qr[o + 1];
