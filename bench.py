@@ -342,6 +342,9 @@ def main() -> int:
     ap.add_argument("--wc-steps", type=int, default=10)
     ap.add_argument("--ot-steps", type=int, default=10)
     ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--tune", nargs=3, type=int, metavar=("VARIANT", "CHUNK", "PROJ"),
+                    help="fixed stage launch shape, i-chunk and projection variant instead of the autotuner "
+                         "(reproducible profiling runs)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain device copy of 2^30 bytes 5 times (counter calibration)")
     args = ap.parse_args()
@@ -369,7 +372,11 @@ def main() -> int:
     S = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=NU, graph=not args.eager)
     U = tg_field(nv, G, n)
     p = G.field(1)
-    tuned = chosen(S.autotune(U, reps=3))
+    if args.tune:
+        G.tune(*args.tune)
+        tuned = [f"fixed: ns_stage variant {args.tune[0]} chunk {args.tune[1]}, projection variant {args.tune[2]}"]
+    else:
+        tuned = chosen(S.autotune(U, reps=3))
     log(f"bench: autotuner: {tuned}")
     log(f"bench: TG {n}^3, warmup {W}, steps {K}, graph={not args.eager}")
     S.step(DT, W, U, p)
