@@ -175,13 +175,31 @@ def wc_bench(nv, n: int, steps: int) -> dict:
     # operands, outputs)
     alg = 680.0 * n ** 3
     finite = bool(torch.isfinite(U).all() and torch.isfinite(rho).all())
+    # app/OT: the same step with the frozen field's Lorentz force (one more
+    # pass per step, 24 B more per node in each U stage)
+    B = (0.8 * torch.stack([-2 * torch.sin(2 * x)[None, :, None] + torch.sin(x)[None, None, :] + 0 * x[:, None, None],
+                            2 * torch.sin(x)[:, None, None] + torch.sin(x)[None, None, :] + 0 * x[None, :, None],
+                            torch.sin(x)[:, None, None] + torch.sin(x)[None, :, None] + 0 * x[None, None, :]])
+         ).contiguous()
+    W.set_magnetic(B, 0.5)
+    W.step(dt, 2, U, rho, p)
+    torch.cuda.synchronize()
+    e0.record()
+    W.step(dt, steps, U, rho, p)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_ot = e0.elapsed_time(e1) / steps
+    alg_ot = (680.0 + 56.0 + 4 * 24.0) * n ** 3  # + the force pass (B 24 in, F 24 out) and F read per U stage
     return {"workload": f"weakly-compressible TG {n}^3 periodic FP64, mu=1/1600, p=50 rho^2, classic RK4 of rho "
                         f"and of rho U (app/TG), dt={dt}, {steps} steps",
             "value": round(n ** 3 / (ms / 1e3) / 1e6, 1), "unit": "Mpoint-steps/s", "ms_per_step": round(ms, 4),
             "alg_bytes_per_step": alg, "achieved_GBps": round(alg / (ms / 1e3) / 1e9, 1),
             "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
             "kernels_us": {k: round(v[1] / v[0] * 1e3, 1) for k, v in st.items()},
-            "ke": nv.ke(G, U), "finite": finite}
+            "ke": nv.ke(G, U), "finite": finite,
+            "ot_frozen_B": {"ms_per_step": round(ms_ot, 4), "value": round(n ** 3 / (ms_ot / 1e3) / 1e6, 1),
+                            "alg_bytes_per_step": alg_ot,
+                            "frac": round(alg_ot / (ms_ot / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
 def ensemble_bench(nv, steps: int) -> dict:

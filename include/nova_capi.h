@@ -248,6 +248,12 @@ int nova_wc_destroy(nova_wc *w);
  * laplacian(mu, U)), U = rhoUOld/rho + c drhoU/rho, UEqn.H:16-33).  U
  * (3 components) and rho in/out, p out; device pointers, stored layout. */
 int nova_wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p);
+/* The magnetic field of app/OT (OT.cpp:92-95 runs rhoEqn.H, pEqn.H,
+ * UEqn.H, BEqn.H; BEqn.H never assigns B, so B stays as set): each U stage
+ * adds the Lorentz terms -div((2 dbu) B, B) + grad(dbu (B & B)) of
+ * app/OT/UEqn.H:22-23, formed once per step.  B (3 components, device,
+ * stored layout) is read, not copied; NULL removes it. */
+int nova_wc_set_magnetic(nova_wc *w, const double *B, double dbu);
 
 /* ---- ensembles of uneven instances (no reference counterpart: NOVA runs
  * one case per process; the step of each instance is that of

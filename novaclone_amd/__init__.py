@@ -105,6 +105,7 @@ _SIGS = {
     "nova_wc_create": [_P, _D, _D, C.POINTER(_P)],
     "nova_wc_destroy": [_P],
     "nova_wc_step": [_P, _D, _I, _P, _P, _P],
+    "nova_wc_set_magnetic": [_P, _P, _D],
     "nova_ensemble_create": [C.POINTER(InstanceDesc), _I, _I, C.POINTER(_P)],
     "nova_ensemble_destroy": [_P],
     "nova_ensemble_set_stream": [_P, _P],
@@ -413,6 +414,12 @@ class WC:
         h = C.c_void_p()
         _check(lib().nova_wc_create(mesh.handle, float(mu), float(pc), C.byref(h)), "nova_wc_create")
         self._h = h
+
+    def set_magnetic(self, B, dbu: float) -> None:
+        """The frozen magnetic field of app/OT and its coefficient DBU; the
+        tensor must stay alive while it is set (None removes it)."""
+        self._B = B
+        _check(lib().nova_wc_set_magnetic(self._h, _ptr(B), float(dbu)), "nova_wc_set_magnetic")
 
     def step(self, dt: float, nsteps: int, U, rho, p) -> None:
         """Advance nsteps: U (3 components) and rho in/out, p out."""

@@ -118,6 +118,7 @@ int ensemble_info(nova_ensemble *e, int64_t out[6]);
 int wc_create(nova_ctx *ctx, double mu, double pc, nova_wc **out);
 int wc_destroy(nova_wc *w);
 int wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p);
+int wc_set_magnetic(nova_wc *w, const double *B, double dbu);
 
 // Mesh::z (src/Mesh/Mesh.cpp:373-407) for padded index kp on one rank.
 static double mesh_z(const nova_grid_desc &d, double dz, double hb, int m, int kp)
@@ -616,6 +617,12 @@ int nova_wc_create(nova_ctx *ctx, double mu, double pc, nova_wc **out)
 }
 
 int nova_wc_destroy(nova_wc *w) { NOVA_GUARD(wc_destroy(w)); }
+
+int nova_wc_set_magnetic(nova_wc *w, const double *B, double dbu)
+{
+    NOVA_REQUIRE(w, "null solver");
+    NOVA_GUARD(wc_set_magnetic(w, B, dbu));
+}
 
 int nova_wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p)
 {

@@ -24,6 +24,10 @@ struct nova_wc {
     double mu = 0.0, pc = 50.0;
     double *ra = nullptr, *rb = nullptr, *rc = nullptr;  // one component each
     double *ua = nullptr, *ub = nullptr, *uc = nullptr;  // three components each
+    // frozen magnetic field of app/OT (caller's array) and its Lorentz force
+    const double *B = nullptr;
+    double dbu = 0.0;
+    double *lf = nullptr;  // three components
 };
 
 namespace nova {
@@ -48,6 +52,7 @@ struct WcUArgs {
     const double *rho;     // frozen rho (stencil)
     const double *p;       // frozen p (stencil)
     const double *ruc_in;  // rho U accumulator (null: rho U0)
+    const double *F;       // Lorentz force -div(2 dbu B, B) + grad(dbu B.B) (may be null)
     double *U_out, *ruc_out;
     double dt, w, cs, mu;
     int last;
@@ -262,8 +267,9 @@ __global__ void __launch_bounds__(kWcTK *kWcTJ) k_wc_u(Grid g, WcUArgs a)
 #pragma unroll
                 for (int l = 0; l < H; l++) gp += g.a[c][l] * (A(2, 0, c, l + 1) - A(2, 0, c, -l - 1));
                 if (c == 2) gp *= zi;
-                const double du =
-                    a.dt * (((-div_xy<H>(g, A, 0, 1, c, zi)) - gp) + lap_c<H>(g, A, 1, c, a.mu, zl, zm));
+                double t = ((-div_xy<H>(g, A, 0, 1, c, zi)) - gp) + lap_c<H>(g, A, 1, c, a.mu, zl, zm);
+                if (a.F) t += a.F[c * g.np + qi];
+                const double du = a.dt * t;
                 const double u0 = a.U0 ? a.U0[c * g.np + qi] : qu[c][H];
                 const double ruold = r * u0;
                 const double ruc = (a.ruc_in ? a.ruc_in[c * g.np + qi] : ruold) + a.w * du;
@@ -286,6 +292,76 @@ __global__ void __launch_bounds__(kWcTK *kWcTJ) k_wc_u(Grid g, WcUArgs a)
     }
 }
 
+struct WcLArgs {
+    const double *B;
+    double *F;
+    double dbu;
+    int ichunk, nkt, njt, nic;
+};
+
+// app/OT/UEqn.H:22-23 for a frozen B: F = -div((2 dbu) B, B) + grad(dbu (B & B))
+// (div.hxx:84-172, grad.cpp:25-62), with B staged in LDS and queued along i.
+template <int H, bool WALLS>
+__global__ void __launch_bounds__(kWcTK *kWcTJ) k_wc_lorentz(Grid g, WcLArgs a)
+{
+    constexpr int LK = kWcTK + 2 * H, LJ = kWcTJ + 2 * H;
+    constexpr int QW = 2 * H + 1;
+    __shared__ double sh[3 * LJ * LK];
+    const WcTile tl = wc_tile(g, a.ichunk, a.nkt, a.njt, a.nic);
+    const int tk = threadIdx.x, tj = threadIdx.y;
+    WcSlots<H> slots;
+    wc_slots<H>(g, tl, slots);
+    int f0;
+    const int km = bmap(tl.k, g.n[2], g.wall[2], f0);
+    double qb[3][QW];
+    auto load_plane = [&](int o, int ii) {
+        int fl[3];
+        const int64_t off = wc_node<WALLS>(g, ii, tl.j, tl.k, fl);
+#pragma unroll
+        for (int c = 0; c < 3; c++) qb[c][o] = wc_vec<WALLS>(g, a.B, c, off, fl);
+    };
+#pragma unroll
+    for (int o = 0; o < 2 * H; o++) load_plane(o, tl.i0 - H + o);
+    const double zi = g.zinv[km];
+    const int64_t plane = (int64_t)g.s[1] * g.s[2];
+    auto S = [&](int c, int r, int cc) -> double { return sh[(c * LJ + r) * LK + cc]; };
+    for (int i = tl.i0; i < tl.i1; i++) {
+        load_plane(2 * H, i + H);
+        wc_stage_plane<H, 3>(g, slots, i, sh,
+                             [&](int64_t off, const int (&fl)[3], int c) { return wc_vec<WALLS>(g, a.B, c, off, fl); });
+        __syncthreads();
+        if (tl.valid) {
+            auto Bv = [&](int c, int d, int o) -> double {
+                if (d == 0 || o == 0) return qb[c][H + (d == 0 ? o : 0)];
+                return S(c, tj + H + (d == 1 ? o : 0), tk + H + (d == 2 ? o : 0));
+            };
+            // field 0 = (2 dbu) B, 1 = B
+            auto A = [&](int f, int c, int d, int o) -> double {
+                const double b = Bv(c, d, o);
+                return f == 0 ? b * (2 * a.dbu) : b;
+            };
+            auto pm = [&](int d, int o) -> double {
+                const double x = Bv(0, d, o), y = Bv(1, d, o), z = Bv(2, d, o);
+                return (x * x + y * y + z * z) * a.dbu;
+            };
+            const int64_t qi = (int64_t)i * plane + (int64_t)tl.j * g.s[2] + tl.k;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                double gs = 0.0;
+#pragma unroll
+                for (int l = 0; l < H; l++) gs += g.a[c][l] * (pm(c, l + 1) - pm(c, -l - 1));
+                if (c == 2) gs *= zi;
+                a.F[c * g.np + qi] = (-div_xy<H>(g, A, 0, 1, c, zi)) + gs;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int o = 0; o < 2 * H; o++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) qb[c][o] = qb[c][o + 1];
+    }
+}
+
 template <class Args>
 void wc_shape(const Grid &g, Args &a)
 {
@@ -299,10 +375,19 @@ void wc_shape(const Grid &g, Args &a)
 }
 
 template <int H>
-int launch_wc_h(nova_ctx *ctx, const WcRhoArgs *ra, const WcUArgs *ua)
+int launch_wc_h(nova_ctx *ctx, const WcRhoArgs *ra, const WcUArgs *ua, const WcLArgs *la)
 {
     const Grid &g = ctx->g;
     const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
+    if (la) {
+        TimedScope ts(ctx, "wc_lorentz");
+        WcLArgs a = *la;
+        wc_shape(g, a);
+        const dim3 grid(a.nkt * a.njt * a.nic), block(kWcTK, kWcTJ, 1);
+        if (walls) hipLaunchKernelGGL((k_wc_lorentz<H, true>), grid, block, 0, ctx->stream, g, a);
+        else hipLaunchKernelGGL((k_wc_lorentz<H, false>), grid, block, 0, ctx->stream, g, a);
+        return check_launch("wc_lorentz");
+    }
     if (ra) {
         TimedScope ts(ctx, "wc_rho_stage");
         WcRhoArgs a = *ra;
@@ -321,13 +406,13 @@ int launch_wc_h(nova_ctx *ctx, const WcRhoArgs *ra, const WcUArgs *ua)
     return check_launch("wc_u_stage");
 }
 
-int launch_wc(nova_ctx *ctx, const WcRhoArgs *ra, const WcUArgs *ua)
+int launch_wc(nova_ctx *ctx, const WcRhoArgs *ra, const WcUArgs *ua, const WcLArgs *la = nullptr)
 {
     switch (ctx->g.h) {
-    case 1: return launch_wc_h<1>(ctx, ra, ua);
-    case 2: return launch_wc_h<2>(ctx, ra, ua);
-    case 3: return launch_wc_h<3>(ctx, ra, ua);
-    case 4: return launch_wc_h<4>(ctx, ra, ua);
+    case 1: return launch_wc_h<1>(ctx, ra, ua, la);
+    case 2: return launch_wc_h<2>(ctx, ra, ua, la);
+    case 3: return launch_wc_h<3>(ctx, ra, ua, la);
+    case 4: return launch_wc_h<4>(ctx, ra, ua, la);
     default: return fail(NOVA_ERR_ARG, "stencil order");
     }
 }
@@ -368,14 +453,25 @@ int wc_create(nova_ctx *ctx, double mu, double pc, nova_wc **out)
 int wc_destroy(nova_wc *w)
 {
     if (!w) return NOVA_OK;
-    for (double *b : {w->ra, w->rb, w->rc, w->ua, w->ub, w->uc})
+    for (double *b : {w->ra, w->rb, w->rc, w->ua, w->ub, w->uc, w->lf})
         if (b) (void)hipFree(b);
     delete w;
     return NOVA_OK;
 }
 
-// One step per iteration: four rho stages (U frozen), then four U stages
-// (rho and p frozen).
+// The frozen magnetic field of app/OT (B is never reassigned there:
+// BEqn.H:26-31 only forms Bstar): its Lorentz force enters every U stage.
+// B stays the caller's array; NULL removes it.
+int wc_set_magnetic(nova_wc *w, const double *B, double dbu)
+{
+    if (B && !w->lf) NOVA_HIP(hipMalloc(&w->lf, sizeof(double) * 3 * w->ctx->g.np));
+    w->B = B;
+    w->dbu = dbu;
+    return NOVA_OK;
+}
+
+// One step per iteration: four rho stages (U frozen), then the Lorentz
+// force of a frozen B (when set), then four U stages (rho and p frozen).
 int wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p)
 {
     nova_ctx *ctx = w->ctx;
@@ -398,6 +494,13 @@ int wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p
             a.last = s == 3;
             if (int rc = launch_wc(ctx, &a, nullptr)) return rc;
         }
+        if (w->B) {
+            WcLArgs l{};
+            l.B = w->B;
+            l.F = w->lf;
+            l.dbu = w->dbu;
+            if (int rc = launch_wc(ctx, nullptr, nullptr, &l)) return rc;
+        }
         const double *uin[4] = {U, w->ua, w->ub, w->ua};
         double *uout[4] = {w->ua, w->ub, w->ua, U};
         for (int s = 0; s < 4; s++) {
@@ -407,6 +510,7 @@ int wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p
             a.rho = rho;
             a.p = p;
             a.ruc_in = s == 0 ? nullptr : w->uc;
+            a.F = w->B ? w->lf : nullptr;
             a.U_out = uout[s];
             a.ruc_out = w->uc;
             a.dt = dt;

@@ -868,20 +868,38 @@ void orc_mhd_step(const orc_mesh *M, orc_poisson *P, double *U, double *B, doubl
     for (int r = 0; r < 4; r++) free(dB[r]);
 }
 
-/* Weakly-compressible step, app/TG/TG.cpp:71-76: rhoEqn.H:10-35 (classic
- * RK4 of rho with U frozen), pEqn.H:2 (p = pc rho^2; pc = 50 for c = 10),
+/* Weakly-compressible step, app/TG/TG.cpp:71-76 (app/OT/OT.cpp:92-95 with
+ * B non-null): rhoEqn.H:10-35 (classic RK4 of rho with U frozen), pEqn.H:2
+ * (p = pc rho^2; pc = 50 for c = 10),
  * UEqn.H:12-35 (classic RK4 of rho U with rho and p frozen).  Field algebra
  * acts on every node of the padded array (Field.hxx:617-1150), the
  * operators on the interior with zero ghosts, and rho and U are refilled
  * after every stage (rhoEqn.H:33, UEqn.H:33).  Stage s starts from the
  * step-start value plus (int(s/2)+1)/2 of the previous increment; the
  * accumulator adds RK4W[s] of each increment. */
-void orc_wc_step(const orc_mesh *M, double *U, double *rho, double *p, double mu, double pc, double dt)
+void orc_wc_step(const orc_mesh *M, double *U, double *rho, double *p, const double *B, double mu, double pc,
+                 double dbu, double dt)
 {
     const size_t n1 = NPTS(M), n3 = n1 * 3;
     double *rU = falloc(M, 3), *dv = falloc(M, 1), *dr = falloc(M, 1), *rc = falloc(M, 1), *r0 = falloc(M, 1);
     double *t1 = falloc(M, 3), *t2 = falloc(M, 3), *gp = falloc(M, 3), *dU = falloc(M, 3);
     double *rUc = falloc(M, 3), *rU0 = falloc(M, 3);
+    /* app/OT/UEqn.H:21-23: the Lorentz terms of a frozen B (BEqn.H never
+     * assigns B): -div((2 DBU) B, B) + grad(DBU (B & B)) */
+    double *t3 = NULL, *t4 = NULL;
+    if (B) {
+        double *X = falloc(M, 3), *s1 = falloc(M, 1);
+        t3 = falloc(M, 3);
+        t4 = falloc(M, 3);
+        #pragma omp parallel for
+        for (size_t q = 0; q < n1; q++) {
+            for (int c = 0; c < 3; c++) X[3 * q + c] = B[3 * q + c] * (2 * dbu);
+            s1[q] = (B[3 * q] * B[3 * q] + B[3 * q + 1] * B[3 * q + 1] + B[3 * q + 2] * B[3 * q + 2]) * dbu;
+        }
+        orc_div_adv(M, X, B, 3, t3);
+        orc_grad(M, s1, t4);
+        free(X); free(s1);
+    }
     memcpy(r0, rho, sizeof(double) * n1);
     memcpy(rc, rho, sizeof(double) * n1);
     for (int rk = 0; rk < 4; rk++) {
@@ -917,7 +935,7 @@ void orc_wc_step(const orc_mesh *M, double *U, double *rho, double *p, double mu
         orc_laplacian(M, mu, U, 3, t2);
         #pragma omp parallel for
         for (size_t q = 0; q < n3; q++) {
-            dU[q] = dt * (((-t1[q]) - gp[q]) + t2[q]);
+            dU[q] = B ? dt * (((((-t1[q]) - gp[q]) + t2[q]) - t3[q]) + t4[q]) : dt * (((-t1[q]) - gp[q]) + t2[q]);
             rUc[q] = rUc[q] + RK4W[rk] * dU[q];
         }
         const double cs = (rk / 2 + 1) / 2.0;
@@ -930,7 +948,7 @@ void orc_wc_step(const orc_mesh *M, double *U, double *rho, double *p, double mu
     }
     (void)n3;
     free(rU); free(dv); free(dr); free(rc); free(r0); free(t1); free(t2); free(gp); free(dU);
-    free(rUc); free(rU0);
+    free(rUc); free(rU0); free(t3); free(t4);
 }
 
 /* ------------------------------------------------------------------ */

@@ -93,8 +93,10 @@ void orc_ns_step(const orc_mesh *, orc_poisson *, double *U, double *Ustar,
 void orc_mhd_step(const orc_mesh *, orc_poisson *, double *U, double *B,
                   double *p, double *pB, double nu, double eta, double dt, int mode);
 /* Weakly-compressible step (app/TG/TG.cpp with rhoEqn.H, pEqn.H, UEqn.H):
- * rho and U in/out, p = pc rho^2 out.  Periodic grids. */
-void orc_wc_step(const orc_mesh *, double *U, double *rho, double *p, double mu, double pc, double dt);
+ * rho and U in/out, p = pc rho^2 out.  B (may be NULL) adds the Lorentz
+ * terms of app/OT/UEqn.H with the frozen B of app/OT (coefficient dbu). */
+void orc_wc_step(const orc_mesh *, double *U, double *rho, double *p, const double *B, double mu, double pc,
+                 double dbu, double dt);
 /* Projection: F <- F - s*grad(lap^-1(div(F)/s)), p receives the potential. */
 void orc_project(const orc_mesh *, orc_poisson *, double *F, double *p, double s);
 

@@ -57,7 +57,7 @@ def _load() -> C.CDLL:
         "orc_poisson_solve": ([_P, _D, _D], None),
         "orc_ns_step": ([_P, _P, _D, _D, _D, C.c_double, C.c_double, C.c_int], None),
         "orc_mhd_step": ([_P, _P, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_int], None),
-        "orc_wc_step": ([_P, _D, _D, _D, C.c_double, C.c_double, C.c_double], None),
+        "orc_wc_step": ([_P, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_double], None),
         "orc_project": ([_P, _P, _D, _D, C.c_double], None),
         "orc_ke": ([_P, _D], C.c_double),
         "orc_maxabs_div": ([_P, _D], C.c_double),
@@ -207,9 +207,10 @@ class OMesh:
         return out
 
     # compact 6th-order scheme
-    def wc_step(self, U, rho, p, mu, pc, dt):
-        """One weakly-compressible step in place (U: nc=3, rho and p: nc=1)."""
-        lib().orc_wc_step(self._h, _d(U), _d(rho), _d(p), mu, pc, dt)
+    def wc_step(self, U, rho, p, mu, pc, dt, B=None, dbu=0.0):
+        """One weakly-compressible step in place (U: nc=3, rho and p: nc=1);
+        B (nc=3, frozen) adds the Lorentz terms of app/OT/UEqn.H."""
+        lib().orc_wc_step(self._h, _d(U), _d(rho), _d(p), _d(B) if B is not None else None, mu, pc, dbu, dt)
 
     def compact_deriv(self, f, direction, order=1):
         out = self.field(f.shape[-1])

@@ -1,8 +1,8 @@
 """Parity of the weakly-compressible step (app/TG/TG.cpp:71-76 with
-rhoEqn.H, pEqn.H, UEqn.H) through the C ABI with the oracle: one step
-per-field relative L2 <= 1e-12, kinetic-energy history over 20 steps
-<= 1e-10 relative, mass conserved to rounding; periodic and y-walled
-grids, stencil orders m = 4, 6, 8."""
+rhoEqn.H, pEqn.H, UEqn.H; app/OT with its frozen magnetic field) through
+the C ABI with the oracle: one step per-field relative L2 <= 1e-12,
+kinetic-energy history over 20 steps <= 1e-10 relative, mass conserved to
+rounding; periodic and y-walled grids, stencil orders m = 4, 6, 8."""
 import math
 
 import numpy as np
@@ -107,6 +107,45 @@ def test_wc_history_and_mass(nv, orc):
     assert err < 1e-10
     assert abs(host(rd).sum() - m0) < 1e-12 * m0
     assert rel_l2(host(Ud), O.interior(Uo)) < 1e-10
+
+
+@pytest.mark.parametrize("wall", [False, True])
+def test_wc_ot_with_frozen_field_matches_oracle(nv, orc, wall):
+    # app/OT (OT.cpp:50-76): U = (-2 sin y, 2 sin x, 0), B = 0.8 (-2 sin 2y +
+    # sin z, 2 sin x + sin z, sin x + sin y), rho = 1, mu = DB = 0.1256,
+    # DBU = 1/2; BEqn.H leaves B as it is, UEqn.H adds its Lorentz force
+    N, L = (24, 16, 32), (TWO_PI, math.pi if wall else TWO_PI, TWO_PI)
+    bc = ("periodic", "wall" if wall else "periodic", "periodic")
+    G = nv.Mesh(N, L, bc=bc)
+    O = orc.OMesh(N, L, wall_y=wall)
+    x = [np.arange(O.stored[a]) * (O.L[a] / O.N[a]) for a in range(3)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    U = np.stack([-2 * np.sin(Y), 2 * np.sin(X), 0 * X])
+    B = 0.8 * np.stack([-2 * np.sin(2 * Y) + np.sin(Z), 2 * np.sin(X) + np.sin(Z), np.sin(X) + np.sin(Y)])
+    if wall:
+        U[1, :, 0, :] = U[1, :, -1, :] = 0.0
+        B[1, :, 0, :] = B[1, :, -1, :] = 0.0
+    rho = np.ones((1, *O.stored))
+    mu, pc, dbu, dt = 0.1256, 50.0, 0.5, 1e-3
+    W = nv.WC(G, mu, pc)
+    Bd = dev(B)
+    W.set_magnetic(Bd, dbu)
+    Ud, rd, pd = dev(U), dev(rho), dev(np.zeros_like(rho))
+    W.step(dt, 3, Ud, rd, pd)
+    Uo, ro, po, Bo = O.from_compact(U), O.from_compact(rho), O.field(1), O.from_compact(B)
+    for _ in range(3):
+        O.wc_step(Uo, ro, po, mu, pc, dt, B=Bo, dbu=dbu)
+    eu = rel_l2(host(Ud), O.interior(Uo))
+    er = rel_l2(host(rd), O.interior(ro))
+    ep = rel_l2(host(pd), O.interior(po))
+    print("wall", wall, "rel L2 U", eu, "rho", er, "p", ep, flush=True)
+    assert eu < RTOL and er < RTOL and ep < RTOL
+    assert np.array_equal(host(Bd), B)
+    # the Lorentz force moved the flow: without B the step differs
+    W.set_magnetic(None, 0.0)
+    U2, r2, p2 = dev(U), dev(rho), dev(np.zeros_like(rho))
+    W.step(dt, 3, U2, r2, p2)
+    assert rel_l2(host(U2), host(Ud)) > 1e-6
 
 
 def test_wc_rejects_bad_arguments(nv):

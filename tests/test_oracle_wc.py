@@ -141,3 +141,28 @@ def test_flow_from_rest_is_driven_by_minus_grad_p_over_rho(orc):
     assert errs[0] < 1e-6
     # the remainder is second order in dt (the advection of the O(dt) flow)
     assert 2.5 < errs[0] / errs[1] < 6.0 or errs[1] < 1e-13
+
+
+def test_lorentz_force_of_a_frozen_field(orc):
+    # app/OT/UEqn.H:22-23 with B = (b(x), 0, 0) frozen, rho = 1, p uniform,
+    # from rest: -div(2 D B, B) + grad(D B.B) = -2 D b D1(b) on the x
+    # component (D1 the sixth-order first derivative), so for
+    # b = eps sin(kx) the flow after a short step is -dt D eps^2 k' sin(2kx)
+    N = (32, 8, 8)
+    M = orc.OMesh(N, (TWO_PI,) * 3)
+    X, Y, Z = grid(M)
+    k, eps, dbu = 3, 0.2, 0.5
+    B = M.field(3)
+    B[..., 0] = eps * np.sin(k * X)
+    h = TWO_PI / N[0]
+    errs = []
+    for dt in (2e-4, 1e-4):
+        U, rho, p = M.field(3), M.field(1), M.field(1)
+        rho[...] = 1.0
+        M.wc_step(U, rho, p, 0.0, 50.0, dt, B=B, dbu=dbu)
+        want = -dt * dbu * eps * eps * k1(k, h) * np.sin(2 * k * X)
+        s = inner(M)
+        assert np.abs(U[s][..., 1:]).max() == 0.0
+        errs.append(np.abs(U[s][..., 0] - want[s]).max() / np.abs(want[s]).max())
+    assert errs[0] < 1e-5
+    assert 2.5 < errs[0] / errs[1] < 6.0 or errs[1] < 1e-13
