@@ -190,6 +190,21 @@ def wc_bench(nv, n: int, steps: int) -> dict:
     torch.cuda.synchronize()
     ms_ot = e0.elapsed_time(e1) / steps
     alg_ot = (680.0 + 56.0 + 4 * 24.0) * n ** 3  # + the force pass (B 24 in, F 24 out) and F read per U stage
+    # app/shearLayerWC order with the temperature equation and buoyancy
+    W.set_magnetic(None, 0.0)
+    T = (0.05 * torch.tanh(2 * (x - math.pi))[None, :, None] + 0 * x[:, None, None] + 0 * x[None, None, :]
+         ).unsqueeze(0).contiguous()
+    W.set_temperature(1.0 / 1600.0 / 7.0, (0.0, -0.1, 0.0))
+    W.step_t(dt, 2, U, rho, p, T)
+    torch.cuda.synchronize()
+    e0.record()
+    W.step_t(dt, steps, U, rho, p, T)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_t = e0.elapsed_time(e1) / steps
+    # + four T stages: rho 8 + U 24 + T 8 stencil, T0 8, accumulator 8 in / 8 out, T 8 out; T read by
+    # the U stages
+    alg_t = (680.0 + 64 + 80 + 80 + 64 + 4 * 8.0) * n ** 3
     return {"workload": f"weakly-compressible TG {n}^3 periodic FP64, mu=1/1600, p=50 rho^2, classic RK4 of rho "
                         f"and of rho U (app/TG), dt={dt}, {steps} steps",
             "value": round(n ** 3 / (ms / 1e3) / 1e6, 1), "unit": "Mpoint-steps/s", "ms_per_step": round(ms, 4),
@@ -199,7 +214,10 @@ def wc_bench(nv, n: int, steps: int) -> dict:
             "ke": nv.ke(G, U), "finite": finite,
             "ot_frozen_B": {"ms_per_step": round(ms_ot, 4), "value": round(n ** 3 / (ms_ot / 1e3) / 1e6, 1),
                             "alg_bytes_per_step": alg_ot,
-                            "frac": round(alg_ot / (ms_ot / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}
+                            "frac": round(alg_ot / (ms_ot / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "shear_layer_T": {"ms_per_step": round(ms_t, 4), "value": round(n ** 3 / (ms_t / 1e3) / 1e6, 1),
+                              "alg_bytes_per_step": alg_t,
+                              "frac": round(alg_t / (ms_t / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
 def ensemble_bench(nv, steps: int) -> dict:

@@ -254,6 +254,14 @@ int nova_wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, doub
  * app/OT/UEqn.H:22-23, formed once per step.  B (3 components, device,
  * stored layout) is read, not copied; NULL removes it. */
 int nova_wc_set_magnetic(nova_wc *w, const double *B, double dbu);
+/* The temperature equation of app/shearLayerWC (TEqn.H: classic RK4 of
+ * rho T with rho and U frozen, drhoT = dt (-div(rho U, T) + laplacian(ka, T)))
+ * and the buoyancy -(rho T) g in UEqn.H; allocates three more buffers. */
+int nova_wc_set_temperature(nova_wc *w, double ka, double gx, double gy, double gz);
+/* Advance nsteps in the order of app/shearLayerWC (shearLayer.cpp:197-200):
+ * UEqn (with buoyancy; p as given, i.e. the previous step's), rhoEqn, pEqn
+ * (p = (pc rho) rho, pc = c^2/(2 rho0)), TEqn.  U, rho, T in/out; p in/out. */
+int nova_wc_step_t(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p, double *T);
 
 /* ---- ensembles of uneven instances (no reference counterpart: NOVA runs
  * one case per process; the step of each instance is that of

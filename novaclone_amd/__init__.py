@@ -106,6 +106,8 @@ _SIGS = {
     "nova_wc_destroy": [_P],
     "nova_wc_step": [_P, _D, _I, _P, _P, _P],
     "nova_wc_set_magnetic": [_P, _P, _D],
+    "nova_wc_set_temperature": [_P, _D, _D, _D, _D],
+    "nova_wc_step_t": [_P, _D, _I, _P, _P, _P, _P],
     "nova_ensemble_create": [C.POINTER(InstanceDesc), _I, _I, C.POINTER(_P)],
     "nova_ensemble_destroy": [_P],
     "nova_ensemble_set_stream": [_P, _P],
@@ -424,6 +426,15 @@ class WC:
     def step(self, dt: float, nsteps: int, U, rho, p) -> None:
         """Advance nsteps: U (3 components) and rho in/out, p out."""
         _check(lib().nova_wc_step(self._h, float(dt), int(nsteps), _ptr(U), _ptr(rho), _ptr(p)), "nova_wc_step")
+
+    def set_temperature(self, ka: float, g) -> None:
+        """Enable the temperature equation and buoyancy of app/shearLayerWC."""
+        _check(lib().nova_wc_set_temperature(self._h, float(ka), *map(float, g)), "nova_wc_set_temperature")
+
+    def step_t(self, dt: float, nsteps: int, U, rho, p, T) -> None:
+        """Advance nsteps in app/shearLayerWC order: U, rho, T in/out, p in/out."""
+        _check(lib().nova_wc_step_t(self._h, float(dt), int(nsteps), _ptr(U), _ptr(rho), _ptr(p), _ptr(T)),
+               "nova_wc_step_t")
 
     def close(self):
         if getattr(self, "_h", None):

@@ -119,6 +119,8 @@ int wc_create(nova_ctx *ctx, double mu, double pc, nova_wc **out);
 int wc_destroy(nova_wc *w);
 int wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p);
 int wc_set_magnetic(nova_wc *w, const double *B, double dbu);
+int wc_set_temperature(nova_wc *w, double ka, const double g[3]);
+int wc_step_t(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p, double *T);
 
 // Mesh::z (src/Mesh/Mesh.cpp:373-407) for padded index kp on one rank.
 static double mesh_z(const nova_grid_desc &d, double dz, double hb, int m, int kp)
@@ -622,6 +624,19 @@ int nova_wc_set_magnetic(nova_wc *w, const double *B, double dbu)
 {
     NOVA_REQUIRE(w, "null solver");
     NOVA_GUARD(wc_set_magnetic(w, B, dbu));
+}
+
+int nova_wc_set_temperature(nova_wc *w, double ka, double gx, double gy, double gz)
+{
+    NOVA_REQUIRE(w, "null solver");
+    const double g[3] = {gx, gy, gz};
+    NOVA_GUARD(wc_set_temperature(w, ka, g));
+}
+
+int nova_wc_step_t(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p, double *T)
+{
+    NOVA_REQUIRE(w && U && rho && p && T && nsteps >= 0, "bad argument");
+    NOVA_GUARD(wc_step_t(w, dt, nsteps, U, rho, p, T));
 }
 
 int nova_wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p)

@@ -28,6 +28,10 @@ struct nova_wc {
     const double *B = nullptr;
     double dbu = 0.0;
     double *lf = nullptr;  // three components
+    // temperature equation of app/shearLayerWC: diffusivity, gravity, stage
+    // buffers and the rho T accumulator (one component each)
+    double ka = 0.0, grav[3] = {0.0, 0.0, 0.0};
+    double *ta = nullptr, *tb = nullptr, *tc = nullptr;
 };
 
 namespace nova {
@@ -43,6 +47,7 @@ struct WcRhoArgs {
     double *rho_out, *rc_out, *p;
     double dt, w, cs, pc;
     int last;
+    int p_left;  // p = (pc rho) rho (app/shearLayerWC/pEqn.H:2) instead of pc (rho rho)
     int ichunk, nkt, njt, nic;
 };
 
@@ -53,6 +58,8 @@ struct WcUArgs {
     const double *p;       // frozen p (stencil)
     const double *ruc_in;  // rho U accumulator (null: rho U0)
     const double *F;       // Lorentz force -div(2 dbu B, B) + grad(dbu B.B) (may be null)
+    const double *T;       // temperature of the buoyancy term -(rho T) g (may be null)
+    double g[3];
     double *U_out, *ruc_out;
     double dt, w, cs, mu;
     int last;
@@ -195,7 +202,7 @@ __global__ void __launch_bounds__(kWcTK *kWcTJ) k_wc_rho(Grid g, WcRhoArgs a)
             const double rc = a.rc_in[qi] + a.w * dr;
             if (a.last) {
                 a.rho_out[qi] = rc;
-                if (a.p) a.p[qi] = a.pc * (rc * rc);
+                if (a.p) a.p[qi] = a.p_left ? (a.pc * rc) * rc : a.pc * (rc * rc);
             } else {
                 a.rho_out[qi] = a.rho0[qi] + a.cs * dr;
                 a.rc_out[qi] = rc;
@@ -269,14 +276,22 @@ __global__ void __launch_bounds__(kWcTK *kWcTJ) k_wc_u(Grid g, WcUArgs a)
                 if (c == 2) gp *= zi;
                 double t = ((-div_xy<H>(g, A, 0, 1, c, zi)) - gp) + lap_c<H>(g, A, 1, c, a.mu, zl, zm);
                 if (a.F) t += a.F[c * g.np + qi];
+                if (a.T) t = t - (r * a.T[qi]) * a.g[c];
                 const double du = a.dt * t;
                 const double u0 = a.U0 ? a.U0[c * g.np + qi] : qu[c][H];
                 const double ruold = r * u0;
                 const double ruc = (a.ruc_in ? a.ruc_in[c * g.np + qi] : ruold) + a.w * du;
+                // the wall-normal component is odd: zero on its walls (the
+                // refill after each stage, UEqn.H:33, with a reflecting wall)
+                bool on_wall = false;
+                if constexpr (WALLS) {
+                    const int idx = c == 0 ? i : (c == 1 ? tl.j : tl.k);
+                    on_wall = g.wall[c] && (idx == 0 || idx == g.n[c]);
+                }
                 if (a.last) {
-                    a.U_out[c * g.np + qi] = ruc / r;
+                    a.U_out[c * g.np + qi] = on_wall ? 0.0 : ruc / r;
                 } else {
-                    a.U_out[c * g.np + qi] = ruold / r + a.cs * (du / r);
+                    a.U_out[c * g.np + qi] = on_wall ? 0.0 : ruold / r + a.cs * (du / r);
                     a.ruc_out[c * g.np + qi] = ruc;
                 }
             }
@@ -286,6 +301,92 @@ __global__ void __launch_bounds__(kWcTK *kWcTJ) k_wc_u(Grid g, WcUArgs a)
         for (int o = 0; o < 2 * H; o++) {
             qr[o] = qr[o + 1];
             qp[o] = qp[o + 1];
+#pragma unroll
+            for (int c = 0; c < 3; c++) qu[c][o] = qu[c][o + 1];
+        }
+    }
+}
+
+struct WcTArgs {
+    const double *T_in;    // stage value (stencil)
+    const double *T0;      // step-start T (null: T_in)
+    const double *rho;     // frozen rho (stencil)
+    const double *U;       // frozen U (stencil)
+    const double *rtc_in;  // rho T accumulator (null: rho T0)
+    double *T_out, *rtc_out;
+    double dt, w, cs, ka;
+    int last;
+    int ichunk, nkt, njt, nic;
+};
+
+// TEqn.H (app/shearLayerWC:16-33): drhoT = dt * (-div(rho*U, T) +
+// laplacian(ka, T)), rtc += w drhoT, T = rhoTOld/rho + cs (drhoT/rho) (the
+// last stage: T = rtc/rho), with rhoTOld = rho*T0.  rho, U and T of the
+// plane are staged in LDS; the own column's i-neighbours walk register
+// queues.
+template <int H, bool WALLS>
+__global__ void __launch_bounds__(kWcTK *kWcTJ) k_wc_t(Grid g, WcTArgs a)
+{
+    constexpr int LK = kWcTK + 2 * H, LJ = kWcTJ + 2 * H;
+    constexpr int QW = 2 * H + 1;
+    __shared__ double sh[5 * LJ * LK];  // rho, U_x, U_y, U_z, T
+    const WcTile tl = wc_tile(g, a.ichunk, a.nkt, a.njt, a.nic);
+    const int tk = threadIdx.x, tj = threadIdx.y;
+    WcSlots<H> slots;
+    wc_slots<H>(g, tl, slots);
+    int f0;
+    const int km = bmap(tl.k, g.n[2], g.wall[2], f0);
+    double qr[QW], qu[3][QW], qt[QW];
+    auto load_plane = [&](int o, int ii) {
+        int fl[3];
+        const int64_t off = wc_node<WALLS>(g, ii, tl.j, tl.k, fl);
+        qr[o] = a.rho[off];
+        qt[o] = a.T_in[off];
+#pragma unroll
+        for (int c = 0; c < 3; c++) qu[c][o] = wc_vec<WALLS>(g, a.U, c, off, fl);
+    };
+#pragma unroll
+    for (int o = 0; o < 2 * H; o++) load_plane(o, tl.i0 - H + o);
+    const double zi = g.zinv[km], zl = g.zlap[km], zm = g.zmet[km];
+    const int64_t plane = (int64_t)g.s[1] * g.s[2];
+    auto S = [&](int c, int r, int cc) -> double { return sh[(c * LJ + r) * LK + cc]; };
+    for (int i = tl.i0; i < tl.i1; i++) {
+        load_plane(2 * H, i + H);
+        wc_stage_plane<H, 5>(g, slots, i, sh, [&](int64_t off, const int (&fl)[3], int c) -> double {
+            if (c == 0) return a.rho[off];
+            if (c == 4) return a.T_in[off];
+            return wc_vec<WALLS>(g, a.U, c - 1, off, fl);
+        });
+        __syncthreads();
+        if (tl.valid) {
+            // field f at the node displaced by o along d: 0 = rho*U (component
+            // c), 1 = T
+            auto A = [&](int f, int c, int d, int o) -> double {
+                if (d == 0 || o == 0) {
+                    const int e = H + (d == 0 ? o : 0);
+                    return f == 0 ? qr[e] * qu[c][e] : qt[e];
+                }
+                const int r = tj + H + (d == 1 ? o : 0), cc = tk + H + (d == 2 ? o : 0);
+                return f == 0 ? S(0, r, cc) * S(1 + c, r, cc) : S(4, r, cc);
+            };
+            const int64_t qi = (int64_t)i * plane + (int64_t)tl.j * g.s[2] + tl.k;
+            const double r = qr[H];
+            const double dT = a.dt * ((-div_xy<H>(g, A, 0, 1, 0, zi)) + lap_c<H>(g, A, 1, 0, a.ka, zl, zm));
+            const double t0 = a.T0 ? a.T0[qi] : qt[H];
+            const double rtold = r * t0;
+            const double rtc = (a.rtc_in ? a.rtc_in[qi] : rtold) + a.w * dT;
+            if (a.last) {
+                a.T_out[qi] = rtc / r;
+            } else {
+                a.T_out[qi] = rtold / r + a.cs * (dT / r);
+                a.rtc_out[qi] = rtc;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int o = 0; o < 2 * H; o++) {
+            qr[o] = qr[o + 1];
+            qt[o] = qt[o + 1];
 #pragma unroll
             for (int c = 0; c < 3; c++) qu[c][o] = qu[c][o + 1];
         }
@@ -375,10 +476,19 @@ void wc_shape(const Grid &g, Args &a)
 }
 
 template <int H>
-int launch_wc_h(nova_ctx *ctx, const WcRhoArgs *ra, const WcUArgs *ua, const WcLArgs *la)
+int launch_wc_h(nova_ctx *ctx, const WcRhoArgs *ra, const WcUArgs *ua, const WcLArgs *la, const WcTArgs *tt)
 {
     const Grid &g = ctx->g;
     const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
+    if (tt) {
+        TimedScope ts(ctx, "wc_t_stage");
+        WcTArgs a = *tt;
+        wc_shape(g, a);
+        const dim3 grid(a.nkt * a.njt * a.nic), block(kWcTK, kWcTJ, 1);
+        if (walls) hipLaunchKernelGGL((k_wc_t<H, true>), grid, block, 0, ctx->stream, g, a);
+        else hipLaunchKernelGGL((k_wc_t<H, false>), grid, block, 0, ctx->stream, g, a);
+        return check_launch("wc_t_stage");
+    }
     if (la) {
         TimedScope ts(ctx, "wc_lorentz");
         WcLArgs a = *la;
@@ -406,13 +516,14 @@ int launch_wc_h(nova_ctx *ctx, const WcRhoArgs *ra, const WcUArgs *ua, const WcL
     return check_launch("wc_u_stage");
 }
 
-int launch_wc(nova_ctx *ctx, const WcRhoArgs *ra, const WcUArgs *ua, const WcLArgs *la = nullptr)
+int launch_wc(nova_ctx *ctx, const WcRhoArgs *ra, const WcUArgs *ua, const WcLArgs *la = nullptr,
+              const WcTArgs *tt = nullptr)
 {
     switch (ctx->g.h) {
-    case 1: return launch_wc_h<1>(ctx, ra, ua, la);
-    case 2: return launch_wc_h<2>(ctx, ra, ua, la);
-    case 3: return launch_wc_h<3>(ctx, ra, ua, la);
-    case 4: return launch_wc_h<4>(ctx, ra, ua, la);
+    case 1: return launch_wc_h<1>(ctx, ra, ua, la, tt);
+    case 2: return launch_wc_h<2>(ctx, ra, ua, la, tt);
+    case 3: return launch_wc_h<3>(ctx, ra, ua, la, tt);
+    case 4: return launch_wc_h<4>(ctx, ra, ua, la, tt);
     default: return fail(NOVA_ERR_ARG, "stencil order");
     }
 }
@@ -453,7 +564,7 @@ int wc_create(nova_ctx *ctx, double mu, double pc, nova_wc **out)
 int wc_destroy(nova_wc *w)
 {
     if (!w) return NOVA_OK;
-    for (double *b : {w->ra, w->rb, w->rc, w->ua, w->ub, w->uc, w->lf})
+    for (double *b : {w->ra, w->rb, w->rc, w->ua, w->ub, w->uc, w->lf, w->ta, w->tb, w->tc})
         if (b) (void)hipFree(b);
     delete w;
     return NOVA_OK;
@@ -470,56 +581,125 @@ int wc_set_magnetic(nova_wc *w, const double *B, double dbu)
     return NOVA_OK;
 }
 
-// One step per iteration: four rho stages (U frozen), then the Lorentz
-// force of a frozen B (when set), then four U stages (rho and p frozen).
+namespace {
+
+// rhoEqn.H: four rho stages with U frozen; the last writes p (pEqn.H).
+int wc_rho_eqn(nova_wc *w, double dt, const double *U, double *rho, double *p, int p_left)
+{
+    const double *rin[4] = {rho, w->ra, w->rb, w->ra};
+    double *rout[4] = {w->ra, w->rb, w->ra, rho};
+    for (int s = 0; s < 4; s++) {
+        WcRhoArgs a{};
+        a.rho_in = rin[s];
+        a.U = U;
+        a.rho0 = rho;
+        a.rc_in = s == 0 ? rho : w->rc;
+        a.rho_out = rout[s];
+        a.rc_out = w->rc;
+        a.p = p;
+        a.dt = dt;
+        a.w = kRk4W[s];
+        a.cs = s < 3 ? kRk4C[s] : 0.0;
+        a.pc = w->pc;
+        a.last = s == 3;
+        a.p_left = p_left;
+        if (int rc = launch_wc(w->ctx, &a, nullptr)) return rc;
+    }
+    return NOVA_OK;
+}
+
+// UEqn.H: the Lorentz force of a frozen B (when set), then four U stages
+// with rho, p (and T) frozen.
+int wc_u_eqn(nova_wc *w, double dt, double *U, const double *rho, const double *p, const double *T)
+{
+    if (w->B) {
+        WcLArgs l{};
+        l.B = w->B;
+        l.F = w->lf;
+        l.dbu = w->dbu;
+        if (int rc = launch_wc(w->ctx, nullptr, nullptr, &l)) return rc;
+    }
+    const double *uin[4] = {U, w->ua, w->ub, w->ua};
+    double *uout[4] = {w->ua, w->ub, w->ua, U};
+    for (int s = 0; s < 4; s++) {
+        WcUArgs a{};
+        a.U_in = uin[s];
+        a.U0 = (s == 0 || s == 3) ? nullptr : U;
+        a.rho = rho;
+        a.p = p;
+        a.ruc_in = s == 0 ? nullptr : w->uc;
+        a.F = w->B ? w->lf : nullptr;
+        a.T = T;
+        for (int c = 0; c < 3; c++) a.g[c] = w->grav[c];
+        a.U_out = uout[s];
+        a.ruc_out = w->uc;
+        a.dt = dt;
+        a.w = kRk4W[s];
+        a.cs = s < 3 ? kRk4C[s] : 0.0;
+        a.mu = w->mu;
+        a.last = s == 3;
+        if (int rc = launch_wc(w->ctx, nullptr, &a)) return rc;
+    }
+    return NOVA_OK;
+}
+
+// TEqn.H: four T stages with rho and U frozen.
+int wc_t_eqn(nova_wc *w, double dt, const double *U, const double *rho, double *T)
+{
+    const double *tin[4] = {T, w->ta, w->tb, w->ta};
+    double *tout[4] = {w->ta, w->tb, w->ta, T};
+    for (int s = 0; s < 4; s++) {
+        WcTArgs a{};
+        a.T_in = tin[s];
+        a.T0 = (s == 0 || s == 3) ? nullptr : T;
+        a.rho = rho;
+        a.U = U;
+        a.rtc_in = s == 0 ? nullptr : w->tc;
+        a.T_out = tout[s];
+        a.rtc_out = w->tc;
+        a.dt = dt;
+        a.w = kRk4W[s];
+        a.cs = s < 3 ? kRk4C[s] : 0.0;
+        a.ka = w->ka;
+        a.last = s == 3;
+        if (int rc = launch_wc(w->ctx, nullptr, nullptr, nullptr, &a)) return rc;
+    }
+    return NOVA_OK;
+}
+
+}  // namespace
+
+// app/TG, app/OT order (TG.cpp:71-76): rhoEqn, pEqn, UEqn.
 int wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p)
 {
-    nova_ctx *ctx = w->ctx;
     for (int n = 0; n < nsteps; n++) {
-        const double *rin[4] = {rho, w->ra, w->rb, w->ra};
-        double *rout[4] = {w->ra, w->rb, w->ra, rho};
-        for (int s = 0; s < 4; s++) {
-            WcRhoArgs a{};
-            a.rho_in = rin[s];
-            a.U = U;
-            a.rho0 = rho;
-            a.rc_in = s == 0 ? rho : w->rc;
-            a.rho_out = rout[s];
-            a.rc_out = w->rc;
-            a.p = p;
-            a.dt = dt;
-            a.w = kRk4W[s];
-            a.cs = s < 3 ? kRk4C[s] : 0.0;
-            a.pc = w->pc;
-            a.last = s == 3;
-            if (int rc = launch_wc(ctx, &a, nullptr)) return rc;
-        }
-        if (w->B) {
-            WcLArgs l{};
-            l.B = w->B;
-            l.F = w->lf;
-            l.dbu = w->dbu;
-            if (int rc = launch_wc(ctx, nullptr, nullptr, &l)) return rc;
-        }
-        const double *uin[4] = {U, w->ua, w->ub, w->ua};
-        double *uout[4] = {w->ua, w->ub, w->ua, U};
-        for (int s = 0; s < 4; s++) {
-            WcUArgs a{};
-            a.U_in = uin[s];
-            a.U0 = (s == 0 || s == 3) ? nullptr : U;
-            a.rho = rho;
-            a.p = p;
-            a.ruc_in = s == 0 ? nullptr : w->uc;
-            a.F = w->B ? w->lf : nullptr;
-            a.U_out = uout[s];
-            a.ruc_out = w->uc;
-            a.dt = dt;
-            a.w = kRk4W[s];
-            a.cs = s < 3 ? kRk4C[s] : 0.0;
-            a.mu = w->mu;
-            a.last = s == 3;
-            if (int rc = launch_wc(ctx, nullptr, &a)) return rc;
-        }
+        if (int rc = wc_rho_eqn(w, dt, U, rho, p, 0)) return rc;
+        if (int rc = wc_u_eqn(w, dt, U, rho, p, nullptr)) return rc;
+    }
+    return NOVA_OK;
+}
+
+// The temperature equation and buoyancy of app/shearLayerWC.
+int wc_set_temperature(nova_wc *w, double ka, const double g[3])
+{
+    if (!(ka >= 0.0)) return fail(NOVA_ERR_ARG, "ka must be >= 0");
+    const size_t b1 = sizeof(double) * w->ctx->g.np;
+    for (double **b : {&w->ta, &w->tb, &w->tc})
+        if (!*b) NOVA_HIP(hipMalloc(b, b1));
+    w->ka = ka;
+    for (int c = 0; c < 3; c++) w->grav[c] = g[c];
+    return NOVA_OK;
+}
+
+// app/shearLayerWC order (shearLayer.cpp:197-200): UEqn with buoyancy (p as
+// given: the previous step's), rhoEqn, pEqn, TEqn.
+int wc_step_t(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p, double *T)
+{
+    if (!w->ta) return fail(NOVA_ERR_ARG, "nova_wc_set_temperature first");
+    for (int n = 0; n < nsteps; n++) {
+        if (int rc = wc_u_eqn(w, dt, U, rho, p, T)) return rc;
+        if (int rc = wc_rho_eqn(w, dt, U, rho, p, 1)) return rc;
+        if (int rc = wc_t_eqn(w, dt, U, rho, T)) return rc;
     }
     return NOVA_OK;
 }

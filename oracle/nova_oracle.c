@@ -868,38 +868,17 @@ void orc_mhd_step(const orc_mesh *M, orc_poisson *P, double *U, double *B, doubl
     for (int r = 0; r < 4; r++) free(dB[r]);
 }
 
-/* Weakly-compressible step, app/TG/TG.cpp:71-76 (app/OT/OT.cpp:92-95 with
- * B non-null): rhoEqn.H:10-35 (classic RK4 of rho with U frozen), pEqn.H:2
- * (p = pc rho^2; pc = 50 for c = 10),
- * UEqn.H:12-35 (classic RK4 of rho U with rho and p frozen).  Field algebra
- * acts on every node of the padded array (Field.hxx:617-1150), the
- * operators on the interior with zero ghosts, and rho and U are refilled
- * after every stage (rhoEqn.H:33, UEqn.H:33).  Stage s starts from the
- * step-start value plus (int(s/2)+1)/2 of the previous increment; the
- * accumulator adds RK4W[s] of each increment. */
-void orc_wc_step(const orc_mesh *M, double *U, double *rho, double *p, const double *B, double mu, double pc,
-                 double dbu, double dt)
+/* The weakly-compressible equation blocks.  Field algebra acts on every
+ * node of the padded array (Field.hxx:617-1150), the operators on the
+ * interior with zero ghosts, and the advanced field is refilled after every
+ * stage.  Stage s starts from the step-start value plus (int(s/2)+1)/2 of
+ * the previous increment; the accumulator adds RK4W[s] of each increment. */
+
+/* rhoEqn.H:10-35: classic RK4 of rho with U frozen, drho = dt (-div(rho U)). */
+static void wc_rho_eqn(const orc_mesh *M, const double *U, double *rho, double dt)
 {
-    const size_t n1 = NPTS(M), n3 = n1 * 3;
+    const size_t n1 = NPTS(M);
     double *rU = falloc(M, 3), *dv = falloc(M, 1), *dr = falloc(M, 1), *rc = falloc(M, 1), *r0 = falloc(M, 1);
-    double *t1 = falloc(M, 3), *t2 = falloc(M, 3), *gp = falloc(M, 3), *dU = falloc(M, 3);
-    double *rUc = falloc(M, 3), *rU0 = falloc(M, 3);
-    /* app/OT/UEqn.H:21-23: the Lorentz terms of a frozen B (BEqn.H never
-     * assigns B): -div((2 DBU) B, B) + grad(DBU (B & B)) */
-    double *t3 = NULL, *t4 = NULL;
-    if (B) {
-        double *X = falloc(M, 3), *s1 = falloc(M, 1);
-        t3 = falloc(M, 3);
-        t4 = falloc(M, 3);
-        #pragma omp parallel for
-        for (size_t q = 0; q < n1; q++) {
-            for (int c = 0; c < 3; c++) X[3 * q + c] = B[3 * q + c] * (2 * dbu);
-            s1[q] = (B[3 * q] * B[3 * q] + B[3 * q + 1] * B[3 * q + 1] + B[3 * q + 2] * B[3 * q + 2]) * dbu;
-        }
-        orc_div_adv(M, X, B, 3, t3);
-        orc_grad(M, s1, t4);
-        free(X); free(s1);
-    }
     memcpy(r0, rho, sizeof(double) * n1);
     memcpy(rc, rho, sizeof(double) * n1);
     for (int rk = 0; rk < 4; rk++) {
@@ -921,8 +900,33 @@ void orc_wc_step(const orc_mesh *M, double *U, double *rho, double *p, const dou
         }
         orc_fill(M, rho, 1, -1);
     }
-    #pragma omp parallel for
-    for (size_t q = 0; q < n1; q++) p[q] = pc * (rho[q] * rho[q]);
+    free(rU); free(dv); free(dr); free(rc); free(r0);
+}
+
+/* UEqn.H (app/TG:12-35, app/OT:12-35, app/shearLayerWC:12-40): classic RK4
+ * of rho U with rho, p (and B, T) frozen:
+ * drhoU = dt (((-div(rho U, U) - grad p) + laplacian(mu, U)) [- div((2 DBU)
+ * B, B) + grad(DBU (B & B))] [- (rho T) g]). */
+static void wc_u_eqn(const orc_mesh *M, double *U, const double *rho, const double *p, const double *B, double dbu,
+                     const double *T, const double g[3], double mu, double dt)
+{
+    const size_t n1 = NPTS(M), n3 = n1 * 3;
+    double *rU = falloc(M, 3), *t1 = falloc(M, 3), *t2 = falloc(M, 3), *gp = falloc(M, 3), *dU = falloc(M, 3);
+    double *rUc = falloc(M, 3), *rU0 = falloc(M, 3);
+    double *t3 = NULL, *t4 = NULL;
+    if (B) {
+        double *X = falloc(M, 3), *s1 = falloc(M, 1);
+        t3 = falloc(M, 3);
+        t4 = falloc(M, 3);
+        #pragma omp parallel for
+        for (size_t q = 0; q < n1; q++) {
+            for (int c = 0; c < 3; c++) X[3 * q + c] = B[3 * q + c] * (2 * dbu);
+            s1[q] = (B[3 * q] * B[3 * q] + B[3 * q + 1] * B[3 * q + 1] + B[3 * q + 2] * B[3 * q + 2]) * dbu;
+        }
+        orc_div_adv(M, X, B, 3, t3);
+        orc_grad(M, s1, t4);
+        free(X); free(s1);
+    }
     #pragma omp parallel for
     for (size_t q = 0; q < n1; q++)
         for (int c = 0; c < 3; c++) rUc[3 * q + c] = rU0[3 * q + c] = rho[q] * U[3 * q + c];
@@ -935,7 +939,10 @@ void orc_wc_step(const orc_mesh *M, double *U, double *rho, double *p, const dou
         orc_laplacian(M, mu, U, 3, t2);
         #pragma omp parallel for
         for (size_t q = 0; q < n3; q++) {
-            dU[q] = B ? dt * (((((-t1[q]) - gp[q]) + t2[q]) - t3[q]) + t4[q]) : dt * (((-t1[q]) - gp[q]) + t2[q]);
+            double t = ((-t1[q]) - gp[q]) + t2[q];
+            if (B) t = (t - t3[q]) + t4[q];
+            if (T) t = t - (rho[q / 3] * T[q / 3]) * g[q % 3];
+            dU[q] = dt * t;
             rUc[q] = rUc[q] + RK4W[rk] * dU[q];
         }
         const double cs = (rk / 2 + 1) / 2.0;
@@ -946,9 +953,64 @@ void orc_wc_step(const orc_mesh *M, double *U, double *rho, double *p, const dou
                                       : rUc[3 * q + c] / rho[q];
         orc_fill(M, U, 3, 1);
     }
-    (void)n3;
-    free(rU); free(dv); free(dr); free(rc); free(r0); free(t1); free(t2); free(gp); free(dU);
-    free(rUc); free(rU0); free(t3); free(t4);
+    free(rU); free(t1); free(t2); free(gp); free(dU); free(rUc); free(rU0); free(t3); free(t4);
+}
+
+/* TEqn.H (app/shearLayerWC:12-36): classic RK4 of rho T with rho and U
+ * frozen, drhoT = dt (-div(rho U, T) + laplacian(ka, T)). */
+static void wc_t_eqn(const orc_mesh *M, const double *U, const double *rho, double *T, double ka, double dt)
+{
+    const size_t n1 = NPTS(M);
+    double *rU = falloc(M, 3), *t1 = falloc(M, 1), *t2 = falloc(M, 1), *dT = falloc(M, 1);
+    double *rTc = falloc(M, 1), *rT0 = falloc(M, 1);
+    #pragma omp parallel for
+    for (size_t q = 0; q < n1; q++) {
+        rTc[q] = rT0[q] = rho[q] * T[q];
+        for (int c = 0; c < 3; c++) rU[3 * q + c] = rho[q] * U[3 * q + c];
+    }
+    for (int rk = 0; rk < 4; rk++) {
+        orc_div_adv(M, rU, T, 1, t1);
+        orc_laplacian(M, ka, T, 1, t2);
+        #pragma omp parallel for
+        for (size_t q = 0; q < n1; q++) {
+            dT[q] = dt * ((-t1[q]) + t2[q]);
+            rTc[q] = rTc[q] + RK4W[rk] * dT[q];
+        }
+        const double cs = (rk / 2 + 1) / 2.0;
+        #pragma omp parallel for
+        for (size_t q = 0; q < n1; q++) T[q] = rk < 3 ? rT0[q] / rho[q] + cs * (dT[q] / rho[q]) : rTc[q] / rho[q];
+        orc_fill(M, T, 1, -1);
+    }
+    free(rU); free(t1); free(t2); free(dT); free(rTc); free(rT0);
+}
+
+/* Weakly-compressible step of app/TG (TG.cpp:71-76) and app/OT (OT.cpp:92-95,
+ * B non-null; BEqn.H never assigns B, so it stays frozen): rhoEqn.H, pEqn.H
+ * (p = pc rho^2; pc = 50 for c = 10), UEqn.H. */
+void orc_wc_step(const orc_mesh *M, double *U, double *rho, double *p, const double *B, double mu, double pc,
+                 double dbu, double dt)
+{
+    const size_t n1 = NPTS(M);
+    const double g0[3] = {0.0, 0.0, 0.0};
+    wc_rho_eqn(M, U, rho, dt);
+    #pragma omp parallel for
+    for (size_t q = 0; q < n1; q++) p[q] = pc * (rho[q] * rho[q]);
+    wc_u_eqn(M, U, rho, p, B, dbu, NULL, g0, mu, dt);
+}
+
+/* Weakly-compressible step of app/shearLayerWC (shearLayer.cpp:197-200):
+ * UEqn.H with the buoyancy -(rho T) g, rhoEqn.H, pEqn.H (p = pc rho^2 with
+ * pc = c^2/(2 rho0)), TEqn.H.  p enters UEqn as given (the previous
+ * step's). */
+void orc_wcs_step(const orc_mesh *M, double *U, double *rho, double *p, double *T, double mu, double ka,
+                  const double g[3], double pc, double dt)
+{
+    const size_t n1 = NPTS(M);
+    wc_u_eqn(M, U, rho, p, NULL, 0.0, T, g, mu, dt);
+    wc_rho_eqn(M, U, rho, dt);
+    #pragma omp parallel for
+    for (size_t q = 0; q < n1; q++) p[q] = (pc * rho[q]) * rho[q]; /* pEqn.H:2, (c*c/2.0/rho0)*rho*rho */
+    wc_t_eqn(M, U, rho, T, ka, dt);
 }
 
 /* ------------------------------------------------------------------ */

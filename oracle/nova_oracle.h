@@ -97,6 +97,10 @@ void orc_mhd_step(const orc_mesh *, orc_poisson *, double *U, double *B,
  * terms of app/OT/UEqn.H with the frozen B of app/OT (coefficient dbu). */
 void orc_wc_step(const orc_mesh *, double *U, double *rho, double *p, const double *B, double mu, double pc,
                  double dbu, double dt);
+/* Weakly-compressible step of app/shearLayerWC (UEqn with buoyancy, rhoEqn,
+ * pEqn, TEqn): U, rho, T in/out; p in (used by UEqn) and out. */
+void orc_wcs_step(const orc_mesh *, double *U, double *rho, double *p, double *T, double mu, double ka,
+                  const double g[3], double pc, double dt);
 /* Projection: F <- F - s*grad(lap^-1(div(F)/s)), p receives the potential. */
 void orc_project(const orc_mesh *, orc_poisson *, double *F, double *p, double s);
 

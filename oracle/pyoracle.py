@@ -58,6 +58,7 @@ def _load() -> C.CDLL:
         "orc_ns_step": ([_P, _P, _D, _D, _D, C.c_double, C.c_double, C.c_int], None),
         "orc_mhd_step": ([_P, _P, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_int], None),
         "orc_wc_step": ([_P, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_double], None),
+        "orc_wcs_step": ([_P, _D, _D, _D, _D, C.c_double, C.c_double, _D, C.c_double, C.c_double], None),
         "orc_project": ([_P, _P, _D, _D, C.c_double], None),
         "orc_ke": ([_P, _D], C.c_double),
         "orc_maxabs_div": ([_P, _D], C.c_double),
@@ -211,6 +212,11 @@ class OMesh:
         """One weakly-compressible step in place (U: nc=3, rho and p: nc=1);
         B (nc=3, frozen) adds the Lorentz terms of app/OT/UEqn.H."""
         lib().orc_wc_step(self._h, _d(U), _d(rho), _d(p), _d(B) if B is not None else None, mu, pc, dbu, dt)
+
+    def wcs_step(self, U, rho, p, T, mu, ka, g, pc, dt):
+        """One app/shearLayerWC step in place (U nc=3; rho, p, T nc=1)."""
+        gv = (C.c_double * 3)(*g)
+        lib().orc_wcs_step(self._h, _d(U), _d(rho), _d(p), _d(T), mu, ka, gv, pc, dt)
 
     def compact_deriv(self, f, direction, order=1):
         out = self.field(f.shape[-1])

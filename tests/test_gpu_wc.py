@@ -1,5 +1,6 @@
 """Parity of the weakly-compressible step (app/TG/TG.cpp:71-76 with
-rhoEqn.H, pEqn.H, UEqn.H; app/OT with its frozen magnetic field) through
+rhoEqn.H, pEqn.H, UEqn.H; app/OT with its frozen magnetic field;
+app/shearLayerWC with the temperature equation and buoyancy) through
 the C ABI with the oracle: one step per-field relative L2 <= 1e-12,
 kinetic-energy history over 20 steps <= 1e-10 relative, mass conserved to
 rounding; periodic and y-walled grids, stencil orders m = 4, 6, 8."""
@@ -148,6 +149,47 @@ def test_wc_ot_with_frozen_field_matches_oracle(nv, orc, wall):
     assert rel_l2(host(U2), host(Ud)) > 1e-6
 
 
+@pytest.mark.parametrize("wall", [False, True])
+def test_wc_shear_layer_with_temperature_matches_oracle(nv, orc, wall):
+    # app/shearLayerWC (shearLayer.cpp:31-45, 114-150, 197-200) with the shear
+    # along y (walls in y: the oracle's walled direction): u = u0/2 tanh(2y/h0)
+    # plus a seeded perturbation, T = t0/2 tanh(2y/h0) plus noise, rho0
+    # uniform, p = c^2/(2 rho0) rho^2, buoyancy along -y
+    mach, re0, ri0, pr, rho0, c = 0.2, 1967.0, 0.08, 7.0, 999.57, 0.0417
+    u0, h0, t0 = c * mach, 0.2, 0.1
+    mu = rho0 * u0 * h0 / re0
+    ka = mu / pr
+    g = (0.0, -ri0 * u0 * u0 / t0 / h0, 0.0)
+    pc = c * c / 2.0 / rho0
+    N, L = (24, 16, 16), (14 * h0, 15 * h0, 1.7 * h0)
+    bc = ("periodic", "wall" if wall else "periodic", "periodic")
+    G = nv.Mesh(N, L, bc=bc)
+    O = orc.OMesh(N, L, wall_y=wall)
+    x = [np.arange(O.stored[a]) * (O.L[a] / O.N[a]) for a in range(3)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    y = Y - 7.5 * h0
+    rng = np.random.default_rng(13)
+    env = 1.0 / np.cosh(2 * y / h0) ** 2
+    U = np.stack([u0 / 2 * np.tanh(2 * y / h0), 0 * X, 0 * X]) + 0.005 * u0 / 2 * rng.uniform(-1, 1, (3, *O.stored)) * env
+    if wall:
+        U[1, :, 0, :] = U[1, :, -1, :] = 0.0
+    T = (t0 / 2 * np.tanh(2 * y / h0) + 0.005 * t0 / 2 * rng.uniform(-1, 1, O.stored) * env)[None]
+    rho = np.full((1, *O.stored), rho0)
+    p = pc * rho * rho
+    dt = 0.5
+    W = nv.WC(G, mu, pc)
+    W.set_temperature(ka, g)
+    Ud, rd, pd, Td = dev(U), dev(rho), dev(p), dev(T)
+    W.step_t(dt, 3, Ud, rd, pd, Td)
+    Uo, ro, po, To = O.from_compact(U), O.from_compact(rho), O.from_compact(p), O.from_compact(T)
+    for _ in range(3):
+        O.wcs_step(Uo, ro, po, To, mu, ka, g, pc, dt)
+    errs = [rel_l2(host(a), O.interior(b)) for a, b in ((Ud, Uo), (rd, ro), (pd, po), (Td, To))]
+    print("wall", wall, "rel L2 U, rho, p, T", errs, flush=True)
+    assert max(errs) < RTOL
+    assert rel_l2(host(Td), T) > 1e-8 and rel_l2(host(rd), rho) > 1e-12
+
+
 def test_wc_rejects_bad_arguments(nv):
     G = nv.Mesh((16, 16, 16), (TWO_PI,) * 3)
     with pytest.raises(nv.NovaError):
@@ -158,3 +200,7 @@ def test_wc_rejects_bad_arguments(nv):
     r = torch.ones((1, 16, 16, 16), dtype=torch.float64, device="cuda")
     with pytest.raises(nv.NovaError):
         W.step(0.01, 1, U, r, None)
+    with pytest.raises(nv.NovaError):  # the temperature equation is not set up
+        W.step_t(0.01, 1, U, r, r.clone(), r.clone())
+    with pytest.raises(nv.NovaError):
+        W.set_temperature(-1.0, (0.0, 0.0, -1.0))

@@ -166,3 +166,66 @@ def test_lorentz_force_of_a_frozen_field(orc):
         errs.append(np.abs(U[s][..., 0] - want[s]).max() / np.abs(want[s]).max())
     assert errs[0] < 1e-5
     assert 2.5 < errs[0] / errs[1] < 6.0 or errs[1] < 1e-13
+
+
+# app/shearLayerWC (shearLayer.cpp:197-200): UEqn with buoyancy, rhoEqn,
+# pEqn, TEqn
+
+
+@pytest.mark.parametrize("k,u0,dt", [(3, 0.7, 0.05), (4, -1.1, 0.03)])
+def test_temperature_wave_follows_rk4_of_the_advection_symbol(orc, k, u0, dt):
+    # uniform rho and U = (u0, 0, 0), g = 0: UEqn and rhoEqn leave the
+    # uniform state alone and d(rho T)/dt = -rho u0 dT/dx
+    N = (32, 8, 8)
+    M = orc.OMesh(N, (TWO_PI,) * 3)
+    X, Y, Z = grid(M)
+    eps = 1e-2
+    U, rho, p = M.field(3), M.field(1), M.field(1)
+    U[..., 0] = u0
+    rho[...] = 1.3
+    p[...] = 7.0
+    T = filled(M, 0.5 + eps * np.cos(k * X), 1)
+    M.wcs_step(U, rho, p, T, 0.0, 0.0, (0.0, 0.0, 0.0), 2.0, dt)
+    h = TWO_PI / N[0]
+    g = rk4(-1j * u0 * k1(k, h) * dt)
+    want = 0.5 + eps * np.real(g * np.exp(1j * k * X))
+    err = np.abs(T[..., 0][inner(M)] - want[inner(M)]).max()
+    assert err < 1e-14, err
+    assert np.all(U[..., 0][inner(M)] == u0) and np.all(rho[..., 0][inner(M)] == 1.3)
+    assert np.all(p[..., 0][inner(M)] == (2.0 * 1.3) * 1.3)
+
+
+@pytest.mark.parametrize("rho0", [1.0, 3.0])
+def test_temperature_diffuses_with_rk4_of_the_diffusion_symbol(orc, rho0):
+    N = (32, 8, 8)
+    k, ka, dt, eps = 5, 0.02, 0.05, 1e-2
+    M = orc.OMesh(N, (TWO_PI,) * 3)
+    X, Y, Z = grid(M)
+    U, rho, p = M.field(3), M.field(1), M.field(1)
+    rho[...] = rho0
+    T = filled(M, eps * np.sin(k * X), 1)
+    M.wcs_step(U, rho, p, T, 0.0, ka, (0.0, 0.0, 0.0), 1.0, dt)
+    h = TWO_PI / N[0]
+    want = rk4(-(ka / rho0) * k2(k, h) * dt) * eps * np.sin(k * X)
+    assert np.abs(T[..., 0][inner(M)] - want[inner(M)]).max() < 1e-16 * 10
+    assert np.abs(U[inner(M)]).max() == 0.0
+
+
+def test_buoyancy_drives_the_flow_along_minus_g(orc):
+    # from rest, rho = 1, p uniform, T = T(x): -(rho T) g is the only force,
+    # no term moves x-only fields along z, so after one step U_z = dt g0 T
+    N = (24, 8, 8)
+    M = orc.OMesh(N, (TWO_PI,) * 3)
+    X, Y, Z = grid(M)
+    g0, dt = 9.81, 0.01
+    U, rho, p = M.field(3), M.field(1), M.field(1)
+    rho[...] = 1.0
+    p[...] = 4.0
+    T0 = 0.3 + 0.1 * np.cos(2 * X)
+    T = filled(M, T0, 1)
+    M.wcs_step(U, rho, p, T, 0.0, 0.0, (0.0, 0.0, -g0), 4.0, dt)
+    s = inner(M)
+    want = dt * g0 * T0
+    assert np.abs(U[s][..., 2] - want[s]).max() < 1e-15
+    assert np.abs(U[s][..., :2]).max() == 0.0
+    assert np.array_equal(T[..., 0][s], T0[s]) and np.all(rho[..., 0][s] == 1.0)
