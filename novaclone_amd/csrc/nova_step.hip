@@ -159,7 +159,8 @@ int launch_stream_h(nova_ctx *ctx, StreamArgs<NF> a, const char *name)
         else
             hipLaunchKernelGGL((k_stage_rows<H, NF, false>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
     } else if constexpr (H == 3 && MODEL == MODEL_NS) {
-        // sixth-order NS stages compile the SSP combination in
+        // sixth-order NS stages compile the SSP combination in (the MHD
+        // stages, at one wave per SIMD, spill when their branches go)
         const int cmb = a.sa.form != FORM_SSP ? 0 : (a.sa.alpha == 0.0 ? 2 : (a.X0[0] ? 1 : 0));
         if (walls) {
             if (cmb == 1) launch_stream_w<H, MODEL, true, 1>(ctx, a);
