@@ -106,7 +106,11 @@ int nova_timing_collect(nova_ctx *ctx, char *buf, size_t buflen);
  * (default: 3), variants 5 and 6 stage 32 x 8 and 16 x 16 tiles at two waves
  * per SIMD with two planes in flight (one-field models), variants 7 and 8 the
  * same with 512-thread 32 x 16 and 64 x 8 tiles (two-field models keep
- * 32 x 8 and 64 x 4 at one wave); stage_ichunk = planes per
+ * 32 x 8 and 64 x 4 at one wave), variants 9 and 10 the 64 x 8 and 32 x 16
+ * tiles with a ring of staged planes (the halo of a plane is loaded with its
+ * interior, H planes ahead; two-field models as for 8 and 7), variants 11
+ * and 12 the 64 x 8 and 32 x 16 tiles with two alternating staged planes
+ * (two-field models: 64 x 4 and 32 x 8 at one wave); stage_ichunk = planes per
  * streamed chunk (0 = auto); proj_variant 0 fuses the divergence into the
  * first FFT pass, 1 computes it (and the gradient correction) with streamed
  * row kernels, 2 (default) with LDS-staged streamed tiles.  Results are

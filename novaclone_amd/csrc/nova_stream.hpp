@@ -43,11 +43,20 @@ struct StreamArgs {
 // MODEL_NS: X = {U}; MODEL_MHD: X = {U, B};  out = stage combination.
 // MODEL_TLM: X = {U, d}, X0[1] = step-start tangent; out[0] = tangent stage.
 // MODEL_ADJ: X = {U, m}; out[0] = ob (m + dt J_R(U)^T m) + yg Y.
-// LDS doubles of one tile's staged plane
-template <int H, int MODEL, int kTK, int kTJ = 256 / kTK>
+// Staging of the (j,k) planes in LDS: STG_ONE holds one plane (two barriers
+// per plane), STG_TWO two alternating planes (one barrier), STG_RING a ring
+// of H + 2 planes whose halos are loaded H planes ahead (one barrier).
+constexpr int STG_ONE = 0, STG_TWO = 1, STG_RING = 2;
+template <int H, int STG>
+constexpr int stage_slots()
+{
+    return STG == STG_ONE ? 1 : (STG == STG_TWO ? 2 : H + 2);
+}
+// LDS doubles of one tile's staged planes
+template <int H, int MODEL, int kTK, int kTJ = 256 / kTK, int STG = STG_ONE>
 constexpr int stage_lds()
 {
-    return 3 * model_nf<MODEL>() * (kTJ + 2 * H) * (kTK + 2 * H);
+    return stage_slots<H, STG>() * 3 * model_nf<MODEL>() * (kTJ + 2 * H) * (kTK + 2 * H);
 }
 
 // A copy of x in a register of its own.  With two prefetch sets the value
@@ -72,7 +81,15 @@ __device__ __attribute__((always_inline)) inline double fresh(double x)
 // Stage combination of a launch: CMB 0 reads the form at run time; the
 // SSP forms are compiled in where the launch knows them (1: alpha X0 +
 // beta (X + dX), step-start fields present; 2: X + dX, none read).
-template <int H, int MODEL, bool WALLS, int kTK, int PF = 1, int kTJ = 256 / kTK, int CMB = 0>
+// STG_RING: the halo of plane p is loaded together with the queue entry
+// (the interior) of plane p, H planes before plane p is computed, and kept
+// in the ring of staged planes: a neighbouring tile loads those nodes as its
+// interior at about the same time, so the halo comes from L2 instead of
+// being fetched again H planes later, after the streaming traffic has
+// evicted it.  With two or more slots, the slot a plane is written to was
+// last read two planes earlier, before the barrier of the previous plane,
+// so one barrier per plane suffices.
+template <int H, int MODEL, bool WALLS, int kTK, int PF = 1, int kTJ = 256 / kTK, int CMB = 0, int STG = STG_ONE>
 __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, const StreamArgs<model_nf<MODEL>()> &a,
                                                                  int t, double *shbuf)
 {
@@ -82,7 +99,11 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
     constexpr int LK = kTK + 2 * H, LJ = kTJ + 2 * H;
     constexpr int NHALO = 2 * H * (kTK + kTJ);
     constexpr int NSLOT = (NHALO + kTK * kTJ - 1) / (kTK * kTJ);
-    auto sh = [&](int n, int r, int c) -> double & { return shbuf[(n * LJ + r) * LK + c]; };
+    // staged plane slots (plane i in slot (i - i0) mod NS)
+    constexpr bool RING = STG == STG_RING;
+    constexpr int NS = stage_slots<H, STG>();
+    constexpr int RS = NC * LJ * LK;
+    auto shr = [&](int slot, int n, int r, int c) -> double & { return shbuf[slot * RS + (n * LJ + r) * LK + c]; };
     const int kt = t % a.nkt;
     const int jt = (t / a.nkt) % a.njt;
     const int ic = t / (a.nkt * a.njt);
@@ -205,7 +226,7 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
 #pragma unroll
                 for (int c = 0; c < 3; c++) P.nq[f][c] = ldp(a.X[f], c, po, col);
         }
-        const int64_t ph = plane_off(p, P.fh);
+        const int64_t ph = plane_off(RING ? p + H : p, P.fh);
 #pragma unroll
         for (int s = 0; s < NSLOT; s++)
 #pragma unroll
@@ -239,24 +260,41 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
     // are one set; PF = 2: two sets alternate, so two planes of loads are
     // in flight while a plane is computed.
     auto body = [&](int i, Pf &Cur, Pf &Nxt) {
+        if constexpr (RING) {
+            // plane i + H into its slot (it held plane i - 2)
+            const int sw = (i + H - i0) % NS;
 #pragma unroll
-        for (int f = 0; f < NF; f++)
+            for (int f = 0; f < NF; f++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) sh(f * 3 + c, tj + H, tk + H) = q[f][c][H];
+                for (int c = 0; c < 3; c++) shr(sw, f * 3 + c, tj + H, tk + H) = q[f][c][2 * H];
 #pragma unroll
-        for (int s = 0; s < NSLOT; s++)
-            if (hon[s])
+            for (int s = 0; s < NSLOT; s++)
+                if (hon[s])
 #pragma unroll
-                for (int n = 0; n < NC; n++)
-                    sh(n, hrow[s], hcolk[s]) = rsign<WALLS>(n % 3, 0, Cur.fh) * hsg[s][n % 3] * Cur.hv[s][n];
+                    for (int n = 0; n < NC; n++)
+                        shr(sw, n, hrow[s], hcolk[s]) = rsign<WALLS>(n % 3, 0, Cur.fh) * hsg[s][n % 3] * Cur.hv[s][n];
+        } else {
+            const int sw = NS == 1 ? 0 : (i - i0) & 1;
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) shr(sw, f * 3 + c, tj + H, tk + H) = q[f][c][H];
+#pragma unroll
+            for (int s = 0; s < NSLOT; s++)
+                if (hon[s])
+#pragma unroll
+                    for (int n = 0; n < NC; n++)
+                        shr(sw, n, hrow[s], hcolk[s]) = rsign<WALLS>(n % 3, 0, Cur.fh) * hsg[s][n % 3] * Cur.hv[s][n];
+        }
         __syncthreads();
         load_pf(Cur, i + PF < i1 ? i + PF : i1 - 1, true);
 
         if (valid) {
+            const int sr = NS == 1 ? 0 : (RING ? (i - i0) % NS : (i - i0) & 1);
             auto A = [&](int f, int c, int d, int o) -> double {
                 if (d == 0) return q[f][c][H + o];
-                if (d == 1) return sh(f * 3 + c, tj + H + o, tk + H);
-                return sh(f * 3 + c, tj + H, tk + H + o);
+                if (d == 1) return shr(sr, f * 3 + c, tj + H + o, tk + H);
+                return shr(sr, f * 3 + c, tj + H, tk + H + o);
             };
             const int64_t pi = (int64_t)i * plane;
             const StageArgs &sa = a.sa;
@@ -313,7 +351,7 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                 }
             }
         }
-        __syncthreads();
+        if constexpr (NS == 1) __syncthreads();
         {
 #pragma unroll
             for (int f = 0; f < NF; f++)
@@ -334,6 +372,28 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
         }
     };
 
+    if constexpr (RING) {
+        // planes i0 .. i0 + H - 1 into slots 0 .. H - 1: interiors from the
+        // queue, halos loaded here (body i writes plane i + H)
+#pragma unroll
+        for (int pp = 0; pp < H; pp++) {
+            int fh;
+            const int64_t ph = plane_off(i0 + pp, fh);
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) shr(pp, f * 3 + c, tj + H, tk + H) = q[f][c][H + pp];
+#pragma unroll
+            for (int s = 0; s < NSLOT; s++)
+                if (hon[s])
+#pragma unroll
+                    for (int f = 0; f < NF; f++)
+#pragma unroll
+                        for (int c = 0; c < 3; c++)
+                            shr(pp, f * 3 + c, hrow[s], hcolk[s]) =
+                                rsign<WALLS>(c, 0, fh) * hsg[s][c] * ldp(a.X[f], c, ph, hsrc[s]);
+        }
+    }
     if constexpr (PF == 1) {
         Pf S;
         load_pf(S, i0, false);
@@ -351,11 +411,11 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
     }
 }
 
-template <int H, int MODEL, bool WALLS, int MINW, int kTK, int kTJ, int CMB>
+template <int H, int MODEL, bool WALLS, int MINW, int kTK, int kTJ, int CMB, int STG = STG_ONE>
 __global__ void __launch_bounds__(kTK *kTJ) __attribute__((amdgpu_waves_per_eu(MINW)))
 k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
 {
-    __shared__ double sh[stage_lds<H, MODEL, kTK, kTJ>()];
+    __shared__ double sh[stage_lds<H, MODEL, kTK, kTJ, STG>()];
     // consecutive tiles share halos, so each XCD (blocks are dealt
     // round-robin over 8) takes a contiguous run of tiles
     const int T = a.nkt * a.njt * a.nic;
@@ -363,7 +423,7 @@ k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
     if (T % 8 == 0) t = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
     // shapes held to one or two waves per SIMD have the registers for a
     // second prefetch set
-    stage_tile<H, MODEL, WALLS, kTK, MINW <= 2 ? 2 : 1, kTJ, CMB>(g, a, t, sh);
+    stage_tile<H, MODEL, WALLS, kTK, MINW <= 2 ? 2 : 1, kTJ, CMB, STG>(g, a, t, sh);
 }
 
 int pick_ichunk(const Grid &g, int nkt, int njt)
@@ -375,7 +435,7 @@ int pick_ichunk(const Grid &g, int nkt, int njt)
     return c;
 }
 
-template <int H, int MODEL, bool WALLS, int CMB, int MINW, int TK, int TJ = 256 / TK>
+template <int H, int MODEL, bool WALLS, int CMB, int MINW, int TK, int TJ = 256 / TK, int STG = STG_ONE>
 void launch_stream_tile(nova_ctx *ctx, StreamArgs<model_nf<MODEL>()> a)
 {
     const Grid &g = ctx->g;
@@ -385,7 +445,7 @@ void launch_stream_tile(nova_ctx *ctx, StreamArgs<model_nf<MODEL>()> a)
     a.ichunk = ic > 0 ? (ic < g.s[0] ? ic : g.s[0]) : pick_ichunk(g, a.nkt, a.njt);
     a.nic = (g.s[0] + a.ichunk - 1) / a.ichunk;
     const int T = a.nkt * a.njt * a.nic;
-    hipLaunchKernelGGL((k_stage_stream<H, MODEL, WALLS, MINW, TK, TJ, CMB>), dim3(T), dim3(TK, TJ, 1), 0, ctx->stream, g,
+    hipLaunchKernelGGL((k_stage_stream<H, MODEL, WALLS, MINW, TK, TJ, CMB, STG>), dim3(T), dim3(TK, TJ, 1), 0, ctx->stream, g,
                        a);
 }
 
@@ -413,6 +473,31 @@ void launch_stream_w(nova_ctx *ctx, const StreamArgs<model_nf<MODEL>()> &a)
     case 8:
         if constexpr (W == 3) launch_stream_tile<H, MODEL, WALLS, CMB, 2, 64, 8>(ctx, a);
         else launch_stream_tile<H, MODEL, WALLS, CMB, 1, 64>(ctx, a);
+        break;
+    // the halo ring (the halo of a plane loaded with its interior, H planes
+    // ahead): 512-thread 64 x 8 and 32 x 16 tiles (one-field models; a ring
+    // that does not fit the 160 KiB of LDS alternates two planes instead)
+    case 9:
+        if constexpr (W == 3) {
+            constexpr int S = stage_lds<H, MODEL, 64, 8, STG_RING>() * 8 <= 160 * 1024 ? STG_RING : STG_TWO;
+            launch_stream_tile<H, MODEL, WALLS, CMB, 2, 64, 8, S>(ctx, a);
+        } else launch_stream_tile<H, MODEL, WALLS, CMB, 1, 64>(ctx, a);
+        break;
+    case 10:
+        if constexpr (W == 3) {
+            constexpr int S = stage_lds<H, MODEL, 32, 16, STG_RING>() * 8 <= 160 * 1024 ? STG_RING : STG_TWO;
+            launch_stream_tile<H, MODEL, WALLS, CMB, 2, 32, 16, S>(ctx, a);
+        } else launch_stream_tile<H, MODEL, WALLS, CMB, 1, 32>(ctx, a);
+        break;
+    // two alternating staged planes (one barrier per plane): 512-thread
+    // 64 x 8 and 32 x 16 tiles, 256-thread 64 x 4 tiles for two-field models
+    case 11:
+        if constexpr (W == 3) launch_stream_tile<H, MODEL, WALLS, CMB, 2, 64, 8, STG_TWO>(ctx, a);
+        else launch_stream_tile<H, MODEL, WALLS, CMB, 1, 64, 4, STG_TWO>(ctx, a);
+        break;
+    case 12:
+        if constexpr (W == 3) launch_stream_tile<H, MODEL, WALLS, CMB, 2, 32, 16, STG_TWO>(ctx, a);
+        else launch_stream_tile<H, MODEL, WALLS, CMB, 1, 32, 8, STG_TWO>(ctx, a);
         break;
     default: launch_stream_tile<H, MODEL, WALLS, CMB, W, 32>(ctx, a); break;
     }

@@ -46,7 +46,7 @@ def test_every_stage_variant_gives_the_same_step(nv, wall):
         U0[1, :, -1, :] = 0.0
     P.project(U0, 1.0)
     ref = None
-    for v in (3, 0, 1, 2, 4, 5, 6, 7, 8):
+    for v in (3, 0, 1, 2, 4, 5, 6, 7, 8, 9, 10, 11, 12):
         for c in (0, 5):
             G.tune(v, c, 1)
             U = U0.clone()
@@ -58,6 +58,41 @@ def test_every_stage_variant_gives_the_same_step(nv, wall):
                 # multiply-adds are fused per source expression, so every
                 # launch shape rounds identically
                 assert torch.equal(U, ref)
+
+
+@pytest.mark.parametrize("wall", [False, True])
+def test_two_field_stage_variants_give_the_same_results(nv, wall):
+    # MHD, tangent-linear and adjoint stages under every launch shape they
+    # take (a variant a model does not have runs its default shape)
+    import torch
+    N = (24, 12, 32)
+    G = nv.Mesh(N, (TWO_PI, 3.0 if wall else TWO_PI, TWO_PI), bc=("periodic", "wall" if wall else "periodic",
+                                                                    "periodic"))
+    P = nv.Poisson(G)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    U0 = torch.randn((3, *G.shape), generator=gen, dtype=torch.float64, device="cuda")
+    B0 = torch.randn((3, *G.shape), generator=gen, dtype=torch.float64, device="cuda")
+    if wall:
+        U0[1, :, 0, :] = U0[1, :, -1, :] = 0.0
+        B0[1, :, 0, :] = B0[1, :, -1, :] = 0.0
+    P.project(U0, 1.0)
+    P.project(B0, 1.0)
+    S = nv.Solver(P, nv.MODEL_MHD, nv.RK_SSP3, nu=1e-2, eta=2e-2)
+    A = nv.Adjoint(nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-2), 2, 4)
+    ref = None
+    for v in (3, 0, 2, 4, 11, 12):
+        G.tune(v, 0, 1)
+        U, B = U0.clone(), B0.clone()
+        S.step(1e-3, 2, U, G.field(1), B, G.field(1))
+        A.forward(1e-3, 3, U0)
+        lam = A.backward(B0)
+        d = A.tlm(B0)
+        out = (U, B, lam, d)
+        if ref is None:
+            ref = tuple(t.clone() for t in out)
+        else:
+            print("variant", v, [rel(o, r) for o, r in zip(out, ref)], flush=True)
+            assert all(torch.equal(o, r) for o, r in zip(out, ref))
 
 
 def test_solver_autotune_keeps_results_and_rerecords_graph(nv):
@@ -74,7 +109,7 @@ def test_solver_autotune_keeps_results_and_rerecords_graph(nv):
     rep = Sg.autotune(U0, reps=2)
     print(rep, flush=True)
     assert "ns_stage chosen variant" in rep and "projection chosen variant" in rep
-    assert rep.count("ns_stage variant") == 45
+    assert rep.count("ns_stage variant") == 65
     Se.step(2e-3, 3, Ue, p)
     Sg.step(2e-3, 3, Ug, p)  # the graph is recorded again with the tuned shapes
     torch.cuda.synchronize()
