@@ -266,6 +266,15 @@ int nova_wc_set_temperature(nova_wc *w, double ka, double gx, double gy, double 
  * UEqn (with buoyancy; p as given, i.e. the previous step's), rhoEqn, pEqn
  * (p = (pc rho) rho, pc = c^2/(2 rho0)), TEqn.  U, rho, T in/out; p in/out. */
 int nova_wc_step_t(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p, double *T);
+/* The initial state of app/shearLayerWC after its initial conditions
+ * (hydrostaticP.H, shearLayer.cpp:139-151): per (i, j) column, from the
+ * node below the top (k = N_z - 1) down to k = 0, p += sum_{k' >= k}
+ * ((T dz) g_z) rho0, accumulated from the top in that order (g_z from
+ * nova_wc_set_temperature, which must come first); the top node k = N_z of a
+ * walled z and the plane i = N_x of a walled x keep p, as the reference's
+ * loop bounds leave them.  Then rho = (p / pc)^(1/2) on every node.  T in,
+ * p in/out, rho out; device pointers, stored layout. */
+int nova_wc_hydrostatic(nova_wc *w, const double *T, double rho0, double *p, double *rho);
 
 /* ---- ensembles of uneven instances (no reference counterpart: NOVA runs
  * one case per process; the step of each instance is that of

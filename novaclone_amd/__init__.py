@@ -108,6 +108,7 @@ _SIGS = {
     "nova_wc_set_magnetic": [_P, _P, _D],
     "nova_wc_set_temperature": [_P, _D, _D, _D, _D],
     "nova_wc_step_t": [_P, _D, _I, _P, _P, _P, _P],
+    "nova_wc_hydrostatic": [_P, _P, _D, _P, _P],
     "nova_ensemble_create": [C.POINTER(InstanceDesc), _I, _I, C.POINTER(_P)],
     "nova_ensemble_destroy": [_P],
     "nova_ensemble_set_stream": [_P, _P],
@@ -435,6 +436,11 @@ class WC:
         """Advance nsteps in app/shearLayerWC order: U, rho, T in/out, p in/out."""
         _check(lib().nova_wc_step_t(self._h, float(dt), int(nsteps), _ptr(U), _ptr(rho), _ptr(p), _ptr(T)),
                "nova_wc_step_t")
+
+    def hydrostatic(self, T, rho0: float, p, rho) -> None:
+        """app/shearLayerWC's initial state (hydrostaticP.H): p += the
+        hydrostatic pressure of T under g_z, then rho = (p/pc)^(1/2)."""
+        _check(lib().nova_wc_hydrostatic(self._h, _ptr(T), float(rho0), _ptr(p), _ptr(rho)), "nova_wc_hydrostatic")
 
     def close(self):
         if getattr(self, "_h", None):

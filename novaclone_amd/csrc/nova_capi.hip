@@ -121,6 +121,7 @@ int wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p
 int wc_set_magnetic(nova_wc *w, const double *B, double dbu);
 int wc_set_temperature(nova_wc *w, double ka, const double g[3]);
 int wc_step_t(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p, double *T);
+int wc_hydrostatic(nova_wc *w, const double *T, double rho0, double *p, double *rho);
 
 // Mesh::z (src/Mesh/Mesh.cpp:373-407) for padded index kp on one rank.
 static double mesh_z(const nova_grid_desc &d, double dz, double hb, int m, int kp)
@@ -637,6 +638,12 @@ int nova_wc_step_t(nova_wc *w, double dt, int nsteps, double *U, double *rho, do
 {
     NOVA_REQUIRE(w && U && rho && p && T && nsteps >= 0, "bad argument");
     NOVA_GUARD(wc_step_t(w, dt, nsteps, U, rho, p, T));
+}
+
+int nova_wc_hydrostatic(nova_wc *w, const double *T, double rho0, double *p, double *rho)
+{
+    NOVA_REQUIRE(w && T && p && rho, "bad argument");
+    NOVA_GUARD(wc_hydrostatic(w, T, rho0, p, rho));
 }
 
 int nova_wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p)

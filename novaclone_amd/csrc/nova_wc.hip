@@ -704,4 +704,43 @@ int wc_step_t(nova_wc *w, double dt, int nsteps, double *U, double *rho, double 
     return NOVA_OK;
 }
 
+// The initial hydrostatic pressure and density of app/shearLayerWC
+// (hydrostaticP.H:1-37, shearLayer.cpp:139-151), run once before the time
+// loop.  One thread per (i, j) column walks its contiguous k run (each
+// cache line serves 16 consecutive k).  The partial sum is formed from the
+// top in the reference's order, ((T dz) g_z) rho0 added to it, unfused.
+__global__ void __launch_bounds__(256) k_wc_hydrostatic(Grid g, const double *__restrict__ T, double *__restrict__ p,
+                                                        double *__restrict__ rho, double dz, double gz, double rho0,
+                                                        double pc)
+{
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= (int64_t)g.s[0] * g.s[1]) return;
+    const int i = (int)(col / g.s[1]);
+    const int64_t base = col * g.s[2];
+    // the reference's i loop stops one node short of its last node (the
+    // periodic image, absent here, or the wall node); its k sum starts one
+    // node below the last (the image, or the top wall node)
+    if (i < g.n[0]) {
+        double acc = 0.0;
+        for (int k = g.n[2] - 1; k >= 0; k--) {
+#pragma clang fp contract(off)
+            acc = acc + ((T[base + k] * dz) * gz) * rho0;
+            p[base + k] = p[base + k] + acc;
+        }
+    }
+    for (int k = 0; k < g.s[2]; k++) rho[base + k] = sqrt(p[base + k] / pc);
+}
+
+int wc_hydrostatic(nova_wc *w, const double *T, double rho0, double *p, double *rho)
+{
+    if (!w->ta) return fail(NOVA_ERR_ARG, "nova_wc_set_temperature first");
+    nova_ctx *ctx = w->ctx;
+    const Grid &g = ctx->g;
+    const int64_t cols = (int64_t)g.s[0] * g.s[1];
+    TimedScope ts(ctx, "wc_hydrostatic");
+    hipLaunchKernelGGL(k_wc_hydrostatic, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, ctx->stream, g, T, p, rho,
+                       ctx->d[2], w->grav[2], rho0, w->pc);
+    return check_launch("wc_hydrostatic");
+}
+
 }  // namespace nova

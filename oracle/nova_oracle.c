@@ -1013,6 +1013,29 @@ void orc_wcs_step(const orc_mesh *M, double *U, double *rho, double *p, double *
     wc_t_eqn(M, U, rho, T, ka, dt);
 }
 
+/* app/shearLayerWC/hydrostaticP.H:1-37 and shearLayer.cpp:141-151.  The
+ * column total of hydrostaticP.H:7-10 only feeds the other ranks' offset,
+ * which the reference leaves out of pHyd (:28), so it is not formed. */
+void orc_wc_hydrostatic(const orc_mesh *M, const double *T, double *p, double *rho, const double g[3],
+                        double rho0, double pc)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    const double dz = M->d[2];
+    #pragma omp parallel for
+    for (int i = h; i < ni - h - 1; i++)
+        for (int j = h; j < nj - h; j++) {
+            double acc = 0.0;
+            for (int k = nk - 2 - h; k >= h; k--) {
+                acc += T[IX(M, i, j, k)] * dz * g[2] * rho0;
+                p[IX(M, i, j, k)] += acc;
+            }
+        }
+    #pragma omp parallel for
+    for (int i = h; i < ni - h; i++)
+        for (int j = h; j < nj - h; j++)
+            for (int k = h; k < nk - h; k++) rho[IX(M, i, j, k)] = pow(p[IX(M, i, j, k)] / pc, 0.5);
+}
+
 /* ------------------------------------------------------------------ */
 /* Diagnostics. */
 

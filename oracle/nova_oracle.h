@@ -101,6 +101,13 @@ void orc_wc_step(const orc_mesh *, double *U, double *rho, double *p, const doub
  * pEqn, TEqn): U, rho, T in/out; p in (used by UEqn) and out. */
 void orc_wcs_step(const orc_mesh *, double *U, double *rho, double *p, double *T, double mu, double ka,
                   const double g[3], double pc, double dt);
+/* Initial hydrostatic pressure and density of app/shearLayerWC
+ * (hydrostaticP.H, shearLayer.cpp:139-151): per (i, j) column of
+ * i in [h, ni-h-1), j in [h, nj-h), from k = nk-2-h down to h,
+ * pHyd(k) = sum_{k' >= k} T dz g[2] rho0 accumulated in that order;
+ * p += pHyd; then rho = pow(p / pc, 0.5) on [h, n-h) in every direction. */
+void orc_wc_hydrostatic(const orc_mesh *, const double *T, double *p, double *rho, const double g[3],
+                        double rho0, double pc);
 /* Projection: F <- F - s*grad(lap^-1(div(F)/s)), p receives the potential. */
 void orc_project(const orc_mesh *, orc_poisson *, double *F, double *p, double s);
 

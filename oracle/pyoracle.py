@@ -59,6 +59,7 @@ def _load() -> C.CDLL:
         "orc_mhd_step": ([_P, _P, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_int], None),
         "orc_wc_step": ([_P, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_double], None),
         "orc_wcs_step": ([_P, _D, _D, _D, _D, C.c_double, C.c_double, _D, C.c_double, C.c_double], None),
+        "orc_wc_hydrostatic": ([_P, _D, _D, _D, _D, C.c_double, C.c_double], None),
         "orc_project": ([_P, _P, _D, _D, C.c_double], None),
         "orc_ke": ([_P, _D], C.c_double),
         "orc_maxabs_div": ([_P, _D], C.c_double),
@@ -207,7 +208,7 @@ class OMesh:
         lib().orc_mhd_diag(self._h, _d(U), _d(B), mu, eta, _d(out))
         return out
 
-    # compact 6th-order scheme
+    # weakly-compressible path
     def wc_step(self, U, rho, p, mu, pc, dt, B=None, dbu=0.0):
         """One weakly-compressible step in place (U: nc=3, rho and p: nc=1);
         B (nc=3, frozen) adds the Lorentz terms of app/OT/UEqn.H."""
@@ -217,6 +218,14 @@ class OMesh:
         """One app/shearLayerWC step in place (U nc=3; rho, p, T nc=1)."""
         gv = (C.c_double * 3)(*g)
         lib().orc_wcs_step(self._h, _d(U), _d(rho), _d(p), _d(T), mu, ka, gv, pc, dt)
+
+    def wc_hydrostatic(self, T, p, rho, g, rho0, pc):
+        """Initial hydrostatic pressure (added to p) and rho = (p/pc)^0.5 of
+        app/shearLayerWC, in place."""
+        gv = (C.c_double * 3)(*g)
+        lib().orc_wc_hydrostatic(self._h, _d(T), _d(p), _d(rho), gv, rho0, pc)
+
+    # compact 6th-order scheme
 
     def compact_deriv(self, f, direction, order=1):
         out = self.field(f.shape[-1])

@@ -190,6 +190,37 @@ def test_wc_shear_layer_with_temperature_matches_oracle(nv, orc, wall):
     assert rel_l2(host(Td), T) > 1e-8 and rel_l2(host(rd), rho) > 1e-12
 
 
+@pytest.mark.parametrize("wall", [False, True])
+def test_wc_hydrostatic_initial_state_matches_oracle(nv, orc, wall):
+    # app/shearLayerWC/hydrostaticP.H and shearLayer.cpp:141-151: p from
+    # pEqn.H at rho = rho0, plus the column sum of T dz g_z rho0 from the top;
+    # then rho = (p/pc)^(1/2).  The sum runs in the reference's order, so the
+    # pressure agrees to rounding of the last add at most.
+    rho0, c, h0, t0 = 999.57, 0.0417, 0.2, 0.1
+    pc = c * c / 2.0 / rho0
+    g = (0.0, 0.0, -0.08 * (0.2 * c) ** 2 / t0 / h0)
+    N, L = (16, 24, 20), (14 * h0, 15 * h0, 1.7 * h0)
+    bc = ("periodic", "wall" if wall else "periodic", "periodic")
+    G = nv.Mesh(N, L, bc=bc)
+    O = orc.OMesh(N, L, wall_y=wall)
+    x = [np.arange(O.stored[a]) * (O.L[a] / O.N[a]) for a in range(3)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    rng = np.random.default_rng(17)
+    T = (t0 / 2 * np.tanh(2 * (Z - 0.85 * h0) / h0) + 0.005 * rng.uniform(-1, 1, O.stored))[None]
+    p = np.full((1, *O.stored), (pc * rho0) * rho0)
+    W = nv.WC(G, 1e-3, pc)
+    W.set_temperature(1e-4, g)
+    Td, pd, rd = dev(T), dev(p), dev(np.zeros_like(p))
+    W.hydrostatic(Td, rho0, pd, rd)
+    To, po, ro = O.from_compact(T), O.from_compact(p), O.field(1)
+    O.wc_hydrostatic(To, po, ro, g, rho0, pc)
+    ep, er = rel_l2(host(pd), O.interior(po)), rel_l2(host(rd), O.interior(ro))
+    print("wall", wall, "rel L2 p", ep, "rho", er, flush=True)
+    assert ep < 1e-15 and er < 1e-15
+    # the column sums moved p well beyond rounding, and rho follows
+    assert rel_l2(host(pd), p) > 1e-6 and rel_l2(host(rd), np.full_like(p, rho0)) > 1e-6
+
+
 def test_wc_rejects_bad_arguments(nv):
     G = nv.Mesh((16, 16, 16), (TWO_PI,) * 3)
     with pytest.raises(nv.NovaError):
@@ -202,5 +233,7 @@ def test_wc_rejects_bad_arguments(nv):
         W.step(0.01, 1, U, r, None)
     with pytest.raises(nv.NovaError):  # the temperature equation is not set up
         W.step_t(0.01, 1, U, r, r.clone(), r.clone())
+    with pytest.raises(nv.NovaError):  # needs the temperature equation's g
+        W.hydrostatic(r, 1.0, r.clone(), r.clone())
     with pytest.raises(nv.NovaError):
         W.set_temperature(-1.0, (0.0, 0.0, -1.0))

@@ -229,3 +229,38 @@ def test_buoyancy_drives_the_flow_along_minus_g(orc):
     assert np.abs(U[s][..., 2] - want[s]).max() < 1e-15
     assert np.abs(U[s][..., :2]).max() == 0.0
     assert np.array_equal(T[..., 0][s], T0[s]) and np.all(rho[..., 0][s] == 1.0)
+
+
+# app/shearLayerWC/hydrostaticP.H and the initial density (shearLayer.cpp:139-151)
+
+
+@pytest.mark.parametrize("wall", [False, True])
+def test_hydrostatic_pressure_is_the_column_sum_from_the_top(orc, wall):
+    # T = a + b z: node k of a column gets sum_{k'=k}^{N-1} (a + b k' dz) dz g rho0
+    # = ((N - k) a + b dz (N(N-1)/2 - k(k-1)/2)) dz g rho0; the node k = N
+    # (the top node) and the columns of the last i node get nothing
+    N = (12, 10, 16)
+    M = orc.OMesh(N, (2.0, 3.0, 1.6), wall_y=wall)
+    h = M.h
+    dz = 1.6 / N[2]
+    a, b, g2, rho0, pc = 0.3, -0.7, -2.5, 999.57, 8.7e-7
+    kk = np.arange(M.n[2]) - h
+    T = M.field(1)
+    T[..., 0] = (a + b * kk * dz)[None, None, :]
+    p0 = M.field(1)
+    p0[...] = 0.8 + 0.01 * np.arange(M.n[1])[None, :, None, None]
+    p, rho = p0.copy(), M.field(1)
+    M.wc_hydrostatic(T, p, rho, (7.0, 11.0, g2), rho0, pc)
+    k = np.arange(N[2] + 1)
+    s = np.where(k < N[2], (N[2] - k) * a + b * dz * (N[2] * (N[2] - 1) / 2 - k * (k - 1) / 2), 0.0)
+    want = s * dz * g2 * rho0
+    dp = (p - p0)[..., 0]
+    I, J, K = slice(h, h + N[0]), slice(h, h + N[1] + 1), slice(h, h + N[2] + 1)
+    assert np.abs(dp[I, J, K] - want[None, None, :]).max() < 1e-12 * np.abs(want).max()
+    assert np.all(dp[h + N[0], J, K] == 0.0)          # i = N (periodic image) excluded
+    assert np.all(dp[:h] == 0.0) and np.all(dp[:, :, :h] == 0.0)  # ghosts untouched
+    # the density follows p: rho^2 pc = p on every interior node
+    R = rho[..., 0][h:h + N[0] + 1, J, K]
+    P = p[..., 0][h:h + N[0] + 1, J, K]
+    assert np.abs(R * R * pc / P - 1.0).max() < 1e-15
+    assert np.all(rho[..., 0][:h] == 0.0)
