@@ -41,7 +41,8 @@ enum {
     NOVA_ERR_HIP = -2,
     NOVA_ERR_UNSUPPORTED = -3,
     NOVA_ERR_STATE = -4,
-    NOVA_ERR_NOMEM = -5
+    NOVA_ERR_NOMEM = -5,
+    NOVA_ERR_IO = -6
 };
 
 enum { NOVA_BC_PERIODIC = 0, NOVA_BC_WALL = 1 };
@@ -79,6 +80,8 @@ typedef struct nova_solver nova_solver;
 typedef struct nova_adjoint nova_adjoint;
 typedef struct nova_ensemble nova_ensemble;
 typedef struct nova_wc nova_wc;
+typedef struct nova_time nova_time;
+typedef struct nova_io nova_io;
 
 /* ---- context / memory (replaces Mesh construction, Mesh.cpp:24-179, and
  * Field storage, src/Field/Field.hxx:21-134) ---- */
@@ -275,6 +278,70 @@ int nova_wc_step_t(nova_wc *w, double dt, int nsteps, double *U, double *rho, do
  * loop bounds leave them.  Then rho = (p / pc)^(1/2) on every node.  T in,
  * p in/out, rho out; device pointers, stored layout. */
 int nova_wc_hydrostatic(nova_wc *w, const double *T, double rho0, double *p, double *rho);
+
+/* ---- output and restart (src/Time/Time.cpp, src/Mesh/Mesh.cpp:224-369,
+ * src/Field/Field.hxx:403-522) ---- */
+/* The run clock.  Time(dt, endTime, writeEvery) (Time.cpp:23-62): output
+ * every write_every steps; Time(dt, endTime, writeInterval) (:64-74): output
+ * when the time reaches the next multiple of the interval; Time(file)
+ * (:77-109): the "dt:", "writeT:", "writeN:", "end:" lines of a settings
+ * file.  nova_time_restart is the restart branch of :40-61: it reads the
+ * output count and times of a restart file (Mesh::writeRestartInfo's
+ * format), resumes the clock at the last time and numbers files on from
+ * the count. */
+int nova_time_create(double dt, double end_time, int write_every, nova_time **out);
+int nova_time_create_interval(double dt, double end_time, double write_interval, nova_time **out);
+int nova_time_create_from_file(const char *file, nova_time **out);
+int nova_time_restart(nova_time *t, const char *restart_file);
+int nova_time_destroy(nova_time *t);
+/* Time::operator++ (Time.cpp:112-139): decide whether this step writes (and
+ * if so count the file and record the current time), then advance the time
+ * by dt and the step count by one.  A settings file without writeN/writeT
+ * leaves no output rule: advancing it is an error. */
+int nova_time_advance(nova_time *t);
+int nova_time_set_dt(nova_time *t, double dt);  /* Time::dt() is writable */
+/* dout: dt, current time, end time, write interval; iout: time step, file
+ * number, write (0/1), log (0/1), number of output times.  run() is
+ * current time < end time. */
+int nova_time_info(const nova_time *t, double dout[4], int64_t iout[5]);
+int nova_time_steps(const nova_time *t, double *out, int64_t cap);
+
+/* Output of one grid's fields.  Writes are asynchronous: the layout change
+ * runs on the context's stream (so the field may be overwritten by work
+ * queued after the call), the copy to host and the file write run behind
+ * it; nova_io_wait blocks until every queued file is written and returns
+ * the first error.  One rank: the processor number in the file names is
+ * 0000. */
+int nova_io_create(nova_ctx *ctx, nova_io **out);
+int nova_io_destroy(nova_io *io);  /* waits for queued writes first */
+int nova_io_wait(nova_io *io);
+/* Field::write (Field.hxx:403-451): nothing unless the clock writes this
+ * step; else path/name.0000.FFFFF.dat (FFFFF = the clock's file number):
+ * 80-byte "description", "part", int 1, "point", then for each component
+ * the N_x+1 x N_y+1 x N_z+1 nodes (the periodic image node N is node 0) as
+ * float32, i fastest, then j, then k.  The reference's 80-byte strings
+ * carry undefined bytes after their NUL; these are NUL-padded. */
+int nova_io_write_field(nova_io *io, const nova_time *t, const char *path, const char *name, const double *F,
+                        int ncomp);
+/* Mesh::write (Mesh.cpp:224-303, geom.p0000.geo: EnSight Gold binary
+ * "block rectilinear", node coordinates as float32, z through the
+ * stretching), writeCase (:306-347, fd.p0000.case) and writeRestartInfo
+ * (:349-365, restart.p.0000.case), when the clock writes this step. */
+int nova_io_write_mesh(nova_io *io, const nova_time *t, const char *path);
+/* Field::read (Field.hxx:455-522): path/name.0000.FFFFF.dat, the header
+ * skipped, float32 nodes into the stored FP64 field (image entries
+ * ignored).  Synchronous; waits for queued writes first. */
+int nova_io_read_field(nova_io *io, const char *path, const char *name, int file_num, double *F, int ncomp);
+/* FP64 checkpoint (no reference counterpart: the reference restarts from
+ * its float32 output): a 64-byte header ("NOVAFP64", version 1, N, wall
+ * flags, m, field count, time, step), each field's component count, then
+ * the fields in the stored layout, bit for bit.  Writing is asynchronous
+ * like nova_io_write_field; reading checks the header against the grid and
+ * the given component counts. */
+int nova_io_checkpoint_write(nova_io *io, const char *file, int nfields, const double *const *fields,
+                             const int *ncomp, double time, int64_t step);
+int nova_io_checkpoint_read(nova_io *io, const char *file, int nfields, double *const *fields, const int *ncomp,
+                            double *time, int64_t *step);
 
 /* ---- ensembles of uneven instances (no reference counterpart: NOVA runs
  * one case per process; the step of each instance is that of

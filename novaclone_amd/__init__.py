@@ -109,6 +109,24 @@ _SIGS = {
     "nova_wc_set_temperature": [_P, _D, _D, _D, _D],
     "nova_wc_step_t": [_P, _D, _I, _P, _P, _P, _P],
     "nova_wc_hydrostatic": [_P, _P, _D, _P, _P],
+    "nova_time_create": [_D, _D, _I, C.POINTER(_P)],
+    "nova_time_create_interval": [_D, _D, _D, C.POINTER(_P)],
+    "nova_time_create_from_file": [C.c_char_p, C.POINTER(_P)],
+    "nova_time_restart": [_P, C.c_char_p],
+    "nova_time_destroy": [_P],
+    "nova_time_advance": [_P],
+    "nova_time_set_dt": [_P, _D],
+    "nova_time_info": [_P, _DP, C.POINTER(C.c_int64)],
+    "nova_time_steps": [_P, _DP, C.c_int64],
+    "nova_io_create": [_P, C.POINTER(_P)],
+    "nova_io_destroy": [_P],
+    "nova_io_wait": [_P],
+    "nova_io_write_field": [_P, _P, C.c_char_p, C.c_char_p, _P, _I],
+    "nova_io_write_mesh": [_P, _P, C.c_char_p],
+    "nova_io_read_field": [_P, C.c_char_p, C.c_char_p, _I, _P, _I],
+    "nova_io_checkpoint_write": [_P, C.c_char_p, _I, C.POINTER(_P), C.POINTER(C.c_int), _D, C.c_int64],
+    "nova_io_checkpoint_read": [_P, C.c_char_p, _I, C.POINTER(_P), C.POINTER(C.c_int), _DP,
+                                C.POINTER(C.c_int64)],
     "nova_ensemble_create": [C.POINTER(InstanceDesc), _I, _I, C.POINTER(_P)],
     "nova_ensemble_destroy": [_P],
     "nova_ensemble_set_stream": [_P, _P],
@@ -399,6 +417,138 @@ class Solver:
         if getattr(self, "_h", None):
             lib().nova_solver_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Time:
+    """The run clock (src/Time/Time.cpp): ``Time(dt, end, writeEvery)`` with
+    an int writes every writeEvery steps, with a float at multiples of that
+    time interval; ``Time.from_file(settings)`` reads "dt:", "writeT:",
+    "writeN:", "end:"; ``restart(file)`` resumes from a restart file.
+    ``advance()`` is ``operator++``."""
+
+    def __init__(self, dt: float, end_time: float, write_every, _handle=None):
+        h = C.c_void_p()
+        if _handle is not None:
+            h = _handle
+        elif isinstance(write_every, float):
+            _check(lib().nova_time_create_interval(float(dt), float(end_time), write_every, C.byref(h)),
+                   "nova_time_create_interval")
+        else:
+            _check(lib().nova_time_create(float(dt), float(end_time), int(write_every), C.byref(h)),
+                   "nova_time_create")
+        self._h = h
+
+    @classmethod
+    def from_file(cls, path) -> "Time":
+        h = C.c_void_p()
+        _check(lib().nova_time_create_from_file(str(path).encode(), C.byref(h)), "nova_time_create_from_file")
+        return cls(0.0, 0.0, 0, _handle=h)
+
+    def restart(self, restart_file) -> None:
+        _check(lib().nova_time_restart(self._h, str(restart_file).encode()), "nova_time_restart")
+
+    def advance(self) -> None:
+        _check(lib().nova_time_advance(self._h), "nova_time_advance")
+
+    def _info(self):
+        d = (C.c_double * 4)()
+        i = (C.c_int64 * 5)()
+        _check(lib().nova_time_info(self._h, d, i), "nova_time_info")
+        return list(d), list(i)
+
+    def dt(self) -> float:
+        return self._info()[0][0]
+
+    def set_dt(self, dt: float) -> None:
+        _check(lib().nova_time_set_dt(self._h, float(dt)), "nova_time_set_dt")
+
+    def curTime(self) -> float:
+        return self._info()[0][1]
+
+    def run(self) -> bool:
+        d, _ = self._info()
+        return d[1] < d[2]
+
+    def write(self) -> bool:
+        return bool(self._info()[1][2])
+
+    def writelog(self) -> bool:
+        return bool(self._info()[1][3])
+
+    def timeStep(self) -> int:
+        return self._info()[1][0]
+
+    def fileNum(self) -> int:
+        return self._info()[1][1]
+
+    def steps(self) -> list:
+        n = self._info()[1][4]
+        out = (C.c_double * max(n, 1))()
+        _check(lib().nova_time_steps(self._h, out, n), "nova_time_steps")
+        return list(out)[:n]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nova_time_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class IO:
+    """EnSight Gold output and restart of one mesh's fields
+    (Mesh::write, Field::write/read) and FP64 checkpoints.  Writes are
+    asynchronous; ``wait()`` blocks until the files are written."""
+
+    def __init__(self, mesh: "Mesh"):
+        self.mesh = mesh
+        h = C.c_void_p()
+        _check(lib().nova_io_create(mesh.handle, C.byref(h)), "nova_io_create")
+        self._h = h
+
+    def write_mesh(self, time: Time, path) -> None:
+        _check(lib().nova_io_write_mesh(self._h, time._h, str(path).encode()), "nova_io_write_mesh")
+
+    def write_field(self, time: Time, path, name: str, F) -> None:
+        _check(lib().nova_io_write_field(self._h, time._h, str(path).encode(), name.encode(), _ptr(F), F.shape[0]),
+               "nova_io_write_field")
+
+    def read_field(self, path, name: str, file_num: int, F) -> None:
+        _check(lib().nova_io_read_field(self._h, str(path).encode(), name.encode(), int(file_num), _ptr(F),
+                                        F.shape[0]), "nova_io_read_field")
+
+    def checkpoint_write(self, file, fields, time: float = 0.0, step: int = 0) -> None:
+        ptrs = (C.c_void_p * len(fields))(*[_ptr(f).value for f in fields])
+        nc = (C.c_int * len(fields))(*[f.shape[0] for f in fields])
+        _check(lib().nova_io_checkpoint_write(self._h, str(file).encode(), len(fields), ptrs, nc, float(time),
+                                              int(step)), "nova_io_checkpoint_write")
+
+    def checkpoint_read(self, file, fields):
+        ptrs = (C.c_void_p * len(fields))(*[_ptr(f).value for f in fields])
+        nc = (C.c_int * len(fields))(*[f.shape[0] for f in fields])
+        t = C.c_double()
+        st = C.c_int64()
+        _check(lib().nova_io_checkpoint_read(self._h, str(file).encode(), len(fields), ptrs, nc, C.byref(t),
+                                             C.byref(st)), "nova_io_checkpoint_read")
+        return t.value, st.value
+
+    def wait(self) -> None:
+        _check(lib().nova_io_wait(self._h), "nova_io_wait")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            h, self._h = self._h, None
+            _check(lib().nova_io_destroy(h), "nova_io_destroy")
 
     def __del__(self):
         try:

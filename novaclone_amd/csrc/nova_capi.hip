@@ -122,6 +122,24 @@ int wc_set_magnetic(nova_wc *w, const double *B, double dbu);
 int wc_set_temperature(nova_wc *w, double ka, const double g[3]);
 int wc_step_t(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p, double *T);
 int wc_hydrostatic(nova_wc *w, const double *T, double rho0, double *p, double *rho);
+int time_create(double dt, double end, int write_every, double interval, bool timebased, nova_time **out);
+int time_create_from_file(const char *file, nova_time **out);
+int time_restart(nova_time *t, const char *file);
+int time_advance(nova_time *t);
+void time_destroy(nova_time *t);
+void time_set_dt(nova_time *t, double dt);
+void time_info(const nova_time *t, double dout[4], int64_t iout[5]);
+void time_steps(const nova_time *t, double *out, int64_t cap);
+int io_create(nova_ctx *ctx, nova_io **out);
+int io_destroy(nova_io *io);
+int io_wait(nova_io *io);
+int io_write_field(nova_io *io, const nova_time *t, const char *path, const char *name, const double *F, int nc);
+int io_write_mesh(nova_io *io, const nova_time *t, const char *path);
+int io_read_field(nova_io *io, const char *path, const char *name, int file_num, double *F, int nc);
+int io_checkpoint_write(nova_io *io, const char *file, int nfields, const double *const *F, const int *nc,
+                        double time, int64_t step);
+int io_checkpoint_read(nova_io *io, const char *file, int nfields, double *const *F, const int *nc, double *time,
+                       int64_t *step);
 
 // Mesh::z (src/Mesh/Mesh.cpp:373-407) for padded index kp on one rank.
 static double mesh_z(const nova_grid_desc &d, double dz, double hb, int m, int kp)
@@ -136,6 +154,9 @@ static double mesh_z(const nova_grid_desc &d, double dz, double hb, int m, int k
     }
     return (kp - m / 2) * dz + d.origin[2];
 }
+
+// z of padded index kp (Mesh::z) for the output coordinates.
+double mesh_z_of(const nova_ctx *ctx, int kp) { return mesh_z(ctx->desc, ctx->d[2], ctx->hb, ctx->m, kp); }
 
 static int ctx_init(nova_ctx *ctx, const nova_grid_desc *desc, int device)
 {
@@ -188,6 +209,7 @@ static int ctx_init(nova_ctx *ctx, const nova_grid_desc *desc, int device)
         const int globN = d.n[2] + 1;
         for (int it = 1; it < 10000; it++) hb = std::asinh(hb / ((globN - 1) * std::pow(d.s1 * d.s2, 0.5)));
     }
+    ctx->hb = hb;
     const double dz = ctx->d[2];
     const int s2 = g.s[2];
     ctx->dzdxi.assign(s2, 0.0);
@@ -644,6 +666,116 @@ int nova_wc_hydrostatic(nova_wc *w, const double *T, double rho0, double *p, dou
 {
     NOVA_REQUIRE(w && T && p && rho, "bad argument");
     NOVA_GUARD(wc_hydrostatic(w, T, rho0, p, rho));
+}
+
+int nova_time_create(double dt, double end_time, int write_every, nova_time **out)
+{
+    NOVA_REQUIRE(out && write_every > 0, "bad argument");
+    NOVA_GUARD(time_create(dt, end_time, write_every, 0.0, false, out));
+}
+
+int nova_time_create_interval(double dt, double end_time, double write_interval, nova_time **out)
+{
+    NOVA_REQUIRE(out && write_interval > 0.0, "bad argument");
+    NOVA_GUARD(time_create(dt, end_time, 0, write_interval, true, out));
+}
+
+int nova_time_create_from_file(const char *file, nova_time **out)
+{
+    NOVA_REQUIRE(file && out, "bad argument");
+    NOVA_GUARD(time_create_from_file(file, out));
+}
+
+int nova_time_restart(nova_time *t, const char *restart_file)
+{
+    NOVA_REQUIRE(t && restart_file, "bad argument");
+    NOVA_GUARD(time_restart(t, restart_file));
+}
+
+int nova_time_destroy(nova_time *t)
+{
+    time_destroy(t);
+    return NOVA_OK;
+}
+
+int nova_time_advance(nova_time *t)
+{
+    NOVA_REQUIRE(t, "bad argument");
+    NOVA_GUARD(time_advance(t));
+}
+
+int nova_time_set_dt(nova_time *t, double dt)
+{
+    NOVA_REQUIRE(t, "bad argument");
+    time_set_dt(t, dt);
+    return NOVA_OK;
+}
+
+int nova_time_info(const nova_time *t, double dout[4], int64_t iout[5])
+{
+    NOVA_REQUIRE(t && dout && iout, "bad argument");
+    time_info(t, dout, iout);
+    return NOVA_OK;
+}
+
+int nova_time_steps(const nova_time *t, double *out, int64_t cap)
+{
+    NOVA_REQUIRE(t && (out || cap == 0) && cap >= 0, "bad argument");
+    time_steps(t, out, cap);
+    return NOVA_OK;
+}
+
+int nova_io_create(nova_ctx *ctx, nova_io **out)
+{
+    NOVA_REQUIRE(ctx && out, "bad argument");
+    NOVA_GUARD(io_create(ctx, out));
+}
+
+int nova_io_destroy(nova_io *io)
+{
+    if (!io) return NOVA_OK;
+    NOVA_GUARD(io_destroy(io));
+}
+
+int nova_io_wait(nova_io *io)
+{
+    NOVA_REQUIRE(io, "bad argument");
+    NOVA_GUARD(io_wait(io));
+}
+
+int nova_io_write_field(nova_io *io, const nova_time *t, const char *path, const char *name, const double *F,
+                        int ncomp)
+{
+    NOVA_REQUIRE(io && t && path && name && F && (ncomp == 1 || ncomp == 3), "bad argument");
+    NOVA_GUARD(io_write_field(io, t, path, name, F, ncomp));
+}
+
+int nova_io_write_mesh(nova_io *io, const nova_time *t, const char *path)
+{
+    NOVA_REQUIRE(io && t && path, "bad argument");
+    NOVA_GUARD(io_write_mesh(io, t, path));
+}
+
+int nova_io_read_field(nova_io *io, const char *path, const char *name, int file_num, double *F, int ncomp)
+{
+    NOVA_REQUIRE(io && path && name && F && file_num >= 0 && (ncomp == 1 || ncomp == 3), "bad argument");
+    NOVA_GUARD(io_read_field(io, path, name, file_num, F, ncomp));
+}
+
+int nova_io_checkpoint_write(nova_io *io, const char *file, int nfields, const double *const *fields,
+                             const int *ncomp, double time, int64_t step)
+{
+    NOVA_REQUIRE(io && file && nfields > 0 && fields && ncomp, "bad argument");
+    for (int f = 0; f < nfields; f++) NOVA_REQUIRE(fields[f] && ncomp[f] > 0, "bad field");
+    NOVA_GUARD(io_checkpoint_write(io, file, nfields, fields, ncomp, time, step));
+}
+
+int nova_io_checkpoint_read(nova_io *io, const char *file, int nfields, double *const *fields, const int *ncomp,
+                            double *time, int64_t *step)
+{
+    NOVA_REQUIRE(io && file && nfields > 0 && fields && ncomp, "bad argument");
+    for (int f = 0; f < nfields; f++) NOVA_REQUIRE(fields[f] && ncomp[f] > 0, "bad field");
+    NOVA_GUARD(io_checkpoint_read(io, file, nfields, fields, ncomp, time, step));
 }
 
 int nova_wc_step(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p)

@@ -96,6 +96,7 @@ struct nova_ctx {
     int m = 6;
     int pad[3] = {0, 0, 0};          // reference padded extents n_d = N_d + 1 + m
     double d[3] = {0, 0, 0};         // spacings
+    double hb = 1.0;                 // hyperbolic stretching constant (Mesh.cpp:57-60)
     double coef[4] = {0, 0, 0, 0};
     std::vector<double> dzdxi, d2zdxi2;  // stored-k metric
     double *d_metric = nullptr;      // zinv | zlap | zmet on the device

@@ -16,6 +16,7 @@
 
 #include <complex.h>
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -1034,6 +1035,142 @@ void orc_wc_hydrostatic(const orc_mesh *M, const double *T, double *p, double *r
     for (int i = h; i < ni - h; i++)
         for (int j = h; j < nj - h; j++)
             for (int k = h; k < nk - h; k++) rho[IX(M, i, j, k)] = pow(p[IX(M, i, j, k)] / pc, 0.5);
+}
+
+/* ------------------------------------------------------------------ */
+/* EnSight Gold output (src/Field/Field.hxx:403-522, src/Mesh/Mesh.cpp:224-365). */
+
+/* An 80-byte string as the reference's buf.str().resize(80) writes it:
+ * the text and its NUL are defined there, the rest is unspecified; zeros
+ * here. */
+static int put80(FILE *f, const char *txt)
+{
+    char b[80];
+    memset(b, 0, sizeof(b));
+    strncpy(b, txt, 79);
+    return fwrite(b, 1, 80, f) == 80 ? 0 : -1;
+}
+
+/* Field.hxx:403-451: path/name.%04d.%05d.dat (processor, file number);
+ * description, part, int 1, point; then every component's interior nodes
+ * [h, n-h) as float, i fastest, then j, then k. */
+int orc_field_write(const orc_mesh *M, const double *f, int nc, const char *path, const char *name, int procNo,
+                    int fileNum)
+{
+    char fn[4096];
+    snprintf(fn, sizeof(fn), "%s/%s.%04d.%05d.dat", path, name, procNo, fileNum);
+    FILE *fo = fopen(fn, "wb");
+    if (!fo) return -1;
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    int partNum = 1, rc = 0;
+    rc |= put80(fo, "description");
+    rc |= put80(fo, "part");
+    rc |= fwrite(&partNum, sizeof(int), 1, fo) == 1 ? 0 : -1;
+    rc |= put80(fo, "point");
+    for (int c = 0; c < nc; c++)
+        for (int k = h; k < nk - h; k++)
+            for (int j = h; j < nj - h; j++)
+                for (int i = h; i < ni - h; i++) {
+                    float v = (float)f[IX(M, i, j, k) * nc + c];
+                    rc |= fwrite(&v, sizeof(float), 1, fo) == 1 ? 0 : -1;
+                }
+    if (fclose(fo) != 0) rc = -1;
+    return rc;
+}
+
+/* Field.hxx:455-491: skip 240 + sizeof(int) bytes, read the same nodes. */
+int orc_field_read(const orc_mesh *M, double *f, int nc, const char *path, const char *name, int procNo,
+                   int fileNum)
+{
+    char fn[4096];
+    snprintf(fn, sizeof(fn), "%s/%s.%04d.%05d.dat", path, name, procNo, fileNum);
+    FILE *fi = fopen(fn, "rb");
+    if (!fi) return -1;
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    int rc = fseek(fi, 240 + sizeof(int), SEEK_SET) == 0 ? 0 : -1;
+    for (int c = 0; c < nc && !rc; c++)
+        for (int k = h; k < nk - h && !rc; k++)
+            for (int j = h; j < nj - h && !rc; j++)
+                for (int i = h; i < ni - h; i++) {
+                    float v;
+                    if (fread(&v, sizeof(float), 1, fi) != 1) {
+                        rc = -1;
+                        break;
+                    }
+                    f[IX(M, i, j, k) * nc + c] = (double)v;
+                }
+    fclose(fi);
+    return rc;
+}
+
+/* Mesh::write (Mesh.cpp:224-303), writeCase (:306-347), writeRestartInfo
+ * (:349-365).  The output times print as the reference's ofstream prints a
+ * double by default, i.e. as printf's %g. */
+int orc_mesh_write(const orc_mesh *M, const char *path, int procNo, const double *steps, int nsteps)
+{
+    char fn[4096];
+    snprintf(fn, sizeof(fn), "%s/geom.p%04d.geo", path, procNo);
+    FILE *fo = fopen(fn, "wb");
+    if (!fo) return -1;
+    int rc = 0, partNum = 1;
+    rc |= put80(fo, "C Binary");
+    rc |= put80(fo, "desc1");
+    rc |= put80(fo, "desc2");
+    rc |= put80(fo, "node id assign");
+    rc |= put80(fo, "element id assign");
+    rc |= put80(fo, "part");
+    rc |= fwrite(&partNum, sizeof(int), 1, fo) == 1 ? 0 : -1;
+    rc |= put80(fo, "part desc");
+    rc |= put80(fo, "block rectilinear");
+    int np[3];
+    for (int a = 0; a < 3; a++) np[a] = M->n[a] - M->m;
+    rc |= fwrite(np, sizeof(int), 3, fo) == 3 ? 0 : -1;
+    for (int i = 0; i < np[0]; i++) {
+        float x = M->o[0] + M->d[0] * i;
+        rc |= fwrite(&x, sizeof(float), 1, fo) == 1 ? 0 : -1;
+    }
+    for (int j = 0; j < np[1]; j++) {
+        float y = M->o[1] + M->d[1] * j;
+        rc |= fwrite(&y, sizeof(float), 1, fo) == 1 ? 0 : -1;
+    }
+    for (int k = 0; k < np[2]; k++) {
+        float z = orc_mesh_z(M, k + M->m / 2);
+        rc |= fwrite(&z, sizeof(float), 1, fo) == 1 ? 0 : -1;
+    }
+    if (fclose(fo) != 0) rc = -1;
+
+    snprintf(fn, sizeof(fn), "%s/fd.p%04d.case", path, procNo);
+    fo = fopen(fn, "w");
+    if (!fo) return -1;
+    fprintf(fo, "FORMAT\n");
+    fprintf(fo, "type: ensight gold\n");
+    fprintf(fo, "GEOMETRY\n");
+    fprintf(fo, "model: geom.p%04d.geo\n", procNo);
+    fprintf(fo, "VARIABLE\n");
+    fprintf(fo, "vector per node: velocity U.%04d.*****.dat\n", procNo);
+    fprintf(fo, "vector per node: Uprime Uprime.%04d.*****.dat\n", procNo);
+    fprintf(fo, "vector per node: Udash Udash.%04d.*****.dat\n", procNo);
+    fprintf(fo, "scalar per node: p  p.%04d.*****.dat\n", procNo);
+    fprintf(fo, "scalar per node: T  T.%04d.*****.dat\n", procNo);
+    fprintf(fo, "vector per node: B B.%04d.*****.dat\n", procNo);
+    fprintf(fo, "vector per node: J J.%04d.*****.dat\n", procNo);
+    fprintf(fo, "TIME\n");
+    fprintf(fo, "time set: 1\n");
+    fprintf(fo, "number of steps: %d\n", nsteps);
+    fprintf(fo, "filename start number: 00001\n");
+    fprintf(fo, "filename increment: 1\n");
+    fprintf(fo, "time values: ");
+    for (int i = 0; i < nsteps; i++) fprintf(fo, "%g\n", steps[i]);
+    fprintf(fo, "\n");
+    if (fclose(fo) != 0) rc = -1;
+
+    snprintf(fn, sizeof(fn), "%s/restart.p.%04d.case", path, procNo);
+    fo = fopen(fn, "w");
+    if (!fo) return -1;
+    fprintf(fo, "%d\n", nsteps);
+    for (int i = 0; i < nsteps; i++) fprintf(fo, "%g\n", steps[i]);
+    if (fclose(fo) != 0) rc = -1;
+    return rc;
 }
 
 /* ------------------------------------------------------------------ */

@@ -108,6 +108,16 @@ void orc_wcs_step(const orc_mesh *, double *U, double *rho, double *p, double *T
  * p += pHyd; then rho = pow(p / pc, 0.5) on [h, n-h) in every direction. */
 void orc_wc_hydrostatic(const orc_mesh *, const double *T, double *p, double *rho, const double g[3],
                         double rho0, double pc);
+/* EnSight Gold output of the reference's Field::write / Field::read
+ * (src/Field/Field.hxx:403-491) and Mesh::write + writeCase +
+ * writeRestartInfo (src/Mesh/Mesh.cpp:224-365) for one rank (procNo) and
+ * the given output times.  f is an nc-component padded field.  Return 0 or
+ * -1 (file error). */
+int orc_field_write(const orc_mesh *, const double *f, int nc, const char *path, const char *name, int procNo,
+                    int fileNum);
+int orc_field_read(const orc_mesh *, double *f, int nc, const char *path, const char *name, int procNo,
+                   int fileNum);
+int orc_mesh_write(const orc_mesh *, const char *path, int procNo, const double *steps, int nsteps);
 /* Projection: F <- F - s*grad(lap^-1(div(F)/s)), p receives the potential. */
 void orc_project(const orc_mesh *, orc_poisson *, double *F, double *p, double s);
 

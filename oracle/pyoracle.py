@@ -60,6 +60,9 @@ def _load() -> C.CDLL:
         "orc_wc_step": ([_P, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_double], None),
         "orc_wcs_step": ([_P, _D, _D, _D, _D, C.c_double, C.c_double, _D, C.c_double, C.c_double], None),
         "orc_wc_hydrostatic": ([_P, _D, _D, _D, _D, C.c_double, C.c_double], None),
+        "orc_field_write": ([_P, _D, C.c_int, C.c_char_p, C.c_char_p, C.c_int, C.c_int], C.c_int),
+        "orc_field_read": ([_P, _D, C.c_int, C.c_char_p, C.c_char_p, C.c_int, C.c_int], C.c_int),
+        "orc_mesh_write": ([_P, C.c_char_p, C.c_int, _D, C.c_int], C.c_int),
         "orc_project": ([_P, _P, _D, _D, C.c_double], None),
         "orc_ke": ([_P, _D], C.c_double),
         "orc_maxabs_div": ([_P, _D], C.c_double),
@@ -225,6 +228,26 @@ class OMesh:
         gv = (C.c_double * 3)(*g)
         lib().orc_wc_hydrostatic(self._h, _d(T), _d(p), _d(rho), gv, rho0, pc)
 
+    # EnSight Gold output
+    def field_write(self, f, path, name, file_num, proc=0):
+        """Field::write of padded field f (shape (*n, nc))."""
+        rc = lib().orc_field_write(self._h, _d(f), f.shape[-1], str(path).encode(), name.encode(), proc, file_num)
+        assert rc == 0, "orc_field_write"
+
+    def field_read(self, f, path, name, file_num, proc=0):
+        rc = lib().orc_field_read(self._h, _d(f), f.shape[-1], str(path).encode(), name.encode(), proc, file_num)
+        assert rc == 0, "orc_field_read"
+
+    def mesh_write(self, path, steps, proc=0):
+        st = np.ascontiguousarray(steps, dtype=np.float64)
+        if st.size == 0:
+            st = np.zeros(1)
+            n = 0
+        else:
+            n = len(steps)
+        rc = lib().orc_mesh_write(self._h, str(path).encode(), proc, _d(st), n)
+        assert rc == 0, "orc_mesh_write"
+
     # compact 6th-order scheme
 
     def compact_deriv(self, f, direction, order=1):
@@ -335,3 +358,59 @@ def compact_kmod(kh, order=1):
 
 def num_threads() -> int:
     return lib().orc_num_threads()
+
+
+class OTime:
+    """The run clock of src/Time/Time.cpp, restated: Time(dt, end, writeEvery)
+    (:23-38), with a float the interval form (:64-74), from_file (:77-109),
+    restart (:40-61) and advance = operator++ (:112-139)."""
+
+    def __init__(self, dt, end, write_every):
+        self.steps = []
+        self.dt, self.cur, self.end = dt, 0.0, end
+        self.time_step, self.file_num = 0, 0
+        self.write, self.log = False, True
+        self.timebased = isinstance(write_every, float)
+        self.write_every = 0 if self.timebased else write_every
+        self.interval = write_every if self.timebased else 0.0
+
+    @classmethod
+    def from_file(cls, path):
+        t = cls(0.0, 0.0, 0)
+        for line in open(path):
+            w = line.split()
+            if not w:
+                continue
+            if w[0] == "dt:":
+                t.dt = float(w[1])
+            elif w[0] == "writeT:":
+                t.timebased, t.interval = True, float(w[1])
+            elif w[0] == "writeN:":
+                t.timebased, t.write_every = False, int(w[1])
+            elif w[0] == "end:":
+                t.end = float(w[1])
+        return t
+
+    def restart(self, path):
+        w = open(path).read().split()
+        n = int(w[0])
+        vals = [float(x) for x in w[1:1 + n]]
+        self.steps += vals
+        if n:
+            self.cur = vals[-1]
+        self.file_num = len(self.steps)
+
+    def advance(self):
+        if self.timebased:
+            self.write = (self.cur + self.dt) >= (self.file_num + 1) * self.interval
+        else:
+            self.write = self.time_step % self.write_every == 0
+        if self.write:
+            self.file_num += 1
+            self.steps.append(self.cur)
+        self.log = self.time_step % 100 == 0
+        self.cur += self.dt
+        self.time_step += 1
+
+    def run(self):
+        return self.cur < self.end
