@@ -220,6 +220,47 @@ def wc_bench(nv, n: int, steps: int) -> dict:
                               "frac": round(alg_t / (ms_t / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
+def io_bench(nv, n: int, writes: int) -> dict:
+    """EnSight Gold output of the TG velocity (Field::write of a 3-component
+    n^3 field): the layout change to float32, i fastest, on the GPU, then the
+    copy to host and the file write behind it."""
+    import shutil
+    import tempfile
+    import torch
+    G = nv.Mesh((n, n, n), (2 * math.pi,) * 3)
+    U = tg_field(nv, G, n)
+    t = nv.Time(DT, 1.0, 1)
+    io = nv.IO(G)
+    d = tempfile.mkdtemp(prefix="nova_io_", dir=os.environ.get("TMPDIR", "/tmp"))
+    try:
+        t.advance()
+        io.write_field(t, d, "U", U)  # sizes the staging slots
+        io.wait()
+        torch.cuda.synchronize()
+        G.timing(True)
+        t0 = time.perf_counter()
+        for _ in range(writes):
+            t.advance()
+            io.write_field(t, d, "U", U)
+        t_call = time.perf_counter() - t0
+        io.wait()
+        t_all = time.perf_counter() - t0
+        st = G.timing_collect()
+        G.timing(False)
+        file_bytes = os.path.getsize(os.path.join(d, f"U.0000.{t.fileNum():05d}.dat"))
+    finally:
+        io.close()
+        shutil.rmtree(d, ignore_errors=True)
+    c, us = st["ensight_pack"]
+    us = us / c * 1e3
+    alg = 8.0 * 3 * n ** 3 + 4.0 * 3 * (n + 1) ** 3  # FP64 stored nodes in, float32 EnSight nodes out
+    return {"workload": f"EnSight Gold write of the TG {n}^3 velocity (3 x {n + 1}^3 float32), {writes} files",
+            "pack_us": round(us, 1), "pack_alg_bytes": alg,
+            "pack_GBps": round(alg / (us / 1e6) / 1e9, 1), "pack_frac": round(alg / (us / 1e6) / 1e9 / HBM_PEAK_GBS, 4),
+            "file_MB": round(file_bytes / 1e6, 1), "call_ms": round(t_call / writes * 1e3, 3),
+            "write_MBps": round(writes * file_bytes / t_all / 1e6, 1)}
+
+
 def ensemble_bench(nv, steps: int) -> dict:
     """48 uneven instances (16 x 32^3, 16 x 48^3, 12 x 64^3, 4 x 96^3), TG and
     Orszag-Tang alternating within each size, Re_i ~ U[400, 1600], 1e-3
@@ -375,6 +416,8 @@ def main() -> int:
     ap.add_argument("--compact-steps", type=int, default=20)
     ap.add_argument("--no-ot", action="store_true")
     ap.add_argument("--no-wc", action="store_true")
+    ap.add_argument("--no-io", action="store_true")
+    ap.add_argument("--io-writes", type=int, default=3)
     ap.add_argument("--wc-steps", type=int, default=10)
     ap.add_argument("--ot-steps", type=int, default=10)
     ap.add_argument("--ensemble-steps", type=int, default=100)
@@ -491,6 +534,14 @@ def main() -> int:
         log(f"bench: WC {wc['ms_per_step']} ms/step, {wc['value']} Mpt-steps/s, "
             f"{wc['achieved_GBps']} GB/s algorithmic")
 
+    iob = None
+    if not args.no_io:
+        torch.cuda.empty_cache()
+        log("bench: EnSight output ...")
+        iob = io_bench(nv, n, args.io_writes)
+        log(f"bench: EnSight pack {iob['pack_us']} us ({iob['pack_GBps']} GB/s algorithmic), "
+            f"{iob['write_MBps']} MB/s to file, {iob['call_ms']} ms per call")
+
     ens = None
     if not args.no_ensemble:
         torch.cuda.empty_cache()
@@ -531,6 +582,7 @@ def main() -> int:
         "compact": cmp,
         "ot_mhd": ot,
         "weakly_compressible": wc,
+        "ensight_output": iob,
         "check": {"ke": ke, "maxabs_div": maxdiv},
     }
     print(json.dumps(out), flush=True)
