@@ -4,7 +4,10 @@
 // right-hand sides are all evaluated at the same U, one pressure projection
 // per step, and the kinetic-energy history written as "t Ek -dEk/dt".
 //
-// usage: tg_poisson N steps dt nu [out.dat]
+// With an output directory, the mesh, U and p are written after every step
+// in EnSight Gold, as app/TG_Piosson/TG.cpp:78-80 does.
+//
+// usage: tg_poisson N steps dt nu [out.dat [output_dir]]
 #include <cstdio>
 #include <iomanip>
 #include <iostream>
@@ -14,7 +17,7 @@
 int main(int argc, char *argv[])
 {
     if (argc < 5) {
-        std::fprintf(stderr, "usage: %s N steps dt nu [out.dat]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s N steps dt nu [out.dat [output_dir]]\n", argv[0]);
         return 2;
     }
     try {
@@ -56,6 +59,11 @@ int main(int argc, char *argv[])
             p_ptr->correctBoundaryConditions();
             U = Ustar - time.dt() * ex::grad(*p_ptr);
             U.correctBoundaryConditions();
+            if (argc > 6) {
+                mesh.write(argv[6]);
+                U.write(argv[6], "U");
+                p_ptr->write(argv[6], "p");
+            }
             if (time.writelog()) std::cout << "Velocity divergence: " << tools::maximum(ex::div(U)) << std::endl;
             // kinetic energy over the unique nodes, on the host mirror
             scalar Ek = 0.0;
@@ -73,6 +81,7 @@ int main(int argc, char *argv[])
                 << -(Ek - EkOld) / time.dt() << "\n";
             EkOld = Ek;
         }
+        mesh.flushOutput();
         std::cout << "done " << time.timeStep() << " steps" << std::endl;
     } catch (const std::exception &e) {
         std::cerr << "tg_poisson: " << e.what() << std::endl;

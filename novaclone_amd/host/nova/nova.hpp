@@ -122,29 +122,104 @@ public:
     }
 };
 
-// ---- Time (src/Time/Time.cpp:22-140) ----
+// ---- Time (src/Time/Time.cpp:22-139): the library's clock ----
+// Time(dt, endTime, writeEvery): output every writeEvery steps from the
+// first; Time(dt, endTime, writeInterval): output when the time reaches the
+// next multiple of the interval; Time(file): "dt:", "writeT:", "writeN:",
+// "end:" lines.  With settings::restart() the clock resumes from
+// ./data/restart.pPPPP.case, the name Time.cpp:44 reads (Mesh::write names
+// its restart file restart.p.PPPP.case).
 class Time {
-    scalar dt_, curTime_ = 0.0, endTime_;
-    int writeEvery_, timeStep_ = 0, fileNum_ = 0;
-    bool write_ = false, log_ = true;
+    nova_time *t_ = nullptr;
+    scalar dt_ = 0.0;  // dt() is writable; pushed to the clock before each step
+
+    void restartIfAsked()
+    {
+        if (settings::restart()) nova::check(nova_time_restart(t_, "./data/restart.p0000.case"), "Time restart");
+    }
+    void info(double d[4], int64_t i[5]) const { nova::check(nova_time_info(t_, d, i), "nova_time_info"); }
 
 public:
-    Time(scalar dt, scalar endTime, int writeEvery) : dt_(dt), endTime_(endTime), writeEvery_(writeEvery) {}
+    Time(scalar dt, scalar endTime, int writeEvery) : dt_(dt)
+    {
+        nova::check(nova_time_create(dt, endTime, writeEvery, &t_), "nova_time_create");
+        restartIfAsked();
+    }
+    Time(scalar dt, scalar endTime, scalar writeInterval) : dt_(dt)
+    {
+        nova::check(nova_time_create_interval(dt, endTime, writeInterval, &t_), "nova_time_create_interval");
+        restartIfAsked();
+    }
+    explicit Time(const std::string &fileName)
+    {
+        nova::check(nova_time_create_from_file(fileName.c_str(), &t_), "nova_time_create_from_file");
+        double d[4];
+        int64_t i[5];
+        info(d, i);
+        dt_ = d[0];
+    }
+    Time(const Time &) = delete;
+    Time &operator=(const Time &) = delete;
+    ~Time() { nova_time_destroy(t_); }
+
     scalar &dt() { return dt_; }
-    bool run() const { return curTime_ < endTime_; }
-    bool write() const { return write_; }
-    bool writelog() const { return log_; }
-    scalar curTime() const { return curTime_; }
-    int timeStep() const { return timeStep_; }
-    int fileNum() const { return fileNum_; }
+    bool run() const
+    {
+        double d[4];
+        int64_t i[5];
+        info(d, i);
+        return d[1] < d[2];
+    }
+    bool write() const
+    {
+        double d[4];
+        int64_t i[5];
+        info(d, i);
+        return i[2] != 0;
+    }
+    bool writelog() const
+    {
+        double d[4];
+        int64_t i[5];
+        info(d, i);
+        return i[3] != 0;
+    }
+    scalar curTime() const
+    {
+        double d[4];
+        int64_t i[5];
+        info(d, i);
+        return d[1];
+    }
+    int timeStep() const
+    {
+        double d[4];
+        int64_t i[5];
+        info(d, i);
+        return (int)i[0];
+    }
+    int fileNum() const
+    {
+        double d[4];
+        int64_t i[5];
+        info(d, i);
+        return (int)i[1];
+    }
+    std::vector<scalar> steps() const
+    {
+        double d[4];
+        int64_t i[5];
+        info(d, i);
+        std::vector<scalar> v((size_t)i[4]);
+        nova::check(nova_time_steps(t_, v.data(), i[4]), "nova_time_steps");
+        return v;
+    }
     void operator++(int)
     {
-        curTime_ += dt_;
-        timeStep_++;
-        write_ = writeEvery_ > 0 && timeStep_ % writeEvery_ == 0;
-        if (write_) fileNum_++;
-        log_ = (timeStep_ % 100 == 0);
+        nova::check(nova_time_set_dt(t_, dt_), "nova_time_set_dt");
+        nova::check(nova_time_advance(t_), "nova_time_advance");
     }
+    const nova_time *handle() const { return t_; }
 };
 
 // single-rank reductions (src/parallelCom/parallelCom.h:244-282)
@@ -159,9 +234,11 @@ class Mesh {
     struct Ctx {
         nova_ctx *ctx = nullptr;
         nova_poisson *ps = nullptr;
+        nova_io *io = nullptr;
         std::map<int, std::vector<double *>> pool;  // free device buffers by component count
         ~Ctx()
         {
+            if (io) nova_io_destroy(io);  // waits for queued writes
             for (auto &kv : pool)
                 for (double *p : kv.second) nova_free(ctx, p);
             if (ps) nova_poisson_destroy(ps);
@@ -259,6 +336,22 @@ public:
         return p;
     }
     void give(int nc, double *p) const { c_->pool[nc].push_back(p); }
+    // output (asynchronous; files land in the order written)
+    nova_io *ioHandle() const
+    {
+        if (!c_->io) nova::check(nova_io_create(c_->ctx, &c_->io), "nova_io_create");
+        return c_->io;
+    }
+    // Mesh::write (Mesh.cpp:224-303) with writeCase and writeRestartInfo
+    void write(const std::string &path) const
+    {
+        nova::check(nova_io_write_mesh(ioHandle(), time_->handle(), path.c_str()), "Mesh::write");
+    }
+    // blocks until every queued file is written
+    void flushOutput() const
+    {
+        if (c_->io) nova::check(nova_io_wait(c_->io), "nova_io_wait");
+    }
     std::shared_ptr<Ctx> keep() const { return c_; }
 };
 
@@ -364,6 +457,18 @@ public:
     {
         if (hostDirty_) syncDevice();
         hostValid_ = false;
+    }
+
+    // Field::write / read (Field.hxx:403-522): EnSight Gold float32 files
+    void write(const std::string &path, const std::string &name) const
+    {
+        nova::check(nova_io_write_field(m_.ioHandle(), m_.time().handle(), path.c_str(), name.c_str(), dev(), NC),
+                    "Field::write");
+    }
+    void read(const std::string &path, const std::string &name) { read(path, name, m_.time().fileNum()); }
+    void read(const std::string &path, const std::string &name, int num)
+    {
+        nova::check(nova_io_read_field(m_.ioHandle(), path.c_str(), name.c_str(), num, devw(), NC), "Field::read");
     }
 
     void operator=(const Field &f)
