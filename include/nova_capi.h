@@ -157,6 +157,9 @@ int nova_div(nova_ctx *ctx, const double *U, double *out);
 int nova_grad(nova_ctx *ctx, const double *p, double *out);
 /* ex::laplacian(scalar, Field<T>), src/ex/laplacian/laplacian.hxx:130-191 */
 int nova_laplacian(nova_ctx *ctx, double nu, const double *f, int ncomp, double *out);
+/* ex::laplacian(Field<scalar> nu, Field<T>), src/ex/laplacian/laplacian.hxx:64-127:
+ * the variable-coefficient form, nu a one-component field (even at walls) */
+int nova_laplacian_var(nova_ctx *ctx, const double *nu, const double *f, int ncomp, double *out);
 /* ex::curl, src/ex/curl/curl.cpp:25-57 (half-shifted literal stencil) */
 int nova_curl_literal(nova_ctx *ctx, const double *U, double *out);
 /* consistent curl from the central stencil (no reference counterpart) */

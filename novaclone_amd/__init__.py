@@ -82,6 +82,7 @@ _SIGS = {
     "nova_div": [_P, _P, _P],
     "nova_grad": [_P, _P, _P],
     "nova_laplacian": [_P, _D, _P, _I, _P],
+    "nova_laplacian_var": [_P, _P, _P, _I, _P],
     "nova_curl_literal": [_P, _P, _P],
     "nova_curl": [_P, _P, _P],
     "nova_axpby": [_P, C.c_size_t, _D, _P, _D, _P, _P],
@@ -323,9 +324,15 @@ class ex:
         return out
 
     @staticmethod
-    def laplacian(mesh: Mesh, nu: float, f: torch.Tensor) -> torch.Tensor:
+    def laplacian(mesh: Mesh, nu, f: torch.Tensor) -> torch.Tensor:
+        """ex::laplacian(nu, f): nu a number, or a one-component field (the
+        variable-coefficient form, laplacian.hxx:64-127)."""
         out = mesh.field(f.shape[0])
-        _check(lib().nova_laplacian(mesh.handle, float(nu), _ptr(f), f.shape[0], _ptr(out)), "nova_laplacian")
+        if isinstance(nu, torch.Tensor):
+            _check(lib().nova_laplacian_var(mesh.handle, _ptr(nu), _ptr(f), f.shape[0], _ptr(out)),
+                   "nova_laplacian_var")
+        else:
+            _check(lib().nova_laplacian(mesh.handle, float(nu), _ptr(f), f.shape[0], _ptr(out)), "nova_laplacian")
         return out
 
     @staticmethod

@@ -484,6 +484,12 @@ int nova_laplacian(nova_ctx *ctx, double nu, const double *f, int ncomp, double 
     NOVA_GUARD(launch_laplacian(ctx, nu, f, ncomp, out));
 }
 
+int nova_laplacian_var(nova_ctx *ctx, const double *nu, const double *f, int ncomp, double *out)
+{
+    NOVA_REQUIRE(ctx && nu && f && out && out != f && out != nu && (ncomp == 1 || ncomp == 3), "bad argument");
+    NOVA_GUARD(launch_laplacian_var(ctx, nu, f, ncomp, out));
+}
+
 int nova_curl_literal(nova_ctx *ctx, const double *U, double *out)
 {
     NOVA_REQUIRE(ctx && U && out && out != U, "bad argument");

@@ -198,6 +198,7 @@ int launch_div_adv(nova_ctx *ctx, const double *U, const double *phi, int nc, do
 int launch_div(nova_ctx *ctx, const double *U, double scale, double *out);
 int launch_grad(nova_ctx *ctx, const double *p, double *out);
 int launch_laplacian(nova_ctx *ctx, double nu, const double *f, int nc, double *out);
+int launch_laplacian_var(nova_ctx *ctx, const double *nu, const double *f, int nc, double *out);
 int launch_curl(nova_ctx *ctx, const double *U, int literal, double *out, int pseudo = 0);
 int launch_axpby(nova_ctx *ctx, size_t n, double a, const double *x, double b, const double *y,
                  double *out);

@@ -135,6 +135,45 @@ __global__ void k_laplacian(Grid g, double nu, const double *__restrict__ f, int
     }
 }
 
+// ex::laplacian(Field<scalar> nu, f) (src/ex/laplacian/laplacian.hxx:64-127):
+// per l each neighbour weighted by nu on its near side (nu(j+l) with
+// f(j+1+l), nu(j-1-l) with f(j-1-l)), the centre by minus the pair's sum,
+// the z terms through 1/dzdxi^2; then, l by l, minus the z first
+// derivative times nu d2zdxi2/dzdxi^3.  nu is an even scalar at walls.
+__global__ void k_laplacian_var(Grid g, const double *__restrict__ nu, const double *__restrict__ f, int nc,
+                                double *__restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.s[2] || j >= g.s[1]) return;
+    const double n0 = ld(nu, g, 1, 0, i, j, k);
+    for (int c = 0; c < nc; c++) {
+        const double f0 = ld(f, g, nc, c, i, j, k);
+        double acc = 0.0;
+        for (int l = 0; l < g.h; l++) {
+            double t = 0.0, cP = 0.0;
+            for (int d = 0; d < 3; d++) {
+                int di, dj, dk;
+                offs(d, 1, di, dj, dk);
+                const double np = ld(nu, g, 1, 0, i + l * di, j + l * dj, k + l * dk);
+                const double nm = ld(nu, g, 1, 0, i - (1 + l) * di, j - (1 + l) * dj, k - (1 + l) * dk);
+                const double fp = ld(f, g, nc, c, i + (1 + l) * di, j + (1 + l) * dj, k + (1 + l) * dk);
+                const double fm = ld(f, g, nc, c, i - (1 + l) * di, j - (1 + l) * dj, k - (1 + l) * dk);
+                const double w = d == 2 ? g.b[2][l] * g.zlap[k] : g.b[d][l];
+                t += np * w * fp + nm * w * fm;
+                cP -= (np + nm) * w;
+            }
+            acc += t + cP * f0;
+        }
+        for (int l = 0; l < g.h; l++) {
+            const double fT = ld(f, g, nc, c, i, j, k + 1 + l), fB = ld(f, g, nc, c, i, j, k - 1 - l);
+            acc -= (g.a[2][l] * fT - g.a[2][l] * fB) * n0 * g.zmet[k];
+        }
+        out[c * g.np + lin(g, i, j, k)] = acc;
+    }
+}
+
 // Load of a 3-component field with the reflection parity of a pseudovector
 // (the curl of a velocity: the wall-normal component even, the others odd).
 __device__ inline double ld_pseudo(const double *__restrict__ f, const Grid &g, int c, int i, int j, int k)
@@ -496,6 +535,12 @@ int launch_grad(nova_ctx *ctx, const double *p, double *out)
 {
     hipLaunchKernelGGL(k_grad, pt_grid(ctx->g), pt_block(), 0, ctx->stream, ctx->g, p, out);
     return check_launch("grad");
+}
+
+int launch_laplacian_var(nova_ctx *ctx, const double *nu, const double *f, int nc, double *out)
+{
+    hipLaunchKernelGGL(k_laplacian_var, pt_grid(ctx->g), pt_block(), 0, ctx->stream, ctx->g, nu, f, nc, out);
+    return check_launch("laplacian_var");
 }
 
 int launch_laplacian(nova_ctx *ctx, double nu, const double *f, int nc, double *out)

@@ -572,6 +572,17 @@ FieldPtr<T> laplacian(const scalar &nu, const Field<T> &f)
     nova::check(nova_laplacian(f.mesh().ctx(), nu, f.dev(), pTraits<T>::nComp, r->devw()), "ex::laplacian");
     return r;
 }
+template <class T>
+FieldPtr<T> laplacian(const scalar &nu, const FieldPtr<T> &f) { return laplacian(nu, *f); }
+// ex::laplacian(Field<scalar> nu, f), src/ex/laplacian/laplacian.hxx:64-127
+template <class T>
+FieldPtr<T> laplacian(const Field<scalar> &nu, const Field<T> &f)
+{
+    auto r = nova::tmp<T>(f.mesh());
+    nova::check(nova_laplacian_var(f.mesh().ctx(), nu.dev(), f.dev(), pTraits<T>::nComp, r->devw()),
+                "ex::laplacian");
+    return r;
+}
 // ex::curl, src/ex/curl/curl.cpp:25-57
 inline FieldPtr<vector> curl(const Field<vector> &U)
 {

@@ -445,6 +445,64 @@ void orc_laplacian(const orc_mesh *M, double nu, const double *f, int nc, double
     }
 }
 
+/* ex::laplacian(Field<scalar> nu, Field<T>), src/ex/laplacian/laplacian.hxx:64-127:
+ * per l, each neighbour weighted by nu on its near side (nu(j+l) with
+ * f(j+1+l), nu(j-1-l) with f(j-1-l)), the centre by minus the sum of the
+ * pair's nu; the z terms divided by dzdxi^2; then per l minus the z first
+ * derivative times nu(i,j,k) d2zdxi2 / dzdxi^3.  nu is a padded scalar. */
+void orc_laplacian_var(const orc_mesh *M, const double *nu, const double *f, int nc, double *out)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    const double dx = M->d[0], dy = M->d[1], dz = M->d[2];
+    const double *dzdxi = M->dzdxi, *d2 = M->d2zdxi2;
+    memset(out, 0, sizeof(double) * NPTS(M) * nc);
+    for (int l = 0; l < h; l++) {
+        const double c = M->coef[l];
+        #pragma omp parallel for
+        for (int i = h; i < ni - h; i++)
+            for (int j = h; j < nj - h; j++)
+                for (int k = h; k < nk - h; k++) {
+                    const double nN = nu[IX(M, i, j + l, k)], nS = nu[IX(M, i, j - 1 - l, k)];
+                    const double nE = nu[IX(M, i + l, j, k)], nW = nu[IX(M, i - 1 - l, j, k)];
+                    const double nT = nu[IX(M, i, j, k + l)], nB = nu[IX(M, i, j, k - 1 - l)];
+                    const double *fN = &f[IX(M, i, j + 1 + l, k) * nc];
+                    const double *fS = &f[IX(M, i, j - 1 - l, k) * nc];
+                    const double *fE = &f[IX(M, i + 1 + l, j, k) * nc];
+                    const double *fW = &f[IX(M, i - 1 - l, j, k) * nc];
+                    const double *fT = &f[IX(M, i, j, k + 1 + l) * nc];
+                    const double *fB = &f[IX(M, i, j, k - 1 - l) * nc];
+                    const double *fP = &f[IX(M, i, j, k) * nc];
+                    double *o = &out[IX(M, i, j, k) * nc];
+                    const double cP = (-(nE + nW) * c / pow((l + 1) * dx, 2)
+                                       - (nN + nS) * c / pow((l + 1) * dy, 2)
+                                       - (nT + nB) * c / pow((l + 1) * dz, 2) / pow(dzdxi[k], 2));
+                    for (int q = 0; q < nc; q++) {
+                        o[q] += (nN * c / pow((l + 1) * dy, 2) * fN[q]
+                                 + nS * c / pow((l + 1) * dy, 2) * fS[q]
+                                 + nE * c / pow((l + 1) * dx, 2) * fE[q]
+                                 + nW * c / pow((l + 1) * dx, 2) * fW[q]
+                                 + nT * c / pow((l + 1) * dz, 2) * fT[q] / pow(dzdxi[k], 2)
+                                 + nB * c / pow((l + 1) * dz, 2) * fB[q] / pow(dzdxi[k], 2)
+                                 + cP * fP[q]);
+                    }
+                }
+    }
+    for (int l = 0; l < h; l++) { /* laplacian.hxx:106-124 (z-metric term) */
+        const double c = M->coef[l];
+        #pragma omp parallel for
+        for (int i = h; i < ni - h; i++)
+            for (int j = h; j < nj - h; j++)
+                for (int k = h; k < nk - h; k++) {
+                    const double *fT = &f[IX(M, i, j, k + 1 + l) * nc];
+                    const double *fB = &f[IX(M, i, j, k - 1 - l) * nc];
+                    double *o = &out[IX(M, i, j, k) * nc];
+                    for (int q = 0; q < nc; q++)
+                        o[q] -= (c / 2.0 / (l + 1) / dz * fT[q] - c / 2.0 / (l + 1) / dz * fB[q])
+                                * nu[IX(M, i, j, k)] * d2[k] / pow(dzdxi[k], 3);
+                }
+    }
+}
+
 /* ex::grad(Field<vector>) -> tensor, src/ex/grad/grad.cpp:65-102 (the
  * half-shifted stencil f(i+l) - f(i-1-l)); out9 is row-major
  * (xx, xy, xz, yx, ...), row = component, column = direction. */

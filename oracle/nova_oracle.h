@@ -68,6 +68,8 @@ void orc_div_adv(const orc_mesh *, const double *U, const double *phi, int ncphi
 void orc_div(const orc_mesh *, const double *U, double *out);
 void orc_grad(const orc_mesh *, const double *p, double *out);
 void orc_laplacian(const orc_mesh *, double nu, const double *f, int nc, double *out);
+/* Variable-coefficient form (laplacian.hxx:64-127); nu is a padded scalar. */
+void orc_laplacian_var(const orc_mesh *, const double *nu, const double *f, int nc, double *out);
 void orc_grad_vec_literal(const orc_mesh *, const double *U, double *out9);
 void orc_curl_literal(const orc_mesh *, const double *U, double *out);
 void orc_curl_central(const orc_mesh *, const double *U, double *out);

@@ -46,6 +46,7 @@ def _load() -> C.CDLL:
         "orc_div": ([_P, _D, _D], None),
         "orc_grad": ([_P, _D, _D], None),
         "orc_laplacian": ([_P, C.c_double, _D, C.c_int, _D], None),
+        "orc_laplacian_var": ([_P, _D, _D, C.c_int, _D], None),
         "orc_grad_vec_literal": ([_P, _D, _D], None),
         "orc_curl_literal": ([_P, _D, _D], None),
         "orc_curl_central": ([_P, _D, _D], None),
@@ -180,6 +181,12 @@ class OMesh:
     def laplacian(self, nu, f):
         out = self.field(f.shape[-1])
         lib().orc_laplacian(self._h, nu, _d(f), f.shape[-1], _d(out))
+        return out
+
+    def laplacian_var(self, nu, f):
+        """ex::laplacian(Field<scalar> nu, f); nu padded with nc = 1."""
+        out = self.field(f.shape[-1])
+        lib().orc_laplacian_var(self._h, _d(nu), _d(f), f.shape[-1], _d(out))
         return out
 
     def grad_vec_literal(self, U):

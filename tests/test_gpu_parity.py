@@ -80,7 +80,9 @@ def test_operators_match_oracle(nv, orc, case):
     cu, pu = rand_fields(O, 1, 3)
     cs, ps = rand_fields(O, 2, 1)
     cv, pv = rand_fields(O, 3, 3)
-    U, s, V = dev(cu), dev(cs), dev(cv)
+    cn = 0.5 + np.random.default_rng(4).uniform(0, 1, (1, *O.stored))
+    pn = O.from_compact(cn)
+    U, s, V, nu = dev(cu), dev(cs), dev(cv), dev(cn)
     chk = {
         "div_adv(U,s)": (nv.ex.div(G, U, s), O.div_adv(pu, ps)),
         "div_adv(U,V)": (nv.ex.div(G, U, V), O.div_adv(pu, pv)),
@@ -88,12 +90,29 @@ def test_operators_match_oracle(nv, orc, case):
         "grad(s)": (nv.ex.grad(G, s), O.grad(ps)),
         "lap(s)": (nv.ex.laplacian(G, 0.3, s), O.laplacian(0.3, ps)),
         "lap(V)": (nv.ex.laplacian(G, 0.3, V), O.laplacian(0.3, pv)),
+        "lap(nu,s)": (nv.ex.laplacian(G, nu, s), O.laplacian_var(pn, ps)),
+        "lap(nu,V)": (nv.ex.laplacian(G, nu, V), O.laplacian_var(pn, pv)),
         "curl_literal(U)": (nv.ex.curl(G, U, literal=True), O.curl_literal(pu)),
         "curl(U)": (nv.ex.curl(G, U, literal=False), O.curl_central(pu)),
     }
     for name, (g, o) in chk.items():
         err = rel_l2(host(g), O.interior(o))
         assert err <= RTOL, (name, err)
+
+
+def test_laplacians_on_a_stretched_grid_match_oracle(nv, orc):
+    # the z-metric terms of laplacian.hxx:106-124 and :170-188 (hyperbolic z)
+    N, L = (12, 10, 24), (1.0, 1.0, 2.0)
+    G = nv.Mesh(N, L, hyp=(0.02, 0.03))
+    O = orc.OMesh(N, L, hyp=(0.02, 0.03))
+    cv, pv = rand_fields(O, 5, 3)
+    cn = 0.5 + np.random.default_rng(6).uniform(0, 1, (1, *O.stored))
+    pn = O.from_compact(cn)
+    V, nu = dev(cv), dev(cn)
+    e1 = rel_l2(host(nv.ex.laplacian(G, nu, V)), O.interior(O.laplacian_var(pn, pv)))
+    e2 = rel_l2(host(nv.ex.laplacian(G, 0.3, V)), O.interior(O.laplacian(0.3, pv)))
+    print("stretched lap(nu,V)", e1, "lap(V)", e2, flush=True)
+    assert e1 <= RTOL and e2 <= RTOL
 
 
 @pytest.mark.parametrize("wall_y", [False, True])
