@@ -82,6 +82,7 @@ typedef struct nova_ensemble nova_ensemble;
 typedef struct nova_wc nova_wc;
 typedef struct nova_time nova_time;
 typedef struct nova_io nova_io;
+typedef struct nova_fdmatrix nova_fdmatrix;
 
 /* ---- context / memory (replaces Mesh construction, Mesh.cpp:24-179, and
  * Field storage, src/Field/Field.hxx:21-134) ---- */
@@ -281,6 +282,40 @@ int nova_wc_step_t(nova_wc *w, double dt, int nsteps, double *U, double *rho, do
  * loop bounds leave them.  Then rho = (p / pc)^(1/2) on every node.  T in,
  * p in/out, rho out; device pointers, stored layout. */
 int nova_wc_hydrostatic(nova_wc *w, const double *T, double rho0, double *p, double *rho);
+
+/* ---- implicit matrix im::fdMatrix<scalar> (src/im/fdMatrix/fdMatrix.hxx)
+ * and im::laplacian (src/im/laplacian/laplacian.hxx:23-58) ----
+ * The matrix keeps its six neighbour coefficient fields (north, south,
+ * east, west, top, bottom), ap and the right-hand side in the reference's
+ * padded layout (nova_ctx_info's padded extents, k fastest).  Fields passed
+ * in or out (phi, f) are stored-layout scalars; the matrix pads them with
+ * the grid's boundary fill, as a reference Field carries its ghosts. */
+/* im::laplacian(mu, phi): al = mu/d^2 and ap = -2 mu (1/dx^2 + 1/dy^2 +
+ * 1/dz^2) on the interior, then deferredCorrection(ex::laplacian(mu, phi))
+ * (fdMatrix.hxx:213-245): rhs = -lap + sum al phi_nb + ap phi. */
+int nova_fd_laplacian(nova_ctx *ctx, double mu, const double *phi, nova_fdmatrix **out);
+int nova_fd_destroy(nova_fdmatrix *m);
+/* operator==(matrix, f) (fdMatrix.hxx:372-382): rhs += f */
+int nova_fd_add_source(nova_fdmatrix *m, const double *f);
+/* relax(urf) (fdMatrix.hxx:248-253) with phi the transported field */
+int nova_fd_relax(nova_fdmatrix *m, const double *phi, double urf);
+/* setReference(i, j, k, val) (fdMatrix.hxx:256-268), padded indices */
+int nova_fd_set_reference(nova_fdmatrix *m, int i, int j, int k, double val);
+/* m += s * other (operator+= with s = 1, operator-= with s = -1,
+ * fdMatrix.hxx:327-356), or m *= s with other = NULL (unary minus, s = -1,
+ * :394-413); both matrices on one context */
+int nova_fd_combine(nova_fdmatrix *m, const nova_fdmatrix *other, double s);
+/* residual() (fdMatrix.hxx:271-310): RMS of rhs - A phi over the interior */
+int nova_fd_residual(nova_fdmatrix *m, const double *phi, double *res);
+/* solve(tol, maxIts) (fdMatrix.hxx:59-209) on phi in place: until the
+ * residual is at most tol, sweeps of Thomas line solves along j, i and k
+ * in the reference's line order (each line sees the lines before it in its
+ * sweep already updated), then the boundary refill; its = iterations run,
+ * res = the last residual computed. */
+int nova_fd_solve(nova_fdmatrix *m, double *phi, double tol, int max_its, int *its, double *res);
+/* A padded array of the matrix: what = 0..5 al(what), 6 ap, 7 rhs, 8 the
+ * padded transported field of the last call. */
+int nova_fd_get(const nova_fdmatrix *m, int what, double *padded_out);
 
 /* ---- output and restart (src/Time/Time.cpp, src/Mesh/Mesh.cpp:224-369,
  * src/Field/Field.hxx:403-522) ---- */

@@ -110,6 +110,15 @@ _SIGS = {
     "nova_wc_set_temperature": [_P, _D, _D, _D, _D],
     "nova_wc_step_t": [_P, _D, _I, _P, _P, _P, _P],
     "nova_wc_hydrostatic": [_P, _P, _D, _P, _P],
+    "nova_fd_laplacian": [_P, _D, _P, C.POINTER(_P)],
+    "nova_fd_destroy": [_P],
+    "nova_fd_add_source": [_P, _P],
+    "nova_fd_relax": [_P, _P, _D],
+    "nova_fd_set_reference": [_P, _I, _I, _I, _D],
+    "nova_fd_combine": [_P, _P, _D],
+    "nova_fd_residual": [_P, _P, _DP],
+    "nova_fd_solve": [_P, _P, _D, _I, C.POINTER(C.c_int), _DP],
+    "nova_fd_get": [_P, _I, _P],
     "nova_time_create": [_D, _D, _I, C.POINTER(_P)],
     "nova_time_create_interval": [_D, _D, _D, C.POINTER(_P)],
     "nova_time_create_from_file": [C.c_char_p, C.POINTER(_P)],
@@ -423,6 +432,77 @@ class Solver:
     def close(self):
         if getattr(self, "_h", None):
             lib().nova_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FdMatrix:
+    """im::fdMatrix<scalar> built by im::laplacian(mu, phi)
+    (src/im/laplacian/laplacian.hxx:23-58): a 7-point matrix with the
+    explicit Laplacian's deferred correction, solved by line Gauss-Seidel
+    sweeps (fdMatrix::solve).  Fields are stored-layout scalars of shape
+    (1, s0, s1, s2); the matrix's own arrays are padded."""
+
+    def __init__(self, mesh: Mesh, mu: float, phi: torch.Tensor, _handle=None):
+        self.mesh = mesh
+        h = C.c_void_p()
+        if _handle is not None:
+            h = _handle
+        else:
+            _check(lib().nova_fd_laplacian(mesh.handle, float(mu), _ptr(phi), C.byref(h)), "nova_fd_laplacian")
+        self._h = h
+
+    def add_source(self, f: torch.Tensor) -> "FdMatrix":
+        """matrix == f: the right-hand side gains f."""
+        _check(lib().nova_fd_add_source(self._h, _ptr(f)), "nova_fd_add_source")
+        return self
+
+    def relax(self, phi: torch.Tensor, urf: float) -> None:
+        _check(lib().nova_fd_relax(self._h, _ptr(phi), float(urf)), "nova_fd_relax")
+
+    def set_reference(self, i: int, j: int, k: int, val: float) -> None:
+        _check(lib().nova_fd_set_reference(self._h, int(i), int(j), int(k), float(val)), "nova_fd_set_reference")
+
+    def __iadd__(self, other: "FdMatrix"):
+        _check(lib().nova_fd_combine(self._h, other._h, 1.0), "nova_fd_combine")
+        return self
+
+    def __isub__(self, other: "FdMatrix"):
+        _check(lib().nova_fd_combine(self._h, other._h, -1.0), "nova_fd_combine")
+        return self
+
+    def negate(self) -> "FdMatrix":
+        _check(lib().nova_fd_combine(self._h, None, -1.0), "nova_fd_combine")
+        return self
+
+    def residual(self, phi: torch.Tensor) -> float:
+        r = C.c_double()
+        _check(lib().nova_fd_residual(self._h, _ptr(phi), C.byref(r)), "nova_fd_residual")
+        return r.value
+
+    def solve(self, phi: torch.Tensor, tol: float, max_its: int):
+        """Solve for phi in place; returns (iterations, last residual)."""
+        its = C.c_int()
+        r = C.c_double()
+        _check(lib().nova_fd_solve(self._h, _ptr(phi), float(tol), int(max_its), C.byref(its), C.byref(r)),
+               "nova_fd_solve")
+        return its.value, r.value
+
+    def padded(self, what: int) -> torch.Tensor:
+        """al(0..5), 6 ap, 7 rhs, 8 the padded transported field."""
+        out = torch.empty(self.mesh.padded, dtype=torch.float64, device=self.mesh.device)
+        _check(lib().nova_fd_get(self._h, int(what), _ptr(out)), "nova_fd_get")
+        torch.cuda.synchronize()
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nova_fd_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -122,6 +122,17 @@ int wc_set_magnetic(nova_wc *w, const double *B, double dbu);
 int wc_set_temperature(nova_wc *w, double ka, const double g[3]);
 int wc_step_t(nova_wc *w, double dt, int nsteps, double *U, double *rho, double *p, double *T);
 int wc_hydrostatic(nova_wc *w, const double *T, double rho0, double *p, double *rho);
+int fd_laplacian(nova_ctx *ctx, double mu, const double *phi, nova_fdmatrix **out);
+int fd_destroy(nova_fdmatrix *m);
+int fd_add_source(nova_fdmatrix *m, const double *f);
+int fd_relax(nova_fdmatrix *m, const double *phi, double urf);
+int fd_set_reference(nova_fdmatrix *m, int i, int j, int k, double val);
+int fd_combine(nova_fdmatrix *m, const nova_fdmatrix *o, double s);
+int fd_residual(nova_fdmatrix *m, const double *phi, double *res);
+int fd_solve(nova_fdmatrix *m, double *phi, double tol, int max_its, int *its_out, double *res_out);
+int fd_get(const nova_fdmatrix *m, double *dst, int what);
+int fd_same_grid(const nova_fdmatrix *a, const nova_fdmatrix *b);
+int fd_inside(const nova_fdmatrix *m, int i, int j, int k);
 int time_create(double dt, double end, int write_every, double interval, bool timebased, nova_time **out);
 int time_create_from_file(const char *file, nova_time **out);
 int time_restart(nova_time *t, const char *file);
@@ -672,6 +683,60 @@ int nova_wc_hydrostatic(nova_wc *w, const double *T, double rho0, double *p, dou
 {
     NOVA_REQUIRE(w && T && p && rho, "bad argument");
     NOVA_GUARD(wc_hydrostatic(w, T, rho0, p, rho));
+}
+
+int nova_fd_laplacian(nova_ctx *ctx, double mu, const double *phi, nova_fdmatrix **out)
+{
+    NOVA_REQUIRE(ctx && phi && out, "bad argument");
+    NOVA_GUARD(fd_laplacian(ctx, mu, phi, out));
+}
+
+int nova_fd_destroy(nova_fdmatrix *m)
+{
+    if (!m) return NOVA_OK;
+    NOVA_GUARD(fd_destroy(m));
+}
+
+int nova_fd_add_source(nova_fdmatrix *m, const double *f)
+{
+    NOVA_REQUIRE(m && f, "bad argument");
+    NOVA_GUARD(fd_add_source(m, f));
+}
+
+int nova_fd_relax(nova_fdmatrix *m, const double *phi, double urf)
+{
+    NOVA_REQUIRE(m && phi && urf != 0.0, "bad argument");
+    NOVA_GUARD(fd_relax(m, phi, urf));
+}
+
+int nova_fd_set_reference(nova_fdmatrix *m, int i, int j, int k, double val)
+{
+    NOVA_REQUIRE(m && fd_inside(m, i, j, k), "bad argument");
+    NOVA_GUARD(fd_set_reference(m, i, j, k, val));
+}
+
+int nova_fd_combine(nova_fdmatrix *m, const nova_fdmatrix *other, double s)
+{
+    NOVA_REQUIRE(m && (!other || fd_same_grid(m, other)), "bad argument");
+    NOVA_GUARD(fd_combine(m, other, s));
+}
+
+int nova_fd_residual(nova_fdmatrix *m, const double *phi, double *res)
+{
+    NOVA_REQUIRE(m && phi && res, "bad argument");
+    NOVA_GUARD(fd_residual(m, phi, res));
+}
+
+int nova_fd_solve(nova_fdmatrix *m, double *phi, double tol, int max_its, int *its, double *res)
+{
+    NOVA_REQUIRE(m && phi && max_its >= 0, "bad argument");
+    NOVA_GUARD(fd_solve(m, phi, tol, max_its, its, res));
+}
+
+int nova_fd_get(const nova_fdmatrix *m, int what, double *padded_out)
+{
+    NOVA_REQUIRE(m && padded_out && what >= 0 && what <= 8, "bad argument");
+    NOVA_GUARD(fd_get(m, padded_out, what));
 }
 
 int nova_time_create(double dt, double end_time, int write_every, nova_time **out)

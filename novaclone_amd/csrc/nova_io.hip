@@ -289,7 +289,7 @@ int copy_out(nova_io *io, int s, size_t bytes)
     return NOVA_OK;
 }
 
-int launch_pack(nova_io *io, const double *F, int nc, float *out)
+int launch_ens_pack(nova_io *io, const double *F, int nc, float *out)
 {
     const nova_ctx *ctx = io->ctx;
     const EnsDims d = ens_dims(ctx);
@@ -472,7 +472,7 @@ int io_write_field(nova_io *io, const nova_time *t, const char *path, const char
     const size_t bytes = sizeof(float) * (size_t)nc * ens_nodes(d);
     int s = -1;
     if (int rc = acquire_slot(io, bytes, s)) return rc;
-    if (int rc = launch_pack(io, F, nc, (float *)io->slot[s].dev)) {
+    if (int rc = launch_ens_pack(io, F, nc, (float *)io->slot[s].dev)) {
         release_slot(io, s);
         return rc;
     }

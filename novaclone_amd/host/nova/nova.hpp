@@ -21,11 +21,13 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iostream>
 #include <map>
 #include <memory>
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "nova_capi.h"
@@ -606,6 +608,92 @@ public:
         nova::check(nova_poisson_solve(p_->mesh().poissonHandle(), rhs_->dev(), p_->devw()), "poisson::solve");
     }
 };
+
+// ---- implicit matrix (src/im/fdMatrix/fdMatrix.h, src/im/laplacian/laplacian.hxx) ----
+namespace im {
+template <class T>
+class fdMatrix {
+    static_assert(std::is_same<T, scalar>::value, "fdMatrix solves scalar fields, as the reference's does");
+    nova_fdmatrix *m_ = nullptr;
+    FieldPtr<scalar> fld_;  // the transported field, solved in place
+
+public:
+    fdMatrix(nova_fdmatrix *m, FieldPtr<scalar> f) : m_(m), fld_(std::move(f)) {}
+    fdMatrix(const fdMatrix &) = delete;
+    fdMatrix &operator=(const fdMatrix &) = delete;
+    ~fdMatrix() { nova_fd_destroy(m_); }
+    nova_fdmatrix *handle() const { return m_; }
+    // fdMatrix.hxx:59-209
+    void solve(scalar tol, int maxIts)
+    {
+        int its = 0;
+        double res = 0.0;
+        nova::check(nova_fd_solve(m_, fld_->devw(), tol, maxIts, &its, &res), "fdMatrix::solve");
+        if (fld_->mesh().time().writelog()) {
+            std::cout << "Poisson Equation converged in " << its << " iterations." << std::endl;
+            std::cout << "Residual: " << res << "." << std::endl;
+        }
+    }
+    void relax(const scalar &urf) { nova::check(nova_fd_relax(m_, fld_->dev(), urf), "fdMatrix::relax"); }
+    T residual()
+    {
+        double r = 0.0;
+        nova::check(nova_fd_residual(m_, fld_->dev(), &r), "fdMatrix::residual");
+        return r;
+    }
+    void setReference(int i, int j, int k, scalar val)
+    {
+        nova::check(nova_fd_set_reference(m_, i, j, k, val), "fdMatrix::setReference");
+    }
+    fdMatrix &operator+=(const fdMatrix &r)
+    {
+        nova::check(nova_fd_combine(m_, r.m_, 1.0), "fdMatrix::operator+=");
+        return *this;
+    }
+    fdMatrix &operator-=(const fdMatrix &r)
+    {
+        nova::check(nova_fd_combine(m_, r.m_, -1.0), "fdMatrix::operator-=");
+        return *this;
+    }
+    void negate() { nova::check(nova_fd_combine(m_, nullptr, -1.0), "fdMatrix::operator-"); }
+    void addSource(const Field<T> &f) { nova::check(nova_fd_add_source(m_, f.dev()), "fdMatrix::operator=="); }
+};
+
+template <class T>
+std::shared_ptr<fdMatrix<T>> laplacian(const scalar mu, std::shared_ptr<Field<T>> fld)
+{
+    nova_fdmatrix *m = nullptr;
+    nova::check(nova_fd_laplacian(fld->mesh().ctx(), mu, fld->dev(), &m), "im::laplacian");
+    return std::make_shared<fdMatrix<T>>(m, fld);
+}
+template <class T>
+std::shared_ptr<fdMatrix<T>> laplacian(std::shared_ptr<Field<T>> fld) { return im::laplacian(scalar(1.0), fld); }
+// global operators (fdMatrix.hxx:359-413): they act on the left operand
+template <class T>
+std::shared_ptr<fdMatrix<T>> operator==(std::shared_ptr<fdMatrix<T>> m, const std::shared_ptr<Field<T>> &f)
+{
+    m->addSource(*f);
+    return m;
+}
+template <class T>
+std::shared_ptr<fdMatrix<T>> operator+(std::shared_ptr<fdMatrix<T>> a, const std::shared_ptr<fdMatrix<T>> &b)
+{
+    *a += *b;
+    return a;
+}
+template <class T>
+std::shared_ptr<fdMatrix<T>> operator-(std::shared_ptr<fdMatrix<T>> a, const std::shared_ptr<fdMatrix<T>> &b)
+{
+    *a -= *b;
+    return a;
+}
+template <class T>
+std::shared_ptr<fdMatrix<T>> operator-(std::shared_ptr<fdMatrix<T>> a)
+{
+    a->negate();
+    return a;
+}
+}  // namespace im
 
 // ---- tools (src/Tools/Tools.h:19-36, Tools.cpp:43-62) ----
 namespace tools {

@@ -503,6 +503,165 @@ void orc_laplacian_var(const orc_mesh *M, const double *nu, const double *f, int
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* Implicit matrix im::fdMatrix<scalar> (src/im/fdMatrix/fdMatrix.hxx) and
+ * im::laplacian (src/im/laplacian/laplacian.hxx:23-58).  al holds the six
+ * padded coefficient fields (north, south, east, west, top, bottom) one
+ * after the other; ap, rhs and phi are padded scalars. */
+
+static double *falloc(const orc_mesh *M, int nc);
+
+#define AL(q, n) al[(size_t)(q) * NPTS(M) + (n)]
+
+void orc_fd_laplacian(const orc_mesh *M, double mu, const double *phi, double *al, double *ap, double *rhs)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    const size_t np = NPTS(M);
+    memset(al, 0, sizeof(double) * 6 * np);
+    memset(ap, 0, sizeof(double) * np);
+    memset(rhs, 0, sizeof(double) * np);
+    for (int i = h; i < ni - h; i++)
+        for (int j = h; j < nj - h; j++)
+            for (int k = h; k < nk - h; k++) {
+                const size_t n = IX(M, i, j, k);
+                AL(0, n) = mu / pow(M->d[1], 2);
+                AL(1, n) = mu / pow(M->d[1], 2);
+                AL(2, n) = mu / pow(M->d[0], 2);
+                AL(3, n) = mu / pow(M->d[0], 2);
+                AL(4, n) = mu / pow(M->d[2], 2);
+                AL(5, n) = mu / pow(M->d[2], 2);
+                ap[n] = -2.0 * mu / pow(M->d[0], 2) - 2.0 * mu / pow(M->d[1], 2) - 2.0 * mu / pow(M->d[2], 2);
+            }
+    /* deferredCorrection(ex::laplacian(mu, phi)), fdMatrix.hxx:213-245 */
+    double *hoa = falloc(M, 1);
+    orc_laplacian(M, mu, phi, 1, hoa);
+    for (int i = h; i < ni - h; i++)
+        for (int j = h; j < nj - h; j++)
+            for (int k = h; k < nk - h; k++) {
+                const size_t n = IX(M, i, j, k);
+                rhs[n] += -hoa[n] + AL(0, n) * phi[IX(M, i, j + 1, k)] + AL(1, n) * phi[IX(M, i, j - 1, k)]
+                          + AL(2, n) * phi[IX(M, i + 1, j, k)] + AL(3, n) * phi[IX(M, i - 1, j, k)]
+                          + AL(4, n) * phi[IX(M, i, j, k + 1)] + AL(5, n) * phi[IX(M, i, j, k - 1)]
+                          + ap[n] * phi[n];
+            }
+    free(hoa);
+}
+
+/* fdMatrix.hxx:271-310 */
+double orc_fd_residual(const orc_mesh *M, const double *al, const double *ap, const double *rhs, const double *phi)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    double res = 0.0;
+    for (int i = h; i < ni - h; i++)
+        for (int j = h; j < nj - h; j++)
+            for (int k = h; k < nk - h; k++) {
+                const size_t n = IX(M, i, j, k);
+                const double r = rhs[n] - AL(0, n) * phi[IX(M, i, j + 1, k)] - AL(1, n) * phi[IX(M, i, j - 1, k)]
+                                 - AL(2, n) * phi[IX(M, i + 1, j, k)] - AL(3, n) * phi[IX(M, i - 1, j, k)]
+                                 - AL(4, n) * phi[IX(M, i, j, k + 1)] - AL(5, n) * phi[IX(M, i, j, k - 1)]
+                                 - ap[n] * phi[n];
+                res += pow(r, 2);
+            }
+    const int cnt = (ni - M->m) * (nj - M->m) * (nk - M->m);
+    return pow(res / cnt, 0.5);
+}
+
+/* tools::tdma, src/Tools/Tools.cpp:43-62 */
+static void tdma(int n, const double *a, const double *b, const double *c, const double *d, double *x, double *cp,
+                 double *dp)
+{
+    cp[0] = c[0] / b[0];
+    dp[0] = d[0] / b[0];
+    for (int i = 1; i < n; i++) {
+        cp[i] = c[i] / (b[i] - a[i] * cp[i - 1]);
+        dp[i] = (d[i] - a[i] * dp[i - 1]) / (b[i] - a[i] * cp[i - 1]);
+    }
+    x[n - 1] = dp[n - 1];
+    for (int i = n - 2; i >= 0; i--) x[i] = dp[i] - cp[i] * x[i + 1];
+}
+
+/* fdMatrix.hxx:59-209: returns the iteration count; *res_out the last
+ * residual computed. */
+int orc_fd_solve(const orc_mesh *M, const double *al, const double *ap, const double *rhs, double *phi, double tol,
+                 int max_its, double *res_out)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    int nmax = ni > nj ? ni : nj;
+    nmax = nmax > nk ? nmax : nk;
+    double *a = malloc(sizeof(double) * nmax), *b = malloc(sizeof(double) * nmax), *c = malloc(sizeof(double) * nmax);
+    double *d = malloc(sizeof(double) * nmax), *x = malloc(sizeof(double) * nmax);
+    double *cp = malloc(sizeof(double) * nmax), *dp = malloc(sizeof(double) * nmax);
+    int its = 0;
+    double res = 0.0;
+    while (its < max_its) {
+        res = orc_fd_residual(M, al, ap, rhs, phi);
+        if (!(res > tol)) break;
+        its++;
+        /* j sweeps */
+        for (int i = h; i < ni - h; i++)
+            for (int k = h; k < nk - h; k++) {
+                for (int j = 0; j < nj; j++) {
+                    const int bnd = j < h || j >= nj - h - 1;
+                    const size_t n = IX(M, i, j, k);
+                    a[j] = bnd ? 0.0 : AL(1, n);
+                    b[j] = bnd ? 1.0 : ap[n];
+                    c[j] = bnd ? 0.0 : AL(0, n);
+                    d[j] = bnd ? phi[n]
+                               : rhs[n] - AL(2, n) * phi[IX(M, i + 1, j, k)] - AL(3, n) * phi[IX(M, i - 1, j, k)]
+                                     - AL(4, n) * phi[IX(M, i, j, k + 1)] - AL(5, n) * phi[IX(M, i, j, k - 1)];
+                }
+                tdma(nj, a, b, c, d, x, cp, dp);
+                for (int j = h; j < nj - h; j++) phi[IX(M, i, j, k)] = x[j];
+            }
+        /* i sweeps */
+        for (int j = h; j < nj - h; j++)
+            for (int k = h; k < nk - h; k++) {
+                for (int i = 0; i < ni; i++) {
+                    const int bnd = i < h || i >= ni - h - 1;
+                    const size_t n = IX(M, i, j, k);
+                    a[i] = bnd ? 0.0 : AL(3, n);
+                    b[i] = bnd ? 1.0 : ap[n];
+                    c[i] = bnd ? 0.0 : AL(2, n);
+                    d[i] = bnd ? phi[n]
+                               : rhs[n] - AL(0, n) * phi[IX(M, i, j + 1, k)] - AL(1, n) * phi[IX(M, i, j - 1, k)]
+                                     - AL(4, n) * phi[IX(M, i, j, k + 1)] - AL(5, n) * phi[IX(M, i, j, k - 1)];
+                }
+                tdma(ni, a, b, c, d, x, cp, dp);
+                for (int i = h; i < ni - h; i++) phi[IX(M, i, j, k)] = x[i];
+            }
+        /* k sweeps */
+        for (int i = h; i < ni - h; i++)
+            for (int j = h; j < nj - h; j++) {
+                for (int k = 0; k < nk; k++) {
+                    const int bnd = k < h || k >= nk - h - 1;
+                    const size_t n = IX(M, i, j, k);
+                    a[k] = bnd ? 0.0 : AL(5, n);
+                    b[k] = bnd ? 1.0 : ap[n];
+                    c[k] = bnd ? 0.0 : AL(4, n);
+                    d[k] = bnd ? phi[n]
+                               : rhs[n] - AL(2, n) * phi[IX(M, i + 1, j, k)] - AL(3, n) * phi[IX(M, i - 1, j, k)]
+                                     - AL(0, n) * phi[IX(M, i, j + 1, k)] - AL(1, n) * phi[IX(M, i, j - 1, k)];
+                }
+                tdma(nk, a, b, c, d, x, cp, dp);
+                for (int k = h; k < nk - h; k++) phi[IX(M, i, j, k)] = x[k];
+            }
+        orc_fill(M, phi, 1, -1); /* transportedFld_->correctBoundaryConditions() */
+    }
+    free(a); free(b); free(c); free(d); free(x); free(cp); free(dp);
+    if (res_out) *res_out = res;
+    return its;
+}
+
+/* relax (fdMatrix.hxx:248-253) over every padded node */
+void orc_fd_relax(const orc_mesh *M, double *ap, double *rhs, const double *phi, double urf)
+{
+    const size_t np = NPTS(M);
+    for (size_t n = 0; n < np; n++) rhs[n] += (1.0 - urf) / urf * (ap[n] * phi[n]);
+    for (size_t n = 0; n < np; n++) ap[n] /= urf;
+}
+
+#undef AL
+
 /* ex::grad(Field<vector>) -> tensor, src/ex/grad/grad.cpp:65-102 (the
  * half-shifted stencil f(i+l) - f(i-1-l)); out9 is row-major
  * (xx, xy, xz, yx, ...), row = component, column = direction. */

@@ -120,6 +120,16 @@ int orc_field_write(const orc_mesh *, const double *f, int nc, const char *path,
 int orc_field_read(const orc_mesh *, double *f, int nc, const char *path, const char *name, int procNo,
                    int fileNum);
 int orc_mesh_write(const orc_mesh *, const char *path, int procNo, const double *steps, int nsteps);
+/* Implicit matrix (src/im/fdMatrix/fdMatrix.hxx, src/im/laplacian/laplacian.hxx:23-58)
+ * for scalars: al = six padded coefficient fields back to back (N, S, E, W,
+ * T, B), ap, rhs and phi padded.  orc_fd_laplacian builds im::laplacian(mu,
+ * phi) with its deferred correction; orc_fd_solve runs fdMatrix::solve on
+ * phi in place and returns the iteration count. */
+void orc_fd_laplacian(const orc_mesh *, double mu, const double *phi, double *al, double *ap, double *rhs);
+double orc_fd_residual(const orc_mesh *, const double *al, const double *ap, const double *rhs, const double *phi);
+int orc_fd_solve(const orc_mesh *, const double *al, const double *ap, const double *rhs, double *phi, double tol,
+                 int max_its, double *res_out);
+void orc_fd_relax(const orc_mesh *, double *ap, double *rhs, const double *phi, double urf);
 /* Projection: F <- F - s*grad(lap^-1(div(F)/s)), p receives the potential. */
 void orc_project(const orc_mesh *, orc_poisson *, double *F, double *p, double s);
 

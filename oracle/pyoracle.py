@@ -61,6 +61,10 @@ def _load() -> C.CDLL:
         "orc_wc_step": ([_P, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_double], None),
         "orc_wcs_step": ([_P, _D, _D, _D, _D, C.c_double, C.c_double, _D, C.c_double, C.c_double], None),
         "orc_wc_hydrostatic": ([_P, _D, _D, _D, _D, C.c_double, C.c_double], None),
+        "orc_fd_laplacian": ([_P, C.c_double, _D, _D, _D, _D], None),
+        "orc_fd_residual": ([_P, _D, _D, _D, _D], C.c_double),
+        "orc_fd_solve": ([_P, _D, _D, _D, _D, C.c_double, C.c_int, _D], C.c_int),
+        "orc_fd_relax": ([_P, _D, _D, _D, C.c_double], None),
         "orc_field_write": ([_P, _D, C.c_int, C.c_char_p, C.c_char_p, C.c_int, C.c_int], C.c_int),
         "orc_field_read": ([_P, _D, C.c_int, C.c_char_p, C.c_char_p, C.c_int, C.c_int], C.c_int),
         "orc_mesh_write": ([_P, C.c_char_p, C.c_int, _D, C.c_int], C.c_int),
@@ -234,6 +238,27 @@ class OMesh:
         app/shearLayerWC, in place."""
         gv = (C.c_double * 3)(*g)
         lib().orc_wc_hydrostatic(self._h, _d(T), _d(p), _d(rho), gv, rho0, pc)
+
+    # implicit matrix im::fdMatrix<scalar> / im::laplacian
+    def fd_laplacian(self, mu, phi):
+        """-> dict(al (6, *n), ap, rhs) for padded scalar phi (shape (*n, 1))."""
+        al = np.zeros((6, *self.n))
+        ap = np.zeros(self.n)
+        rhs = np.zeros(self.n)
+        lib().orc_fd_laplacian(self._h, mu, _d(np.ascontiguousarray(phi)), _d(al), _d(ap), _d(rhs))
+        return {"al": al, "ap": ap, "rhs": rhs}
+
+    def fd_residual(self, A, phi):
+        return lib().orc_fd_residual(self._h, _d(A["al"]), _d(A["ap"]), _d(A["rhs"]), _d(phi))
+
+    def fd_solve(self, A, phi, tol, max_its):
+        """Solve in place on padded phi; returns (iterations, last residual)."""
+        r = np.zeros(1)
+        its = lib().orc_fd_solve(self._h, _d(A["al"]), _d(A["ap"]), _d(A["rhs"]), _d(phi), tol, max_its, _d(r))
+        return its, r[0]
+
+    def fd_relax(self, A, phi, urf):
+        lib().orc_fd_relax(self._h, _d(A["ap"]), _d(A["rhs"]), _d(phi), urf)
 
     # EnSight Gold output
     def field_write(self, f, path, name, file_num, proc=0):
