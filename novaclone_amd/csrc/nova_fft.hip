@@ -123,9 +123,24 @@ __device__ inline void dft_small(double2 *a)
     }
 }
 
+// LDS position of element e of a line held contiguously (ES = 1): the low
+// four bits are XORed with bits 2-5 ^ 6-9, a permutation inside each
+// aligned 16-element block, so that the radix-4 passes (legs 1, 4, 16, 64
+// elements apart), the digit-reversed load and the natural-order and
+// mirrored reads each hit 16 distinct 16-byte bank groups in 16
+// consecutive lanes, or nearly so.  Lengths that are not a multiple of 16
+// keep the plain layout.
+template <int N>
+__device__ inline int swz(int e)
+{
+    if constexpr (N % 16 == 0) return e ^ (((e >> 2) ^ (e >> 6)) & 15);
+    else return e;
+}
+
 // One radix pass over NLINES line FFTs of length N held in LDS; element e
-// of line l at buf[l*LS + e*ES].  DIT twiddles before the butterfly, DIF
-// after it.  Consecutive threads walk the index that is contiguous in LDS.
+// of line l at buf[l*LS + e*ES] (at buf[l*LS + swz(e)] when ES = 1).  DIT
+// twiddles before the butterfly, DIF after it.  Consecutive threads walk
+// the index that is contiguous in LDS.
 template <int N, int S, bool DIT, bool INV, int NLINES, int LS, int ES>
 __device__ inline void fft_pass(double2 *buf, const double2 *tw)
 {
@@ -142,10 +157,15 @@ __device__ inline void fft_pass(double2 *buf, const double2 *tw)
         else { line = w / NB; bf = w % NB; }
         const int g = bf / Lp;
         const int q = bf % Lp;
-        double2 *b0 = buf + line * LS + (g * L + q) * ES;
+        double2 *bl = buf + line * LS;
+        const int e0 = g * L + q;
+        auto at = [&](int p) -> double2 & {
+            if constexpr (ES == 1) return bl[swz<N>(e0 + p * Lp)];
+            else return bl[(e0 + p * Lp) * ES];
+        };
         double2 a[R];
 #pragma unroll
-        for (int p = 0; p < R; p++) a[p] = b0[p * Lp * ES];
+        for (int p = 0; p < R; p++) a[p] = at(p);
         if constexpr (DIT) {
 #pragma unroll
             for (int p = 1; p < R; p++) {
@@ -164,7 +184,7 @@ __device__ inline void fft_pass(double2 *buf, const double2 *tw)
             }
         }
 #pragma unroll
-        for (int p = 0; p < R; p++) b0[p * Lp * ES] = a[p];
+        for (int p = 0; p < R; p++) at(p) = a[p];
     }
     __syncthreads();
 }
@@ -267,7 +287,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fwd_z(Grid g, const double2 *__
         const int j = j0 + jj;
         double v = 0.0;
         if (j < sy) v = src_is_vector ? div_at<WALLS>(g, src, i, j, k, scale) : src[lin(g, i, j, k)] * scale;
-        double *slot = (double *)&buf[(jj >> 1) * N + sperm[k]];
+        double *slot = (double *)&buf[(jj >> 1) * N + swz<N>(sperm[k])];
         slot[jj & 1] = v;
     }
     __syncthreads();
@@ -278,7 +298,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fwd_z(Grid g, const double2 *__
         const int j = j0 + jj;
         if (j >= sy) continue;
         const double2 *Z = buf + (jj >> 1) * N;
-        const double2 z = Z[kc], zm = Z[(N - kc) % N];
+        const double2 z = Z[swz<N>(kc)], zm = Z[swz<N>((N - kc) % N)];
         double2 out;
         if ((jj & 1) == 0) out = make_double2(0.5 * (z.x + zm.x), 0.5 * (z.y - zm.y));
         else out = make_double2(0.5 * (z.y + zm.y), -0.5 * (z.x - zm.x));
@@ -307,14 +327,14 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_y(const double2 *__restrict
             const int src = (EXT && q > Ny) ? 2 * Ny - q : q;
             v = H[L * sy + src];
         }
-        buf[ln * N + sperm[q]] = v;
+        buf[ln * N + swz<N>(sperm[q])] = v;
     }
     __syncthreads();
     lds_fft<N, true, INV, LB, N, 1>(buf, stw);
     for (int e = threadIdx.x; e < LB * sy; e += kFftThreads) {
         const int ln = e / sy, q = e % sy;
         const int64_t L = l0 + ln;
-        if (L < nlines_total) H[L * sy + q] = buf[ln * N + q];
+        if (L < nlines_total) H[L * sy + q] = buf[ln * N + swz<N>(q)];
     }
 }
 
@@ -396,15 +416,17 @@ __global__ void __launch_bounds__(kFftThreads) k_inv_z(Grid g, const double2 *__
             if (j < sy) v = H[((int64_t)kc * Nx + i) * sy + j];
             double2 *Z = buf + pr * N;
             const bool mirror = kc > 0 && 2 * kc < N;
+            const int s0 = swz<N>(sperm[kc]);
+            const int s1 = mirror ? swz<N>(sperm[N - kc]) : 0;
             if (phase == 0) {
-                Z[sperm[kc]] = v;
-                if (mirror) Z[sperm[N - kc]] = make_double2(v.x, -v.y);
+                Z[s0] = v;
+                if (mirror) Z[s1] = make_double2(v.x, -v.y);
             } else {
-                const double2 t = Z[sperm[kc]];
-                Z[sperm[kc]] = make_double2(t.x - v.y, t.y + v.x);
+                const double2 t = Z[s0];
+                Z[s0] = make_double2(t.x - v.y, t.y + v.x);
                 if (mirror) {
-                    const double2 u = Z[sperm[N - kc]];  // + i * conj(v)
-                    Z[sperm[N - kc]] = make_double2(u.x + v.y, u.y + v.x);
+                    const double2 u = Z[s1];  // + i * conj(v)
+                    Z[s1] = make_double2(u.x + v.y, u.y + v.x);
                 }
             }
         }
@@ -415,7 +437,7 @@ __global__ void __launch_bounds__(kFftThreads) k_inv_z(Grid g, const double2 *__
         const int jj = e / N, k = e % N;
         const int j = j0 + jj;
         if (j >= sy) continue;
-        const double2 z = buf[(jj >> 1) * N + k];
+        const double2 z = buf[(jj >> 1) * N + swz<N>(k)];
         p[lin(g, i, j, k)] = ((jj & 1) ? z.y : z.x) * inv_norm;
     }
 }
