@@ -663,21 +663,25 @@ __global__ void __launch_bounds__(kRK *kRJ) k_correct_rows(Grid g, RowTiles t, c
 
 // ---- LDS-staged streamed divergence and gradient correction ----
 //
-// A 32 x 8 (k x j) tile walks i over a chunk like the stage kernels: the
+// A 32 x 16 (k x j) tile walks i over a chunk like the stage kernels: the
 // i-neighbours of its column sit in a register queue, the j and k
 // neighbours come from LDS rows and columns staged per plane, and the loads
 // of the next two planes are in flight while a plane is combined.  Values
 // are kept as loaded and the wall reflection signs applied when they are
 // staged.  DIV: out = div(F) * scale (the arithmetic of k_div_rows);
 // otherwise F -= s grad(p) (that of k_correct_rows).
-constexpr int kPK = 32, kPJ = 8;
+// (512 threads: against 32 x 8 the loads per node of a staged plane drop
+// from 2.08 to 1.63 for p with both halos and from 1.47 to 1.28 for F_y,
+// F_z with one halo each; measured 4% faster divergence and 3% faster
+// correction at 256^3)
+constexpr int kPK = 32, kPJ = 16;
 
 template <int H, bool DIV, bool WALLS>
-__global__ void __launch_bounds__(256) k_proj_stream(Grid g, RowTiles t, const double *__restrict__ src, double sc,
+__global__ void __launch_bounds__(kPK *kPJ) k_proj_stream(Grid g, RowTiles t, const double *__restrict__ src, double sc,
                                                      double *__restrict__ dst, BatchArgs ba)
 {
-    constexpr int NH = 2 * H * kPK + kPJ * 2 * H;  // halo elements per plane (<= 256)
-    static_assert(NH <= 256, "one halo slot per thread");
+    constexpr int NH = 2 * H * kPK + kPJ * 2 * H;  // halo elements per plane
+    static_assert(NH <= kPK * kPJ, "one halo slot per thread");
     const int T = t.nkt * t.njt * t.nic;
     int bt = blockIdx.x;
     if (T % 8 == 0) bt = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
@@ -851,9 +855,11 @@ int launch_proj_stream_h(nova_ctx *ctx, const double *src, double sc, double *ds
     const dim3 grid((unsigned)(t.nkt * t.njt * t.nic), (unsigned)B);
     const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
     if (walls)
-        hipLaunchKernelGGL((k_proj_stream<H, DIV, true>), grid, dim3(256), 0, ctx->stream, g, t, src, sc, dst, ba);
+        hipLaunchKernelGGL((k_proj_stream<H, DIV, true>), grid, dim3(kPK * kPJ), 0, ctx->stream, g, t, src, sc, dst,
+                           ba);
     else
-        hipLaunchKernelGGL((k_proj_stream<H, DIV, false>), grid, dim3(256), 0, ctx->stream, g, t, src, sc, dst, ba);
+        hipLaunchKernelGGL((k_proj_stream<H, DIV, false>), grid, dim3(kPK * kPJ), 0, ctx->stream, g, t, src, sc, dst,
+                           ba);
     return check_launch(DIV ? "div_stream" : "correct_stream");
 }
 

@@ -381,16 +381,21 @@ int solver_autotune(nova_solver *sv, const double *U, const double *B, int reps,
     if (reps < 1) return fail(NOVA_ERR_ARG, "reps must be >= 1");
     std::string rep;
     // a middle stage (with the step-start operand, as two of the three
-    // stages of a step have)
+    // stages of a step have), its step-start operand a separate copy of the
+    // input so that a trial moves the bytes a step's stage moves (an
+    // operand read twice from one array would mostly hit the cache)
     const StageArgs sa = ssp(1, 1e-3);
+    const size_t vb = sizeof(double) * 3 * ctx->g.np;
+    NOVA_HIP(hipMemcpyAsync(sv->wb, U, vb, hipMemcpyDeviceToDevice, ctx->stream));
+    if (mhd) NOVA_HIP(hipMemcpyAsync(sv->wd, B, vb, hipMemcpyDeviceToDevice, ctx->stream));
     // the staging variants 9-12 stay out of the search: in a full step they
     // measure level with 7 and 8, while a single stage timed on its own
     // (warm in the last-level cache) can favour them
     const int variants[] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
     const int variants_mhd[] = {0, 1, 2, 3, 4, 11, 12};
     auto run = [&]() -> int {
-        if (mhd) return mhd_stage(ctx, U, B, U, B, sv->wa, sv->wb, sa, sv->nu, sv->eta);
-        return ns_stage(ctx, U, U, sv->wa, sa, sv->nu);
+        if (mhd) return mhd_stage(ctx, U, B, sv->wb, sv->wd, sv->wa, sv->wc, sa, sv->nu, sv->eta);
+        return ns_stage(ctx, U, sv->wb, sv->wa, sa, sv->nu);
     };
     if (int rc = tune_stage_model(ctx, mhd ? MODEL_MHD : MODEL_NS, mhd ? variants_mhd : variants, mhd ? 7 : 9, reps, run,
                                   mhd ? "mhd_stage" : "ns_stage", rep))
