@@ -538,9 +538,13 @@ def main() -> int:
     if not args.no_io:
         torch.cuda.empty_cache()
         log("bench: EnSight output ...")
-        iob = io_bench(nv, n, args.io_writes)
-        log(f"bench: EnSight pack {iob['pack_us']} us ({iob['pack_GBps']} GB/s algorithmic), "
-            f"{iob['write_MBps']} MB/s to file, {iob['call_ms']} ms per call")
+        try:
+            iob = io_bench(nv, n, args.io_writes)
+            log(f"bench: EnSight pack {iob['pack_us']} us ({iob['pack_GBps']} GB/s algorithmic), "
+                f"{iob['write_MBps']} MB/s to file, {iob['call_ms']} ms per call")
+        except nv.NovaError as e:  # a full or absent scratch disk must not cost the line
+            iob = {"error": str(e)}
+            log(f"bench: EnSight output skipped: {e}")
 
     ens = None
     if not args.no_ensemble:
