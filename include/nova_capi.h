@@ -205,7 +205,7 @@ int nova_solver_use_graph(nova_solver *sv, int enable);
  * same scheme. */
 int nova_solver_set_scheme(nova_solver *sv, int scheme);
 /* Autotuner (no reference counterpart): times every launch shape of the
- * solver's stage kernel (variants x chunks of 0, 8, 16, 32, 64 planes) and
+ * solver's stage kernel (variants x chunks of 0, 8, 16, 32, 64, 128 planes) and
  * the three projection variants on U (and B), reps launches each after one
  * untimed launch, and keeps the fastest.  U and B are not modified.  When
  * report is non-null it receives one line per trial and the choices. */

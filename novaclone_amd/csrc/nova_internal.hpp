@@ -163,7 +163,10 @@ template <class F>
 int tune_stage_model(nova_ctx *ctx, int model, const int *variants, int nvar, int reps, F &&run, const char *name,
                      std::string &rep)
 {
-    static const int kChunks[] = {0, 8, 16, 32, 64};
+    // planes per chunk (0: automatic); 128 gives a 256-plane grid two
+    // chunks, so that one-block-per-CU shapes fill the GPU in one round with
+    // half the queue fills of 64
+    static const int kChunks[] = {0, 8, 16, 32, 64, 128};
     double best = 1e300;
     int bv = ctx->stage_variant[model], bc = ctx->stage_ichunk[model];
     for (int iv = 0; iv < nvar; iv++)
