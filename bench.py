@@ -502,6 +502,23 @@ def main() -> int:
     step_alg = 384.0 * n ** 3  # minimal-traffic model of one RK3 + projection step
     step_roofline = {"alg_bytes_per_step": step_alg, "achieved_GBps": round(step_alg / (ms_per_step / 1e3) / 1e9, 1),
                      "frac": round(step_alg / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+    # the box's own ceiling: a plain device-to-device copy of 1 GiB
+    # (read + write bytes per second)
+    cx = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+    cy = torch.empty_like(cx)
+    cy.copy_(cx)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(10):
+        cy.copy_(cx)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2.0 * cx.numel() * 8 * 10 / (e0.elapsed_time(e1) / 1e3) / 1e9
+    del cx, cy
+    step_roofline["box_copy_GBps"] = round(copy_gbs, 1)
+    step_roofline["frac_of_box_copy"] = round(step_alg / (ms_per_step / 1e3) / 1e9 / copy_gbs, 4)
+    roofline["frac_of_box_copy"] = round(dk["GBps"] / copy_gbs, 4)
+    log(f"bench: device copy {copy_gbs:.0f} GB/s; step at {100 * step_roofline['frac_of_box_copy']:.1f}% of it")
 
     adj = None
     if not args.no_adjoint:
