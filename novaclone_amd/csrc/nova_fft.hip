@@ -58,7 +58,11 @@ namespace nova {
 
 namespace {
 
+// threads per block: the k passes (r2c, c2r) and the j and i passes (512
+// threads made the j passes 4 us and the i pass 8 us faster at 256^3, and
+// the c2r pass 3 us slower)
 constexpr int kFftThreads = 256;
+constexpr int kLineThreads = 512;
 constexpr double kEps = 1.0e-16;  // src/Tools/Tools.cpp:7
 
 __device__ inline double2 cmul(double2 a, double2 b)
@@ -141,7 +145,7 @@ __device__ inline int swz(int e)
 // of line l at buf[l*LS + e*ES] (at buf[l*LS + swz(e)] when ES = 1).  DIT
 // twiddles before the butterfly, DIF after it.  Consecutive threads walk
 // the index that is contiguous in LDS.
-template <int N, int S, bool DIT, bool INV, int NLINES, int LS, int ES>
+template <int N, int S, bool DIT, bool INV, int NLINES, int LS, int ES, int NT>
 __device__ inline void fft_pass(double2 *buf, const double2 *tw)
 {
     constexpr int R = plan_radix(N, S);
@@ -151,7 +155,7 @@ __device__ inline void fft_pass(double2 *buf, const double2 *tw)
     constexpr int TSTEP = N / L;
     constexpr int TOTAL = NB * NLINES;
     constexpr bool BATCH_FAST = ES != 1;
-    for (int w = threadIdx.x; w < TOTAL; w += kFftThreads) {
+    for (int w = threadIdx.x; w < TOTAL; w += NT) {
         int line, bf;
         if constexpr (BATCH_FAST) { line = w % NLINES; bf = w / NLINES; }
         else { line = w / NB; bf = w % NB; }
@@ -191,21 +195,21 @@ __device__ inline void fft_pass(double2 *buf, const double2 *tw)
 
 // All passes: DIT expects digit-reversed input and leaves natural order;
 // DIF takes natural order and leaves digit-reversed output.
-template <int N, bool DIT, bool INV, int NLINES, int LS, int ES, int IT = 0>
+template <int N, bool DIT, bool INV, int NLINES, int LS, int ES, int NT, int IT = 0>
 __device__ inline void lds_fft(double2 *buf, const double2 *tw)
 {
     constexpr int NR = plan_nr(N);
     static_assert(NR > 0, "FFT length must be 2^a 3^b");
     if constexpr (IT < NR) {
-        fft_pass<N, DIT ? IT : NR - 1 - IT, DIT, INV, NLINES, LS, ES>(buf, tw);
-        lds_fft<N, DIT, INV, NLINES, LS, ES, IT + 1>(buf, tw);
+        fft_pass<N, DIT ? IT : NR - 1 - IT, DIT, INV, NLINES, LS, ES, NT>(buf, tw);
+        lds_fft<N, DIT, INV, NLINES, LS, ES, NT, IT + 1>(buf, tw);
     }
 }
 
-template <int N>
+template <int N, int NT>
 __device__ inline void load_tables(double2 *stw, int *sperm, const double2 *tw, const int *perm)
 {
-    for (int e = threadIdx.x; e < N; e += kFftThreads) {
+    for (int e = threadIdx.x; e < N; e += NT) {
         stw[e] = tw[e];
         sperm[e] = perm[e];
     }
@@ -279,7 +283,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fwd_z(Grid g, const double2 *__
     __shared__ double2 stw[N];
     __shared__ int sperm[N];
     __shared__ double2 buf[(ZL / 2) * N];
-    load_tables<N>(stw, sperm, tw, perm);
+    load_tables<N, kFftThreads>(stw, sperm, tw, perm);
     const int i = blockIdx.y;
     const int j0 = blockIdx.x * ZL;
     for (int e = threadIdx.x; e < ZL * N; e += kFftThreads) {
@@ -296,7 +300,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fwd_z(Grid g, const double2 *__
         slot[jj & 1] = v;
     }
     __syncthreads();
-    lds_fft<N, true, false, ZL / 2, N, 1>(buf, stw);
+    lds_fft<N, true, false, ZL / 2, N, 1, kFftThreads>(buf, stw);
     constexpr int Kc = N / 2 + 1;
     for (int e = threadIdx.x; e < ZL * Kc; e += kFftThreads) {
         const int jj = e % ZL, kc = e / ZL;
@@ -314,17 +318,17 @@ __global__ void __launch_bounds__(kFftThreads) k_fwd_z(Grid g, const double2 *__
 // passes 2 and 4: lines along j (contiguous).  EXT = even extension of the
 // sy = N_y + 1 stored nodes to length N = 2 N_y (DCT-I through the FFT).
 template <int N, bool EXT, bool INV>
-__global__ void __launch_bounds__(kFftThreads) k_fft_y(const double2 *__restrict__ tw, const int *__restrict__ perm,
+__global__ void __launch_bounds__(kLineThreads) k_fft_y(const double2 *__restrict__ tw, const int *__restrict__ perm,
                                                        double2 *__restrict__ H, int64_t nlines_total, int sy)
 {
     constexpr int LB = ylines<N>();
     __shared__ double2 stw[N];
     __shared__ int sperm[N];
     __shared__ double2 buf[LB * N];
-    load_tables<N>(stw, sperm, tw, perm);
+    load_tables<N, kLineThreads>(stw, sperm, tw, perm);
     const int64_t l0 = (int64_t)blockIdx.x * LB;
     constexpr int Ny = EXT ? N / 2 : N;
-    for (int e = threadIdx.x; e < LB * N; e += kFftThreads) {
+    for (int e = threadIdx.x; e < LB * N; e += kLineThreads) {
         const int ln = e / N, q = e % N;
         const int64_t L = l0 + ln;
         double2 v = make_double2(0.0, 0.0);
@@ -335,8 +339,8 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_y(const double2 *__restrict
         buf[ln * N + swz<N>(sperm[q])] = v;
     }
     __syncthreads();
-    lds_fft<N, true, INV, LB, N, 1>(buf, stw);
-    for (int e = threadIdx.x; e < LB * sy; e += kFftThreads) {
+    lds_fft<N, true, INV, LB, N, 1, kLineThreads>(buf, stw);
+    for (int e = threadIdx.x; e < LB * sy; e += kLineThreads) {
         const int ln = e / sy, q = e % sy;
         const int64_t L = l0 + ln;
         if (L < nlines_total) H[L * sy + q] = buf[ln * N + swz<N>(q)];
@@ -346,7 +350,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_y(const double2 *__restrict
 // pass 3: lines along i, a batch of consecutive j; forward, divide by
 // -k'^2, inverse.
 template <int N>
-__global__ void __launch_bounds__(kFftThreads) k_fft_x(const double2 *__restrict__ tw, const int *__restrict__ perm,
+__global__ void __launch_bounds__(kLineThreads) k_fft_x(const double2 *__restrict__ tw, const int *__restrict__ perm,
                                                        double2 *__restrict__ H, int sy,
                                                        const double *__restrict__ kpx, const double *__restrict__ kpy,
                                                        const double *__restrict__ kpz, int Kc)
@@ -355,12 +359,12 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x(const double2 *__restrict
     __shared__ double2 stw[N];
     __shared__ int sperm[N];
     __shared__ double2 buf[N * JB];
-    load_tables<N>(stw, sperm, tw, perm);
+    load_tables<N, kLineThreads>(stw, sperm, tw, perm);
     // blockIdx.y runs over (batch member, kc) of the [b][kc][i][j] layout
     const int kc = blockIdx.y % Kc;
     const int j0 = blockIdx.x * JB;
     double2 *Hs = H + (int64_t)blockIdx.y * N * sy;
-    for (int e = threadIdx.x; e < N * JB; e += kFftThreads) {
+    for (int e = threadIdx.x; e < N * JB; e += kLineThreads) {
         const int jj = e % JB, i = e / JB;
         const int j = j0 + jj;
         double2 v = make_double2(0.0, 0.0);
@@ -368,9 +372,9 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x(const double2 *__restrict
         buf[sperm[i] * JB + jj] = v;
     }
     __syncthreads();
-    lds_fft<N, true, false, JB, 1, JB>(buf, stw);
+    lds_fft<N, true, false, JB, 1, JB, kLineThreads>(buf, stw);
     const double kz = kpz[kc];
-    for (int e = threadIdx.x; e < N * JB; e += kFftThreads) {
+    for (int e = threadIdx.x; e < N * JB; e += kLineThreads) {
         const int jj = e % JB, i = e / JB;
         const int j = j0 + jj;
         if (j >= sy) continue;
@@ -386,8 +390,8 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x(const double2 *__restrict
         buf[i * JB + jj] = v;
     }
     __syncthreads();
-    lds_fft<N, false, true, JB, 1, JB>(buf, stw);
-    for (int e = threadIdx.x; e < N * JB; e += kFftThreads) {
+    lds_fft<N, false, true, JB, 1, JB, kLineThreads>(buf, stw);
+    for (int e = threadIdx.x; e < N * JB; e += kLineThreads) {
         const int jj = e % JB, i = e / JB;
         const int j = j0 + jj;
         if (j < sy) Hs[(int64_t)i * sy + j] = buf[sperm[i] * JB + jj];
@@ -407,7 +411,7 @@ __global__ void __launch_bounds__(kFftThreads) k_inv_z(Grid g, const double2 *__
     __shared__ double2 stw[N];
     __shared__ int sperm[N];
     __shared__ double2 buf[(ZL / 2) * N];
-    load_tables<N>(stw, sperm, tw, perm);
+    load_tables<N, kFftThreads>(stw, sperm, tw, perm);
     const int i = blockIdx.y;
     const int j0 = blockIdx.x * ZL;
     constexpr int Kc = N / 2 + 1;
@@ -437,7 +441,7 @@ __global__ void __launch_bounds__(kFftThreads) k_inv_z(Grid g, const double2 *__
         }
         __syncthreads();
     }
-    lds_fft<N, true, true, ZL / 2, N, 1>(buf, stw);
+    lds_fft<N, true, true, ZL / 2, N, 1, kFftThreads>(buf, stw);
     for (int e = threadIdx.x; e < ZL * N; e += kFftThreads) {
         const int jj = e / N, k = e % N;
         const int j = j0 + jj;
@@ -982,10 +986,10 @@ int launch_fft_y(hipStream_t st, int n, bool ext, bool inv, const double2 *tw, c
 #define CASE(L)                                                                                   \
     case L: {                                                                                     \
         const unsigned blocks = (unsigned)((nlines + ylines<L>() - 1) / ylines<L>());             \
-        if (ext && inv) hipLaunchKernelGGL((k_fft_y<L, true, true>), dim3(blocks), dim3(kFftThreads), 0, st, tw, perm, H, nlines, sy);   \
-        else if (ext) hipLaunchKernelGGL((k_fft_y<L, true, false>), dim3(blocks), dim3(kFftThreads), 0, st, tw, perm, H, nlines, sy);   \
-        else if (inv) hipLaunchKernelGGL((k_fft_y<L, false, true>), dim3(blocks), dim3(kFftThreads), 0, st, tw, perm, H, nlines, sy);   \
-        else hipLaunchKernelGGL((k_fft_y<L, false, false>), dim3(blocks), dim3(kFftThreads), 0, st, tw, perm, H, nlines, sy);          \
+        if (ext && inv) hipLaunchKernelGGL((k_fft_y<L, true, true>), dim3(blocks), dim3(kLineThreads), 0, st, tw, perm, H, nlines, sy);   \
+        else if (ext) hipLaunchKernelGGL((k_fft_y<L, true, false>), dim3(blocks), dim3(kLineThreads), 0, st, tw, perm, H, nlines, sy);   \
+        else if (inv) hipLaunchKernelGGL((k_fft_y<L, false, true>), dim3(blocks), dim3(kLineThreads), 0, st, tw, perm, H, nlines, sy);   \
+        else hipLaunchKernelGGL((k_fft_y<L, false, false>), dim3(blocks), dim3(kLineThreads), 0, st, tw, perm, H, nlines, sy);          \
         return NOVA_OK;                                                                           \
     }
         NOVA_FFT_LENGTHS(CASE)
@@ -1001,7 +1005,7 @@ int launch_fft_x(hipStream_t st, int n, int Kc, int B, const double2 *tw, const 
 #define CASE(L)                                                                                           \
     case L: {                                                                                             \
         dim3 grid((sy + xbatch_t<L>() - 1) / xbatch_t<L>(), Kc * B);                                       \
-        hipLaunchKernelGGL((k_fft_x<L>), grid, dim3(kFftThreads), 0, st, tw, perm, H, sy, kpx, kpy, kpz, Kc); \
+        hipLaunchKernelGGL((k_fft_x<L>), grid, dim3(kLineThreads), 0, st, tw, perm, H, sy, kpx, kpy, kpz, Kc); \
         return NOVA_OK;                                                                                   \
     }
         NOVA_FFT_LENGTHS(CASE)
