@@ -263,6 +263,12 @@ __device__ inline double div_at(const Grid &g, const double *__restrict__ F, int
 // Real lines per block in the k passes.
 template <int N>
 constexpr int zlines() { return (N >= 128 && N <= 256) ? 8 : (N <= 384 ? 16 : (N <= 512 ? 8 : 4)); }
+// The r2c pass at 256: 16 real lines on 512 threads (68.7 against 72.1 us;
+// the c2r pass measured 10 us slower so, and keeps 8 lines on 256).
+template <int N>
+constexpr int zlines_fwd() { return N == 256 ? 16 : zlines<N>(); }
+template <int N>
+constexpr int zthreads_fwd() { return N == 256 ? 512 : kFftThreads; }
 // Lines per block along j, and batch width along i.
 template <int N>
 constexpr int ylines() { return N >= 2048 ? 1 : (2048 / N > 16 ? 16 : 2048 / N); }
@@ -271,22 +277,23 @@ constexpr int xbatch_t() { return N >= 2048 ? 1 : (2048 / N > 8 ? 8 : 2048 / N);
 
 // pass 1: source -> half spectrum H[kc][i][j].
 template <int N, bool WALLS>
-__global__ void __launch_bounds__(kFftThreads) k_fwd_z(Grid g, const double2 *__restrict__ tw,
+__global__ void __launch_bounds__(zthreads_fwd<N>()) k_fwd_z(Grid g, const double2 *__restrict__ tw,
                                                        const int *__restrict__ perm, const double *__restrict__ src,
                                                        int src_is_vector, double scale, double2 *__restrict__ H,
                                                        int Nx, int sy, int64_t src_bs)
 {
-    constexpr int ZL = zlines<N>();
+    constexpr int ZL = zlines_fwd<N>();
+    constexpr int NT = zthreads_fwd<N>();
     // batch member blockIdx.z: its source and half spectrum
     src += blockIdx.z * src_bs;
     H += (int64_t)blockIdx.z * (N / 2 + 1) * Nx * sy;
     __shared__ double2 stw[N];
     __shared__ int sperm[N];
     __shared__ double2 buf[(ZL / 2) * N];
-    load_tables<N, kFftThreads>(stw, sperm, tw, perm);
+    load_tables<N, NT>(stw, sperm, tw, perm);
     const int i = blockIdx.y;
     const int j0 = blockIdx.x * ZL;
-    for (int e = threadIdx.x; e < ZL * N; e += kFftThreads) {
+    for (int e = threadIdx.x; e < ZL * N; e += NT) {
         const int jj = e / N, k = e % N;
         const int j = j0 + jj;
         double v = 0.0;
@@ -300,9 +307,9 @@ __global__ void __launch_bounds__(kFftThreads) k_fwd_z(Grid g, const double2 *__
         slot[jj & 1] = v;
     }
     __syncthreads();
-    lds_fft<N, true, false, ZL / 2, N, 1, kFftThreads>(buf, stw);
+    lds_fft<N, true, false, ZL / 2, N, 1, NT>(buf, stw);
     constexpr int Kc = N / 2 + 1;
-    for (int e = threadIdx.x; e < ZL * Kc; e += kFftThreads) {
+    for (int e = threadIdx.x; e < ZL * Kc; e += NT) {
         const int jj = e % ZL, kc = e / ZL;
         const int j = j0 + jj;
         if (j >= sy) continue;
@@ -968,10 +975,10 @@ int launch_fwd_z(hipStream_t st, int n, dim3 grid, const Grid &g, const double2 
     switch (n) {
 #define CASE(L)                                                                                                    \
     case L:                                                                                                        \
-        if (walls) hipLaunchKernelGGL((k_fwd_z<L, true>), grid, dim3(kFftThreads), 0, st, g, tw, perm, src, vec,   \
-                                      scale, H, Nx, sy, src_bs);                                                   \
-        else hipLaunchKernelGGL((k_fwd_z<L, false>), grid, dim3(kFftThreads), 0, st, g, tw, perm, src, vec, scale, \
-                                H, Nx, sy, src_bs);                                                                \
+        if (walls) hipLaunchKernelGGL((k_fwd_z<L, true>), grid, dim3(zthreads_fwd<L>()), 0, st, g, tw, perm, src,  \
+                                      vec, scale, H, Nx, sy, src_bs);                                              \
+        else hipLaunchKernelGGL((k_fwd_z<L, false>), grid, dim3(zthreads_fwd<L>()), 0, st, g, tw, perm, src, vec,  \
+                                scale, H, Nx, sy, src_bs);                                                         \
         return NOVA_OK;
         NOVA_FFT_LENGTHS(CASE)
 #undef CASE
@@ -1032,6 +1039,16 @@ int zlines_rt(int n)
 {
     switch (n) {
 #define CASE(L) case L: return zlines<L>();
+        NOVA_FFT_LENGTHS(CASE)
+#undef CASE
+    }
+    return 16;
+}
+
+int zlines_fwd_rt(int n)
+{
+    switch (n) {
+#define CASE(L) case L: return zlines_fwd<L>();
         NOVA_FFT_LENGTHS(CASE)
 #undef CASE
     }
@@ -1212,8 +1229,8 @@ static int run_pipeline(nova_poisson *ps, const double *src, int src_is_vector, 
     const Grid &g = ctx->g;
     hipStream_t st = ctx->stream;
     const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
-    const int zl = zlines_rt(ps->Nz);
-    const dim3 zgrid((ps->sy + zl - 1) / zl, ps->Nx, B);
+    const int zl = zlines_rt(ps->Nz), zlf = zlines_fwd_rt(ps->Nz);
+    const dim3 zgrid((ps->sy + zl - 1) / zl, ps->Nx, B), zgrid_fwd((ps->sy + zlf - 1) / zlf, ps->Nx, B);
     const int64_t ylines_total = (int64_t)ps->Kc * ps->Nx * B;
     int64_t src_bs = ba.f_bs;
     int rc;
@@ -1232,7 +1249,7 @@ static int run_pipeline(nova_poisson *ps, const double *src, int src_is_vector, 
     }
     {
         TimedScope ts(ctx, src_is_vector ? "fft_div_r2c_z" : "fft_r2c_z");
-        rc = launch_fwd_z(st, ps->Nz, zgrid, g, ps->fz.tw, ps->fz.perm, src, src_is_vector, scale, ps->H, ps->Nx,
+        rc = launch_fwd_z(st, ps->Nz, zgrid_fwd, g, ps->fz.tw, ps->fz.perm, src, src_is_vector, scale, ps->H, ps->Nx,
                           ps->sy, walls, src_bs);
     }
     if (rc || (rc = check_launch("poisson fwd_z"))) return rc;
