@@ -422,32 +422,21 @@ __global__ void __launch_bounds__(kFftThreads) k_inv_z(Grid g, const double2 *__
     const int i = blockIdx.y;
     const int j0 = blockIdx.x * ZL;
     constexpr int Kc = N / 2 + 1;
-    // even j: A terms; odd j: i*B terms (Z = A + iB, Hermitian-extended)
-#pragma unroll
-    for (int phase = 0; phase < 2; phase++) {
-        for (int e = threadIdx.x; e < (ZL / 2) * Kc; e += kFftThreads) {
-            const int pr = e % (ZL / 2), kc = e / (ZL / 2);
-            const int j = j0 + 2 * pr + phase;
-            double2 v = make_double2(0.0, 0.0);
-            if (j < sy) v = H[((int64_t)kc * Nx + i) * sy + j];
-            double2 *Z = buf + pr * N;
-            const bool mirror = kc > 0 && 2 * kc < N;
-            const int s0 = swz<N>(sperm[kc]);
-            const int s1 = mirror ? swz<N>(sperm[N - kc]) : 0;
-            if (phase == 0) {
-                Z[s0] = v;
-                if (mirror) Z[s1] = make_double2(v.x, -v.y);
-            } else {
-                const double2 t = Z[s0];
-                Z[s0] = make_double2(t.x - v.y, t.y + v.x);
-                if (mirror) {
-                    const double2 u = Z[s1];  // + i * conj(v)
-                    Z[s1] = make_double2(u.x + v.y, u.y + v.x);
-                }
-            }
-        }
-        __syncthreads();
+    // Z = A + iB, Hermitian-extended: A from the even j, B from the odd j of
+    // a pair; a lane reads both (32 contiguous bytes)
+    for (int e = threadIdx.x; e < (ZL / 2) * Kc; e += kFftThreads) {
+        const int pr = e % (ZL / 2), kc = e / (ZL / 2);
+        const int j = j0 + 2 * pr;
+        const int64_t row = ((int64_t)kc * Nx + i) * sy;
+        double2 v0 = make_double2(0.0, 0.0), v1 = make_double2(0.0, 0.0);
+        if (j < sy) v0 = H[row + j];
+        if (j + 1 < sy) v1 = H[row + j + 1];
+        double2 *Z = buf + pr * N;
+        const bool mirror = kc > 0 && 2 * kc < N;
+        Z[swz<N>(sperm[kc])] = make_double2(v0.x - v1.y, v0.y + v1.x);
+        if (mirror) Z[swz<N>(sperm[N - kc])] = make_double2(v0.x + v1.y, -v0.y + v1.x);  // + i conj(v1)
     }
+    __syncthreads();
     lds_fft<N, true, true, ZL / 2, N, 1, kFftThreads>(buf, stw);
     for (int e = threadIdx.x; e < ZL * N; e += kFftThreads) {
         const int jj = e / N, k = e % N;
