@@ -125,7 +125,8 @@ enum {
     NOVA_KERNEL_NS_STAGE = 0,
     NOVA_KERNEL_MHD_STAGE = 1,
     NOVA_KERNEL_TLM_STAGE = 2,
-    NOVA_KERNEL_ADJ_STAGE = 3
+    NOVA_KERNEL_ADJ_STAGE = 3,
+    NOVA_KERNEL_NS_FIRST_STAGE = 4  /* the first SSP-RK3 stage of the sixth-order NS step */
 };
 /* Launch shape of one stage kernel (variants as for nova_ctx_tune). */
 int nova_ctx_tune_kernel(nova_ctx *ctx, int kernel, int stage_variant, int stage_ichunk);

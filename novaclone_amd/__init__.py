@@ -26,7 +26,7 @@ NOVA_OK = 0
 BC_PERIODIC, BC_WALL = 0, 1
 RK_LITERAL4, RK_SSP3, RK_SHEAR4 = 0, 1, 2
 MODEL_NS, MODEL_MHD = 0, 1
-KERNEL_NS_STAGE, KERNEL_MHD_STAGE, KERNEL_TLM_STAGE, KERNEL_ADJ_STAGE = 0, 1, 2, 3
+KERNEL_NS_STAGE, KERNEL_MHD_STAGE, KERNEL_TLM_STAGE, KERNEL_ADJ_STAGE, KERNEL_NS_FIRST_STAGE = 0, 1, 2, 3, 4
 SCHEME_EXPLICIT, SCHEME_COMPACT6 = 0, 1
 # reference BC kinds for the padded-layout fill (src/BC/BCenums.h:8)
 PBC_PERIODIC, PBC_NEUMANN, PBC_DIRICHLET, PBC_EVEN, PBC_ODD = 0, 1, 2, 10, 11

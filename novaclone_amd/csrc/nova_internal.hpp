@@ -84,8 +84,10 @@ struct nova_ctx {
     int timing = 0;                        // record events around launches
     // launch shape of the stage kernels per model (NS, MHD, tangent-linear,
     // adjoint): variant 0..4 and planes per streamed chunk (0: automatic)
-    int stage_variant[4] = {3, 3, 3, 3};
-    int stage_ichunk[4] = {0, 0, 0, 0};
+    // (slot 4: the first SSP stage of the sixth-order NS step, which reads
+    // no step-start operand and has its own best shape)
+    int stage_variant[5] = {3, 3, 3, 3, 3};
+    int stage_ichunk[5] = {0, 0, 0, 0, 0};
     int tune_epoch = 0;                    // bumped on every change of the launch shapes
     int proj_variant = 2;                  // 0: divergence fused into the r2c pass, 1: row kernels, 2: LDS tiles
     std::vector<nova::TimedLaunch> tlog;   // launches since the last collect

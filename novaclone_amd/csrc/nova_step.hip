@@ -147,10 +147,17 @@ int launch_stream_h(nova_ctx *ctx, StreamArgs<NF> a, const char *name)
     const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
     TimedScope ts(ctx, name);
     constexpr int MODEL = NF == 1 ? MODEL_NS : MODEL_MHD;
-    if (ctx->stage_variant[MODEL] == 1) {
+    // sixth-order NS stages compile the SSP combination in (the MHD stages,
+    // at one wave per SIMD, spill when their branches go); the first stage
+    // (combination 2) has a launch-shape slot of its own
+    const int cmb = (H == 3 && MODEL == MODEL_NS && a.sa.form == FORM_SSP)
+                        ? (a.sa.alpha == 0.0 ? 2 : (a.X0[0] ? 1 : 0))
+                        : 0;
+    const int slot = cmb == 2 ? 4 : MODEL;
+    if (ctx->stage_variant[slot] == 1) {
         a.nkt = (g.s[2] + kTK - 1) / kTK;
         a.njt = (g.s[1] + kTJ - 1) / kTJ;
-        const int ic = ctx->stage_ichunk[MODEL];
+        const int ic = ctx->stage_ichunk[slot];
         a.ichunk = ic > 0 ? (ic < g.s[0] ? ic : g.s[0]) : pick_ichunk(g, a.nkt, a.njt);
         a.nic = (g.s[0] + a.ichunk - 1) / a.ichunk;
         const int T = a.nkt * a.njt * a.nic;
@@ -159,9 +166,6 @@ int launch_stream_h(nova_ctx *ctx, StreamArgs<NF> a, const char *name)
         else
             hipLaunchKernelGGL((k_stage_rows<H, NF, false>), dim3(T), dim3(kTK, kTJ, 1), 0, ctx->stream, g, a);
     } else if constexpr (H == 3 && MODEL == MODEL_NS) {
-        // sixth-order NS stages compile the SSP combination in (the MHD
-        // stages, at one wave per SIMD, spill when their branches go)
-        const int cmb = a.sa.form != FORM_SSP ? 0 : (a.sa.alpha == 0.0 ? 2 : (a.X0[0] ? 1 : 0));
         if (walls) {
             if (cmb == 1) launch_stream_w<H, MODEL, true, 1>(ctx, a);
             else if (cmb == 2) launch_stream_w<H, MODEL, true, 2>(ctx, a);
@@ -400,6 +404,13 @@ int solver_autotune(nova_solver *sv, const double *U, const double *B, int reps,
     if (int rc = tune_stage_model(ctx, mhd ? MODEL_MHD : MODEL_NS, mhd ? variants_mhd : variants, mhd ? 7 : 9, reps, run,
                                   mhd ? "mhd_stage" : "ns_stage", rep))
         return rc;
+    if (!mhd && ctx->g.h == 3) {
+        // the first stage (no step-start operand) in its own slot; the
+        // staging variants 11 and 12 join the search here
+        const int first[] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 11, 12};
+        auto run_first = [&]() -> int { return ns_stage(ctx, U, nullptr, sv->wa, ssp(0, 1e-3), sv->nu); };
+        if (int rc = tune_stage_model(ctx, 4, first, 11, reps, run_first, "ns_first_stage", rep)) return rc;
+    }
     // projection variants, on a copy of U
     NOVA_HIP(hipMemcpyAsync(sv->wa, U, sizeof(double) * 3 * ctx->g.np, hipMemcpyDeviceToDevice, ctx->stream));
     double best = 1e300;

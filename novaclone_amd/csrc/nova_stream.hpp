@@ -426,6 +426,14 @@ k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
     stage_tile<H, MODEL, WALLS, kTK, MINW <= 2 ? 2 : 1, kTJ, CMB, STG>(g, a, t, sh);
 }
 
+// Launch-shape slot of a stage: the model, or slot 4 for the first SSP
+// stage of the NS model (compiled-in combination 2)
+template <int MODEL, int CMB>
+constexpr int stage_slot()
+{
+    return MODEL == MODEL_NS && CMB == 2 ? 4 : MODEL;
+}
+
 int pick_ichunk(const Grid &g, int nkt, int njt)
 {
     // about 2048 tiles (8 blocks per CU on 256 CUs), chunks of 8..64 planes
@@ -441,7 +449,7 @@ void launch_stream_tile(nova_ctx *ctx, StreamArgs<model_nf<MODEL>()> a)
     const Grid &g = ctx->g;
     a.nkt = (g.s[2] + TK - 1) / TK;
     a.njt = (g.s[1] + TJ - 1) / TJ;
-    const int ic = ctx->stage_ichunk[MODEL];
+    const int ic = ctx->stage_ichunk[stage_slot<MODEL, CMB>()];
     a.ichunk = ic > 0 ? (ic < g.s[0] ? ic : g.s[0]) : pick_ichunk(g, a.nkt, a.njt);
     a.nic = (g.s[0] + a.ichunk - 1) / a.ichunk;
     const int T = a.nkt * a.njt * a.nic;
@@ -456,7 +464,7 @@ void launch_stream_w(nova_ctx *ctx, const StreamArgs<model_nf<MODEL>()> &a)
     // fields need more than 256 registers and run unspilled at one wave per
     // SIMD (spilled shapes measure 3-4x slower)
     constexpr int W = model_nf<MODEL>() == 1 ? 3 : 1;
-    switch (ctx->stage_variant[MODEL]) {
+    switch (ctx->stage_variant[stage_slot<MODEL, CMB>()]) {
     case 0: launch_stream_tile<H, MODEL, WALLS, CMB, 1, 64>(ctx, a); break;
     case 2: launch_stream_tile<H, MODEL, WALLS, CMB, W, 64>(ctx, a); break;
     case 4: launch_stream_tile<H, MODEL, WALLS, CMB, W, 16>(ctx, a); break;

@@ -60,6 +60,27 @@ def test_every_stage_variant_gives_the_same_step(nv, wall):
                 assert torch.equal(U, ref)
 
 
+def test_first_stage_slot_gives_the_same_step(nv):
+    # the first SSP stage takes its launch shape from a slot of its own
+    import torch
+    G = nv.Mesh((24, 12, 32), (TWO_PI,) * 3)
+    P = nv.Poisson(G)
+    S = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-2)
+    gen = torch.Generator(device="cuda").manual_seed(9)
+    U0 = torch.randn((3, *G.shape), generator=gen, dtype=torch.float64, device="cuda")
+    P.project(U0, 1.0)
+    G.tune(8, 0, 2)
+    ref = U0.clone()
+    S.step(1e-3, 2, ref, G.field(1))
+    for v, c in ((11, 16), (1, 0), (5, 128), (0, 8)):
+        G.tune_kernel(nv.KERNEL_NS_FIRST_STAGE, v, c)
+        U = U0.clone()
+        S.step(1e-3, 2, U, G.field(1))
+        assert torch.equal(U, ref), (v, c)
+    with pytest.raises(nv.NovaError):
+        G.tune_kernel(5, 3, 0)
+
+
 @pytest.mark.parametrize("wall", [False, True])
 def test_two_field_stage_variants_give_the_same_results(nv, wall):
     # MHD, tangent-linear and adjoint stages under every launch shape they
@@ -110,6 +131,7 @@ def test_solver_autotune_keeps_results_and_rerecords_graph(nv):
     print(rep, flush=True)
     assert "ns_stage chosen variant" in rep and "projection chosen variant" in rep
     assert rep.count("ns_stage variant") == 54
+    assert "ns_first_stage chosen variant" in rep and rep.count("ns_first_stage variant") == 66
     Se.step(2e-3, 3, Ue, p)
     Sg.step(2e-3, 3, Ug, p)  # the graph is recorded again with the tuned shapes
     torch.cuda.synchronize()
