@@ -293,18 +293,35 @@ __global__ void __launch_bounds__(zthreads_fwd<N>()) k_fwd_z(Grid g, const doubl
     load_tables<N, NT>(stw, sperm, tw, perm);
     const int i = blockIdx.y;
     const int j0 = blockIdx.x * ZL;
-    for (int e = threadIdx.x; e < ZL * N; e += NT) {
-        const int jj = e / N, k = e % N;
-        const int j = j0 + jj;
-        double v = 0.0;
-        // the scalar source is read once: a non-temporal load leaves the
-        // last-level cache to the half spectrum written here, which the j
-        // pass reads next (that pass 69 -> 58 us at 256^3)
-        if (j < sy)
-            v = src_is_vector ? div_at<WALLS>(g, src, i, j, k, scale)
-                              : __builtin_nontemporal_load(&src[lin(g, i, j, k)]) * scale;
-        double *slot = (double *)&buf[(jj >> 1) * N + swz<N>(sperm[k])];
-        slot[jj & 1] = v;
+    if (src_is_vector) {
+        for (int e = threadIdx.x; e < ZL * N; e += NT) {
+            const int jj = e / N, k = e % N;
+            const int j = j0 + jj;
+            double *slot = (double *)&buf[(jj >> 1) * N + swz<N>(sperm[k])];
+            slot[jj & 1] = j < sy ? div_at<WALLS>(g, src, i, j, k, scale) : 0.0;
+        }
+    } else {
+        // A lane issues all its loads before its first LDS write: a loop
+        // that writes each value as it lands keeps one load per wave in
+        // flight, and the pass waits on memory latency.  The scalar source
+        // is read once: a non-temporal load leaves the last-level cache to
+        // the half spectrum written here, which the j pass reads next.
+        // The loads are unconditional (rows past the grid read its last
+        // row, zeroed below), so that no branch separates them.
+        constexpr int PER = (ZL * N + NT - 1) / NT;
+        double v[PER];
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int e = threadIdx.x + r * NT;
+            const int j = j0 + e / N;
+            v[r] = __builtin_nontemporal_load(&src[lin(g, i, j < sy ? j : sy - 1, e % N)]);
+        }
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int e = threadIdx.x + r * NT;
+            const int jj = e / N, k = e % N;
+            if (e < ZL * N) ((double *)&buf[(jj >> 1) * N + swz<N>(sperm[k])])[jj & 1] = j0 + jj < sy ? v[r] * scale : 0.0;
+        }
     }
     __syncthreads();
     lds_fft<N, true, false, ZL / 2, N, 1, NT>(buf, stw);
@@ -335,15 +352,24 @@ __global__ void __launch_bounds__(kLineThreads) k_fft_y(const double2 *__restric
     load_tables<N, kLineThreads>(stw, sperm, tw, perm);
     const int64_t l0 = (int64_t)blockIdx.x * LB;
     constexpr int Ny = EXT ? N / 2 : N;
-    for (int e = threadIdx.x; e < LB * N; e += kLineThreads) {
+    // all of a lane's loads in flight before its LDS writes
+    constexpr int PER = (LB * N + kLineThreads - 1) / kLineThreads;
+    double2 v[PER];
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        const int e = threadIdx.x + r * kLineThreads;
         const int ln = e / N, q = e % N;
         const int64_t L = l0 + ln;
-        double2 v = make_double2(0.0, 0.0);
-        if (L < nlines_total) {
+        v[r] = make_double2(0.0, 0.0);
+        if (e < LB * N && L < nlines_total) {
             const int src = (EXT && q > Ny) ? 2 * Ny - q : q;
-            v = H[L * sy + src];
+            v[r] = H[L * sy + src];
         }
-        buf[ln * N + swz<N>(sperm[q])] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        const int e = threadIdx.x + r * kLineThreads;
+        if (e < LB * N) buf[(e / N) * N + swz<N>(sperm[e % N])] = v[r];
     }
     __syncthreads();
     lds_fft<N, true, INV, LB, N, 1, kLineThreads>(buf, stw);
@@ -371,12 +397,21 @@ __global__ void __launch_bounds__(kLineThreads) k_fft_x(const double2 *__restric
     const int kc = blockIdx.y % Kc;
     const int j0 = blockIdx.x * JB;
     double2 *Hs = H + (int64_t)blockIdx.y * N * sy;
-    for (int e = threadIdx.x; e < N * JB; e += kLineThreads) {
+    // all of a lane's loads in flight before its LDS writes
+    constexpr int PER = (N * JB + kLineThreads - 1) / kLineThreads;
+    double2 v[PER];
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        const int e = threadIdx.x + r * kLineThreads;
         const int jj = e % JB, i = e / JB;
         const int j = j0 + jj;
-        double2 v = make_double2(0.0, 0.0);
-        if (j < sy) v = Hs[(int64_t)i * sy + j];
-        buf[sperm[i] * JB + jj] = v;
+        v[r] = make_double2(0.0, 0.0);
+        if (e < N * JB && j < sy) v[r] = Hs[(int64_t)i * sy + j];
+    }
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        const int e = threadIdx.x + r * kLineThreads;
+        if (e < N * JB) buf[sperm[e / JB] * JB + e % JB] = v[r];
     }
     __syncthreads();
     lds_fft<N, true, false, JB, 1, JB, kLineThreads>(buf, stw);
@@ -424,17 +459,28 @@ __global__ void __launch_bounds__(kFftThreads) k_inv_z(Grid g, const double2 *__
     constexpr int Kc = N / 2 + 1;
     // Z = A + iB, Hermitian-extended: A from the even j, B from the odd j of
     // a pair; a lane reads both (32 contiguous bytes)
-    for (int e = threadIdx.x; e < (ZL / 2) * Kc; e += kFftThreads) {
+    // (all of a lane's loads in flight before its LDS writes)
+    constexpr int PER = ((ZL / 2) * Kc + kFftThreads - 1) / kFftThreads;
+    double2 v0[PER], v1[PER];
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        const int e = threadIdx.x + r * kFftThreads;
         const int pr = e % (ZL / 2), kc = e / (ZL / 2);
         const int j = j0 + 2 * pr;
         const int64_t row = ((int64_t)kc * Nx + i) * sy;
-        double2 v0 = make_double2(0.0, 0.0), v1 = make_double2(0.0, 0.0);
-        if (j < sy) v0 = H[row + j];
-        if (j + 1 < sy) v1 = H[row + j + 1];
+        v0[r] = v1[r] = make_double2(0.0, 0.0);
+        if (e < (ZL / 2) * Kc && j < sy) v0[r] = H[row + j];
+        if (e < (ZL / 2) * Kc && j + 1 < sy) v1[r] = H[row + j + 1];
+    }
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        const int e = threadIdx.x + r * kFftThreads;
+        const int pr = e % (ZL / 2), kc = e / (ZL / 2);
+        if (e >= (ZL / 2) * Kc) continue;
         double2 *Z = buf + pr * N;
         const bool mirror = kc > 0 && 2 * kc < N;
-        Z[swz<N>(sperm[kc])] = make_double2(v0.x - v1.y, v0.y + v1.x);
-        if (mirror) Z[swz<N>(sperm[N - kc])] = make_double2(v0.x + v1.y, -v0.y + v1.x);  // + i conj(v1)
+        Z[swz<N>(sperm[kc])] = make_double2(v0[r].x - v1[r].y, v0[r].y + v1[r].x);
+        if (mirror) Z[swz<N>(sperm[N - kc])] = make_double2(v0[r].x + v1[r].y, -v0[r].y + v1[r].x);  // + i conj(v1)
     }
     __syncthreads();
     lds_fft<N, true, true, ZL / 2, N, 1, kFftThreads>(buf, stw);
