@@ -408,6 +408,18 @@ __global__ void __launch_bounds__(kLineThreads) k_fft_x(const double2 *__restric
         v[r] = make_double2(0.0, 0.0);
         if (e < N * JB && j < sy) v[r] = Hs[(int64_t)i * sy + j];
     }
+    const double kz = kpz[kc];
+    // the wavenumbers of a lane's elements, loaded with the
+    // spectrum, so that they land during the forward transform (clamped
+    // indices past the end are never used)
+    double kxr[PER], kyr[PER];
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        const int e = threadIdx.x + r * kLineThreads;
+        const int j = j0 + e % JB;
+        kxr[r] = kpx[e < N * JB ? e / JB : 0];
+        kyr[r] = kpy[j < sy ? j : 0];
+    }
 #pragma unroll
     for (int r = 0; r < PER; r++) {
         const int e = threadIdx.x + r * kLineThreads;
@@ -415,15 +427,16 @@ __global__ void __launch_bounds__(kLineThreads) k_fft_x(const double2 *__restric
     }
     __syncthreads();
     lds_fft<N, true, false, JB, 1, JB, kLineThreads>(buf, stw);
-    const double kz = kpz[kc];
-    for (int e = threadIdx.x; e < N * JB; e += kLineThreads) {
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        const int e = threadIdx.x + r * kLineThreads;
         const int jj = e % JB, i = e / JB;
         const int j = j0 + jj;
-        if (j >= sy) continue;
+        if (e >= N * JB || j >= sy) continue;
         double k2;
         {
 #pragma clang fp contract(off)
-            const double kx = kpx[i], ky = kpy[j];
+            const double kx = kxr[r], ky = kyr[r];
             k2 = kx * kx + ky * ky + kz * kz;
         }
         double2 v = buf[i * JB + jj];
