@@ -170,21 +170,28 @@ __device__ inline void fft_pass(double2 *buf, const double2 *tw)
         double2 a[R];
 #pragma unroll
         for (int p = 0; p < R; p++) a[p] = at(p);
+        // the pass with Lp = 1 has q = 0: every twiddle is tw[0] = (1, 0),
+        // whose product leaves a value unchanged, so neither the table
+        // reads nor the products are issued
         if constexpr (DIT) {
+            if constexpr (Lp > 1) {
 #pragma unroll
-            for (int p = 1; p < R; p++) {
-                double2 wv = tw[p * q * TSTEP];
-                if (INV) wv.y = -wv.y;
-                a[p] = cmul(a[p], wv);
+                for (int p = 1; p < R; p++) {
+                    double2 wv = tw[p * q * TSTEP];
+                    if (INV) wv.y = -wv.y;
+                    a[p] = cmul(a[p], wv);
+                }
             }
             dft_small<R, INV>(a);
         } else {
             dft_small<R, INV>(a);
+            if constexpr (Lp > 1) {
 #pragma unroll
-            for (int p = 1; p < R; p++) {
-                double2 wv = tw[p * q * TSTEP];
-                if (INV) wv.y = -wv.y;
-                a[p] = cmul(a[p], wv);
+                for (int p = 1; p < R; p++) {
+                    double2 wv = tw[p * q * TSTEP];
+                    if (INV) wv.y = -wv.y;
+                    a[p] = cmul(a[p], wv);
+                }
             }
         }
 #pragma unroll
