@@ -147,7 +147,10 @@ def test_reference_bc_entries_bitwise(nv, orc):
                                          ((32, 32, 32), (TWO_PI,) * 3, False),
                                          ((12, 16, 24), (1.0, 2.0, 1.0), True),
                                          ((256, 8, 12), (TWO_PI, 1.0, 1.0), False),
-                                         ((12, 8, 256), (1.0, 1.0, TWO_PI), False)])
+                                         ((12, 8, 256), (1.0, 1.0, TWO_PI), False),
+                                         ((8, 384, 12), (1.0, 2.0, 1.0), True),
+                                         ((8, 12, 512), (1.0, 1.0, TWO_PI), False),
+                                         ((768, 8, 12), (TWO_PI, 1.0, 1.0), False)])
 def test_poisson_and_projection_match_oracle(nv, orc, N, L, wall_y):
     G, O = pair(nv, orc, N, L, wall_y=wall_y)
     P = nv.Poisson(G)
