@@ -455,7 +455,9 @@ def main() -> int:
         G.tune(*args.tune)
         tuned = [f"fixed: ns_stage variant {args.tune[0]} chunk {args.tune[1]}, projection variant {args.tune[2]}"]
     else:
-        tuned = chosen(S.autotune(U, reps=3))
+        # eight timed launches per trial: with three, launch-to-launch noise
+        # decided between shapes a few microseconds apart
+        tuned = chosen(S.autotune(U, reps=8))
     log(f"bench: autotuner: {tuned}")
     log(f"bench: TG {n}^3, warmup {W}, steps {K}, graph={not args.eager}")
     S.step(DT, W, U, p)
