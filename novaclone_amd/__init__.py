@@ -196,6 +196,22 @@ def _ptr(t: torch.Tensor | None):
     return C.c_void_p(t.data_ptr())
 
 
+def _fld(t: torch.Tensor | None, mesh, ncomp: int, what: str = "field", npts: int | None = None):
+    """_ptr of a field that the kernels read or write as ncomp components of
+    the mesh's stored nodes (npts, default mesh.np): anything else would be
+    an out-of-bounds device access, so the element count and the device are
+    checked before the call."""
+    if t is None:
+        return None
+    n = mesh.np if npts is None else npts
+    if t.numel() != ncomp * n:
+        raise NovaError(f"{what}: {t.numel()} elements, the mesh needs {ncomp} x {n} = {ncomp * n} "
+                        f"(shape {tuple(t.shape)})")
+    if t.device != mesh.device:
+        raise NovaError(f"{what} is on {t.device}, the mesh on {mesh.device}")
+    return _ptr(t)
+
+
 class Mesh:
     """Grid + device context (reference ``Mesh``, src/Mesh/Mesh.cpp:24-179).
 
@@ -292,13 +308,15 @@ class Mesh:
     def pack_padded(self, f: torch.Tensor) -> torch.Tensor:
         nc = f.shape[0]
         out = torch.empty((*self.padded, nc), dtype=torch.float64, device=self.device)
-        _check(lib().nova_pack_padded(self._h, _ptr(f), nc, _ptr(out)), "nova_pack_padded")
+        _check(lib().nova_pack_padded(self._h, _fld(f, self, nc), nc, _ptr(out)), "nova_pack_padded")
         return out
 
     def unpack_padded(self, padded: torch.Tensor) -> torch.Tensor:
         nc = padded.shape[-1]
         out = self.field(nc)
-        _check(lib().nova_unpack_padded(self._h, _ptr(padded.contiguous()), nc, _ptr(out)), "nova_unpack_padded")
+        npad = self.padded[0] * self.padded[1] * self.padded[2]
+        _check(lib().nova_unpack_padded(self._h, _fld(padded.contiguous(), self, nc, "padded field", npad), nc,
+                                        _ptr(out)), "nova_unpack_padded")
         return out
 
     def bc_fill_padded(self, padded: torch.Tensor, entries) -> None:
@@ -309,7 +327,9 @@ class Mesh:
         kinds = (C.c_int * max(n, 1))(*[e[1] for e in entries])
         comps = (C.c_int * max(n, 1))(*[e[2] for e in entries])
         vals = (C.c_double * max(n, 1))(*[float(e[3]) for e in entries])
-        _check(lib().nova_bc_fill_padded(self._h, _ptr(padded), padded.shape[-1], n, dirs, kinds, comps, vals),
+        npad = self.padded[0] * self.padded[1] * self.padded[2]
+        pp = _fld(padded, self, padded.shape[-1], "padded field", npad)
+        _check(lib().nova_bc_fill_padded(self._h, pp, padded.shape[-1], n, dirs, kinds, comps, vals),
                "nova_bc_fill_padded")
 
 
@@ -320,16 +340,17 @@ class ex:
     def div(mesh: Mesh, U: torch.Tensor, phi: torch.Tensor | None = None) -> torch.Tensor:
         if phi is None:
             out = mesh.field(1)
-            _check(lib().nova_div(mesh.handle, _ptr(U), _ptr(out)), "nova_div")
+            _check(lib().nova_div(mesh.handle, _fld(U, mesh, 3, "U"), _ptr(out)), "nova_div")
             return out
         out = mesh.field(phi.shape[0])
-        _check(lib().nova_div_adv(mesh.handle, _ptr(U), _ptr(phi), phi.shape[0], _ptr(out)), "nova_div_adv")
+        _check(lib().nova_div_adv(mesh.handle, _fld(U, mesh, 3, "U"), _fld(phi, mesh, phi.shape[0], "phi"),
+                                  phi.shape[0], _ptr(out)), "nova_div_adv")
         return out
 
     @staticmethod
     def grad(mesh: Mesh, p: torch.Tensor) -> torch.Tensor:
         out = mesh.field(3)
-        _check(lib().nova_grad(mesh.handle, _ptr(p), _ptr(out)), "nova_grad")
+        _check(lib().nova_grad(mesh.handle, _fld(p, mesh, 1, "p"), _ptr(out)), "nova_grad")
         return out
 
     @staticmethod
@@ -338,17 +359,19 @@ class ex:
         variable-coefficient form, laplacian.hxx:64-127)."""
         out = mesh.field(f.shape[0])
         if isinstance(nu, torch.Tensor):
-            _check(lib().nova_laplacian_var(mesh.handle, _ptr(nu), _ptr(f), f.shape[0], _ptr(out)),
+            _check(lib().nova_laplacian_var(mesh.handle, _fld(nu, mesh, 1, "nu"), _fld(f, mesh, f.shape[0]),
+                                            f.shape[0], _ptr(out)),
                    "nova_laplacian_var")
         else:
-            _check(lib().nova_laplacian(mesh.handle, float(nu), _ptr(f), f.shape[0], _ptr(out)), "nova_laplacian")
+            _check(lib().nova_laplacian(mesh.handle, float(nu), _fld(f, mesh, f.shape[0]), f.shape[0], _ptr(out)),
+                   "nova_laplacian")
         return out
 
     @staticmethod
     def curl(mesh: Mesh, U: torch.Tensor, literal: bool = True) -> torch.Tensor:
         out = mesh.field(3)
         fn = lib().nova_curl_literal if literal else lib().nova_curl
-        _check(fn(mesh.handle, _ptr(U), _ptr(out)), "nova_curl")
+        _check(fn(mesh.handle, _fld(U, mesh, 3, "U"), _ptr(out)), "nova_curl")
         return out
 
 
@@ -356,7 +379,8 @@ def compact_deriv(mesh: "Mesh", f: torch.Tensor, direction: int, order: int = 1)
     """Compact sixth-order derivative (order 1 or 2) of every component of f
     along a direction (nova_compact_deriv)."""
     out = torch.empty_like(f)
-    _check(lib().nova_compact_deriv(mesh.handle, _ptr(f), f.shape[0], int(direction), int(order), _ptr(out)),
+    _check(lib().nova_compact_deriv(mesh.handle, _fld(f, mesh, f.shape[0]), f.shape[0], int(direction), int(order),
+                                    _ptr(out)),
            "nova_compact_deriv")
     return out
 
@@ -381,11 +405,13 @@ class Poisson:
     def solve(self, rhs: torch.Tensor, p: torch.Tensor | None = None) -> torch.Tensor:
         if p is None:
             p = self.mesh.field(1)
-        _check(lib().nova_poisson_solve(self._h, _ptr(rhs), _ptr(p)), "nova_poisson_solve")
+        _check(lib().nova_poisson_solve(self._h, _fld(rhs, self.mesh, 1, "rhs"), _fld(p, self.mesh, 1, "p")),
+               "nova_poisson_solve")
         return p
 
     def project(self, F: torch.Tensor, s: float = 1.0, p: torch.Tensor | None = None) -> None:
-        _check(lib().nova_project(self._h, _ptr(F), float(s), _ptr(p)), "nova_project")
+        _check(lib().nova_project(self._h, _fld(F, self.mesh, 3, "F"), float(s), _fld(p, self.mesh, 1, "p")),
+               "nova_project")
 
     def close(self):
         if getattr(self, "_h", None):
@@ -418,14 +444,16 @@ class Solver:
             _check(lib().nova_solver_use_graph(h, 1), "nova_solver_use_graph")
 
     def step(self, dt: float, nsteps: int, U, p, B=None, pB=None) -> None:
-        _check(lib().nova_solver_step(self._h, float(dt), int(nsteps), _ptr(U), _ptr(B), _ptr(p), _ptr(pB)),
-               "nova_solver_step")
+        m = self.mesh
+        _check(lib().nova_solver_step(self._h, float(dt), int(nsteps), _fld(U, m, 3, "U"), _fld(B, m, 3, "B"),
+                                      _fld(p, m, 1, "p"), _fld(pB, m, 1, "pB")), "nova_solver_step")
 
     def autotune(self, U, B=None, reps: int = 3) -> str:
         """Time every stage launch shape and both projection variants on U
         (and B), keep the fastest, and return the trial report."""
         buf = C.create_string_buffer(1 << 16)
-        _check(lib().nova_solver_autotune(self._h, _ptr(U), _ptr(B), int(reps), buf, len(buf)),
+        _check(lib().nova_solver_autotune(self._h, _fld(U, self.mesh, 3, "U"), _fld(B, self.mesh, 3, "B"),
+                                          int(reps), buf, len(buf)),
                "nova_solver_autotune")
         return buf.value.decode()
 
@@ -454,16 +482,17 @@ class FdMatrix:
         if _handle is not None:
             h = _handle
         else:
-            _check(lib().nova_fd_laplacian(mesh.handle, float(mu), _ptr(phi), C.byref(h)), "nova_fd_laplacian")
+            _check(lib().nova_fd_laplacian(mesh.handle, float(mu), _fld(phi, mesh, 1, "phi"), C.byref(h)),
+                   "nova_fd_laplacian")
         self._h = h
 
     def add_source(self, f: torch.Tensor) -> "FdMatrix":
         """matrix == f: the right-hand side gains f."""
-        _check(lib().nova_fd_add_source(self._h, _ptr(f)), "nova_fd_add_source")
+        _check(lib().nova_fd_add_source(self._h, _fld(f, self.mesh, 1)), "nova_fd_add_source")
         return self
 
     def relax(self, phi: torch.Tensor, urf: float) -> None:
-        _check(lib().nova_fd_relax(self._h, _ptr(phi), float(urf)), "nova_fd_relax")
+        _check(lib().nova_fd_relax(self._h, _fld(phi, self.mesh, 1, "phi"), float(urf)), "nova_fd_relax")
 
     def set_reference(self, i: int, j: int, k: int, val: float) -> None:
         _check(lib().nova_fd_set_reference(self._h, int(i), int(j), int(k), float(val)), "nova_fd_set_reference")
@@ -482,14 +511,15 @@ class FdMatrix:
 
     def residual(self, phi: torch.Tensor) -> float:
         r = C.c_double()
-        _check(lib().nova_fd_residual(self._h, _ptr(phi), C.byref(r)), "nova_fd_residual")
+        _check(lib().nova_fd_residual(self._h, _fld(phi, self.mesh, 1, "phi"), C.byref(r)), "nova_fd_residual")
         return r.value
 
     def solve(self, phi: torch.Tensor, tol: float, max_its: int):
         """Solve for phi in place; returns (iterations, last residual)."""
         its = C.c_int()
         r = C.c_double()
-        _check(lib().nova_fd_solve(self._h, _ptr(phi), float(tol), int(max_its), C.byref(its), C.byref(r)),
+        _check(lib().nova_fd_solve(self._h, _fld(phi, self.mesh, 1, "phi"), float(tol), int(max_its), C.byref(its),
+                                   C.byref(r)),
                "nova_fd_solve")
         return its.value, r.value
 
@@ -607,21 +637,22 @@ class IO:
         _check(lib().nova_io_write_mesh(self._h, time._h, str(path).encode()), "nova_io_write_mesh")
 
     def write_field(self, time: Time, path, name: str, F) -> None:
-        _check(lib().nova_io_write_field(self._h, time._h, str(path).encode(), name.encode(), _ptr(F), F.shape[0]),
+        _check(lib().nova_io_write_field(self._h, time._h, str(path).encode(), name.encode(),
+                                         _fld(F, self.mesh, F.shape[0], name), F.shape[0]),
                "nova_io_write_field")
 
     def read_field(self, path, name: str, file_num: int, F) -> None:
-        _check(lib().nova_io_read_field(self._h, str(path).encode(), name.encode(), int(file_num), _ptr(F),
-                                        F.shape[0]), "nova_io_read_field")
+        _check(lib().nova_io_read_field(self._h, str(path).encode(), name.encode(), int(file_num),
+                                        _fld(F, self.mesh, F.shape[0], name), F.shape[0]), "nova_io_read_field")
 
     def checkpoint_write(self, file, fields, time: float = 0.0, step: int = 0) -> None:
-        ptrs = (C.c_void_p * len(fields))(*[_ptr(f).value for f in fields])
+        ptrs = (C.c_void_p * len(fields))(*[_fld(f, self.mesh, f.shape[0]).value for f in fields])
         nc = (C.c_int * len(fields))(*[f.shape[0] for f in fields])
         _check(lib().nova_io_checkpoint_write(self._h, str(file).encode(), len(fields), ptrs, nc, float(time),
                                               int(step)), "nova_io_checkpoint_write")
 
     def checkpoint_read(self, file, fields):
-        ptrs = (C.c_void_p * len(fields))(*[_ptr(f).value for f in fields])
+        ptrs = (C.c_void_p * len(fields))(*[_fld(f, self.mesh, f.shape[0]).value for f in fields])
         nc = (C.c_int * len(fields))(*[f.shape[0] for f in fields])
         t = C.c_double()
         st = C.c_int64()
@@ -659,11 +690,13 @@ class WC:
         """The frozen magnetic field of app/OT and its coefficient DBU; the
         tensor must stay alive while it is set (None removes it)."""
         self._B = B
-        _check(lib().nova_wc_set_magnetic(self._h, _ptr(B), float(dbu)), "nova_wc_set_magnetic")
+        _check(lib().nova_wc_set_magnetic(self._h, _fld(B, self.mesh, 3, "B"), float(dbu)), "nova_wc_set_magnetic")
 
     def step(self, dt: float, nsteps: int, U, rho, p) -> None:
         """Advance nsteps: U (3 components) and rho in/out, p out."""
-        _check(lib().nova_wc_step(self._h, float(dt), int(nsteps), _ptr(U), _ptr(rho), _ptr(p)), "nova_wc_step")
+        m = self.mesh
+        _check(lib().nova_wc_step(self._h, float(dt), int(nsteps), _fld(U, m, 3, "U"), _fld(rho, m, 1, "rho"),
+                                  _fld(p, m, 1, "p")), "nova_wc_step")
 
     def set_temperature(self, ka: float, g) -> None:
         """Enable the temperature equation and buoyancy of app/shearLayerWC."""
@@ -671,13 +704,16 @@ class WC:
 
     def step_t(self, dt: float, nsteps: int, U, rho, p, T) -> None:
         """Advance nsteps in app/shearLayerWC order: U, rho, T in/out, p in/out."""
-        _check(lib().nova_wc_step_t(self._h, float(dt), int(nsteps), _ptr(U), _ptr(rho), _ptr(p), _ptr(T)),
-               "nova_wc_step_t")
+        m = self.mesh
+        _check(lib().nova_wc_step_t(self._h, float(dt), int(nsteps), _fld(U, m, 3, "U"), _fld(rho, m, 1, "rho"),
+                                    _fld(p, m, 1, "p"), _fld(T, m, 1, "T")), "nova_wc_step_t")
 
     def hydrostatic(self, T, rho0: float, p, rho) -> None:
         """app/shearLayerWC's initial state (hydrostaticP.H): p += the
         hydrostatic pressure of T under g_z, then rho = (p/pc)^(1/2)."""
-        _check(lib().nova_wc_hydrostatic(self._h, _ptr(T), float(rho0), _ptr(p), _ptr(rho)), "nova_wc_hydrostatic")
+        m = self.mesh
+        _check(lib().nova_wc_hydrostatic(self._h, _fld(T, m, 1, "T"), float(rho0), _fld(p, m, 1, "p"),
+                                         _fld(rho, m, 1, "rho")), "nova_wc_hydrostatic")
 
     def close(self):
         if getattr(self, "_h", None):
@@ -693,32 +729,35 @@ class WC:
 
 def ke(mesh: Mesh, U: torch.Tensor) -> float:
     out = C.c_double()
-    _check(lib().nova_reduce_ke(mesh.handle, _ptr(U), C.byref(out)), "nova_reduce_ke")
+    _check(lib().nova_reduce_ke(mesh.handle, _fld(U, mesh, 3, "U"), C.byref(out)), "nova_reduce_ke")
     return out.value
 
 
 def maxabs_div(mesh: Mesh, U: torch.Tensor) -> float:
     out = C.c_double()
-    _check(lib().nova_reduce_maxabs_div(mesh.handle, _ptr(U), C.byref(out)), "nova_reduce_maxabs_div")
+    _check(lib().nova_reduce_maxabs_div(mesh.handle, _fld(U, mesh, 3, "U"), C.byref(out)), "nova_reduce_maxabs_div")
     return out.value
 
 
 def dissipation(mesh: Mesh, U: torch.Tensor, nu: float) -> float:
     out = C.c_double()
-    _check(lib().nova_reduce_dissipation(mesh.handle, _ptr(U), float(nu), C.byref(out)), "nova_reduce_dissipation")
+    _check(lib().nova_reduce_dissipation(mesh.handle, _fld(U, mesh, 3, "U"), float(nu), C.byref(out)),
+           "nova_reduce_dissipation")
     return out.value
 
 
 def mhd_diag(mesh: Mesh, U, B, mu: float, eta: float):
     out = (C.c_double * 8)()
-    _check(lib().nova_mhd_diag(mesh.handle, _ptr(U), _ptr(B), float(mu), float(eta), out), "nova_mhd_diag")
+    _check(lib().nova_mhd_diag(mesh.handle, _fld(U, mesh, 3, "U"), _fld(B, mesh, 3, "B"), float(mu), float(eta), out),
+           "nova_mhd_diag")
     return list(out)
 
 
 def tdma_batched(mesh: Mesh, a, b, c, d, cyclic: bool = False) -> torch.Tensor:
     nb, n = a.shape
     x = torch.empty_like(d)
-    _check(lib().nova_tdma_batched(mesh.handle, n, nb, _ptr(a), _ptr(b), _ptr(c), _ptr(d), _ptr(x), int(cyclic)),
+    a, b, c, d = (_fld(v, mesh, 1, nm, nb * n) for v, nm in zip((a, b, c, d), "abcd"))
+    _check(lib().nova_tdma_batched(mesh.handle, n, nb, a, b, c, d, _ptr(x), int(cyclic)),
            "nova_tdma_batched")
     return x
 
@@ -726,7 +765,8 @@ def tdma_batched(mesh: Mesh, a, b, c, d, cyclic: bool = False) -> torch.Tensor:
 def weighted_dot(mesh: Mesh, a: torch.Tensor, b: torch.Tensor) -> float:
     """sum over nodes of w a.b, w = 1/2 on wall nodes."""
     out = C.c_double()
-    _check(lib().nova_weighted_dot(mesh.handle, _ptr(a), _ptr(b), a.shape[0], C.byref(out)), "nova_weighted_dot")
+    _check(lib().nova_weighted_dot(mesh.handle, _fld(a, mesh, a.shape[0], "a"), _fld(b, mesh, a.shape[0], "b"),
+                                   a.shape[0], C.byref(out)), "nova_weighted_dot")
     return out.value
 
 
@@ -734,7 +774,7 @@ def enstrophy(mesh: Mesh, U: torch.Tensor, grad: bool = False):
     """J = dV sum w |curl U|^2; with grad=True also dJ/dU = 2 dV curl curl U."""
     J = C.c_double()
     g = mesh.field(3) if grad else None
-    _check(lib().nova_enstrophy(mesh.handle, _ptr(U), C.byref(J), _ptr(g)), "nova_enstrophy")
+    _check(lib().nova_enstrophy(mesh.handle, _fld(U, mesh, 3, "U"), C.byref(J), _ptr(g)), "nova_enstrophy")
     return (J.value, g) if grad else J.value
 
 
@@ -752,17 +792,20 @@ class Adjoint:
 
     def forward(self, dt: float, nsteps: int, U0: torch.Tensor) -> torch.Tensor:
         U = torch.empty_like(U0)
-        _check(lib().nova_adjoint_forward(self._h, float(dt), int(nsteps), _ptr(U0), _ptr(U)), "nova_adjoint_forward")
+        _check(lib().nova_adjoint_forward(self._h, float(dt), int(nsteps), _fld(U0, self.mesh, 3, "U0"), _ptr(U)),
+               "nova_adjoint_forward")
         return U
 
     def backward(self, lam: torch.Tensor) -> torch.Tensor:
         out = torch.empty_like(lam)
-        _check(lib().nova_adjoint_backward(self._h, _ptr(lam.contiguous()), _ptr(out)), "nova_adjoint_backward")
+        _check(lib().nova_adjoint_backward(self._h, _fld(lam.contiguous(), self.mesh, 3, "lambda"), _ptr(out)),
+               "nova_adjoint_backward")
         return out
 
     def tlm(self, d0: torch.Tensor) -> torch.Tensor:
         out = torch.empty_like(d0)
-        _check(lib().nova_adjoint_tlm(self._h, _ptr(d0.contiguous()), _ptr(out)), "nova_adjoint_tlm")
+        _check(lib().nova_adjoint_tlm(self._h, _fld(d0.contiguous(), self.mesh, 3, "dU0"), _ptr(out)),
+               "nova_adjoint_tlm")
         return out
 
     def autotune(self, reps: int = 3) -> str:
@@ -830,8 +873,12 @@ class Ensemble:
                 "slots": out[5]}
 
     def load(self, i: int, U: torch.Tensor, B: torch.Tensor | None = None) -> None:
-        _check(lib().nova_ensemble_copy(self._h, int(i), 0, _ptr(U.contiguous()), _ptr(B), None, None),
-               "nova_ensemble_copy")
+        shp = self.shapes[i]
+        npts = shp[0] * shp[1] * shp[2]
+        if self.models[i] == MODEL_MHD and B is None:
+            raise NovaError(f"instance {i} is MHD: load needs B")
+        _check(lib().nova_ensemble_copy(self._h, int(i), 0, _fld(U.contiguous(), self, 3, "U", npts),
+                                        _fld(B, self, 3, "B", npts), None, None), "nova_ensemble_copy")
 
     def store(self, i: int):
         """(U, B, p, pB) of instance i (B and pB are None for NS)."""

@@ -14,6 +14,13 @@
  */
 #include "nova_oracle.h"
 
+/* ORC_PLANT selects one deliberate mistake (test builds only, see
+ * tests/test_oracle_planted.py): 0, the default, is the faithful oracle.
+ * A non-zero value exists to show that the oracle's pins fail on it. */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+
 #include <complex.h>
 #include <math.h>
 #include <stdio.h>
@@ -85,10 +92,10 @@ int orc_num_threads(void)
 double orc_mesh_z(const orc_mesh *M, int k)
 {
     if (M->hyp) {
-        double A = pow(M->s1 / M->s2, 0.5);
+        double A = ORC_PLANT == 10 ? pow(M->s2 / M->s1, 0.5) : pow(M->s1 / M->s2, 0.5);
         int globK = k - M->m / 2;
         int globN = M->N[2] + 1;
-        double R = (double)globK / (double)(globN - 1) - 0.5;
+        double R = (double)globK / (double)(globN - 1) - (ORC_PLANT == 11 ? 0.45 : 0.5);
         double b = M->hb;
         double U = 1.0 + tanh(b * R) / tanh(b / 2.0);
         return U / (2.0 * A + (1 - A) * U) * M->L[2];
@@ -939,7 +946,12 @@ static void rhs_u(const orc_mesh *M, const double *U, const double *B, double nu
             s1[q] = 0.5 * (B[3 * q] * B[3 * q] + B[3 * q + 1] * B[3 * q + 1] + B[3 * q + 2] * B[3 * q + 2]);
         orc_grad(M, s1, t2);
         #pragma omp parallel for
-        for (size_t q = 0; q < n3; q++) dU[q] = (dU[q] + t1[q]) - t2[q];
+        for (size_t q = 0; q < n3; q++) {
+            if (ORC_PLANT == 1) dU[q] = (dU[q] - t1[q]) - t2[q];      /* Lorentz div(B,B) sign */
+            else if (ORC_PLANT == 2) dU[q] = dU[q] - t2[q];          /* div(B,B) dropped */
+            else if (ORC_PLANT == 3) dU[q] = (dU[q] + t1[q]) + t2[q]; /* magnetic pressure sign */
+            else dU[q] = (dU[q] + t1[q]) - t2[q];
+        }
     }
     (void)s2;
     #pragma omp parallel for
@@ -955,10 +967,22 @@ static void rhs_b(const orc_mesh *M, const double *U, const double *B, double et
     orc_div_adv(M, U, B, 3, t1);
     orc_div_adv(M, B, U, 3, t2);
     #pragma omp parallel for
-    for (size_t q = 0; q < n3; q++) dB[q] = (-t1[q]) + t2[q];
+    for (size_t q = 0; q < n3; q++) dB[q] = ORC_PLANT == 4 ? t1[q] - t2[q] : (-t1[q]) + t2[q]; /* 4: induction sign */
     orc_laplacian(M, eta, B, 3, t1);
     #pragma omp parallel for
     for (size_t q = 0; q < n3; q++) dB[q] = dt * (dB[q] + t1[q]);
+}
+
+/* The two right-hand sides of app/OT_Poisson/UEqn.H:16-23 and BEqn.H:15-23
+ * per unit dt, for the pins: dU = -div(U,U) + laplacian(nu,U) + div(B,B)
+ * - grad(0.5*(B&B)), dB = -div(U,B) + div(B,U) + laplacian(eta,B). */
+void orc_mhd_rhs(const orc_mesh *M, const double *U, const double *B, double nu, double eta, double *dU,
+                 double *dB)
+{
+    double *t1 = falloc(M, 3), *t2 = falloc(M, 3), *s1 = falloc(M, 1);
+    rhs_u(M, U, B, nu, 1.0, dU, t1, t2, s1, NULL);
+    rhs_b(M, U, B, eta, 1.0, dB, t1, t2);
+    free(t1); free(t2); free(s1);
 }
 
 /* Projection F <- F - s*grad(p), lap p = div(F)/s (RKLoop.H:30-38 with
@@ -1052,12 +1076,16 @@ void orc_mhd_step(const orc_mesh *M, orc_poisson *P, double *U, double *B, doubl
         /* BEqn.H:15-41 (staged RK4 of B with U frozen, then cleaning) */
         static const double A[4][4] = {{0.5, 0, 0, 0}, {0, 0.5, 0, 0}, {0, 0, 1.0, 0},
                                        {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0}};
+        static const double Aw[4][4] = {{0.5, 0, 0, 0}, {0, 0.5, 0, 0}, {0, 0, 1.0, 0},
+                                        {0.25, 0.25, 0.25, 0.25}};
         for (int rk = 0; rk < 4; rk++) {
-            rhs_b(M, U, B, eta, dt, dB[rk], t1, t2);
+            /* 5: the B stages see the step-start U instead of the projected one;
+             * 6: the last row of RK4a replaced */
+            rhs_b(M, ORC_PLANT == 5 ? U0 : U, B, eta, dt, dB[rk], t1, t2);
             #pragma omp parallel for
             for (size_t q = 0; q < n3; q++) {
                 double s = B0[q];
-                for (int j = 0; j <= rk; j++) s = s + A[rk][j] * dB[j][q];
+                for (int j = 0; j <= rk; j++) s = s + (ORC_PLANT == 6 ? Aw[rk][j] : A[rk][j]) * dB[j][q];
                 B[q] = s;
             }
             orc_fill(M, B, 3, 1);
@@ -1072,6 +1100,7 @@ void orc_mhd_step(const orc_mesh *M, orc_poisson *P, double *U, double *B, doubl
             for (size_t q = 0; q < n3; q++) {
                 double vu = U[q] + dU[q], vb = B[q] + dB[0][q];
                 if (s == 0) { Ut[q] = vu; Bt[q] = vb; }
+                else if (s == 1 && ORC_PLANT == 7) { Ut[q] = 0.25 * U0[q] + 0.75 * vu; Bt[q] = 0.25 * B0[q] + 0.75 * vb; }
                 else if (s == 1) { Ut[q] = 0.75 * U0[q] + 0.25 * vu; Bt[q] = 0.75 * B0[q] + 0.25 * vb; }
                 else { Ut[q] = (1.0 / 3.0) * U0[q] + (2.0 / 3.0) * vu; Bt[q] = (1.0 / 3.0) * B0[q] + (2.0 / 3.0) * vb; }
             }
@@ -1426,13 +1455,13 @@ double orc_dissipation(const orc_mesh *M, const double *U, double nu)
             for (int j = h; j < nj - h - 1; j++)
                 for (int k = h; k < nk - h - 1; k++) {
                     const double *v = &g[IX(M, i, j, k) * 3];
-                    acc += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                    acc += ORC_PLANT == 9 ? v[0] * v[0] + v[1] * v[1] : v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
                     if (c == 0) n++;
                 }
     }
     free(g);
     free(s);
-    return nu * acc / n;
+    return ORC_PLANT == 8 ? acc / n : nu * acc / n; /* 8: nu missing */
 }
 
 /* tools::maximum(ex::div(U)), src/Tools/Tools.h:19-36 over [h, n-h). */
@@ -1459,7 +1488,8 @@ void orc_mhd_diag(const orc_mesh *M, const double *U, const double *B, double mu
 {
     const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
     double *J = falloc(M, 3), *W = falloc(M, 3);
-    orc_curl_literal(M, B, J);
+    if (ORC_PLANT == 12) orc_curl_central(M, B, J); /* 12: J from the consistent curl */
+    else orc_curl_literal(M, B, J);
     orc_curl_literal(M, U, W);
     double Ek = 0, Em = 0, Jmax = 0, Umax = 0, Jav = 0, om = 0;
     long n = 0;
@@ -1469,7 +1499,7 @@ void orc_mhd_diag(const orc_mesh *M, const double *U, const double *B, double mu
                 const size_t q = IX(M, i, j, k) * 3;
                 const double *u = &U[q], *b = &B[q], *jj = &J[q], *w = &W[q];
                 Ek += 0.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-                Em += 0.5 * (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+                Em += (ORC_PLANT == 13 ? 1.0 : 0.5) * (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
                 double jm = sqrt(jj[0] * jj[0] + jj[1] * jj[1] + jj[2] * jj[2]);
                 double um = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
                 Jmax = Jmax > jm ? Jmax : jm;
@@ -1480,7 +1510,7 @@ void orc_mhd_diag(const orc_mesh *M, const double *U, const double *B, double mu
             }
     Ek /= n; Em /= n; Jav /= n; om /= n;
     out[0] = Ek; out[1] = Em; out[2] = Jav; out[3] = Jmax; out[4] = om;
-    out[5] = -mu * (om * om) - eta * (Jav * Jav);
+    out[5] = ORC_PLANT == 14 ? -mu * (om * om) + eta * (Jav * Jav) : -mu * (om * om) - eta * (Jav * Jav);
     out[6] = Umax; out[7] = 0.0;
     free(J); free(W);
 }

@@ -92,6 +92,8 @@ void orc_ns_step(const orc_mesh *, orc_poisson *, double *U, double *Ustar,
                  double *p, double nu, double dt, int mode);
 /* One MHD step.  mode 0 = app/OT_Poisson/{UEqn,BEqn}.H literally;
  * mode 1 = SSP-RK3 with u and B projected at every stage. */
+void orc_mhd_rhs(const orc_mesh *, const double *U, const double *B, double nu, double eta,
+                 double *dU, double *dB);
 void orc_mhd_step(const orc_mesh *, orc_poisson *, double *U, double *B,
                   double *p, double *pB, double nu, double eta, double dt, int mode);
 /* Weakly-compressible step (app/TG/TG.cpp with rhoEqn.H, pEqn.H, UEqn.H):

@@ -7,13 +7,16 @@ arrays in the reference's padded AoS layout, shape (n_i, n_j, n_k, ncomp).
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB = HERE / "_build" / "libnova_oracle.so"
+# NOVA_ORACLE_LIB points the binding at another build of the same source
+# (the planted-mistake builds of tests/test_oracle_planted.py)
+LIB = Path(os.environ.get("NOVA_ORACLE_LIB", HERE / "_build" / "libnova_oracle.so"))
 
 NORTH, SOUTH, EAST, WEST, TOP, BOTTOM = range(6)
 BC_PERIODIC, BC_NEUMANN, BC_DIRICHLET, BC_EVEN, BC_ODD = 0, 1, 2, 10, 11
@@ -57,6 +60,7 @@ def _load() -> C.CDLL:
         "orc_poisson_k2": ([_P, _D], None),
         "orc_poisson_solve": ([_P, _D, _D], None),
         "orc_ns_step": ([_P, _P, _D, _D, _D, C.c_double, C.c_double, C.c_int], None),
+        "orc_mhd_rhs": ([_P, _D, _D, C.c_double, C.c_double, _D, _D], None),
         "orc_mhd_step": ([_P, _P, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_int], None),
         "orc_wc_step": ([_P, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_double], None),
         "orc_wcs_step": ([_P, _D, _D, _D, _D, C.c_double, C.c_double, _D, C.c_double, C.c_double], None),
@@ -216,6 +220,16 @@ class OMesh:
 
     def dissipation(self, U, nu):
         return lib().orc_dissipation(self._h, _d(U), nu)
+
+    def mhd_rhs(self, U, B, nu, eta):
+        """(dU, dB) per unit dt of the MHD step (UEqn.H:16-23, BEqn.H:15-23)."""
+        dU, dB = self.field(3), self.field(3)
+        lib().orc_mhd_rhs(self._h, _d(U), _d(B), nu, eta, _d(dU), _d(dB))
+        return dU, dB
+
+    def z(self, k):
+        """Mesh::z(k) at padded index k."""
+        return lib().orc_mesh_z(self._h, int(k))
 
     def mhd_diag(self, U, B, mu, eta):
         out = np.zeros(8)

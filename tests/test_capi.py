@@ -58,3 +58,48 @@ def test_no_device_is_an_error_not_a_fallback():
     assert b"hipSetDevice" in lib.nova_last_error()
     with pytest.raises(nv.NovaError):
         nv.Mesh((16, 16, 16), (1.0, 1.0, 1.0))
+
+
+class _StubMesh:
+    """Stands in for a Mesh of 4 x 4 x 4 stored nodes (no device, no handle):
+    the shape checks must raise before anything reaches the library."""
+    np = 64
+    padded = (10, 10, 10)
+    handle = None
+    _h = None
+
+    def __init__(self):
+        import torch
+        self.device = torch.device("cuda", 0)
+
+    def field(self, nc=3):
+        import torch
+        return torch.zeros((nc, 4, 4, 4), dtype=torch.float64)
+
+
+def test_field_sizes_are_checked_before_any_launch():
+    import torch
+    import novaclone_amd as nv
+    m = _StubMesh()
+    short = torch.zeros((3, 4, 4, 3), dtype=torch.float64)   # one plane short
+    scalar = torch.zeros((1, 4, 4, 4), dtype=torch.float64)
+    vector = torch.zeros((3, 4, 4, 4), dtype=torch.float64)
+    with pytest.raises(nv.NovaError, match="elements"):
+        nv.ex.div(m, short)
+    with pytest.raises(nv.NovaError, match="elements"):
+        nv.ex.grad(m, vector)  # three components where one is read
+    with pytest.raises(nv.NovaError, match="elements"):
+        nv.ke(m, scalar)
+    sv = object.__new__(nv.Solver)
+    sv._h, sv.mesh = None, m
+    with pytest.raises(nv.NovaError, match="elements"):
+        sv.step(1e-3, 1, short, scalar)
+    with pytest.raises(nv.NovaError, match="elements"):
+        sv.step(1e-3, 1, None, vector)  # p must be one component
+    ps = object.__new__(nv.Poisson)
+    ps._h, ps.mesh = None, m
+    with pytest.raises(nv.NovaError, match="elements"):
+        ps.project(short, 1.0)
+    # the right size on the wrong device is refused too
+    with pytest.raises(nv.NovaError, match="is on cpu"):
+        nv.ex.div(m, vector)
