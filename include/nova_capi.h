@@ -49,7 +49,10 @@ enum { NOVA_BC_PERIODIC = 0, NOVA_BC_WALL = 1 };
 
 /* Time-integration compositions (nova_solver_create). */
 enum {
-    NOVA_RK_LITERAL4 = 0, /* app/TG_Piosson/RKLoop.H:15-38: frozen-stage RK4, one projection */
+    NOVA_RK_LITERAL4 = 0, /* app/TG_Piosson/RKLoop.H:15-38: frozen-stage RK4, one projection;
+                           * MHD: app/OT_Poisson/UEqn.H:14-40 then BEqn.H:14-41 (frozen-stage
+                           * U and its projection, classic RK4 of B with that U frozen,
+                           * cleaning with s = 1) */
     NOVA_RK_SSP3 = 1,     /* SSP-RK3 (Shu-Osher), one projection per step */
     NOVA_RK_SHEAR4 = 2    /* app/shearLayer/UEqn.H:14-38: projection at every stage */
 };
@@ -195,7 +198,11 @@ int nova_solver_create(nova_poisson *ps, int model, int mode, double nu, double 
                        nova_solver **out);
 int nova_solver_destroy(nova_solver *sv);
 /* Advance nsteps of size dt.  U (and B for MHD) in/out, p (and pB) out.
- * NS: app/TG_Piosson/RKLoop.H; MHD: app/OT_Poisson/{UEqn,BEqn}.H. */
+ * NS: app/TG_Piosson/RKLoop.H; MHD: app/OT_Poisson/{UEqn,BEqn}.H (the
+ * reference's composition with NOVA_RK_LITERAL4, both fields projected
+ * every stage with NOVA_RK_SSP3).  The current density J = ex::curl(B)
+ * that BEqn.H:43 forms after the cleaning is output only and is left to
+ * nova_curl_literal / nova_mhd_diag. */
 int nova_solver_step(nova_solver *sv, double dt, int nsteps, double *U, double *B,
                      double *p, double *pB);
 /* Record the step sequence as a HIP graph and replay it (1) or launch

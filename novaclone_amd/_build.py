@@ -18,7 +18,7 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_objs"
 LIB = PKG / "libnova_hip.so"
 SOURCES = ["nova_capi.hip", "nova_ops.hip", "nova_fft.hip", "nova_step.hip", "nova_adjoint.hip", "nova_ensemble.hip",
-           "nova_compact.hip", "nova_wc.hip", "nova_io.hip", "nova_implicit.hip"]
+           "nova_compact.hip", "nova_wc.hip", "nova_io.hip", "nova_implicit.hip", "nova_mhd_lit.hip"]
 ARCH = os.environ.get("NOVA_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -47,7 +47,7 @@ def _compile(src: str) -> Path:
 
 def build(verbose: bool = False) -> Path:
     BUILD.mkdir(exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(4, os.cpu_count() or 1)) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(6, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", str(tmp), *map(str, objs)]

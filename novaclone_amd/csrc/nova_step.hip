@@ -22,6 +22,9 @@ int compact_mhd_stage(nova_ctx *ctx, const double *U, const double *B, const dou
                       double *oU, double *oB, const StageArgs &sa, double nu, double eta, double *R);
 int project(nova_poisson *ps, double *F, double s, double *p);
 nova_ctx *poisson_ctx(nova_poisson *ps);
+int mhd_u_stage(nova_ctx *ctx, const double *U, const double *B, double *out, const StageArgs &sa, double nu);
+int mhd_b_stage(nova_ctx *ctx, const double *U, const double *B, const double *B0, double *acc, double *out,
+                double beta, double w, int first, int last, double dt, double eta);
 }  // namespace nova
 
 struct nova_solver {
@@ -286,7 +289,32 @@ int mhd_step_once(nova_solver *sv, double dt, double *U, double *B, double *p, d
 {
     nova_ctx *ctx = sv->ctx;
     int rc;
-    if (sv->mode != NOVA_RK_SSP3) return fail(NOVA_ERR_UNSUPPORTED, "MHD supports the SSP-RK3 composition");
+    if (sv->mode == NOVA_RK_LITERAL4) {
+        // app/OT_Poisson/UEqn.H:14-40 then BEqn.H:14-41 (app/mhdjet alike)
+        if (sv->scheme != NOVA_SCHEME_EXPLICIT)
+            return fail(NOVA_ERR_UNSUPPORTED, "the reference's MHD composition uses the explicit stencils");
+        StageArgs a{};
+        a.form = FORM_LIT4;
+        a.dt = dt;
+        for (int j = 0; j < 4; j++) a.w[j] = kRK4W[j];
+        if ((rc = mhd_u_stage(ctx, U, B, sv->wa, a, sv->nu))) return rc;
+        if ((rc = project(sv->ps, sv->wa, dt, p))) return rc;
+        if ((rc = copy3(ctx, U, sv->wa))) return rc;
+        // classic RK4 of B with the new U frozen: stage rk writes
+        // B0 + RK4a[rk][rk] dB_rk (the earlier entries of the row are zero)
+        // and adds RK4a[3][rk] dB_rk to the running sum in wa; the last stage
+        // writes the sum into B, then B is cleaned (s = 1)
+        static const double kDiag[3] = {0.5, 0.5, 1.0};  // settings.cpp:44-48
+        const double *in[4] = {B, sv->wb, sv->wc, sv->wd};
+        double *out[4] = {sv->wb, sv->wc, sv->wd, B};
+        for (int rk = 0; rk < 4; rk++)
+            if ((rc = mhd_b_stage(ctx, U, in[rk], B, sv->wa, out[rk], rk < 3 ? kDiag[rk] : 0.0, kRK4W[rk], rk == 0,
+                                  rk == 3, dt, sv->eta)))
+                return rc;
+        return project(sv->ps, B, 1.0, pB);
+    }
+    if (sv->mode != NOVA_RK_SSP3)
+        return fail(NOVA_ERR_UNSUPPORTED, "MHD supports the SSP-RK3 and the reference's (LITERAL4) compositions");
     if (sv->scheme == NOVA_SCHEME_COMPACT6) {
         double *R = sv->rwork;
         if ((rc = compact_mhd_stage(ctx, U, B, nullptr, nullptr, sv->wa, sv->wb, ssp(0, dt), sv->nu, sv->eta, R)))

@@ -8,12 +8,18 @@
 namespace nova {
 namespace {
 
-enum { MODEL_NS = 0, MODEL_MHD = 1, MODEL_TLM = 2, MODEL_ADJ = 3 };
+// MODEL_MHDU / MODEL_MHDB: the two halves of the reference's MHD step
+// (app/OT_Poisson/UEqn.H and BEqn.H), each reading both fields' stencils
+// and writing one field
+enum { MODEL_NS = 0, MODEL_MHD = 1, MODEL_TLM = 2, MODEL_ADJ = 3, MODEL_MHDU = 4, MODEL_MHDB = 5 };
 template <int MODEL>
 constexpr int model_nf() { return MODEL == MODEL_NS ? 1 : 2; }
 // whether the stage combination of the model reads a step-start field f
 template <int MODEL>
-constexpr bool model_x0(int f) { return MODEL == MODEL_MHD || (MODEL == MODEL_NS && f == 0) || (MODEL == MODEL_TLM && f == 1); }
+constexpr bool model_x0(int f)
+{
+    return MODEL == MODEL_MHD || (MODEL == MODEL_NS && f == 0) || ((MODEL == MODEL_TLM || MODEL == MODEL_MHDB) && f == 1);
+}
 
 // ---- the streamed stage kernel ----
 //
@@ -43,6 +49,12 @@ struct StreamArgs {
 // MODEL_NS: X = {U}; MODEL_MHD: X = {U, B};  out = stage combination.
 // MODEL_TLM: X = {U, d}, X0[1] = step-start tangent; out[0] = tangent stage.
 // MODEL_ADJ: X = {U, m}; out[0] = ob (m + dt J_R(U)^T m) + yg Y.
+// MODEL_MHDU: X = {U, B}; out[0] = U + sum_j w_j dU, dU = dt R_U(U, B)
+//   (sa.form FORM_LIT4).
+// MODEL_MHDB: X = {U, B}, X0[1] = B at the step start, acc = the running
+//   RK4 sum; dB = dt R_B(U, B); s = (acc_a != 0 ? acc : X0[1]) + w_0 dB;
+//   alpha != 0 (last stage): out[0] = s; else acc = s and
+//   out[0] = X0[1] + beta dB.
 // Staging of the (j,k) planes in LDS: STG_ONE holds one plane (two barriers
 // per plane), STG_TWO two alternating planes (one barrier), STG_RING a ring
 // of H + 2 planes whose halos are loaded H planes ahead (one barrier).
@@ -199,6 +211,7 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
         int fq, fh;             // reflection flags of the queue plane and the halo plane
     };
     constexpr bool ADJ = MODEL == MODEL_ADJ;
+    constexpr bool ACC = ADJ || MODEL == MODEL_MHDB;  // reads the pointwise accumulator
     // Every lane issues the same loads for every plane: slots past the halo
     // read column (0, 0), lanes outside the grid read their mapped column,
     // and the prefetch past the end of a chunk re-reads its last plane.  With
@@ -213,10 +226,10 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                     P.nx0[f][c] = (model_x0<MODEL>(f) && (CMB == 1 || (CMB == 0 && a.X0[f])))
                                       ? ldp(a.X0[f], c, (int64_t)p * plane, col)
                                       : 0.0;
-            if constexpr (ADJ) {
+            if constexpr (ACC) {
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    P.ny[c] = a.Y ? ldp(a.Y, c, (int64_t)p * plane, col) : 0.0;
+                    P.ny[c] = (ADJ && a.Y) ? ldp(a.Y, c, (int64_t)p * plane, col) : 0.0;
                     P.nacc[c] = (a.acc && a.acc_a != 0.0) ? ldp(a.acc, c, (int64_t)p * plane, col) : 0.0;
                 }
             }
@@ -246,10 +259,10 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                             ? ldp(a.X0[f], c, (int64_t)i0 * plane, col)
                             : 0.0;
     double yc[3] = {0.0, 0.0, 0.0}, accc[3] = {0.0, 0.0, 0.0};
-    if constexpr (ADJ) {
+    if constexpr (ACC) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            yc[c] = a.Y ? ldp(a.Y, c, (int64_t)i0 * plane, col) : 0.0;
+            yc[c] = (ADJ && a.Y) ? ldp(a.Y, c, (int64_t)i0 * plane, col) : 0.0;
             accc[c] = (a.acc && a.acc_a != 0.0) ? ldp(a.acc, c, (int64_t)i0 * plane, col) : 0.0;
         }
     }
@@ -335,6 +348,28 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                     if (a.acc) stp(a.acc, c, pi, col, (a.acc_a != 0.0 ? a.acc_a * accc[c] : 0.0) + a.acc_b * m);
                     stp(a.out[0], c, pi, col, o);
                 }
+            } else if constexpr (MODEL == MODEL_MHDU) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const double du =
+                        sa.dt * ((((-div_xy<H>(g, A, 0, 0, c, zi)) + lap_c<H>(g, A, 0, c, a.nu, zl, zm)) +
+                                  div_xy<H>(g, A, 1, 1, c, zi)) - grad_pm<H>(g, A, 1, c, zi));
+                    const double u = A(0, c, 0, 0);
+                    stp(a.out[0], c, pi, col, combine(sa, u, u, du));
+                }
+            } else if constexpr (MODEL == MODEL_MHDB) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const double db = sa.dt * (((-div_xy<H>(g, A, 0, 1, c, zi)) + div_xy<H>(g, A, 1, 0, c, zi)) +
+                                               lap_c<H>(g, A, 1, c, a.eta, zl, zm));
+                    const double s = (a.acc_a != 0.0 ? accc[c] : x0c[1][c]) + sa.w[0] * db;
+                    if (sa.alpha != 0.0) {
+                        stp(a.out[0], c, pi, col, s);
+                    } else {
+                        stp(a.acc, c, pi, col, s);
+                        stp(a.out[0], c, pi, col, x0c[1][c] + sa.beta * db);
+                    }
+                }
             } else {
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
@@ -362,10 +397,10 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                     q[f][c][2 * H] = fresh<PF>(rsign<WALLS>(c, 0, Nxt.fq) * sgn_col[c] * Nxt.nq[f][c]);
                     x0c[f][c] = fresh<PF>(Nxt.nx0[f][c]);
                 }
-            if constexpr (ADJ) {
+            if constexpr (ACC) {
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    yc[c] = fresh<PF>(Nxt.ny[c]);
+                    if constexpr (ADJ) yc[c] = fresh<PF>(Nxt.ny[c]);
                     accc[c] = fresh<PF>(Nxt.nacc[c]);
                 }
             }
@@ -431,7 +466,7 @@ k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
 template <int MODEL, int CMB>
 constexpr int stage_slot()
 {
-    return MODEL == MODEL_NS && CMB == 2 ? 4 : MODEL;
+    return MODEL == MODEL_NS && CMB == 2 ? 4 : (MODEL >= MODEL_MHDU ? MODEL_MHD : MODEL);
 }
 
 int pick_ichunk(const Grid &g, int nkt, int njt)

@@ -1,0 +1,131 @@
+"""The timed configuration itself: Taylor-Green 256^3, FP64, SSP-RK3 + one
+FFT projection per step (app/TG_Piosson/RKLoop.H:1-54 with the SSP-RK3
+tableau), under the launch shapes the benchmark times, against the oracle
+and against the default shapes; plus the multi-chunk streaming paths of
+those shapes on grids small enough for the oracle, and one compact-scheme
+step at 256^3."""
+import math
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import RTOL, dev, host, pair, rel_l2
+
+pytestmark = pytest.mark.gpu
+TWO_PI = 2.0 * math.pi
+NU = 1.0 / 1600.0
+DT = 2e-3
+# the shapes of the profiled benchmark run: NS stages variant 8 (64 x 8
+# tiles) and the first stage variant 11 (two staged planes), both streaming
+# 128-plane chunks, projection variant 2
+BENCH_SHAPES = ((8, 128), (11, 128), 2)
+
+
+@pytest.fixture(scope="module")
+def nv():
+    import torch
+    import novaclone_amd as nv
+    assert torch.cuda.is_available(), "GPU tests need a device"
+    nv.lib()
+    return nv
+
+
+def tg(O, seed=None, amp=0.0):
+    x = [np.arange(O.stored[a]) * O.L[a] / O.N[a] for a in range(3)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    c = np.stack([np.sin(X) * np.cos(Y) * np.cos(Z), -np.cos(X) * np.sin(Y) * np.cos(Z), 0 * X])
+    if seed is not None:
+        c = c + amp * np.random.default_rng(seed).standard_normal(c.shape)
+    return np.ascontiguousarray(c)
+
+
+def set_shapes(nv, G, shapes):
+    (v, c), (vf, cf), pv = shapes
+    G.tune(v, c, pv)
+    G.tune_kernel(nv.KERNEL_NS_FIRST_STAGE, vf, cf)
+
+
+def gpu_step(nv, G, c, shapes, nsteps=1, autotune=False):
+    import torch
+    U, p = dev(c), G.field(1)
+    S = nv.Solver(nv.Poisson(G), nv.MODEL_NS, nv.RK_SSP3, nu=NU, graph=True)
+    if shapes is not None:
+        set_shapes(nv, G, shapes)
+    if autotune:
+        print(S.autotune(U, reps=2).splitlines()[-3:], flush=True)
+    S.step(DT, nsteps, U, p)
+    torch.cuda.synchronize()
+    return U, p
+
+
+def test_headline_step_at_256_matches_oracle_and_default_shapes(nv, orc):
+    import torch
+    N = (256, 256, 256)
+    G, O = pair(nv, orc, N, (TWO_PI,) * 3)
+    c = tg(O)
+    print("256^3 TG, one SSP-RK3 step, bench shapes", BENCH_SHAPES, flush=True)
+    U, p = gpu_step(nv, G, c, BENCH_SHAPES)
+    Ud, pd = gpu_step(nv, G, c, ((3, 0), (3, 0), 2))
+    assert torch.equal(U, Ud) and torch.equal(p, pd)
+    del Ud, pd
+    Uo, po = O.from_compact(c), O.field(1)
+    orc.OPoisson(O).ns_step(Uo, po, NU, DT, orc.RK_SSP3)
+    eu = rel_l2(host(U), O.interior(Uo))
+    ep = rel_l2(host(p), O.interior(po))
+    print("rel L2 U", eu, "p", ep, flush=True)
+    assert eu <= RTOL and ep <= RTOL
+
+
+def test_headline_autotuned_shapes_give_the_default_step(nv):
+    # whatever the autotuner picks on this box is what the benchmark times
+    import torch
+    N = (256, 256, 256)
+    G = nv.Mesh(N, (TWO_PI,) * 3)
+
+    class _O:
+        stored, L = N, (TWO_PI,) * 3
+    _O.N = N
+    c = tg(_O)
+    U, p = gpu_step(nv, G, c, None, nsteps=2, autotune=True)
+    Ud, pd = gpu_step(nv, G, c, ((3, 0), (3, 0), 2), nsteps=2)
+    assert torch.equal(U, Ud) and torch.equal(p, pd)
+
+
+@pytest.mark.parametrize("N,shapes", [
+    ((144, 16, 192), BENCH_SHAPES),            # 144 planes in a 128-plane and a 16-plane chunk, 3 k tiles
+    ((54, 24, 72), ((8, 16), (11, 16), 2)),    # three 16-plane chunks and a 6-plane one, a partial k tile
+    ((54, 24, 72), ((7, 16), (12, 8), 1)),     # 32 x 16 tiles, row kernels in the projection
+    ((36, 24, 144), ((9, 16), (10, 32), 0)),   # ring staging, divergence fused into the first FFT pass
+])
+def test_multi_chunk_streaming_matches_oracle(nv, orc, N, shapes):
+    import torch
+    L = (TWO_PI, TWO_PI, TWO_PI)
+    G, O = pair(nv, orc, N, L)
+    c = tg(O, seed=2, amp=1e-2)
+    Uo = O.from_compact(c)
+    OP = orc.OPoisson(O)
+    OP.project(Uo, 1.0)
+    c = O.interior(Uo)
+    U, p = gpu_step(nv, G, c, shapes, nsteps=2)
+    Ud, pd = gpu_step(nv, G, c, ((3, 0), (3, 0), 2), nsteps=2)
+    assert torch.equal(U, Ud)
+    po = O.field(1)
+    for _ in range(2):
+        OP.ns_step(Uo, po, NU, DT, orc.RK_SSP3)
+    e = rel_l2(host(U), O.interior(Uo))
+    print(N, shapes, "rel L2", e, flush=True)
+    assert e <= RTOL
+
+
+def test_compact_step_at_256_matches_oracle(nv, orc):
+    N = (256, 256, 256)
+    G, O = pair(nv, orc, N, (TWO_PI,) * 3)
+    c = tg(O)
+    U, p = dev(c), G.field(1)
+    S = nv.Solver(nv.Poisson(G, scheme=nv.SCHEME_COMPACT6), nv.MODEL_NS, nv.RK_SSP3, nu=NU)
+    S.step(DT, 1, U, p)
+    Uo, po = O.from_compact(c), O.field(1)
+    orc.OPoisson(O, compact=True).compact_ns_step(Uo, po, NU, DT)
+    e = rel_l2(host(U), O.interior(Uo))
+    print("compact 256^3 rel L2", e, flush=True)
+    assert e <= RTOL
