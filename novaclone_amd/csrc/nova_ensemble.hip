@@ -71,6 +71,7 @@ k_stage_group(const EnsInstance *__restrict__ inst, int ninst, int ntiles, int s
             a.out[0] = stage == 0 ? I.W1[0] : (stage == 1 ? I.W2[0] : I.U[0]);
             a.sa = sa;
             a.nu = I.nu;
+            a.nk = ns_k(I.g, I.nu, sa.dt);  // the products the host forms for a standalone stage
             a.ichunk = I.ichunk;
             a.nkt = I.nkt;
             a.njt = I.njt;

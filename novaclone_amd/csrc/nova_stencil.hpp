@@ -97,52 +97,101 @@ __device__ inline double grad_pm(const Grid &g, const Acc &A, int fb, int d, dou
     return d == 2 ? ad * zi : ad;
 }
 
-// NS right-hand side of all three components, direction by direction:
-// out[c] = -div(U, U_c) + nu lap(U_c) with the same terms as div_xy and
-// lap_c, organised so that only one direction's neighbours are live.
-template <int H, class Acc>
-__device__ inline void ns_rhs(const Grid &g, const Acc &A, double nu, double zi, double zl, double zm,
-                              double (&out)[3])
+// NS stage increment du = dt (-div(U, U) + nu lap(U)) of all three
+// components, with the weights folded per direction d and offset l + 1
+// (the same terms as div_xy and lap_c, regrouped):
+//   du_c = sum_d sum_l [ bp U_c(+) + bm U_c(-) ] + U_c(0) (dv - lam)
+//   bp = k1 (U_d(+) + U_d(0)) + kp,   bm = km - k1 (U_d(-) + U_d(0))
+//   dv = sum_d sum_l k1 (U_d(+) - U_d(-))
+// with k1 = -dt/2 wa_l s_d (s_d = 1/h_d, times 1/dzdxi on z),
+// kp = km = dt nu wb_l s2_d (s2_d = 1/h_d^2, times 1/dzdxi^2 on z; on z
+// minus / plus the metric term dt nu (d2zdxi2/dzdxi^3) wa_l / h_z) and
+// lam = dt nu wb_sum2 sum_d s2_d.  The advective half of bp, bm carries
+// the conservative flux U_c U_d and the U_d D U_c part of the skew form,
+// dv the U_c div U part (U(+), U(-), U(0): the values at the offset node
+// +(l+1), -(l+1) along d and at the node itself).  About 120 FP64
+// operations per node for m = 6.
+
+// Launch-uniform coefficients of the x and y directions and the z factors
+// before the per-k metric (products only, so host and device round them
+// alike).
+struct NsK {
+    double k1[2][kMaxH], kb[2][kMaxH];   // x, y: -dt/2 wa_l / h_d, dt nu wb_l / h_d^2
+    double k1z[kMaxH], kbz[kMaxH], kmz[kMaxH];  // z: -dt/2 wa_l / h_z, dt nu wb_l / h_z^2, dt nu wa_l / h_z
+    double lxy, lz;                      // dt nu wb_sum2 (1/h_x^2 + 1/h_y^2), dt nu wb_sum2 / h_z^2
+};
+
+__host__ __device__ inline NsK ns_k(const Grid &g, double nu, double dt)
 {
-    // per direction d and component c, with the weights factored as in the
-    // reference (c_l / (2(l+1)) times 1/h_d):
-    //   S1 = sum_l wa_l (U_c^+ U_d^+ - U_c^- U_d^-)   conservative flux
-    //   S2 = sum_l wa_l (U_c^+ - U_c^-)               first derivative
-    //   SL = sum_l wb_l (U_c^+ + U_c^-)               second derivative
-    // adv_c = sum_d s_d (S1 + U_d S2) + U_c div(U);  -div(U,U_c) = -adv_c / 2
+    NsK k{};
+    const double hdt = -0.5 * dt, ndt = nu * dt;
+    for (int l = 0; l < g.h; l++) {
+        for (int d = 0; d < 2; d++) {
+            k.k1[d][l] = hdt * (g.wa[l] * g.ih[d]);
+            k.kb[d][l] = ndt * (g.wb[l] * g.ih2[d]);
+        }
+        k.k1z[l] = hdt * (g.wa[l] * g.ih[2]);
+        k.kbz[l] = ndt * (g.wb[l] * g.ih2[2]);
+        k.kmz[l] = ndt * (g.wa[l] * g.ih[2]);
+    }
+    const double sxy = g.ih2[0] + g.ih2[1];
+    k.lxy = ndt * (g.wb_sum2 * sxy);
+    k.lz = ndt * (g.wb_sum2 * g.ih2[2]);
+    return k;
+}
+
+// The z coefficients of one thread (its k fixes the metric factors).
+template <int H>
+struct NsZ {
+    double k1[H], kp[H], km[H];
+    double lam;
+};
+
+template <int H>
+__device__ inline NsZ<H> ns_z(const NsK &k, double zi, double zl, double zm)
+{
+    NsZ<H> z;
+#pragma unroll
+    for (int l = 0; l < H; l++) {
+        z.k1[l] = k.k1z[l] * zi;
+        const double b = k.kbz[l] * zl, mt = k.kmz[l] * zm;
+        z.kp[l] = b - mt;
+        z.km[l] = b + mt;
+    }
+    z.lam = k.lxy + k.lz * zl;
+    return z;
+}
+
+template <int H, class Acc>
+__device__ inline void ns_du(const NsK &k, const NsZ<H> &z, const Acc &A, double (&du)[3])
+{
     double u0[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) u0[c] = A(0, c, 0, 0);
-    double adv[3] = {0.0, 0.0, 0.0}, lap[3] = {0.0, 0.0, 0.0}, dz1[3] = {0.0, 0.0, 0.0};
-    double divu = 0.0;
+    double acc[3] = {0.0, 0.0, 0.0}, dv = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        double s1[3] = {0.0, 0.0, 0.0}, s2[3] = {0.0, 0.0, 0.0}, sl[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int l = 0; l < H; l++) {
+            const double k1 = d == 2 ? z.k1[l] : k.k1[d][l];
+            const double kp = d == 2 ? z.kp[l] : k.kb[d][l];
+            const double km = d == 2 ? z.km[l] : k.kb[d][l];
             const double up = A(0, d, d, l + 1), um = A(0, d, d, -l - 1);
+            const double bp = __builtin_fma(k1, up + u0[d], kp);
+            const double bm = __builtin_fma(-k1, um + u0[d], km);
+            dv = __builtin_fma(k1, up - um, dv);
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 const double cp = c == d ? up : A(0, c, d, l + 1);
                 const double cm = c == d ? um : A(0, c, d, -l - 1);
-                s1[c] += g.wa[l] * (cp * up - cm * um);
-                s2[c] += g.wa[l] * (cp - cm);
-                sl[c] += g.wb[l] * (cp + cm);
+                acc[c] = __builtin_fma(bp, cp, acc[c]);
+                acc[c] = __builtin_fma(bm, cm, acc[c]);
             }
         }
-        const double sa = d == 2 ? g.ih[2] * zi : g.ih[d];
-        const double sb = d == 2 ? g.ih2[2] * zl : g.ih2[d];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            adv[c] += sa * (s1[c] + u0[d] * s2[c]);
-            lap[c] += sb * (sl[c] - g.wb_sum2 * u0[c]);
-            if (d == 2) dz1[c] = g.ih[2] * s2[c];
-        }
-        divu += sa * s2[d];
     }
+    const double g0 = dv - z.lam;
 #pragma unroll
-    for (int c = 0; c < 3; c++)
-        out[c] = (-(0.5 * (adv[c] + u0[c] * divu))) + nu * (lap[c] - zm * dz1[c]);
+    for (int c = 0; c < 3; c++) du[c] = __builtin_fma(u0[c], g0, acc[c]);
 }
 
 // Tangent-linear right-hand side J_R(X) d = -B(d, X) - B(X, d) + nu L d for

@@ -84,6 +84,8 @@ __global__ void __launch_bounds__(kTK *kTJ) k_stage_rows(Grid g, StreamArgs<NF> 
             for (int c = 0; c < 3; c++) q[f][c][o] = rsign<WALLS>(c, 0, fi) * a.X[f][c * g.np + po + col];
     }
     const double zi = g.zinv[k], zl = g.zlap[k], zm = g.zmet[k];
+    NsZ<H> nz{};
+    if (NF == 1) nz = ns_z<H>(a.nk, zi, zl, zm);
 
     for (int i = i0; i < i1; i++) {
         double nq[NF][3];
@@ -108,12 +110,12 @@ __global__ void __launch_bounds__(kTK *kTJ) k_stage_rows(Grid g, StreamArgs<NF> 
         const StageArgs &sa = a.sa;
         if (NF == 1) {
             double r[3];
-            ns_rhs<H>(g, A, a.nu, zi, zl, zm, r);
+            ns_du<H>(a.nk, nz, A, r);
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 const double u = q[0][c][H];
                 const double u0 = a.X0[0] ? a.X0[0][c * g.np + qi] : u;
-                a.out[0][c * g.np + qi] = combine(sa, u, u0, sa.dt * r[c]);
+                a.out[0][c * g.np + qi] = combine(sa, u, u0, r[c]);
             }
         } else {
 #pragma unroll
@@ -206,6 +208,7 @@ int ns_stage(nova_ctx *ctx, const double *U, const double *U0, double *out, cons
     a.out[0] = out;
     a.sa = sa;
     a.nu = nu;
+    a.nk = ns_k(ctx->g, nu, sa.dt);
     return launch_stream<1>(ctx, a, "ns_stage");
 }
 

@@ -38,6 +38,7 @@ struct StreamArgs {
     double *out[NF];
     StageArgs sa;
     double nu, eta;
+    NsK nk;  // NS stages: ns_k(g, nu, sa.dt)
     // adjoint stage only: out = ob * T + yg * Y, acc = acc_a * acc + acc_b * m
     const double *Y;
     double *acc;
@@ -249,6 +250,8 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
     };
 
     const double zi = g.zinv[km], zl = g.zlap[km], zm = g.zmet[km];
+    NsZ<H> nz{};
+    if constexpr (MODEL == MODEL_NS) nz = ns_z<H>(a.nk, zi, zl, zm);
     // step-start values of the own column, one plane ahead like the queue
     double x0c[NF][3];
 #pragma unroll
@@ -313,10 +316,10 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
             const StageArgs &sa = a.sa;
             if constexpr (MODEL == MODEL_NS) {
                 double r[3];
-                ns_rhs<H>(g, A, a.nu, zi, zl, zm, r);
+                ns_du<H>(a.nk, nz, A, r);
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    const double du = sa.dt * r[c];
+                    const double du = r[c];
                     const double u = A(0, c, 0, 0);
                     if constexpr (CMB == 1) {
                         const double v = u + du;

@@ -92,10 +92,10 @@ def test_headline_autotuned_shapes_give_the_default_step(nv):
 
 
 @pytest.mark.parametrize("N,shapes", [
-    ((144, 16, 192), BENCH_SHAPES),            # 144 planes in a 128-plane and a 16-plane chunk, 3 k tiles
-    ((54, 24, 72), ((8, 16), (11, 16), 2)),    # three 16-plane chunks and a 6-plane one, a partial k tile
-    ((54, 24, 72), ((7, 16), (12, 8), 1)),     # 32 x 16 tiles, row kernels in the projection
-    ((36, 24, 144), ((9, 16), (10, 32), 0)),   # ring staging, divergence fused into the first FFT pass
+    ((192, 16, 192), BENCH_SHAPES),            # 192 planes in a 128-plane and a 64-plane chunk, 3 k tiles
+    ((48, 24, 96), ((8, 16), (11, 32), 2)),    # 16-plane chunks; the first stage a 32 and a 16; a partial k tile
+    ((48, 24, 96), ((7, 16), (12, 8), 1)),     # 32 x 16 tiles, row kernels in the projection
+    ((48, 24, 192), ((9, 32), (10, 32), 0)),   # ring staging, divergence fused into the first FFT pass
 ])
 def test_multi_chunk_streaming_matches_oracle(nv, orc, N, shapes):
     import torch
