@@ -43,7 +43,7 @@ def ot_fields(O, seed, amp=1e-3):
 
 CASES = [
     dict(N=(24, 24, 24), L=(TWO_PI,) * 3, m=6, wall_y=False),
-    dict(N=(16, 20, 12), L=(TWO_PI, 1.5 * TWO_PI, TWO_PI), m=4, wall_y=False),
+    dict(N=(16, 24, 12), L=(TWO_PI, 1.5 * TWO_PI, TWO_PI), m=4, wall_y=False),
     dict(N=(16, 12, 16), L=(TWO_PI, 3.0, TWO_PI), m=6, wall_y=True),
     dict(N=(12, 16, 16), L=(TWO_PI,) * 3, m=8, wall_y=False),
 ]
