@@ -423,6 +423,17 @@ int nova_ensemble_info(nova_ensemble *e, int64_t out[6]);
  * periodic corner entries a[0] (row 0, col n-1) and c[n-1] (row n-1, col 0). */
 int nova_tdma_batched(nova_ctx *ctx, int n, int nbatch, const double *a, const double *b,
                       const double *c, const double *d, double *x, int cyclic);
+/* No counterpart in the reference: nova_tdma_batched for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).  It
+ * calls no hipMalloc/hipFree: the forward-sweep work array (3 n nbatch
+ * doubles) lives in a per-GPU workspace of nova_tdma_noalloc_max()
+ * doubles (16 MiB) that the library holds until the process ends; larger
+ * calls fail with NOVA_ERR_ARG.  The workspace needs no clearing.  Every
+ * call on one GPU shares that GPU's workspace: calls that overlap on one
+ * GPU (other streams, other host threads) race. */
+int nova_tdma_batched_noalloc(nova_ctx *ctx, int n, int nbatch, const double *a, const double *b,
+                              const double *c, const double *d, double *x, int cyclic);
+size_t nova_tdma_noalloc_max(void);
 
 #ifdef __cplusplus
 }

@@ -103,6 +103,7 @@ _SIGS = {
     "nova_reduce_dissipation": [_P, _P, _D, _DP],
     "nova_mhd_diag": [_P, _P, _P, _D, _D, _DP],
     "nova_tdma_batched": [_P, _I, _I, _P, _P, _P, _P, _P, _I],
+    "nova_tdma_batched_noalloc": [_P, _I, _I, _P, _P, _P, _P, _P, _I],
     "nova_wc_create": [_P, _D, _D, C.POINTER(_P)],
     "nova_wc_destroy": [_P],
     "nova_wc_step": [_P, _D, _I, _P, _P, _P],
@@ -169,6 +170,8 @@ def load_library(path: os.PathLike | str = LIB_PATH) -> C.CDLL:
         fn.restype = C.c_int
     lib.nova_last_error.argtypes = []
     lib.nova_last_error.restype = C.c_char_p
+    lib.nova_tdma_noalloc_max.argtypes = []
+    lib.nova_tdma_noalloc_max.restype = C.c_size_t
     return lib
 
 

@@ -252,7 +252,8 @@ static int ctx_init(nova_ctx *ctx, const nova_grid_desc *desc, int device)
     g.zlap = ctx->d_metric + s2;
     g.zmet = ctx->d_metric + 2 * s2;
     NOVA_HIP(hipHostMalloc((void **)&ctx->h_red, sizeof(double) * 16, hipHostMallocDefault));
-    return NOVA_OK;
+    // the device's copy of the tdma workspace, resolved with the device current
+    return tdma_scratch_init(device);
 }
 
 static void ctx_release(nova_ctx *ctx)
@@ -928,5 +929,15 @@ int nova_tdma_batched(nova_ctx *ctx, int n, int nbatch, const double *a, const d
     if (nbatch == 0) return NOVA_OK;
     NOVA_GUARD(launch_tdma(ctx, n, nbatch, a, b, c, d, x, cyclic));
 }
+
+int nova_tdma_batched_noalloc(nova_ctx *ctx, int n, int nbatch, const double *a, const double *b, const double *c,
+                              const double *d, double *x, int cyclic)
+{
+    NOVA_REQUIRE(ctx && a && b && c && d && x && n >= 2 && nbatch >= 0, "bad argument");
+    if (nbatch == 0) return NOVA_OK;
+    NOVA_GUARD(launch_tdma_noalloc(ctx, n, nbatch, a, b, c, d, x, cyclic));
+}
+
+size_t nova_tdma_noalloc_max(void) { return tdma_scratch_max(); }
 
 }  // extern "C"

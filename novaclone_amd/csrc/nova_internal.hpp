@@ -218,6 +218,11 @@ int reduce_mhd(nova_ctx *ctx, const double *U, const double *B, const double *J,
                const double *W, double mu, double eta, double out[8]);
 int launch_tdma(nova_ctx *ctx, int n, int nb, const double *a, const double *b, const double *c,
                 const double *d, double *x, int cyclic);
+// the same solve with its work array in the library's per-device workspace
+int tdma_scratch_init(int device);
+size_t tdma_scratch_max();
+int launch_tdma_noalloc(nova_ctx *ctx, int n, int nb, const double *a, const double *b, const double *c,
+                        const double *d, double *x, int cyclic);
 // compact sixth-order scheme (nova_compact.hip)
 int compact_supported(nova_ctx *ctx);
 double compact_kmod(double kh, int order);

@@ -6,6 +6,9 @@
 // maps, so the k-neighbours of a 64-wide wave share cache lines and the
 // j/i neighbours hit L2.  The fused time-step kernels (nova_step.hip) carry
 // the bandwidth-critical work; these serve the per-operator API.
+#include <mutex>
+#include <string>
+
 #include "nova_internal.hpp"
 
 namespace nova {
@@ -694,6 +697,52 @@ int reduce_mhd(nova_ctx *ctx, const double *U, const double *B, const double *J,
     out[6] = ctx->h_red[5];
     out[7] = 0.0;
     return NOVA_OK;
+}
+
+// Work array of nova_tdma_batched_noalloc: a file-scope device array, one
+// copy per GPU, held until the process ends, so that the entry point
+// allocates nothing.  Host code never uses the name `scratch` as an
+// address: tdma_scratch_init resolves each device's copy with
+// hipGetSymbolAddress once, with that device current, and every launch
+// passes the resolved pointer.  Calls that overlap on one GPU share it
+// and race.
+constexpr size_t kTdmaScratchDoubles = size_t(1) << 21;  // 16 MiB: 3 n nbatch doubles at most
+static __device__ __align__(256) double scratch[kTdmaScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int tdma_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(NOVA_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return NOVA_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[device] = static_cast<double *>(ptr);
+    return NOVA_OK;
+}
+
+size_t tdma_scratch_max() { return kTdmaScratchDoubles; }
+
+int launch_tdma_noalloc(nova_ctx *ctx, int n, int nb, const double *a, const double *b, const double *c,
+                        const double *d, double *x, int cyclic)
+{
+    if ((size_t)3 * (size_t)n * (size_t)nb > kTdmaScratchDoubles)
+        return fail(NOVA_ERR_ARG, "nova_tdma_batched_noalloc: 3 n nbatch = " + std::to_string(3 * (size_t)n * nb) +
+                                      " doubles exceed its workspace of " + std::to_string(kTdmaScratchDoubles));
+    double *work = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        work = g_scratch_ptr[ctx->device];
+    }
+    if (!work) return fail(NOVA_ERR_STATE, "the device's tdma workspace was not resolved at context creation");
+    hipLaunchKernelGGL(k_tdma, dim3((nb + 127) / 128), dim3(128), 0, ctx->stream, n, nb, a, b, c, d, x, work, cyclic);
+    return check_launch("tdma_noalloc");
 }
 
 int launch_tdma(nova_ctx *ctx, int n, int nb, const double *a, const double *b, const double *c,

@@ -24,7 +24,7 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in syms if not hasattr(lib, s)]
     assert not missing, missing
     # the python mirror binds exactly the declared set
-    assert set(nv._SIGS) | {"nova_last_error"} == set(syms)
+    assert set(nv._SIGS) | {"nova_last_error", "nova_tdma_noalloc_max"} == set(syms)
 
 
 def test_bad_arguments_are_rejected_with_a_message():
