@@ -55,29 +55,60 @@ def tg_field(nv, G, n):
     return U
 
 
-def cpu_baseline(n: int, steps: int) -> dict:
+def _median_step_s(step, warm: int, reps: int) -> float:
+    for _ in range(warm):
+        step()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        step()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2]
+
+
+def cpu_baseline(n: int, n1: int = 64, warm: int = 2, reps: int = 3) -> dict:
+    """The oracle's TG step (SSP-RK3 + FFT projection) timed on the host:
+    (i) faithful (the reference's operator-at-a-time passes with zeroed
+    temporaries, l-outer loops and a full complex FFT) and (ii) fused (one
+    OpenMP pass per stage, same arithmetic), each on all OpenMP threads at
+    n^3 and on one core at n1^3; steady_clock-like wall time per step,
+    median of `reps` steps after `warm` warm-up steps.  `value` is (i) on all
+    threads (the denominator of the GPU/CPU ratio)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import numpy as np
     import pyoracle
     L = (2 * math.pi,) * 3
-    O = pyoracle.OMesh((n, n, n), L)
-    P = pyoracle.OPoisson(O)
-    x = np.arange(n) * (2 * math.pi / n)
-    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
-    c = np.stack([np.sin(X) * np.cos(Y) * np.cos(Z), -np.cos(X) * np.sin(Y) * np.cos(Z), 0 * X])
-    del X, Y, Z
-    U = O.from_compact(c)
-    del c
-    p, Us = O.field(1), O.field(3)
-    P.ns_step(U, p, NU, DT, pyoracle.RK_SSP3, Us)  # warm-up
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        P.ns_step(U, p, NU, DT, pyoracle.RK_SSP3, Us)
-    dt = time.perf_counter() - t0
     cores = pyoracle.num_threads()
-    return {"value": n ** 3 * steps / dt / 1e6, "unit": "Mpoint-steps/s", "cores": cores, "kind": "port",
-            "sample": f"oracle TG {n}^3 SSP-RK3 + FFT projection, {steps} timed steps after 1 warm-up, "
-                      f"{cores} OpenMP threads, {dt:.1f} s"}
+
+    def setup(m):
+        O = pyoracle.OMesh((m, m, m), L)
+        P = pyoracle.OPoisson(O)
+        x = np.arange(m) * (2 * math.pi / m)
+        X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+        c = np.stack([np.sin(X) * np.cos(Y) * np.cos(Z), -np.cos(X) * np.sin(Y) * np.cos(Z), 0 * X])
+        del X, Y, Z
+        return O, P, O.from_compact(c)
+
+    out = {}
+    t_all = time.perf_counter()
+    for label, m, threads in (("all_threads", n, cores), ("one_core", n1, 1)):
+        pyoracle.set_threads(threads)
+        O, P, U = setup(m)
+        p, W1, W2 = O.field(1), O.field(3), O.field(3)
+        faithful = _median_step_s(lambda: P.ns_step(U, p, NU, DT, pyoracle.RK_SSP3, W1), warm, reps)
+        fused = _median_step_s(lambda: P.ns_step_fused(U, p, NU, DT, W1, W2), warm, reps)
+        out[label] = {"N": m, "threads": threads, "faithful_s_per_step": round(faithful, 4),
+                      "fused_s_per_step": round(fused, 4),
+                      "faithful_Mpoint_steps_per_s": round(m ** 3 / faithful / 1e6, 3),
+                      "fused_Mpoint_steps_per_s": round(m ** 3 / fused / 1e6, 3)}
+        del O, P, U, p, W1, W2
+    pyoracle.set_threads(cores)
+    a = out["all_threads"]
+    return {"value": a["faithful_Mpoint_steps_per_s"], "unit": "Mpoint-steps/s", "cores": cores, "kind": "port",
+            "sample": f"oracle TG SSP-RK3 + FFT projection: faithful and fused variants at {n}^3 on {cores} "
+                      f"OpenMP threads and at {n1}^3 on 1 core, median of {reps} steps after {warm} warm-up "
+                      f"steps each ({time.perf_counter() - t_all:.1f} s in all); value = faithful, all threads",
+            "variants": out}
 
 
 def shear_layer(nv, G, seed=5):
@@ -129,6 +160,11 @@ def adjoint_ratio(nv, steps: int, every: int) -> dict:
     tf = time.perf_counter() - t0
     UT = A.forward(dt, steps, U0)
     tuned += chosen(A.autotune(reps=3))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    UT = A.forward(dt, steps, U0)  # the checkpointing forward sweep every gradient needs
+    torch.cuda.synchronize()
+    tcf = time.perf_counter() - t0
     J, g = nv.enstrophy(G, UT, grad=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -138,6 +174,9 @@ def adjoint_ratio(nv, steps: int, every: int) -> dict:
     gn = nv.weighted_dot(G, g0, g0) ** 0.5
     A.close()
     return {"ratio": round(tb / tf, 3), "forward_s": round(tf, 3), "adjoint_s": round(tb, 3),
+            "checkpoint_forward_s": round(tcf, 3), "gradient_ratio": round((tcf + tb) / tf, 3),
+            "ratios": "ratio = adjoint (reverse) sweep / forward solve; gradient_ratio = (checkpointing forward "
+                      "+ reverse sweep) / forward solve, the whole cost of one gradient",
             "workload": f"shear layer 256x384x256 FP64, free-slip y walls (DCT-I), Re=1000, dt=0.02, "
                         f"{steps} SSP-RK3 steps, checkpoints every {every}; gradient of final enstrophy",
             "J": J, "grad_norm": gn, "tuning": tuned}
@@ -400,13 +439,122 @@ def chosen(report: str) -> list:
     return [ln for ln in report.splitlines() if "chosen" in ln]
 
 
+def _kname(full: str) -> str:
+    """rocprof kernel name without return type, namespaces and arguments."""
+    s = full.replace("nova::(anonymous namespace)::", "").replace("nova::", "")
+    if s.startswith("void "):
+        s = s[5:]
+    depth = 0
+    for i, ch in enumerate(s):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            return s[:i]
+    return s
+
+
+def _csv_rows(d: str, suffix: str) -> list:
+    import csv
+    rows = []
+    for f in sorted(Path(d).rglob(f"*{suffix}")):
+        with open(f, newline="") as fh:
+            rows += list(csv.DictReader(fh))
+    return rows
+
+
+# algorithmic bytes per node of the TG step's kernels, by rocprof kernel name
+# prefix (the seventh template argument of k_stage_stream: 2 = the first
+# SSP stage, which reads no step-start operand)
+def _alg_per_node(name: str, n: int) -> float | None:
+    half = (n // 2 + 1) / n * 16.0
+    if name.startswith("k_stage_stream<3, 0, false"):
+        return 48.0 if name.split(",")[6].strip() == "2" else 72.0
+    if name.startswith("k_stage_rows<3, 1"):
+        return 72.0
+    table = {"k_proj_stream<3, true": 32.0, "k_proj_stream<3, false": 56.0, "k_fwd_z<": 8.0 + half,
+             "k_fft_y<": 2 * half, "k_fft_x<": 2 * half, "k_inv_z<": half + 8.0, "k_div_rows": 32.0,
+             "k_correct_rows": 56.0}
+    for k, v in table.items():
+        if name.startswith(k):
+            return v
+    return None
+
+
+def summarize(dirs: list, commit: str | None, n: int) -> int:
+    """Kernel-statistics directories print a markdown table; a FETCH_SIZE
+    and a WRITE_SIZE directory plus their copy calibrations (in the order
+    fetch, write, fetch-calibration, write-calibration) print the per-kernel
+    HBM traffic JSON that the benchmark line cites."""
+    pts = n ** 3
+    if len(dirs) == 1:
+        rows = _csv_rows(dirs[0], "kernel_stats.csv")
+        print("| kernel | calls | avg (us) | algorithmic MB / launch | algorithmic GB/s | % of 8 TB/s |")
+        print("|---|---|---|---|---|---|")
+        for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"]))[:24]:
+            name = _kname(r["Name"])
+            avg = float(r["AverageNs"]) / 1e3
+            a = _alg_per_node(name, n)
+            if a is None:
+                print(f"| `{name[:90]}` | {r['Calls']} | {avg:.1f} | | | |")
+            else:
+                gbs = a * pts / (avg / 1e6) / 1e9
+                print(f"| `{name[:90]}` | {r['Calls']} | {avg:.1f} | {a * pts / 1e6:.0f} | {gbs:.0f} | "
+                      f"{100 * gbs / HBM_PEAK_GBS:.1f} |")
+        return 0
+    fetch, write, cf, cw = dirs
+
+    def per_kernel(d, counter):
+        acc = {}
+        for r in _csv_rows(d, "counter_collection.csv"):
+            if r.get("Counter_Name") != counter:
+                continue
+            name = _kname(r["Kernel_Name"])
+            v = acc.setdefault(name, [0.0, 0])
+            v[0] += float(r["Counter_Value"])
+            v[1] += 1
+        return {k: (v[0] / v[1], v[1]) for k, v in acc.items()}
+
+    def calib(d, counter):
+        vals = [v for k, (v, c) in per_kernel(d, counter).items() if "copyBuffer" in k or "copy" in k.lower()]
+        return max(vals) if vals else None
+
+    kf, kw = per_kernel(fetch, "FETCH_SIZE"), per_kernel(write, "WRITE_SIZE")
+    gib_kib = float(1 << 20)  # the calibration copy moves 1 GiB = 2^20 KiB each way
+    sf = gib_kib / calib(cf, "FETCH_SIZE")
+    sw = gib_kib / calib(cw, "WRITE_SIZE")
+    out = {"_commit": commit, "_grid": f"{n}^3",
+           "_method": f"rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE (KiB per dispatch, averaged over dispatches), "
+                      f"scaled by a plain 1 GiB copy's readings: fetch x {sf:.3f}, write x {sw:.3f}"}
+    groups = {"ns_stage": lambda k: k.startswith("k_stage_stream<3, 0, false"),
+              "div_rows": lambda k: k.startswith("k_proj_stream<3, true") or k.startswith("k_div_rows"),
+              "grad_correct": lambda k: k.startswith("k_proj_stream<3, false") or k.startswith("k_correct_rows"),
+              "fft_r2c_z": lambda k: k.startswith("k_fwd_z<"), "fft_y": lambda k: k.startswith("k_fft_y<"),
+              "fft_x_solve": lambda k: k.startswith("k_fft_x<"), "fft_c2r_z": lambda k: k.startswith("k_inv_z<")}
+    for g, sel in groups.items():
+        ks = [k for k in kf if sel(k) and k in kw]
+        if not ks:
+            continue
+        # the launches of the step dominate their group (the autotuner's trials
+        # run a few calls each): weight by dispatch count
+        tot = sum(kf[k][1] for k in ks)
+        f = sum(kf[k][0] * kf[k][1] for k in ks) / tot * 1024 * sf
+        w = sum(kw[k][0] * kw[k][1] for k in ks) / tot * 1024 * sw
+        out[g] = {"kernels": {k: {"dispatches": kf[k][1], "fetch_bytes": round(kf[k][0] * 1024 * sf),
+                                  "write_bytes": round(kw[k][0] * 1024 * sw)} for k in ks},
+                  "bytes_per_launch": round(f + w)}
+    print(json.dumps(out, indent=1))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="timed CPU steps per variant (median)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=500)
@@ -424,9 +572,17 @@ def main() -> int:
     ap.add_argument("--tune", nargs=3, type=int, metavar=("VARIANT", "CHUNK", "PROJ"),
                     help="fixed stage launch shape, i-chunk and projection variant instead of the autotuner "
                          "(reproducible profiling runs)")
+    ap.add_argument("--tune-first", nargs=2, type=int, metavar=("VARIANT", "CHUNK"),
+                    help="with --tune: the launch shape of the first SSP stage")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain device copy of 2^30 bytes 5 times (counter calibration)")
+    ap.add_argument("--summarize", nargs="+", metavar="ROCPROF_DIR",
+                    help="no GPU work: summarize rocprofv3 output directories (kernel statistics, or FETCH_SIZE "
+                         "and WRITE_SIZE counter runs with their copy calibrations) as markdown / JSON")
+    ap.add_argument("--commit", default=None, help="with --summarize: the source commit the runs measured")
     args = ap.parse_args()
+    if args.summarize:
+        return summarize(args.summarize, args.commit, args.n)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is design-only",
               file=sys.stderr)
@@ -454,6 +610,9 @@ def main() -> int:
     if args.tune:
         G.tune(*args.tune)
         tuned = [f"fixed: ns_stage variant {args.tune[0]} chunk {args.tune[1]}, projection variant {args.tune[2]}"]
+        if args.tune_first:
+            G.tune_kernel(nv.KERNEL_NS_FIRST_STAGE, *args.tune_first)
+            tuned.append(f"fixed: ns_first_stage variant {args.tune_first[0]} chunk {args.tune_first[1]}")
     else:
         # eight timed launches per trial: with three, launch-to-launch noise
         # decided between shapes a few microseconds apart
@@ -490,16 +649,25 @@ def main() -> int:
         kernels[name] = {"launches": cnt, "avg_ms": round(avg_ms, 5), "alg_bytes": b,
                          "GBps": round(b / (avg_ms / 1e3) / 1e9, 1)}
     dom = max(stats, key=lambda k: stats[k][1])
-    traffic = None
+    # HBM traffic per launch cannot be counted from inside this process (the
+    # counters need rocprofv3 --pmc around it): it is read from the committed
+    # counter summary, which names the command, the source commit and the
+    # kernels it measured, and is labelled as such
+    traffic, traffic_src = None, None
     tfile = ROOT / "profiles" / "pmc_traffic.json"
     if tfile.exists() and n == 256:
         try:
-            traffic = json.loads(tfile.read_text()).get(dom, {}).get("bytes_per_launch")
+            tj = json.loads(tfile.read_text())
+            entry = tj.get(dom, {})
+            traffic = entry.get("bytes_per_launch")
+            if traffic is not None:
+                traffic_src = {"file": "profiles/pmc_traffic.json", "measured_at_commit": tj.get("_commit"),
+                               "kernels": entry.get("kernels"), "method": tj.get("_method")}
         except (ValueError, AttributeError):
             traffic = None
     dk = kernels[dom]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dk["GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "frac": round(dk["GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_ms": dk["avg_ms"]}
     step_alg = 384.0 * n ** 3  # minimal-traffic model of one RK3 + projection step
     step_roofline = {"alg_bytes_per_step": step_alg, "achieved_GBps": round(step_alg / (ms_per_step / 1e3) / 1e9, 1),
@@ -576,7 +744,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (oracle) ...")
-        cpu = cpu_baseline(n, args.cpu_steps)
+        cpu = cpu_baseline(n, reps=args.cpu_steps)
         log(f"bench: CPU {cpu['value']:.3f} Mpt-steps/s on {cpu['cores']} threads")
 
     out = {

@@ -28,6 +28,7 @@ int mhd_u_stage(nova_ctx *ctx, const double *U, const double *B, double *out, co
     a.out[0] = out;
     a.sa = sa;
     a.nu = nu;
+    a.mk = mhd_k(ctx->g, nu, 0.0, sa.dt);
     return launch_stream_model<MODEL_MHDU>(ctx, a, "mhd_u_block");
 }
 
@@ -47,6 +48,7 @@ int mhd_b_stage(nova_ctx *ctx, const double *U, const double *B, const double *B
     a.sa.w[0] = w;
     a.sa.alpha = last ? 1.0 : 0.0;
     a.eta = eta;
+    a.mk = mhd_k(ctx->g, 0.0, eta, dt);
     return launch_stream_model<MODEL_MHDB>(ctx, a, "mhd_b_stage");
 }
 

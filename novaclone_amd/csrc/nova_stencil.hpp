@@ -194,6 +194,143 @@ __device__ inline void ns_du(const NsK &k, const NsZ<H> &z, const Acc &A, double
     for (int c = 0; c < 3; c++) du[c] = __builtin_fma(u0[c], g0, acc[c]);
 }
 
+// MHD stage increments of app/OT_Poisson/UEqn.H:16-23 and BEqn.H:15-23,
+//   du = dt (-div(U, U) + nu lap(U) + div(B, B) - grad(|B|^2 / 2))
+//   db = dt (-div(U, B) + div(B, U) + eta lap(B)),
+// regrouped per direction d and offset l + 1 as ns_du, with
+// k1 = -dt/2 wa_l s_d and the Laplacian weights kn = dt nu wb_l s2_d,
+// ke = dt eta wb_l s2_d (on z with the metric term, as in ns_du):
+//   du_c += (k1 aU+ + kn+) U_c(+) + (kn- - k1 aU-) U_c(-) + b+ B_c(+) + b- B_c(-)
+//   db_c += (k1 aU+ + ke+) B_c(+) + (ke- - k1 aU-) B_c(-) + b+ U_c(+) + b- U_c(-)
+//   du_d += k1 (|B(+)|^2 - |B(-)|^2)
+// with aU+- = U_d(+-) + U_d(0), b+ = -k1 (B_d(+) + B_d(0)),
+// b- = k1 (B_d(-) + B_d(0)), and at the end du_c += U_c(0) (dvU - lam_n)
+// - B_c(0) dvB, db_c += B_c(0) (dvU - lam_e) - U_c(0) dvB, with
+// dvX = sum k1 (X_d(+) - X_d(-)).  The per-direction factors are kept and the
+// offset weights wa_l, wb_l applied where they are used: two fields leave
+// too few registers to hold every coefficient.  Used by the U block and the
+// B stages of the reference's composition (one increment each); the SSP
+// stage that forms both increments at once keeps the per-component div_xy /
+// lap_c form, whose working set fits one wave's registers (the regrouped
+// form of both increments together spills).
+struct MhdK {
+    double K[3], N[3], E[3];  // -dt/2 / h_d, dt nu / h_d^2, dt eta / h_d^2
+    double mn, me;            // dt nu / h_z, dt eta / h_z (times wa_l d2zdxi2/dzdxi^3: the z-metric term)
+    double lnxy, lnz, lexy, lez;
+};
+
+__host__ __device__ inline MhdK mhd_k(const Grid &g, double nu, double eta, double dt)
+{
+    MhdK k{};
+    const double hdt = -0.5 * dt, ndt = nu * dt, edt = eta * dt;
+    for (int d = 0; d < 3; d++) {
+        k.K[d] = hdt * g.ih[d];
+        k.N[d] = ndt * g.ih2[d];
+        k.E[d] = edt * g.ih2[d];
+    }
+    k.mn = ndt * g.ih[2];
+    k.me = edt * g.ih[2];
+    const double sxy = g.ih2[0] + g.ih2[1];
+    k.lnxy = ndt * (g.wb_sum2 * sxy);
+    k.lnz = ndt * (g.wb_sum2 * g.ih2[2]);
+    k.lexy = edt * (g.wb_sum2 * sxy);
+    k.lez = edt * (g.wb_sum2 * g.ih2[2]);
+    return k;
+}
+
+// The z factors of one thread.
+struct MhdZ {
+    double K, N, E, Mn, Me, lamn, lame;
+};
+
+__device__ inline MhdZ mhd_z(const MhdK &k, double zi, double zl, double zm)
+{
+    MhdZ z;
+    z.K = k.K[2] * zi;
+    z.N = k.N[2] * zl;
+    z.E = k.E[2] * zl;
+    z.Mn = k.mn * zm;
+    z.Me = k.me * zm;
+    z.lamn = k.lnxy + k.lnz * zl;
+    z.lame = k.lexy + k.lez * zl;
+    return z;
+}
+
+// WU / WB: which increments to form (the reference's composition advances U
+// and B in separate blocks); fields 0 = U, 1 = B of the accessor.
+template <int H, bool WU, bool WB, class Acc>
+__device__ inline void mhd_dudb(const Grid &g, const MhdK &k, const MhdZ &z, const Acc &A, double (&du)[3],
+                                double (&db)[3])
+{
+    double u0[3], b0[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        u0[c] = A(0, c, 0, 0);
+        b0[c] = A(1, c, 0, 0);
+        du[c] = 0.0;
+        db[c] = 0.0;
+    }
+    double dvu = 0.0, dvb = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+#pragma unroll
+        for (int l = 0; l < H; l++) {
+            const double wa = g.wa[l], wb = g.wb[l];
+            const double k1 = (d == 2 ? z.K : k.K[d]) * wa;
+            const double up = A(0, d, d, l + 1), um = A(0, d, d, -l - 1);
+            const double bp = A(1, d, d, l + 1), bm = A(1, d, d, -l - 1);
+            const double aup = up + u0[d], aum = um + u0[d];
+            const double bcp = -k1 * (bp + b0[d]), bcm = k1 * (bm + b0[d]);
+            dvu = __builtin_fma(k1, up - um, dvu);
+            dvb = __builtin_fma(k1, bp - bm, dvb);
+            double ap = 0.0, am = 0.0, ep = 0.0, em = 0.0;
+            if constexpr (WU) {
+                const double b = (d == 2 ? z.N : k.N[d]) * wb;
+                const double mt = d == 2 ? z.Mn * wa : 0.0;
+                ap = __builtin_fma(k1, aup, b - mt);
+                am = __builtin_fma(-k1, aum, b + mt);
+                double q = 0.0;
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const double vp = c == d ? bp : A(1, c, d, l + 1), vm = c == d ? bm : A(1, c, d, -l - 1);
+                    q = __builtin_fma(vp, vp, q);
+                    q = __builtin_fma(-vm, vm, q);
+                }
+                du[d] = __builtin_fma(k1, q, du[d]);
+            }
+            if constexpr (WB) {
+                const double b = (d == 2 ? z.E : k.E[d]) * wb;
+                const double mt = d == 2 ? z.Me * wa : 0.0;
+                ep = __builtin_fma(k1, aup, b - mt);
+                em = __builtin_fma(-k1, aum, b + mt);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double cup = c == d ? up : A(0, c, d, l + 1), cum = c == d ? um : A(0, c, d, -l - 1);
+                const double cbp = c == d ? bp : A(1, c, d, l + 1), cbm = c == d ? bm : A(1, c, d, -l - 1);
+                if constexpr (WU) {
+                    du[c] = __builtin_fma(ap, cup, du[c]);
+                    du[c] = __builtin_fma(am, cum, du[c]);
+                    du[c] = __builtin_fma(bcp, cbp, du[c]);
+                    du[c] = __builtin_fma(bcm, cbm, du[c]);
+                }
+                if constexpr (WB) {
+                    db[c] = __builtin_fma(ep, cbp, db[c]);
+                    db[c] = __builtin_fma(em, cbm, db[c]);
+                    db[c] = __builtin_fma(bcp, cup, db[c]);
+                    db[c] = __builtin_fma(bcm, cum, db[c]);
+                }
+            }
+        }
+    }
+    const double gu = dvu - z.lamn, gb = dvu - z.lame;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if constexpr (WU) du[c] = __builtin_fma(u0[c], gu, __builtin_fma(-b0[c], dvb, du[c]));
+        if constexpr (WB) db[c] = __builtin_fma(b0[c], gb, __builtin_fma(-u0[c], dvb, db[c]));
+    }
+}
+
 // Tangent-linear right-hand side J_R(X) d = -B(d, X) - B(X, d) + nu L d for
 // all three components (X = field 0, d = field 1), direction by direction:
 //   B(d,X)_c + B(X,d)_c = 1/2 sum_d [ D_d(d_d X_c + X_d d_c) + d_d D_d X_c

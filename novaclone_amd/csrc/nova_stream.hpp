@@ -38,7 +38,8 @@ struct StreamArgs {
     double *out[NF];
     StageArgs sa;
     double nu, eta;
-    NsK nk;  // NS stages: ns_k(g, nu, sa.dt)
+    NsK nk;   // NS stages: ns_k(g, nu, sa.dt)
+    MhdK mk;  // MHD stages: mhd_k(g, nu, eta, sa.dt)
     // adjoint stage only: out = ob * T + yg * Y, acc = acc_a * acc + acc_b * m
     const double *Y;
     double *acc;
@@ -251,7 +252,9 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
 
     const double zi = g.zinv[km], zl = g.zlap[km], zm = g.zmet[km];
     NsZ<H> nz{};
+    MhdZ mz{};
     if constexpr (MODEL == MODEL_NS) nz = ns_z<H>(a.nk, zi, zl, zm);
+    if constexpr (MODEL == MODEL_MHDU || MODEL == MODEL_MHDB) mz = mhd_z(a.mk, zi, zl, zm);
     // step-start values of the own column, one plane ahead like the queue
     double x0c[NF][3];
 #pragma unroll
@@ -352,19 +355,19 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                     stp(a.out[0], c, pi, col, o);
                 }
             } else if constexpr (MODEL == MODEL_MHDU) {
+                double du[3], db[3];
+                mhd_dudb<H, true, false>(g, a.mk, mz, A, du, db);
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    const double du =
-                        sa.dt * ((((-div_xy<H>(g, A, 0, 0, c, zi)) + lap_c<H>(g, A, 0, c, a.nu, zl, zm)) +
-                                  div_xy<H>(g, A, 1, 1, c, zi)) - grad_pm<H>(g, A, 1, c, zi));
                     const double u = A(0, c, 0, 0);
-                    stp(a.out[0], c, pi, col, combine(sa, u, u, du));
+                    stp(a.out[0], c, pi, col, combine(sa, u, u, du[c]));
                 }
             } else if constexpr (MODEL == MODEL_MHDB) {
+                double du[3], dbv[3];
+                mhd_dudb<H, false, true>(g, a.mk, mz, A, du, dbv);
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    const double db = sa.dt * (((-div_xy<H>(g, A, 0, 1, c, zi)) + div_xy<H>(g, A, 1, 0, c, zi)) +
-                                               lap_c<H>(g, A, 1, c, a.eta, zl, zm));
+                    const double db = dbv[c];
                     const double s = (a.acc_a != 0.0 ? accc[c] : x0c[1][c]) + sa.w[0] * db;
                     if (sa.alpha != 0.0) {
                         stp(a.out[0], c, pi, col, s);

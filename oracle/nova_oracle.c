@@ -1054,6 +1054,111 @@ void orc_ns_step(const orc_mesh *M, orc_poisson *P, double *U, double *Ustar, do
     free(U0); free(dU); free(t1); free(t2);
 }
 
+/* CPU baseline variant (ii), "fused": the SSP-RK3 step of orc_ns_step with
+ * the same operators evaluated one node at a time in a single OpenMP pass
+ * per stage (weights precomputed, every term of -div(U,U) + laplacian(nu,U)
+ * and the stage combination accumulated in registers, the output written
+ * once), the projection with one fused divergence pass, the same FFT solve
+ * and one fused correction pass.  No reference row: it times the
+ * reference's arithmetic without its operator-at-a-time memory traffic.
+ * Agrees with orc_ns_step(..., ORC_RK_SSP3) to rounding. */
+static void fused_ns_stage(const orc_mesh *M, const double *X, const double *U0, double *out, double nu,
+                           double dt, double alpha, double beta)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    double a[3][4], b[3][4];
+    for (int l = 0; l < h; l++)
+        for (int d = 0; d < 3; d++) {
+            a[d][l] = M->coef[l] / 2.0 / (l + 1) / M->d[d];
+            b[d][l] = M->coef[l] / (((l + 1) * M->d[d]) * ((l + 1) * M->d[d]));
+        }
+    const long sj = (long)nk, si = (long)nj * nk, sk = 1;
+    const long st[3] = {si, sj, sk};
+    #pragma omp parallel for collapse(2)
+    for (int i = h; i < ni - h; i++)
+        for (int j = h; j < nj - h; j++)
+            for (int k = h; k < nk - h; k++) {
+                const long o = (long)IX(M, i, j, k);
+                const double zi = 1.0 / M->dzdxi[k], zl = zi * zi;
+                const double zm = M->d2zdxi2[k] * zi * zi * zi;
+                const double *x0 = &X[o * 3];
+                double adv[3] = {0, 0, 0}, lap[3] = {0, 0, 0}, dz1[3] = {0, 0, 0};
+                for (int d = 0; d < 3; d++) {
+                    const double sa = d == 2 ? zi : 1.0, sb = d == 2 ? zl : 1.0;
+                    for (int l = 0; l < h; l++) {
+                        const double *xp = &X[(o + (l + 1) * st[d]) * 3], *xm = &X[(o - (l + 1) * st[d]) * 3];
+                        for (int q = 0; q < 3; q++) {
+                            adv[q] += sa * a[d][l] * ((xp[q] * xp[d] - xm[q] * xm[d]) + x0[d] * (xp[q] - xm[q])
+                                                      + x0[q] * (xp[d] - xm[d]));
+                            lap[q] += sb * b[d][l] * ((xp[q] + xm[q]) - 2.0 * x0[q]);
+                            if (d == 2) dz1[q] += a[2][l] * (xp[q] - xm[q]);
+                        }
+                    }
+                }
+                for (int q = 0; q < 3; q++) {
+                    const double du = dt * ((-(0.5 * adv[q])) + nu * (lap[q] - zm * dz1[q]));
+                    const double v = x0[q] + du;
+                    out[o * 3 + q] = alpha != 0.0 ? alpha * U0[o * 3 + q] + beta * v : v;
+                }
+            }
+}
+
+void orc_ns_step_fused(const orc_mesh *M, orc_poisson *P, double *U, double *W1, double *W2, double *p, double nu,
+                       double dt)
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    fused_ns_stage(M, U, U, W1, nu, dt, 0.0, 1.0);
+    orc_fill(M, W1, 3, 1);
+    fused_ns_stage(M, W1, U, W2, nu, dt, 0.75, 0.25);
+    orc_fill(M, W2, 3, 1);
+    fused_ns_stage(M, W2, U, W1, nu, dt, 1.0 / 3.0, 2.0 / 3.0);
+    orc_fill(M, W1, 3, 1);
+    /* projection: divergence / dt in one pass, the FFT solve, one correction pass */
+    double *dv = falloc(M, 1);
+    const long st[3] = {(long)nj * nk, (long)nk, 1};
+    double a[3][4];
+    for (int l = 0; l < h; l++)
+        for (int d = 0; d < 3; d++) a[d][l] = M->coef[l] / 2.0 / (l + 1) / M->d[d];
+    #pragma omp parallel for collapse(2)
+    for (int i = h; i < ni - h; i++)
+        for (int j = h; j < nj - h; j++)
+            for (int k = h; k < nk - h; k++) {
+                const long o = (long)IX(M, i, j, k);
+                double s = 0.0;
+                for (int d = 0; d < 3; d++) {
+                    double t = 0.0;
+                    for (int l = 0; l < h; l++) t += a[d][l] * (W1[(o + (l + 1) * st[d]) * 3 + d] - W1[(o - (l + 1) * st[d]) * 3 + d]);
+                    s += d == 2 ? t / M->dzdxi[k] : t;
+                }
+                dv[o] = s / dt;
+            }
+    orc_poisson_solve(P, dv, p);
+    orc_fill(M, p, 1, -1);
+    #pragma omp parallel for collapse(2)
+    for (int i = h; i < ni - h; i++)
+        for (int j = h; j < nj - h; j++)
+            for (int k = h; k < nk - h; k++) {
+                const long o = (long)IX(M, i, j, k);
+                for (int d = 0; d < 3; d++) {
+                    double t = 0.0;
+                    for (int l = 0; l < h; l++) t += a[d][l] * (p[o + (l + 1) * st[d]] - p[o - (l + 1) * st[d]]);
+                    if (d == 2) t /= M->dzdxi[k];
+                    U[o * 3 + d] = W1[o * 3 + d] - dt * t;
+                }
+            }
+    orc_fill(M, U, 3, 1);
+    free(dv);
+}
+
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 void orc_mhd_step(const orc_mesh *M, orc_poisson *P, double *U, double *B, double *p,
                   double *pB, double nu, double eta, double dt, int mode)
 {

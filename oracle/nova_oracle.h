@@ -92,6 +92,9 @@ void orc_ns_step(const orc_mesh *, orc_poisson *, double *U, double *Ustar,
                  double *p, double nu, double dt, int mode);
 /* One MHD step.  mode 0 = app/OT_Poisson/{UEqn,BEqn}.H literally;
  * mode 1 = SSP-RK3 with u and B projected at every stage. */
+void orc_ns_step_fused(const orc_mesh *, orc_poisson *, double *U, double *W1, double *W2, double *p, double nu,
+                       double dt);
+void orc_set_threads(int n);
 void orc_mhd_rhs(const orc_mesh *, const double *U, const double *B, double nu, double eta,
                  double *dU, double *dB);
 void orc_mhd_step(const orc_mesh *, orc_poisson *, double *U, double *B,

@@ -60,6 +60,8 @@ def _load() -> C.CDLL:
         "orc_poisson_k2": ([_P, _D], None),
         "orc_poisson_solve": ([_P, _D, _D], None),
         "orc_ns_step": ([_P, _P, _D, _D, _D, C.c_double, C.c_double, C.c_int], None),
+        "orc_ns_step_fused": ([_P, _P, _D, _D, _D, _D, C.c_double, C.c_double], None),
+        "orc_set_threads": ([C.c_int], None),
         "orc_mhd_rhs": ([_P, _D, _D, C.c_double, C.c_double, _D, _D], None),
         "orc_mhd_step": ([_P, _P, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_int], None),
         "orc_wc_step": ([_P, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_double], None),
@@ -359,6 +361,12 @@ class OPoisson:
             Ustar = self.mesh.field(3)
         lib().orc_ns_step(self.mesh._h, self._h, _d(U), _d(Ustar), _d(p), nu, dt, mode)
 
+    def ns_step_fused(self, U, p, nu, dt, W1=None, W2=None):
+        """SSP-RK3 step with one fused pass per stage (CPU baseline variant)."""
+        W1 = self.mesh.field(3) if W1 is None else W1
+        W2 = self.mesh.field(3) if W2 is None else W2
+        lib().orc_ns_step_fused(self.mesh._h, self._h, _d(U), _d(W1), _d(W2), _d(p), nu, dt)
+
     def mhd_step(self, U, B, p, pB, nu, eta, dt, mode):
         lib().orc_mhd_step(self.mesh._h, self._h, _d(U), _d(B), _d(p), _d(pB), nu, eta, dt, mode)
 
@@ -404,6 +412,10 @@ def compact_kmod(kh, order=1):
 
 def num_threads() -> int:
     return lib().orc_num_threads()
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))
 
 
 class OTime:

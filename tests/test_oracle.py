@@ -359,3 +359,24 @@ def test_mesh_metric_uniform_and_stretched(orc):
     assert abs(z0) < 1e-14 and abs(z1 - 2.0) < 1e-14
     zs = [lib().orc_mesh_z(Ms._h, k) for k in range(h, h + 17)]
     assert np.all(np.diff(zs) > 0)
+
+
+@pytest.mark.parametrize("wall_y,hyp", [(False, None), (True, None), (False, (0.02, 0.03))])
+def test_fused_cpu_baseline_step_equals_the_faithful_step(orc, wall_y, hyp):
+    # the fused variant times the same arithmetic: it must agree with the
+    # operator-at-a-time step to rounding
+    rng = np.random.default_rng(5)
+    M = orc.OMesh((12, 16, 12), (TWO_PI, 3.0 if wall_y else TWO_PI, TWO_PI), wall_y=wall_y, hyp=hyp)
+    P = orc.OPoisson(M)
+    c = rng.standard_normal((3, *M.stored))
+    if wall_y:
+        c[1, :, 0, :] = c[1, :, -1, :] = 0.0
+    U = M.from_compact(c)
+    P.project(U, 1.0)
+    V = U.copy()
+    p, q = M.field(1), M.field(1)
+    P.ns_step(U, p, 0.05, 0.01, orc.RK_SSP3)
+    P.ns_step_fused(V, q, 0.05, 0.01)
+    S = interior_slices(M)
+    assert np.max(np.abs(U[S] - V[S])) < 1e-13 * np.max(np.abs(U[S]))
+    assert np.max(np.abs(p[S] - q[S])) < 1e-12 * np.max(np.abs(p[S]))
