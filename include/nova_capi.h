@@ -172,6 +172,33 @@ int nova_curl(nova_ctx *ctx, const double *U, double *out);
 /* out = a*x + b*y over count doubles (Field algebra, Field.hxx:617-1150) */
 int nova_axpby(nova_ctx *ctx, size_t count, double a, const double *x, double b,
                const double *y, double *out);
+/* Pointwise Field products (src/Field/Field.h:227-363, Field.cpp:88-150 over
+ * the node operators of src/Types), each node computed as the reference's
+ * node type computes it (same operand order, no fused multiply-adds):
+ *   NOVA_OP_DOT       a & b: nca = ncb = 1 or 3 -> 1 component
+ *   NOVA_OP_CROSS     a ^ b: 3, 3 -> 3
+ *   NOVA_OP_SCALE     a * b with a a scalar field: 1, ncb -> ncb
+ *   NOVA_OP_DIVIDE    a / b with b a scalar field: nca, 1 -> nca
+ *   NOVA_OP_DDOT      a && b of tensors (9, 9 -> 1), as tensor.cpp:237-251
+ *                     writes it: the sum over every component pair (i, j)
+ *   NOVA_OP_SQRT      sqrt(a): 1 -> 1 (b unused)
+ *   NOVA_OP_OUTER     a * b of vectors: 3, 3 -> 9 (tensor, row-major)
+ *   NOVA_OP_ADD_CONST a + cval (nca components; Field - T with cval = -T)
+ * out may alias a or b only for DOT, SQRT, SCALE, DIVIDE and ADD_CONST when
+ * it has the components of the aliased operand. */
+enum { NOVA_OP_DOT = 0, NOVA_OP_CROSS = 1, NOVA_OP_SCALE = 2, NOVA_OP_DIVIDE = 3, NOVA_OP_DDOT = 4,
+       NOVA_OP_SQRT = 5, NOVA_OP_OUTER = 6, NOVA_OP_ADD_CONST = 7 };
+int nova_field_op(nova_ctx *ctx, int op, const double *a, int nca, const double *b, int ncb,
+                  const double *cval, double *out);
+/* ex::grad(Field<vector>) -> Field<tensor>, src/ex/grad/grad.cpp:65-102: the
+ * half-shifted literal stencil, 9 components, row = component, column =
+ * direction (the tensor that ex::curl takes the antisymmetric part of). */
+int nova_grad_vec_literal(nova_ctx *ctx, const double *U, double *out9);
+/* tools::CFL(U, mesh), src/Tools/Tools.cpp:9-41: out[0..2] the values it
+ * prints, with its cfl_y = max(cfl_x, U_y dt / dy at the last node of the
+ * loop) and cfl_z alike, as written; out[3..5] the per-direction maxima of
+ * |U_d| dt / h_d (the Courant numbers). */
+int nova_cfl(nova_ctx *ctx, const double *U, double dt, double out[6]);
 
 /* ---- Poisson (src/poisson/poisson.cpp) ---- */
 /* poisson::poisson + wavenumbersFFT_, poisson.cpp:6-164,337-396.  Periodic

@@ -30,6 +30,10 @@ KERNEL_NS_STAGE, KERNEL_MHD_STAGE, KERNEL_TLM_STAGE, KERNEL_ADJ_STAGE, KERNEL_NS
 SCHEME_EXPLICIT, SCHEME_COMPACT6 = 0, 1
 # reference BC kinds for the padded-layout fill (src/BC/BCenums.h:8)
 PBC_PERIODIC, PBC_NEUMANN, PBC_DIRICHLET, PBC_EVEN, PBC_ODD = 0, 1, 2, 10, 11
+# pointwise Field products (nova_field_op)
+OP_DOT, OP_CROSS, OP_SCALE, OP_DIVIDE, OP_DDOT, OP_SQRT, OP_OUTER, OP_ADD_CONST = range(8)
+_OP_OUT = {OP_DOT: lambda a, b: 1, OP_CROSS: lambda a, b: 3, OP_SCALE: lambda a, b: b, OP_DIVIDE: lambda a, b: a,
+           OP_DDOT: lambda a, b: 1, OP_SQRT: lambda a, b: 1, OP_OUTER: lambda a, b: 9, OP_ADD_CONST: lambda a, b: a}
 DIRS = {"north": 0, "south": 1, "east": 2, "west": 3, "top": 4, "bottom": 5}
 
 
@@ -86,6 +90,9 @@ _SIGS = {
     "nova_curl_literal": [_P, _P, _P],
     "nova_curl": [_P, _P, _P],
     "nova_axpby": [_P, C.c_size_t, _D, _P, _D, _P, _P],
+    "nova_field_op": [_P, _I, _P, _I, _P, _I, _DP, _P],
+    "nova_grad_vec_literal": [_P, _P, _P],
+    "nova_cfl": [_P, _P, _D, _DP],
     "nova_poisson_create": [_P, C.POINTER(_P)],
     "nova_poisson_destroy": [_P],
     "nova_poisson_solve": [_P, _P, _P],
@@ -728,6 +735,36 @@ class WC:
             self.close()
         except Exception:
             pass
+
+
+def field_op(mesh: Mesh, op: int, a: torch.Tensor, b: torch.Tensor | None = None, cval=None) -> torch.Tensor:
+    """The reference's pointwise Field products (Field.h:227-363): OP_DOT
+    (a & b), OP_CROSS (a ^ b), OP_SCALE (scalar field a times b), OP_DIVIDE
+    (a over scalar field b), OP_DDOT (tensors a && b), OP_SQRT, OP_OUTER
+    (vectors a * b), OP_ADD_CONST (a + cval)."""
+    nca = a.shape[0]
+    ncb = b.shape[0] if b is not None else 0
+    out = mesh.field(_OP_OUT[op](nca, ncb))
+    cv = (C.c_double * 3)(*(list(cval) + [0.0] * 3)[:3]) if cval is not None else None
+    _check(lib().nova_field_op(mesh.handle, int(op), _fld(a, mesh, nca, "a"), nca, _fld(b, mesh, ncb, "b"), ncb,
+                               cv, _ptr(out)), "nova_field_op")
+    return out
+
+
+def grad_vec_literal(mesh: Mesh, U: torch.Tensor) -> torch.Tensor:
+    """ex::grad(Field<vector>): the literal half-shifted gradient tensor
+    (9 components, row = component, column = direction)."""
+    out = mesh.field(9)
+    _check(lib().nova_grad_vec_literal(mesh.handle, _fld(U, mesh, 3, "U"), _ptr(out)), "nova_grad_vec_literal")
+    return out
+
+
+def cfl(mesh: Mesh, U: torch.Tensor, dt: float):
+    """tools::CFL: (cfl_x, cfl_y, cfl_z) as the reference prints them, then
+    the Courant numbers max |U_d| dt / h_d."""
+    out = (C.c_double * 6)()
+    _check(lib().nova_cfl(mesh.handle, _fld(U, mesh, 3, "U"), float(dt), out), "nova_cfl")
+    return list(out)
 
 
 def ke(mesh: Mesh, U: torch.Tensor) -> float:

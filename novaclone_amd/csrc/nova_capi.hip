@@ -520,6 +520,38 @@ int nova_axpby(nova_ctx *ctx, size_t count, double a, const double *x, double b,
     NOVA_GUARD(launch_axpby(ctx, count, a, x, b, y, out));
 }
 
+int nova_field_op(nova_ctx *ctx, int op, const double *a, int nca, const double *b, int ncb, const double *cval,
+                  double *out)
+{
+    NOVA_REQUIRE(ctx && a && out, "bad argument");
+    bool ok = false;
+    switch (op) {
+    case NOVA_OP_DOT: ok = b && nca == ncb && (nca == 1 || nca == 3); break;
+    case NOVA_OP_CROSS: ok = b && nca == 3 && ncb == 3 && out != a && out != b; break;
+    case NOVA_OP_SCALE: ok = b && nca == 1 && (ncb == 1 || ncb == 3 || ncb == 9) && out != a; break;
+    case NOVA_OP_DIVIDE: ok = b && ncb == 1 && (nca == 1 || nca == 3 || nca == 9) && out != b; break;
+    case NOVA_OP_DDOT: ok = b && nca == 9 && ncb == 9; break;
+    case NOVA_OP_SQRT: ok = nca == 1; break;
+    case NOVA_OP_OUTER: ok = b && nca == 3 && ncb == 3 && out != a && out != b; break;
+    case NOVA_OP_ADD_CONST: ok = cval && (nca == 1 || nca == 3); break;
+    default: break;
+    }
+    NOVA_REQUIRE(ok, "nova_field_op: operator and component counts do not match (see nova_capi.h)");
+    NOVA_GUARD(field_op(ctx, op, a, nca, b, ncb, cval, out));
+}
+
+int nova_grad_vec_literal(nova_ctx *ctx, const double *U, double *out9)
+{
+    NOVA_REQUIRE(ctx && U && out9 && U != out9, "bad argument");
+    NOVA_GUARD(grad_vec_literal(ctx, U, out9));
+}
+
+int nova_cfl(nova_ctx *ctx, const double *U, double dt, double out[6])
+{
+    NOVA_REQUIRE(ctx && U && out, "bad argument");
+    NOVA_GUARD(reduce_cfl(ctx, U, dt, out));
+}
+
 int nova_poisson_create(nova_ctx *ctx, nova_poisson **out)
 {
     NOVA_REQUIRE(ctx && out, "null argument");

@@ -210,6 +210,10 @@ int launch_axpby(nova_ctx *ctx, size_t n, double a, const double *x, double b, c
 int launch_pack(nova_ctx *ctx, const double *compact, int nc, double *padded);
 int launch_unpack(nova_ctx *ctx, const double *padded, int nc, double *compact);
 int launch_bc_fill(nova_ctx *ctx, double *padded, int nc, int dir, int kind, int comp, double val);
+int field_op(nova_ctx *ctx, int op, const double *a, int nca, const double *b, int ncb, const double *cval,
+             double *out);
+int grad_vec_literal(nova_ctx *ctx, const double *U, double *out);
+int reduce_cfl(nova_ctx *ctx, const double *U, double dt, double out[6]);
 int reduce_ke(nova_ctx *ctx, const double *U, double *out);
 int reduce_maxabs(nova_ctx *ctx, const double *f, double *out);
 int reduce_dissipation(nova_ctx *ctx, const double *U, double nu, double *out);

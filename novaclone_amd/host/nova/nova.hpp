@@ -51,9 +51,30 @@ public:
     scalar &operator[](int i) { return v_[i]; }
 };
 
+// src/Types/tensor/tensor.h:25: nine components, row-major (xx, xy, xz, yx, ...)
+class tensor {
+    scalar v_[9];
+
+public:
+    tensor() : v_{0, 0, 0, 0, 0, 0, 0, 0, 0} {}
+    tensor(scalar a) : v_{a, a, a, a, a, a, a, a, a} {}
+    scalar operator[](int i) const { return v_[i]; }
+    scalar &operator[](int i) { return v_[i]; }
+    scalar xx() const { return v_[0]; }
+    scalar xy() const { return v_[1]; }
+    scalar xz() const { return v_[2]; }
+    scalar yx() const { return v_[3]; }
+    scalar yy() const { return v_[4]; }
+    scalar yz() const { return v_[5]; }
+    scalar zx() const { return v_[6]; }
+    scalar zy() const { return v_[7]; }
+    scalar zz() const { return v_[8]; }
+};
+
 template <class T> struct pTraits;
 template <> struct pTraits<scalar> { static constexpr int nComp = 1; };
 template <> struct pTraits<vector> { static constexpr int nComp = 3; };
+template <> struct pTraits<tensor> { static constexpr int nComp = 9; };
 
 namespace nova {
 
@@ -249,6 +270,7 @@ class Mesh {
     };
     std::shared_ptr<Ctx> c_;
     nova_grid_desc d_{};
+    std::string zone_ = ".";  // where the <name>BC files are read
     int n_[3] = {0, 0, 0};  // padded extents
     int64_t np_ = 0;
     Time *time_ = nullptr;
@@ -285,12 +307,17 @@ public:
         d_.m = settings::m();
         const std::string zone = fileName.substr(0, fileName.find_last_of('/') == std::string::npos
                                                         ? 0 : fileName.find_last_of('/'));
+        // a direction whose UBC entries are not periodic is walled
+        // (src/BC/BCenums.cpp:3-5 names: east/west = x, north/south = y,
+        // top/bottom = z)
         std::ifstream ubc((zone.empty() ? std::string(".") : zone) + "/UBC");
         std::string dir, type;
         while (ubc >> dir >> type) {
-            if ((dir == "north" || dir == "south") && type != "periodic") d_.bc[1] = NOVA_BC_WALL;
+            const int a = (dir == "east" || dir == "west") ? 0 : (dir == "north" || dir == "south") ? 1 : 2;
+            if (type != "periodic") d_.bc[a] = NOVA_BC_WALL;
             std::getline(ubc, line);
         }
+        zone_ = zone.empty() ? std::string(".") : zone;
         open(device);
     }
     Mesh(const std::array<int, 3> &N, const std::array<scalar, 3> &L, Time &runTime,
@@ -309,6 +336,8 @@ public:
     int ni() const { return n_[0]; }
     int nj() const { return n_[1]; }
     int nk() const { return n_[2]; }
+    bool walled(int a) const { return d_.bc[a] == NOVA_BC_WALL; }
+    const std::string &zone() const { return zone_; }
     scalar dx() const { return d_.L[0] / d_.n[0]; }
     scalar dy() const { return d_.L[1] / d_.n[1]; }
     scalar dz() const { return d_.L[2] / d_.n[2]; }
@@ -411,6 +440,41 @@ public:
     {
         d_ = m_.take(NC);
         *this = init;
+        if (!name_.empty()) setBC(name_ + "BC");
+    }
+    // Field::setBC (src/Field/Field.hxx:519-547): the "dir type val comp"
+    // lines of <zone>/<file> (a missing file leaves the field as it is).
+    // The device applies the grid's conditions when it loads a stencil:
+    // periodic directions wrap, walled ones reflect about the wall node with
+    // the wall-normal component of a vector odd and everything else even.
+    // An entry is accepted when it states that: "periodic" on a periodic
+    // direction, "dirichlet 0" for the wall-normal component, "neumann 0"
+    // for the others (neumannBC.hxx:99-102 would also overwrite the wall
+    // node with its neighbour; the reflection keeps the wall node a solved
+    // node).  Anything else cannot be represented and throws.
+    void setBC(const std::string &file)
+    {
+        std::ifstream fin(m_.zone() + "/" + file);
+        if (!fin.is_open()) return;
+        std::string line;
+        static const std::map<std::string, int> axis = {{"east", 0}, {"west", 0}, {"north", 1},
+                                                        {"south", 1}, {"top", 2}, {"bottom", 2}};
+        while (std::getline(fin, line)) {
+            std::stringstream sp(line);
+            std::string dir, type;
+            scalar val = 0.0;
+            int comp = 0;
+            if (!(sp >> dir >> type >> val >> comp)) continue;
+            const auto it = axis.find(dir);
+            if (it == axis.end()) throw nova::Error(file + ": unknown direction " + dir);
+            const int a = it->second;
+            if (comp < 0 || comp >= NC) throw nova::Error(file + ": component " + std::to_string(comp) + " out of range");
+            std::string want = "periodic";
+            if (m_.walled(a)) want = (NC == 3 && comp == a) ? "dirichlet" : "neumann";
+            if (type != want || val != 0.0)
+                throw nova::Error(file + ": '" + line + "' is not the " + (m_.walled(a) ? "walled" : "periodic") +
+                                  " condition the device applies there (" + want + " 0)");
+        }
     }
     Field(Mesh &m) : Field(m, T(0.0)) {}
     Field(const Field &f) : m_(f.m_), name_(f.name_)
@@ -507,6 +571,7 @@ public:
 
 template <> inline scalar Field<scalar>::comp(const scalar &v, int) { return v; }
 template <> inline scalar Field<vector>::comp(const vector &v, int c) { return v[c]; }
+template <> inline scalar Field<tensor>::comp(const tensor &v, int c) { return v[c]; }
 
 template <class T> using FieldPtr = std::shared_ptr<Field<T>>;
 
@@ -539,6 +604,74 @@ template <class T> FieldPtr<T> operator*(scalar s, const Field<T> &a) { return n
 template <class T> FieldPtr<T> operator*(scalar s, const FieldPtr<T> &a) { return s * *a; }
 template <class T> FieldPtr<T> operator/(const FieldPtr<T> &a, scalar s) { return nova::lincomb<T>(1.0 / s, *a, 0.0, nullptr); }
 
+// pointwise products (Field.h:227-363), one device pass each (nova_field_op)
+namespace nova {
+template <class R, class A, class B>
+FieldPtr<R> fieldop(int op, const Field<A> &a, const Field<B> *b, const double *cval = nullptr)
+{
+    auto r = tmp<R>(a.mesh());
+    check(nova_field_op(a.mesh().ctx(), op, a.dev(), pTraits<A>::nComp, b ? b->dev() : nullptr,
+                        b ? pTraits<B>::nComp : 0, cval, r->devw()),
+          "field operator");
+    return r;
+}
+}  // namespace nova
+// inner product (operator&, Field.h:340-363): vector & vector, scalar & scalar
+template <class T> FieldPtr<scalar> operator&(const Field<T> &a, const Field<T> &b)
+{
+    return nova::fieldop<scalar>(NOVA_OP_DOT, a, &b);
+}
+template <class T> FieldPtr<scalar> operator&(const Field<T> &a, const FieldPtr<T> &b) { return a & *b; }
+template <class T> FieldPtr<scalar> operator&(const FieldPtr<T> &a, const FieldPtr<T> &b) { return *a & *b; }
+template <class T> FieldPtr<scalar> operator&(const FieldPtr<T> &a, const Field<T> &b) { return *a & b; }
+// cross product (operator^, Field.cpp:117-150)
+inline FieldPtr<vector> operator^(const Field<vector> &a, const Field<vector> &b)
+{
+    return nova::fieldop<vector>(NOVA_OP_CROSS, a, &b);
+}
+// scalar field times a field (operator*, Field.h:242-293: outerProductType<scalar, T> = T)
+template <class T> FieldPtr<T> operator*(const Field<scalar> &s, const Field<T> &f)
+{
+    auto r = nova::tmp<T>(f.mesh());
+    nova::check(nova_field_op(f.mesh().ctx(), NOVA_OP_SCALE, s.dev(), 1, f.dev(), pTraits<T>::nComp, nullptr,
+                              r->devw()),
+                "operator*");
+    return r;
+}
+inline FieldPtr<vector> operator*(const Field<vector> &f, const Field<scalar> &s) { return s * f; }
+template <class T> FieldPtr<T> operator*(const FieldPtr<scalar> &s, const Field<T> &f) { return *s * f; }
+template <class T> FieldPtr<T> operator*(const Field<scalar> &s, const FieldPtr<T> &f) { return s * *f; }
+template <class T> FieldPtr<T> operator*(const FieldPtr<scalar> &s, const FieldPtr<T> &f) { return *s * *f; }
+// vector times vector: the outer product, a tensor field
+inline FieldPtr<tensor> operator*(const Field<vector> &a, const Field<vector> &b)
+{
+    return nova::fieldop<tensor>(NOVA_OP_OUTER, a, &b);
+}
+// a field over a scalar field (operator/, Field.h:266-293)
+template <class T> FieldPtr<T> operator/(const Field<T> &f, const Field<scalar> &s)
+{
+    return nova::fieldop<T>(NOVA_OP_DIVIDE, f, &s);
+}
+template <class T> FieldPtr<T> operator/(const FieldPtr<T> &f, const Field<scalar> &s) { return *f / s; }
+template <class T> FieldPtr<T> operator/(const Field<T> &f, const FieldPtr<scalar> &s) { return f / *s; }
+template <class T> FieldPtr<T> operator/(const FieldPtr<T> &f, const FieldPtr<scalar> &s) { return *f / *s; }
+// tensor double inner product (operator&&, Field.h:323), as tensor.cpp:237-251 writes it
+inline FieldPtr<scalar> operator&&(const FieldPtr<tensor> &a, const FieldPtr<tensor> &b)
+{
+    return nova::fieldop<scalar>(NOVA_OP_DDOT, *a, b.get());
+}
+// sqrt (Field.cpp:88-113)
+inline FieldPtr<scalar> sqrt(const Field<scalar> &f) { return nova::fieldop<scalar, scalar, scalar>(NOVA_OP_SQRT, f, nullptr); }
+inline FieldPtr<scalar> sqrt(const FieldPtr<scalar> &f) { return sqrt(*f); }
+// a field minus a constant (operator-, Field.h:189-201)
+template <class T> FieldPtr<T> operator-(const Field<T> &f, const T &v)
+{
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int q = 0; q < pTraits<T>::nComp; q++) c[q] = -Field<T>::comp(v, q);
+    return nova::fieldop<T, T, T>(NOVA_OP_ADD_CONST, f, nullptr, c);
+}
+template <class T> FieldPtr<T> operator-(const FieldPtr<T> &f, const T &v) { return *f - v; }
+
 // ---- explicit operators (src/ex) ----
 namespace ex {
 // ex::div(U, phi), src/ex/div/div.hxx:84-172
@@ -550,6 +683,8 @@ FieldPtr<T> div(const Field<vector> &U, const Field<T> &phi)
     return r;
 }
 template <class T> FieldPtr<T> div(const FieldPtr<vector> &U, const FieldPtr<T> &phi) { return div(*U, *phi); }
+template <class T> FieldPtr<T> div(const FieldPtr<vector> &U, const Field<T> &phi) { return div(*U, phi); }
+template <class T> FieldPtr<T> div(const Field<vector> &U, const FieldPtr<T> &phi) { return div(U, *phi); }
 // ex::div(U), src/ex/div/div.hxx:22-81
 inline FieldPtr<scalar> div(const Field<vector> &U)
 {
@@ -585,6 +720,14 @@ FieldPtr<T> laplacian(const Field<scalar> &nu, const Field<T> &f)
                 "ex::laplacian");
     return r;
 }
+// ex::grad(Field<vector>) -> Field<tensor>, src/ex/grad/grad.cpp:65-102
+inline FieldPtr<tensor> grad(const Field<vector> &U)
+{
+    auto r = nova::tmp<tensor>(U.mesh());
+    nova::check(nova_grad_vec_literal(U.mesh().ctx(), U.dev(), r->devw()), "ex::grad");
+    return r;
+}
+inline FieldPtr<tensor> grad(const FieldPtr<vector> &U) { return grad(*U); }
 // ex::curl, src/ex/curl/curl.cpp:25-57
 inline FieldPtr<vector> curl(const Field<vector> &U)
 {
@@ -699,6 +842,21 @@ std::shared_ptr<fdMatrix<T>> operator-(std::shared_ptr<fdMatrix<T>> a)
 namespace tools {
 const scalar pi = 3.14159265358979323846;
 const scalar eps = 1.0e-16;
+// tools::CFL (src/Tools/Tools.cpp:9-41): a device reduction, printed as
+// the reference prints it (its cfl_y and cfl_z take max(cfl_x, ...) at the
+// last node, as written); the Courant numbers max |U_d| dt / h_d come back
+// in the returned array's last three entries.
+inline std::array<scalar, 6> CFL(const Field<vector> &U, const Mesh &mesh)
+{
+    std::array<scalar, 6> c{};
+    nova::check(nova_cfl(mesh.ctx(), U.dev(), mesh.time().dt(), c.data()), "tools::CFL");
+    std::cout << "================================" << std::endl;
+    std::cout << "  CFL x: " << c[0] << ".\n";
+    std::cout << "  CFL y: " << c[1] << ".\n";
+    std::cout << "  CFL z: " << c[2] << ".\n";
+    std::cout << "================================" << std::endl;
+    return c;
+}
 inline scalar maximum(const FieldPtr<scalar> &f)
 {
     // max |f| over [m/2, n - m/2) = every stored node

@@ -1569,6 +1569,29 @@ double orc_dissipation(const orc_mesh *M, const double *U, double nu)
     return ORC_PLANT == 8 ? acc / n : nu * acc / n; /* 8: nu missing */
 }
 
+/* tools::CFL(U, mesh), src/Tools/Tools.cpp:9-41, as written: over
+ * [h, n-h) in every direction, cfl_x = max(cfl_x, U_x dt / dx) from 0 and
+ * cfl_y = max(cfl_x, U_y dt / dy), cfl_z = max(cfl_x, U_z dt / dz) at every
+ * node (so they keep only the last node's terms).  out[3..5]: the maxima
+ * of |U_d| dt / h_d over the same nodes. */
+void orc_cfl(const orc_mesh *M, const double *U, double dt, double out[6])
+{
+    const int ni = M->n[0], nj = M->n[1], nk = M->n[2], h = M->h;
+    double cx = 0.0, cy = 0.0, cz = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
+    for (int i = h; i < ni - h; i++)
+        for (int j = h; j < nj - h; j++)
+            for (int k = h; k < nk - h; k++) {
+                const double *u = &U[IX(M, i, j, k) * 3];
+                cx = fmax(cx, u[0] * dt / M->d[0]);
+                cy = fmax(cx, u[1] * dt / M->d[1]);
+                cz = fmax(cx, u[2] * dt / M->d[2]);
+                ax = fmax(ax, fabs(u[0]) * dt / M->d[0]);
+                ay = fmax(ay, fabs(u[1]) * dt / M->d[1]);
+                az = fmax(az, fabs(u[2]) * dt / M->d[2]);
+            }
+    out[0] = cx; out[1] = cy; out[2] = cz; out[3] = ax; out[4] = ay; out[5] = az;
+}
+
 /* tools::maximum(ex::div(U)), src/Tools/Tools.h:19-36 over [h, n-h). */
 double orc_maxabs_div(const orc_mesh *M, const double *U)
 {

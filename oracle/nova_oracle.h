@@ -143,6 +143,7 @@ void orc_project(const orc_mesh *, orc_poisson *, double *F, double *p, double s
 double orc_ke(const orc_mesh *, const double *U);
 double orc_maxabs_div(const orc_mesh *, const double *U);
 double orc_dissipation(const orc_mesh *, const double *U, double nu);
+void orc_cfl(const orc_mesh *, const double *U, double dt, double out[6]);
 void orc_mhd_diag(const orc_mesh *, const double *U, const double *B,
                   double mu, double eta, double out[8]);
 

@@ -78,6 +78,7 @@ def _load() -> C.CDLL:
         "orc_ke": ([_P, _D], C.c_double),
         "orc_maxabs_div": ([_P, _D], C.c_double),
         "orc_dissipation": ([_P, _D, C.c_double], C.c_double),
+        "orc_cfl": ([_P, _D, C.c_double, _D], None),
         "orc_mhd_diag": ([_P, _D, _D, C.c_double, C.c_double, _D], None),
         "orc_tdma": ([C.c_int, _D, _D, _D, _D, _D], None),
         "orc_num_threads": ([], C.c_int),
@@ -219,6 +220,11 @@ class OMesh:
 
     def maxabs_div(self, U):
         return lib().orc_maxabs_div(self._h, _d(U))
+
+    def cfl(self, U, dt):
+        out = np.zeros(6)
+        lib().orc_cfl(self._h, _d(U), dt, _d(out))
+        return out
 
     def dissipation(self, U, nu):
         return lib().orc_dissipation(self._h, _d(U), nu)

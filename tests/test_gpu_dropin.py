@@ -69,3 +69,84 @@ def test_tg_poisson_app_writes_ensight_output(orc, tmp_path):
         assert (outdir / f"U.0000.{q:05d}.dat").read_bytes()[:244] == (ref / f"U.0000.{q:05d}.dat").read_bytes()[:244]
         # the states agree to 1e-12; their float32 roundings to one float32 ulp
         assert np.max(np.abs(got - exp)) <= 2.0 ** -23 * np.max(np.abs(exp))
+
+
+def _ot_ic(O):
+    # app/OT_Poisson/OT.cpp:73-88
+    x = np.arange(O.stored[0]) * (2 * math.pi / O.N[0])
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    u = np.stack([-2.0 * np.sin(Y), 2.0 * np.sin(X), 0 * X])
+    b = np.stack([(-2.0 * np.sin(2.0 * Y)) + np.sin(Z), (2.0 * np.sin(X)) + np.sin(Z), np.sin(X) + np.sin(Y)])
+    return u, 0.8 * b
+
+
+def test_ot_poisson_app_matches_oracle(orc, tmp_path):
+    """app/OT_Poisson as written (UEqn.H + BEqn.H with B&B, the RK4a loops,
+    the cleaning and J = curl(B)) through the drop-in API: its Ek, Em, Jav,
+    Jmax, omega, epsilon, Umax history against the oracle's restatement of the
+    same step (mode 0) and OT.cpp's diagnostics, 1e-10 relative."""
+    exe = ROOT / "novaclone_amd" / "host" / "bin" / "ot_poisson"
+    N, steps, dt, mu, eta = 16, 4, 1e-3, 0.057, 0.057
+    out = tmp_path / "dat.dat"
+    print("running", exe, flush=True)
+    r = subprocess.run([str(exe), str(N), str(steps), repr(dt), repr(mu), repr(eta), str(out)], capture_output=True,
+                       text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "CFL x:" in r.stdout
+    got = np.loadtxt(out)
+    assert got.shape == (steps, 10)
+    O = orc.OMesh((N, N, N), (2 * math.pi,) * 3)
+    u, b = _ot_ic(O)
+    U, B = O.from_compact(u), O.from_compact(b)
+    P = orc.OPoisson(O)
+    p, pB = O.field(1), O.field(1)
+    for s in range(steps):
+        P.mhd_step(U, B, p, pB, mu, eta, dt, 0)
+        d = O.mhd_diag(U, B, mu, eta)
+        # columns: t Ek Em Jav Jmax omega epsilon -dEk/dt dEm/dt Umax
+        exp = [d[0], d[1], d[2], d[3], d[4], d[5]]
+        assert np.max(np.abs(got[s, 1:7] - exp) / np.abs(exp)) <= 1e-10, (s, got[s], d)
+        assert abs(got[s, 9] - d[6]) <= 1e-10 * d[6]
+
+
+def test_tg_wc_app_matches_oracle(orc, tmp_path):
+    """app/TG as written (rhoEqn.H with -ex::div(rho*U), pEqn.H's
+    50.0*(rho*rho), UEqn.H with rhoUOld/rho + c*(drhoU/rho)) through the
+    drop-in API: its kinetic-energy history against the oracle's
+    weakly-compressible step, 1e-10 relative."""
+    exe = ROOT / "novaclone_amd" / "host" / "bin" / "tg_wc"
+    N, steps, dt, mu = 16, 4, 1e-3, 1.0 / 1600.0
+    out = tmp_path / "dat.dat"
+    r = subprocess.run([str(exe), str(N), str(steps), repr(dt), repr(mu), str(out)], capture_output=True,
+                       text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    got = np.loadtxt(out)
+    assert got.shape == (steps, 3)
+    O = orc.OMesh((N, N, N), (2 * math.pi,) * 3)
+    pi = 3.1415926536  # TG.cpp:25
+    x = -pi + np.arange(N) * (2 * math.pi / N)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    U = O.from_compact(np.stack([np.sin(X) * np.cos(Y) * np.cos(Z), -np.cos(X) * np.sin(Y) * np.cos(Z), 0 * X]))
+    pc = 50 + 1.0 / 16.0 * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2)
+    p = O.from_compact(pc[None])
+    rho = O.from_compact(np.power(pc / 50.0, 0.5)[None])
+    for s in range(steps):
+        O.wc_step(U, rho, p, mu, 50.0, dt)
+        ek = O.ke(U)
+        assert abs(got[s, 1] - ek) <= 1e-10 * ek, (s, got[s, 1], ek)
+
+
+def test_bc_files_are_checked_against_the_device_rule(tmp_path):
+    # Field::setBC (Field.hxx:519-547) reads <zone>/<name>BC; an entry the
+    # device cannot apply is an error, not a silent change of the problem
+    exe = ROOT / "novaclone_amd" / "host" / "bin" / "ot_poisson"
+    lines = [f"{d} periodic 0 {c}" for c in range(3) for d in ("north", "south", "east", "west", "top", "bottom")]
+    (tmp_path / "UBC").write_text("\n".join(lines) + "\n")
+    (tmp_path / "BBC").write_text("\n".join(lines) + "\n")
+    r = subprocess.run([str(exe), "8", "1", "1e-3", "0.05", "0.05", "d.dat"], capture_output=True, text=True,
+                       timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    (tmp_path / "BBC").write_text("north neumann 0 0\n")
+    r = subprocess.run([str(exe), "8", "1", "1e-3", "0.05", "0.05", "d.dat"], capture_output=True, text=True,
+                       timeout=120, cwd=tmp_path)
+    assert r.returncode == 1 and "BBC" in r.stderr and "periodic" in r.stderr, r.stderr

@@ -237,3 +237,33 @@ def test_stretched_z_is_vinokur_two_sided(orc, s1, s2):
     # symmetric stretching (s1 = s2): z(N - k) = L - z(k)
     if s1 == s2:
         assert np.max(np.abs(zeta[::-1] - (1 - zeta))) < 1e-14
+
+
+def test_cfl_as_written(orc):
+    """tools::CFL: cfl_x is the maximum of U_x dt/dx from 0; cfl_y and cfl_z
+    keep max(cfl_x, U_d dt/h_d) of the last node of the loop (the padded
+    index n - m/2 - 1 in every direction: the image of node 0)."""
+    N, L = (8, 12, 16), (1.0, 2.0, 3.0)
+    M = orc.OMesh(N, L)
+    hx, hy, hz = (L[a] / N[a] for a in range(3))
+    dt = 0.01
+    rng = np.random.default_rng(3)
+    U = M.from_compact(rng.standard_normal((3, *M.stored)))
+    U[..., 1] *= 5.0
+    h = M.h
+    last = (M.n[0] - h - 1, M.n[1] - h - 1, M.n[2] - h - 1)
+    S = inner(M)
+    cx = max(0.0, np.max(U[S][..., 0]) * dt / hx)
+    out = M.cfl(U, dt)
+    assert out[0] == pytest.approx(cx, rel=1e-15)
+    assert out[1] == pytest.approx(max(cx, U[last][1] * dt / hy), rel=1e-15)
+    assert out[2] == pytest.approx(max(cx, U[last][2] * dt / hz), rel=1e-15)
+    for d, hd in enumerate((hx, hy, hz)):
+        assert out[3 + d] == pytest.approx(np.max(np.abs(U[S][..., d])) * dt / hd, rel=1e-15)
+    # the last node is the periodic image of node 0
+    assert np.array_equal(U[last], U[h, h, h])
+    # a uniform field: cfl_y = max(cfl_x, v dt/dy)
+    V = np.zeros_like(U)
+    V[..., 0], V[..., 1], V[..., 2] = -1.0, 2.0, 0.5
+    out = M.cfl(V, dt)
+    assert out[0] == 0.0 and out[1] == 2.0 * dt / hy and out[2] == 0.5 * dt / hz
