@@ -416,6 +416,21 @@ int nova_io_checkpoint_write(nova_io *io, const char *file, int nfields, const d
 int nova_io_checkpoint_read(nova_io *io, const char *file, int nfields, double *const *fields, const int *ncomp,
                             double *time, int64_t *step);
 
+/* HDF5 dumps of app/mhdjet (mhdjet.cpp:25-89, 277-400): component comp of
+ * an ncomp-component stored field to its own file, one dataset "/3DArray"
+ * of native doubles with dimensions (N_x, N_y, N_z) (the global cell
+ * counts), element [i][j][k] = node (i, j, k), k fastest (the first N_d
+ * stored nodes of a walled direction).  Synchronous.  The HDF5 C library
+ * (1.10 ABI) is opened at the first call: NOVA_HDF5_LIB, else libhdf5.so,
+ * else /opt/conda/lib/libhdf5.so; without it the call returns
+ * NOVA_ERR_UNSUPPORTED. */
+int nova_hdf5_write(nova_ctx *ctx, const char *file, const double *F, int ncomp, int comp);
+/* The same file from a host array of extents stored[] >= dims[] (k
+ * fastest), and a reader of "/3DArray" (dims out; host may be NULL to query
+ * the extents, else cap doubles at least). */
+int nova_hdf5_write_host(const char *file, const int64_t dims[3], const int64_t stored[3], const double *host);
+int nova_hdf5_read_host(const char *file, int64_t dims[3], double *host, size_t cap);
+
 /* ---- ensembles of uneven instances (no reference counterpart: NOVA runs
  * one case per process; the step of each instance is that of
  * nova_solver_step with NOVA_RK_SSP3, app/TG_Piosson/RKLoop.H for

@@ -93,6 +93,9 @@ _SIGS = {
     "nova_field_op": [_P, _I, _P, _I, _P, _I, _DP, _P],
     "nova_grad_vec_literal": [_P, _P, _P],
     "nova_cfl": [_P, _P, _D, _DP],
+    "nova_hdf5_write": [_P, C.c_char_p, _P, _I, _I],
+    "nova_hdf5_write_host": [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _DP],
+    "nova_hdf5_read_host": [C.c_char_p, C.POINTER(C.c_int64), _DP, C.c_size_t],
     "nova_poisson_create": [_P, C.POINTER(_P)],
     "nova_poisson_destroy": [_P],
     "nova_poisson_solve": [_P, _P, _P],
@@ -765,6 +768,33 @@ def cfl(mesh: Mesh, U: torch.Tensor, dt: float):
     out = (C.c_double * 6)()
     _check(lib().nova_cfl(mesh.handle, _fld(U, mesh, 3, "U"), float(dt), out), "nova_cfl")
     return list(out)
+
+
+def hdf5_write(mesh: Mesh, file, F: torch.Tensor, comp: int = 0) -> None:
+    """app/mhdjet's HDF5 dump of one component of F (dataset /3DArray)."""
+    _check(lib().nova_hdf5_write(mesh.handle, str(file).encode(), _fld(F, mesh, F.shape[0], "F"), F.shape[0],
+                                 int(comp)), "nova_hdf5_write")
+
+
+def hdf5_write_host(file, data, dims=None) -> None:
+    """Write a host float64 array (s0, s1, s2) as /3DArray of extents dims
+    (default: the array's)."""
+    import numpy as np
+    a = np.ascontiguousarray(data, dtype=np.float64)
+    st = (C.c_int64 * 3)(*a.shape)
+    dm = (C.c_int64 * 3)(*(dims if dims is not None else a.shape))
+    _check(lib().nova_hdf5_write_host(str(file).encode(), dm, st, a.ctypes.data_as(_DP)), "nova_hdf5_write_host")
+
+
+def hdf5_read_host(file):
+    """/3DArray of an HDF5 file as a float64 numpy array."""
+    import numpy as np
+    dm = (C.c_int64 * 3)()
+    _check(lib().nova_hdf5_read_host(str(file).encode(), dm, None, 0), "nova_hdf5_read_host")
+    out = np.empty(tuple(dm), dtype=np.float64)
+    _check(lib().nova_hdf5_read_host(str(file).encode(), dm, out.ctypes.data_as(_DP), out.size),
+           "nova_hdf5_read_host")
+    return out
 
 
 def ke(mesh: Mesh, U: torch.Tensor) -> float:

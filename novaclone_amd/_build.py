@@ -19,7 +19,7 @@ BUILD = PKG / "_objs"
 LIB = PKG / "libnova_hip.so"
 SOURCES = ["nova_capi.hip", "nova_ops.hip", "nova_fft.hip", "nova_step.hip", "nova_adjoint.hip", "nova_ensemble.hip",
            "nova_compact.hip", "nova_wc.hip", "nova_io.hip", "nova_implicit.hip", "nova_mhd_lit.hip",
-           "nova_algebra.hip"]
+           "nova_algebra.hip", "nova_hdf5.hip"]
 ARCH = os.environ.get("NOVA_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -51,7 +51,7 @@ def build(verbose: bool = False) -> Path:
     with cf.ThreadPoolExecutor(max_workers=min(6, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", str(tmp), *map(str, objs), "-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
@@ -62,7 +62,7 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
-APPS = ["tg_poisson", "ot_poisson", "tg_wc"]
+APPS = ["tg_poisson", "ot_poisson", "tg_wc", "mhdjet"]
 
 
 def build_apps() -> None:

@@ -552,6 +552,24 @@ int nova_cfl(nova_ctx *ctx, const double *U, double dt, double out[6])
     NOVA_GUARD(reduce_cfl(ctx, U, dt, out));
 }
 
+int nova_hdf5_write(nova_ctx *ctx, const char *file, const double *F, int ncomp, int comp)
+{
+    NOVA_REQUIRE(ctx && file && F && comp >= 0 && comp < ncomp, "bad argument");
+    NOVA_GUARD(hdf5_write_field(ctx, file, F, comp));
+}
+
+int nova_hdf5_write_host(const char *file, const int64_t dims[3], const int64_t stored[3], const double *host)
+{
+    NOVA_REQUIRE(file && dims && stored && host, "bad argument");
+    NOVA_GUARD(hdf5_write_host(file, dims, stored, host));
+}
+
+int nova_hdf5_read_host(const char *file, int64_t dims[3], double *host, size_t cap)
+{
+    NOVA_REQUIRE(file && dims, "bad argument");
+    NOVA_GUARD(hdf5_read_host(file, dims, host, cap));
+}
+
 int nova_poisson_create(nova_ctx *ctx, nova_poisson **out)
 {
     NOVA_REQUIRE(ctx && out, "null argument");

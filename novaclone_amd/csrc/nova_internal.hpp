@@ -214,6 +214,9 @@ int field_op(nova_ctx *ctx, int op, const double *a, int nca, const double *b, i
              double *out);
 int grad_vec_literal(nova_ctx *ctx, const double *U, double *out);
 int reduce_cfl(nova_ctx *ctx, const double *U, double dt, double out[6]);
+int hdf5_write_host(const char *file, const int64_t dims[3], const int64_t stored[3], const double *host);
+int hdf5_read_host(const char *file, int64_t dims[3], double *host, size_t cap);
+int hdf5_write_field(nova_ctx *ctx, const char *file, const double *F, int comp);
 int reduce_ke(nova_ctx *ctx, const double *U, double *out);
 int reduce_maxabs(nova_ctx *ctx, const double *f, double *out);
 int reduce_dissipation(nova_ctx *ctx, const double *U, double nu, double *out);

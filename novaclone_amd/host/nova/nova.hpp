@@ -838,6 +838,16 @@ std::shared_ptr<fdMatrix<T>> operator-(std::shared_ptr<fdMatrix<T>> a)
 }
 }  // namespace im
 
+// ---- the HDF5 dumps of app/mhdjet (mhdjet.cpp:25-89, 277-400) ----
+namespace nova {
+// component comp of f as dataset /3DArray of file (N_x x N_y x N_z doubles)
+template <class T>
+void writeHDF5(const std::string &file, const Field<T> &f, int comp = 0)
+{
+    check(nova_hdf5_write(f.mesh().ctx(), file.c_str(), f.dev(), pTraits<T>::nComp, comp), "HDF5 write");
+}
+}  // namespace nova
+
 // ---- tools (src/Tools/Tools.h:19-36, Tools.cpp:43-62) ----
 namespace tools {
 const scalar pi = 3.14159265358979323846;

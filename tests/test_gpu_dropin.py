@@ -150,3 +150,49 @@ def test_bc_files_are_checked_against_the_device_rule(tmp_path):
     r = subprocess.run([str(exe), "8", "1", "1e-3", "0.05", "0.05", "d.dat"], capture_output=True, text=True,
                        timeout=120, cwd=tmp_path)
     assert r.returncode == 1 and "BBC" in r.stderr and "periodic" in r.stderr, r.stderr
+
+
+def test_mhdjet_app_hdf5_dumps_match_oracle(orc, tmp_path):
+    """app/mhdjet as written (the OT_Poisson step with the jet's state) through
+    the drop-in API, with its HDF5 dumps at every write step: the files carry
+    the reference's names, and their /3DArray contents equal the oracle's
+    state (U, p, B, J = literal curl B), started from the driver's own initial
+    state (its random draws).  Bar: 1e-12 of the field's norm per dumped
+    component (the components U_y, U_z, B_y, B_z are 1e-4 .. 1e-8 of their
+    field, so their own relative differences are the field's rounding
+    amplified); for J = curl B, whose size is that of the small part of B,
+    the scale is |J| + |B| / h (the curl of B's rounding); for p, which the
+    correction applies as dt grad p, |p| + |U| h / dt (a difference of p on
+    that scale moves U by 1e-12 of |U|)."""
+    import novaclone_amd as nv
+    exe = ROOT / "novaclone_amd" / "host" / "bin" / "mhdjet"
+    N, steps, dt = (16, 32, 16), 2, 0.02
+    mu = eta = 0.000285714
+    ic = tmp_path / "ic"
+    ic.mkdir()
+    r = subprocess.run([str(exe), *map(str, N), str(steps), repr(dt), "1", str(ic)], capture_output=True, text=True,
+                       timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    O = orc.OMesh(N, (13.96, 20.0, 13.96))
+    u0 = np.stack([nv.hdf5_read_host(ic / f"ic_U{c}.h5") for c in "xyz"])
+    b0 = np.stack([nv.hdf5_read_host(ic / f"ic_B{c}.h5") for c in "xyz"])
+    assert u0.shape == (3, *N) and np.all(b0[0] == 0.015) and np.all(b0[1:] == 0.0)
+    U, B = O.from_compact(u0), O.from_compact(b0)
+    P = orc.OPoisson(O)
+    p, pB = O.field(1), O.field(1)
+    h5 = tmp_path / "h5data"
+    for s in range(steps):
+        P.mhd_step(U, B, p, pB, mu, eta, dt, 0)
+        J = O.curl_literal(B)
+        t = f"{s * dt:.6f}.h5"
+        Ui, Bi, Ji, pi = O.interior(U), O.interior(B), O.interior(J), O.interior(p)
+        hmin = min(O.L[a] / N[a] for a in range(3))
+        scale = {"U": np.linalg.norm(Ui), "B": np.linalg.norm(Bi), "p": np.linalg.norm(pi) + np.linalg.norm(Ui) * hmin / dt,
+                 "J": np.linalg.norm(Ji) + np.linalg.norm(Bi) / hmin}
+        ref = {"Ux": Ui[0], "Uy_": Ui[1], "Uz_": Ui[2], "p_": pi[0], "Bx_": Bi[0], "By_": Bi[1], "Bz_": Bi[2],
+               "Jx_": Ji[0], "Jy_": Ji[1], "Jz_": Ji[2]}
+        for name, exp in ref.items():
+            got = nv.hdf5_read_host(h5 / f"{name}{t}")
+            e = np.linalg.norm(got - exp) / scale[name[0]]
+            print(name, t, e, flush=True)
+            assert got.shape == N and e <= 1e-12, (name, t, e)
