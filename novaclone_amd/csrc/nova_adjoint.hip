@@ -13,9 +13,12 @@
 //   and P^T = P (the solve inverts D.G, which is symmetric).
 // Reverse step, given m = dJ/dU':  m3 = P m;  m2 = 2/3 T(U2, m3);
 //   m1 = 1/4 T(U1, m2);  dJ/dU = T(U, m1) + 3/4 m2 + 1/3 m3,
-//   with T(X, m) = m + dt J_R(X)^T m.
+//   with T(X, m) = m + dt J_R(X)^T m: three adjoint stage launches, the last
+//   reading m2 and m3 at its own nodes and writing over m3.
 // Checkpoints of U every C steps are kept; each segment is recomputed once
 // in the reverse sweep, storing the stage inputs (U, U1, U2) of its steps.
+// The forward sweep stores the stage inputs of its last segment as it goes,
+// so the reverse sweep starts without a recompute.
 #include "nova_internal.hpp"
 #include "nova_stencil.hpp"
 #include "nova_stream.hpp"
@@ -50,6 +53,7 @@ struct nova_adjoint {
     double dt = 0.0;
     std::vector<double *> ckpt;   // U at steps 0, C, 2C, ...
     std::vector<double *> seg;    // per step of a segment: U, U1, U2
+    int seg_held = -1;            // the segment whose stage inputs seg holds (-1: none)
     double *w[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // work vector fields
     double *ptmp = nullptr;
     int scheme = NOVA_SCHEME_EXPLICIT;
@@ -100,28 +104,37 @@ int tlm_stage(nova_adjoint *ad, const double *X, const double *V, const double *
     return launch_stream_model<MODEL_TLM>(ad->ctx, a, "tlm_stage");
 }
 
-// adjoint stage out = ob (M + dt J_R(X)^T M) + yg Y, acc = acc_a acc + acc_b M
-int adj_stage(nova_adjoint *ad, const double *X, const double *M, double *out, double ob, const double *Y, double yg,
-              double *acc, double acc_a, double acc_b)
+LinArgs adj_args(nova_adjoint *ad, const double *X, const double *M, double *out)
 {
-    if (ad->scheme == NOVA_SCHEME_COMPACT6)
-        return compact_adj_stage(ad->ctx, X, M, out, ob, Y, yg, acc, acc_a, acc_b, ad->dt, ad->nu, ad->rwork);
     LinArgs a{};
     a.X[0] = X;
     a.X[1] = M;
     a.out[0] = out;
+    a.nu = ad->nu;
+    a.sa.form = FORM_SSP;
+    a.sa.dt = ad->dt;
+    a.ak = adj_k(ad->ctx->g, ad->nu, ad->dt);
+    return a;
+}
+
+// explicit adjoint stages: out = ob T(X, M), and out = T(X, M) + yg Y + zg Z
+// (Z may be out)
+int adj_stage_scaled(nova_adjoint *ad, const double *X, const double *M, double *out, double ob)
+{
+    LinArgs a = adj_args(ad, X, M, out);
     a.ob = ob;
+    return launch_stream_model<MODEL_ADJ, 1>(ad->ctx, a, "adj_stage");
+}
+
+int adj_stage_sum(nova_adjoint *ad, const double *X, const double *M, double *out, const double *Y, double yg,
+                  const double *Z, double zg)
+{
+    LinArgs a = adj_args(ad, X, M, out);
     a.Y = Y;
     a.yg = yg;
-    a.acc = acc;
-    a.acc_a = acc_a;
-    a.acc_b = acc_b;
-    a.nu = ad->nu;
-    StageArgs sa{};
-    sa.form = FORM_SSP;
-    sa.dt = ad->dt;
-    a.sa = sa;
-    return launch_stream_model<MODEL_ADJ>(ad->ctx, a, "adj_stage");
+    a.Z = Z;
+    a.zg = zg;
+    return launch_stream_model<MODEL_ADJ, 2>(ad->ctx, a, "adj_stage");
 }
 
 // Forward step from U storing the stage inputs U1, U2; the next state is
@@ -145,6 +158,7 @@ int recompute_segment(nova_adjoint *ad, int s, int *count)
     int n1 = n0 + C;
     if (n1 > ad->nsteps) n1 = ad->nsteps;
     int rc;
+    ad->seg_held = -1;
     if ((rc = copy3(ad->ctx, ad->seg[0], ad->ckpt[s]))) return rc;
     for (int n = n0; n < n1 - 1; n++) {  // the segment's last step is not needed forward
         const int q = n - n0;
@@ -159,6 +173,7 @@ int recompute_segment(nova_adjoint *ad, int s, int *count)
             return rc;
     }
     *count = n1 - n0;
+    ad->seg_held = s;
     return NOVA_OK;
 }
 
@@ -242,12 +257,23 @@ int adjoint_forward(nova_adjoint *ad, double dt, int nsteps, const double *U0, d
     if (nsteps < 1 || nsteps > ad->max_steps) return fail(NOVA_ERR_ARG, "nsteps outside the checkpoint budget");
     ad->dt = dt;
     ad->nsteps = nsteps;
+    ad->seg_held = -1;
     int rc;
     if (U != U0 && (rc = copy3(ad->ctx, U, U0))) return rc;
-    for (int n = 0; n < nsteps; n++) {
-        if (n % ad->every == 0 && (rc = copy3(ad->ctx, ad->ckpt[n / ad->every], U))) return rc;
+    const int C = ad->every, last = (nsteps - 1) / C;
+    for (int n = 0; n < last * C; n++) {
+        if (n % C == 0 && (rc = copy3(ad->ctx, ad->ckpt[n / C], U))) return rc;
         if ((rc = fwd_step_store(ad, U, ad->w[0], ad->w[1], U))) return rc;
     }
+    // the last segment runs in the segment store, keeping its stage inputs
+    if ((rc = copy3(ad->ctx, ad->ckpt[last], U))) return rc;
+    if ((rc = copy3(ad->ctx, ad->seg[0], U))) return rc;
+    for (int n = last * C; n < nsteps; n++) {
+        const int q = n - last * C;
+        double *nxt = n + 1 < nsteps ? ad->seg[3 * q + 3] : U;
+        if ((rc = fwd_step_store(ad, ad->seg[3 * q], ad->seg[3 * q + 1], ad->seg[3 * q + 2], nxt))) return rc;
+    }
+    ad->seg_held = last;
     return NOVA_OK;
 }
 
@@ -259,15 +285,32 @@ int adjoint_backward(nova_adjoint *ad, const double *lamT, double *lam0)
     int rc;
     if (L != lamT && (rc = copy3(ctx, L, lamT))) return rc;
     const int nseg = (ad->nsteps + ad->every - 1) / ad->every;
+    const bool compact = ad->scheme == NOVA_SCHEME_COMPACT6;
     for (int s = nseg - 1; s >= 0; s--) {
         int cnt = 0;
-        if ((rc = recompute_segment(ad, s, &cnt))) return rc;
+        if (ad->seg_held == s) {
+            cnt = (s + 1) * ad->every < ad->nsteps ? ad->every : ad->nsteps - s * ad->every;
+        } else if ((rc = recompute_segment(ad, s, &cnt))) {
+            return rc;
+        }
         for (int q = cnt - 1; q >= 0; q--) {
             double *U = ad->seg[3 * q], *U1 = ad->seg[3 * q + 1], *U2 = ad->seg[3 * q + 2];
             if ((rc = project(ad->ps, L, 1.0, ad->ptmp))) return rc;  // m3 = P m
-            if ((rc = adj_stage(ad, U2, L, A, 2.0 / 3.0, nullptr, 0.0, ACC, 0.0, 1.0 / 3.0))) return rc;
-            if ((rc = adj_stage(ad, U1, A, B, 0.25, nullptr, 0.0, ACC, 1.0, 0.75))) return rc;
-            if ((rc = adj_stage(ad, U, B, L, 1.0, ACC, 1.0, nullptr, 0.0, 0.0))) return rc;
+            if (compact) {
+                // the pencil passes carry one pointwise addend and an accumulator
+                const double dt = ad->dt, nu = ad->nu;
+                if ((rc = compact_adj_stage(ctx, U2, L, A, 2.0 / 3.0, nullptr, 0.0, ACC, 0.0, 1.0 / 3.0, dt, nu,
+                                            ad->rwork)))
+                    return rc;
+                if ((rc = compact_adj_stage(ctx, U1, A, B, 0.25, nullptr, 0.0, ACC, 1.0, 0.75, dt, nu, ad->rwork)))
+                    return rc;
+                if ((rc = compact_adj_stage(ctx, U, B, L, 1.0, ACC, 1.0, nullptr, 0.0, 0.0, dt, nu, ad->rwork)))
+                    return rc;
+            } else {
+                if ((rc = adj_stage_scaled(ad, U2, L, A, 2.0 / 3.0))) return rc;            // m2
+                if ((rc = adj_stage_scaled(ad, U1, A, B, 0.25))) return rc;                 // m1
+                if ((rc = adj_stage_sum(ad, U, B, L, A, 0.75, L, 1.0 / 3.0))) return rc;    // dJ/dU
+            }
         }
     }
     return NOVA_OK;
@@ -283,22 +326,15 @@ int adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t len)
     std::string rep;
     if (ad->scheme == NOVA_SCHEME_COMPACT6)  // pencil passes have no launch-shape choice
         return write_report("compact scheme: no stage launch shapes to tune\n", report, len);
-    const int variants[] = {0, 2, 3, 4, 11, 12};
+    const int variants[] = {0, 2, 3, 4, 5, 6, 11, 12};
     // the forward state and a distinct second field (a copy), so that the
     // trials move the same bytes as the reverse sweep
     const double *X = ad->ckpt[0];
     double *V = ad->w[2];
     if (int rc = copy3(ctx, V, X)) return rc;
     if (int rc = copy3(ctx, ad->w[3], X)) return rc;
-    auto adj = [&]() -> int {
-        LinArgs a = lin_args(ad, X, V);
-        a.out[0] = ad->w[0];
-        a.ob = 1.0;
-        a.acc = ad->w[1];
-        a.acc_b = 1.0;
-        return launch_stream_model<MODEL_ADJ>(ctx, a, "adj_stage");
-    };
-    if (int rc = tune_stage_model(ctx, MODEL_ADJ, variants, 6, reps, adj, "adj_stage", rep)) return rc;
+    auto adj = [&]() -> int { return adj_stage_scaled(ad, X, V, ad->w[0], 1.0); };
+    if (int rc = tune_stage_model(ctx, MODEL_ADJ, variants, 8, reps, adj, "adj_stage", rep)) return rc;
     auto tlm = [&]() -> int {
         LinArgs a = lin_args(ad, X, V);
         a.out[0] = ad->w[0];
@@ -306,7 +342,7 @@ int adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t len)
         a.sa = ssp_args(1, ad->dt);
         return launch_stream_model<MODEL_TLM>(ctx, a, "tlm_stage");
     };
-    if (int rc = tune_stage_model(ctx, MODEL_TLM, variants, 6, reps, tlm, "tlm_stage", rep)) return rc;
+    if (int rc = tune_stage_model(ctx, MODEL_TLM, variants, 8, reps, tlm, "tlm_stage", rep)) return rc;
     return write_report(rep, report, len);
 }
 

@@ -40,17 +40,21 @@ struct StreamArgs {
     double nu, eta;
     NsK nk;   // NS stages: ns_k(g, nu, sa.dt)
     MhdK mk;  // MHD stages: mhd_k(g, nu, eta, sa.dt)
-    // adjoint stage only: out = ob * T + yg * Y, acc = acc_a * acc + acc_b * m
-    const double *Y;
+    AdjK ak;  // adjoint stages: adj_k(g, nu, sa.dt)
+    // adjoint stage: out = ob T (combination 1) or T + yg Y + zg Z
+    // (combination 2), T = m + dt J_R(X)^T m; MHD B stages: the accumulator
+    const double *Y, *Z;
     double *acc;
-    double ob, yg, acc_a, acc_b;
+    double ob, yg, zg, acc_a, acc_b;
     int ichunk;
     int nkt, njt, nic;
 };
 
 // MODEL_NS: X = {U}; MODEL_MHD: X = {U, B};  out = stage combination.
 // MODEL_TLM: X = {U, d}, X0[1] = step-start tangent; out[0] = tangent stage.
-// MODEL_ADJ: X = {U, m}; out[0] = ob (m + dt J_R(U)^T m) + yg Y.
+// MODEL_ADJ: X = {U, m}; T = m + dt J_R(U)^T m; out[0] = ob T (CMB 1) or
+//   T + yg Y + zg Z (CMB 2, Y and Z read at the node itself; Z may be
+//   out[0]: each node is read and then written by the same lane).
 // MODEL_MHDU: X = {U, B}; out[0] = U + sum_j w_j dU, dU = dt R_U(U, B)
 //   (sa.form FORM_LIT4).
 // MODEL_MHDB: X = {U, B}, X0[1] = B at the step start, acc = the running
@@ -209,11 +213,13 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
         double nq[NF][3];
         double hv[NSLOT][NC];
         double nx0[NF][3];
-        double ny[3], nacc[3];  // adjoint stage: pointwise Y and accumulator
+        double ny[3], nacc[3];  // adjoint combination 2: Y and Z; MHD B stages: the accumulator
         int fq, fh;             // reflection flags of the queue plane and the halo plane
     };
     constexpr bool ADJ = MODEL == MODEL_ADJ;
-    constexpr bool ACC = ADJ || MODEL == MODEL_MHDB;  // reads the pointwise accumulator
+    static_assert(!ADJ || CMB == 1 || CMB == 2, "adjoint stages take a compiled-in combination");
+    // reads pointwise addends (adjoint combination 2) or the accumulator (MHD B stages)
+    constexpr bool ACC = (ADJ && CMB == 2) || MODEL == MODEL_MHDB;
     // Every lane issues the same loads for every plane: slots past the halo
     // read column (0, 0), lanes outside the grid read their mapped column,
     // and the prefetch past the end of a chunk re-reads its last plane.  With
@@ -231,8 +237,12 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
             if constexpr (ACC) {
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    P.ny[c] = (ADJ && a.Y) ? ldp(a.Y, c, (int64_t)p * plane, col) : 0.0;
-                    P.nacc[c] = (a.acc && a.acc_a != 0.0) ? ldp(a.acc, c, (int64_t)p * plane, col) : 0.0;
+                    if constexpr (ADJ) {
+                        P.ny[c] = ldp(a.Y, c, (int64_t)p * plane, col);
+                        P.nacc[c] = ldp(a.Z, c, (int64_t)p * plane, col);
+                    } else {
+                        P.nacc[c] = a.acc_a != 0.0 ? ldp(a.acc, c, (int64_t)p * plane, col) : 0.0;
+                    }
                 }
             }
             const int64_t po = plane_off(p + H, P.fq);
@@ -253,7 +263,9 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
     const double zi = g.zinv[km], zl = g.zlap[km], zm = g.zmet[km];
     NsZ<H> nz{};
     MhdZ mz{};
+    AdjZ<H> az{};
     if constexpr (MODEL == MODEL_NS) nz = ns_z<H>(a.nk, zi, zl, zm);
+    if constexpr (ADJ) az = adj_z<H>(a.ak, zi, zl);
     if constexpr (MODEL == MODEL_MHDU || MODEL == MODEL_MHDB) mz = mhd_z(a.mk, zi, zl, zm);
     // step-start values of the own column, one plane ahead like the queue
     double x0c[NF][3];
@@ -268,8 +280,12 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
     if constexpr (ACC) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            yc[c] = (ADJ && a.Y) ? ldp(a.Y, c, (int64_t)i0 * plane, col) : 0.0;
-            accc[c] = (a.acc && a.acc_a != 0.0) ? ldp(a.acc, c, (int64_t)i0 * plane, col) : 0.0;
+            if constexpr (ADJ) {
+                yc[c] = ldp(a.Y, c, (int64_t)i0 * plane, col);
+                accc[c] = ldp(a.Z, c, (int64_t)i0 * plane, col);
+            } else {
+                accc[c] = a.acc_a != 0.0 ? ldp(a.acc, c, (int64_t)i0 * plane, col) : 0.0;
+            }
         }
     }
 
@@ -345,14 +361,12 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                 }
             } else if constexpr (MODEL == MODEL_ADJ) {
                 double r[3];
-                adj_rhs<H>(g, A, a.nu, zi, zl, r);
+                adj_du<H>(a.ak, az, A, r);
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    const double m = q[1][c][H];
-                    double o = a.ob * (m + sa.dt * r[c]);
-                    if (a.Y) o += a.yg * yc[c];
-                    if (a.acc) stp(a.acc, c, pi, col, (a.acc_a != 0.0 ? a.acc_a * accc[c] : 0.0) + a.acc_b * m);
-                    stp(a.out[0], c, pi, col, o);
+                    const double t = q[1][c][H] + r[c];
+                    if constexpr (CMB == 1) stp(a.out[0], c, pi, col, a.ob * t);
+                    else stp(a.out[0], c, pi, col, (t + a.yg * yc[c]) + a.zg * accc[c]);
                 }
             } else if constexpr (MODEL == MODEL_MHDU) {
                 double du[3], db[3];
@@ -462,9 +476,10 @@ k_stage_stream(Grid g, StreamArgs<model_nf<MODEL>()> a)
     const int T = a.nkt * a.njt * a.nic;
     int t = blockIdx.x;
     if (T % 8 == 0) t = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
-    // shapes held to one or two waves per SIMD have the registers for a
-    // second prefetch set
-    stage_tile<H, MODEL, WALLS, kTK, MINW <= 2 ? 2 : 1, kTJ, CMB, STG>(g, a, t, sh);
+    // shapes held to one wave per SIMD, and one-field shapes held to two,
+    // have the registers for a second prefetch set
+    constexpr int PF = (MINW == 1 || (MINW == 2 && model_nf<MODEL>() == 1)) ? 2 : 1;
+    stage_tile<H, MODEL, WALLS, kTK, PF, kTJ, CMB, STG>(g, a, t, sh);
 }
 
 // Launch-shape slot of a stage: the model, or slot 4 for the first SSP
@@ -509,9 +524,17 @@ void launch_stream_w(nova_ctx *ctx, const StreamArgs<model_nf<MODEL>()> &a)
     case 0: launch_stream_tile<H, MODEL, WALLS, CMB, 1, 64>(ctx, a); break;
     case 2: launch_stream_tile<H, MODEL, WALLS, CMB, W, 64>(ctx, a); break;
     case 4: launch_stream_tile<H, MODEL, WALLS, CMB, W, 16>(ctx, a); break;
-    // two waves per SIMD with two planes of loads in flight (one-field models)
-    case 5: launch_stream_tile<H, MODEL, WALLS, CMB, W == 3 ? 2 : 1, 32>(ctx, a); break;
-    case 6: launch_stream_tile<H, MODEL, WALLS, CMB, W == 3 ? 2 : 1, 16>(ctx, a); break;
+    // two waves per SIMD: one-field models with two planes of loads in
+    // flight; two-field models with one prefetch set on 32 x 8 tiles (one
+    // staged plane, or two alternating), which spills a few registers
+    case 5:
+        if constexpr (W == 3) launch_stream_tile<H, MODEL, WALLS, CMB, 2, 32>(ctx, a);
+        else launch_stream_tile<H, MODEL, WALLS, CMB, 2, 32, 8>(ctx, a);
+        break;
+    case 6:
+        if constexpr (W == 3) launch_stream_tile<H, MODEL, WALLS, CMB, 2, 16>(ctx, a);
+        else launch_stream_tile<H, MODEL, WALLS, CMB, 2, 32, 8, STG_TWO>(ctx, a);
+        break;
     // 512-thread tiles (32 x 16, 64 x 8): a smaller halo ring per node.  A
     // 512-thread block is held to 256 registers per lane, which two queued
     // fields exceed, so two-field models keep their 256-thread shapes here
@@ -554,21 +577,21 @@ void launch_stream_w(nova_ctx *ctx, const StreamArgs<model_nf<MODEL>()> &a)
 
 // Streamed launch for any model and stencil half-width (the row-stream
 // variant of the forward stages is dispatched by their own launcher).
-template <int MODEL>
+template <int MODEL, int CMB = 0>
 int launch_stream_model(nova_ctx *ctx, const StreamArgs<model_nf<MODEL>()> &a, const char *name)
 {
     const Grid &g = ctx->g;
     const bool walls = g.wall[0] || g.wall[1] || g.wall[2];
     TimedScope ts(ctx, name);
     switch (g.h * 2 + (walls ? 1 : 0)) {
-    case 2: launch_stream_w<1, MODEL, false>(ctx, a); break;
-    case 3: launch_stream_w<1, MODEL, true>(ctx, a); break;
-    case 4: launch_stream_w<2, MODEL, false>(ctx, a); break;
-    case 5: launch_stream_w<2, MODEL, true>(ctx, a); break;
-    case 6: launch_stream_w<3, MODEL, false>(ctx, a); break;
-    case 7: launch_stream_w<3, MODEL, true>(ctx, a); break;
-    case 8: launch_stream_w<4, MODEL, false>(ctx, a); break;
-    case 9: launch_stream_w<4, MODEL, true>(ctx, a); break;
+    case 2: launch_stream_w<1, MODEL, false, CMB>(ctx, a); break;
+    case 3: launch_stream_w<1, MODEL, true, CMB>(ctx, a); break;
+    case 4: launch_stream_w<2, MODEL, false, CMB>(ctx, a); break;
+    case 5: launch_stream_w<2, MODEL, true, CMB>(ctx, a); break;
+    case 6: launch_stream_w<3, MODEL, false, CMB>(ctx, a); break;
+    case 7: launch_stream_w<3, MODEL, true, CMB>(ctx, a); break;
+    case 8: launch_stream_w<4, MODEL, false, CMB>(ctx, a); break;
+    case 9: launch_stream_w<4, MODEL, true, CMB>(ctx, a); break;
     default: return fail(NOVA_ERR_ARG, "stencil order");
     }
     return check_launch(name);

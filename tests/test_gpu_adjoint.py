@@ -122,6 +122,79 @@ def test_autograd_harness(nv):
     assert torch.equal(X.grad, g0)
 
 
+def shear_layer(nv, N, L, seed=5, kmax=4):
+    """u = tanh(2y), y in [-L_y/2, L_y/2], plus a seeded random-phase
+    perturbation of amplitude 1e-3 on the modes kx, kz <= kmax localised in
+    y, projected: the adjoint benchmark's initial state on a smaller grid."""
+    import torch
+    G = nv.Mesh(N, L, bc=("periodic", "wall", "periodic"))
+    P = nv.Poisson(G)
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    sx, sy, sz = G.shape
+    x = torch.arange(sx, dtype=torch.float64, device="cuda") * (L[0] / N[0])
+    y = torch.arange(sy, dtype=torch.float64, device="cuda") * (L[1] / N[1]) - L[1] / 2
+    z = torch.arange(sz, dtype=torch.float64, device="cuda") * (L[2] / N[2])
+    U = torch.zeros((3, sx, sy, sz), dtype=torch.float64, device="cuda")
+    U[0] += torch.tanh(2 * y)[None, :, None]
+    env = torch.exp(-y * y)[None, :, None]
+    for c in range(3):
+        for kx in range(kmax + 1):
+            for kz in range(kmax + 1):
+                if kx or kz:
+                    ph = float(torch.rand(1, generator=gen)) * TWO_PI
+                    U[c] += 1e-3 * torch.cos(TWO_PI * kx * x[:, None, None] / L[0]
+                                             + TWO_PI * kz * z[None, None, :] / L[2] + ph) * env
+    U[1, :, 0, :] = 0.0
+    U[1, :, -1, :] = 0.0
+    U = U.contiguous()
+    P.project(U, 1.0)
+    return G, P, U
+
+
+def test_shear_layer_gradient_midsize(nv):
+    """The adjoint benchmark's setting on a 64 x 96 x 64 grid (the same
+    spacings, free-slip y walls, Re = 1000, dt = 0.02): 50 SSP-RK3 steps with
+    checkpoints every 5, so the reverse sweep recomputes nine segments and
+    takes the last one from the forward sweep.  Dot product of the
+    tangent-linear model and the adjoint; the enstrophy gradient against
+    16-direction central differences (as the small-grid test); a second
+    reverse sweep (which recomputes the last segment too) equals the first
+    bit for bit."""
+    import torch
+    N, L = (64, 96, 64), (TWO_PI, 3.0, TWO_PI)
+    G, P, U0 = shear_layer(nv, N, L)
+    A = nv.Adjoint(nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-3), 5, 50)
+    dt, n = 0.02, 50
+    U = A.forward(dt, n, U0)
+    d = rand_like(U0, 21, True)
+    m = rand_like(U0, 22, True)
+    Ld = A.tlm(d)
+    LTm = A.backward(m)
+    lhs = nv.weighted_dot(G, Ld, m)
+    rhs = nv.weighted_dot(G, d, LTm)
+    print("dot 64x96x64 50 steps", lhs, rhs, abs(lhs - rhs) / abs(lhs), flush=True)
+    assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), abs(rhs))
+    U = A.forward(dt, n, U0)
+    J, gT = nv.enstrophy(G, U, grad=True)
+    g0 = A.backward(gT)
+    g1 = A.backward(gT)
+    assert torch.equal(g0, g1)
+    Jf = lambda X: nv.enstrophy(G, A.forward(dt, n, X.contiguous()))
+    u = 2.0 ** -53
+    for seed in range(16):
+        dd = rand_like(U0, 300 + seed, True)
+        dd = dd / dd.abs().max()
+        ad = nv.weighted_dot(G, g0, dd)
+        e4 = 1e-4
+        fd4 = (-Jf(U0 + 2 * e4 * dd) + 8 * Jf(U0 + e4 * dd) - 8 * Jf(U0 - e4 * dd) + Jf(U0 - 2 * e4 * dd)) / (12 * e4)
+        e2 = 1e-6
+        fd2 = (Jf(U0 + e2 * dd) - Jf(U0 - e2 * dd)) / (2 * e2)
+        print(f"fd 64x96x64 seed={seed} ad={ad:.12e} fd4 rel={abs(fd4 - ad) / abs(ad):.2e} "
+              f"fd2 rel={abs(fd2 - ad) / abs(ad):.2e} J={J:.3e}", flush=True)
+        assert abs(fd4 - ad) <= 1e-8 * abs(ad)
+        assert abs(fd2 - ad) <= 1e-8 * abs(ad) + 4 * u * J / e2
+
+
 def test_adjoint_cost_ratio(nv):
     """Adjoint sweep (segment recompute + reverse stages) against the plain
     forward solve of the same steps, at 64^3 (timing only, reported)."""
