@@ -276,6 +276,14 @@ int nova_adjoint_tlm(nova_adjoint *ad, const double *dU0, double *dU_final);
 /* Autotuner for the adjoint and tangent-linear stage kernels, run on the
  * first checkpoint of the last forward run (see nova_solver_autotune). */
 int nova_adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t report_len);
+/* Further segment stores (3*checkpoint_every vector fields each) within
+ * max_bytes of device memory, at most one per checkpoint but the first; as
+ * many as the device can allocate are kept and *segments (may be null)
+ * receives their count.  The forward sweep then keeps the stage inputs of
+ * its last 1 + *segments segments, which the reverse sweep and the
+ * tangent-linear model use without recomputing them.  max_bytes = 0
+ * releases them.  Takes effect from the next nova_adjoint_forward. */
+int nova_adjoint_reserve(nova_adjoint *ad, uint64_t max_bytes, int *segments);
 /* out = sum_nodes w a.b over ncomp components. */
 int nova_weighted_dot(nova_ctx *ctx, const double *a, const double *b, int ncomp, double *out);
 /* J = dV sum_nodes w |curl U|^2 (central stencil curl); dJdU (may be NULL)

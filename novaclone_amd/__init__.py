@@ -161,6 +161,7 @@ _SIGS = {
     "nova_adjoint_backward": [_P, _P, _P],
     "nova_adjoint_tlm": [_P, _P, _P],
     "nova_adjoint_autotune": [_P, _I, C.c_char_p, C.c_size_t],
+    "nova_adjoint_reserve": [_P, C.c_uint64, _P],
     "nova_weighted_dot": [_P, _P, _P, _I, _DP],
     "nova_enstrophy": [_P, _P, _DP, _P],
 }
@@ -884,6 +885,14 @@ class Adjoint:
         buf = C.create_string_buffer(1 << 16)
         _check(lib().nova_adjoint_autotune(self._h, int(reps), buf, len(buf)), "nova_adjoint_autotune")
         return buf.value.decode()
+
+    def reserve(self, max_bytes: int) -> int:
+        """Further segment stores within max_bytes of device memory, so that
+        the forward sweep keeps the stage inputs of its last segments and the
+        reverse sweep recomputes fewer; returns how many were allocated."""
+        n = C.c_int(0)
+        _check(lib().nova_adjoint_reserve(self._h, C.c_uint64(int(max_bytes)), C.byref(n)), "nova_adjoint_reserve")
+        return n.value
 
     def close(self):
         if getattr(self, "_h", None):

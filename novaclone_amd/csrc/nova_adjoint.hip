@@ -16,9 +16,10 @@
 //   with T(X, m) = m + dt J_R(X)^T m: three adjoint stage launches, the last
 //   reading m2 and m3 at its own nodes and writing over m3.
 // Checkpoints of U every C steps are kept; each segment is recomputed once
-// in the reverse sweep, storing the stage inputs (U, U1, U2) of its steps.
-// The forward sweep stores the stage inputs of its last segment as it goes,
-// so the reverse sweep starts without a recompute.
+// in the reverse sweep, storing the stage inputs (U, U1, U2) of its steps in
+// a segment store.  The forward sweep writes the stage inputs of its last
+// segments into the stores it has (one, plus those nova_adjoint_reserve
+// added) as it goes, and the reverse sweep recomputes only the others.
 #include "nova_internal.hpp"
 #include "nova_stencil.hpp"
 #include "nova_stream.hpp"
@@ -52,8 +53,10 @@ struct nova_adjoint {
     int nsteps = 0;
     double dt = 0.0;
     std::vector<double *> ckpt;   // U at steps 0, C, 2C, ...
-    std::vector<double *> seg;    // per step of a segment: U, U1, U2
-    int seg_held = -1;            // the segment whose stage inputs seg holds (-1: none)
+    // segment stores: per step of a segment U, U1, U2; store 0 receives the
+    // recomputed segments, the others only segments of the forward sweep
+    std::vector<std::vector<double *>> store;
+    std::vector<int> held;        // the segment whose stage inputs each store holds (-1: none)
     double *w[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // work vector fields
     double *ptmp = nullptr;
     int scheme = NOVA_SCHEME_EXPLICIT;
@@ -158,22 +161,52 @@ int recompute_segment(nova_adjoint *ad, int s, int *count)
     int n1 = n0 + C;
     if (n1 > ad->nsteps) n1 = ad->nsteps;
     int rc;
-    ad->seg_held = -1;
-    if ((rc = copy3(ad->ctx, ad->seg[0], ad->ckpt[s]))) return rc;
+    std::vector<double *> &seg = ad->store[0];
+    ad->held[0] = -1;
+    if ((rc = copy3(ad->ctx, seg[0], ad->ckpt[s]))) return rc;
     for (int n = n0; n < n1 - 1; n++) {  // the segment's last step is not needed forward
         const int q = n - n0;
-        if ((rc = fwd_step_store(ad, ad->seg[3 * q], ad->seg[3 * q + 1], ad->seg[3 * q + 2], ad->seg[3 * q + 3])))
-            return rc;
+        if ((rc = fwd_step_store(ad, seg[3 * q], seg[3 * q + 1], seg[3 * q + 2], seg[3 * q + 3]))) return rc;
     }
     {
         // stage inputs of the last step only
         const int q = n1 - 1 - n0;
-        if ((rc = fwd_stage(ad, ad->seg[3 * q], nullptr, ad->seg[3 * q + 1], ssp_args(0, ad->dt)))) return rc;
-        if ((rc = fwd_stage(ad, ad->seg[3 * q + 1], ad->seg[3 * q], ad->seg[3 * q + 2], ssp_args(1, ad->dt))))
-            return rc;
+        if ((rc = fwd_stage(ad, seg[3 * q], nullptr, seg[3 * q + 1], ssp_args(0, ad->dt)))) return rc;
+        if ((rc = fwd_stage(ad, seg[3 * q + 1], seg[3 * q], seg[3 * q + 2], ssp_args(1, ad->dt)))) return rc;
     }
     *count = n1 - n0;
-    ad->seg_held = s;
+    ad->held[0] = s;
+    return NOVA_OK;
+}
+
+// The stage inputs of segment s: a store that holds them, else store 0
+// after a recompute.
+int segment_inputs(nova_adjoint *ad, int s, double *const **seg, int *count)
+{
+    for (size_t k = 0; k < ad->store.size(); k++)
+        if (ad->held[k] == s) {
+            *seg = ad->store[k].data();
+            *count = (s + 1) * ad->every < ad->nsteps ? ad->every : ad->nsteps - s * ad->every;
+            return NOVA_OK;
+        }
+    if (int rc = recompute_segment(ad, s, count)) return rc;
+    *seg = ad->store[0].data();
+    return NOVA_OK;
+}
+
+int alloc_store(size_t bytes, int nfields, std::vector<double *> &st)
+{
+    for (int q = 0; q < nfields; q++) {
+        double *p = nullptr;
+        const hipError_t e = hipMalloc((void **)&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();  // an allocation failure is not sticky
+            for (double *x : st) (void)hipFree(x);
+            st.clear();
+            return NOVA_ERR_HIP;
+        }
+        st.push_back(p);
+    }
     return NOVA_OK;
 }
 
@@ -217,10 +250,12 @@ int adjoint_create(nova_solver *sv, int every, int max_steps, nova_adjoint **out
         e = hipMalloc((void **)&p, bytes);
         ad->ckpt.push_back(p);
     }
+    ad->store.resize(1);
+    ad->held.assign(1, -1);
     for (int q = 0; q < 3 * every && e == hipSuccess; q++) {
         double *p = nullptr;
         e = hipMalloc((void **)&p, bytes);
-        ad->seg.push_back(p);
+        ad->store[0].push_back(p);
     }
     for (int q = 0; q < 5 && e == hipSuccess; q++) e = hipMalloc((void **)&ad->w[q], bytes);
     if (e == hipSuccess) e = hipMalloc((void **)&ad->ptmp, sizeof(double) * ctx->g.np);
@@ -228,7 +263,7 @@ int adjoint_create(nova_solver *sv, int every, int max_steps, nova_adjoint **out
         e = hipMalloc((void **)&ad->rwork, sizeof(double) * 4 * ctx->g.np);
     if (e != hipSuccess) {
         for (double *p : ad->ckpt) if (p) (void)hipFree(p);
-        for (double *p : ad->seg) if (p) (void)hipFree(p);
+        for (double *p : ad->store[0]) if (p) (void)hipFree(p);
         for (double *p : ad->w) if (p) (void)hipFree(p);
         if (ad->ptmp) (void)hipFree(ad->ptmp);
         if (ad->rwork) (void)hipFree(ad->rwork);
@@ -244,11 +279,36 @@ int adjoint_destroy(nova_adjoint *ad)
     if (!ad) return NOVA_OK;
     (void)hipStreamSynchronize(ad->ctx->stream);
     for (double *p : ad->ckpt) if (p) (void)hipFree(p);
-    for (double *p : ad->seg) if (p) (void)hipFree(p);
+    for (auto &st : ad->store)
+        for (double *p : st) if (p) (void)hipFree(p);
     for (double *p : ad->w) if (p) (void)hipFree(p);
     if (ad->ptmp) (void)hipFree(ad->ptmp);
     if (ad->rwork) (void)hipFree(ad->rwork);
     delete ad;
+    return NOVA_OK;
+}
+
+// Segment stores beyond the first, within max_bytes of device memory (at
+// most one per checkpoint but the first); 0 releases them.
+int adjoint_reserve(nova_adjoint *ad, uint64_t max_bytes, int *count)
+{
+    NOVA_HIP(hipStreamSynchronize(ad->ctx->stream));
+    for (size_t k = 1; k < ad->store.size(); k++)
+        for (double *p : ad->store[k]) (void)hipFree(p);
+    ad->store.resize(1);
+    ad->held.resize(1);
+    const size_t bytes = sizeof(double) * 3 * (size_t)ad->ctx->g.np;
+    const uint64_t per = (uint64_t)bytes * 3 * ad->every;
+    const int nck = (int)ad->ckpt.size();
+    int want = (int)(max_bytes / per);
+    if (want > nck - 1) want = nck - 1;
+    for (int k = 0; k < want; k++) {
+        std::vector<double *> st;
+        if (alloc_store(bytes, 3 * ad->every, st)) break;  // as many as the device holds
+        ad->store.push_back(std::move(st));
+        ad->held.push_back(-1);
+    }
+    if (count) *count = (int)ad->store.size() - 1;
     return NOVA_OK;
 }
 
@@ -257,23 +317,30 @@ int adjoint_forward(nova_adjoint *ad, double dt, int nsteps, const double *U0, d
     if (nsteps < 1 || nsteps > ad->max_steps) return fail(NOVA_ERR_ARG, "nsteps outside the checkpoint budget");
     ad->dt = dt;
     ad->nsteps = nsteps;
-    ad->seg_held = -1;
+    for (int &h : ad->held) h = -1;
     int rc;
     if (U != U0 && (rc = copy3(ad->ctx, U, U0))) return rc;
-    const int C = ad->every, last = (nsteps - 1) / C;
-    for (int n = 0; n < last * C; n++) {
-        if (n % C == 0 && (rc = copy3(ad->ctx, ad->ckpt[n / C], U))) return rc;
-        if ((rc = fwd_step_store(ad, U, ad->w[0], ad->w[1], U))) return rc;
+    const int C = ad->every, nseg = (nsteps + C - 1) / C;
+    const int nst = (int)ad->store.size();
+    const int first_held = nseg > nst ? nseg - nst : 0;
+    for (int s = 0; s < nseg; s++) {
+        const int n0 = s * C, cnt = n0 + C < nsteps ? C : nsteps - n0;
+        if ((rc = copy3(ad->ctx, ad->ckpt[s], U))) return rc;
+        if (s < first_held) {
+            for (int q = 0; q < cnt; q++)
+                if ((rc = fwd_step_store(ad, U, ad->w[0], ad->w[1], U))) return rc;
+            continue;
+        }
+        // the last segments run in the stores (the last one in store 0, which
+        // the reverse sweep frees first), keeping their stage inputs
+        const int k = nseg - 1 - s;
+        std::vector<double *> &st = ad->store[k];
+        if ((rc = copy3(ad->ctx, st[0], U))) return rc;
+        for (int q = 0; q < cnt; q++)
+            if ((rc = fwd_step_store(ad, st[3 * q], st[3 * q + 1], st[3 * q + 2], q + 1 < cnt ? st[3 * q + 3] : U)))
+                return rc;
+        ad->held[k] = s;
     }
-    // the last segment runs in the segment store, keeping its stage inputs
-    if ((rc = copy3(ad->ctx, ad->ckpt[last], U))) return rc;
-    if ((rc = copy3(ad->ctx, ad->seg[0], U))) return rc;
-    for (int n = last * C; n < nsteps; n++) {
-        const int q = n - last * C;
-        double *nxt = n + 1 < nsteps ? ad->seg[3 * q + 3] : U;
-        if ((rc = fwd_step_store(ad, ad->seg[3 * q], ad->seg[3 * q + 1], ad->seg[3 * q + 2], nxt))) return rc;
-    }
-    ad->seg_held = last;
     return NOVA_OK;
 }
 
@@ -288,13 +355,10 @@ int adjoint_backward(nova_adjoint *ad, const double *lamT, double *lam0)
     const bool compact = ad->scheme == NOVA_SCHEME_COMPACT6;
     for (int s = nseg - 1; s >= 0; s--) {
         int cnt = 0;
-        if (ad->seg_held == s) {
-            cnt = (s + 1) * ad->every < ad->nsteps ? ad->every : ad->nsteps - s * ad->every;
-        } else if ((rc = recompute_segment(ad, s, &cnt))) {
-            return rc;
-        }
+        double *const *seg = nullptr;
+        if ((rc = segment_inputs(ad, s, &seg, &cnt))) return rc;
         for (int q = cnt - 1; q >= 0; q--) {
-            double *U = ad->seg[3 * q], *U1 = ad->seg[3 * q + 1], *U2 = ad->seg[3 * q + 2];
+            double *U = seg[3 * q], *U1 = seg[3 * q + 1], *U2 = seg[3 * q + 2];
             if ((rc = project(ad->ps, L, 1.0, ad->ptmp))) return rc;  // m3 = P m
             if (compact) {
                 // the pencil passes carry one pointwise addend and an accumulator
@@ -356,9 +420,10 @@ int adjoint_tlm(nova_adjoint *ad, const double *d0, double *dT)
     const int nseg = (ad->nsteps + ad->every - 1) / ad->every;
     for (int s = 0; s < nseg; s++) {
         int cnt = 0;
-        if ((rc = recompute_segment(ad, s, &cnt))) return rc;
+        double *const *seg = nullptr;
+        if ((rc = segment_inputs(ad, s, &seg, &cnt))) return rc;
         for (int q = 0; q < cnt; q++) {
-            double *U = ad->seg[3 * q], *U1 = ad->seg[3 * q + 1], *U2 = ad->seg[3 * q + 2];
+            double *U = seg[3 * q], *U1 = seg[3 * q + 1], *U2 = seg[3 * q + 2];
             if ((rc = tlm_stage(ad, U, D, nullptr, D1, ssp_args(0, ad->dt)))) return rc;
             if ((rc = tlm_stage(ad, U1, D1, D, D2, ssp_args(1, ad->dt)))) return rc;
             if ((rc = tlm_stage(ad, U2, D2, D, D, ssp_args(2, ad->dt)))) return rc;

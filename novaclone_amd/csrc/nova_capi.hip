@@ -108,6 +108,7 @@ int adjoint_forward(nova_adjoint *ad, double dt, int nsteps, const double *U0, d
 int adjoint_backward(nova_adjoint *ad, const double *lamT, double *lam0);
 int adjoint_tlm(nova_adjoint *ad, const double *d0, double *dT);
 int adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t len);
+int adjoint_reserve(nova_adjoint *ad, uint64_t max_bytes, int *count);
 int ensemble_create(const nova_instance_desc *inst, int n, int device, nova_ensemble **out);
 int ensemble_destroy(nova_ensemble *e);
 int ensemble_set_stream(nova_ensemble *e, hipStream_t st);
@@ -701,6 +702,12 @@ int nova_adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t repor
 {
     NOVA_REQUIRE(ad && reps >= 1 && (!report || report_len > 0), "bad argument");
     NOVA_GUARD(adjoint_autotune(ad, reps, report, report_len));
+}
+
+int nova_adjoint_reserve(nova_adjoint *ad, uint64_t max_bytes, int *segments)
+{
+    NOVA_REQUIRE(ad, "null adjoint");
+    NOVA_GUARD(adjoint_reserve(ad, max_bytes, segments));
 }
 
 int nova_wc_create(nova_ctx *ctx, double mu, double pc, nova_wc **out)

@@ -480,6 +480,17 @@ __device__ inline double rsign(int c, int d, int flip)
     return WALLS ? bsign(3, c, d, flip) : 1.0;
 }
 
+// v, negated when bit c of m is set: the value of component c of a vector
+// reached through a reflection across a wall normal to direction c (walls
+// only).  The sign bit is flipped, which equals a product with -1 exactly.
+template <bool WALLS>
+__device__ inline double rneg(double v, unsigned m, int c)
+{
+    if constexpr (!WALLS) return v;
+    const long long b = (long long)((m >> c) & 1u) << 63;
+    return __longlong_as_double(__double_as_longlong(v) ^ b);
+}
+
 // One wrap suffices on a periodic axis (|offset| <= 4 < n); walls reflect.
 template <bool WALLS>
 __device__ inline int wrap(int q, int n, int wall, int &flip)

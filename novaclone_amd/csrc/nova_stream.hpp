@@ -149,14 +149,15 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
     const int jm = bmap(j, g.n[1], g.wall[1], fj);
     const int km = bmap(k, g.n[2], g.wall[2], fk);
     const uint32_t col = (uint32_t)jm * g.s[2] + km;
-    double sgn_col[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) sgn_col[c] = rsign<WALLS>(c, 1, fj) * rsign<WALLS>(c, 2, fk);
+    // reflection bits: bit c set when the value reached component c of a
+    // vector through a wall normal to direction c (bit 0, the i plane, is
+    // added per plane)
+    const unsigned mcol = (fj ? 2u : 0u) | (fk ? 4u : 0u);
 
     // halo slots of this thread: LDS position, source column, signs
     int hrow[NSLOT], hcolk[NSLOT];
     uint32_t hsrc[NSLOT];
-    double hsg[NSLOT][3];
+    unsigned hmk[NSLOT];
     bool hon[NSLOT];
 #pragma unroll
     for (int s = 0; s < NSLOT; s++) {
@@ -183,8 +184,7 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
         hrow[s] = r;
         hcolk[s] = cc;
         hsrc[s] = (uint32_t)jm2 * g.s[2] + km2;
-#pragma unroll
-        for (int c = 0; c < 3; c++) hsg[s][c] = rsign<WALLS>(c, 1, f1) * rsign<WALLS>(c, 2, f2);
+        hmk[s] = (f1 ? 2u : 0u) | (f2 ? 4u : 0u);
     }
 
     // loaders
@@ -202,7 +202,7 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
         for (int f = 0; f < NF; f++)
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                q[f][c][o] = rsign<WALLS>(c, 0, fi) * sgn_col[c] * ldp(a.X[f], c, po, col);
+                q[f][c][o] = rneg<WALLS>(ldp(a.X[f], c, po, col), mcol | (fi ? 1u : 0u), c);
             }
     }
     // prefetched data of plane p: the queue entry of plane p + H, the halo of
@@ -307,7 +307,7 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                 if (hon[s])
 #pragma unroll
                     for (int n = 0; n < NC; n++)
-                        shr(sw, n, hrow[s], hcolk[s]) = rsign<WALLS>(n % 3, 0, Cur.fh) * hsg[s][n % 3] * Cur.hv[s][n];
+                        shr(sw, n, hrow[s], hcolk[s]) = rneg<WALLS>(Cur.hv[s][n], hmk[s] | (Cur.fh ? 1u : 0u), n % 3);
         } else {
             const int sw = NS == 1 ? 0 : (i - i0) & 1;
 #pragma unroll
@@ -319,7 +319,7 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                 if (hon[s])
 #pragma unroll
                     for (int n = 0; n < NC; n++)
-                        shr(sw, n, hrow[s], hcolk[s]) = rsign<WALLS>(n % 3, 0, Cur.fh) * hsg[s][n % 3] * Cur.hv[s][n];
+                        shr(sw, n, hrow[s], hcolk[s]) = rneg<WALLS>(Cur.hv[s][n], hmk[s] | (Cur.fh ? 1u : 0u), n % 3);
         }
         __syncthreads();
         load_pf(Cur, i + PF < i1 ? i + PF : i1 - 1, true);
@@ -414,7 +414,7 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                 for (int c = 0; c < 3; c++) {
 #pragma unroll
                     for (int o = 0; o < 2 * H; o++) q[f][c][o] = q[f][c][o + 1];
-                    q[f][c][2 * H] = fresh<PF>(rsign<WALLS>(c, 0, Nxt.fq) * sgn_col[c] * Nxt.nq[f][c]);
+                    q[f][c][2 * H] = fresh<PF>(rneg<WALLS>(Nxt.nq[f][c], mcol | (Nxt.fq ? 1u : 0u), c));
                     x0c[f][c] = fresh<PF>(Nxt.nx0[f][c]);
                 }
             if constexpr (ACC) {
@@ -446,7 +446,7 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
 #pragma unroll
                         for (int c = 0; c < 3; c++)
                             shr(pp, f * 3 + c, hrow[s], hcolk[s]) =
-                                rsign<WALLS>(c, 0, fh) * hsg[s][c] * ldp(a.X[f], c, ph, hsrc[s]);
+                                rneg<WALLS>(ldp(a.X[f], c, ph, hsrc[s]), hmk[s] | (fh ? 1u : 0u), c);
         }
     }
     if constexpr (PF == 1) {

@@ -195,6 +195,33 @@ def test_shear_layer_gradient_midsize(nv):
         assert abs(fd2 - ad) <= 1e-8 * abs(ad) + 4 * u * J / e2
 
 
+def test_reserved_segment_stores_give_the_same_results(nv):
+    """Segment stores kept from the forward sweep (nova_adjoint_reserve)
+    replace recomputed segments bit for bit, in the reverse sweep and the
+    tangent-linear model, for every number of stores; a partial last segment
+    (20 steps, checkpoints every 3)."""
+    import torch
+    G, P, U0 = shear_layer(nv, (32, 48, 32), (TWO_PI / 2, 1.5, TWO_PI / 2), kmax=2)
+    A = nv.Adjoint(nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-3), 3, 20)
+    dt, n = 0.01, 20
+    U = A.forward(dt, n, U0)
+    _, g = nv.enstrophy(G, U, grad=True)
+    d = rand_like(U0, 31, True)
+    l0 = A.backward(g)
+    t0 = A.tlm(d)
+    per = 3 * 3 * 3 * G.np * 8  # one store: 3 C vector fields
+    for want, expect in ((2 * per, 2), (1 << 50, 6), (0, 0)):
+        got = A.reserve(want)
+        print("stores", want, got, flush=True)
+        assert got == expect
+        assert torch.equal(A.forward(dt, n, U0), U)
+        assert torch.equal(A.backward(g), l0)
+        assert torch.equal(A.backward(g), l0)  # the stores still hold their segments
+        assert torch.equal(A.tlm(d), t0)
+        assert torch.equal(A.backward(g), l0)
+    A.close()
+
+
 def test_adjoint_cost_ratio(nv):
     """Adjoint sweep (segment recompute + reverse stages) against the plain
     forward solve of the same steps, at 64^3 (timing only, reported)."""
