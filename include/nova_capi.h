@@ -111,13 +111,17 @@ int nova_timing_collect(nova_ctx *ctx, char *buf, size_t buflen);
  * only; the others use variant 3), variants 2, 3 and 4 stage 64 x 4, 32 x 8
  * and 16 x 16 tiles with one-field models held to three waves per SIMD
  * (default: 3), variants 5 and 6 stage 32 x 8 and 16 x 16 tiles at two waves
- * per SIMD with two planes in flight (one-field models), variants 7 and 8 the
+ * per SIMD with two planes in flight (one-field models; two-field models:
+ * 32 x 8 tiles at two waves with one plane in flight, one staged plane or two
+ * alternating), variants 7 and 8 the
  * same with 512-thread 32 x 16 and 64 x 8 tiles (two-field models keep
  * 32 x 8 and 64 x 4 at one wave), variants 9 and 10 the 64 x 8 and 32 x 16
  * tiles with a ring of staged planes (the halo of a plane is loaded with its
  * interior, H planes ahead; two-field models as for 8 and 7), variants 11
  * and 12 the 64 x 8 and 32 x 16 tiles with two alternating staged planes
- * (two-field models: 64 x 4 and 32 x 8 at one wave); stage_ichunk = planes per
+ * (two-field models: 64 x 4 and 32 x 8 at one wave), variants 13 and 14
+ * blocks of one and two waves (16 x 4, 32 x 4) with two alternating planes
+ * (two-field models; one-field models run variant 3); stage_ichunk = planes per
  * streamed chunk (0 = auto); proj_variant 0 fuses the divergence into the
  * first FFT pass, 1 computes it (and the gradient correction) with streamed
  * row kernels, 2 (default) with LDS-staged streamed tiles.  Results are

@@ -390,7 +390,7 @@ int adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t len)
     std::string rep;
     if (ad->scheme == NOVA_SCHEME_COMPACT6)  // pencil passes have no launch-shape choice
         return write_report("compact scheme: no stage launch shapes to tune\n", report, len);
-    const int variants[] = {0, 2, 3, 4, 5, 6, 11, 12};
+    const int variants[] = {0, 2, 3, 4, 5, 6, 11, 12, 13, 14};
     // the forward state and a distinct second field (a copy), so that the
     // trials move the same bytes as the reverse sweep
     const double *X = ad->ckpt[0];
@@ -398,7 +398,7 @@ int adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t len)
     if (int rc = copy3(ctx, V, X)) return rc;
     if (int rc = copy3(ctx, ad->w[3], X)) return rc;
     auto adj = [&]() -> int { return adj_stage_scaled(ad, X, V, ad->w[0], 1.0); };
-    if (int rc = tune_stage_model(ctx, MODEL_ADJ, variants, 8, reps, adj, "adj_stage", rep)) return rc;
+    if (int rc = tune_stage_model(ctx, MODEL_ADJ, variants, 10, reps, adj, "adj_stage", rep)) return rc;
     auto tlm = [&]() -> int {
         LinArgs a = lin_args(ad, X, V);
         a.out[0] = ad->w[0];
@@ -406,7 +406,7 @@ int adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t len)
         a.sa = ssp_args(1, ad->dt);
         return launch_stream_model<MODEL_TLM>(ctx, a, "tlm_stage");
     };
-    if (int rc = tune_stage_model(ctx, MODEL_TLM, variants, 8, reps, tlm, "tlm_stage", rep)) return rc;
+    if (int rc = tune_stage_model(ctx, MODEL_TLM, variants, 10, reps, tlm, "tlm_stage", rep)) return rc;
     return write_report(rep, report, len);
 }
 

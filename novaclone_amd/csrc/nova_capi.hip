@@ -396,7 +396,7 @@ int nova_timing_collect(nova_ctx *ctx, char *buf, size_t buflen)
 
 int nova_ctx_tune(nova_ctx *ctx, int stage_variant, int stage_ichunk, int proj_variant)
 {
-    NOVA_REQUIRE(ctx && stage_variant >= 0 && stage_variant <= 12 && stage_ichunk >= 0 && proj_variant >= 0 &&
+    NOVA_REQUIRE(ctx && stage_variant >= 0 && stage_variant <= 14 && stage_ichunk >= 0 && proj_variant >= 0 &&
                      proj_variant <= 2,
                  "bad argument");
     for (int k = 0; k < 5; k++) {
@@ -411,7 +411,7 @@ int nova_ctx_tune(nova_ctx *ctx, int stage_variant, int stage_ichunk, int proj_v
 int nova_ctx_tune_kernel(nova_ctx *ctx, int kernel, int stage_variant, int stage_ichunk)
 {
     NOVA_REQUIRE(ctx && kernel >= NOVA_KERNEL_NS_STAGE && kernel <= NOVA_KERNEL_NS_FIRST_STAGE && stage_variant >= 0 &&
-                     stage_variant <= 12 && stage_ichunk >= 0,
+                     stage_variant <= 14 && stage_ichunk >= 0,
                  "bad argument");
     ctx->stage_variant[kernel] = stage_variant;
     ctx->stage_ichunk[kernel] = stage_ichunk;

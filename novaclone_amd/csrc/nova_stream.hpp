@@ -571,6 +571,18 @@ void launch_stream_w(nova_ctx *ctx, const StreamArgs<model_nf<MODEL>()> &a)
         if constexpr (W == 3) launch_stream_tile<H, MODEL, WALLS, CMB, 2, 32, 16, STG_TWO>(ctx, a);
         else launch_stream_tile<H, MODEL, WALLS, CMB, 1, 32, 8, STG_TWO>(ctx, a);
         break;
+    // two-field models: blocks of one wave (16 x 4) and two waves (32 x 4),
+    // two alternating staged planes, so that a CU holds several independent
+    // blocks at one wave per SIMD instead of one block whose four waves meet
+    // at every plane's barrier
+    case 13:
+        if constexpr (W == 3) launch_stream_tile<H, MODEL, WALLS, CMB, W, 32>(ctx, a);
+        else launch_stream_tile<H, MODEL, WALLS, CMB, 1, 16, 4, STG_TWO>(ctx, a);
+        break;
+    case 14:
+        if constexpr (W == 3) launch_stream_tile<H, MODEL, WALLS, CMB, W, 32>(ctx, a);
+        else launch_stream_tile<H, MODEL, WALLS, CMB, 1, 32, 4, STG_TWO>(ctx, a);
+        break;
     default: launch_stream_tile<H, MODEL, WALLS, CMB, W, 32>(ctx, a); break;
     }
 }

@@ -101,7 +101,7 @@ def test_two_field_stage_variants_give_the_same_results(nv, wall):
     S = nv.Solver(P, nv.MODEL_MHD, nv.RK_SSP3, nu=1e-2, eta=2e-2)
     A = nv.Adjoint(nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-2), 2, 4)
     ref = None
-    for v in (3, 0, 2, 4, 5, 6, 11, 12):
+    for v in (3, 0, 2, 4, 5, 6, 11, 12, 13, 14):
         G.tune(v, 0, 1)
         U, B = U0.clone(), B0.clone()
         S.step(1e-3, 2, U, G.field(1), B, G.field(1))
