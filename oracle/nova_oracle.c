@@ -11,6 +11,15 @@
  * accumulation order of every point is unchanged.
  *
  * Layout: padded AoS, index ((i*nj + j)*nk + k)*nc + c (k fastest).
+ *
+ * Pins (every function here is checked against something other than its
+ * own formula): discrete eigenvalues of the stencils on Fourier modes,
+ * brute-force and closed-form Poisson solves, dense solves of the
+ * tridiagonal and implicit systems, stability polynomials of the RK
+ * compositions, closed forms of the MHD, weakly-compressible, dissipation
+ * and diagnostic functions, the stretched mesh's inverse map, the EnSight
+ * byte layout (tests/test_oracle*.py); planted mistakes (ORC_PLANT) fail
+ * those pins (tests/test_oracle_planted.py).
  */
 #include "nova_oracle.h"
 
