@@ -137,6 +137,23 @@ enum {
 };
 /* Launch shape of one stage kernel (variants as for nova_ctx_tune). */
 int nova_ctx_tune_kernel(nova_ctx *ctx, int kernel, int stage_variant, int stage_ichunk);
+/* The reference's boundary fills on the walls normal to direction dir,
+ * src/BC/dirichletBC.hxx:90-125 and neumannBC.hxx:100-135 with value 0, as
+ * Field::setBC (Field.hxx:519-547) installs them from a BC file: kind[c]
+ * for component c of vector fields (scalar fields take neumann 0, as the
+ * reference's pBC files state).  Stencils then reflect component c oddly
+ * (Dirichlet) or evenly (Neumann) about the wall node, and the solver steps
+ * and projections overwrite the wall nodes after every assignment as the
+ * fills do (Dirichlet: 0; Neumann: the neighbouring interior node), the
+ * potential included.  Without this call a walled direction reflects the
+ * wall-normal component oddly and the others evenly and keeps the computed
+ * wall nodes.  The NS model only (MHD, weakly-compressible and adjoint
+ * steps refuse a context with these fills). */
+enum { NOVA_WALL_NEUMANN0 = 1, NOVA_WALL_DIRICHLET0 = 2 };
+int nova_ctx_set_wall_bc(nova_ctx *ctx, int dir, const int kind[3]);
+/* The wall-node overwrite of those fills on one stored field (ncomp 1 or
+ * 3), for callers composing their own steps (a no-op without them). */
+int nova_wall_fill(nova_ctx *ctx, double *F, int ncomp);
 
 int nova_alloc(nova_ctx *ctx, int ncomp, double **dptr);
 int nova_free(nova_ctx *ctx, double *dptr);
@@ -219,6 +236,13 @@ int nova_project(nova_poisson *ps, double *F, double s, double *p);
 /* Derivative scheme of the projection (NOVA_SCHEME_*): compact divergence
  * and gradient passes and the compact modified wavenumbers in the solve. */
 int nova_poisson_set_scheme(nova_poisson *ps, int scheme);
+/* Solve across a walled y: NOVA_WALL_SOLVE_EVEN (default) the even extension
+ * (a DCT-I through an FFT of length 2 N_y); NOVA_WALL_SOLVE_REFERENCE the
+ * reference's own FFT solve (poisson.cpp:167-265, which its walled apps
+ * call), y taken as periodic over the nodes 0 .. N_y - 1 and the wall node
+ * N_y left to the boundary fill. */
+enum { NOVA_WALL_SOLVE_EVEN = 0, NOVA_WALL_SOLVE_REFERENCE = 1 };
+int nova_poisson_set_wall_solve(nova_poisson *ps, int mode);
 /* Compact sixth-order derivative (order 1 or 2) along direction dir of each
  * of ncomp components (no reference counterpart; Lele's tridiagonal scheme
  * with alpha = 1/3 resp. 2/11, solved per line by a wavefront). */
@@ -243,6 +267,12 @@ int nova_solver_use_graph(nova_solver *sv, int enable);
  * compact scheme needs SSP-RK3 (NS or MHD) and a Poisson solver set to the
  * same scheme. */
 int nova_solver_set_scheme(nova_solver *sv, int scheme);
+/* A constant vector field F added to the NS right-hand side of every stage,
+ * dU = dt (R(U) + F): the Lorentz force div(B, B) - grad(B & B / 2) of a
+ * magnetic field that does not evolve (app/shearLayer/UEqn.H:16-26 with
+ * BEqn.H switched off, shearLayer.cpp:178-179).  F stays the caller's and
+ * must outlive the solver's use of it; NULL removes it. */
+int nova_solver_set_force(nova_solver *sv, const double *F);
 /* Autotuner (no reference counterpart): times every launch shape of the
  * solver's stage kernel (variants x chunks of 0, 8, 16, 32, 64, 128 planes) and
  * the three projection variants on U (and B), reps launches each after one

@@ -28,6 +28,8 @@ RK_LITERAL4, RK_SSP3, RK_SHEAR4 = 0, 1, 2
 MODEL_NS, MODEL_MHD = 0, 1
 KERNEL_NS_STAGE, KERNEL_MHD_STAGE, KERNEL_TLM_STAGE, KERNEL_ADJ_STAGE, KERNEL_NS_FIRST_STAGE = 0, 1, 2, 3, 4
 SCHEME_EXPLICIT, SCHEME_COMPACT6 = 0, 1
+WALL_NEUMANN0, WALL_DIRICHLET0 = 1, 2          # nova_ctx_set_wall_bc kinds
+WALL_SOLVE_EVEN, WALL_SOLVE_REFERENCE = 0, 1   # nova_poisson_set_wall_solve
 # reference BC kinds for the padded-layout fill (src/BC/BCenums.h:8)
 PBC_PERIODIC, PBC_NEUMANN, PBC_DIRICHLET, PBC_EVEN, PBC_ODD = 0, 1, 2, 10, 11
 # pointwise Field products (nova_field_op)
@@ -161,6 +163,10 @@ _SIGS = {
     "nova_adjoint_backward": [_P, _P, _P],
     "nova_adjoint_tlm": [_P, _P, _P],
     "nova_adjoint_autotune": [_P, _I, C.c_char_p, C.c_size_t],
+    "nova_ctx_set_wall_bc": [_P, _I, _P],
+    "nova_wall_fill": [_P, _P, _I],
+    "nova_poisson_set_wall_solve": [_P, _I],
+    "nova_solver_set_force": [_P, _P],
     "nova_adjoint_reserve": [_P, C.c_uint64, _P],
     "nova_weighted_dot": [_P, _P, _P, _I, _DP],
     "nova_enstrophy": [_P, _P, _DP, _P],
@@ -277,6 +283,19 @@ class Mesh:
         (0 = automatic) and the projection variant."""
         _check(lib().nova_ctx_tune(self._h, int(stage_variant), int(stage_ichunk), int(proj_variant)),
                "nova_ctx_tune")
+
+    def set_wall_bc(self, direction: int, kinds) -> None:
+        """The reference's boundary fills (dirichletBC / neumannBC with value
+        0) on the walls normal to ``direction``: ``kinds[c]`` per vector
+        component (WALL_NEUMANN0 or WALL_DIRICHLET0); wall nodes are then
+        overwritten after every assignment, as the fills do."""
+        k = (C.c_int * 3)(*[int(x) for x in kinds])
+        _check(lib().nova_ctx_set_wall_bc(self._h, int(direction), k), "nova_ctx_set_wall_bc")
+
+    def wall_fill(self, F: torch.Tensor) -> None:
+        """The wall-node overwrite of those fills on one field, in place."""
+        nc = F.shape[0] if F.dim() == 4 else 1
+        _check(lib().nova_wall_fill(self._h, _fld(F, self, nc), int(nc)), "nova_wall_fill")
 
     def tune_kernel(self, kernel: int, stage_variant: int, stage_ichunk: int = 0) -> None:
         """Launch shape of one stage kernel (KERNEL_NS_STAGE .. KERNEL_ADJ_STAGE)."""
@@ -427,6 +446,12 @@ class Poisson:
         _check(lib().nova_project(self._h, _fld(F, self.mesh, 3, "F"), float(s), _fld(p, self.mesh, 1, "p")),
                "nova_project")
 
+    def set_wall_solve(self, mode: int) -> None:
+        """WALL_SOLVE_EVEN (the even extension, default) or
+        WALL_SOLVE_REFERENCE (the reference's FFT, y periodic over the nodes
+        0 .. N_y - 1) across a walled y."""
+        _check(lib().nova_poisson_set_wall_solve(self._h, int(mode)), "nova_poisson_set_wall_solve")
+
     def close(self):
         if getattr(self, "_h", None):
             lib().nova_poisson_destroy(self._h)
@@ -461,6 +486,12 @@ class Solver:
         m = self.mesh
         _check(lib().nova_solver_step(self._h, float(dt), int(nsteps), _fld(U, m, 3, "U"), _fld(B, m, 3, "B"),
                                       _fld(p, m, 1, "p"), _fld(pB, m, 1, "pB")), "nova_solver_step")
+
+    def set_force(self, F: torch.Tensor | None) -> None:
+        """A constant vector field added to the NS right-hand side of every
+        stage (kept referenced here while it is in use)."""
+        self._force = F
+        _check(lib().nova_solver_set_force(self._h, _fld(F, self.mesh, 3, "F")), "nova_solver_set_force")
 
     def autotune(self, U, B=None, reps: int = 3) -> str:
         """Time every stage launch shape and both projection variants on U

@@ -19,7 +19,7 @@ BUILD = PKG / "_objs"
 LIB = PKG / "libnova_hip.so"
 SOURCES = ["nova_capi.hip", "nova_ops.hip", "nova_fft.hip", "nova_step.hip", "nova_adjoint.hip", "nova_ensemble.hip",
            "nova_compact.hip", "nova_wc.hip", "nova_io.hip", "nova_implicit.hip", "nova_mhd_lit.hip",
-           "nova_algebra.hip", "nova_hdf5.hip"]
+           "nova_algebra.hip", "nova_hdf5.hip", "nova_walls.hip"]
 ARCH = os.environ.get("NOVA_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -62,7 +62,7 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
-APPS = ["tg_poisson", "ot_poisson", "tg_wc", "mhdjet"]
+APPS = ["tg_poisson", "ot_poisson", "tg_wc", "mhdjet", "shear_layer"]
 
 
 def build_apps() -> None:

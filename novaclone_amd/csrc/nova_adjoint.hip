@@ -233,6 +233,7 @@ int adjoint_create(nova_solver *sv, int every, int max_steps, nova_adjoint **out
     nova_poisson *ps = solver_poisson(sv);
     nova_ctx *ctx = poisson_ctx(ps);
     if (ctx->desc.hyp) return fail(NOVA_ERR_UNSUPPORTED, "the adjoint needs a uniform z grid");
+    if (ctx->wall_fill) return fail(NOVA_ERR_UNSUPPORTED, "the adjoint takes the reflection rule at walls");
     if (every < 1 || max_steps < 1) return fail(NOVA_ERR_ARG, "checkpoint interval and step budget must be >= 1");
     nova_adjoint *ad = new nova_adjoint();
     ad->sv = sv;

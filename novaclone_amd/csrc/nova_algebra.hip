@@ -85,8 +85,8 @@ __global__ void k_grad_vec(Grid g, const double *__restrict__ U, double *__restr
                 int fp, fm, ip[3] = {i, j, k}, im[3] = {i, j, k};
                 ip[d] = bmap(idx[d] + l, g.n[d], g.wall[d], fp);
                 im[d] = bmap(idx[d] - 1 - l, g.n[d], g.wall[d], fm);
-                const double vp = bsign(3, q, d, fp) * U[q * g.np + ((int64_t)ip[0] * g.s[1] + ip[1]) * g.s[2] + ip[2]];
-                const double vm = bsign(3, q, d, fm) * U[q * g.np + ((int64_t)im[0] * g.s[1] + im[1]) * g.s[2] + im[2]];
+                const double vp = bsign(g, 3, q, d, fp) * U[q * g.np + ((int64_t)ip[0] * g.s[1] + ip[1]) * g.s[2] + ip[2]];
+                const double vm = bsign(g, 3, q, d, fm) * U[q * g.np + ((int64_t)im[0] * g.s[1] + im[1]) * g.s[2] + im[2]];
                 const double w = d == 2 ? g.al[2][l] * g.zinv[k] : g.al[d][l];
                 acc += w * (vp - vm);
             }

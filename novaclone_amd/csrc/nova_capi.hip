@@ -100,6 +100,8 @@ int solver_step(nova_solver *sv, double dt, int nsteps, double *U, double *B, do
 int solver_use_graph(nova_solver *sv, int enable);
 int solver_autotune(nova_solver *sv, const double *U, const double *B, int reps, char *report, size_t len);
 int solver_set_scheme(nova_solver *sv, int scheme);
+int solver_set_force(nova_solver *sv, const double *F);
+int poisson_set_wall_solve(nova_poisson *ps, int mode);
 int poisson_set_scheme(nova_poisson *ps, int scheme);
 nova_ctx *poisson_ctx(nova_poisson *ps);
 int adjoint_create(nova_solver *sv, int every, int max_steps, nova_adjoint **out);
@@ -193,6 +195,7 @@ static int ctx_init(nova_ctx *ctx, const nova_grid_desc *desc, int device)
     for (int a = 0; a < 3; a++) {
         g.n[a] = d.n[a];
         g.wall[a] = d.bc[a] == NOVA_BC_WALL;
+        g.odd[a] = 1u << a;  // the wall-normal velocity: dirichletBC 0; the others neumannBC 0
         g.s[a] = d.n[a] + (g.wall[a] ? 1 : 0);
         ctx->d[a] = d.L[a] / d.n[a];  // Mesh.cpp:68-70
         ctx->pad[a] = d.n[a] + 1 + d.m;  // Mesh.cpp:89-91,136-138
@@ -615,6 +618,30 @@ int nova_solver_set_scheme(nova_solver *sv, int scheme)
 {
     NOVA_REQUIRE(sv, "null solver");
     NOVA_GUARD(solver_set_scheme(sv, scheme));
+}
+
+int nova_solver_set_force(nova_solver *sv, const double *F)
+{
+    NOVA_REQUIRE(sv, "null solver");
+    NOVA_GUARD(solver_set_force(sv, F));
+}
+
+int nova_poisson_set_wall_solve(nova_poisson *ps, int mode)
+{
+    NOVA_REQUIRE(ps, "null Poisson solver");
+    NOVA_GUARD(poisson_set_wall_solve(ps, mode));
+}
+
+int nova_ctx_set_wall_bc(nova_ctx *ctx, int dir, const int kind[3])
+{
+    NOVA_REQUIRE(ctx && kind, "null argument");
+    NOVA_GUARD(set_wall_bc(ctx, dir, kind));
+}
+
+int nova_wall_fill(nova_ctx *ctx, double *F, int ncomp)
+{
+    NOVA_REQUIRE(ctx && F && (ncomp == 1 || ncomp == 3), "bad argument");
+    NOVA_GUARD(wall_fill(ctx, F, ncomp));
 }
 
 int nova_poisson_set_scheme(nova_poisson *ps, int scheme)

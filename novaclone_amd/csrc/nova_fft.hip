@@ -41,7 +41,7 @@ struct nova_poisson {
     nova_ctx *ctx = nullptr;
     int Nx = 0, Ny = 0, Nz = 0;
     int sy = 0;      // stored y extent
-    int wall_y = 0;
+    int wall_y = 0;  // 1: a walled y solved on the even extension
     int Ty = 0;      // y transform length
     int Kc = 0;      // Nz/2 + 1
     nova::FftLen fx, fy, fz;
@@ -251,7 +251,7 @@ __device__ inline double ldw(const double *__restrict__ f, const Grid &g, int nc
     const int kk = wrapq<WALLS>(k, g.n[2], g.wall[2], fk);
     const double v = f[c * g.np + lin(g, ii, jj, kk)];
     if (!WALLS) return v;
-    return bsign(nc, c, 0, fi) * bsign(nc, c, 1, fj) * bsign(nc, c, 2, fk) * v;
+    return bsign(g, nc, c, 0, fi) * bsign(g, nc, c, 1, fj) * bsign(g, nc, c, 2, fk) * v;
 }
 
 // div(F) * scale at a stored node (src/ex/div/div.hxx:37-77).
@@ -380,8 +380,12 @@ __global__ void __launch_bounds__(kLineThreads) k_fft_y(const double2 *__restric
     }
     __syncthreads();
     lds_fft<N, true, INV, LB, N, 1, kLineThreads>(buf, stw);
-    for (int e = threadIdx.x; e < LB * sy; e += kLineThreads) {
-        const int ln = e / sy, q = e % sy;
+    // the stored part of each line: sy entries of the extension, or the N of
+    // a periodic line (a walled y solved periodically over its first N
+    // nodes leaves entry N, the wall node's, untouched)
+    const int qmax = sy < N ? sy : N;
+    for (int e = threadIdx.x; e < LB * qmax; e += kLineThreads) {
+        const int ln = e / qmax, q = e % qmax;
         const int64_t L = l0 + ln;
         if (L < nlines_total) H[L * sy + q] = buf[ln * N + swz<N>(q)];
     }
@@ -612,22 +616,24 @@ __global__ void __launch_bounds__(kRK *kRJ) k_div_rows(Grid g, RowTiles t, const
     ColNbrs<H, WALLS> nb;
     nb.init(g, j, k);
     const double *Fx = F, *Fy = F + g.np, *Fz = F + 2 * g.np;
+    // F_y and F_z change sign through a reflection when they are odd there
+    const bool oy = (g.odd[1] >> 1) & 1u, oz = g.wall[2] && ((g.odd[2] >> 2) & 1u);
     double q[2 * H + 1];
 #pragma unroll
     for (int o = 0; o < 2 * H + 1; o++) {
         int fi;
         const int im = wrapq<WALLS>(i0 - H + o, g.n[0], g.wall[0], fi);
-        q[o] = (WALLS && fi ? -1.0 : 1.0) * Fx[(int64_t)im * plane + col];
+        q[o] = (WALLS && fi && (g.odd[0] & 1u) ? -1.0 : 1.0) * Fx[(int64_t)im * plane + col];
     }
     double vy[2 * H], wz[2 * H];
     auto load_plane = [&](int i, double (&y)[2 * H], double (&z)[2 * H]) {
         const int64_t pi = (int64_t)i * plane;
 #pragma unroll
         for (int l = 0; l < H; l++) {
-            y[l] = nb.sg(l, 0) * Fy[pi + nb.jp[l]];
-            y[H + l] = nb.sg(l, 1) * Fy[pi + nb.jm[l]];
-            z[l] = (g.wall[2] ? nb.sg(l, 2) : 1.0) * Fz[pi + nb.kp[l]];
-            z[H + l] = (g.wall[2] ? nb.sg(l, 3) : 1.0) * Fz[pi + nb.km[l]];
+            y[l] = (oy ? nb.sg(l, 0) : 1.0) * Fy[pi + nb.jp[l]];
+            y[H + l] = (oy ? nb.sg(l, 1) : 1.0) * Fy[pi + nb.jm[l]];
+            z[l] = (oz ? nb.sg(l, 2) : 1.0) * Fz[pi + nb.kp[l]];
+            z[H + l] = (oz ? nb.sg(l, 3) : 1.0) * Fz[pi + nb.km[l]];
         }
     };
     load_plane(i0, vy, wz);
@@ -637,7 +643,7 @@ __global__ void __launch_bounds__(kRK *kRJ) k_div_rows(Grid g, RowTiles t, const
         if (i + 1 < i1) {
             int fi;
             const int im = wrapq<WALLS>(i + H + 1, g.n[0], g.wall[0], fi);
-            nq = (WALLS && fi ? -1.0 : 1.0) * Fx[(int64_t)im * plane + col];
+            nq = (WALLS && fi && (g.odd[0] & 1u) ? -1.0 : 1.0) * Fx[(int64_t)im * plane + col];
             load_plane(i + 1, ny, nz);
         }
         double ax = 0.0, ay = 0.0, az = 0.0;
@@ -810,9 +816,11 @@ __global__ void __launch_bounds__(kPK *kPJ) k_proj_stream(Grid g, RowTiles t, co
         hsrc = (int64_t)jm2 * g.s[2] + km2;
         hf = hk == 1 ? f1 : f2;  // the reflection that matters for the staged component
     }
-    // sign of a staged value: DIV stages F_y (odd across y walls) and F_z
-    // (odd across z walls); p is even
-    auto sgn = [&](int flip) -> double { return (DIV && WALLS && flip) ? -1.0 : 1.0; };
+    // sign of a staged value: DIV stages F_y (reflected across y walls) and
+    // F_z (across z walls), each odd there or not as Grid::odd says; p is even
+    const bool ox = g.odd[0] & 1u, oy = (g.odd[1] >> 1) & 1u, oz = (g.odd[2] >> 2) & 1u;
+    auto sgn_y = [&](int flip) -> double { return (DIV && WALLS && flip && oy) ? -1.0 : 1.0; };
+    auto sgn_z = [&](int flip) -> double { return (DIV && WALLS && flip && oz) ? -1.0 : 1.0; };
 
     struct Pf {
         double q;      // queue entry (F_x or p) of plane p + H
@@ -850,18 +858,18 @@ __global__ void __launch_bounds__(kPK *kPJ) k_proj_stream(Grid g, RowTiles t, co
     for (int o = 0; o < 2 * H + 1; o++) {
         int fi;
         const int64_t po = plane_of(i0 - H + o, fi);
-        q[o] = (DIV && WALLS && fi ? -1.0 : 1.0) * qsrc[po + col];
+        q[o] = (DIV && WALLS && fi && ox ? -1.0 : 1.0) * qsrc[po + col];
     }
     const double zi = g.zinv[km];
-    const double cyz_sign_y = sgn(fj), cyz_sign_z = sgn(fk);
+    const double cyz_sign_y = sgn_y(fj), cyz_sign_z = sgn_z(fk);
 
     auto body = [&](int i, Pf &Cur, Pf &Nxt) {
         // stage plane i
         if (DIV) {
             sa[tj + H][tk + H] = cyz_sign_y * Cur.cy;
             sb[tj][tk + H] = cyz_sign_z * Cur.cz;
-            if (hk == 1) sa[hr][hc] = sgn(hf) * Cur.h;
-            else if (hk == 2) sb[hr - H][hc] = sgn(hf) * Cur.h;
+            if (hk == 1) sa[hr][hc] = sgn_y(hf) * Cur.h;
+            else if (hk == 2) sb[hr - H][hc] = sgn_z(hf) * Cur.h;
         } else {
             sa[tj + H][tk + H] = q[H];
             if (hk) sa[hr][hc] = Cur.h;
@@ -898,7 +906,7 @@ __global__ void __launch_bounds__(kPK *kPJ) k_proj_stream(Grid g, RowTiles t, co
         if (i + 1 < i1) {
 #pragma unroll
             for (int o = 0; o < 2 * H; o++) q[o] = q[o + 1];
-            q[2 * H] = (DIV && WALLS && Nxt.fq ? -1.0 : 1.0) * Nxt.q;
+            q[2 * H] = (DIV && WALLS && Nxt.fq && ox ? -1.0 : 1.0) * Nxt.q;
         }
     };
     Pf A, B;
@@ -1258,6 +1266,30 @@ int poisson_set_scheme(nova_poisson *ps, int scheme)
 
 int poisson_scheme(nova_poisson *ps) { return ps->scheme; }
 
+// Even extension (DCT-I) or the reference's periodic solve over the first
+// N_y nodes of a walled y: the y transform length, its tables, the
+// normalisation and the wavenumbers follow.
+int poisson_set_wall_solve(nova_poisson *ps, int mode)
+{
+    nova_ctx *ctx = ps->ctx;
+    if (mode != NOVA_WALL_SOLVE_EVEN && mode != NOVA_WALL_SOLVE_REFERENCE) return fail(NOVA_ERR_ARG, "unknown wall solve");
+    if (!ctx->g.wall[1]) return fail(NOVA_ERR_ARG, "the grid has no walls in y");
+    const int ext = mode == NOVA_WALL_SOLVE_EVEN;
+    if (ext == ps->wall_y) return NOVA_OK;
+    const int Ty = ext ? 2 * ps->Ny : ps->Ny;
+    if (!fft_length_supported(Ty)) return fail(NOVA_ERR_UNSUPPORTED, "FFT length of the y solve");
+    NOVA_HIP(hipDeviceSynchronize());
+    nova::FftLen fy;
+    if (int rc = make_len(fy, Ty)) return rc;
+    free_len(ps->fy);
+    ps->fy = fy;
+    ps->wall_y = ext;
+    ps->Ty = Ty;
+    ps->norm = (double)((long)ps->Nx * (long)ps->Ty * (long)ps->Nz);
+    ctx->tune_epoch++;  // recorded step graphs hold the old transform
+    return poisson_set_scheme(ps, ps->scheme);  // the wavenumbers of the new y period
+}
+
 int poisson_destroy(nova_poisson *ps)
 {
     if (!ps) return NOVA_OK;
@@ -1357,6 +1389,9 @@ int project(nova_poisson *ps, double *F, double s, double *p)
         return compact_grad_correct(ctx, p, s, F);
     }
     if (int rc = poisson_run(ps, F, 1, 1.0 / s, p)) return rc;
+    // the reference's fills: p.correctBoundaryConditions() before the
+    // gradient (app/shearLayer/UEqn.H:32-34)
+    if (int rc = wall_fill(ctx, p, 1)) return rc;
     const Grid &g = ctx->g;
     dim3 grid((g.s[2] + 63) / 64, (g.s[1] + 3) / 4, g.s[0]);
     TimedScope ts(ctx, "grad_correct");

@@ -22,6 +22,7 @@ struct Grid {
     int s[3];          // stored extents
     int n[3];          // cells
     int wall[3];       // 1 = reflecting walls at nodes 0 and n
+    unsigned odd[3];   // bit c: component c of a vector field is odd across walls normal to d
     int h;             // m/2
     int64_t np;        // s0*s1*s2
     double a[3][kMaxH];  // first derivative weights c_l / (2 (l+1) h_d)
@@ -57,11 +58,12 @@ __host__ __device__ inline int bmap(int q, int n, int wall, int &flip)
 }
 
 // Sign of component c of an ncomp-component field reached through a
-// reflection across a wall normal to direction d: the wall-normal
-// component of a vector is odd, everything else even.
-__host__ __device__ inline double bsign(int ncomp, int c, int d, int flip)
+// reflection across a wall normal to direction d: odd components of a
+// vector (g.odd[d], by default the wall-normal one) change sign, scalars
+// and the other components are even.
+__host__ __device__ inline double bsign(const Grid &g, int ncomp, int c, int d, int flip)
 {
-    return (flip && ncomp == 3 && c == d) ? -1.0 : 1.0;
+    return (flip && ncomp == 3 && ((g.odd[d] >> c) & 1u)) ? -1.0 : 1.0;
 }
 
 struct DevBuf {
@@ -105,6 +107,10 @@ struct nova_ctx {
     double *d_red = nullptr;         // reduction partials
     size_t red_cap = 0;
     double *h_red = nullptr;         // pinned host result slot
+    // the reference's wall fills (nova_ctx_set_wall_bc): wall nodes are
+    // overwritten after every assignment, per direction and component
+    int wall_fill = 0;
+    int wall_kind[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 };
 
 namespace nova {
@@ -214,6 +220,8 @@ int field_op(nova_ctx *ctx, int op, const double *a, int nca, const double *b, i
              double *out);
 int grad_vec_literal(nova_ctx *ctx, const double *U, double *out);
 int reduce_cfl(nova_ctx *ctx, const double *U, double dt, double out[6]);
+int wall_fill(nova_ctx *ctx, double *F, int nc);
+int set_wall_bc(nova_ctx *ctx, int dir, const int kind[3]);
 int hdf5_write_host(const char *file, const int64_t dims[3], const int64_t stored[3], const double *host);
 int hdf5_read_host(const char *file, int64_t dims[3], double *host, size_t cap);
 int hdf5_write_field(nova_ctx *ctx, const char *file, const double *F, int comp);

@@ -29,7 +29,7 @@ __device__ inline double ld(const double *__restrict__ f, const Grid &g, int nc,
     int ii = bmap(i, g.n[0], g.wall[0], fi);
     int jj = bmap(j, g.n[1], g.wall[1], fj);
     int kk = bmap(k, g.n[2], g.wall[2], fk);
-    double s = bsign(nc, c, 0, fi) * bsign(nc, c, 1, fj) * bsign(nc, c, 2, fk);
+    double s = bsign(g, nc, c, 0, fi) * bsign(g, nc, c, 1, fj) * bsign(g, nc, c, 2, fk);
     return s * f[c * g.np + lin(g, ii, jj, kk)];
 }
 

@@ -475,13 +475,13 @@ __device__ inline void adj_du(const AdjK &k, const AdjZ<H> &z, const Acc &A, dou
 // Sign of a value reached through a reflection (walls only; a periodic
 // grid has no reflections and carries no signs).
 template <bool WALLS>
-__device__ inline double rsign(int c, int d, int flip)
+__device__ inline double rsign(const Grid &g, int c, int d, int flip)
 {
-    return WALLS ? bsign(3, c, d, flip) : 1.0;
+    return WALLS ? bsign(g, 3, c, d, flip) : 1.0;
 }
 
-// v, negated when bit c of m is set: the value of component c of a vector
-// reached through a reflection across a wall normal to direction c (walls
+// v, negated when bit c of m is set: m is the exclusive or of the odd
+// masks (Grid::odd) of the directions the value was reflected across (walls
 // only).  The sign bit is flipped, which equals a product with -1 exactly.
 template <bool WALLS>
 __device__ inline double rneg(double v, unsigned m, int c)

@@ -34,6 +34,7 @@ struct nova_solver {
     double nu = 0.0, eta = 0.0;
     double *wa = nullptr, *wb = nullptr, *wc = nullptr, *wd = nullptr;  // stage buffers (3 comps each)
     double *div_tmp = nullptr;
+    const double *force = nullptr;  // constant right-hand-side term (nova_solver_set_force)
     int use_graph = 0;
     int scheme = NOVA_SCHEME_EXPLICIT;
     double *rwork = nullptr;  // compact right-hand-side accumulator (3 comps, 6 for MHD)
@@ -81,7 +82,7 @@ __global__ void __launch_bounds__(kTK *kTJ) k_stage_rows(Grid g, StreamArgs<NF> 
 #pragma unroll
         for (int f = 0; f < NF; f++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) q[f][c][o] = rsign<WALLS>(c, 0, fi) * a.X[f][c * g.np + po + col];
+            for (int c = 0; c < 3; c++) q[f][c][o] = rsign<WALLS>(g, c, 0, fi) * a.X[f][c * g.np + po + col];
     }
     const double zi = g.zinv[k], zl = g.zlap[k], zm = g.zmet[k];
     NsZ<H> nz{};
@@ -95,7 +96,7 @@ __global__ void __launch_bounds__(kTK *kTJ) k_stage_rows(Grid g, StreamArgs<NF> 
 #pragma unroll
             for (int f = 0; f < NF; f++)
 #pragma unroll
-                for (int c = 0; c < 3; c++) nq[f][c] = rsign<WALLS>(c, 0, fi) * a.X[f][c * g.np + po + col];
+                for (int c = 0; c < 3; c++) nq[f][c] = rsign<WALLS>(g, c, 0, fi) * a.X[f][c * g.np + po + col];
         }
         const int64_t pi = (int64_t)i * plane;
         auto A = [&](int f, int c, int d, int o) -> double {
@@ -104,7 +105,7 @@ __global__ void __launch_bounds__(kTK *kTJ) k_stage_rows(Grid g, StreamArgs<NF> 
             int64_t off;
             if (d == 1) off = pi + (int64_t)wrap<WALLS>(j + o, g.n[1], g.wall[1], fl) * s2 + k;
             else off = pi + (int64_t)j * s2 + wrap<WALLS>(k + o, g.n[2], g.wall[2], fl);
-            return rsign<WALLS>(c, d, fl) * a.X[f][c * g.np + off];
+            return rsign<WALLS>(g, c, d, fl) * a.X[f][c * g.np + off];
         };
         const int64_t qi = pi + col;
         const StageArgs &sa = a.sa;
@@ -259,19 +260,31 @@ int ns_step_once(nova_solver *sv, double dt, double *U, double *p)
         if ((rc = compact_ns_stage(ctx, sv->wb, U, U, ssp(2, dt), sv->nu, sv->rwork))) return rc;
         return project(sv->ps, U, dt, p);
     }
+    // one stage: the stage kernel, the constant force with the weight the
+    // stage gives its increment, the reference's wall fill of the result
+    auto stage = [&](const double *X, const double *X0, double *out, const StageArgs &a, double fw) -> int {
+        if (int r = ns_stage(ctx, X, X0, out, a, sv->nu)) return r;
+        if (sv->force)
+            if (int r = launch_axpby(ctx, 3 * (size_t)ctx->g.np, 1.0, out, fw * dt, sv->force, out)) return r;
+        return wall_fill(ctx, out, 3);
+    };
+    auto proj = [&](double *F, double s) -> int {
+        if (int r = project(sv->ps, F, s, p)) return r;
+        return wall_fill(ctx, F, 3);  // U.correctBoundaryConditions()
+    };
     if (sv->mode == NOVA_RK_SSP3) {
-        if ((rc = ns_stage(ctx, U, nullptr, sv->wa, ssp(0, dt), sv->nu))) return rc;
-        if ((rc = ns_stage(ctx, sv->wa, U, sv->wb, ssp(1, dt), sv->nu))) return rc;
-        if ((rc = ns_stage(ctx, sv->wb, U, U, ssp(2, dt), sv->nu))) return rc;
-        return project(sv->ps, U, dt, p);
+        if ((rc = stage(U, nullptr, sv->wa, ssp(0, dt), 1.0))) return rc;
+        if ((rc = stage(sv->wa, U, sv->wb, ssp(1, dt), 0.25))) return rc;
+        if ((rc = stage(sv->wb, U, U, ssp(2, dt), 2.0 / 3.0))) return rc;
+        return proj(U, dt);
     }
     if (sv->mode == NOVA_RK_LITERAL4) {
         StageArgs a{};
         a.form = FORM_LIT4;
         a.dt = dt;
         for (int j = 0; j < 4; j++) a.w[j] = kRK4W[j];
-        if ((rc = ns_stage(ctx, U, nullptr, sv->wa, a, sv->nu))) return rc;
-        if ((rc = project(sv->ps, sv->wa, dt, p))) return rc;
+        if ((rc = stage(U, nullptr, sv->wa, a, ((kRK4W[0] + kRK4W[1]) + kRK4W[2]) + kRK4W[3]))) return rc;
+        if ((rc = proj(sv->wa, dt))) return rc;
         return copy3(ctx, U, sv->wa);
     }
     // NOVA_RK_SHEAR4
@@ -281,17 +294,19 @@ int ns_step_once(nova_solver *sv, double dt, double *U, double *p)
         a.form = FORM_SHEAR;
         a.dt = dt;
         a.beta = kRK4W[rk];
-        if ((rc = ns_stage(ctx, U, sv->wb, sv->wa, a, sv->nu))) return rc;
-        if ((rc = project(sv->ps, sv->wa, dt * kRK4C[rk], p))) return rc;
+        if ((rc = stage(U, sv->wb, sv->wa, a, kRK4W[rk]))) return rc;
+        if ((rc = proj(sv->wa, dt * kRK4C[rk]))) return rc;
         if ((rc = copy3(ctx, U, sv->wa))) return rc;
     }
     return NOVA_OK;
 }
 
+
 int mhd_step_once(nova_solver *sv, double dt, double *U, double *B, double *p, double *pB)
 {
     nova_ctx *ctx = sv->ctx;
     int rc;
+    if (ctx->wall_fill) return fail(NOVA_ERR_UNSUPPORTED, "the reference's wall fills are set up for the NS model");
     if (sv->mode == NOVA_RK_LITERAL4) {
         // app/OT_Poisson/UEqn.H:14-40 then BEqn.H:14-41 (app/mhdjet alike)
         if (sv->scheme != NOVA_SCHEME_EXPLICIT)
@@ -465,11 +480,21 @@ int solver_autotune(nova_solver *sv, const double *U, const double *B, int reps,
     return write_report(rep, report, len);
 }
 
+int solver_set_force(nova_solver *sv, const double *F)
+{
+    if (F && sv->model != NOVA_MODEL_NS) return fail(NOVA_ERR_UNSUPPORTED, "a constant force is an NS model term");
+    if (F && sv->scheme != NOVA_SCHEME_EXPLICIT) return fail(NOVA_ERR_UNSUPPORTED, "a constant force needs the explicit scheme");
+    sv->force = F;
+    sv->ctx->tune_epoch++;  // a recorded step graph lacks (or holds) the force
+    return NOVA_OK;
+}
+
 int solver_set_scheme(nova_solver *sv, int scheme)
 {
     if (scheme != NOVA_SCHEME_EXPLICIT && scheme != NOVA_SCHEME_COMPACT6) return fail(NOVA_ERR_ARG, "unknown scheme");
     if (scheme == NOVA_SCHEME_COMPACT6) {
         if (sv->mode != NOVA_RK_SSP3) return fail(NOVA_ERR_UNSUPPORTED, "the compact scheme covers SSP-RK3");
+        if (sv->force) return fail(NOVA_ERR_UNSUPPORTED, "a constant force needs the explicit scheme");
         if (int rc = compact_supported(sv->ctx)) return rc;
         const int nc = sv->model == NOVA_MODEL_MHD ? 6 : 3;
         if (!sv->rwork) NOVA_HIP(hipMalloc((void **)&sv->rwork, sizeof(double) * nc * sv->ctx->g.np));

@@ -149,10 +149,10 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
     const int jm = bmap(j, g.n[1], g.wall[1], fj);
     const int km = bmap(k, g.n[2], g.wall[2], fk);
     const uint32_t col = (uint32_t)jm * g.s[2] + km;
-    // reflection bits: bit c set when the value reached component c of a
-    // vector through a wall normal to direction c (bit 0, the i plane, is
-    // added per plane)
-    const unsigned mcol = (fj ? 2u : 0u) | (fk ? 4u : 0u);
+    // reflection masks: bit c set when component c changes sign on the way
+    // the value was loaded (the i plane's reflection is added per plane)
+    const unsigned mcol = (fj ? g.odd[1] : 0u) ^ (fk ? g.odd[2] : 0u);
+    const unsigned modd_i = g.odd[0];
 
     // halo slots of this thread: LDS position, source column, signs
     int hrow[NSLOT], hcolk[NSLOT];
@@ -184,7 +184,7 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
         hrow[s] = r;
         hcolk[s] = cc;
         hsrc[s] = (uint32_t)jm2 * g.s[2] + km2;
-        hmk[s] = (f1 ? 2u : 0u) | (f2 ? 4u : 0u);
+        hmk[s] = (f1 ? g.odd[1] : 0u) ^ (f2 ? g.odd[2] : 0u);
     }
 
     // loaders
@@ -202,7 +202,7 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
         for (int f = 0; f < NF; f++)
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                q[f][c][o] = rneg<WALLS>(ldp(a.X[f], c, po, col), mcol | (fi ? 1u : 0u), c);
+                q[f][c][o] = rneg<WALLS>(ldp(a.X[f], c, po, col), mcol ^ (fi ? modd_i : 0u), c);
             }
     }
     // prefetched data of plane p: the queue entry of plane p + H, the halo of
@@ -307,7 +307,7 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                 if (hon[s])
 #pragma unroll
                     for (int n = 0; n < NC; n++)
-                        shr(sw, n, hrow[s], hcolk[s]) = rneg<WALLS>(Cur.hv[s][n], hmk[s] | (Cur.fh ? 1u : 0u), n % 3);
+                        shr(sw, n, hrow[s], hcolk[s]) = rneg<WALLS>(Cur.hv[s][n], hmk[s] ^ (Cur.fh ? modd_i : 0u), n % 3);
         } else {
             const int sw = NS == 1 ? 0 : (i - i0) & 1;
 #pragma unroll
@@ -319,7 +319,7 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                 if (hon[s])
 #pragma unroll
                     for (int n = 0; n < NC; n++)
-                        shr(sw, n, hrow[s], hcolk[s]) = rneg<WALLS>(Cur.hv[s][n], hmk[s] | (Cur.fh ? 1u : 0u), n % 3);
+                        shr(sw, n, hrow[s], hcolk[s]) = rneg<WALLS>(Cur.hv[s][n], hmk[s] ^ (Cur.fh ? modd_i : 0u), n % 3);
         }
         __syncthreads();
         load_pf(Cur, i + PF < i1 ? i + PF : i1 - 1, true);
@@ -414,7 +414,7 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
                 for (int c = 0; c < 3; c++) {
 #pragma unroll
                     for (int o = 0; o < 2 * H; o++) q[f][c][o] = q[f][c][o + 1];
-                    q[f][c][2 * H] = fresh<PF>(rneg<WALLS>(Nxt.nq[f][c], mcol | (Nxt.fq ? 1u : 0u), c));
+                    q[f][c][2 * H] = fresh<PF>(rneg<WALLS>(Nxt.nq[f][c], mcol ^ (Nxt.fq ? modd_i : 0u), c));
                     x0c[f][c] = fresh<PF>(Nxt.nx0[f][c]);
                 }
             if constexpr (ACC) {
@@ -446,7 +446,7 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
 #pragma unroll
                         for (int c = 0; c < 3; c++)
                             shr(pp, f * 3 + c, hrow[s], hcolk[s]) =
-                                rneg<WALLS>(ldp(a.X[f], c, ph, hsrc[s]), hmk[s] | (fh ? 1u : 0u), c);
+                                rneg<WALLS>(ldp(a.X[f], c, ph, hsrc[s]), hmk[s] ^ (fh ? modd_i : 0u), c);
         }
     }
     if constexpr (PF == 1) {

@@ -104,7 +104,7 @@ __device__ inline int64_t wc_node(const Grid &g, int i, int j, int k, int (&fl)[
 template <bool WALLS>
 __device__ inline double wc_vec(const Grid &g, const double *F, int c, int64_t off, const int (&fl)[3])
 {
-    return rsign<WALLS>(c, 0, fl[0]) * rsign<WALLS>(c, 1, fl[1]) * rsign<WALLS>(c, 2, fl[2]) * F[c * g.np + off];
+    return rsign<WALLS>(g, c, 0, fl[0]) * rsign<WALLS>(g, c, 1, fl[1]) * rsign<WALLS>(g, c, 2, fl[2]) * F[c * g.np + off];
 }
 
 // The thread's share of the haloed (j, k) plane of the tile: slot s holds
@@ -539,6 +539,7 @@ int wc_create(nova_ctx *ctx, double mu, double pc, nova_wc **out)
 {
     if (!ctx || !out) return fail(NOVA_ERR_ARG, "null argument");
     if (!(mu >= 0.0)) return fail(NOVA_ERR_ARG, "mu must be >= 0");
+    if (ctx->wall_fill) return fail(NOVA_ERR_UNSUPPORTED, "the weakly-compressible steps take the reflection rule at walls");
     nova_wc *w = new nova_wc;
     w->ctx = ctx;
     w->mu = mu;

@@ -259,6 +259,10 @@ class Mesh {
         nova_poisson *ps = nullptr;
         nova_io *io = nullptr;
         std::map<int, std::vector<double *>> pool;  // free device buffers by component count
+        // the reference's wall fills per walled direction, from the BC files:
+        // 0 none, 1 implied by a scalar field's file, 2 given by a vector's
+        int wallSet[3] = {0, 0, 0};
+        int wallKind[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         ~Ctx()
         {
             if (io) nova_io_destroy(io);  // waits for queued writes
@@ -320,9 +324,11 @@ public:
         zone_ = zone.empty() ? std::string(".") : zone;
         open(device);
     }
+    // zone: where the fields' <name>BC files are read
     Mesh(const std::array<int, 3> &N, const std::array<scalar, 3> &L, Time &runTime,
-         const std::array<scalar, 3> &origin = {0.0, 0.0, 0.0}, bool wallY = false, int device = 0)
-        : time_(&runTime)
+         const std::array<scalar, 3> &origin = {0.0, 0.0, 0.0}, bool wallY = false, int device = 0,
+         const std::string &zone = ".")
+        : zone_(zone), time_(&runTime)
     {
         for (int a = 0; a < 3; a++) {
             d_.n[a] = N[a];
@@ -350,8 +356,33 @@ public:
     int64_t points() const { return np_; }
     nova_poisson *poissonHandle() const
     {
-        if (!c_->ps) nova::check(nova_poisson_create(c_->ctx, &c_->ps), "nova_poisson_create");
+        if (!c_->ps) {
+            nova::check(nova_poisson_create(c_->ctx, &c_->ps), "nova_poisson_create");
+            if (walled(1) && c_->wallSet[1])  // the reference's FFT solve across the walls
+                nova::check(nova_poisson_set_wall_solve(c_->ps, NOVA_WALL_SOLVE_REFERENCE), "poisson walls");
+        }
         return c_->ps;
+    }
+    // The reference's fills on the walls normal to a (dirichletBC /
+    // neumannBC with value 0, per vector component), installed by the BC
+    // files: every vector field of the mesh shares them, since the device
+    // reflects a component the same way in every stencil.  A scalar field's
+    // file (neumann 0) installs the fills with the wall-normal component
+    // odd until a vector field's file names the components.
+    void setWallBC(int a, const std::array<int, 3> &kind, bool fromVector, const std::string &file)
+    {
+        if (fromVector && c_->wallSet[a] == 2) {
+            for (int c = 0; c < 3; c++)
+                if (c_->wallKind[a][c] != kind[c])
+                    throw nova::Error(file + ": the vector fields of one mesh must share their wall conditions");
+            return;
+        }
+        if (!fromVector && c_->wallSet[a]) return;
+        nova::check(nova_ctx_set_wall_bc(c_->ctx, a, kind.data()), "nova_ctx_set_wall_bc");
+        for (int c = 0; c < 3; c++) c_->wallKind[a][c] = kind[c];
+        c_->wallSet[a] = fromVector ? 2 : 1;
+        if (c_->ps && a == 1)
+            nova::check(nova_poisson_set_wall_solve(c_->ps, NOVA_WALL_SOLVE_REFERENCE), "poisson walls");
     }
     // pool-owned device buffers (the reference's reuseTmp, reuseTmp.hxx:19-50)
     double *take(int nc) const
@@ -444,14 +475,14 @@ public:
     }
     // Field::setBC (src/Field/Field.hxx:519-547): the "dir type val comp"
     // lines of <zone>/<file> (a missing file leaves the field as it is).
-    // The device applies the grid's conditions when it loads a stencil:
-    // periodic directions wrap, walled ones reflect about the wall node with
-    // the wall-normal component of a vector odd and everything else even.
-    // An entry is accepted when it states that: "periodic" on a periodic
-    // direction, "dirichlet 0" for the wall-normal component, "neumann 0"
-    // for the others (neumannBC.hxx:99-102 would also overwrite the wall
-    // node with its neighbour; the reflection keeps the wall node a solved
-    // node).  Anything else cannot be represented and throws.
+    // Periodic directions take "periodic".  On a walled direction every
+    // component takes "dirichlet 0" or "neumann 0" (dirichletBC.hxx:90-125,
+    // neumannBC.hxx:100-135): the stencils reflect the component oddly or
+    // evenly about the wall node and correctBoundaryConditions() overwrites
+    // the wall node as those fills do; scalar fields take "neumann 0".
+    // Anything else (non-zero values, a periodic entry on a walled direction
+    // or the reverse, a vector missing a component's entry) cannot be
+    // represented and throws.
     void setBC(const std::string &file)
     {
         std::ifstream fin(m_.zone() + "/" + file);
@@ -459,6 +490,7 @@ public:
         std::string line;
         static const std::map<std::string, int> axis = {{"east", 0}, {"west", 0}, {"north", 1},
                                                         {"south", 1}, {"top", 2}, {"bottom", 2}};
+        int kind[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         while (std::getline(fin, line)) {
             std::stringstream sp(line);
             std::string dir, type;
@@ -469,11 +501,30 @@ public:
             if (it == axis.end()) throw nova::Error(file + ": unknown direction " + dir);
             const int a = it->second;
             if (comp < 0 || comp >= NC) throw nova::Error(file + ": component " + std::to_string(comp) + " out of range");
-            std::string want = "periodic";
-            if (m_.walled(a)) want = (NC == 3 && comp == a) ? "dirichlet" : "neumann";
-            if (type != want || val != 0.0)
-                throw nova::Error(file + ": '" + line + "' is not the " + (m_.walled(a) ? "walled" : "periodic") +
-                                  " condition the device applies there (" + want + " 0)");
+            const bool ok = m_.walled(a) ? ((type == "neumann" || (NC == 3 && type == "dirichlet")) && val == 0.0)
+                                         : type == "periodic";
+            if (!ok)
+                throw nova::Error(file + ": '" + line + "' is not a condition the device applies on a " +
+                                  (m_.walled(a) ? "walled direction (dirichlet 0 or neumann 0 per vector component, "
+                                                  "neumann 0 for scalars)"
+                                                : "periodic direction (periodic)"));
+            if (m_.walled(a)) {
+                const int k = type == "dirichlet" ? NOVA_WALL_DIRICHLET0 : NOVA_WALL_NEUMANN0;
+                if (kind[a][comp] && kind[a][comp] != k)
+                    throw nova::Error(file + ": the two walls of a direction must agree (" + line + ")");
+                kind[a][comp] = k;
+            }
+        }
+        for (int a = 0; a < 3; a++) {
+            if (!m_.walled(a) || !kind[a][0]) continue;
+            if (NC == 3) {
+                if (!kind[a][1] || !kind[a][2]) throw nova::Error(file + ": a wall entry for every component is needed");
+                m_.setWallBC(a, {kind[a][0], kind[a][1], kind[a][2]}, true, file);
+            } else {
+                std::array<int, 3> k = {NOVA_WALL_NEUMANN0, NOVA_WALL_NEUMANN0, NOVA_WALL_NEUMANN0};
+                k[a] = NOVA_WALL_DIRICHLET0;
+                m_.setWallBC(a, k, false, file);
+            }
         }
     }
     Field(Mesh &m) : Field(m, T(0.0)) {}
@@ -518,11 +569,14 @@ public:
         return d_;
     }
     // ghosts are never stored on the device: every kernel applies the BC
-    // on load, and the host mirror is filled with them on transfer
+    // on load, and the host mirror is filled with them on transfer; with the
+    // reference's wall fills (installed by BC files) the wall nodes are
+    // overwritten here, as the fills do
     void correctBoundaryConditions()
     {
         if (hostDirty_) syncDevice();
         hostValid_ = false;
+        if (NC == 1 || NC == 3) nova::check(nova_wall_fill(m_.ctx(), d_, NC), "correctBoundaryConditions");
     }
 
     // Field::write / read (Field.hxx:403-522): EnSight Gold float32 files

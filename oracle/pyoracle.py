@@ -377,6 +377,55 @@ class OPoisson:
         lib().orc_mhd_step(self.mesh._h, self._h, _d(U), _d(B), _d(p), _d(pB), nu, eta, dt, mode)
 
 
+BC_NAMES = {"north": NORTH, "south": SOUTH, "east": EAST, "west": WEST, "top": TOP, "bottom": BOTTOM}
+BC_KINDS = {"periodic": BC_PERIODIC, "neumann": BC_NEUMANN, "dirichlet": BC_DIRICHLET}
+
+
+def bc_entries(text: str, ncomp: int):
+    """The "dir type val comp" lines of a BC file (src/Field/Field.hxx:519-547)
+    as bc_apply entries in the order Field::correctBoundaryConditions
+    applies them (Field.hxx:310-322: bc_[dir + 6 comp], index order)."""
+    table = {}
+    for line in text.splitlines():
+        f = line.split()
+        if len(f) < 4:
+            continue
+        d, kind, val, comp = BC_NAMES[f[0]], BC_KINDS[f[1]], float(f[2]), int(f[3])
+        table[d + 6 * comp] = (d, kind, comp, val)
+    return [table[i] for i in sorted(table) if table[i][2] < ncomp]
+
+
+SHEAR_RK4 = (1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0)  # app/shearLayer/shearLayer.cpp:145 (RK4)
+RK4C = (0.5, 0.5, 1.0, 1.0)                               # src/settings/settings.cpp:51 (RK4c)
+
+
+def shear_layer_step(M: "OMesh", P: "OPoisson", U, B, p, mu, dt, ubc, pbc):
+    """One time step of app/shearLayer (shearLayer.cpp:159-185; TEqn and
+    BEqn switched off, lines 178-179) with UEqn.H:1-54 as written, on the
+    padded fields of the walled mesh M, in place: per stage rk
+        dU = dt (((-div(U, U) + lap(mu, U)) + div(B, B)) - grad(0.5 (B & B)))
+        U* = UOld + RK4[rk] dU;  U*.correctBoundaryConditions()
+        p = solve(div(U*) / dt / RK4c[rk]);  p.correctBoundaryConditions()
+        U = U* - (RK4c[rk] dt) grad(p);  U.correctBoundaryConditions()
+    with the literal fills of the BC entries ubc, pbc (bc_entries) and the
+    reference's FFT solve (poisson.cpp:167-265), which takes nodes
+    0 .. N - 1 of every direction: P is an OPoisson on the periodic mesh of
+    the same cells (the padded layouts agree), p's wall node is left to
+    its fill.  B keeps its values (BEqn off)."""
+    UOld = U.copy()
+    BB = (B[..., 0] * B[..., 0] + B[..., 1] * B[..., 1]) + B[..., 2] * B[..., 2]  # vector.cpp:120-127
+    half = np.ascontiguousarray((0.5 * BB)[..., None])
+    for rk in range(4):
+        dU = dt * ((((-M.div_adv(U, U)) + M.laplacian(mu, U)) + M.div_adv(B, B)) - M.grad(half))
+        Ustar = UOld + SHEAR_RK4[rk] * dU
+        M.bc_apply(Ustar, ubc)
+        rhs = np.ascontiguousarray(M.div(Ustar) / dt / RK4C[rk])
+        P.solve(rhs, p)
+        M.bc_apply(p, pbc)
+        U[...] = Ustar - (RK4C[rk] * dt) * M.grad(p)
+        M.bc_apply(U, ubc)
+
+
 def coef(m):
     out = np.zeros(4)
     lib().orc_coef(m, _d(out))

@@ -196,3 +196,39 @@ def test_mhdjet_app_hdf5_dumps_match_oracle(orc, tmp_path):
             e = np.linalg.norm(got - exp) / scale[name[0]]
             print(name, t, e, flush=True)
             assert got.shape == N and e <= 1e-12, (name, t, e)
+
+
+def test_shear_layer_app_matches_oracle(orc, tmp_path):
+    """app/shearLayer as written, from a zone holding the example's mesh and
+    BC files (examples/shearLayer), through the drop-in API: Field::setBC
+    installs the reference's wall fills, poisson takes the reference's FFT
+    solve, UEqn.H runs operator by operator; U and p after three steps
+    against the oracle's restatement, 1e-11 relative L2."""
+    from test_gpu_shear_literal import BBC, DT, L, MU, N, PBC, UBC, initial_state
+    exe = ROOT / "novaclone_amd" / "host" / "bin" / "shear_layer"
+    (tmp_path / "mesh").write_text(f"origin: 0 0 0\nN: {N[0]} {N[1]} {N[2]} \nL: {L[0]} {L[1]} {L[2]}\n")
+    (tmp_path / "UBC").write_text(UBC)
+    (tmp_path / "pBC").write_text(PBC)
+    (tmp_path / "BBC").write_text(BBC)
+    steps = 3
+    out = tmp_path / "state.bin"
+    print("running", exe, flush=True)
+    r = subprocess.run([str(exe), str(tmp_path), str(steps), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "CFL x:" in r.stdout and "Velocity divergence:" in r.stdout
+    s = (N[0], N[1] + 1, N[2])
+    got = np.fromfile(out, dtype=np.float64).reshape(4, *s)
+    O = orc.OMesh(N, L, wall_y=True)
+    P = orc.OPoisson(orc.OMesh(N, L))
+    ubc, pbc, bbc = orc.bc_entries(UBC, 3), orc.bc_entries(PBC, 1), orc.bc_entries(BBC, 3)
+    U, B, p = initial_state(O)
+    O.bc_apply(U, ubc)
+    O.bc_apply(p, pbc)
+    O.bc_apply(B, bbc)
+    for _ in range(steps):
+        orc.shear_layer_step(O, P, U, B, p, MU, DT, ubc, pbc)
+    u, q = O.interior(U), O.interior(p)
+    eu = np.linalg.norm(got[:3] - u) / np.linalg.norm(u)
+    ep = np.linalg.norm(got[3] - q[0]) / np.linalg.norm(q[0])
+    print("shear_layer app: rel L2 U", eu, "p", ep, flush=True)
+    assert eu <= 1e-11 and ep <= 1e-11
