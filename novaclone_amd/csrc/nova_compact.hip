@@ -40,6 +40,7 @@ struct CmpScheme {
     double ca, cb;          // right-hand-side weights, divided by kappa
     double rp[9];           // (-r)^q, q = 0..8
     double rho2[kScanMax];  // rho^(2^s), rho = (-r)^MP
+    int dsh[kScanMax];      // lane distance of scan step s: 2^s mod P
     int K;                  // scan steps
 };
 
@@ -88,6 +89,7 @@ CmpDir make_dir(int n, double h, int mp)
         S.K = 0;
         for (int s = 0; s < kScanMax; s++) {
             S.rho2[s] = rr;
+            S.dsh[s] = (1 << s) % D.P;
             if (std::fabs(rr) > 1e-20) S.K = s + 1;
             rr *= rr;
         }
@@ -97,7 +99,11 @@ CmpDir make_dir(int n, double h, int mp)
 
 // ---- wavefront line algebra ----
 
-template <int MP>
+// UR: scan steps unrolled to kScanMax, each taken when the scheme needs it
+// (a uniform branch); fewer scalar instructions, more registers (the k
+// passes and the MHD passes take it, whose occupancy the registers do not
+// set lower)
+template <int MP, bool UR = false>
 __device__ inline void cmp_solve(const CmpScheme &S, int P, int lane, double (&x)[MP])
 {
     const double r = S.r;
@@ -106,12 +112,43 @@ __device__ inline void cmp_solve(const CmpScheme &S, int P, int lane, double (&x
 #pragma unroll
     for (int j = 1; j < MP; j++) w[j] = x[j] - r * w[j - 1];
     double acc = w[MP - 1];
-    for (int s = 0; s < S.K; s++) {
-        const int d = (1 << s) % P;
-        const int src = lane - d < 0 ? lane - d + P : lane - d;
-        const double o = __shfl(acc, src, 64);
-        acc = acc + S.rho2[s] * o;
-    }
+    // the scan's lane distances come precomputed (a modulo by the lane
+    // count on every step cost tens of scalar instructions)
+    auto scan_dn = [&](double &v) {
+        if constexpr (UR) {
+#pragma unroll
+            for (int s = 0; s < kScanMax; s++) {
+                if (s >= S.K) break;
+                const int d = S.dsh[s];
+                const int src = lane - d < 0 ? lane - d + P : lane - d;
+                v = v + S.rho2[s] * __shfl(v, src, 64);
+            }
+        } else {
+            for (int s = 0; s < S.K; s++) {
+                const int d = S.dsh[s];
+                const int src = lane - d < 0 ? lane - d + P : lane - d;
+                v = v + S.rho2[s] * __shfl(v, src, 64);
+            }
+        }
+    };
+    auto scan_up = [&](double &v) {
+        if constexpr (UR) {
+#pragma unroll
+            for (int s = 0; s < kScanMax; s++) {
+                if (s >= S.K) break;
+                const int d = S.dsh[s];
+                const int src = lane + d >= P ? lane + d - P : lane + d;
+                v = v + S.rho2[s] * __shfl(v, src, 64);
+            }
+        } else {
+            for (int s = 0; s < S.K; s++) {
+                const int d = S.dsh[s];
+                const int src = lane + d >= P ? lane + d - P : lane + d;
+                v = v + S.rho2[s] * __shfl(v, src, 64);
+            }
+        }
+    };
+    scan_dn(acc);
     const double cin = __shfl(acc, lane == 0 ? P - 1 : lane - 1, 64);
 #pragma unroll
     for (int j = 0; j < MP; j++) w[j] = w[j] + S.rp[j + 1] * cin;
@@ -119,12 +156,7 @@ __device__ inline void cmp_solve(const CmpScheme &S, int P, int lane, double (&x
 #pragma unroll
     for (int j = MP - 2; j >= 0; j--) x[j] = w[j] - r * x[j + 1];
     acc = x[0];
-    for (int s = 0; s < S.K; s++) {
-        const int d = (1 << s) % P;
-        const int src = lane + d >= P ? lane + d - P : lane + d;
-        const double o = __shfl(acc, src, 64);
-        acc = acc + S.rho2[s] * o;
-    }
+    scan_up(acc);
     const double cin2 = __shfl(acc, lane + 1 >= P ? lane + 1 - P : lane + 1, 64);
 #pragma unroll
     for (int j = 0; j < MP; j++) x[j] = x[j] + S.rp[MP - j] * cin2;
@@ -162,11 +194,74 @@ __device__ inline void cmp_rhs(const CmpScheme &S, int P, int lane, const double
     }
 }
 
-template <int MP, int ORDER>
+template <int MP, int ORDER, bool UR = false>
 __device__ inline void cmp_deriv(const CmpDir &D, int lane, const double (&f)[MP], double (&out)[MP])
 {
     cmp_rhs<MP, ORDER>(D.s[ORDER - 1], D.P, lane, f, out);
-    cmp_solve<MP>(D.s[ORDER - 1], D.P, lane, out);
+    cmp_solve<MP, UR>(D.s[ORDER - 1], D.P, lane, out);
+}
+
+// NS systems of one scheme solved together: the same operations per system
+// as cmp_solve (so the same results bit for bit), with every step of the
+// cross-lane scans issued for all systems before any of them is used, so
+// that NS shuffles are in flight at once instead of one dependent chain per
+// system.
+template <int MP, int NS>
+__device__ inline void cmp_solve_n(const CmpScheme &S, int P, int lane, double (&x)[NS][MP])
+{
+    const double r = S.r;
+    double w[NS][MP], acc[NS], o[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        w[s][0] = x[s][0];
+#pragma unroll
+        for (int j = 1; j < MP; j++) w[s][j] = x[s][j] - r * w[s][j - 1];
+        acc[s] = w[s][MP - 1];
+    }
+    for (int st = 0; st < S.K; st++) {
+        const int d = S.dsh[st];
+        const int src = lane - d < 0 ? lane - d + P : lane - d;
+#pragma unroll
+        for (int s = 0; s < NS; s++) o[s] = __shfl(acc[s], src, 64);
+#pragma unroll
+        for (int s = 0; s < NS; s++) acc[s] = acc[s] + S.rho2[st] * o[s];
+    }
+    const int lm1 = lane == 0 ? P - 1 : lane - 1;
+#pragma unroll
+    for (int s = 0; s < NS; s++) o[s] = __shfl(acc[s], lm1, 64);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+#pragma unroll
+        for (int j = 0; j < MP; j++) w[s][j] = w[s][j] + S.rp[j + 1] * o[s];
+        x[s][MP - 1] = w[s][MP - 1];
+#pragma unroll
+        for (int j = MP - 2; j >= 0; j--) x[s][j] = w[s][j] - r * x[s][j + 1];
+        acc[s] = x[s][0];
+    }
+    for (int st = 0; st < S.K; st++) {
+        const int d = S.dsh[st];
+        const int src = lane + d >= P ? lane + d - P : lane + d;
+#pragma unroll
+        for (int s = 0; s < NS; s++) o[s] = __shfl(acc[s], src, 64);
+#pragma unroll
+        for (int s = 0; s < NS; s++) acc[s] = acc[s] + S.rho2[st] * o[s];
+    }
+    const int lp1 = lane + 1 >= P ? lane + 1 - P : lane + 1;
+#pragma unroll
+    for (int s = 0; s < NS; s++) o[s] = __shfl(acc[s], lp1, 64);
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int j = 0; j < MP; j++) x[s][j] = x[s][j] + S.rp[MP - j] * o[s];
+}
+
+// NS compact derivatives of one order, f[s] -> out[s]
+template <int MP, int ORDER, int NS>
+__device__ inline void cmp_deriv_n(const CmpDir &D, int lane, const double (&f)[NS][MP], double (&out)[NS][MP])
+{
+#pragma unroll
+    for (int s = 0; s < NS; s++) cmp_rhs<MP, ORDER>(D.s[ORDER - 1], D.P, lane, f[s], out[s]);
+    cmp_solve_n<MP, NS>(D.s[ORDER - 1], D.P, lane, out);
 }
 
 // ---- pass policies: inputs differentiated along the pass direction and
@@ -189,6 +284,10 @@ struct DerivW {
         else cmp_deriv<MP, 2>(D, lane, in[0], out[0]);
     }
     __device__ static void store(const Args &a, const Grid &, int, int64_t q, double v, double) { a.out[q] = v; }
+    // the i / j passes store in two phases: every operand a store reads is
+    // loaded first, then every result combined and written
+    __device__ static double operand(const Args &, const Grid &, int, int64_t) { return 0.0; }
+    __device__ static void store_op(const Args &a, const Grid &, int, int64_t q, double v, double) { a.out[q] = v; }
     __device__ static void store_point(const Args &a, const Grid &g, int64_t q, const double *v, const double *x)
     {
         store(a, g, 0, q, v[0], x[0]);
@@ -214,25 +313,48 @@ struct NsW {
     __device__ static void compute(const Args &a, const CmpDir &D, int lane, double (&u)[NI][MP],
                                    double (&out)[NO][MP])
     {
-        double dud[MP];
-        cmp_deriv<MP, 1>(D, lane, u[DIR], dud);
+        if constexpr (DIR == 2) {
+            // k lines (no LDS tile, occupancy set by registers): one system
+            // at a time, fewer registers live
+            double dud[MP];
+            cmp_deriv<MP, 1, true>(D, lane, u[DIR], dud);
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            double duc[MP], fl[MP], dfl[MP], l2[MP];
-            if (c == DIR) {
+            for (int c = 0; c < 3; c++) {
+                double duc[MP], fl[MP], dfl[MP], l2[MP];
+                if (c == DIR) {
 #pragma unroll
-                for (int j = 0; j < MP; j++) duc[j] = dud[j];
-            } else {
-                cmp_deriv<MP, 1>(D, lane, u[c], duc);
+                    for (int j = 0; j < MP; j++) duc[j] = dud[j];
+                } else {
+                    cmp_deriv<MP, 1, true>(D, lane, u[c], duc);
+                }
+#pragma unroll
+                for (int j = 0; j < MP; j++) fl[j] = u[DIR][j] * u[c][j];
+                cmp_deriv<MP, 1, true>(D, lane, fl, dfl);
+                cmp_deriv<MP, 2, true>(D, lane, u[c], l2);
+#pragma unroll
+                for (int j = 0; j < MP; j++)
+                    out[c][j] = -0.5 * ((dfl[j] + u[DIR][j] * duc[j]) + u[c][j] * dud[j]) + a.nu * l2[j];
             }
+            return;
+        }
+        // i and j lines (occupancy set by the LDS tile): the six
+        // first-derivative systems (D U_c, D (U_d U_c)) together, then the
+        // three second-derivative ones, the same operations in the same order
+        double f1[6][MP], d1[6][MP], d2[3][MP];
 #pragma unroll
-            for (int j = 0; j < MP; j++) fl[j] = u[DIR][j] * u[c][j];
-            cmp_deriv<MP, 1>(D, lane, fl, dfl);
-            cmp_deriv<MP, 2>(D, lane, u[c], l2);
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int j = 0; j < MP; j++) {
+                f1[c][j] = u[c][j];
+                f1[3 + c][j] = u[DIR][j] * u[c][j];
+            }
+        cmp_deriv_n<MP, 1, 6>(D, lane, f1, d1);
+        cmp_deriv_n<MP, 2, 3>(D, lane, u, d2);
+#pragma unroll
+        for (int c = 0; c < 3; c++)
 #pragma unroll
             for (int j = 0; j < MP; j++)
-                out[c][j] = -0.5 * ((dfl[j] + u[DIR][j] * duc[j]) + u[c][j] * dud[j]) + a.nu * l2[j];
-        }
+                out[c][j] = -0.5 * ((d1[3 + c][j] + u[DIR][j] * d1[c][j]) + u[c][j] * d1[DIR][j]) + a.nu * d2[c][j];
     }
     __device__ static void store(const Args &a, const Grid &g, int o, int64_t q, double v, double x)
     {
@@ -246,6 +368,14 @@ struct NsW {
             const double x0 = a.U0 ? a.U0[e] : x;
             a.out[e] = combine(a.sa, x, x0, du);
         }
+    }
+    __device__ static double operand(const Args &a, const Grid &g, int o, int64_t q)
+    {
+        return a.mode == 1 ? a.R[o * g.np + q] : 0.0;
+    }
+    __device__ static void store_op(const Args &a, const Grid &g, int o, int64_t q, double v, double r)
+    {
+        a.R[o * g.np + q] = a.mode == 1 ? r + v : v;  // the i and j passes: modes 0 and 1
     }    __device__ static void store_point(const Args &a, const Grid &g, int64_t q, const double *v, const double *x)
     {
 #pragma unroll
@@ -276,6 +406,13 @@ struct DivW {
         else if (a.mode == 1) a.d[q] = a.d[q] + v;
         else a.d[q] = (a.d[q] + v) * a.scale;
     }
+    __device__ static double operand(const Args &a, const Grid &, int, int64_t q) { return a.mode ? a.d[q] : 0.0; }
+    __device__ static void store_op(const Args &a, const Grid &, int, int64_t q, double v, double d)
+    {
+        if (a.mode == 0) a.d[q] = v;
+        else if (a.mode == 1) a.d[q] = d + v;
+        else a.d[q] = (d + v) * a.scale;
+    }
     __device__ static void store_point(const Args &a, const Grid &g, int64_t q, const double *v, const double *x)
     {
         store(a, g, 0, q, v[0], x[0]);
@@ -300,6 +437,11 @@ struct GradW {
     __device__ static void store(const Args &a, const Grid &, int, int64_t q, double v, double)
     {
         a.F[q] = a.F[q] - a.s * v;
+    }
+    __device__ static double operand(const Args &a, const Grid &, int, int64_t q) { return a.F[q]; }
+    __device__ static void store_op(const Args &a, const Grid &, int, int64_t q, double v, double f)
+    {
+        a.F[q] = f - a.s * v;
     }
     __device__ static void store_point(const Args &a, const Grid &g, int64_t q, const double *v, const double *x)
     {
@@ -369,6 +511,14 @@ struct TlmW {
             const double x0 = a.V0 ? a.V0[e] : x;
             a.out[e] = combine(a.sa, x, x0, du);
         }
+    }
+    __device__ static double operand(const Args &a, const Grid &g, int o, int64_t q)
+    {
+        return a.mode == 1 ? a.R[o * g.np + q] : 0.0;
+    }
+    __device__ static void store_op(const Args &a, const Grid &g, int o, int64_t q, double v, double r)
+    {
+        a.R[o * g.np + q] = a.mode == 1 ? r + v : v;
     }    __device__ static void store_point(const Args &a, const Grid &g, int64_t q, const double *v, const double *x)
     {
 #pragma unroll
@@ -440,6 +590,14 @@ struct AdjW {
         if (a.mode == 0) a.R[e] = v;
         else a.R[e] = a.R[e] + v;
     }
+    __device__ static double operand(const Args &a, const Grid &g, int o, int64_t q)
+    {
+        return a.mode ? a.R[o * g.np + q] : 0.0;
+    }
+    __device__ static void store_op(const Args &a, const Grid &g, int o, int64_t q, double v, double r)
+    {
+        a.R[o * g.np + q] = a.mode ? r + v : v;
+    }
     __device__ static void store_point(const Args &a, const Grid &g, int64_t q, const double *v, const double *x)
     {
         if (a.mode < 2) {
@@ -487,8 +645,8 @@ struct MhdW {
         double (&u)[3][MP] = *reinterpret_cast<double(*)[3][MP]>(&in[0][0]);
         double (&b)[3][MP] = *reinterpret_cast<double(*)[3][MP]>(&in[3][0]);
         double dud[MP], dbd[MP];
-        cmp_deriv<MP, 1>(D, lane, u[DIR], dud);
-        cmp_deriv<MP, 1>(D, lane, b[DIR], dbd);
+        cmp_deriv<MP, 1, true>(D, lane, u[DIR], dud);
+        cmp_deriv<MP, 1, true>(D, lane, b[DIR], dbd);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             double duc[MP], dbc[MP], f[MP], d1[MP], d2[MP], d3[MP], d4[MP], l2[MP];
@@ -499,33 +657,33 @@ struct MhdW {
                     dbc[j] = dbd[j];
                 }
             } else {
-                cmp_deriv<MP, 1>(D, lane, u[c], duc);
-                cmp_deriv<MP, 1>(D, lane, b[c], dbc);
+                cmp_deriv<MP, 1, true>(D, lane, u[c], duc);
+                cmp_deriv<MP, 1, true>(D, lane, b[c], dbc);
             }
 #pragma unroll
             for (int j = 0; j < MP; j++) f[j] = u[DIR][j] * u[c][j];
-            cmp_deriv<MP, 1>(D, lane, f, d1);
+            cmp_deriv<MP, 1, true>(D, lane, f, d1);
 #pragma unroll
             for (int j = 0; j < MP; j++) f[j] = b[DIR][j] * b[c][j];
-            cmp_deriv<MP, 1>(D, lane, f, d2);
+            cmp_deriv<MP, 1, true>(D, lane, f, d2);
 #pragma unroll
             for (int j = 0; j < MP; j++) f[j] = u[DIR][j] * b[c][j];
-            cmp_deriv<MP, 1>(D, lane, f, d3);
+            cmp_deriv<MP, 1, true>(D, lane, f, d3);
             if (c == DIR) {
 #pragma unroll
                 for (int j = 0; j < MP; j++) d4[j] = d3[j];  // b_d u_d = u_d b_d
             } else {
 #pragma unroll
                 for (int j = 0; j < MP; j++) f[j] = b[DIR][j] * u[c][j];
-                cmp_deriv<MP, 1>(D, lane, f, d4);
+                cmp_deriv<MP, 1, true>(D, lane, f, d4);
             }
-            cmp_deriv<MP, 2>(D, lane, u[c], l2);
+            cmp_deriv<MP, 2, true>(D, lane, u[c], l2);
 #pragma unroll
             for (int j = 0; j < MP; j++)
                 out[c][j] = ((-0.5 * ((d1[j] + u[DIR][j] * duc[j]) + u[c][j] * dud[j])) +
                              0.5 * ((d2[j] + b[DIR][j] * dbc[j]) + b[c][j] * dbd[j])) +
                             a.nu * l2[j];
-            cmp_deriv<MP, 2>(D, lane, b[c], l2);
+            cmp_deriv<MP, 2, true>(D, lane, b[c], l2);
 #pragma unroll
             for (int j = 0; j < MP; j++)
                 out[3 + c][j] = ((-0.5 * ((d3[j] + u[DIR][j] * dbc[j]) + b[c][j] * dud[j])) +
@@ -535,7 +693,7 @@ struct MhdW {
         double pm[MP], dpm[MP];
 #pragma unroll
         for (int j = 0; j < MP; j++) pm[j] = 0.5 * ((b[0][j] * b[0][j] + b[1][j] * b[1][j]) + b[2][j] * b[2][j]);
-        cmp_deriv<MP, 1>(D, lane, pm, dpm);
+        cmp_deriv<MP, 1, true>(D, lane, pm, dpm);
 #pragma unroll
         for (int j = 0; j < MP; j++) out[DIR][j] = out[DIR][j] - dpm[j];
     }
@@ -544,6 +702,14 @@ struct MhdW {
         const int64_t e = o * g.np + q;
         if (a.mode == 0) a.R[e] = v;
         else a.R[e] = a.R[e] + v;
+    }
+    __device__ static double operand(const Args &a, const Grid &g, int o, int64_t q)
+    {
+        return a.mode ? a.R[o * g.np + q] : 0.0;
+    }
+    __device__ static void store_op(const Args &a, const Grid &g, int o, int64_t q, double v, double r)
+    {
+        a.R[o * g.np + q] = a.mode ? r + v : v;
     }
     __device__ static void store_point(const Args &a, const Grid &g, int64_t q, const double *v, const double *x)
     {
@@ -565,12 +731,23 @@ struct MhdW {
     }
 };
 
-// k lines per LDS tile: 48 KB of LDS at most
+// k lines per LDS tile and threads per block: single-field passes
+// (derivative, divergence, gradient correction; few registers) at up to
+// 256 points a line take 16 consecutive k (128-byte rows, 32 KB) on 512
+// threads; the model passes 48 KB at most on 256 threads, since their
+// registers leave three waves per SIMD (measured at 256^3: the NS passes
+// ran 454 / 523 us with 8 k on 256 threads and 589 / 742 us with 16 on 512,
+// the divergence and correction 115 / 115 and 94 / 88 us)
 template <int MP, class W>
 constexpr int tile_lines()
 {
     constexpr int nb = W::NI > W::NO ? W::NI : W::NO;
-    return (nb <= 3 ? 8 : 4) / (MP <= 4 ? 1 : 2);
+    return nb == 1 && MP <= 4 ? 16 : (nb <= 3 ? 8 : 4) / (MP <= 4 ? 1 : 2);
+}
+template <int MP, class W>
+constexpr int tile_threads()
+{
+    return tile_lines<MP, W>() == 16 ? 512 : 256;
 }
 
 // Lines along k: one wavefront per (i, j) line, no LDS.
@@ -605,9 +782,10 @@ __global__ void __launch_bounds__(256) k_cmp_pass_k(Grid g, CmpDir D, typename W
 // Lines along i (DIR 0) or j (DIR 1): a block stages TL consecutive k
 // lines through LDS; each wavefront solves TL / 4 of them.
 template <int DIR, int MP, class W>
-__global__ void __launch_bounds__(256) k_cmp_pass_t(Grid g, CmpDir D, typename W::Args a)
+__global__ void __launch_bounds__((tile_threads<MP, W>())) k_cmp_pass_t(Grid g, CmpDir D, typename W::Args a)
 {
     constexpr int TL = tile_lines<MP, W>();
+    constexpr int NT = tile_threads<MP, W>(), NW = NT / 64;
     constexpr int NB = W::NI > W::NO ? W::NI : W::NO;
     constexpr int NMAX = 64 * MP;
     __shared__ double sh[NB * TL * NMAX];
@@ -617,35 +795,101 @@ __global__ void __launch_bounds__(256) k_cmp_pass_t(Grid g, CmpDir D, typename W
     const int k0 = (blockIdx.x % nkt) * TL;
     const int64_t lstride = DIR == 0 ? (int64_t)g.s[1] * g.s[2] : g.s[2];  // step along the line
     const int64_t base = (DIR == 0 ? (int64_t)fixed * g.s[2] : (int64_t)fixed * g.s[1] * g.s[2]) + k0;
-    auto slot = [&](int c, int kk, int q) { return (c * TL + kk) * NMAX + (q % MP) * P + q / MP; };
-    for (int e = threadIdx.x; e < W::NI * N * TL; e += 256) {
-        const int c = e / (N * TL), rem = e % (N * TL);
-        const int q = rem / TL, kk = rem % TL;
-        double v = 0.0;
-        if (k0 + kk < g.s[2]) v = W::input(a, g, c)[base + q * lstride + kk];
-        sh[slot(c, kk, q)] = v;
+    // element (c, kk, q) sits in row (c TL + kk) MP + q % MP of P doubles at
+    // column q / MP rotated by rot(row): the solve reads whole rows (any
+    // rotation keeps that free of bank conflicts), and the staging copies,
+    // whose lanes run over kk and q, hit distinct banks
+    auto rot = [&](int row) { return (2 * row + ((row >> 5) & 1)) % P; };
+    auto col = [&](int row, int c0) {
+        const int v = c0 + rot(row);
+        return v >= P ? v - P : v;
+    };
+    auto slot = [&](int c, int kk, int q) {
+        const int row = (c * TL + kk) * MP + q % MP;
+        return (c * TL + kk) * NMAX + (q % MP) * P + col(row, q / MP);
+    };
+    // one-field passes: a lane's loads go out in groups of GS, all in
+    // flight before the group's LDS writes (a loop that writes each value as
+    // it lands keeps one load in flight); the two-field models, whose
+    // registers are spoken for, keep the plain loop
+    constexpr int GS = 8;
+    constexpr bool GROUPED = W::NI == 1;
+    if constexpr (GROUPED) {
+        constexpr int PIN = (W::NI * NMAX * TL + NT - 1) / NT;
+#pragma unroll
+        for (int r0 = 0; r0 < PIN; r0 += GS) {
+            double v[GS];
+#pragma unroll
+            for (int u = 0; u < GS; u++) {
+                const int e = threadIdx.x + (r0 + u) * NT;
+                const int rem = e % (N * TL), q = rem / TL, kk = rem % TL;
+                v[u] = 0.0;
+                if (r0 + u < PIN && e < W::NI * N * TL && k0 + kk < g.s[2])
+                    v[u] = W::input(a, g, e / (N * TL))[base + q * lstride + kk];
+            }
+#pragma unroll
+            for (int u = 0; u < GS; u++) {
+                const int e = threadIdx.x + (r0 + u) * NT;
+                const int rem = e % (N * TL);
+                if (r0 + u < PIN && e < W::NI * N * TL) sh[slot(e / (N * TL), rem % TL, rem / TL)] = v[u];
+            }
+        }
+    } else {
+        for (int e = threadIdx.x; e < W::NI * N * TL; e += NT) {
+            const int c = e / (N * TL), rem = e % (N * TL);
+            const int q = rem / TL, kk = rem % TL;
+            double v = 0.0;
+            if (k0 + kk < g.s[2]) v = W::input(a, g, c)[base + q * lstride + kk];
+            sh[slot(c, kk, q)] = v;
+        }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const bool on = lane < P;
-    for (int kk = wave; kk < TL; kk += 4) {
+    for (int kk = wave; kk < TL; kk += NW) {
         double in[W::NI][MP], out[W::NO][MP];
 #pragma unroll
         for (int c = 0; c < W::NI; c++)
 #pragma unroll
-            for (int j = 0; j < MP; j++) in[c][j] = on ? sh[(c * TL + kk) * NMAX + j * P + lane] : 0.0;
+            for (int j = 0; j < MP; j++)
+                in[c][j] = on ? sh[(c * TL + kk) * NMAX + j * P + col((c * TL + kk) * MP + j, on ? lane : 0)] : 0.0;
         W::template compute<MP, DIR>(a, D, lane, in, out);
         if (on)
 #pragma unroll
             for (int o = 0; o < W::NO; o++)
 #pragma unroll
-                for (int j = 0; j < MP; j++) sh[(o * TL + kk) * NMAX + j * P + lane] = out[o][j];
+                for (int j = 0; j < MP; j++) sh[(o * TL + kk) * NMAX + j * P + col((o * TL + kk) * MP + j, lane)] = out[o][j];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < W::NO * N * TL; e += 256) {
-        const int o = e / (N * TL), rem = e % (N * TL);
-        const int q = rem / TL, kk = rem % TL;
-        if (k0 + kk < g.s[2]) W::store(a, g, o, base + q * lstride + kk, sh[slot(o, kk, q)], 0.0);
+    // the operands the stores read, a group loaded, then its results written
+    constexpr int POUT = (W::NO * NMAX * TL + NT - 1) / NT;
+    if constexpr (!GROUPED) {
+        for (int e = threadIdx.x; e < W::NO * N * TL; e += NT) {
+            const int o = e / (N * TL), rem = e % (N * TL);
+            const int q = rem / TL, kk = rem % TL;
+            if (k0 + kk < g.s[2]) W::store(a, g, o, base + q * lstride + kk, sh[slot(o, kk, q)], 0.0);
+        }
+        return;
+    }
+#pragma unroll
+    for (int r0 = 0; r0 < POUT; r0 += GS) {
+        double opnd[GS];
+#pragma unroll
+        for (int u = 0; u < GS; u++) {
+            const int e = threadIdx.x + (r0 + u) * NT;
+            const int rem = e % (N * TL), q = rem / TL, kk = rem % TL;
+            opnd[u] = (r0 + u < POUT && e < W::NO * N * TL && k0 + kk < g.s[2])
+                          ? W::operand(a, g, e / (N * TL), base + q * lstride + kk)
+                          : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < GS; u++) {
+            const int e = threadIdx.x + (r0 + u) * NT;
+            const int o = e / (N * TL), rem = e % (N * TL);
+            const int q = rem / TL, kk = rem % TL;
+            if (r0 + u < POUT && e < W::NO * N * TL && k0 + kk < g.s[2])
+                W::store_op(a, g, o, base + q * lstride + kk, sh[slot(o, kk, q)], opnd[u]);
+        }
     }
 }
 
@@ -661,11 +905,12 @@ int launch_pass_mp(nova_ctx *ctx, int dir, const CmpDir &D, const typename W::Ar
         constexpr int TL = tile_lines<MP, W>();
         const int nkt = (g.s[2] + TL - 1) / TL;
         const int fixed = dir == 0 ? g.s[1] : g.s[0];
+        constexpr int NT = tile_threads<MP, W>();
         if (dir == 0)
-            hipLaunchKernelGGL((k_cmp_pass_t<0, MP, W>), dim3((unsigned)(fixed * nkt)), dim3(256), 0, ctx->stream, g,
+            hipLaunchKernelGGL((k_cmp_pass_t<0, MP, W>), dim3((unsigned)(fixed * nkt)), dim3(NT), 0, ctx->stream, g,
                                D, a);
         else
-            hipLaunchKernelGGL((k_cmp_pass_t<1, MP, W>), dim3((unsigned)(fixed * nkt)), dim3(256), 0, ctx->stream, g,
+            hipLaunchKernelGGL((k_cmp_pass_t<1, MP, W>), dim3((unsigned)(fixed * nkt)), dim3(NT), 0, ctx->stream, g,
                                D, a);
     }
     return check_launch("compact pass");

@@ -81,3 +81,31 @@ def test_ensemble_rejects_bad_descriptions(nv):
         nv.Ensemble([((16,) * 3, (TWO_PI,) * 3, 0, 1e-3, 0.0)], m=5)
     with pytest.raises(nv.NovaError):
         nv.Ensemble([((16,) * 3, (TWO_PI,) * 3, 0, 1e-3, 0.0), ((20,) * 3, (TWO_PI,) * 3, 0, 1e-3, 0.0)])
+
+
+def test_benchmark_size_classes_equal_their_standalone_runs(nv):
+    """The benchmark's radix-3 size classes (48^3 and 96^3, FFT lengths with a
+    factor 3) and a 64^3 class, NS and MHD members, two steps: every member
+    bit for bit its standalone run under graph replay."""
+    import torch
+    dt, steps = 2e-3, 2
+    inst = [((48, 48, 48), 0, 1 / 1600), ((48, 48, 48), 1, 1 / 1000), ((96, 96, 96), 0, 1 / 1600),
+            ((96, 96, 96), 1, 1 / 1000), ((64, 64, 64), 0, 1 / 1600)]
+    E = nv.Ensemble([(N, (TWO_PI,) * 3, model, nu, nu) for N, model, nu in inst], graph=True)
+    runs = []
+    for q, (N, model, nu) in enumerate(inst):
+        G, P, U, B = initial(nv, N, model, 70 + q)
+        E.load(q, U, B)
+        runs.append((G, P, U.clone(), None if B is None else B.clone(), model, nu))
+    E.step(dt, steps)
+    for q, (G, P, U, B, model, nu) in enumerate(runs):
+        S = nv.Solver(P, model, nv.RK_SSP3, nu=nu, eta=nu)
+        p, pB = G.field(1), G.field(1)
+        S.step(dt, steps, U, p, B, pB if B is not None else None)
+        Ue, Be, pe, pBe = E.store(q)
+        torch.cuda.synchronize()
+        print("instance", q, G.N, model, "max|dU|", (Ue - U).abs().max().item(), flush=True)
+        assert torch.equal(Ue, U) and torch.equal(pe, p)
+        if B is not None:
+            assert torch.equal(Be, B) and torch.equal(pBe, pB)
+    E.close()
