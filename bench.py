@@ -148,11 +148,12 @@ def adjoint_ratio(nv, steps: int, every: int) -> dict:
     U0 = shear_layer(nv, G)
     P.project(U0, 1.0)
     A = nv.Adjoint(nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-3), every, steps)
-    # HBM beyond the checkpoints holds the stage inputs of the forward sweep's
-    # last segments (3 x every vector fields each), which the reverse sweep
-    # then does not recompute; 24 GiB stay free for the rest of the run
+    # HBM beyond the checkpoints holds the step-start states of the forward
+    # sweep's last steps (one vector field each), whose segments the reverse
+    # sweep then does not recompute; 24 GiB stay free for the rest of the run
     free, _ = torch.cuda.mem_get_info()
-    resident = A.reserve(max(0, free - (24 << 30)))
+    extra = A.reserve(max(0, free - (24 << 30)))
+    resident = min(steps, 3 * every + extra)
     dt = 0.02
     tuned = chosen(S.autotune(U0, reps=3))
     U, p = U0.clone(), G.field(1)
@@ -184,8 +185,8 @@ def adjoint_ratio(nv, steps: int, every: int) -> dict:
                       "+ reverse sweep) / forward solve, the whole cost of one gradient",
             "workload": f"shear layer 256x384x256 FP64, free-slip y walls (DCT-I), Re=1000, dt=0.02, "
                         f"{steps} SSP-RK3 steps, checkpoints every {every}; gradient of final enstrophy",
-            "resident_segments": 1 + resident,
-            "recomputed_segments": max(0, (steps + every - 1) // every - 1 - resident),
+            "resident_steps": resident,
+            "recomputed_steps": steps - resident,
             "J": J, "grad_norm": gn, "tuning": tuned}
 
 

@@ -298,7 +298,9 @@ int nova_mhd_diag(nova_ctx *ctx, const double *U, const double *B, double mu, do
  * weight 1/2 on wall nodes (nova_weighted_dot); gradients are with respect
  * to that inner product. ---- */
 /* Reserve checkpoint memory for up to max_steps steps, one checkpoint of U
- * every checkpoint_every steps (plus 3*checkpoint_every stage fields). */
+ * every checkpoint_every steps, plus a pool of 3*checkpoint_every vector
+ * fields that keeps the step-start states of the forward sweep's last steps
+ * (one field per step). */
 int nova_adjoint_create(nova_solver *sv, int checkpoint_every, int max_steps, nova_adjoint **out);
 int nova_adjoint_destroy(nova_adjoint *ad);
 /* U = forward(U0) over nsteps of dt, storing the checkpoints. */
@@ -310,14 +312,15 @@ int nova_adjoint_tlm(nova_adjoint *ad, const double *dU0, double *dU_final);
 /* Autotuner for the adjoint and tangent-linear stage kernels, run on the
  * first checkpoint of the last forward run (see nova_solver_autotune). */
 int nova_adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t report_len);
-/* Further segment stores (3*checkpoint_every vector fields each) within
- * max_bytes of device memory, at most one per checkpoint but the first; as
- * many as the device can allocate are kept and *segments (may be null)
- * receives their count.  The forward sweep then keeps the stage inputs of
- * its last 1 + *segments segments, which the reverse sweep and the
- * tangent-linear model use without recomputing them.  max_bytes = 0
- * releases them.  Takes effect from the next nova_adjoint_forward. */
-int nova_adjoint_reserve(nova_adjoint *ad, uint64_t max_bytes, int *segments);
+/* Further pool fields (one vector field each) within max_bytes of device
+ * memory, at most max_steps - 3*checkpoint_every of them; as many as the
+ * device can allocate are kept and *fields (may be null) receives their
+ * count.  The forward sweep keeps the step-start state U of its last
+ * min(pool, nsteps) steps; the reverse sweep rebuilds the stage inputs U1, U2
+ * of those steps with two stage launches each and recomputes the earlier
+ * steps from their checkpoints a segment at a time.  max_bytes = 0 releases
+ * them.  Takes effect from the next nova_adjoint_forward. */
+int nova_adjoint_reserve(nova_adjoint *ad, uint64_t max_bytes, int *fields);
 /* out = sum_nodes w a.b over ncomp components. */
 int nova_weighted_dot(nova_ctx *ctx, const double *a, const double *b, int ncomp, double *out);
 /* J = dV sum_nodes w |curl U|^2 (central stencil curl); dJdU (may be NULL)

@@ -918,9 +918,10 @@ class Adjoint:
         return buf.value.decode()
 
     def reserve(self, max_bytes: int) -> int:
-        """Further segment stores within max_bytes of device memory, so that
-        the forward sweep keeps the stage inputs of its last segments and the
-        reverse sweep recomputes fewer; returns how many were allocated."""
+        """Further pool fields (one vector field each) within max_bytes of
+        device memory, so that the forward sweep keeps the step-start states
+        of more of its last steps and the reverse sweep recomputes fewer
+        segments; returns how many were allocated."""
         n = C.c_int(0)
         _check(lib().nova_adjoint_reserve(self._h, C.c_uint64(int(max_bytes)), C.byref(n)), "nova_adjoint_reserve")
         return n.value

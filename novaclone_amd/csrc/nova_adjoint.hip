@@ -15,11 +15,15 @@
 //   m1 = 1/4 T(U1, m2);  dJ/dU = T(U, m1) + 3/4 m2 + 1/3 m3,
 //   with T(X, m) = m + dt J_R(X)^T m: three adjoint stage launches, the last
 //   reading m2 and m3 at its own nodes and writing over m3.
-// Checkpoints of U every C steps are kept; each segment is recomputed once
-// in the reverse sweep, storing the stage inputs (U, U1, U2) of its steps in
-// a segment store.  The forward sweep writes the stage inputs of its last
-// segments into the stores it has (one, plus those nova_adjoint_reserve
-// added) as it goes, and the reverse sweep recomputes only the others.
+// Checkpoints of U every C steps are kept.  A pool of vector fields (3C
+// from nova_adjoint_create, more from nova_adjoint_reserve) holds the step
+// start states U of the forward sweep's last S = min(pool, nsteps) steps,
+// one field per step; the reverse sweep rebuilds U1 and U2 of such a step
+// with two forward stage launches.  The earlier steps are recomputed in the
+// reverse sweep a segment at a time from their checkpoint, storing (U, U1,
+// U2) per step in the first 3C pool fields, which hold steps the reverse
+// sweep has already passed by then.  The tangent-linear model walks forward
+// with a running state instead.
 #include "nova_internal.hpp"
 #include "nova_stencil.hpp"
 #include "nova_stream.hpp"
@@ -53,10 +57,15 @@ struct nova_adjoint {
     int nsteps = 0;
     double dt = 0.0;
     std::vector<double *> ckpt;   // U at steps 0, C, 2C, ...
-    // segment stores: per step of a segment U, U1, U2; store 0 receives the
-    // recomputed segments, the others only segments of the forward sweep
-    std::vector<std::vector<double *>> store;
-    std::vector<int> held;        // the segment whose stage inputs each store holds (-1: none)
+    // the pool: pool[q] holds U of step res_first + q while slot_step[q]
+    // says so; fields 0 .. 3C-1 also take the (U, U1, U2) of a recomputed
+    // segment (rc_seg), whose step n0 + q reads U from its checkpoint for
+    // q = 0 and from pool[3q] otherwise, U1 from pool[3q+1], U2 from pool[3q+2]
+    std::vector<double *> pool;
+    std::vector<int> slot_step;
+    int n_create = 0;             // pool fields allocated by nova_adjoint_create
+    int res_first = 0, nres = 0;  // resident steps res_first .. res_first + nres - 1
+    int rc_seg = -1, rc_cnt = 0;  // recomputed segment and its number of steps
     double *w[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // work vector fields
     double *ptmp = nullptr;
     int scheme = NOVA_SCHEME_EXPLICIT;
@@ -152,60 +161,69 @@ int fwd_step_store(nova_adjoint *ad, double *U, double *U1, double *U2, double *
     return project(ad->ps, Unext, ad->dt, ad->ptmp);
 }
 
-// Recompute segment s (steps s*C .. min((s+1)*C, n) - 1), storing
-// (U, U1, U2) per step.  Returns the number of steps in the segment.
-int recompute_segment(nova_adjoint *ad, int s, int *count)
+// Whether pool slot q holds U of step n (a resident step).
+bool resident(const nova_adjoint *ad, int n, int *q)
 {
-    const int C = ad->every;
-    const int n0 = s * C;
-    int n1 = n0 + C;
-    if (n1 > ad->nsteps) n1 = ad->nsteps;
+    const int k = n - ad->res_first;
+    if (k < 0 || k >= ad->nres || ad->slot_step[k] != n) return false;
+    *q = k;
+    return true;
+}
+
+// U1 and U2 of a step from its U, into the work fields w[3] and w[4].
+int rebuild_stages(nova_adjoint *ad, double *U, double *st[3])
+{
     int rc;
-    std::vector<double *> &seg = ad->store[0];
-    ad->held[0] = -1;
-    if ((rc = copy3(ad->ctx, seg[0], ad->ckpt[s]))) return rc;
-    for (int n = n0; n < n1 - 1; n++) {  // the segment's last step is not needed forward
-        const int q = n - n0;
-        if ((rc = fwd_step_store(ad, seg[3 * q], seg[3 * q + 1], seg[3 * q + 2], seg[3 * q + 3]))) return rc;
+    st[0] = U;
+    st[1] = ad->w[3];
+    st[2] = ad->w[4];
+    if ((rc = fwd_stage(ad, U, nullptr, st[1], ssp_args(0, ad->dt)))) return rc;
+    return fwd_stage(ad, st[1], U, st[2], ssp_args(1, ad->dt));
+}
+
+// Recompute steps n0 .. n of segment s = n / C from its checkpoint, storing
+// (U, U1, U2) per step in the pool's first 3C fields; the resident steps
+// those fields held are dropped.
+int recompute_segment(nova_adjoint *ad, int n)
+{
+    const int C = ad->every, s = n / C, n0 = s * C, cnt = n - n0 + 1;
+    int rc;
+    ad->rc_seg = -1;
+    for (int q = 1; q < 3 * cnt; q++) ad->slot_step[q] = -1;
+    std::vector<double *> &P = ad->pool;
+    for (int q = 0; q < cnt; q++) {
+        double *U = q == 0 ? ad->ckpt[s] : P[3 * q];
+        if ((rc = fwd_stage(ad, U, nullptr, P[3 * q + 1], ssp_args(0, ad->dt)))) return rc;
+        if ((rc = fwd_stage(ad, P[3 * q + 1], U, P[3 * q + 2], ssp_args(1, ad->dt)))) return rc;
+        if (q + 1 < cnt) {  // the last step's next state is not needed
+            if ((rc = fwd_stage(ad, P[3 * q + 2], U, P[3 * q + 3], ssp_args(2, ad->dt)))) return rc;
+            if ((rc = project(ad->ps, P[3 * q + 3], ad->dt, ad->ptmp))) return rc;
+        }
     }
-    {
-        // stage inputs of the last step only
-        const int q = n1 - 1 - n0;
-        if ((rc = fwd_stage(ad, seg[3 * q], nullptr, seg[3 * q + 1], ssp_args(0, ad->dt)))) return rc;
-        if ((rc = fwd_stage(ad, seg[3 * q + 1], seg[3 * q], seg[3 * q + 2], ssp_args(1, ad->dt)))) return rc;
-    }
-    *count = n1 - n0;
-    ad->held[0] = s;
+    ad->rc_seg = s;
+    ad->rc_cnt = cnt;
     return NOVA_OK;
 }
 
-// The stage inputs of segment s: a store that holds them, else store 0
-// after a recompute.
-int segment_inputs(nova_adjoint *ad, int s, double *const **seg, int *count)
+// (U, U1, U2) of step n held by the recomputed segment, if it holds them.
+bool recomputed(nova_adjoint *ad, int n, double *st[3])
 {
-    for (size_t k = 0; k < ad->store.size(); k++)
-        if (ad->held[k] == s) {
-            *seg = ad->store[k].data();
-            *count = (s + 1) * ad->every < ad->nsteps ? ad->every : ad->nsteps - s * ad->every;
-            return NOVA_OK;
-        }
-    if (int rc = recompute_segment(ad, s, count)) return rc;
-    *seg = ad->store[0].data();
-    return NOVA_OK;
+    const int C = ad->every, q = n - ad->rc_seg * C;
+    if (ad->rc_seg < 0 || n / C != ad->rc_seg || q >= ad->rc_cnt) return false;
+    st[0] = q == 0 ? ad->ckpt[ad->rc_seg] : ad->pool[3 * q];
+    st[1] = ad->pool[3 * q + 1];
+    st[2] = ad->pool[3 * q + 2];
+    return true;
 }
 
-int alloc_store(size_t bytes, int nfields, std::vector<double *> &st)
+// The stage inputs of step n for the reverse sweep.
+int reverse_inputs(nova_adjoint *ad, int n, double *st[3])
 {
-    for (int q = 0; q < nfields; q++) {
-        double *p = nullptr;
-        const hipError_t e = hipMalloc((void **)&p, bytes);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();  // an allocation failure is not sticky
-            for (double *x : st) (void)hipFree(x);
-            st.clear();
-            return NOVA_ERR_HIP;
-        }
-        st.push_back(p);
+    int q;
+    if (resident(ad, n, &q)) return rebuild_stages(ad, ad->pool[q], st);
+    if (!recomputed(ad, n, st)) {
+        if (int rc = recompute_segment(ad, n)) return rc;
+        recomputed(ad, n, st);
     }
     return NOVA_OK;
 }
@@ -251,12 +269,10 @@ int adjoint_create(nova_solver *sv, int every, int max_steps, nova_adjoint **out
         e = hipMalloc((void **)&p, bytes);
         ad->ckpt.push_back(p);
     }
-    ad->store.resize(1);
-    ad->held.assign(1, -1);
     for (int q = 0; q < 3 * every && e == hipSuccess; q++) {
         double *p = nullptr;
         e = hipMalloc((void **)&p, bytes);
-        ad->store[0].push_back(p);
+        ad->pool.push_back(p);
     }
     for (int q = 0; q < 5 && e == hipSuccess; q++) e = hipMalloc((void **)&ad->w[q], bytes);
     if (e == hipSuccess) e = hipMalloc((void **)&ad->ptmp, sizeof(double) * ctx->g.np);
@@ -264,13 +280,15 @@ int adjoint_create(nova_solver *sv, int every, int max_steps, nova_adjoint **out
         e = hipMalloc((void **)&ad->rwork, sizeof(double) * 4 * ctx->g.np);
     if (e != hipSuccess) {
         for (double *p : ad->ckpt) if (p) (void)hipFree(p);
-        for (double *p : ad->store[0]) if (p) (void)hipFree(p);
+        for (double *p : ad->pool) if (p) (void)hipFree(p);
         for (double *p : ad->w) if (p) (void)hipFree(p);
         if (ad->ptmp) (void)hipFree(ad->ptmp);
         if (ad->rwork) (void)hipFree(ad->rwork);
         delete ad;
         return hip_fail(e, "adjoint checkpoint memory");
     }
+    ad->n_create = (int)ad->pool.size();
+    ad->slot_step.assign(ad->pool.size(), -1);
     *out = ad;
     return NOVA_OK;
 }
@@ -280,8 +298,7 @@ int adjoint_destroy(nova_adjoint *ad)
     if (!ad) return NOVA_OK;
     (void)hipStreamSynchronize(ad->ctx->stream);
     for (double *p : ad->ckpt) if (p) (void)hipFree(p);
-    for (auto &st : ad->store)
-        for (double *p : st) if (p) (void)hipFree(p);
+    for (double *p : ad->pool) if (p) (void)hipFree(p);
     for (double *p : ad->w) if (p) (void)hipFree(p);
     if (ad->ptmp) (void)hipFree(ad->ptmp);
     if (ad->rwork) (void)hipFree(ad->rwork);
@@ -289,27 +306,31 @@ int adjoint_destroy(nova_adjoint *ad)
     return NOVA_OK;
 }
 
-// Segment stores beyond the first, within max_bytes of device memory (at
-// most one per checkpoint but the first); 0 releases them.
+// Pool fields beyond the 3C of nova_adjoint_create within max_bytes of
+// device memory (at most one per step of the budget beyond those); 0
+// releases them.
 int adjoint_reserve(nova_adjoint *ad, uint64_t max_bytes, int *count)
 {
     NOVA_HIP(hipStreamSynchronize(ad->ctx->stream));
-    for (size_t k = 1; k < ad->store.size(); k++)
-        for (double *p : ad->store[k]) (void)hipFree(p);
-    ad->store.resize(1);
-    ad->held.resize(1);
+    for (size_t k = ad->n_create; k < ad->pool.size(); k++) (void)hipFree(ad->pool[k]);
+    ad->pool.resize(ad->n_create);
     const size_t bytes = sizeof(double) * 3 * (size_t)ad->ctx->g.np;
-    const uint64_t per = (uint64_t)bytes * 3 * ad->every;
-    const int nck = (int)ad->ckpt.size();
-    int want = (int)(max_bytes / per);
-    if (want > nck - 1) want = nck - 1;
-    for (int k = 0; k < want; k++) {
-        std::vector<double *> st;
-        if (alloc_store(bytes, 3 * ad->every, st)) break;  // as many as the device holds
-        ad->store.push_back(std::move(st));
-        ad->held.push_back(-1);
+    uint64_t want = max_bytes / bytes;
+    const int room = ad->max_steps - ad->n_create;
+    if (want > (uint64_t)(room > 0 ? room : 0)) want = room > 0 ? room : 0;
+    for (uint64_t k = 0; k < want; k++) {
+        double *p = nullptr;
+        if (hipMalloc((void **)&p, bytes) != hipSuccess) {  // as many as the device holds
+            (void)hipGetLastError();  // an allocation failure is not sticky
+            break;
+        }
+        ad->pool.push_back(p);
     }
-    if (count) *count = (int)ad->store.size() - 1;
+    // the resident steps of the last forward sweep are kept where their
+    // slots survive
+    ad->slot_step.resize(ad->pool.size(), -1);
+    if (ad->nres > (int)ad->pool.size()) ad->nres = (int)ad->pool.size();
+    if (count) *count = (int)ad->pool.size() - ad->n_create;
     return NOVA_OK;
 }
 
@@ -318,29 +339,24 @@ int adjoint_forward(nova_adjoint *ad, double dt, int nsteps, const double *U0, d
     if (nsteps < 1 || nsteps > ad->max_steps) return fail(NOVA_ERR_ARG, "nsteps outside the checkpoint budget");
     ad->dt = dt;
     ad->nsteps = nsteps;
-    for (int &h : ad->held) h = -1;
+    const int P = (int)ad->pool.size();
+    ad->nres = P < nsteps ? P : nsteps;
+    ad->res_first = nsteps - ad->nres;
+    ad->rc_seg = -1;
+    for (int &t : ad->slot_step) t = -1;
     int rc;
-    if (U != U0 && (rc = copy3(ad->ctx, U, U0))) return rc;
-    const int C = ad->every, nseg = (nsteps + C - 1) / C;
-    const int nst = (int)ad->store.size();
-    const int first_held = nseg > nst ? nseg - nst : 0;
-    for (int s = 0; s < nseg; s++) {
-        const int n0 = s * C, cnt = n0 + C < nsteps ? C : nsteps - n0;
-        if ((rc = copy3(ad->ctx, ad->ckpt[s], U))) return rc;
-        if (s < first_held) {
-            for (int q = 0; q < cnt; q++)
-                if ((rc = fwd_step_store(ad, U, ad->w[0], ad->w[1], U))) return rc;
-            continue;
-        }
-        // the last segments run in the stores (the last one in store 0, which
-        // the reverse sweep frees first), keeping their stage inputs
-        const int k = nseg - 1 - s;
-        std::vector<double *> &st = ad->store[k];
-        if ((rc = copy3(ad->ctx, st[0], U))) return rc;
-        for (int q = 0; q < cnt; q++)
-            if ((rc = fwd_step_store(ad, st[3 * q], st[3 * q + 1], st[3 * q + 2], q + 1 < cnt ? st[3 * q + 3] : U)))
-                return rc;
-        ad->held[k] = s;
+    const int C = ad->every;
+    // the running state: U, until the resident steps start in the pool
+    double *cur = U;
+    if (ad->res_first == 0) cur = ad->pool[0];
+    if (cur != U0 && (rc = copy3(ad->ctx, cur, U0))) return rc;
+    for (int n = 0; n < nsteps; n++) {
+        if (n % C == 0 && (rc = copy3(ad->ctx, ad->ckpt[n / C], cur))) return rc;
+        double *next = U;
+        if (n + 1 >= ad->res_first && n + 1 < nsteps) next = ad->pool[n + 1 - ad->res_first];
+        if (n >= ad->res_first) ad->slot_step[n - ad->res_first] = n;
+        if ((rc = fwd_step_store(ad, cur, ad->w[0], ad->w[1], next))) return rc;
+        cur = next;
     }
     return NOVA_OK;
 }
@@ -352,30 +368,26 @@ int adjoint_backward(nova_adjoint *ad, const double *lamT, double *lam0)
     double *L = lam0, *A = ad->w[0], *B = ad->w[1], *ACC = ad->w[2];
     int rc;
     if (L != lamT && (rc = copy3(ctx, L, lamT))) return rc;
-    const int nseg = (ad->nsteps + ad->every - 1) / ad->every;
     const bool compact = ad->scheme == NOVA_SCHEME_COMPACT6;
-    for (int s = nseg - 1; s >= 0; s--) {
-        int cnt = 0;
-        double *const *seg = nullptr;
-        if ((rc = segment_inputs(ad, s, &seg, &cnt))) return rc;
-        for (int q = cnt - 1; q >= 0; q--) {
-            double *U = seg[3 * q], *U1 = seg[3 * q + 1], *U2 = seg[3 * q + 2];
-            if ((rc = project(ad->ps, L, 1.0, ad->ptmp))) return rc;  // m3 = P m
-            if (compact) {
-                // the pencil passes carry one pointwise addend and an accumulator
-                const double dt = ad->dt, nu = ad->nu;
-                if ((rc = compact_adj_stage(ctx, U2, L, A, 2.0 / 3.0, nullptr, 0.0, ACC, 0.0, 1.0 / 3.0, dt, nu,
-                                            ad->rwork)))
-                    return rc;
-                if ((rc = compact_adj_stage(ctx, U1, A, B, 0.25, nullptr, 0.0, ACC, 1.0, 0.75, dt, nu, ad->rwork)))
-                    return rc;
-                if ((rc = compact_adj_stage(ctx, U, B, L, 1.0, ACC, 1.0, nullptr, 0.0, 0.0, dt, nu, ad->rwork)))
-                    return rc;
-            } else {
-                if ((rc = adj_stage_scaled(ad, U2, L, A, 2.0 / 3.0))) return rc;            // m2
-                if ((rc = adj_stage_scaled(ad, U1, A, B, 0.25))) return rc;                 // m1
-                if ((rc = adj_stage_sum(ad, U, B, L, A, 0.75, L, 1.0 / 3.0))) return rc;    // dJ/dU
-            }
+    for (int n = ad->nsteps - 1; n >= 0; n--) {
+        double *st[3];
+        if ((rc = reverse_inputs(ad, n, st))) return rc;
+        double *U = st[0], *U1 = st[1], *U2 = st[2];
+        if ((rc = project(ad->ps, L, 1.0, ad->ptmp))) return rc;  // m3 = P m
+        if (compact) {
+            // the pencil passes carry one pointwise addend and an accumulator
+            const double dt = ad->dt, nu = ad->nu;
+            if ((rc = compact_adj_stage(ctx, U2, L, A, 2.0 / 3.0, nullptr, 0.0, ACC, 0.0, 1.0 / 3.0, dt, nu,
+                                        ad->rwork)))
+                return rc;
+            if ((rc = compact_adj_stage(ctx, U1, A, B, 0.25, nullptr, 0.0, ACC, 1.0, 0.75, dt, nu, ad->rwork)))
+                return rc;
+            if ((rc = compact_adj_stage(ctx, U, B, L, 1.0, ACC, 1.0, nullptr, 0.0, 0.0, dt, nu, ad->rwork)))
+                return rc;
+        } else {
+            if ((rc = adj_stage_scaled(ad, U2, L, A, 2.0 / 3.0))) return rc;            // m2
+            if ((rc = adj_stage_scaled(ad, U1, A, B, 0.25))) return rc;                 // m1
+            if ((rc = adj_stage_sum(ad, U, B, L, A, 0.75, L, 1.0 / 3.0))) return rc;    // dJ/dU
         }
     }
     return NOVA_OK;
@@ -399,7 +411,8 @@ int adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t len)
     if (int rc = copy3(ctx, V, X)) return rc;
     if (int rc = copy3(ctx, ad->w[3], X)) return rc;
     auto adj = [&]() -> int { return adj_stage_scaled(ad, X, V, ad->w[0], 1.0); };
-    if (int rc = tune_stage_model(ctx, MODEL_ADJ, variants, 10, reps, adj, "adj_stage", rep)) return rc;
+    constexpr int nv = (int)(sizeof(variants) / sizeof(variants[0]));
+    if (int rc = tune_stage_model(ctx, MODEL_ADJ, variants, nv, reps, adj, "adj_stage", rep)) return rc;
     auto tlm = [&]() -> int {
         LinArgs a = lin_args(ad, X, V);
         a.out[0] = ad->w[0];
@@ -407,7 +420,7 @@ int adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t len)
         a.sa = ssp_args(1, ad->dt);
         return launch_stream_model<MODEL_TLM>(ctx, a, "tlm_stage");
     };
-    if (int rc = tune_stage_model(ctx, MODEL_TLM, variants, 10, reps, tlm, "tlm_stage", rep)) return rc;
+    if (int rc = tune_stage_model(ctx, MODEL_TLM, variants, nv, reps, tlm, "tlm_stage", rep)) return rc;
     return write_report(rep, report, len);
 }
 
@@ -416,19 +429,37 @@ int adjoint_tlm(nova_adjoint *ad, const double *d0, double *dT)
     if (ad->nsteps < 1) return fail(NOVA_ERR_STATE, "run nova_adjoint_forward first");
     nova_ctx *ctx = ad->ctx;
     double *D = dT, *D1 = ad->w[0], *D2 = ad->w[1];
+    // steps neither resident nor recomputed walk forward from their
+    // checkpoint in w[2] (cur_n: the step whose U it holds, -1: none)
+    double *cur = ad->w[2];
+    int cur_n = -1;
     int rc;
     if (D != d0 && (rc = copy3(ctx, D, d0))) return rc;
-    const int nseg = (ad->nsteps + ad->every - 1) / ad->every;
-    for (int s = 0; s < nseg; s++) {
-        int cnt = 0;
-        double *const *seg = nullptr;
-        if ((rc = segment_inputs(ad, s, &seg, &cnt))) return rc;
-        for (int q = 0; q < cnt; q++) {
-            double *U = seg[3 * q], *U1 = seg[3 * q + 1], *U2 = seg[3 * q + 2];
-            if ((rc = tlm_stage(ad, U, D, nullptr, D1, ssp_args(0, ad->dt)))) return rc;
-            if ((rc = tlm_stage(ad, U1, D1, D, D2, ssp_args(1, ad->dt)))) return rc;
-            if ((rc = tlm_stage(ad, U2, D2, D, D, ssp_args(2, ad->dt)))) return rc;
-            if ((rc = project(ad->ps, D, ad->dt, ad->ptmp))) return rc;
+    for (int n = 0; n < ad->nsteps; n++) {
+        double *st[3];
+        int q;
+        bool walk = false;
+        if (resident(ad, n, &q)) {
+            if ((rc = rebuild_stages(ad, ad->pool[q], st))) return rc;
+        } else if (!recomputed(ad, n, st)) {
+            if (cur_n != n) {
+                const int s = n / ad->every;
+                if ((rc = copy3(ctx, cur, ad->ckpt[s]))) return rc;
+                for (int k = s * ad->every; k < n; k++)
+                    if ((rc = fwd_step_store(ad, cur, ad->w[3], ad->w[4], cur))) return rc;
+            }
+            if ((rc = rebuild_stages(ad, cur, st))) return rc;
+            walk = true;
+        }
+        if ((rc = tlm_stage(ad, st[0], D, nullptr, D1, ssp_args(0, ad->dt)))) return rc;
+        if ((rc = tlm_stage(ad, st[1], D1, D, D2, ssp_args(1, ad->dt)))) return rc;
+        if ((rc = tlm_stage(ad, st[2], D2, D, D, ssp_args(2, ad->dt)))) return rc;
+        if ((rc = project(ad->ps, D, ad->dt, ad->ptmp))) return rc;
+        cur_n = -1;
+        if (walk && n + 1 < ad->nsteps) {  // advance the running state
+            if ((rc = fwd_stage(ad, st[2], cur, cur, ssp_args(2, ad->dt)))) return rc;
+            if ((rc = project(ad->ps, cur, ad->dt, ad->ptmp))) return rc;
+            cur_n = n + 1;
         }
     }
     return NOVA_OK;

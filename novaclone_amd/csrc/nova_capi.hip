@@ -731,10 +731,10 @@ int nova_adjoint_autotune(nova_adjoint *ad, int reps, char *report, size_t repor
     NOVA_GUARD(adjoint_autotune(ad, reps, report, report_len));
 }
 
-int nova_adjoint_reserve(nova_adjoint *ad, uint64_t max_bytes, int *segments)
+int nova_adjoint_reserve(nova_adjoint *ad, uint64_t max_bytes, int *fields)
 {
     NOVA_REQUIRE(ad, "null adjoint");
-    NOVA_GUARD(adjoint_reserve(ad, max_bytes, segments));
+    NOVA_GUARD(adjoint_reserve(ad, max_bytes, fields));
 }
 
 int nova_wc_create(nova_ctx *ctx, double mu, double pc, nova_wc **out)

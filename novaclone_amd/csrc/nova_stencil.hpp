@@ -430,6 +430,52 @@ __device__ inline AdjZ<H> adj_z(const AdjK &k, double zi, double zl)
     return z;
 }
 
+// The terms of the adjoint increment that come from the offsets along one
+// direction d (weights h[l], kb[l] of that direction): A_c = sum_l [bp m_c(+)
+// + bm m_c(-)], dv = sum_l h (X_d(+) - X_d(-)), T = sum_l h t_l (the T_d term
+// above), each summed over l from zero.  x0, m0: the values at the node.
+template <int H, class Acc>
+__device__ inline void adj_dir(const double *h, const double *kb, const Acc &A, int d, const double (&x0)[3],
+                               const double (&m0)[3], double (&Ad)[3], double &dv, double &T)
+{
+    Ad[0] = Ad[1] = Ad[2] = 0.0;
+    dv = 0.0;
+    T = 0.0;
+#pragma unroll
+    for (int l = 0; l < H; l++) {
+        double P[3], M[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            P[c] = A(0, c, d, l + 1) + x0[c];
+            M[c] = A(0, c, d, -l - 1) + x0[c];
+        }
+        const double bp = __builtin_fma(h[l], P[d], kb[l]);
+        const double bm = __builtin_fma(-h[l], M[d], kb[l]);
+        dv = __builtin_fma(h[l], A(0, d, d, l + 1) - A(0, d, d, -l - 1), dv);
+        double t = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double mp = A(1, c, d, l + 1), mm = A(1, c, d, -l - 1);
+            Ad[c] = __builtin_fma(bp, mp, Ad[c]);
+            Ad[c] = __builtin_fma(bm, mm, Ad[c]);
+            t = __builtin_fma(mp - m0[c], P[c], t);
+            t = __builtin_fma(m0[c] - mm, M[c], t);
+        }
+        T = __builtin_fma(h[l], t, T);
+    }
+}
+
+// The increment from the three directions' terms: the sums over directions
+// are taken as (x + y) + z, so that a kernel computing the directions in
+// different waves (the direction-split adjoint stage) gives the same bits.
+__device__ inline void adj_combine(const double (&Ad)[3][3], const double (&dv)[3], const double (&T)[3],
+                                   const double (&m0)[3], double lam, double (&du)[3])
+{
+    const double g0 = ((dv[0] + dv[1]) + dv[2]) + lam;
+#pragma unroll
+    for (int c = 0; c < 3; c++) du[c] = __builtin_fma(-m0[c], g0, (Ad[0][c] + Ad[1][c]) + Ad[2][c]) + T[c];
+}
+
 template <int H, class Acc>
 __device__ inline void adj_du(const AdjK &k, const AdjZ<H> &z, const Acc &A, double (&du)[3])
 {
@@ -439,37 +485,11 @@ __device__ inline void adj_du(const AdjK &k, const AdjZ<H> &z, const Acc &A, dou
         x0[c] = A(0, c, 0, 0);
         m0[c] = A(1, c, 0, 0);
     }
-    double acc[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0}, dv = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-#pragma unroll
-        for (int l = 0; l < H; l++) {
-            const double h = d == 2 ? z.h[l] : k.h[d][l];
-            const double kb = d == 2 ? z.kb[l] : k.kb[d][l];
-            double P[3], M[3];
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                P[c] = A(0, c, d, l + 1) + x0[c];
-                M[c] = A(0, c, d, -l - 1) + x0[c];
-            }
-            const double bp = __builtin_fma(h, P[d], kb);
-            const double bm = __builtin_fma(-h, M[d], kb);
-            dv = __builtin_fma(h, A(0, d, d, l + 1) - A(0, d, d, -l - 1), dv);
-            double t = 0.0;
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const double mp = A(1, c, d, l + 1), mm = A(1, c, d, -l - 1);
-                acc[c] = __builtin_fma(bp, mp, acc[c]);
-                acc[c] = __builtin_fma(bm, mm, acc[c]);
-                t = __builtin_fma(mp - m0[c], P[c], t);
-                t = __builtin_fma(m0[c] - mm, M[c], t);
-            }
-            T[d] = __builtin_fma(h, t, T[d]);
-        }
-    }
-    const double g0 = dv + z.lam;
-#pragma unroll
-    for (int c = 0; c < 3; c++) du[c] = __builtin_fma(-m0[c], g0, acc[c]) + T[c];
+    double Ad[3][3], dv[3], T[3];
+    adj_dir<H>(k.h[0], k.kb[0], A, 0, x0, m0, Ad[0], dv[0], T[0]);
+    adj_dir<H>(k.h[1], k.kb[1], A, 1, x0, m0, Ad[1], dv[1], T[1]);
+    adj_dir<H>(z.h, z.kb, A, 2, x0, m0, Ad[2], dv[2], T[2]);
+    adj_combine(Ad, dv, T, m0, z.lam, du);
 }
 
 // Sign of a value reached through a reflection (walls only; a periodic

@@ -195,11 +195,13 @@ def test_shear_layer_gradient_midsize(nv):
         assert abs(fd2 - ad) <= 1e-8 * abs(ad) + 4 * u * J / e2
 
 
-def test_reserved_segment_stores_give_the_same_results(nv):
-    """Segment stores kept from the forward sweep (nova_adjoint_reserve)
-    replace recomputed segments bit for bit, in the reverse sweep and the
-    tangent-linear model, for every number of stores; a partial last segment
-    (20 steps, checkpoints every 3)."""
+def test_reserved_pool_fields_give_the_same_results(nv):
+    """Step-start states kept from the forward sweep (the pool of 3C fields
+    from nova_adjoint_create plus those of nova_adjoint_reserve) replace
+    recomputed segments bit for bit, in the reverse sweep and the
+    tangent-linear model, for every pool size: resident steps starting at a
+    segment boundary, inside a segment, every step resident, and the default
+    pool; a partial last segment (20 steps, checkpoints every 3)."""
     import torch
     G, P, U0 = shear_layer(nv, (32, 48, 32), (TWO_PI / 2, 1.5, TWO_PI / 2), kmax=2)
     A = nv.Adjoint(nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-3), 3, 20)
@@ -209,16 +211,17 @@ def test_reserved_segment_stores_give_the_same_results(nv):
     d = rand_like(U0, 31, True)
     l0 = A.backward(g)
     t0 = A.tlm(d)
-    per = 3 * 3 * 3 * G.np * 8  # one store: 3 C vector fields
-    for want, expect in ((2 * per, 2), (1 << 50, 6), (0, 0)):
+    field = 3 * G.np * 8  # one pool field
+    for want, expect in ((2 * field, 2), (7 * field + 5, 7), (1 << 50, 11), (0, 0)):
         got = A.reserve(want)
-        print("stores", want, got, flush=True)
+        print("pool fields", want, got, flush=True)
         assert got == expect
         assert torch.equal(A.forward(dt, n, U0), U)
         assert torch.equal(A.backward(g), l0)
-        assert torch.equal(A.backward(g), l0)  # the stores still hold their segments
+        assert torch.equal(A.backward(g), l0)  # after a reverse sweep that recomputed
         assert torch.equal(A.tlm(d), t0)
         assert torch.equal(A.backward(g), l0)
+        assert torch.equal(A.tlm(d), t0)
     A.close()
 
 
