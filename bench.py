@@ -150,9 +150,10 @@ def adjoint_ratio(nv, steps: int, every: int) -> dict:
     A = nv.Adjoint(nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=1e-3), every, steps)
     # HBM beyond the checkpoints holds the step-start states of the forward
     # sweep's last steps (one vector field each), whose segments the reverse
-    # sweep then does not recompute; 24 GiB stay free for the rest of the run
+    # sweep then does not recompute; 8 GiB stay free for the rest of the run
+    # (it allocates about six more vector fields, 3.6 GB)
     free, _ = torch.cuda.mem_get_info()
-    extra = A.reserve(max(0, free - (24 << 30)))
+    extra = A.reserve(max(0, free - (8 << 30)))
     resident = min(steps, 3 * every + extra)
     dt = 0.02
     tuned = chosen(S.autotune(U0, reps=3))

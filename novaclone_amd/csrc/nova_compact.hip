@@ -97,6 +97,18 @@ CmpDir make_dir(int n, double h, int mp)
     return D;
 }
 
+// Unsigned division by a run-time divisor d >= 2 as a multiply-high by
+// m = ceil(2^32 / d): x m / 2^32 = x / d + x e / 2^32 with 0 <= e < 1, whose
+// floor is floor(x / d) while x < 2^32 / d, which the tile indices (x below
+// 2^16, d at most 2^13) satisfy.  A division by a run-time value otherwise
+// costs a few dozen instructions, on every staged element.
+struct FastDiv {
+    uint32_t m, d;
+    __device__ explicit FastDiv(uint32_t dd) : m(0xFFFFFFFFu / dd + 1u), d(dd) {}
+    __device__ uint32_t div(uint32_t x) const { return __umulhi(x, m); }
+    __device__ uint32_t mod(uint32_t x) const { return x - div(x) * d; }
+};
+
 // ---- wavefront line algebra ----
 
 // UR: scan steps unrolled to kScanMax, each taken when the scheme needs it
@@ -270,6 +282,7 @@ __device__ inline void cmp_deriv_n(const CmpDir &D, int lane, const double (&f)[
 // out = D^order f (one component)
 struct DerivW {
     static constexpr int NI = 1, NO = 1;
+    static constexpr bool PAIR = false;  // the k pass stores point by point
     struct Args {
         const double *f;
         double *out;
@@ -299,6 +312,7 @@ struct DerivW {
 // mode 0: R = c_d; 1: R += c_d; 2: out = stage combination with dt (R + c_d).
 struct NsW {
     static constexpr int NI = 3, NO = 3;
+    static constexpr bool PAIR = true;  // the k pass stores two points at a time (store_pair)
     struct Args {
         const double *U;
         const double *U0;
@@ -376,6 +390,28 @@ struct NsW {
     __device__ static void store_op(const Args &a, const Grid &g, int o, int64_t q, double v, double r)
     {
         a.R[o * g.np + q] = a.mode == 1 ? r + v : v;  // the i and j passes: modes 0 and 1
+    }
+    // store() for the points q and q + 1 (q even on a line of even length,
+    // so 16-byte aligned): the same expressions, 16-byte loads and stores
+    __device__ static void store_pair(const Args &a, const Grid &g, int64_t q, const double (&v)[2][NO],
+                                      const double (&x)[2][NI])
+    {
+#pragma unroll
+        for (int o = 0; o < 3; o++) {
+            const int64_t e = o * g.np + q;
+            double2 *R2 = (double2 *)(a.R + e);
+            if (a.mode == 0) {
+                *R2 = double2{v[0][o], v[1][o]};
+            } else if (a.mode == 1) {
+                const double2 r = *R2;
+                *R2 = double2{r.x + v[0][o], r.y + v[1][o]};
+            } else {
+                const double2 r = *R2;
+                const double2 x0 = a.U0 ? *(const double2 *)(a.U0 + e) : double2{x[0][o], x[1][o]};
+                const double du0 = a.sa.dt * (r.x + v[0][o]), du1 = a.sa.dt * (r.y + v[1][o]);
+                *(double2 *)(a.out + e) = double2{combine(a.sa, x[0][o], x0.x, du0), combine(a.sa, x[1][o], x0.y, du1)};
+            }
+        }
     }    __device__ static void store_point(const Args &a, const Grid &g, int64_t q, const double *v, const double *x)
     {
 #pragma unroll
@@ -387,6 +423,7 @@ struct NsW {
 // 2: d = (d + D F_d) * scale
 struct DivW {
     static constexpr int NI = 1, NO = 1;
+    static constexpr bool PAIR = false;  // the k pass stores point by point
     struct Args {
         const double *F;  // the pass direction's component
         double *d;
@@ -422,6 +459,7 @@ struct DivW {
 // gradient correction along one direction: F_d -= s D_d p
 struct GradW {
     static constexpr int NI = 1, NO = 1;
+    static constexpr bool PAIR = false;  // the k pass stores point by point
     struct Args {
         const double *p;
         double *F;  // the pass direction's component
@@ -455,6 +493,7 @@ struct GradW {
 // stage input and v0 the step-start tangent.
 struct TlmW {
     static constexpr int NI = 6, NO = 3;
+    static constexpr bool PAIR = false;  // the k pass stores point by point
     struct Args {
         const double *X, *V, *V0;
         double *R, *out;
@@ -534,6 +573,7 @@ struct TlmW {
 // out = ob (m + dt (R_c - m_c div X)) + yg Y, acc = acc_a acc + acc_b m.
 struct AdjW {
     static constexpr int NI = 6, NO = 4;
+    static constexpr bool PAIR = false;  // the k pass stores point by point
     struct Args {
         const double *X, *M;
         const double *Y;
@@ -626,6 +666,7 @@ struct AdjW {
 //   dB_c = -Bf(U,B)_c + Bf(B,U)_c + eta D2_d B_c;  modes as NsW (6 comps).
 struct MhdW {
     static constexpr int NI = 6, NO = 6;
+    static constexpr bool PAIR = false;  // the k pass stores point by point
     struct Args {
         const double *U, *B, *U0, *B0;
         double *R;  // 6 components
@@ -763,19 +804,45 @@ __global__ void __launch_bounds__(256) k_cmp_pass_k(Grid g, CmpDir D, typename W
 #pragma unroll
     for (int c = 0; c < W::NI; c++) {
         const double *f = W::input(a, g, c) + base;
+        if constexpr (MP % 2 == 0) {
+            // a lane's MP points are consecutive and start at an even node
+            // of a line of even length: 16-byte loads, two points each
 #pragma unroll
-        for (int j = 0; j < MP; j++) in[c][j] = on ? f[lane * MP + j] : 0.0;
+            for (int j = 0; j < MP; j += 2) {
+                const double2 v = on ? *(const double2 *)(f + lane * MP + j) : double2{0.0, 0.0};
+                in[c][j] = v.x;
+                in[c][j + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < MP; j++) in[c][j] = on ? f[lane * MP + j] : 0.0;
+        }
     }
     W::template compute<MP, 2>(a, D, lane, in, out);
     if (!on) return;
+    if constexpr (W::PAIR && MP % 2 == 0) {
 #pragma unroll
-    for (int j = 0; j < MP; j++) {
-        double v[W::NO], x[W::NI];
+        for (int j = 0; j < MP; j += 2) {
+            double v[2][W::NO], x[2][W::NI];
 #pragma unroll
-        for (int o = 0; o < W::NO; o++) v[o] = out[o][j];
+            for (int h = 0; h < 2; h++) {
 #pragma unroll
-        for (int c = 0; c < W::NI; c++) x[c] = in[c][j];
-        W::store_point(a, g, base + lane * MP + j, v, x);
+                for (int o = 0; o < W::NO; o++) v[h][o] = out[o][j + h];
+#pragma unroll
+                for (int c = 0; c < W::NI; c++) x[h][c] = in[c][j + h];
+            }
+            W::store_pair(a, g, base + lane * MP + j, v, x);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < MP; j++) {
+            double v[W::NO], x[W::NI];
+#pragma unroll
+            for (int o = 0; o < W::NO; o++) v[o] = out[o][j];
+#pragma unroll
+            for (int c = 0; c < W::NI; c++) x[c] = in[c][j];
+            W::store_point(a, g, base + lane * MP + j, v, x);
+        }
     }
 }
 
@@ -799,7 +866,8 @@ __global__ void __launch_bounds__((tile_threads<MP, W>())) k_cmp_pass_t(Grid g, 
     // column q / MP rotated by rot(row): the solve reads whole rows (any
     // rotation keeps that free of bank conflicts), and the staging copies,
     // whose lanes run over kk and q, hit distinct banks
-    auto rot = [&](int row) { return (2 * row + ((row >> 5) & 1)) % P; };
+    const FastDiv dP((uint32_t)P), dNT((uint32_t)(N * TL));
+    auto rot = [&](int row) { return (int)dP.mod((uint32_t)(2 * row + ((row >> 5) & 1))); };
     auto col = [&](int row, int c0) {
         const int v = c0 + rot(row);
         return v >= P ? v - P : v;
@@ -822,25 +890,48 @@ __global__ void __launch_bounds__((tile_threads<MP, W>())) k_cmp_pass_t(Grid g, 
 #pragma unroll
             for (int u = 0; u < GS; u++) {
                 const int e = threadIdx.x + (r0 + u) * NT;
-                const int rem = e % (N * TL), q = rem / TL, kk = rem % TL;
+                const int rem = (int)dNT.mod(e), q = rem / TL, kk = rem % TL;
                 v[u] = 0.0;
                 if (r0 + u < PIN && e < W::NI * N * TL && k0 + kk < g.s[2])
-                    v[u] = W::input(a, g, e / (N * TL))[base + q * lstride + kk];
+                    v[u] = W::input(a, g, (int)dNT.div(e))[base + q * lstride + kk];
             }
 #pragma unroll
             for (int u = 0; u < GS; u++) {
                 const int e = threadIdx.x + (r0 + u) * NT;
-                const int rem = e % (N * TL);
-                if (r0 + u < PIN && e < W::NI * N * TL) sh[slot(e / (N * TL), rem % TL, rem / TL)] = v[u];
+                const int rem = (int)dNT.mod(e);
+                if (r0 + u < PIN && e < W::NI * N * TL) sh[slot((int)dNT.div(e), rem % TL, rem / TL)] = v[u];
             }
         }
     } else {
-        for (int e = threadIdx.x; e < W::NI * N * TL; e += NT) {
-            const int c = e / (N * TL), rem = e % (N * TL);
-            const int q = rem / TL, kk = rem % TL;
-            double v = 0.0;
-            if (k0 + kk < g.s[2]) v = W::input(a, g, c)[base + q * lstride + kk];
-            sh[slot(c, kk, q)] = v;
+        // the same elements as e = tid + r NT (c = e / (N TL), q = e % (N TL)
+        // / TL, kk = e % TL), walked incrementally: kk stays tid % TL, q steps
+        // by NT / TL (a multiple of MP, so q % MP and the rotation of the
+        // element's LDS row change only with c) and wraps past N into the
+        // next component
+        constexpr int QS = NT / TL;
+        static_assert(QS % MP == 0, "the walk keeps q % MP");
+        const int kk = threadIdx.x % TL;
+        const bool kin = k0 + kk < g.s[2];
+        int q = threadIdx.x / TL, c = 0;
+        while (c < W::NI && q >= N) {
+            q -= N;
+            c++;
+        }
+        while (c < W::NI) {
+            const int row = (c * TL + kk) * MP + q % MP, r = rot(row);
+            double *const srow = sh + (c * TL + kk) * NMAX + (q % MP) * P;
+            const double *src = W::input(a, g, c) + base + kk + q * lstride;
+            for (; q < N; q += QS, src += QS * lstride) {
+                int cc = q / MP + r;
+                cc -= cc >= P ? P : 0;
+                srow[cc] = kin ? *src : 0.0;
+            }
+            q -= N;
+            c++;
+            while (c < W::NI && q >= N) {
+                q -= N;
+                c++;
+            }
         }
     }
     __syncthreads();
@@ -848,11 +939,15 @@ __global__ void __launch_bounds__((tile_threads<MP, W>())) k_cmp_pass_t(Grid g, 
     const bool on = lane < P;
     for (int kk = wave; kk < TL; kk += NW) {
         double in[W::NI][MP], out[W::NO][MP];
+        // unconditional reads (lanes past P read lane 0's place), then a select
+        const int ln = on ? lane : 0;
 #pragma unroll
         for (int c = 0; c < W::NI; c++)
 #pragma unroll
-            for (int j = 0; j < MP; j++)
-                in[c][j] = on ? sh[(c * TL + kk) * NMAX + j * P + col((c * TL + kk) * MP + j, on ? lane : 0)] : 0.0;
+            for (int j = 0; j < MP; j++) {
+                const double v = sh[(c * TL + kk) * NMAX + j * P + col((c * TL + kk) * MP + j, ln)];
+                in[c][j] = on ? v : 0.0;
+            }
         W::template compute<MP, DIR>(a, D, lane, in, out);
         if (on)
 #pragma unroll
@@ -864,10 +959,30 @@ __global__ void __launch_bounds__((tile_threads<MP, W>())) k_cmp_pass_t(Grid g, 
     // the operands the stores read, a group loaded, then its results written
     constexpr int POUT = (W::NO * NMAX * TL + NT - 1) / NT;
     if constexpr (!GROUPED) {
-        for (int e = threadIdx.x; e < W::NO * N * TL; e += NT) {
-            const int o = e / (N * TL), rem = e % (N * TL);
-            const int q = rem / TL, kk = rem % TL;
-            if (k0 + kk < g.s[2]) W::store(a, g, o, base + q * lstride + kk, sh[slot(o, kk, q)], 0.0);
+        // the staging loop's walk over (o, q) with kk fixed
+        constexpr int QS = NT / TL;
+        const int kk = threadIdx.x % TL;
+        if (k0 + kk >= g.s[2]) return;  // no barriers below
+        int q = threadIdx.x / TL, o = 0;
+        while (o < W::NO && q >= N) {
+            q -= N;
+            o++;
+        }
+        while (o < W::NO) {
+            const int row = (o * TL + kk) * MP + q % MP, r = rot(row);
+            const double *const srow = sh + (o * TL + kk) * NMAX + (q % MP) * P;
+            int64_t off = base + kk + q * lstride;
+            for (; q < N; q += QS, off += QS * lstride) {
+                int cc = q / MP + r;
+                cc -= cc >= P ? P : 0;
+                W::store(a, g, o, off, srow[cc], 0.0);
+            }
+            q -= N;
+            o++;
+            while (o < W::NO && q >= N) {
+                q -= N;
+                o++;
+            }
         }
         return;
     }
@@ -877,15 +992,15 @@ __global__ void __launch_bounds__((tile_threads<MP, W>())) k_cmp_pass_t(Grid g, 
 #pragma unroll
         for (int u = 0; u < GS; u++) {
             const int e = threadIdx.x + (r0 + u) * NT;
-            const int rem = e % (N * TL), q = rem / TL, kk = rem % TL;
+            const int rem = (int)dNT.mod(e), q = rem / TL, kk = rem % TL;
             opnd[u] = (r0 + u < POUT && e < W::NO * N * TL && k0 + kk < g.s[2])
-                          ? W::operand(a, g, e / (N * TL), base + q * lstride + kk)
+                          ? W::operand(a, g, (int)dNT.div(e), base + q * lstride + kk)
                           : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < GS; u++) {
             const int e = threadIdx.x + (r0 + u) * NT;
-            const int o = e / (N * TL), rem = e % (N * TL);
+            const int o = (int)dNT.div(e), rem = (int)dNT.mod(e);
             const int q = rem / TL, kk = rem % TL;
             if (r0 + u < POUT && e < W::NO * N * TL && k0 + kk < g.s[2])
                 W::store_op(a, g, o, base + q * lstride + kk, sh[slot(o, kk, q)], opnd[u]);
