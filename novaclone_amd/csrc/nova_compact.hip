@@ -921,12 +921,22 @@ __global__ void __launch_bounds__((tile_threads<MP, W>())) k_cmp_pass_t(Grid g, 
             const int row = (c * TL + kk) * MP + q % MP, r = rot(row);
             double *const srow = sh + (c * TL + kk) * NMAX + (q % MP) * P;
             const double *src = W::input(a, g, c) + base + kk + q * lstride;
-            for (; q < N; q += QS, src += QS * lstride) {
-                int cc = q / MP + r;
-                cc -= cc >= P ? P : 0;
-                srow[cc] = kin ? *src : 0.0;
+            // GS elements of the line at a time: their loads all in flight
+            // before the group's LDS writes; the walk then goes on from the
+            // first element past the line (into the next component)
+            const int qe = q < N ? q + (N - q + QS - 1) / QS * QS : q;
+            for (; q < N; q += GS * QS, src += GS * QS * lstride) {
+                double v[GS];
+#pragma unroll
+                for (int u = 0; u < GS; u++) v[u] = kin && q + u * QS < N ? src[u * QS * lstride] : 0.0;
+#pragma unroll
+                for (int u = 0; u < GS; u++) {
+                    int cc = (q + u * QS) / MP + r;
+                    cc -= cc >= P ? P : 0;
+                    if (q + u * QS < N) srow[cc] = v[u];
+                }
             }
-            q -= N;
+            q = qe - N;
             c++;
             while (c < W::NI && q >= N) {
                 q -= N;
@@ -972,12 +982,21 @@ __global__ void __launch_bounds__((tile_threads<MP, W>())) k_cmp_pass_t(Grid g, 
             const int row = (o * TL + kk) * MP + q % MP, r = rot(row);
             const double *const srow = sh + (o * TL + kk) * NMAX + (q % MP) * P;
             int64_t off = base + kk + q * lstride;
-            for (; q < N; q += QS, off += QS * lstride) {
-                int cc = q / MP + r;
-                cc -= cc >= P ? P : 0;
-                W::store(a, g, o, off, srow[cc], 0.0);
+            // the operands of GS stores loaded first, then their results
+            const int qe = q < N ? q + (N - q + QS - 1) / QS * QS : q;
+            for (; q < N; q += GS * QS, off += GS * QS * lstride) {
+                double opnd[GS];
+#pragma unroll
+                for (int u = 0; u < GS; u++)
+                    opnd[u] = q + u * QS < N ? W::operand(a, g, o, off + u * QS * lstride) : 0.0;
+#pragma unroll
+                for (int u = 0; u < GS; u++) {
+                    int cc = (q + u * QS) / MP + r;
+                    cc -= cc >= P ? P : 0;
+                    if (q + u * QS < N) W::store_op(a, g, o, off + u * QS * lstride, srow[cc], opnd[u]);
+                }
             }
-            q -= N;
+            q = qe - N;
             o++;
             while (o < W::NO && q >= N) {
                 q -= N;
