@@ -137,20 +137,29 @@ __device__ inline void wc_slots(const Grid &g, const WcTile &tl, WcSlots<H> &S)
 
 // Stages NC values per node of the haloed plane i of the tile in LDS
 // (sh[c][e]); val(off, fl, c) gives component c at a node.
+// Every slot's values are loaded (a slot past the plane reads the mapped
+// node wc_slots gave it, a valid one) before any is written, so that the
+// plane's loads are in flight together instead of one slot at a time.
 template <int H, int NC, class Val>
 __device__ inline void wc_stage_plane(const Grid &g, const WcSlots<H> &S, int i, double *sh, Val val)
 {
     constexpr int NP = WcSlots<H>::LK * WcSlots<H>::LJ;
+    constexpr int NS = WcSlots<H>::NS;
     int fl[3];
     const int64_t po = (int64_t)bmap(i, g.n[0], g.wall[0], fl[0]) * g.s[1] * g.s[2];
+    double v[NS][NC];
 #pragma unroll
-    for (int s = 0; s < WcSlots<H>::NS; s++) {
-        if (S.pos[s] < 0) continue;
+    for (int s = 0; s < NS; s++) {
         fl[1] = S.fj[s];
         fl[2] = S.fk[s];
 #pragma unroll
-        for (int c = 0; c < NC; c++) sh[c * NP + S.pos[s]] = val(po + S.off[s], fl, c);
+        for (int c = 0; c < NC; c++) v[s][c] = val(po + S.off[s], fl, c);
     }
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        if (S.pos[s] >= 0)
+#pragma unroll
+            for (int c = 0; c < NC; c++) sh[c * NP + S.pos[s]] = v[s][c];
 }
 
 // rhoEqn.H:19-33: drho = dt * (-div(rho*U)) (src/ex/div/div.hxx:22-81),
