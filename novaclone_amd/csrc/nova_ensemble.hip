@@ -143,6 +143,12 @@ struct nova_ensemble {
     hipStream_t gstream = nullptr;
     hipGraphExec_t gexec = nullptr;
     double gdt = 0.0;
+    // the size classes' projections run side by side: class i on side[i],
+    // forked from the step's stream after each stage (fork) and joined back
+    // (done[i]) before the next stage; default-priority streams
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> done;
+    hipEvent_t fork = nullptr;
 };
 
 namespace nova {
@@ -161,6 +167,9 @@ void release(nova_ensemble *e)
     if (e->dinst) (void)hipFree(e->dinst);
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->gstream) (void)hipStreamDestroy(e->gstream);
+    for (hipStream_t s : e->side) (void)hipStreamDestroy(s);
+    for (hipEvent_t v : e->done) (void)hipEventDestroy(v);
+    if (e->fork) (void)hipEventDestroy(e->fork);
 }
 
 template <int H>
@@ -186,7 +195,10 @@ int step_once(nova_ensemble *e, hipStream_t st, double dt)
             }
         }
         if (rc) return rc;
-        for (auto &c : e->classes) {
+        const bool fan = !e->side.empty();
+        if (fan) NOVA_HIP(hipEventRecord(e->fork, st));
+        for (size_t ci = 0; ci < e->classes.size(); ci++) {
+            auto &c = e->classes[ci];
             // after stages 1 and 2 only the MHD members project (velocity with
             // dt, field with 1); after stage 3 every member does
             double *region = s == 0 ? c.W1 : (s == 1 ? c.W2 : c.U);
@@ -194,9 +206,18 @@ int step_once(nova_ensemble *e, hipStream_t st, double dt)
             const int B = c.slots() - first;
             if (B == 0) continue;
             const int split = c.nT + c.nM - first;
+            if (fan) {
+                NOVA_HIP(hipStreamWaitEvent(e->side[ci], e->fork, 0));
+                c.ctx->stream = e->side[ci];
+            }
             if ((rc = project_batch(c.ps, region + first * 3 * c.np, 3 * c.np, B, split, dt, 1.0, c.P + first * c.np,
                                     c.np)))
                 return rc;
+            if (fan) {
+                NOVA_HIP(hipEventRecord(e->done[ci], e->side[ci]));
+                NOVA_HIP(hipStreamWaitEvent(st, e->done[ci], 0));
+                c.ctx->stream = st;
+            }
         }
     }
     return NOVA_OK;
@@ -345,6 +366,22 @@ int ensemble_create(const nova_instance_desc *inst, int n, int device, nova_ense
     const int resident = std::max(1, cus) * std::max(1, per_cu);
     e->nblocks = std::min(resident, e->ntiles);
     for (auto &cl : e->classes) cl.ctx->stream = e->stream;
+    if (e->classes.size() > 1) {
+        err = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
+        for (size_t ci = 0; ci < e->classes.size() && err == hipSuccess; ci++) {
+            hipStream_t st = nullptr;
+            hipEvent_t ev = nullptr;
+            err = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+            if (err == hipSuccess) e->side.push_back(st);
+            if (err == hipSuccess) err = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+            if (err == hipSuccess) e->done.push_back(ev);
+        }
+        if (err != hipSuccess) {
+            release(e);
+            delete e;
+            return hip_fail(err, "ensemble side streams");
+        }
+    }
     *out = e;
     return NOVA_OK;
 }
