@@ -205,10 +205,12 @@ int nova_axpby(nova_ctx *ctx, size_t count, double a, const double *x, double b,
  *   NOVA_OP_SQRT      sqrt(a): 1 -> 1 (b unused)
  *   NOVA_OP_OUTER     a * b of vectors: 3, 3 -> 9 (tensor, row-major)
  *   NOVA_OP_ADD_CONST a + cval (nca components; Field - T with cval = -T)
+ *   NOVA_OP_FILL      out = cval: nca (at most 9) components, every stored
+ *                     node (Field = T, Field.hxx:141-305); a and b unused
  * out may alias a or b only for DOT, SQRT, SCALE, DIVIDE and ADD_CONST when
  * it has the components of the aliased operand. */
 enum { NOVA_OP_DOT = 0, NOVA_OP_CROSS = 1, NOVA_OP_SCALE = 2, NOVA_OP_DIVIDE = 3, NOVA_OP_DDOT = 4,
-       NOVA_OP_SQRT = 5, NOVA_OP_OUTER = 6, NOVA_OP_ADD_CONST = 7 };
+       NOVA_OP_SQRT = 5, NOVA_OP_OUTER = 6, NOVA_OP_ADD_CONST = 7, NOVA_OP_FILL = 8 };
 int nova_field_op(nova_ctx *ctx, int op, const double *a, int nca, const double *b, int ncb,
                   const double *cval, double *out);
 /* ex::grad(Field<vector>) -> Field<tensor>, src/ex/grad/grad.cpp:65-102: the

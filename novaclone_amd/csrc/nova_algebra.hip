@@ -69,6 +69,16 @@ __global__ void k_field_op(int op, int64_t n, int64_t np, const double *a, int n
     }
 }
 
+// out = the constant vals over nc components (Field = T)
+struct FillVals {
+    double v[9];
+};
+__global__ void k_fill(int64_t n, int64_t np, int nc, FillVals vals, double *out)
+{
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < nc; c++) out[c * np + q] = vals.v[c];
+}
+
 // The literal vector gradient of src/ex/grad/grad.cpp:65-102 (the
 // half-shifted stencil f(i + l) - f(i - 1 - l)) as a 9-component field,
 // row = component, column = direction.
@@ -153,6 +163,13 @@ int field_op(nova_ctx *ctx, int op, const double *a, int nca, const double *b, i
              double *out)
 {
     const int64_t np = ctx->g.np;
+    if (op == NOVA_OP_FILL) {
+        FillVals v{};
+        for (int c = 0; c < nca; c++) v.v[c] = cval[c];
+        TimedScope ts(ctx, "field_fill");
+        hipLaunchKernelGGL(k_fill, dim3(grid_blocks(np)), dim3(kThreads), 0, ctx->stream, np, np, nca, v, out);
+        return check_launch("field_fill");
+    }
     const double c0 = cval ? cval[0] : 0.0, c1 = cval && nca > 1 ? cval[1] : 0.0, c2 = cval && nca > 2 ? cval[2] : 0.0;
     TimedScope ts(ctx, "field_op");
     hipLaunchKernelGGL(k_field_op, dim3(grid_blocks(np)), dim3(kThreads), 0, ctx->stream, op, np, np, a, nca, b, ncb,

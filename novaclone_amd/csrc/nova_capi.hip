@@ -527,7 +527,7 @@ int nova_axpby(nova_ctx *ctx, size_t count, double a, const double *x, double b,
 int nova_field_op(nova_ctx *ctx, int op, const double *a, int nca, const double *b, int ncb, const double *cval,
                   double *out)
 {
-    NOVA_REQUIRE(ctx && a && out, "bad argument");
+    NOVA_REQUIRE(ctx && out && (a || op == NOVA_OP_FILL), "bad argument");
     bool ok = false;
     switch (op) {
     case NOVA_OP_DOT: ok = b && nca == ncb && (nca == 1 || nca == 3); break;
@@ -538,6 +538,7 @@ int nova_field_op(nova_ctx *ctx, int op, const double *a, int nca, const double 
     case NOVA_OP_SQRT: ok = nca == 1; break;
     case NOVA_OP_OUTER: ok = b && nca == 3 && ncb == 3 && out != a && out != b; break;
     case NOVA_OP_ADD_CONST: ok = cval && (nca == 1 || nca == 3); break;
+    case NOVA_OP_FILL: ok = cval && nca >= 1 && nca <= 9; break;
     default: break;
     }
     NOVA_REQUIRE(ok, "nova_field_op: operator and component counts do not match (see nova_capi.h)");

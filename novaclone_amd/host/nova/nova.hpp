@@ -420,6 +420,8 @@ public:
 template <class T>
 class Field {
     static constexpr int NC = pTraits<T>::nComp;
+    // the host mirror is read and written as NC consecutive doubles per node
+    static_assert(sizeof(T) == NC * sizeof(double), "node types are NC packed doubles");
     Mesh m_;
     std::string name_;
     double *d_ = nullptr;                 // device, compact SoA
@@ -432,12 +434,11 @@ class Field {
         if (hostValid_) return;
         h_.resize((size_t)m_.ni() * m_.nj() * m_.nk());
         const size_t np = h_.size() * NC;
-        std::vector<double> tmp(np);
         double *dpad = padBuffer();
         nova::check(nova_pack_padded(m_.ctx(), d_, NC, dpad), "nova_pack_padded");
-        nova::check(nova_memcpy_d2h(m_.ctx(), tmp.data(), dpad, np), "nova_memcpy_d2h");
+        // the node types are NC consecutive doubles: copy straight into the mirror
+        nova::check(nova_memcpy_d2h(m_.ctx(), (double *)h_.data(), dpad, np), "nova_memcpy_d2h");
         releasePad(dpad);
-        std::memcpy((void *)h_.data(), tmp.data(), np * sizeof(double));
         hostValid_ = true;
     }
     double *padBuffer() const
@@ -528,6 +529,11 @@ public:
         }
     }
     Field(Mesh &m) : Field(m, T(0.0)) {}
+    // an operator's result: storage only, every stored node is written by
+    // the operator before anything reads it (the reference zero-fills its
+    // pool temporaries, reuseTmp.hxx:47, which every operator then overwrites)
+    struct Uninit {};
+    Field(Mesh &m, Uninit) : m_(m) { d_ = m_.take(NC); }
     Field(const Field &f) : m_(f.m_), name_(f.name_)
     {
         d_ = m_.take(NC);
@@ -599,11 +605,10 @@ public:
     void operator=(const std::shared_ptr<Field> &f) { *this = *f; }
     void operator=(const T &v)
     {
-        // constant fill through the host mirror of a single node pattern
-        std::vector<double> tmp((size_t)NC * m_.points());
-        for (int c = 0; c < NC; c++)
-            for (int64_t q = 0; q < m_.points(); q++) tmp[(size_t)c * m_.points() + q] = comp(v, c);
-        nova::check(nova_memcpy_h2d(m_.ctx(), d_, tmp.data(), tmp.size()), "fill");
+        // constant fill on the device
+        double cv[NC];
+        for (int c = 0; c < NC; c++) cv[c] = comp(v, c);
+        nova::check(nova_field_op(m_.ctx(), NOVA_OP_FILL, nullptr, NC, nullptr, 0, cv, d_), "fill");
         hostValid_ = false;
         hostDirty_ = false;
     }
@@ -633,7 +638,7 @@ namespace nova {
 template <class T>
 FieldPtr<T> tmp(const Mesh &m)
 {
-    return std::make_shared<Field<T>>(const_cast<Mesh &>(m), T(0.0));
+    return std::make_shared<Field<T>>(const_cast<Mesh &>(m), typename Field<T>::Uninit{});
 }
 template <class T>
 FieldPtr<T> lincomb(scalar a, const Field<T> &x, scalar b, const Field<T> *y)
