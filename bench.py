@@ -180,6 +180,41 @@ def adjoint_ratio(nv, steps: int, every: int) -> dict:
     tb = time.perf_counter() - t0
     gn = nv.weighted_dot(G, g0, g0) ** 0.5
     A.close()
+    # the gradient against central differences (eps = 1e-6) on 16 seeded
+    # directions at this size (untimed): J(U0 + eps d) from the replayed
+    # forward solve, directions with max |d| = 1 and the wall-normal
+    # component zero on the walls; the bound allows the difference quotient
+    # its roundoff, 4 u J / eps (u = 2^-53)
+    # (and the fourth-order quotient at eps4 = 1e-4, whose roundoff, about
+    # u J / eps4, leaves the 1e-8 agreement measurable)
+    eps, eps4, u = 1e-6, 1e-4, 2.0 ** -53
+    fd = {"directions": 16, "eps": eps, "rel_err": [], "within_bound": True,
+          "bound": "|fd - ad| <= 1e-8 |ad| + 4 u J / eps", "eps4": eps4, "rel_err4": []}
+
+    def J_at(d, e):
+        U = (U0 + e * d).contiguous()
+        S.step(dt, steps, U, p)
+        return nv.enstrophy(G, U)
+
+    for seed in range(16):
+        gen = torch.Generator(device="cuda").manual_seed(500 + seed)
+        d = torch.randn(U0.shape, generator=gen, dtype=torch.float64, device="cuda")
+        d[1, :, 0, :] = 0.0
+        d[1, :, -1, :] = 0.0
+        d = (d / d.abs().max()).contiguous()
+        fdq = (J_at(d, eps) - J_at(d, -eps)) / (2 * eps)
+        fd4 = (-J_at(d, 2 * eps4) + 8 * J_at(d, eps4) - 8 * J_at(d, -eps4) + J_at(d, -2 * eps4)) / (12 * eps4)
+        ad = nv.weighted_dot(G, g0, d)
+        fd["rel_err"].append(float(f"{abs(fdq - ad) / abs(ad):.3e}"))
+        fd["rel_err4"].append(float(f"{abs(fd4 - ad) / abs(ad):.3e}"))
+        if abs(fdq - ad) > 1e-8 * abs(ad) + 4 * u * J / eps:
+            fd["within_bound"] = False
+        if seed % 4 == 3:
+            log(f"bench: adjoint FD check, {seed + 1} of 16 directions")
+    fd["max_rel_err"] = max(fd["rel_err"])
+    fd["max_rel_err4"] = max(fd["rel_err4"])
+    log(f"bench: adjoint gradient vs central differences, max rel err {fd['max_rel_err']} (eps {eps}, within "
+        f"bound {fd['within_bound']}), {fd['max_rel_err4']} (fourth order, eps {eps4})")
     return {"ratio": round(tb / tf, 3), "forward_s": round(tf, 3), "adjoint_s": round(tb, 3),
             "checkpoint_forward_s": round(tcf, 3), "gradient_ratio": round((tcf + tb) / tf, 3),
             "ratios": "ratio = adjoint (reverse) sweep / forward solve; gradient_ratio = (checkpointing forward "
@@ -188,7 +223,7 @@ def adjoint_ratio(nv, steps: int, every: int) -> dict:
                         f"{steps} SSP-RK3 steps, checkpoints every {every}; gradient of final enstrophy",
             "resident_steps": resident,
             "recomputed_steps": steps - resident,
-            "J": J, "grad_norm": gn, "tuning": tuned}
+            "J": J, "grad_norm": gn, "fd_check": fd, "tuning": tuned}
 
 
 def wc_bench(nv, n: int, steps: int) -> dict:

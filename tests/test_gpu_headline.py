@@ -129,3 +129,38 @@ def test_compact_step_at_256_matches_oracle(nv, orc):
     e = rel_l2(host(U), O.interior(Uo))
     print("compact 256^3 rel L2", e, flush=True)
     assert e <= RTOL
+
+
+def test_ot_mhd_step_at_256_matches_oracle(nv, orc):
+    """The OT-type MHD configuration at its full size: 256^3 periodic,
+    u = (-sin y, sin x, 0), B = (-sin y, sin 2x, 0) plus 1e-3 Gaussian noise
+    (seed 3), both projected once, then one SSP-RK3 step with both fields
+    projected every stage (Re = Rm = 1000, dt = 1e-3) under the autotuned
+    launch shapes, against the oracle; and the OT diagnostics of the result."""
+    import torch
+    N = (256, 256, 256)
+    G, O = pair(nv, orc, N, (TWO_PI,) * 3)
+    x = [np.arange(O.stored[a]) * TWO_PI / N[a] for a in range(3)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    rng = np.random.default_rng(3)
+    cu = np.stack([-np.sin(Y), np.sin(X), 0 * X]) + 1e-3 * rng.standard_normal((3, *O.stored))
+    cb = np.stack([-np.sin(Y), np.sin(2 * X), 0 * X]) + 1e-3 * rng.standard_normal((3, *O.stored))
+    del X, Y, Z
+    OP = orc.OPoisson(O)
+    Uo, Bo = O.from_compact(cu), O.from_compact(cb)
+    OP.project(Uo, 1.0)
+    OP.project(Bo, 1.0)
+    cu, cb = O.interior(Uo), O.interior(Bo)
+    U, B, p, pB = dev(cu), dev(cb), G.field(1), G.field(1)
+    S = nv.Solver(nv.Poisson(G), nv.MODEL_MHD, nv.RK_SSP3, nu=1e-3, eta=1e-3, graph=True)
+    print(S.autotune(U, B, reps=2).splitlines()[-2:], flush=True)
+    S.step(1e-3, 1, U, p, B, pB)
+    torch.cuda.synchronize()
+    po, pBo = O.field(1), O.field(1)
+    OP.mhd_step(Uo, Bo, po, pBo, 1e-3, 1e-3, 1e-3, 1)
+    eu, eb = rel_l2(host(U), O.interior(Uo)), rel_l2(host(B), O.interior(Bo))
+    print("OT MHD 256^3 one step rel L2", eu, eb, flush=True)
+    assert eu <= RTOL and eb <= RTOL
+    d = nv.mhd_diag(G, U, B, 1e-3, 1e-3)
+    do = O.mhd_diag(Uo, Bo, 1e-3, 1e-3)
+    assert np.allclose(d[:7], do[:7], rtol=1e-12, atol=1e-14)
