@@ -140,8 +140,11 @@ __device__ __attribute__((always_inline)) inline void stage_tile(const Grid &g, 
     auto ldp = [&](const double *b, int c, int64_t po, uint32_t o) -> double {
         return *(const double *)((const char *)(b + c * g.np + po) + (uint32_t)(o * 8u));
     };
+    // results are stored non-temporally: a stage writes its whole output
+    // once (hundreds of MB, more than the caches hold), and streaming it past
+    // L2 leaves the cache to the halo planes that neighbouring tiles re-read
     auto stp = [&](double *b, int c, int64_t po, uint32_t o, double v) {
-        *(double *)((char *)(b + c * g.np + po) + (uint32_t)(o * 8u)) = v;
+        __builtin_nontemporal_store(v, (double *)((char *)(b + c * g.np + po) + (uint32_t)(o * 8u)));
     };
 
     // own column
