@@ -1,11 +1,14 @@
 // nova_ops.hip -- operator-level kernels of the NOVA hot path on gfx950.
 //
 // These are the drop-in counterparts of the reference's src/ex operators,
-// its BC fills and its diagnostics, on the compact layout.  One thread per
-// output node; stencil neighbours are loaded through the boundary index
-// maps, so the k-neighbours of a 64-wide wave share cache lines and the
-// j/i neighbours hit L2.  The fused time-step kernels (nova_step.hip) carry
-// the bandwidth-critical work; these serve the per-operator API.
+// its BC fills and its diagnostics, on the compact layout.  The operators
+// the reference apps call every stage (ex::div of a product, ex::div,
+// ex::grad, ex::laplacian) are streamed kernels (k_sop_stream: LDS plane
+// tiles and an i register queue); the others run one thread per output
+// node, with stencil neighbours loaded through the boundary index maps, so
+// the k-neighbours of a 64-wide wave share cache lines and the j/i
+// neighbours hit L2.  The fused time-step kernels (nova_step.hip) carry the
+// bandwidth-critical work of the library's own time loops.
 #include <mutex>
 #include <string>
 
@@ -40,101 +43,244 @@ __device__ inline void offs(int d, int q, int &di, int &dj, int &dk)
     dk = d == 2 ? q : 0;
 }
 
-// ex::div(U, phi): mean of the conservative and the expanded forms,
-// 0.5 * sum_d sum_l a_l [ (U_d phi)^+ - (U_d phi)^- + U_d (phi^+ - phi^-)
-// + phi (U_d^+ - U_d^-) ] (src/ex/div/div.hxx:101-169).
-__global__ void k_div_adv(Grid g, const double *__restrict__ U, const double *__restrict__ phi,
-                          int nc, double *__restrict__ out)
+// ---- streamed operators: ex::div(U, phi), ex::div(U), ex::grad(p),
+// ex::laplacian(nu, f) ----
+//
+// A 64 x 4 block owns a (k, j) tile and walks i over a chunk of planes.
+// Every input component the operator reads is staged plane by plane in LDS
+// with its j and k halos (H nodes each side); the i-neighbours of the
+// block's own column stay in a register queue of planes i-H .. i+H, and the
+// loads of the next two planes are in flight while a plane is computed.
+// Each input value is read from global memory about once (plus the halo
+// ring and the 2H warm-up planes of a chunk), and every load addresses a
+// precomputed column plus a plane offset: the boundary index maps are
+// evaluated once per column and once per plane, not per neighbour.
+// Each operator's arithmetic is written term for term as the reference's
+// (cited at each case), in the same order for every node.
+enum { SOP_DIV_ADV = 0, SOP_LAPLACIAN = 1, SOP_DIV = 2, SOP_GRAD = 3 };
+constexpr int kOK = 64, kOJ = 4;
+
+// staged components: DIV_ADV U_x, U_y, U_z then phi's (none when phi is U);
+// LAPLACIAN f's; DIV U's; GRAD p
+template <int OP, int NCP, bool SHARED>
+constexpr int sop_staged()
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k >= g.s[2] || j >= g.s[1]) return;
-    const double zi = g.zinv[k];
-    for (int c = 0; c < nc; c++) {
-        const double pc = ld(phi, g, nc, c, i, j, k);
-        double acc = 0.0;
-        for (int d = 0; d < 3; d++) {
-            const double u0 = ld(U, g, 3, d, i, j, k);
-            double ad = 0.0;
-            for (int l = 0; l < g.h; l++) {
-                int di, dj, dk;
-                offs(d, l + 1, di, dj, dk);
-                const double up = ld(U, g, 3, d, i + di, j + dj, k + dk);
-                const double um = ld(U, g, 3, d, i - di, j - dj, k - dk);
-                const double pp = ld(phi, g, nc, c, i + di, j + dj, k + dk);
-                const double pm = ld(phi, g, nc, c, i - di, j - dj, k - dk);
-                ad += g.a[d][l] * ((pp * up - pm * um) + u0 * (pp - pm) + pc * (up - um));
+    return OP == SOP_DIV_ADV ? (SHARED ? 3 : 3 + NCP) : (OP == SOP_LAPLACIAN ? NCP : (OP == SOP_DIV ? 3 : 1));
+}
+
+struct SopArgs {
+    const double *src[6];  // staged component s
+    // bit d of flip[s]: component s changes sign when loaded through a
+    // reflection across the walls normal to d (a vector component odd there)
+    unsigned flip[6];
+    double *out;
+    double nu;     // LAPLACIAN
+    double scale;  // DIV
+    int ichunk, nkt, njt, nic;
+};
+
+template <int H, int OP, int NCP, bool SHARED, bool WALLS>
+__global__ void __launch_bounds__(kOK *kOJ) k_sop_stream(Grid g, SopArgs a)
+{
+    constexpr int NS = sop_staged<OP, NCP, SHARED>();
+    constexpr int QW = 2 * H + 1, LJ = kOJ + 2 * H, LK = kOK + 2 * H;
+    constexpr int NHALO = 2 * H * (kOK + kOJ);
+    constexpr int NSLOT = (NHALO + kOK * kOJ - 1) / (kOK * kOJ);
+    __shared__ double sh[NS][LJ][LK];
+
+    // tiles: k fastest, then j, then the i chunk; blocks go to the eight
+    // XCDs in turn, so XCD x is given the x-th eighth of the tiles and the
+    // tiles that share halo rows share its L2
+    const int T = a.nkt * a.njt * a.nic;
+    int bt = blockIdx.x;
+    if (T % 8 == 0) bt = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
+    const int kt = bt % a.nkt, jt = (bt / a.nkt) % a.njt, ic = bt / (a.nkt * a.njt);
+    const int tk = threadIdx.x % kOK, tj = threadIdx.x / kOK;
+    const int k = kt * kOK + tk, j = jt * kOJ + tj;
+    const bool valid = k < g.s[2] && j < g.s[1];
+    const int i0 = ic * a.ichunk;
+    const int i1 = i0 + a.ichunk < g.s[0] ? i0 + a.ichunk : g.s[0];
+    const int64_t plane = (int64_t)g.s[1] * g.s[2];
+
+    // own column and its reflection flags (bit 1: j, bit 2: k)
+    int fj, fk;
+    const int jm = bmap(j, g.n[1], g.wall[1], fj);
+    const int km = bmap(k, g.n[2], g.wall[2], fk);
+    const int64_t col = (int64_t)jm * g.s[2] + km;
+    const unsigned cfl = (fj ? 2u : 0u) | (fk ? 4u : 0u);
+    // halo slots of this thread: LDS row and column, source column, flags
+    int hr[NSLOT], hc[NSLOT];
+    int64_t hsrc[NSLOT];
+    unsigned hfl[NSLOT];
+    bool hon[NSLOT];
+#pragma unroll
+    for (int sl = 0; sl < NSLOT; sl++) {
+        const int e = threadIdx.x + sl * kOK * kOJ;
+        int jj = 0, kk = 0;
+        hon[sl] = e < NHALO;
+        hr[sl] = hc[sl] = 0;
+        if (e < 2 * H * kOK) {  // j halo rows
+            const int rr = e / kOK, cc = e % kOK;
+            hr[sl] = rr < H ? rr : kOJ + rr;
+            hc[sl] = cc + H;
+            jj = jt * kOJ + (rr < H ? rr - H : kOJ + rr - H);
+            kk = kt * kOK + cc;
+        } else if (e < NHALO) {  // k halo columns
+            const int e2 = e - 2 * H * kOK;
+            const int rr = e2 / (2 * H), c2 = e2 % (2 * H);
+            hr[sl] = rr + H;
+            hc[sl] = c2 < H ? c2 : kOK + c2;
+            jj = jt * kOJ + rr;
+            kk = kt * kOK + (c2 < H ? c2 - H : kOK + c2 - H);
+        }
+        int f1, f2;
+        const int jm2 = bmap(jj, g.n[1], g.wall[1], f1);
+        const int km2 = bmap(kk, g.n[2], g.wall[2], f2);
+        hsrc[sl] = (int64_t)jm2 * g.s[2] + km2;
+        hfl[sl] = (f1 ? 2u : 0u) | (f2 ? 4u : 0u);
+    }
+    // value of component s loaded through the reflections in flags fl
+    auto sgn = [&](int s, unsigned fl, double v) -> double {
+        if constexpr (WALLS) return (__popc(fl & a.flip[s]) & 1) ? -v : v;
+        else return v;
+    };
+    auto plane_of = [&](int ii, unsigned &fl) -> int64_t {
+        int fi;
+        const int im = bmap(ii, g.n[0], g.wall[0], fi);
+        fl = fi ? 1u : 0u;
+        return (int64_t)im * plane;
+    };
+
+    // loads of plane p: the queue entry of plane p + H (own column) and the
+    // halo of plane p, kept as loaded (signs applied when staged)
+    struct Pf {
+        double nq[NS];
+        double hv[NSLOT][NS];
+        unsigned fq, fh;
+    };
+    auto load_pf = [&](Pf &P, int p, bool queue) {
+        if (queue) {
+            const int64_t po = plane_of(p + H, P.fq);
+#pragma unroll
+            for (int s = 0; s < NS; s++) P.nq[s] = a.src[s][po + col];
+        }
+        const int64_t pp = plane_of(p, P.fh);
+#pragma unroll
+        for (int sl = 0; sl < NSLOT; sl++)
+#pragma unroll
+            for (int s = 0; s < NS; s++) P.hv[sl][s] = hon[sl] ? a.src[s][pp + hsrc[sl]] : 0.0;
+    };
+
+    double q[NS][QW];
+#pragma unroll
+    for (int o = 0; o < QW; o++) {
+        unsigned fl;
+        const int64_t po = plane_of(i0 - H + o, fl);
+#pragma unroll
+        for (int s = 0; s < NS; s++) q[s][o] = sgn(s, fl | cfl, a.src[s][po + col]);
+    }
+    // neighbour of the node at offset o (o = -H .. H) along direction d
+    auto nb = [&](int s, int d, int o) -> double {
+        return d == 0 ? q[s][H + o] : (d == 1 ? sh[s][tj + H + o][tk + H] : sh[s][tj + H][tk + H + o]);
+    };
+
+    auto body = [&](int i, Pf &Cur, Pf &Nxt) {
+#pragma unroll
+        for (int s = 0; s < NS; s++) sh[s][tj + H][tk + H] = q[s][H];
+#pragma unroll
+        for (int sl = 0; sl < NSLOT; sl++)
+            if (hon[sl])
+#pragma unroll
+                for (int s = 0; s < NS; s++) sh[s][hr[sl]][hc[sl]] = sgn(s, Cur.fh | hfl[sl], Cur.hv[sl][s]);
+        __syncthreads();
+        if (i + 2 < i1) load_pf(Cur, i + 2, true);
+        if (valid) {
+            const int64_t q0 = (int64_t)i * plane + col;
+            if constexpr (OP == SOP_DIV_ADV) {
+                // src/ex/div/div.hxx:101-169: the mean of the conservative
+                // and the expanded forms, per direction and stencil offset
+                // (U_d phi)^+ - (U_d phi)^- + U_d (phi^+ - phi^-) + phi (U_d^+ - U_d^-)
+                const double zi = g.zinv[k];
+#pragma unroll
+                for (int c = 0; c < NCP; c++) {
+                    const int sp = SHARED ? c : 3 + c;
+                    const double pc = q[sp][H];
+                    double acc = 0.0;
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const double u0 = q[d][H];
+                        double ad = 0.0;
+#pragma unroll
+                        for (int l = 0; l < H; l++) {
+                            const double up = nb(d, d, l + 1), um = nb(d, d, -l - 1);
+                            const double pp = nb(sp, d, l + 1), pm = nb(sp, d, -l - 1);
+                            ad += g.a[d][l] * ((pp * up - pm * um) + u0 * (pp - pm) + pc * (up - um));
+                        }
+                        acc += d == 2 ? ad * zi : ad;
+                    }
+                    __builtin_nontemporal_store(0.5 * acc, a.out + c * g.np + q0);
+                }
+            } else if constexpr (OP == SOP_LAPLACIAN) {
+                // src/ex/laplacian/laplacian.hxx:140-188: the second
+                // derivatives, z through 1/dzdxi^2, minus nu d2zdxi2/dzdxi^3
+                // times the z first derivative
+#pragma unroll
+                for (int c = 0; c < NCP; c++) {
+                    const double f0 = q[c][H];
+                    double acc = 0.0, dz1 = 0.0;
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        double ad = 0.0;
+#pragma unroll
+                        for (int l = 0; l < H; l++) {
+                            const double fp = nb(c, d, l + 1), fm = nb(c, d, -l - 1);
+                            ad += g.b[d][l] * ((fp + fm) - 2.0 * f0);
+                            if (d == 2) dz1 += g.a[2][l] * (fp - fm);
+                        }
+                        acc += d == 2 ? ad * g.zlap[k] : ad;
+                    }
+                    __builtin_nontemporal_store(a.nu * acc - a.nu * g.zmet[k] * dz1, a.out + c * g.np + q0);
+                }
+            } else if constexpr (OP == SOP_DIV) {
+                // src/ex/div/div.hxx:37-77
+                double acc = 0.0;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    double ad = 0.0;
+#pragma unroll
+                    for (int l = 0; l < H; l++) ad += g.a[d][l] * (nb(d, d, l + 1) - nb(d, d, -l - 1));
+                    acc += d == 2 ? ad * g.zinv[k] : ad;
+                }
+                __builtin_nontemporal_store(acc * a.scale, a.out + q0);
+            } else {
+                // src/ex/grad/grad.cpp:33-58
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    double ad = 0.0;
+#pragma unroll
+                    for (int l = 0; l < H; l++) ad += g.a[d][l] * (nb(0, d, l + 1) - nb(0, d, -l - 1));
+                    __builtin_nontemporal_store(d == 2 ? ad * g.zinv[k] : ad, a.out + d * g.np + q0);
+                }
             }
-            acc += d == 2 ? ad * zi : ad;
         }
-        out[c * g.np + lin(g, i, j, k)] = 0.5 * acc;
-    }
-}
-
-// ex::div(U) * scale (src/ex/div/div.hxx:37-77).
-__global__ void k_div(Grid g, const double *__restrict__ U, double scale, double *__restrict__ out)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k >= g.s[2] || j >= g.s[1]) return;
-    double acc = 0.0;
-    for (int d = 0; d < 3; d++) {
-        double ad = 0.0;
-        for (int l = 0; l < g.h; l++) {
-            int di, dj, dk;
-            offs(d, l + 1, di, dj, dk);
-            ad += g.a[d][l] * (ld(U, g, 3, d, i + di, j + dj, k + dk) - ld(U, g, 3, d, i - di, j - dj, k - dk));
-        }
-        acc += d == 2 ? ad * g.zinv[k] : ad;
-    }
-    out[lin(g, i, j, k)] = acc * scale;
-}
-
-// ex::grad(p) (src/ex/grad/grad.cpp:33-58).
-__global__ void k_grad(Grid g, const double *__restrict__ p, double *__restrict__ out)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k >= g.s[2] || j >= g.s[1]) return;
-    for (int d = 0; d < 3; d++) {
-        double ad = 0.0;
-        for (int l = 0; l < g.h; l++) {
-            int di, dj, dk;
-            offs(d, l + 1, di, dj, dk);
-            ad += g.a[d][l] * (ld(p, g, 1, 0, i + di, j + dj, k + dk) - ld(p, g, 1, 0, i - di, j - dj, k - dk));
-        }
-        out[d * g.np + lin(g, i, j, k)] = d == 2 ? ad * g.zinv[k] : ad;
-    }
-}
-
-// ex::laplacian(nu, f) with the z-metric correction
-// (src/ex/laplacian/laplacian.hxx:140-188).
-__global__ void k_laplacian(Grid g, double nu, const double *__restrict__ f, int nc,
-                            double *__restrict__ out)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k >= g.s[2] || j >= g.s[1]) return;
-    for (int c = 0; c < nc; c++) {
-        const double f0 = ld(f, g, nc, c, i, j, k);
-        double acc = 0.0, dz1 = 0.0;
-        for (int d = 0; d < 3; d++) {
-            double ad = 0.0;
-            for (int l = 0; l < g.h; l++) {
-                int di, dj, dk;
-                offs(d, l + 1, di, dj, dk);
-                const double fp = ld(f, g, nc, c, i + di, j + dj, k + dk);
-                const double fm = ld(f, g, nc, c, i - di, j - dj, k - dk);
-                ad += g.b[d][l] * ((fp + fm) - 2.0 * f0);
-                if (d == 2) dz1 += g.a[2][l] * (fp - fm);
+        __syncthreads();
+        if (i + 1 < i1) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+#pragma unroll
+                for (int o = 0; o < 2 * H; o++) q[s][o] = q[s][o + 1];
+                q[s][2 * H] = sgn(s, Nxt.fq | cfl, Nxt.nq[s]);
             }
-            acc += d == 2 ? ad * g.zlap[k] : ad;
         }
-        out[c * g.np + lin(g, i, j, k)] = nu * acc - nu * g.zmet[k] * dz1;
+    };
+    Pf A, B;
+    load_pf(A, i0, false);
+    A.fq = 0;
+    B.fq = B.fh = 0;
+    if (i0 + 1 < i1) load_pf(B, i0 + 1, true);
+    for (int i = i0; i < i1; i += 2) {
+        body(i, A, B);
+        if (i + 1 < i1) body(i + 1, B, A);
     }
 }
 
@@ -522,22 +668,89 @@ dim3 pt_grid(const Grid &g) { return dim3((g.s[2] + 63) / 64, (g.s[1] + 3) / 4, 
 
 }  // namespace
 
+namespace {
+
+// the tiling of a streamed-operator launch: i chunks enough for about 2048
+// blocks (eight per CU), at least 8 planes (and 4H) long so that the 2H
+// warm-up planes stay a small part of a chunk
+template <int H>
+void sop_tiling(const Grid &g, SopArgs &a)
+{
+    a.nkt = (g.s[2] + kOK - 1) / kOK;
+    a.njt = (g.s[1] + kOJ - 1) / kOJ;
+    const int tiles = a.nkt * a.njt;
+    const int nic = (2048 + tiles - 1) / tiles;
+    const int minchunk = 4 * H > 8 ? 4 * H : 8;
+    int ichunk = (g.s[0] + nic - 1) / nic;
+    if (ichunk < minchunk) ichunk = minchunk;
+    if (ichunk > g.s[0]) ichunk = g.s[0];
+    a.ichunk = ichunk;
+    a.nic = (g.s[0] + ichunk - 1) / ichunk;
+}
+
+template <int H, int OP, int NCP, bool SHARED>
+int launch_sop_h(nova_ctx *ctx, SopArgs &a, const char *name)
+{
+    const Grid &g = ctx->g;
+    sop_tiling<H>(g, a);
+    const dim3 grid(a.nkt * a.njt * a.nic);
+    if (g.wall[0] || g.wall[1] || g.wall[2])
+        hipLaunchKernelGGL((k_sop_stream<H, OP, NCP, SHARED, true>), grid, dim3(kOK * kOJ), 0, ctx->stream, g, a);
+    else
+        hipLaunchKernelGGL((k_sop_stream<H, OP, NCP, SHARED, false>), grid, dim3(kOK * kOJ), 0, ctx->stream, g, a);
+    return check_launch(name);
+}
+
+template <int OP, int NCP, bool SHARED = false>
+int launch_sop(nova_ctx *ctx, SopArgs &a, const char *name)
+{
+    switch (ctx->g.h) {
+    case 1: return launch_sop_h<1, OP, NCP, SHARED>(ctx, a, name);
+    case 2: return launch_sop_h<2, OP, NCP, SHARED>(ctx, a, name);
+    case 3: return launch_sop_h<3, OP, NCP, SHARED>(ctx, a, name);
+    case 4: return launch_sop_h<4, OP, NCP, SHARED>(ctx, a, name);
+    }
+    return fail(NOVA_ERR_ARG, "stencil half-width outside 1..4");
+}
+
+// staged component s = component c of an nc-component field at f: its
+// reflection signs are those of ld()
+void sop_src(const Grid &g, SopArgs &a, int s, const double *f, int nc, int c)
+{
+    a.src[s] = f + c * g.np;
+    a.flip[s] = 0;
+    if (nc == 3)
+        for (int d = 0; d < 3; d++)
+            if ((g.odd[d] >> c) & 1u) a.flip[s] |= 1u << d;
+}
+
+}  // namespace
+
 int launch_div_adv(nova_ctx *ctx, const double *U, const double *phi, int nc, double *out)
 {
-    hipLaunchKernelGGL(k_div_adv, pt_grid(ctx->g), pt_block(), 0, ctx->stream, ctx->g, U, phi, nc, out);
-    return check_launch("div_adv");
+    SopArgs a{};
+    a.out = out;
+    for (int d = 0; d < 3; d++) sop_src(ctx->g, a, d, U, 3, d);
+    if (phi == U && nc == 3) return launch_sop<SOP_DIV_ADV, 3, true>(ctx, a, "div_adv");
+    for (int c = 0; c < nc; c++) sop_src(ctx->g, a, 3 + c, phi, nc, c);
+    return nc == 3 ? launch_sop<SOP_DIV_ADV, 3>(ctx, a, "div_adv") : launch_sop<SOP_DIV_ADV, 1>(ctx, a, "div_adv");
 }
 
 int launch_div(nova_ctx *ctx, const double *U, double scale, double *out)
 {
-    hipLaunchKernelGGL(k_div, pt_grid(ctx->g), pt_block(), 0, ctx->stream, ctx->g, U, scale, out);
-    return check_launch("div");
+    SopArgs a{};
+    a.out = out;
+    a.scale = scale;
+    for (int d = 0; d < 3; d++) sop_src(ctx->g, a, d, U, 3, d);
+    return launch_sop<SOP_DIV, 3>(ctx, a, "div");
 }
 
 int launch_grad(nova_ctx *ctx, const double *p, double *out)
 {
-    hipLaunchKernelGGL(k_grad, pt_grid(ctx->g), pt_block(), 0, ctx->stream, ctx->g, p, out);
-    return check_launch("grad");
+    SopArgs a{};
+    a.out = out;
+    sop_src(ctx->g, a, 0, p, 1, 0);
+    return launch_sop<SOP_GRAD, 1>(ctx, a, "grad");
 }
 
 int launch_laplacian_var(nova_ctx *ctx, const double *nu, const double *f, int nc, double *out)
@@ -548,8 +761,11 @@ int launch_laplacian_var(nova_ctx *ctx, const double *nu, const double *f, int n
 
 int launch_laplacian(nova_ctx *ctx, double nu, const double *f, int nc, double *out)
 {
-    hipLaunchKernelGGL(k_laplacian, pt_grid(ctx->g), pt_block(), 0, ctx->stream, ctx->g, nu, f, nc, out);
-    return check_launch("laplacian");
+    SopArgs a{};
+    a.out = out;
+    a.nu = nu;
+    for (int c = 0; c < nc; c++) sop_src(ctx->g, a, c, f, nc, c);
+    return nc == 3 ? launch_sop<SOP_LAPLACIAN, 3>(ctx, a, "laplacian") : launch_sop<SOP_LAPLACIAN, 1>(ctx, a, "laplacian");
 }
 
 int launch_curl(nova_ctx *ctx, const double *U, int literal, double *out, int pseudo)

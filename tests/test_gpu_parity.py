@@ -70,6 +70,10 @@ CASES = [
     dict(N=(12, 12, 16), L=(TWO_PI, TWO_PI, TWO_PI), m=4, wall_y=False),
     dict(N=(16, 16, 16), L=(1.0, 1.0, 1.0), m=8, wall_y=False),
     dict(N=(16, 12, 16), L=(2.0, 1.0, 2.0), m=6, wall_y=True),
+    # partial (k, j) tiles of the streamed operators (70 = 64 + 6 k nodes,
+    # 13 = 3 x 4 + 1 j rows) and i chunks of 8 planes with a short last one
+    dict(N=(37, 13, 70), L=(1.0, 0.5, 2.0), m=6, wall_y=False),
+    dict(N=(21, 10, 66), L=(1.0, 1.0, 1.0), m=2, wall_y=True),
 ]
 
 
@@ -86,6 +90,7 @@ def test_operators_match_oracle(nv, orc, case):
     chk = {
         "div_adv(U,s)": (nv.ex.div(G, U, s), O.div_adv(pu, ps)),
         "div_adv(U,V)": (nv.ex.div(G, U, V), O.div_adv(pu, pv)),
+        "div_adv(U,U)": (nv.ex.div(G, U, U), O.div_adv(pu, pu)),
         "div(U)": (nv.ex.div(G, U), O.div(pu)),
         "grad(s)": (nv.ex.grad(G, s), O.grad(ps)),
         "lap(s)": (nv.ex.laplacian(G, 0.3, s), O.laplacian(0.3, ps)),
@@ -111,8 +116,11 @@ def test_laplacians_on_a_stretched_grid_match_oracle(nv, orc):
     V, nu = dev(cv), dev(cn)
     e1 = rel_l2(host(nv.ex.laplacian(G, nu, V)), O.interior(O.laplacian_var(pn, pv)))
     e2 = rel_l2(host(nv.ex.laplacian(G, 0.3, V)), O.interior(O.laplacian(0.3, pv)))
-    print("stretched lap(nu,V)", e1, "lap(V)", e2, flush=True)
-    assert e1 <= RTOL and e2 <= RTOL
+    # the 1/dzdxi factor of the z first derivatives (div.hxx:71-72, :135-163)
+    e3 = rel_l2(host(nv.ex.div(G, V, V)), O.interior(O.div_adv(pv, pv)))
+    e4 = rel_l2(host(nv.ex.div(G, V)), O.interior(O.div(pv)))
+    print("stretched lap(nu,V)", e1, "lap(V)", e2, "div_adv(V,V)", e3, "div(V)", e4, flush=True)
+    assert e1 <= RTOL and e2 <= RTOL and e3 <= RTOL and e4 <= RTOL
 
 
 @pytest.mark.parametrize("wall_y", [False, True])
