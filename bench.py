@@ -308,7 +308,9 @@ def dropin_bench(n: int, steps: int) -> dict:
     drop-in API (novaclone_amd/host/apps/tg_poisson.cpp, the reference app's
     per-operator expressions and its host kinetic-energy loop), run as a
     child process at n^3; the driver reports the device work per step and the
-    host loop per step, averaged over the steps after the first."""
+    host loop per step (of which the host mirror's download of U, into
+    page-locked memory, on its own), averaged over the steps after the
+    first."""
     import subprocess
     import tempfile
     exe = ROOT / "novaclone_amd" / "host" / "bin" / "tg_poisson"
@@ -329,6 +331,7 @@ def dropin_bench(n: int, steps: int) -> dict:
                         f"from ex::div, ex::laplacian and Field algebra, one projection per step) and its host "
                         f"kinetic-energy loop over the host mirror, {steps} steps",
             "device_ms_per_step": round(dev, 3), "host_ke_ms_per_step": round(ke, 3),
+            "mirror_download_ms_per_step": round(f.get("mirror_download_ms_per_step", float("nan")), 3),
             "ms_per_step": round(dev + ke, 3),
             "value": round(n ** 3 / ((dev + ke) * 1e-3) / 1e6, 1),
             "device_value": round(n ** 3 / (dev * 1e-3) / 1e6, 1), "unit": "Mpoint-steps/s"}

@@ -159,6 +159,12 @@ int nova_alloc(nova_ctx *ctx, int ncomp, double **dptr);
 int nova_free(nova_ctx *ctx, double *dptr);
 int nova_memcpy_h2d(nova_ctx *ctx, double *dst, const double *src, size_t count);
 int nova_memcpy_d2h(nova_ctx *ctx, double *dst, const double *src, size_t count);
+/* Page-locked host memory for buffers that cross to and from the device
+ * (the C++ layer's host mirrors, which stand in for the reference's host
+ * arrays, src/Field/Field.h:62): nova_memcpy_* from and to it runs at
+ * the link's rate without a staging copy.  Freed with nova_host_free. */
+int nova_host_alloc(size_t bytes, void **ptr);
+int nova_host_free(void *ptr);
 
 /* ---- reference-layout boundary (Field::correctBoundaryConditions,
  * src/Field/Field.hxx:310-401, with periodicBC/dirichletBC/neumannBC) ---- */

@@ -81,6 +81,8 @@ _SIGS = {
     "nova_free": [_P, _P],
     "nova_memcpy_h2d": [_P, _P, _P, C.c_size_t],
     "nova_memcpy_d2h": [_P, _P, _P, C.c_size_t],
+    "nova_host_alloc": [C.c_size_t, C.POINTER(_P)],
+    "nova_host_free": [_P],
     "nova_pack_padded": [_P, _P, _I, _P],
     "nova_unpack_padded": [_P, _P, _I, _P],
     "nova_bc_fill_padded": [_P, _P, _I, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), _DP],

@@ -452,6 +452,20 @@ int nova_memcpy_d2h(nova_ctx *ctx, double *dst, const double *src, size_t count)
     return NOVA_OK;
 }
 
+int nova_host_alloc(size_t bytes, void **ptr)
+{
+    NOVA_REQUIRE(ptr && bytes > 0, "bad argument");
+    *ptr = nullptr;
+    NOVA_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+    return NOVA_OK;
+}
+
+int nova_host_free(void *ptr)
+{
+    if (ptr) NOVA_HIP(hipHostFree(ptr));
+    return NOVA_OK;
+}
+
 int nova_pack_padded(nova_ctx *ctx, const double *compact, int ncomp, double *padded)
 {
     NOVA_REQUIRE(ctx && compact && padded && (ncomp == 1 || ncomp == 3), "bad argument");
