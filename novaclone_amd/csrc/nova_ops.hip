@@ -13,6 +13,7 @@
 #include <string>
 
 #include "nova_internal.hpp"
+#include "nova_pipe.hpp"
 
 namespace nova {
 
@@ -46,156 +47,51 @@ __device__ inline void offs(int d, int q, int &di, int &dj, int &dk)
 // ---- streamed operators: ex::div(U, phi), ex::div(U), ex::grad(p),
 // ex::laplacian(nu, f) ----
 //
-// A 64 x 4 block owns a (k, j) tile and walks i over a chunk of planes.
-// Every input component the operator reads is staged plane by plane in LDS
-// with its j and k halos (H nodes each side); the i-neighbours of the
-// block's own column stay in a register queue of planes i-H .. i+H, and the
-// loads of the next two planes are in flight while a plane is computed.
-// Each input value is read from global memory about once (plus the halo
-// ring and the 2H warm-up planes of a chunk), and every load addresses a
-// precomputed column plus a plane offset: the boundary index maps are
-// evaluated once per column and once per plane, not per neighbour.
+// Each is one pass of the plane-streaming pipeline (nova_pipe.hpp): LDS
+// plane tiles for the j and k neighbours, an i register queue, the loads of
+// the next two planes in flight, the boundary maps evaluated once per column
+// and per plane.  A component is queued or staged only where the operator
+// reads its neighbours: div(U, phi) queues U_x and stages U_y, U_z (each
+// velocity component is differenced along its own direction only).
 // Each operator's arithmetic is written term for term as the reference's
 // (cited at each case), in the same order for every node.
 enum { SOP_DIV_ADV = 0, SOP_LAPLACIAN = 1, SOP_DIV = 2, SOP_GRAD = 3 };
-constexpr int kOK = 64, kOJ = 4;
 
-// staged components: DIV_ADV U_x, U_y, U_z then phi's (none when phi is U);
+// components: DIV_ADV U_x, U_y, U_z then phi's (none when phi is U);
 // LAPLACIAN f's; DIV U's; GRAD p
 template <int OP, int NCP, bool SHARED>
 constexpr int sop_staged()
 {
     return OP == SOP_DIV_ADV ? (SHARED ? 3 : 3 + NCP) : (OP == SOP_LAPLACIAN ? NCP : (OP == SOP_DIV ? 3 : 1));
 }
+// queued (i-neighbours) and staged (j, k neighbours) components
+template <int OP, int NCP, bool SHARED>
+constexpr unsigned sop_qm()
+{
+    constexpr unsigned all = (1u << sop_staged<OP, NCP, SHARED>()) - 1u;
+    return (OP == SOP_DIV_ADV && !SHARED) ? (1u | (((1u << NCP) - 1u) << 3)) : (OP == SOP_DIV ? 1u : all);
+}
+template <int OP, int NCP, bool SHARED>
+constexpr unsigned sop_lm()
+{
+    constexpr unsigned all = (1u << sop_staged<OP, NCP, SHARED>()) - 1u;
+    return (OP == SOP_DIV_ADV && !SHARED) ? (6u | (((1u << NCP) - 1u) << 3)) : (OP == SOP_DIV ? 6u : all);
+}
 
 struct SopArgs {
-    const double *src[6];  // staged component s
-    // bit d of flip[s]: component s changes sign when loaded through a
-    // reflection across the walls normal to d (a vector component odd there)
-    unsigned flip[6];
     double *out;
     double nu;     // LAPLACIAN
     double scale;  // DIV
-    int ichunk, nkt, njt, nic;
 };
 
 template <int H, int OP, int NCP, bool SHARED, bool WALLS>
-__global__ void __launch_bounds__(kOK *kOJ) k_sop_stream(Grid g, SopArgs a)
+__global__ void __launch_bounds__(pipe::threads(4))
+    __attribute__((amdgpu_waves_per_eu(OP == SOP_DIV_ADV && !SHARED && NCP == 3 && H <= 3 ? 2 : 1)))
+    k_sop_stream(Grid g, pipe::Launch L, pipe::Src<sop_staged<OP, NCP, SHARED>()> S, SopArgs a)
 {
     constexpr int NS = sop_staged<OP, NCP, SHARED>();
-    constexpr int QW = 2 * H + 1, LJ = kOJ + 2 * H, LK = kOK + 2 * H;
-    constexpr int NHALO = 2 * H * (kOK + kOJ);
-    constexpr int NSLOT = (NHALO + kOK * kOJ - 1) / (kOK * kOJ);
-    __shared__ double sh[NS][LJ][LK];
-
-    // tiles: k fastest, then j, then the i chunk; blocks go to the eight
-    // XCDs in turn, so XCD x is given the x-th eighth of the tiles and the
-    // tiles that share halo rows share its L2
-    const int T = a.nkt * a.njt * a.nic;
-    int bt = blockIdx.x;
-    if (T % 8 == 0) bt = (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8;
-    const int kt = bt % a.nkt, jt = (bt / a.nkt) % a.njt, ic = bt / (a.nkt * a.njt);
-    const int tk = threadIdx.x % kOK, tj = threadIdx.x / kOK;
-    const int k = kt * kOK + tk, j = jt * kOJ + tj;
-    const bool valid = k < g.s[2] && j < g.s[1];
-    const int i0 = ic * a.ichunk;
-    const int i1 = i0 + a.ichunk < g.s[0] ? i0 + a.ichunk : g.s[0];
-    const int64_t plane = (int64_t)g.s[1] * g.s[2];
-
-    // own column and its reflection flags (bit 1: j, bit 2: k)
-    int fj, fk;
-    const int jm = bmap(j, g.n[1], g.wall[1], fj);
-    const int km = bmap(k, g.n[2], g.wall[2], fk);
-    const int64_t col = (int64_t)jm * g.s[2] + km;
-    const unsigned cfl = (fj ? 2u : 0u) | (fk ? 4u : 0u);
-    // halo slots of this thread: LDS row and column, source column, flags
-    int hr[NSLOT], hc[NSLOT];
-    int64_t hsrc[NSLOT];
-    unsigned hfl[NSLOT];
-    bool hon[NSLOT];
-#pragma unroll
-    for (int sl = 0; sl < NSLOT; sl++) {
-        const int e = threadIdx.x + sl * kOK * kOJ;
-        int jj = 0, kk = 0;
-        hon[sl] = e < NHALO;
-        hr[sl] = hc[sl] = 0;
-        if (e < 2 * H * kOK) {  // j halo rows
-            const int rr = e / kOK, cc = e % kOK;
-            hr[sl] = rr < H ? rr : kOJ + rr;
-            hc[sl] = cc + H;
-            jj = jt * kOJ + (rr < H ? rr - H : kOJ + rr - H);
-            kk = kt * kOK + cc;
-        } else if (e < NHALO) {  // k halo columns
-            const int e2 = e - 2 * H * kOK;
-            const int rr = e2 / (2 * H), c2 = e2 % (2 * H);
-            hr[sl] = rr + H;
-            hc[sl] = c2 < H ? c2 : kOK + c2;
-            jj = jt * kOJ + rr;
-            kk = kt * kOK + (c2 < H ? c2 - H : kOK + c2 - H);
-        }
-        int f1, f2;
-        const int jm2 = bmap(jj, g.n[1], g.wall[1], f1);
-        const int km2 = bmap(kk, g.n[2], g.wall[2], f2);
-        hsrc[sl] = (int64_t)jm2 * g.s[2] + km2;
-        hfl[sl] = (f1 ? 2u : 0u) | (f2 ? 4u : 0u);
-    }
-    // value of component s loaded through the reflections in flags fl
-    auto sgn = [&](int s, unsigned fl, double v) -> double {
-        if constexpr (WALLS) return (__popc(fl & a.flip[s]) & 1) ? -v : v;
-        else return v;
-    };
-    auto plane_of = [&](int ii, unsigned &fl) -> int64_t {
-        int fi;
-        const int im = bmap(ii, g.n[0], g.wall[0], fi);
-        fl = fi ? 1u : 0u;
-        return (int64_t)im * plane;
-    };
-
-    // loads of plane p: the queue entry of plane p + H (own column) and the
-    // halo of plane p, kept as loaded (signs applied when staged)
-    struct Pf {
-        double nq[NS];
-        double hv[NSLOT][NS];
-        unsigned fq, fh;
-    };
-    auto load_pf = [&](Pf &P, int p, bool queue) {
-        if (queue) {
-            const int64_t po = plane_of(p + H, P.fq);
-#pragma unroll
-            for (int s = 0; s < NS; s++) P.nq[s] = a.src[s][po + col];
-        }
-        const int64_t pp = plane_of(p, P.fh);
-#pragma unroll
-        for (int sl = 0; sl < NSLOT; sl++)
-#pragma unroll
-            for (int s = 0; s < NS; s++) P.hv[sl][s] = hon[sl] ? a.src[s][pp + hsrc[sl]] : 0.0;
-    };
-
-    double q[NS][QW];
-#pragma unroll
-    for (int o = 0; o < QW; o++) {
-        unsigned fl;
-        const int64_t po = plane_of(i0 - H + o, fl);
-#pragma unroll
-        for (int s = 0; s < NS; s++) q[s][o] = sgn(s, fl | cfl, a.src[s][po + col]);
-    }
-    // neighbour of the node at offset o (o = -H .. H) along direction d
-    auto nb = [&](int s, int d, int o) -> double {
-        return d == 0 ? q[s][H + o] : (d == 1 ? sh[s][tj + H + o][tk + H] : sh[s][tj + H][tk + H + o]);
-    };
-
-    auto body = [&](int i, Pf &Cur, Pf &Nxt) {
-#pragma unroll
-        for (int s = 0; s < NS; s++) sh[s][tj + H][tk + H] = q[s][H];
-#pragma unroll
-        for (int sl = 0; sl < NSLOT; sl++)
-            if (hon[sl])
-#pragma unroll
-                for (int s = 0; s < NS; s++) sh[s][hr[sl]][hc[sl]] = sgn(s, Cur.fh | hfl[sl], Cur.hv[sl][s]);
-        __syncthreads();
-        if (i + 2 < i1) load_pf(Cur, i + 2, true);
-        if (valid) {
-            const int64_t q0 = (int64_t)i * plane + col;
+    pipe::run<H, NS, sop_qm<OP, NCP, SHARED>(), sop_lm<OP, NCP, SHARED>(), WALLS>(
+        g, L, S, [&](int, int, int k, int64_t q0, const auto &nb) {
             if constexpr (OP == SOP_DIV_ADV) {
                 // src/ex/div/div.hxx:101-169: the mean of the conservative
                 // and the expanded forms, per direction and stencil offset
@@ -204,11 +100,11 @@ __global__ void __launch_bounds__(kOK *kOJ) k_sop_stream(Grid g, SopArgs a)
 #pragma unroll
                 for (int c = 0; c < NCP; c++) {
                     const int sp = SHARED ? c : 3 + c;
-                    const double pc = q[sp][H];
+                    const double pc = nb(sp, 0, 0);
                     double acc = 0.0;
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
-                        const double u0 = q[d][H];
+                        const double u0 = nb(d, d, 0);
                         double ad = 0.0;
 #pragma unroll
                         for (int l = 0; l < H; l++) {
@@ -226,7 +122,7 @@ __global__ void __launch_bounds__(kOK *kOJ) k_sop_stream(Grid g, SopArgs a)
                 // times the z first derivative
 #pragma unroll
                 for (int c = 0; c < NCP; c++) {
-                    const double f0 = q[c][H];
+                    const double f0 = nb(c, 0, 0);
                     double acc = 0.0, dz1 = 0.0;
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
@@ -262,26 +158,7 @@ __global__ void __launch_bounds__(kOK *kOJ) k_sop_stream(Grid g, SopArgs a)
                     __builtin_nontemporal_store(d == 2 ? ad * g.zinv[k] : ad, a.out + d * g.np + q0);
                 }
             }
-        }
-        __syncthreads();
-        if (i + 1 < i1) {
-#pragma unroll
-            for (int s = 0; s < NS; s++) {
-#pragma unroll
-                for (int o = 0; o < 2 * H; o++) q[s][o] = q[s][o + 1];
-                q[s][2 * H] = sgn(s, Nxt.fq | cfl, Nxt.nq[s]);
-            }
-        }
-    };
-    Pf A, B;
-    load_pf(A, i0, false);
-    A.fq = 0;
-    B.fq = B.fh = 0;
-    if (i0 + 1 < i1) load_pf(B, i0 + 1, true);
-    for (int i = i0; i < i1; i += 2) {
-        body(i, A, B);
-        if (i + 1 < i1) body(i + 1, B, A);
-    }
+        });
 }
 
 // ex::laplacian(Field<scalar> nu, f) (src/ex/laplacian/laplacian.hxx:64-127):
@@ -670,58 +547,47 @@ dim3 pt_grid(const Grid &g) { return dim3((g.s[2] + 63) / 64, (g.s[1] + 3) / 4, 
 
 namespace {
 
-// the tiling of a streamed-operator launch: i chunks enough for about 2048
-// blocks (eight per CU), at least 8 planes (and 4H) long so that the 2H
-// warm-up planes stay a small part of a chunk
-template <int H>
-void sop_tiling(const Grid &g, SopArgs &a)
-{
-    a.nkt = (g.s[2] + kOK - 1) / kOK;
-    a.njt = (g.s[1] + kOJ - 1) / kOJ;
-    const int tiles = a.nkt * a.njt;
-    const int nic = (2048 + tiles - 1) / tiles;
-    const int minchunk = 4 * H > 8 ? 4 * H : 8;
-    int ichunk = (g.s[0] + nic - 1) / nic;
-    if (ichunk < minchunk) ichunk = minchunk;
-    if (ichunk > g.s[0]) ichunk = g.s[0];
-    a.ichunk = ichunk;
-    a.nic = (g.s[0] + ichunk - 1) / ichunk;
-}
-
 template <int H, int OP, int NCP, bool SHARED>
-int launch_sop_h(nova_ctx *ctx, SopArgs &a, const char *name)
+int launch_sop_h(nova_ctx *ctx, const pipe::Src<sop_staged<OP, NCP, SHARED>()> &S, const SopArgs &a, const char *name)
 {
     const Grid &g = ctx->g;
-    sop_tiling<H>(g, a);
-    const dim3 grid(a.nkt * a.njt * a.nic);
+    const pipe::Launch L = pipe::tiling<H>(g);
     if (g.wall[0] || g.wall[1] || g.wall[2])
-        hipLaunchKernelGGL((k_sop_stream<H, OP, NCP, SHARED, true>), grid, dim3(kOK * kOJ), 0, ctx->stream, g, a);
+        hipLaunchKernelGGL((k_sop_stream<H, OP, NCP, SHARED, true>), dim3(pipe::blocks(L)), dim3(pipe::threads(4)), 0,
+                           ctx->stream, g, L, S, a);
     else
-        hipLaunchKernelGGL((k_sop_stream<H, OP, NCP, SHARED, false>), grid, dim3(kOK * kOJ), 0, ctx->stream, g, a);
+        hipLaunchKernelGGL((k_sop_stream<H, OP, NCP, SHARED, false>), dim3(pipe::blocks(L)), dim3(pipe::threads(4)), 0,
+                           ctx->stream, g, L, S, a);
     return check_launch(name);
 }
 
 template <int OP, int NCP, bool SHARED = false>
-int launch_sop(nova_ctx *ctx, SopArgs &a, const char *name)
+int launch_sop(nova_ctx *ctx, const pipe::Src<sop_staged<OP, NCP, SHARED>()> &S, const SopArgs &a, const char *name)
 {
     switch (ctx->g.h) {
-    case 1: return launch_sop_h<1, OP, NCP, SHARED>(ctx, a, name);
-    case 2: return launch_sop_h<2, OP, NCP, SHARED>(ctx, a, name);
-    case 3: return launch_sop_h<3, OP, NCP, SHARED>(ctx, a, name);
-    case 4: return launch_sop_h<4, OP, NCP, SHARED>(ctx, a, name);
+    case 1: return launch_sop_h<1, OP, NCP, SHARED>(ctx, S, a, name);
+    case 2: return launch_sop_h<2, OP, NCP, SHARED>(ctx, S, a, name);
+    case 3: return launch_sop_h<3, OP, NCP, SHARED>(ctx, S, a, name);
+    case 4: return launch_sop_h<4, OP, NCP, SHARED>(ctx, S, a, name);
     }
     return fail(NOVA_ERR_ARG, "stencil half-width outside 1..4");
 }
 
-// staged component s = component c of an nc-component field at f: its
-// reflection signs are those of ld()
-void sop_src(const Grid &g, SopArgs &a, int s, const double *f, int nc, int c)
+template <int NCP>
+int launch_div_adv_nc(nova_ctx *ctx, const double *U, const double *phi, const SopArgs &a)
 {
-    a.src[s] = f + c * g.np;
-    a.flip[s] = 0;
-    if (nc == 3)
-        for (int d = 0; d < 3; d++)
-            if ((g.odd[d] >> c) & 1u) a.flip[s] |= 1u << d;
+    pipe::Src<3 + NCP> S;
+    for (int d = 0; d < 3; d++) pipe::set_src(ctx->g, S, d, U, 3, d);
+    for (int c = 0; c < NCP; c++) pipe::set_src(ctx->g, S, 3 + c, phi, NCP, c);
+    return launch_sop<SOP_DIV_ADV, NCP>(ctx, S, a, "div_adv");
+}
+
+template <int NCP>
+int launch_laplacian_nc(nova_ctx *ctx, const double *f, const SopArgs &a)
+{
+    pipe::Src<NCP> S;
+    for (int c = 0; c < NCP; c++) pipe::set_src(ctx->g, S, c, f, NCP, c);
+    return launch_sop<SOP_LAPLACIAN, NCP>(ctx, S, a, "laplacian");
 }
 
 }  // namespace
@@ -730,10 +596,12 @@ int launch_div_adv(nova_ctx *ctx, const double *U, const double *phi, int nc, do
 {
     SopArgs a{};
     a.out = out;
-    for (int d = 0; d < 3; d++) sop_src(ctx->g, a, d, U, 3, d);
-    if (phi == U && nc == 3) return launch_sop<SOP_DIV_ADV, 3, true>(ctx, a, "div_adv");
-    for (int c = 0; c < nc; c++) sop_src(ctx->g, a, 3 + c, phi, nc, c);
-    return nc == 3 ? launch_sop<SOP_DIV_ADV, 3>(ctx, a, "div_adv") : launch_sop<SOP_DIV_ADV, 1>(ctx, a, "div_adv");
+    if (phi == U && nc == 3) {
+        pipe::Src<3> S;
+        for (int d = 0; d < 3; d++) pipe::set_src(ctx->g, S, d, U, 3, d);
+        return launch_sop<SOP_DIV_ADV, 3, true>(ctx, S, a, "div_adv");
+    }
+    return nc == 3 ? launch_div_adv_nc<3>(ctx, U, phi, a) : launch_div_adv_nc<1>(ctx, U, phi, a);
 }
 
 int launch_div(nova_ctx *ctx, const double *U, double scale, double *out)
@@ -741,16 +609,18 @@ int launch_div(nova_ctx *ctx, const double *U, double scale, double *out)
     SopArgs a{};
     a.out = out;
     a.scale = scale;
-    for (int d = 0; d < 3; d++) sop_src(ctx->g, a, d, U, 3, d);
-    return launch_sop<SOP_DIV, 3>(ctx, a, "div");
+    pipe::Src<3> S;
+    for (int d = 0; d < 3; d++) pipe::set_src(ctx->g, S, d, U, 3, d);
+    return launch_sop<SOP_DIV, 3>(ctx, S, a, "div");
 }
 
 int launch_grad(nova_ctx *ctx, const double *p, double *out)
 {
     SopArgs a{};
     a.out = out;
-    sop_src(ctx->g, a, 0, p, 1, 0);
-    return launch_sop<SOP_GRAD, 1>(ctx, a, "grad");
+    pipe::Src<1> S;
+    pipe::set_src(ctx->g, S, 0, p, 1, 0);
+    return launch_sop<SOP_GRAD, 1>(ctx, S, a, "grad");
 }
 
 int launch_laplacian_var(nova_ctx *ctx, const double *nu, const double *f, int nc, double *out)
@@ -764,8 +634,7 @@ int launch_laplacian(nova_ctx *ctx, double nu, const double *f, int nc, double *
     SopArgs a{};
     a.out = out;
     a.nu = nu;
-    for (int c = 0; c < nc; c++) sop_src(ctx->g, a, c, f, nc, c);
-    return nc == 3 ? launch_sop<SOP_LAPLACIAN, 3>(ctx, a, "laplacian") : launch_sop<SOP_LAPLACIAN, 1>(ctx, a, "laplacian");
+    return nc == 3 ? launch_laplacian_nc<3>(ctx, f, a) : launch_laplacian_nc<1>(ctx, f, a);
 }
 
 int launch_curl(nova_ctx *ctx, const double *U, int literal, double *out, int pseudo)
