@@ -55,21 +55,25 @@ __device__ inline void offs(int d, int q, int &di, int &dj, int &dk)
 // velocity component is differenced along its own direction only).
 // Each operator's arithmetic is written term for term as the reference's
 // (cited at each case), in the same order for every node.
-enum { SOP_DIV_ADV = 0, SOP_LAPLACIAN = 1, SOP_DIV = 2, SOP_GRAD = 3 };
+enum { SOP_DIV_ADV = 0, SOP_LAPLACIAN = 1, SOP_DIV = 2, SOP_GRAD = 3, SOP_CURL_LIT = 4, SOP_CURL = 5 };
 
 // components: DIV_ADV U_x, U_y, U_z then phi's (none when phi is U);
-// LAPLACIAN f's; DIV U's; GRAD p
+// LAPLACIAN f's; DIV and CURL U's; GRAD p
 template <int OP, int NCP, bool SHARED>
 constexpr int sop_staged()
 {
-    return OP == SOP_DIV_ADV ? (SHARED ? 3 : 3 + NCP) : (OP == SOP_LAPLACIAN ? NCP : (OP == SOP_DIV ? 3 : 1));
+    return OP == SOP_DIV_ADV ? (SHARED ? 3 : 3 + NCP)
+                             : (OP == SOP_LAPLACIAN ? NCP : (OP == SOP_GRAD ? 1 : 3));
 }
-// queued (i-neighbours) and staged (j, k neighbours) components
+// queued (i-neighbours) and staged (j, k neighbours) components: the curl
+// differences U_y and U_z along i, each component along the two directions
+// other than its own
 template <int OP, int NCP, bool SHARED>
 constexpr unsigned sop_qm()
 {
     constexpr unsigned all = (1u << sop_staged<OP, NCP, SHARED>()) - 1u;
-    return (OP == SOP_DIV_ADV && !SHARED) ? (1u | (((1u << NCP) - 1u) << 3)) : (OP == SOP_DIV ? 1u : all);
+    return (OP == SOP_DIV_ADV && !SHARED) ? (1u | (((1u << NCP) - 1u) << 3))
+                                          : (OP == SOP_DIV ? 1u : ((OP == SOP_CURL_LIT || OP == SOP_CURL) ? 6u : all));
 }
 template <int OP, int NCP, bool SHARED>
 constexpr unsigned sop_lm()
@@ -148,7 +152,7 @@ __global__ void __launch_bounds__(pipe::threads(4))
                     acc += d == 2 ? ad * g.zinv[k] : ad;
                 }
                 __builtin_nontemporal_store(acc * a.scale, a.out + q0);
-            } else {
+            } else if constexpr (OP == SOP_GRAD) {
                 // src/ex/grad/grad.cpp:33-58
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
@@ -157,6 +161,27 @@ __global__ void __launch_bounds__(pipe::threads(4))
                     for (int l = 0; l < H; l++) ad += g.a[d][l] * (nb(0, d, l + 1) - nb(0, d, -l - 1));
                     __builtin_nontemporal_store(d == 2 ? ad * g.zinv[k] : ad, a.out + d * g.np + q0);
                 }
+            } else {
+                // curl: literal = antisymmetric part of the half-shifted
+                // gradient (src/ex/grad/grad.cpp:73-98, src/ex/curl/curl.cpp:38-53,
+                // offsets +l and -(l + 1)); otherwise the central stencil
+                double gr[3][3];
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        if (c == d) continue;
+                        double ad = 0.0;
+#pragma unroll
+                        for (int l = 0; l < H; l++) {
+                            if constexpr (OP == SOP_CURL_LIT) ad += g.al[d][l] * (nb(c, d, l) - nb(c, d, -l - 1));
+                            else ad += g.a[d][l] * (nb(c, d, l + 1) - nb(c, d, -l - 1));
+                        }
+                        gr[c][d] = d == 2 ? ad * g.zinv[k] : ad;
+                    }
+                __builtin_nontemporal_store(gr[2][1] - gr[1][2], a.out + q0);
+                __builtin_nontemporal_store(gr[0][2] - gr[2][0], a.out + g.np + q0);
+                __builtin_nontemporal_store(gr[1][0] - gr[0][1], a.out + 2 * g.np + q0);
             }
         });
 }
@@ -198,54 +223,6 @@ __global__ void k_laplacian_var(Grid g, const double *__restrict__ nu, const dou
         }
         out[c * g.np + lin(g, i, j, k)] = acc;
     }
-}
-
-// Load of a 3-component field with the reflection parity of a pseudovector
-// (the curl of a velocity: the wall-normal component even, the others odd).
-__device__ inline double ld_pseudo(const double *__restrict__ f, const Grid &g, int c, int i, int j, int k)
-{
-    int fi, fj, fk;
-    int ii = bmap(i, g.n[0], g.wall[0], fi);
-    int jj = bmap(j, g.n[1], g.wall[1], fj);
-    int kk = bmap(k, g.n[2], g.wall[2], fk);
-    double s = (fi && c != 0 ? -1.0 : 1.0) * (fj && c != 1 ? -1.0 : 1.0) * (fk && c != 2 ? -1.0 : 1.0);
-    return s * f[c * g.np + lin(g, ii, jj, kk)];
-}
-
-// curl: literal = antisymmetric part of the half-shifted gradient
-// (src/ex/grad/grad.cpp:73-98, src/ex/curl/curl.cpp:38-53); otherwise the
-// central stencil per component.  pseudo: the input is itself a curl.
-__global__ void k_curl(Grid g, const double *__restrict__ U, int literal, int pseudo, double *__restrict__ out)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k >= g.s[2] || j >= g.s[1]) return;
-    double gr[3][3];
-    for (int c = 0; c < 3; c++)
-        for (int d = 0; d < 3; d++) {
-            double ad = 0.0;
-            for (int l = 0; l < g.h; l++) {
-                int di, dj, dk;
-                if (literal) {
-                    offs(d, l, di, dj, dk);
-                    int ei, ej, ek;
-                    offs(d, l + 1, ei, ej, ek);
-                    ad += g.al[d][l] * (ld(U, g, 3, c, i + di, j + dj, k + dk) - ld(U, g, 3, c, i - ei, j - ej, k - ek));
-                } else if (pseudo) {
-                    offs(d, l + 1, di, dj, dk);
-                    ad += g.a[d][l] * (ld_pseudo(U, g, c, i + di, j + dj, k + dk) - ld_pseudo(U, g, c, i - di, j - dj, k - dk));
-                } else {
-                    offs(d, l + 1, di, dj, dk);
-                    ad += g.a[d][l] * (ld(U, g, 3, c, i + di, j + dj, k + dk) - ld(U, g, 3, c, i - di, j - dj, k - dk));
-                }
-            }
-            gr[c][d] = d == 2 ? ad * g.zinv[k] : ad;
-        }
-    const int64_t q = lin(g, i, j, k);
-    out[q] = gr[2][1] - gr[1][2];
-    out[g.np + q] = gr[0][2] - gr[2][0];
-    out[2 * g.np + q] = gr[1][0] - gr[0][1];
 }
 
 __global__ void k_axpby(size_t n, double a, const double *__restrict__ x, double b,
@@ -639,8 +616,16 @@ int launch_laplacian(nova_ctx *ctx, double nu, const double *f, int nc, double *
 
 int launch_curl(nova_ctx *ctx, const double *U, int literal, double *out, int pseudo)
 {
-    hipLaunchKernelGGL(k_curl, pt_grid(ctx->g), pt_block(), 0, ctx->stream, ctx->g, U, literal, pseudo, out);
-    return check_launch("curl");
+    SopArgs a{};
+    a.out = out;
+    pipe::Src<3> S;
+    for (int c = 0; c < 3; c++) {
+        pipe::set_src(ctx->g, S, c, U, 3, c);
+        // a pseudovector (the input is itself a curl, central stencil only):
+        // the wall-normal component even, the others odd
+        if (pseudo && !literal) S.flip[c] = 7u & ~(1u << c);
+    }
+    return literal ? launch_sop<SOP_CURL_LIT, 3>(ctx, S, a, "curl") : launch_sop<SOP_CURL, 3>(ctx, S, a, "curl");
 }
 
 int launch_axpby(nova_ctx *ctx, size_t n, double a, const double *x, double b, const double *y, double *out)
