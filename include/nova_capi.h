@@ -510,7 +510,7 @@ int nova_ensemble_step(nova_ensemble *e, double dt, int nsteps);
 /* Record a step as a HIP graph and replay it (1) or launch directly (0). */
 int nova_ensemble_use_graph(nova_ensemble *e, int enable);
 /* out: instances, size classes, work-list tiles per stage, persistent
- * blocks, total nodes, field slots. */
+ * blocks (both models' launches together), total nodes, field slots. */
 int nova_ensemble_info(nova_ensemble *e, int64_t out[6]);
 
 /* ---- batched tridiagonal (src/Tools/Tools.cpp:43-62, tools::tdma) ----

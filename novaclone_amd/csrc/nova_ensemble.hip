@@ -48,11 +48,15 @@ struct EnsInstance {
     int tile0;                     // first work-list tile
 };
 
-template <int H>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1)))
+// One model's tiles of a stage: a persistent grid walks the work list of
+// that model's instances.  The two models run as two launches, each with its
+// own registers and LDS (a single launch over both held the NS tiles to the
+// MHD stage's one wave per SIMD and its two-field LDS plane).
+template <int H, int MODEL>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODEL == MODEL_MHD || H >= 4 ? 1 : 2)))
 k_stage_group(const EnsInstance *__restrict__ inst, int ninst, int ntiles, int stage, StageArgs sa)
 {
-    __shared__ double sh[stage_lds<H, MODEL_MHD, kEnsTK>()];
+    __shared__ double sh[stage_lds<H, MODEL, kEnsTK>()];
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         // owner of tile t: the last instance whose first tile is <= t
         int lo = 0, hi = ninst - 1;
@@ -63,8 +67,7 @@ k_stage_group(const EnsInstance *__restrict__ inst, int ninst, int ntiles, int s
         }
         const EnsInstance &I = inst[lo];
         const int tl = t - I.tile0;
-        const int nf = I.model == MODEL_MHD ? 2 : 1;
-        if (nf == 1) {
+        if constexpr (MODEL == MODEL_NS) {
             StreamArgs<1> a{};
             a.X[0] = stage == 0 ? I.U[0] : (stage == 1 ? I.W1[0] : I.W2[0]);
             a.X0[0] = stage == 0 ? nullptr : I.U[0];
@@ -136,8 +139,12 @@ struct nova_ensemble {
     hipStream_t stream = nullptr;        // caller's stream (0 = default)
     std::vector<Class> classes;
     std::vector<Member> members;
+    // work lists: the NS instances (hinst[0 .. nns)) then the MHD ones, each
+    // list's tiles numbered from 0; one persistent launch per model
     std::vector<nova::EnsInstance> hinst;
     nova::EnsInstance *dinst = nullptr;
+    int nns = 0;
+    int ntiles_m[2] = {0, 0}, nblocks_m[2] = {0, 0};  // [0] NS, [1] MHD
     int ntiles = 0, nblocks = 0, h = 3;
     int use_graph = 0;
     hipStream_t gstream = nullptr;
@@ -175,9 +182,21 @@ void release(nova_ensemble *e)
 template <int H>
 int launch_group(nova_ensemble *e, hipStream_t st, int stage, double dt)
 {
-    hipLaunchKernelGGL((k_stage_group<H>), dim3(e->nblocks), dim3(kEnsTK, 256 / kEnsTK, 1), 0, st, e->dinst,
-                       (int)e->hinst.size(), e->ntiles, stage, ssp_stage(stage, dt));
+    const int nm = (int)e->hinst.size() - e->nns;
+    if (e->nns > 0)
+        hipLaunchKernelGGL((k_stage_group<H, MODEL_NS>), dim3(e->nblocks_m[0]), dim3(kEnsTK, 256 / kEnsTK, 1), 0, st,
+                           e->dinst, e->nns, e->ntiles_m[0], stage, ssp_stage(stage, dt));
+    if (nm > 0)
+        hipLaunchKernelGGL((k_stage_group<H, MODEL_MHD>), dim3(e->nblocks_m[1]), dim3(kEnsTK, 256 / kEnsTK, 1), 0, st,
+                           e->dinst + e->nns, nm, e->ntiles_m[1], stage, ssp_stage(stage, dt));
     return check_launch("ensemble stage");
+}
+
+// blocks of one model's launch that are resident at once
+template <int H, int MODEL>
+hipError_t group_residency(int *per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_stage_group<H, MODEL>, 256, 0);
 }
 
 int step_once(nova_ensemble *e, hipStream_t st, double dt)
@@ -317,42 +336,49 @@ int ensemble_create(const nova_instance_desc *inst, int n, int device, nova_ense
         return rc;
     }
     // work list: per instance its tiles, in member order
-    int tiles = 0;
-    for (int i = 0; i < n; i++) {
-        const auto &mb = e->members[i];
-        const auto &cl = e->classes[mb.cls];
-        const Grid &g = cl.ctx->g;
-        EnsInstance I{};
-        I.g = g;
-        const int64_t f3 = 3 * cl.np;
-        I.U[0] = cl.U + mb.slotU * f3;
-        I.W1[0] = cl.W1 + mb.slotU * f3;
-        I.W2[0] = cl.W2 + mb.slotU * f3;
-        if (mb.slotB >= 0) {
-            I.U[1] = cl.U + mb.slotB * f3;
-            I.W1[1] = cl.W1 + mb.slotB * f3;
-            I.W2[1] = cl.W2 + mb.slotB * f3;
+    // NS instances first, then MHD: each model's work list is contiguous
+    for (int pass = 0; pass < 2; pass++) {
+        int tiles = 0;
+        for (int i = 0; i < n; i++) {
+            const auto &mb = e->members[i];
+            const bool mhd = mb.model == NOVA_MODEL_MHD;
+            if (mhd != (pass == 1)) continue;
+            const auto &cl = e->classes[mb.cls];
+            const Grid &g = cl.ctx->g;
+            EnsInstance I{};
+            I.g = g;
+            const int64_t f3 = 3 * cl.np;
+            I.U[0] = cl.U + mb.slotU * f3;
+            I.W1[0] = cl.W1 + mb.slotU * f3;
+            I.W2[0] = cl.W2 + mb.slotU * f3;
+            if (mb.slotB >= 0) {
+                I.U[1] = cl.U + mb.slotB * f3;
+                I.W1[1] = cl.W1 + mb.slotB * f3;
+                I.W2[1] = cl.W2 + mb.slotB * f3;
+            }
+            I.nu = inst[i].nu;
+            I.eta = inst[i].eta;
+            I.model = mhd ? MODEL_MHD : MODEL_NS;
+            I.nkt = (g.s[2] + kEnsTK - 1) / kEnsTK;
+            I.njt = (g.s[1] + 256 / kEnsTK - 1) / (256 / kEnsTK);
+            I.ichunk = kEnsChunk < g.s[0] ? kEnsChunk : g.s[0];
+            I.nic = (g.s[0] + I.ichunk - 1) / I.ichunk;
+            I.tile0 = tiles;
+            tiles += I.nkt * I.njt * I.nic;
+            e->hinst.push_back(I);
         }
-        I.nu = inst[i].nu;
-        I.eta = inst[i].eta;
-        I.model = mb.model == NOVA_MODEL_MHD ? MODEL_MHD : MODEL_NS;
-        I.nkt = (g.s[2] + kEnsTK - 1) / kEnsTK;
-        I.njt = (g.s[1] + 256 / kEnsTK - 1) / (256 / kEnsTK);
-        I.ichunk = kEnsChunk < g.s[0] ? kEnsChunk : g.s[0];
-        I.nic = (g.s[0] + I.ichunk - 1) / I.ichunk;
-        I.tile0 = tiles;
-        tiles += I.nkt * I.njt * I.nic;
-        e->hinst.push_back(I);
+        e->ntiles_m[pass] = tiles;
+        if (pass == 0) e->nns = (int)e->hinst.size();
     }
-    e->ntiles = tiles;
-    int cus = 0, per_cu = 0;
+    e->ntiles = e->ntiles_m[0] + e->ntiles_m[1];
+    int cus = 0, per_cu[2] = {0, 0};
     hipError_t err = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-    if (err == hipSuccess) {
+    for (int m = 0; m < 2 && err == hipSuccess; m++) {
         switch (e->h) {
-        case 1: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stage_group<1>, 256, 0); break;
-        case 2: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stage_group<2>, 256, 0); break;
-        case 3: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stage_group<3>, 256, 0); break;
-        default: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stage_group<4>, 256, 0); break;
+        case 1: err = m ? group_residency<1, MODEL_MHD>(&per_cu[m]) : group_residency<1, MODEL_NS>(&per_cu[m]); break;
+        case 2: err = m ? group_residency<2, MODEL_MHD>(&per_cu[m]) : group_residency<2, MODEL_NS>(&per_cu[m]); break;
+        case 3: err = m ? group_residency<3, MODEL_MHD>(&per_cu[m]) : group_residency<3, MODEL_NS>(&per_cu[m]); break;
+        default: err = m ? group_residency<4, MODEL_MHD>(&per_cu[m]) : group_residency<4, MODEL_NS>(&per_cu[m]); break;
         }
     }
     if (err == hipSuccess) err = hipMalloc((void **)&e->dinst, sizeof(EnsInstance) * e->hinst.size());
@@ -363,8 +389,9 @@ int ensemble_create(const nova_instance_desc *inst, int n, int device, nova_ense
         delete e;
         return hip_fail(err, "ensemble set-up");
     }
-    const int resident = std::max(1, cus) * std::max(1, per_cu);
-    e->nblocks = std::min(resident, e->ntiles);
+    for (int m = 0; m < 2; m++)
+        e->nblocks_m[m] = std::min(std::max(1, cus) * std::max(1, per_cu[m]), std::max(1, e->ntiles_m[m]));
+    e->nblocks = (e->ntiles_m[0] ? e->nblocks_m[0] : 0) + (e->ntiles_m[1] ? e->nblocks_m[1] : 0);
     for (auto &cl : e->classes) cl.ctx->stream = e->stream;
     if (e->classes.size() > 1) {
         err = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
