@@ -62,7 +62,7 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
-APPS = ["tg_poisson", "ot_poisson", "tg_wc", "mhdjet", "shear_layer"]
+APPS = ["tg_poisson", "ot_poisson", "tg_wc", "mhdjet", "shear_layer", "field_semantics"]
 
 
 def build_apps() -> None:

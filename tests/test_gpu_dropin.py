@@ -37,6 +37,22 @@ def test_tg_poisson_app_matches_oracle(orc, tmp_path):
     assert np.max(np.abs(got[:, 1] - ke) / np.array(ke)) < 1e-12
 
 
+@pytest.mark.parametrize("N", [16, 20])
+def test_field_value_semantics(tmp_path, N):
+    """The drop-in's Field assignment keeps the reference's value semantics
+    (src/Field/Field.h:160-367: every operator returns a new temporary,
+    assignment copies) while an unshared expression result hands over its
+    buffer: a named FieldPtr keeps its value after being assigned from, an
+    expression that reads its own target (W = W + W) gives its value, and host
+    writes through the page-locked mirror reach the device."""
+    exe = ROOT / "novaclone_amd" / "host" / "bin" / "field_semantics"
+    assert exe.exists(), "build first (__graft_entry__.build)"
+    print("running", exe, N, flush=True)
+    r = subprocess.run([str(exe), str(N)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "field_semantics: ok" in r.stdout
+
+
 def test_tg_poisson_app_writes_ensight_output(orc, tmp_path):
     # app/TG_Piosson/TG.cpp:78-80: mesh, U and p after every step
     exe = ROOT / "novaclone_amd" / "host" / "bin" / "tg_poisson"
