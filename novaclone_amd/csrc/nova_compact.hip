@@ -685,51 +685,84 @@ struct MhdW {
     {
         double (&u)[3][MP] = *reinterpret_cast<double(*)[3][MP]>(&in[0][0]);
         double (&b)[3][MP] = *reinterpret_cast<double(*)[3][MP]>(&in[3][0]);
-        double dud[MP], dbd[MP];
-        cmp_deriv<MP, 1, true>(D, lane, u[DIR], dud);
-        cmp_deriv<MP, 1, true>(D, lane, b[DIR], dbd);
+        // The systems are solved in batches (cmp_deriv_n: the same operations
+        // per system as one solve at a time, with the batch's cross-lane scan
+        // steps interleaved): first u_d and b_d, then per component c the
+        // first derivatives of u_c, b_c and the four products together, and
+        // the second derivatives of u_c and b_c together.
+        double dd[2][MP];
+        {
+            double f[2][MP];
+#pragma unroll
+            for (int j = 0; j < MP; j++) {
+                f[0][j] = u[DIR][j];
+                f[1][j] = b[DIR][j];
+            }
+            cmp_deriv_n<MP, 1, 2>(D, lane, f, dd);
+        }
+        const double(&dud)[MP] = dd[0];
+        const double(&dbd)[MP] = dd[1];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            double duc[MP], dbc[MP], f[MP], d1[MP], d2[MP], d3[MP], d4[MP], l2[MP];
+            // first derivatives: u_c, b_c, u_d u_c, b_d b_c, u_d b_c, b_d u_c
+            double r[6][MP];
             if (c == DIR) {
+                double f[3][MP], r3[3][MP];
 #pragma unroll
                 for (int j = 0; j < MP; j++) {
-                    duc[j] = dud[j];
-                    dbc[j] = dbd[j];
+                    f[0][j] = u[DIR][j] * u[c][j];
+                    f[1][j] = b[DIR][j] * b[c][j];
+                    f[2][j] = u[DIR][j] * b[c][j];
+                }
+                cmp_deriv_n<MP, 1, 3>(D, lane, f, r3);
+#pragma unroll
+                for (int j = 0; j < MP; j++) {
+                    r[0][j] = dud[j];
+                    r[1][j] = dbd[j];
+                    r[2][j] = r3[0][j];
+                    r[3][j] = r3[1][j];
+                    r[4][j] = r3[2][j];
+                    r[5][j] = r3[2][j];  // b_d u_d = u_d b_d
                 }
             } else {
-                cmp_deriv<MP, 1, true>(D, lane, u[c], duc);
-                cmp_deriv<MP, 1, true>(D, lane, b[c], dbc);
+                double f[6][MP];
+#pragma unroll
+                for (int j = 0; j < MP; j++) {
+                    f[0][j] = u[c][j];
+                    f[1][j] = b[c][j];
+                    f[2][j] = u[DIR][j] * u[c][j];
+                    f[3][j] = b[DIR][j] * b[c][j];
+                    f[4][j] = u[DIR][j] * b[c][j];
+                    f[5][j] = b[DIR][j] * u[c][j];
+                }
+                cmp_deriv_n<MP, 1, 6>(D, lane, f, r);
             }
+            const double(&duc)[MP] = r[0];
+            const double(&dbc)[MP] = r[1];
+            const double(&d1)[MP] = r[2];
+            const double(&d2)[MP] = r[3];
+            const double(&d3)[MP] = r[4];
+            const double(&d4)[MP] = r[5];
+            double l2[2][MP];
+            {
+                double f[2][MP];
 #pragma unroll
-            for (int j = 0; j < MP; j++) f[j] = u[DIR][j] * u[c][j];
-            cmp_deriv<MP, 1, true>(D, lane, f, d1);
-#pragma unroll
-            for (int j = 0; j < MP; j++) f[j] = b[DIR][j] * b[c][j];
-            cmp_deriv<MP, 1, true>(D, lane, f, d2);
-#pragma unroll
-            for (int j = 0; j < MP; j++) f[j] = u[DIR][j] * b[c][j];
-            cmp_deriv<MP, 1, true>(D, lane, f, d3);
-            if (c == DIR) {
-#pragma unroll
-                for (int j = 0; j < MP; j++) d4[j] = d3[j];  // b_d u_d = u_d b_d
-            } else {
-#pragma unroll
-                for (int j = 0; j < MP; j++) f[j] = b[DIR][j] * u[c][j];
-                cmp_deriv<MP, 1, true>(D, lane, f, d4);
+                for (int j = 0; j < MP; j++) {
+                    f[0][j] = u[c][j];
+                    f[1][j] = b[c][j];
+                }
+                cmp_deriv_n<MP, 2, 2>(D, lane, f, l2);
             }
-            cmp_deriv<MP, 2, true>(D, lane, u[c], l2);
 #pragma unroll
             for (int j = 0; j < MP; j++)
                 out[c][j] = ((-0.5 * ((d1[j] + u[DIR][j] * duc[j]) + u[c][j] * dud[j])) +
                              0.5 * ((d2[j] + b[DIR][j] * dbc[j]) + b[c][j] * dbd[j])) +
-                            a.nu * l2[j];
-            cmp_deriv<MP, 2, true>(D, lane, b[c], l2);
+                            a.nu * l2[0][j];
 #pragma unroll
             for (int j = 0; j < MP; j++)
                 out[3 + c][j] = ((-0.5 * ((d3[j] + u[DIR][j] * dbc[j]) + b[c][j] * dud[j])) +
                                  0.5 * ((d4[j] + b[DIR][j] * duc[j]) + u[c][j] * dbd[j])) +
-                                a.eta * l2[j];
+                                a.eta * l2[1][j];
         }
         double pm[MP], dpm[MP];
 #pragma unroll
