@@ -687,19 +687,22 @@ struct MhdW {
         double (&b)[3][MP] = *reinterpret_cast<double(*)[3][MP]>(&in[3][0]);
         // The systems are solved in batches (cmp_deriv_n: the same operations
         // per system as one solve at a time, with the batch's cross-lane scan
-        // steps interleaved): first u_d and b_d, then per component c the
-        // first derivatives of u_c, b_c and the four products together, and
-        // the second derivatives of u_c and b_c together.
-        double dd[2][MP];
+        // steps interleaved): first u_d, b_d and the magnetic pressure
+        // 0.5 |B|^2, then the six second derivatives, then per component c
+        // the first derivatives of u_c, b_c and the four products together.
+        double dd[3][MP];
         {
-            double f[2][MP];
+            double f[3][MP];
 #pragma unroll
             for (int j = 0; j < MP; j++) {
                 f[0][j] = u[DIR][j];
                 f[1][j] = b[DIR][j];
+                f[2][j] = 0.5 * ((b[0][j] * b[0][j] + b[1][j] * b[1][j]) + b[2][j] * b[2][j]);
             }
-            cmp_deriv_n<MP, 1, 2>(D, lane, f, dd);
+            cmp_deriv_n<MP, 1, 3>(D, lane, f, dd);
         }
+        double l2a[6][MP];
+        cmp_deriv_n<MP, 2, 6>(D, lane, in, l2a);
         const double(&dud)[MP] = dd[0];
         const double(&dbd)[MP] = dd[1];
 #pragma unroll
@@ -743,31 +746,20 @@ struct MhdW {
             const double(&d2)[MP] = r[3];
             const double(&d3)[MP] = r[4];
             const double(&d4)[MP] = r[5];
-            double l2[2][MP];
-            {
-                double f[2][MP];
-#pragma unroll
-                for (int j = 0; j < MP; j++) {
-                    f[0][j] = u[c][j];
-                    f[1][j] = b[c][j];
-                }
-                cmp_deriv_n<MP, 2, 2>(D, lane, f, l2);
-            }
+            const double(&l2u)[MP] = l2a[c];
+            const double(&l2b)[MP] = l2a[3 + c];
 #pragma unroll
             for (int j = 0; j < MP; j++)
                 out[c][j] = ((-0.5 * ((d1[j] + u[DIR][j] * duc[j]) + u[c][j] * dud[j])) +
                              0.5 * ((d2[j] + b[DIR][j] * dbc[j]) + b[c][j] * dbd[j])) +
-                            a.nu * l2[0][j];
+                            a.nu * l2u[j];
 #pragma unroll
             for (int j = 0; j < MP; j++)
                 out[3 + c][j] = ((-0.5 * ((d3[j] + u[DIR][j] * dbc[j]) + b[c][j] * dud[j])) +
                                  0.5 * ((d4[j] + b[DIR][j] * duc[j]) + u[c][j] * dbd[j])) +
-                                a.eta * l2[1][j];
+                                a.eta * l2b[j];
         }
-        double pm[MP], dpm[MP];
-#pragma unroll
-        for (int j = 0; j < MP; j++) pm[j] = 0.5 * ((b[0][j] * b[0][j] + b[1][j] * b[1][j]) + b[2][j] * b[2][j]);
-        cmp_deriv<MP, 1, true>(D, lane, pm, dpm);
+        const double(&dpm)[MP] = dd[2];
 #pragma unroll
         for (int j = 0; j < MP; j++) out[DIR][j] = out[DIR][j] - dpm[j];
     }
