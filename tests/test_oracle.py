@@ -361,6 +361,31 @@ def test_mesh_metric_uniform_and_stretched(orc):
     assert np.all(np.diff(zs) > 0)
 
 
+def test_stretched_mesh_end_slopes_and_symmetry(orc):
+    """Interior pins of the stretched z (src/Mesh/Mesh.cpp:370-404), from what
+    its definition implies rather than from the formula: with xi = k/N, b the
+    fixed point of b = asinh(b / (N sqrt(s1 s2))) and A = sqrt(s1/s2), the map
+    has dz/dk = L s2 at the bottom wall and L s1 at the top (the end slopes
+    2b/sinh(b) of U times 1/(2A) and A/2, with sinh(b)/b = 1/(N sqrt(s1 s2)));
+    with s1 = s2 it is symmetric, z(k) + z(N - k) = L.  A wrong fixed-point
+    argument, an inverted A or a sign in the tanh argument fails these."""
+    from pyoracle import lib
+    N, L = 256, 2.0
+    # one-sided sixth-order first derivative at the end node
+    c = np.array([-49 / 20, 6, -15 / 2, 20 / 3, -15 / 4, 6 / 5, -1 / 6])
+    for s1, s2 in [(0.002, 0.003), (0.003, 0.0015)]:
+        M = orc.OMesh((8, 8, N), (1.0, 1.0, L), hyp=(s1, s2))
+        h = M.h
+        z = np.array([lib().orc_mesh_z(M._h, h + k) for k in range(N + 1)])
+        assert abs(z[0]) < 1e-14 and abs(z[-1] - L) < 1e-14
+        d0, dN = c @ z[:7], -(c @ z[::-1][:7])
+        assert abs(d0 / (L * s2) - 1) < 1e-9, (s1, s2, d0)
+        assert abs(dN / (L * s1) - 1) < 1e-9, (s1, s2, dN)
+    M = orc.OMesh((8, 8, N), (1.0, 1.0, L), hyp=(0.002, 0.002))
+    z = np.array([lib().orc_mesh_z(M._h, M.h + k) for k in range(N + 1)])
+    assert np.max(np.abs(z + z[::-1] - L)) < 1e-13
+
+
 @pytest.mark.parametrize("wall_y,hyp", [(False, None), (True, None), (False, (0.02, 0.03))])
 def test_fused_cpu_baseline_step_equals_the_faithful_step(orc, wall_y, hyp):
     # the fused variant times the same arithmetic: it must agree with the

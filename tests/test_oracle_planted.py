@@ -34,7 +34,8 @@ PLANTS = {
     13: "MHD diagnostics: Em without the 1/2",
     14: "MHD diagnostics: epsilon with +eta Jav^2",
 }
-PINS = ["tests/test_oracle_mhd.py", "tests/test_oracle.py::test_mesh_metric_uniform_and_stretched"]
+PINS = ["tests/test_oracle_mhd.py", "tests/test_oracle.py::test_mesh_metric_uniform_and_stretched",
+        "tests/test_oracle.py::test_stretched_mesh_end_slopes_and_symmetry"]
 
 
 def _build(n: int, out: Path) -> Path:
