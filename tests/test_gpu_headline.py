@@ -164,3 +164,31 @@ def test_ot_mhd_step_at_256_matches_oracle(nv, orc):
     d = nv.mhd_diag(G, U, B, 1e-3, 1e-3)
     do = O.mhd_diag(Uo, Bo, 1e-3, 1e-3)
     assert np.allclose(d[:7], do[:7], rtol=1e-12, atol=1e-14)
+
+
+def test_wc_step_at_256_matches_oracle(nv, orc):
+    """The weakly-compressible TG configuration of app/TG at its full size:
+    256^3 periodic, the state of TG.cpp:39-55 (p = 50 + (cos 2x + cos 2y)
+    (cos 2z + 2)/16, rho = (p/50)^(1/2)) with a 1e-3 seeded perturbation of
+    U, mu = 1/1600, p_c = 50, dt = 1e-3, one classic-RK4 step of rho and
+    rho U against the oracle."""
+    N = (256, 256, 256)
+    G, O = pair(nv, orc, N, (TWO_PI,) * 3)
+    x = [np.arange(O.stored[a]) * TWO_PI / N[a] for a in range(3)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    U = np.stack([np.sin(X) * np.cos(Y) * np.cos(Z), -np.cos(X) * np.sin(Y) * np.cos(Z), 0 * X])
+    U += 1e-3 * np.random.default_rng(11).standard_normal(U.shape)
+    p = 50 + 1.0 / 16.0 * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2)
+    del X, Y, Z
+    rho = np.sqrt(p / 50.0)[None]
+    mu, pc, dt = 1.0 / 1600.0, 50.0, 1e-3
+    W = nv.WC(G, mu, pc)
+    Ud, rd, pd = dev(U), dev(rho), G.field(1)
+    W.step(dt, 1, Ud, rd, pd)
+    Uo, ro, po = O.from_compact(U), O.from_compact(rho), O.field(1)
+    del U, rho, p
+    O.wc_step(Uo, ro, po, mu, pc, dt)
+    eu, er, ep = (rel_l2(host(Ud), O.interior(Uo)), rel_l2(host(rd), O.interior(ro)),
+                  rel_l2(host(pd), O.interior(po)))
+    print("WC 256^3 one step rel L2 U", eu, "rho", er, "p", ep, flush=True)
+    assert eu <= RTOL and er <= RTOL and ep <= RTOL
