@@ -43,6 +43,19 @@ def test_bad_arguments_are_rejected_with_a_message():
     assert lib.nova_ctx_create(None, 0, C.byref(h)) == -1
 
 
+def test_host_allocation_arguments_are_checked():
+    """nova_host_alloc (the drop-in's page-locked host mirrors) refuses a
+    null result pointer and a zero size before any runtime call;
+    nova_host_free(NULL) is a no-op."""
+    import novaclone_amd as nv
+    lib = nv.load_library()
+    p = C.c_void_p()
+    assert lib.nova_host_alloc(0, C.byref(p)) == -1 and not p.value
+    assert b"bad argument" in lib.nova_last_error()
+    assert lib.nova_host_alloc(64, None) == -1
+    assert lib.nova_host_free(None) == 0
+
+
 def test_no_device_is_an_error_not_a_fallback():
     import torch
     if torch.cuda.is_available():
