@@ -88,13 +88,22 @@ struct SopArgs {
     double scale;  // DIV
 };
 
+// tile rows: 64 x 8 tiles (one halo slot per thread: 1.84 loads per node
+// and component against 2.59 on 64 x 4) measured 4-8% faster for div and
+// grad at 256^3 and the same for the Laplacian; the curl, 1% slower there,
+// and div(U, phi) at H = 4, which spills within 256 registers, keep 64 x 4
+template <int H, int OP, int NCP, bool SHARED>
+constexpr int sop_kj()
+{
+    return (OP == SOP_CURL_LIT || OP == SOP_CURL || (H >= 4 && OP == SOP_DIV_ADV && !SHARED && NCP == 3)) ? 4 : 8;
+}
 template <int H, int OP, int NCP, bool SHARED, bool WALLS>
-__global__ void __launch_bounds__(pipe::threads(4))
+__global__ void __launch_bounds__(pipe::threads(sop_kj<H, OP, NCP, SHARED>()))
     __attribute__((amdgpu_waves_per_eu(OP == SOP_DIV_ADV && !SHARED && NCP == 3 && H <= 3 ? 2 : 1)))
     k_sop_stream(Grid g, pipe::Launch L, pipe::Src<sop_staged<OP, NCP, SHARED>()> S, SopArgs a)
 {
     constexpr int NS = sop_staged<OP, NCP, SHARED>();
-    pipe::run<H, NS, sop_qm<OP, NCP, SHARED>(), sop_lm<OP, NCP, SHARED>(), WALLS>(
+    pipe::run<H, NS, sop_qm<OP, NCP, SHARED>(), sop_lm<OP, NCP, SHARED>(), WALLS, sop_kj<H, OP, NCP, SHARED>()>(
         g, L, S, [&](int, int, int k, int64_t q0, const auto &nb) {
             if constexpr (OP == SOP_DIV_ADV) {
                 // src/ex/div/div.hxx:101-169: the mean of the conservative
@@ -528,12 +537,13 @@ template <int H, int OP, int NCP, bool SHARED>
 int launch_sop_h(nova_ctx *ctx, const pipe::Src<sop_staged<OP, NCP, SHARED>()> &S, const SopArgs &a, const char *name)
 {
     const Grid &g = ctx->g;
-    const pipe::Launch L = pipe::tiling<H>(g);
+    constexpr int KJ = sop_kj<H, OP, NCP, SHARED>();
+    const pipe::Launch L = pipe::tiling<H, KJ>(g);
     if (g.wall[0] || g.wall[1] || g.wall[2])
-        hipLaunchKernelGGL((k_sop_stream<H, OP, NCP, SHARED, true>), dim3(pipe::blocks(L)), dim3(pipe::threads(4)), 0,
+        hipLaunchKernelGGL((k_sop_stream<H, OP, NCP, SHARED, true>), dim3(pipe::blocks(L)), dim3(pipe::threads(KJ)), 0,
                            ctx->stream, g, L, S, a);
     else
-        hipLaunchKernelGGL((k_sop_stream<H, OP, NCP, SHARED, false>), dim3(pipe::blocks(L)), dim3(pipe::threads(4)), 0,
+        hipLaunchKernelGGL((k_sop_stream<H, OP, NCP, SHARED, false>), dim3(pipe::blocks(L)), dim3(pipe::threads(KJ)), 0,
                            ctx->stream, g, L, S, a);
     return check_launch(name);
 }
