@@ -104,12 +104,12 @@ __global__ void __launch_bounds__(pipe::threads(sop_kj<H, OP, NCP, SHARED>()))
 {
     constexpr int NS = sop_staged<OP, NCP, SHARED>();
     pipe::run<H, NS, sop_qm<OP, NCP, SHARED>(), sop_lm<OP, NCP, SHARED>(), WALLS, sop_kj<H, OP, NCP, SHARED>()>(
-        g, L, S, [&](int, int, int k, int64_t q0, const auto &nb) {
+        g, L, S, [&](int, int, int, int64_t q0, const auto &nb, const pipe::ZK &z) {
             if constexpr (OP == SOP_DIV_ADV) {
                 // src/ex/div/div.hxx:101-169: the mean of the conservative
                 // and the expanded forms, per direction and stencil offset
                 // (U_d phi)^+ - (U_d phi)^- + U_d (phi^+ - phi^-) + phi (U_d^+ - U_d^-)
-                const double zi = g.zinv[k];
+                const double zi = z.zi;
 #pragma unroll
                 for (int c = 0; c < NCP; c++) {
                     const int sp = SHARED ? c : 3 + c;
@@ -146,9 +146,9 @@ __global__ void __launch_bounds__(pipe::threads(sop_kj<H, OP, NCP, SHARED>()))
                             ad += g.b[d][l] * ((fp + fm) - 2.0 * f0);
                             if (d == 2) dz1 += g.a[2][l] * (fp - fm);
                         }
-                        acc += d == 2 ? ad * g.zlap[k] : ad;
+                        acc += d == 2 ? ad * z.zl : ad;
                     }
-                    __builtin_nontemporal_store(a.nu * acc - a.nu * g.zmet[k] * dz1, a.out + c * g.np + q0);
+                    __builtin_nontemporal_store(a.nu * acc - a.nu * z.zm * dz1, a.out + c * g.np + q0);
                 }
             } else if constexpr (OP == SOP_DIV) {
                 // src/ex/div/div.hxx:37-77
@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(pipe::threads(sop_kj<H, OP, NCP, SHARED>()))
                     double ad = 0.0;
 #pragma unroll
                     for (int l = 0; l < H; l++) ad += g.a[d][l] * (nb(d, d, l + 1) - nb(d, d, -l - 1));
-                    acc += d == 2 ? ad * g.zinv[k] : ad;
+                    acc += d == 2 ? ad * z.zi : ad;
                 }
                 __builtin_nontemporal_store(acc * a.scale, a.out + q0);
             } else if constexpr (OP == SOP_GRAD) {
@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(pipe::threads(sop_kj<H, OP, NCP, SHARED>()))
                     double ad = 0.0;
 #pragma unroll
                     for (int l = 0; l < H; l++) ad += g.a[d][l] * (nb(0, d, l + 1) - nb(0, d, -l - 1));
-                    __builtin_nontemporal_store(d == 2 ? ad * g.zinv[k] : ad, a.out + d * g.np + q0);
+                    __builtin_nontemporal_store(d == 2 ? ad * z.zi : ad, a.out + d * g.np + q0);
                 }
             } else {
                 // curl: literal = antisymmetric part of the half-shifted
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(pipe::threads(sop_kj<H, OP, NCP, SHARED>()))
                             if constexpr (OP == SOP_CURL_LIT) ad += g.al[d][l] * (nb(c, d, l) - nb(c, d, -l - 1));
                             else ad += g.a[d][l] * (nb(c, d, l + 1) - nb(c, d, -l - 1));
                         }
-                        gr[c][d] = d == 2 ? ad * g.zinv[k] : ad;
+                        gr[c][d] = d == 2 ? ad * z.zi : ad;
                     }
                 __builtin_nontemporal_store(gr[2][1] - gr[1][2], a.out + q0);
                 __builtin_nontemporal_store(gr[0][2] - gr[2][0], a.out + g.np + q0);

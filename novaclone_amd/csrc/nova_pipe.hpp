@@ -39,6 +39,13 @@ struct Launch {
     int ichunk, nkt, njt, nic;
 };
 
+// the z metric of a thread's column (per stored k): 1/dzdxi, 1/dzdxi^2,
+// d2zdxi2/dzdxi^3, loaded once per thread (per plane, the compiler could
+// not hoist them past the result stores, which it cannot tell apart)
+struct ZK {
+    double zi, zl, zm;
+};
+
 // i chunks enough for about 2048 blocks of 256 threads (eight per CU), at
 // least 8 planes (and 4H) long so that the 2H warm-up planes stay a small
 // part of a chunk; KJ: rows of a tile (64 x KJ threads)
@@ -73,8 +80,9 @@ inline void set_src(const Grid &g, Src<NS> &S, int s, const double *f, int nc, i
             if ((g.odd[d] >> c) & 1u) S.flip[s] |= 1u << d;
 }
 
-// The pipeline.  compute(i, j, k, q0, nb) runs for every node of the block's
-// chunk inside the grid (q0: the node's stored offset); nb(s, d, o) is
+// The pipeline.  compute(i, j, k, q0, nb, zk) runs for every node of the
+// block's chunk inside the grid (q0: the node's stored offset; zk: the z
+// metric of the node's k); nb(s, d, o) is
 // component s at the node displaced by o (-H .. H) along d: d = 0 needs s in
 // QM, d = 1 or 2 with o != 0 needs s in LM, o = 0 either.  Blocks map to
 // tiles so that each of the eight XCDs (given blocks in turn) holds a
@@ -113,6 +121,7 @@ __device__ __attribute__((always_inline)) inline void run(const Grid &g, const L
     const int km = bmap(k, g.n[2], g.wall[2], fk);
     const int64_t col = (int64_t)jm * g.s[2] + km;
     const unsigned cfl = (fj ? 2u : 0u) | (fk ? 4u : 0u);
+    const ZK zk{g.zinv[km], g.zlap[km], g.zmet[km]};
     // halo slots of this thread: LDS row and column, source column, flags
     int hr[NSLOT], hc[NSLOT];
     int64_t hsrc[NSLOT];
@@ -248,7 +257,7 @@ __device__ __attribute__((always_inline)) inline void run(const Grid &g, const L
                 load_pf(Cur, i + 2, true);
                 load_c(Ccur, i + 2);
             }
-            if (valid) compute(i, j, k, (int64_t)i * plane + col, nb);
+            if (valid) compute(i, j, k, (int64_t)i * plane + col, nb, zk);
             __syncthreads();
             if (i + 1 < i1) shift(Nxt);
         };
@@ -294,7 +303,7 @@ __device__ __attribute__((always_inline)) inline void run(const Grid &g, const L
                     load_c(CA, i + 1);
                 }
             }
-            if (valid) compute(i, j, k, (int64_t)i * plane + col, nb);
+            if (valid) compute(i, j, k, (int64_t)i * plane + col, nb, zk);
             __syncthreads();
             if constexpr (PD == 1) {
                 if (i + 1 < i1) shift(A);

@@ -74,9 +74,9 @@ struct WcUArgs {
 template <int H, bool WALLS, int KJ, int PD>
 __global__ void __launch_bounds__(pipe::threads(KJ)) k_wc_rho(Grid g, pipe::Launch L, pipe::Src<4> S, WcRhoArgs a)
 {
-    pipe::run<H, 4, 0b0011u, 0b1101u, WALLS, KJ, PD>(g, L, S, [&](int, int, int k, int64_t qi, const auto &nb) {
+    pipe::run<H, 4, 0b0011u, 0b1101u, WALLS, KJ, PD>(g, L, S, [&](int, int, int, int64_t qi, const auto &nb, const pipe::ZK &z) {
         auto flux = [&](int d, int o) -> double { return nb(0, d, o) * nb(1 + d, d, o); };
-        const double zi = g.zinv[k];
+        const double zi = z.zi;
         double ax = 0.0, ay = 0.0, az = 0.0;
 #pragma unroll
         for (int l = 0; l < H; l++) {
@@ -104,14 +104,14 @@ __global__ void __launch_bounds__(pipe::threads(KJ)) k_wc_rho(Grid g, pipe::Laun
 template <int H, bool WALLS, int KJ, int PD>
 __global__ void __launch_bounds__(pipe::threads(KJ)) k_wc_u(Grid g, pipe::Launch L, pipe::Src<5> S, WcUArgs a)
 {
-    pipe::run<H, 5, 0b11111u, 0b11111u, WALLS, KJ, PD>(g, L, S, [&](int i, int j, int k, int64_t qi, const auto &nb) {
+    pipe::run<H, 5, 0b11111u, 0b11111u, WALLS, KJ, PD>(g, L, S, [&](int i, int j, int k, int64_t qi, const auto &nb, const pipe::ZK &z) {
         // field f at the node displaced by o along d: 0 = rho*U, 1 = U, 2 = p
         auto A = [&](int f, int c, int d, int o) -> double {
             if (f == 2) return nb(4, d, o);
             const double u = nb(1 + c, d, o);
             return f == 0 ? nb(0, d, o) * u : u;
         };
-        const double zi = g.zinv[k], zl = g.zlap[k], zm = g.zmet[k];
+        const double zi = z.zi, zl = z.zl, zm = z.zm;
         const double r = nb(0, 0, 0);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -162,13 +162,13 @@ struct WcTArgs {
 template <int H, bool WALLS, int KJ, int PD>
 __global__ void __launch_bounds__(pipe::threads(KJ)) k_wc_t(Grid g, pipe::Launch L, pipe::Src<5> S, WcTArgs a)
 {
-    pipe::run<H, 5, 0b10011u, 0b11101u, WALLS, KJ, PD>(g, L, S, [&](int, int, int k, int64_t qi, const auto &nb) {
+    pipe::run<H, 5, 0b10011u, 0b11101u, WALLS, KJ, PD>(g, L, S, [&](int, int, int, int64_t qi, const auto &nb, const pipe::ZK &z) {
         // field f at the node displaced by o along d: 0 = rho*U (component
         // c), 1 = T
         auto A = [&](int f, int c, int d, int o) -> double {
             return f == 0 ? nb(0, d, o) * nb(1 + c, d, o) : nb(4, d, o);
         };
-        const double zi = g.zinv[k], zl = g.zlap[k], zm = g.zmet[k];
+        const double zi = z.zi, zl = z.zl, zm = z.zm;
         const double r = nb(0, 0, 0);
         const double dT = a.dt * ((-div_xy<H>(g, A, 0, 1, 0, zi)) + lap_c<H>(g, A, 1, 0, a.ka, zl, zm));
         const double t0 = a.T0 ? a.T0[qi] : nb(4, 0, 0);
@@ -194,7 +194,7 @@ struct WcLArgs {
 template <int H, bool WALLS, int KJ, int PD>
 __global__ void __launch_bounds__(pipe::threads(KJ)) k_wc_lorentz(Grid g, pipe::Launch L, pipe::Src<3> S, WcLArgs a)
 {
-    pipe::run<H, 3, 0b111u, 0b111u, WALLS, KJ, PD>(g, L, S, [&](int, int, int k, int64_t qi, const auto &nb) {
+    pipe::run<H, 3, 0b111u, 0b111u, WALLS, KJ, PD>(g, L, S, [&](int, int, int, int64_t qi, const auto &nb, const pipe::ZK &z) {
         // field 0 = (2 dbu) B, 1 = B
         auto A = [&](int f, int c, int d, int o) -> double {
             const double b = nb(c, d, o);
@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(pipe::threads(KJ)) k_wc_lorentz(Grid g, pipe::
             const double x = nb(0, d, o), y = nb(1, d, o), z = nb(2, d, o);
             return (x * x + y * y + z * z) * a.dbu;
         };
-        const double zi = g.zinv[k];
+        const double zi = z.zi;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             double gs = 0.0;
