@@ -103,7 +103,10 @@ __global__ void __launch_bounds__(pipe::threads(sop_kj<H, OP, NCP, SHARED>()))
     k_sop_stream(Grid g, pipe::Launch L, pipe::Src<sop_staged<OP, NCP, SHARED>()> S, SopArgs a)
 {
     constexpr int NS = sop_staged<OP, NCP, SHARED>();
-    pipe::run<H, NS, sop_qm<OP, NCP, SHARED>(), sop_lm<OP, NCP, SHARED>(), WALLS, sop_kj<H, OP, NCP, SHARED>()>(
+    // planes of loads in flight: the Laplacian with one (353 -> 330 us for a
+    // vector at 256^3, its registers allowing more waves), the others two
+    constexpr int PD = OP == SOP_LAPLACIAN ? 1 : 2;
+    pipe::run<H, NS, sop_qm<OP, NCP, SHARED>(), sop_lm<OP, NCP, SHARED>(), WALLS, sop_kj<H, OP, NCP, SHARED>(), PD>(
         g, L, S, [&](int, int, int, int64_t q0, const auto &nb, const pipe::ZK &z) {
             if constexpr (OP == SOP_DIV_ADV) {
                 // src/ex/div/div.hxx:101-169: the mean of the conservative
