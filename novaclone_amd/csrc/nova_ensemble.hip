@@ -3,10 +3,11 @@
 //
 // Instances of different sizes and models (Navier-Stokes TG-type, MHD
 // OT-type) advance together with the SSP-RK3 composition of nova_step.hip:
-//   * each RK stage of every instance is one persistent launch: a fixed
-//     grid of blocks walks a tile work list spanning all instances, and each
-//     tile runs the same streamed stage body (stage_tile) as the single-grid
-//     kernels, so an instance's result equals its standalone run bit for bit;
+//   * each RK stage of every instance is two persistent launches, one per
+//     model: a fixed grid of blocks walks the tile work list of that model's
+//     instances, and each tile runs the same streamed stage body (stage_tile)
+//     as the single-grid kernels, so an instance's result equals its
+//     standalone run bit for bit;
 //   * instances with the same grid form a size class that shares a context
 //     (metric, wavenumbers) and one batched Poisson solver; a class's fields
 //     sit in contiguous slots [NS U | MHD U | MHD B] so that each projection
