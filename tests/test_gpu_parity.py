@@ -74,6 +74,9 @@ CASES = [
     # 13 = 3 x 4 + 1 j rows) and i chunks of 8 planes with a short last one
     dict(N=(37, 13, 70), L=(1.0, 0.5, 2.0), m=6, wall_y=False),
     dict(N=(21, 10, 66), L=(1.0, 1.0, 1.0), m=2, wall_y=True),
+    # m = 8 with walls: the 64 x 4 tiles of div(U, phi) at that order and the
+    # reflection signs at the widest stencil
+    dict(N=(16, 14, 20), L=(1.0, 1.0, 1.0), m=8, wall_y=True),
 ]
 
 
