@@ -46,6 +46,8 @@ CASES = [
     dict(N=(16, 24, 12), L=(TWO_PI, 1.5 * TWO_PI, TWO_PI), m=4, wall_y=False),
     dict(N=(16, 12, 16), L=(TWO_PI, 3.0, TWO_PI), m=6, wall_y=True),
     dict(N=(12, 16, 16), L=(TWO_PI,) * 3, m=8, wall_y=False),
+    dict(N=(16, 16, 24), L=(TWO_PI,) * 3, m=2, wall_y=False),
+    dict(N=(16, 12, 16), L=(TWO_PI, 3.0, TWO_PI), m=8, wall_y=True),
 ]
 
 

@@ -61,6 +61,12 @@ CASES = [
     dict(N=(16, 16, 24), L=(TWO_PI,) * 3, m=4, wall_y=False),
     dict(N=(16, 12, 16), L=(TWO_PI,) * 3, m=8, wall_y=False),
     dict(N=(24, 16, 16), L=(TWO_PI, math.pi, TWO_PI), m=6, wall_y=True),
+    # walls at every other stencil width: m = 2 with two k tiles (70 = 64 + 6)
+    # and j rows that do not fill the last tile, m = 4 with several i chunks,
+    # m = 8 with no plane of loads in flight in the momentum stage
+    dict(N=(20, 14, 70), L=(TWO_PI, math.pi, TWO_PI), m=2, wall_y=True),
+    dict(N=(40, 10, 20), L=(TWO_PI, math.pi, TWO_PI), m=4, wall_y=True),
+    dict(N=(16, 12, 16), L=(TWO_PI, math.pi, TWO_PI), m=8, wall_y=True),
 ]
 
 
