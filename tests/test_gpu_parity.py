@@ -223,6 +223,38 @@ def test_wall_case_step_matches_oracle(nv, orc):
     assert rel_l2(host(U), O.interior(Uo)) <= RTOL
 
 
+@pytest.mark.parametrize("m,wall_y", [(2, False), (4, False), (8, False), (2, True), (4, True), (8, True)])
+def test_ns_and_mhd_steps_at_other_stencil_widths(nv, orc, m, wall_y):
+    """The stage kernels of the stencil widths besides m = 6, periodic and
+    walled: one SSP-RK3 NS step and one SSP-RK3 MHD step from projected
+    random fields, against the oracle."""
+    N, L = (16, 12, 24), (TWO_PI, 3.0, TWO_PI)
+    G, O = pair(nv, orc, N, L, m=m, wall_y=wall_y)
+    rng = np.random.default_rng(40 + m + 10 * wall_y)
+    OP = orc.OPoisson(O)
+    start = []
+    for _ in range(2):
+        F = O.from_compact(wall_clean(O, rng.standard_normal((3, *O.stored))))
+        OP.project(F, 1.0)
+        start.append(np.array(O.interior(F)))
+    cu, cb = start
+    nu, eta, dt = 1e-2, 2e-2, 1e-3
+    Uo, po = O.from_compact(cu), O.field(1)
+    OP.ns_step(Uo, po, nu, dt, 1)
+    U, p = dev(cu), G.field(1)
+    nv.Solver(nv.Poisson(G), nv.MODEL_NS, nv.RK_SSP3, nu=nu).step(dt, 1, U, p)
+    e = rel_l2(host(U), O.interior(Uo))
+    Uo, Bo, po, pBo = O.from_compact(cu), O.from_compact(cb), O.field(1), O.field(1)
+    OP.mhd_step(Uo, Bo, po, pBo, nu, eta, dt, 1)
+    U, B, p, pB = dev(cu), dev(cb), G.field(1), G.field(1)
+    nv.Solver(nv.Poisson(G), nv.MODEL_MHD, nv.RK_SSP3, nu=nu, eta=eta).step(dt, 1, U, p, B, pB)
+    eu, eb = rel_l2(host(U), O.interior(Uo)), rel_l2(host(B), O.interior(Bo))
+    print("m", m, "wall", wall_y, "NS", e, "MHD", eu, eb, flush=True)
+    assert e <= RTOL and eu <= RTOL and eb <= RTOL
+    # the steps moved the fields well beyond the tolerance
+    assert rel_l2(host(U), cu) > 1e-6 and rel_l2(host(B), cb) > 1e-6
+
+
 def test_mhd_one_step_matches_oracle(nv, orc):
     N = (24, 24, 24)
     G, O = pair(nv, orc, N, (TWO_PI,) * 3)
