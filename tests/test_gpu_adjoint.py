@@ -21,10 +21,10 @@ def nv():
     return nv
 
 
-def flow(nv, N, L, wall=False, seed=0, amp=1e-2):
+def flow(nv, N, L, wall=False, seed=0, amp=1e-2, m=6):
     import torch
     bc = ("periodic", "wall" if wall else "periodic", "periodic")
-    G = nv.Mesh(N, L, bc=bc)
+    G = nv.Mesh(N, L, m=m, bc=bc)
     P = nv.Poisson(G)
     gen = torch.Generator(device="cuda").manual_seed(seed)
     s = G.shape
@@ -69,6 +69,24 @@ def test_dot_product_tlm_adjoint(nv, wall, nsteps, every):
     lhs = nv.weighted_dot(G, Ld, m)
     rhs = nv.weighted_dot(G, d, LTm)
     print("dot", wall, nsteps, lhs, rhs, flush=True)
+    assert abs(lhs - rhs) <= 1e-11 * max(abs(lhs), abs(rhs))
+
+
+@pytest.mark.parametrize("m,wall", [(2, False), (4, False), (8, False), (2, True), (4, True), (8, True)])
+def test_dot_product_tlm_adjoint_other_stencil_widths(nv, m, wall):
+    """<L d, m> = <d, L^T m> over three steps (checkpoints every 2) with the
+    adjoint stage kernels of the other stencil widths."""
+    N = (16, 12, 24)
+    L = (TWO_PI,) * 3 if not wall else (4.0, 3.0, 4.0)
+    G, P, U0 = flow(nv, N, L, wall=wall, seed=5, m=m)
+    S = nv.Solver(P, nv.MODEL_NS, nv.RK_SSP3, nu=2e-2)
+    A = nv.Adjoint(S, 2, 8)
+    A.forward(0.01, 3, U0)
+    d = rand_like(U0, 6, wall)
+    mm = rand_like(U0, 7, wall)
+    lhs = nv.weighted_dot(G, A.tlm(d), mm)
+    rhs = nv.weighted_dot(G, d, A.backward(mm))
+    print("dot m", m, "wall", wall, lhs, rhs, flush=True)
     assert abs(lhs - rhs) <= 1e-11 * max(abs(lhs), abs(rhs))
 
 
